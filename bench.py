@@ -1,0 +1,224 @@
+"""Throughput of the PIV hot path on one MI355X.
+
+Workload (the metric's configuration): one 2048x2048 uint16 image pair with
+12-bit particle images, passes 64 -> 32 -> 16 px at 50 % overlap, bilinear
+window deformation between passes, 3x3 median validation (tol 2) with spring
+replacement, GCV smoothn after the non-final passes, FP32.  A step is one
+full pass loop on one pair whose frames are already resident in HBM; outputs
+stay on the device.
+
+Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU
+(pairs are independent; multi-GPU use shards the pair stream, no exchange).
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "openpiv-python-gpu_amd")
+for _p in (PKG, ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "pairs/s at 2048², 64→32→16 px; learned vs plain at ≤0.05 px RMS error"
+
+# Algorithmic FP32 FLOPs per window of the zero-padded FFT correlation
+# (5 N log2 N per complex transform, forward row stage pruned for the zero
+# padding): 12.5 F^2 log2 F + 6 F (F/2 + 1).
+def fft_flops_per_window(F):
+    return 12.5 * F * F * np.log2(F) + 6 * F * (F // 2 + 1)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def stage_kernel_times(torch, fa_u16, piv, reps=20):
+    """Average duration of each pass's correlation launch and of the
+    deformation launch, timed with events on the stream the kernels run on."""
+    from openpiv_gpu import _native
+    lib = _native.load_library()
+    stream = torch.cuda.current_stream()
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    ht, wd = fa_u16.shape
+    d_u16 = torch.from_numpy(np.ascontiguousarray(fa_u16).view(np.int16)).to("cuda")
+    d_f32 = torch.from_numpy(fa_u16.astype(np.float32)).to("cuda")
+    desc = piv._plan_desc(_native.PIV_U16)
+    out = []
+    for k in range(desc.n_passes):
+        p = desc.passes[k]
+        m, n = p.field_m, p.field_n
+        u = torch.empty((m, n), dtype=torch.float32, device="cuda")
+        v = torch.empty_like(u)
+        if k == 0:
+            src, code = d_u16, _native.PIV_U16
+        else:
+            src, code = d_f32, _native.PIV_F32
+        ptr = ctypes.c_void_p(src.data_ptr())
+
+        def launch():
+            _native.check(lib.piv_stage_correlate(ptr, ptr, code, ht, wd, ctypes.byref(p), None, None,
+                                                  ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(v.data_ptr()),
+                                                  None, None, None, sh))
+        for _ in range(3):
+            launch()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            launch()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        flops = fft_flops_per_window(p.fft_size) * m * n
+        out.append({"pass": k, "kernel": f"corr_kernel<F={p.fft_size}>", "windows": m * n, "fft": p.fft_size,
+                    "avg_ms": ms, "flop": flops, "tflops": flops / (ms * 1e-3) / 1e12})
+    # deformation (pass 2 geometry): reads both uint16 frames, writes two FP32 frames
+    p = desc.passes[min(1, desc.n_passes - 1)]
+    up = torch.zeros((p.field_m, p.field_n), dtype=torch.float32, device="cuda") + 1.5
+    oa = torch.empty((ht, wd), dtype=torch.float32, device="cuda")
+    ob = torch.empty_like(oa)
+    ptr = ctypes.c_void_p(d_u16.data_ptr())
+
+    def dlaunch():
+        _native.check(lib.piv_stage_deform(ptr, ptr, _native.PIV_U16, ht, wd, ctypes.byref(p),
+                                           ctypes.c_void_p(up.data_ptr()), ctypes.c_void_p(up.data_ptr()),
+                                           ctypes.c_void_p(oa.data_ptr()), ctypes.c_void_p(ob.data_ptr()), sh))
+    for _ in range(3):
+        dlaunch()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        dlaunch()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = ht * wd * (2 * 2 + 2 * 4)
+    deform = {"kernel": "deform_kernel<uint16>", "avg_ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9}
+    return out, deform
+
+
+def cpu_baseline(fa, fb, pairs=1):
+    from oracle import piv_oracle as O
+    s = O.OracleSettings(min_search_size=16, search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2.0)
+    t0 = time.perf_counter()
+    for _ in range(pairs):
+        O.piv(fa, fb, s)
+    dt = time.perf_counter() - t0
+    return {"value": pairs / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+            "sample": f"{pairs} pair(s) of the 2048x2048 uint16 workload, full 64->32->16 loop, NumPy/SciPy "
+                      f"oracle (FP32 correlation, FP64 smoothn), single thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--skip-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-kernel-timing", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for {max(args.gpus, world)}): image pairs are independent, "
+              "so several GPUs would each run this same loop on their share of the pair stream.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from openpiv_gpu import synthetic
+    from openpiv_gpu.gpu_process import PIVGPU
+
+    log("generating the 2048x2048 uint16 pair (seed 2)")
+    fa, fb, disp = synthetic.cfg_2048_uint16(seed=2)
+    piv = PIVGPU((2048, 2048), 16, search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2,
+                 dtype_f="float32", use_graph=not args.no_graph)
+    from openpiv_gpu import _native
+    plan = piv._get_plan(_native.PIV_U16)
+    m, n = piv.piv_fields[-1].field_shape
+    dev = torch.device("cuda", 0)
+    # uint16 bits carried in an int16 tensor (torch is only the allocator)
+    da = torch.from_numpy(np.ascontiguousarray(fa).view(np.int16)).to(dev)
+    db = torch.from_numpy(np.ascontiguousarray(fb).view(np.int16)).to(dev)
+    u = torch.empty((m, n), dtype=torch.float32, device=dev)
+    v = torch.empty_like(u)
+    vi = torch.empty((m, n), dtype=torch.uint8, device=dev)
+    vf = torch.empty_like(vi)
+    torch.cuda.synchronize()
+    ptrs = [t.data_ptr() for t in (da, db, u, v, vi, vf)]
+
+    def step():
+        plan.run(ptrs[0], ptrs[1], True, ptrs[2], ptrs[3], ptrs[4], ptrs[5], True)
+
+    log(f"warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = 1e3 / ms_per_step
+
+    # accuracy of this run against the known displacement
+    x, y = piv.coords
+    ut, vt = disp(np.asarray(x, np.float64), np.asarray(y, np.float64))
+    uh, vh = u.cpu().numpy(), v.cpu().numpy()
+    rms = float(np.sqrt(np.mean((uh - ut) ** 2 + (vh - vt) ** 2) / 2))
+
+    # host-to-host rate (NumPy in/out, includes PCIe both ways)
+    t2 = time.perf_counter()
+    h2h_n = 5
+    for _ in range(h2h_n):
+        piv.run_host(fa, fb)
+    h2h = h2h_n / (time.perf_counter() - t2)
+
+    roof = None
+    kernels = None
+    deform = None
+    if not args.skip_kernel_timing:
+        log("per-kernel timing")
+        kernels, deform = stage_kernel_times(torch, fa, piv)
+        dom = max(kernels, key=lambda r: r["avg_ms"])
+        roof = {"bound": "mfma", "compute": "fp32-valu (no matrix cores; FP32 vector peak)",
+                "kernel": dom["kernel"], "achieved": round(dom["tflops"], 3), "peak": 157.3, "unit": "TFLOP/s",
+                "frac": round(dom["tflops"] / 157.3, 4), "traffic": None,
+                "per_launch_flop": dom["flop"], "avg_launch_ms": round(dom["avg_ms"], 5)}
+
+    cpu = None
+    if not args.skip_cpu_baseline:
+        log("CPU baseline (oracle, 1 pair)")
+        cpu = cpu_baseline(fa, fb, 1)
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2048x2048 uint16 pair (12-bit particles), passes 64->32->16 px, 50% overlap, "
+                               "deformation + median(2) + spring + GCV smoothn, plain method",
+                   "pairs_per_step": 1, "graph": not args.no_graph},
+        "rms_px_vs_truth": round(rms, 5),
+        "host_to_host_pairs_per_s": round(h2h, 2),
+        "roofline": roof,
+        "kernels": kernels,
+        "deform_hbm": None if deform is None else {
+            "bound": "hbm", "achieved": round(deform["gbs"], 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(deform["gbs"] / 8000.0, 4), "avg_launch_ms": round(deform["avg_ms"], 5),
+            "per_launch_bytes": deform["bytes"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

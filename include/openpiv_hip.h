@@ -1,0 +1,154 @@
+/*
+ * openpiv_hip.h -- C ABI of the MI355X (gfx950) PIV hot path.
+ *
+ * This is the drop-in boundary under the Python classes piv_gpu / PIVGPU
+ * (openpiv_gpu/gpu_process.py of the reference).  The reference reaches the
+ * GPU through CuPy: NVRTC kernels launched with RawModule.get_function()
+ * (gpu_process.py:561-562, 1136-1145, 1184-1190) plus CuPy library calls
+ * (cuFFT, reductions, nanmedian, spsolve, DCT).  Every entry point below
+ * replaces one of those interfaces (cited per function) or the pass loop
+ * that strings them together.
+ *
+ * Conventions
+ *   - plain C types only; every function returns 0 on success and a negative
+ *     PIV_E_* code on failure, with a message from piv_last_error()
+ *     (thread-local);
+ *   - "device" pointers are HIP device addresses on the plan's device; the
+ *     caller owns all input and output buffers, the library owns plan
+ *     workspaces;
+ *   - a plan is not reentrant: one plan per host thread.
+ */
+#ifndef OPENPIV_HIP_H
+#define OPENPIV_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIV_MAX_PASSES 16
+
+enum piv_status {
+  PIV_OK = 0,
+  PIV_E_INVALID = -1,     /* bad argument or unsupported setting */
+  PIV_E_HIP = -2,         /* HIP runtime error (message has the HIP text) */
+  PIV_E_NO_DEVICE = -3,   /* no usable GPU */
+  PIV_E_NOMEM = -4
+};
+
+enum piv_in_dtype { PIV_U8 = 0, PIV_U16 = 1, PIV_F32 = 2 };
+enum piv_subpixel { PIV_GAUSSIAN = 0, PIV_PARABOLIC = 1, PIV_CENTROID = 2 };
+
+/* One pass of the schedule.  Geometry is computed by the caller with the
+ * closed forms of PIVFieldGPU (gpu_process.py:897-947); the library checks
+ * it for consistency with the frame shape. */
+typedef struct piv_pass_desc {
+  int32_t search_size;        /* window size ss (gpu_process.py:675-680) */
+  int32_t spacing;            /* ss - int(overlap_ratio * ss)           */
+  int32_t x_offset, y_offset; /* centring offsets                        */
+  int32_t field_m, field_n;   /* field shape                             */
+  int32_t fft_size;           /* 2**ceil(log2(ss * n_fft))               */
+  int32_t normalize;          /* subtract window means                   */
+  int32_t subpixel;           /* enum piv_subpixel                       */
+  float deforming_par;        /* gpu_process.py:1165-1171                */
+  int32_t validate;           /* 1 -> median test with median_tol        */
+  float median_tol;
+  int32_t num_replacing_iters;/* 0 -> no spring replacement              */
+  int32_t smooth;             /* smoothn after this pass                 */
+  double smoothing_par;       /* <= 0 -> GCV-chosen                      */
+} piv_pass_desc;
+
+typedef struct piv_plan_desc {
+  int32_t height, width;      /* frame shape                             */
+  int32_t in_dtype;           /* enum piv_in_dtype                       */
+  int32_t n_passes;
+  piv_pass_desc passes[PIV_MAX_PASSES];
+  float out_scale;            /* scaling_par / dt (gpu_process.py:847-849) */
+  int32_t use_graph;          /* capture the pass loop in a hipGraph     */
+} piv_plan_desc;
+
+typedef struct piv_plan piv_plan;
+
+/* Error text of the last failing call on this thread. */
+const char *piv_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+int piv_device_count(void);
+
+/* Builds a plan for one frame shape and pass schedule: allocates every
+ * per-pass buffer and the smoothn tables.  Replaces PIVGPU.__init__ /
+ * init_piv_fields and the two RawModule compilations
+ * (gpu_process.py:465-562, 649-673). */
+int piv_plan_create(int device, const piv_plan_desc *desc, piv_plan **out);
+void piv_plan_destroy(piv_plan *plan);
+
+/* Runs the whole pass loop on one image pair.  Replaces PIVGPU.__call__
+ * (gpu_process.py:564-647) including the host-to-device upload (:582-583).
+ * frame_a/frame_b: height*width elements of in_dtype, on the host
+ * (frames_on_device == 0) or the device.  Outputs (final pass, field_m x
+ * field_n of the last pass, row-major): u, v as float32; val_init = flags
+ * before replacement (PIVGPU.init_val_locations, :770-771); val_final =
+ * flags after replacement (PIVGPU.val_locations, :783-784).  Any output
+ * pointer may be NULL.  outputs_on_device selects host or device outputs.
+ * Synchronous: returns after the outputs are written. */
+int piv_run(piv_plan *plan, const void *frame_a, const void *frame_b, int frames_on_device, float *u, float *v,
+            uint8_t *val_init, uint8_t *val_final, int outputs_on_device);
+
+/* Diagnostics of the last run: smoothing parameter log10(s) chosen after
+ * pass k (NaN when no smoothing ran), and outlier counts before/after
+ * replacement. */
+int piv_plan_last_smoothing(const piv_plan *plan, int pass, double *log10_s);
+int piv_plan_last_counts(const piv_plan *plan, int pass, int32_t *n_before, int32_t *n_after);
+
+/* ---- stage entry points (device pointers, caller-owned; stream may be
+ * NULL for the null stream) -------------------------------------------- */
+
+/* Window extraction + normalisation + zero-padded FFT correlation + peak +
+ * sub-pixel fit for every window of one pass.  Replaces cuda_get_stack
+ * (gpu_process.py:1122-1147, 1420-1449) and CorrelationGPU.__call__'s
+ * correlation chain (:1051-1097, 1201-1310).  up/vp (field_m x field_n) are
+ * added to the result when non-NULL (:631-632).  i_peak/j_peak/planes are
+ * optional (planes: N x F x F fftshift-ed correlation, float32). */
+int piv_stage_correlate(const void *frame_a, const void *frame_b, int in_dtype, int height, int width,
+                        const piv_pass_desc *pass, const float *up, const float *vp, float *u, float *v,
+                        int32_t *i_peak, int32_t *j_peak, float *planes, void *stream);
+
+/* Frame deformation: predictor (field_m x field_n on the pass grid)
+ * interpolated onto every pixel with linear extrapolation, then the backward
+ * bilinear warp of frame A at dt-1 and frame B at dt.  Replaces
+ * CorrelationGPU.deform_frame + cuda_interpolate_frame
+ * (gpu_process.py:1149-1192, 1508-1547).  Outputs are float32 frames. */
+int piv_stage_deform(const void *frame_a, const void *frame_b, int in_dtype, int height, int width,
+                     const piv_pass_desc *pass, const float *up, const float *vp, float *out_a, float *out_b,
+                     void *stream);
+
+/* Predictor interpolation from one pass grid to the next.  Replaces
+ * PIVGPU.interpolate_displacement (gpu_process.py:730-742). */
+int piv_stage_grid_interp(const piv_pass_desc *from, const piv_pass_desc *to, const float *u_in,
+                          const float *v_in, float *u_out, float *v_out, void *stream);
+
+/* 3x3 median validation.  Replaces ValidationGPU.median_validation
+ * (gpu_validation.py:136-141, 166-210).  count (device int32) is
+ * incremented by the number of flagged vectors. */
+int piv_stage_median_validate(const float *u, const float *v, int m, int n, float median_tol, uint8_t *flags,
+                              int32_t *count, void *stream);
+
+/* Spring replacement of the flagged vectors in place.  Replaces
+ * ReplacementGPU spring path + PIVGPU.replace_outliers
+ * (gpu_validation.py:336-371, 381-415; gpu_process.py:788-834).  count is
+ * the device int32 written by piv_stage_median_validate. */
+int piv_stage_spring_replace(float *u, float *v, const uint8_t *flags, const int32_t *count, int m, int n,
+                             void *stream);
+
+/* smoothn of u and v in place; smoothing_par <= 0 selects the GCV search.
+ * log10_s_out (device double, may be NULL) receives the chosen log10(s).
+ * Replaces SmoothnGPU.__call__ (gpu_smoothn.py:123-231, 284-372). */
+int piv_stage_smoothn(float *u, float *v, int m, int n, double smoothing_par, double *log10_s_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENPIV_HIP_H */

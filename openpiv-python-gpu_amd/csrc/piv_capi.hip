@@ -1,0 +1,540 @@
+// C ABI and pass-loop executor (PIVGPU.__call__, gpu_process.py:564-647).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/openpiv_hip.h"
+#include "piv_kernels.h"
+
+using namespace piv;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                           \
+  do {                                                                                          \
+    hipError_t _e = (expr);                                                                     \
+    if (_e != hipSuccess) return fail(PIV_E_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int check_pass(const piv_pass_desc& p, int ht, int wd) {
+  if (p.search_size < 2 || p.spacing < 1) return fail(PIV_E_INVALID, "search_size/spacing out of range");
+  if (!is_pow2(p.fft_size) || p.fft_size < 16 || p.fft_size > 128)
+    return fail(PIV_E_INVALID, "fft_size %d unsupported (power of two in [16, 128])", p.fft_size);
+  if (p.search_size > p.fft_size || (p.fft_size - p.search_size) % 2)
+    return fail(PIV_E_INVALID, "search_size %d incompatible with fft_size %d", p.search_size, p.fft_size);
+  if (p.field_m < 1 || p.field_n < 1) return fail(PIV_E_INVALID, "empty field");
+  if (p.x_offset < 0 || p.y_offset < 0 || (p.field_m - 1) * p.spacing + p.search_size + p.y_offset > ht ||
+      (p.field_n - 1) * p.spacing + p.search_size + p.x_offset > wd)
+    return fail(PIV_E_INVALID, "window grid exceeds the frame");
+  if (p.subpixel < 0 || p.subpixel > 2) return fail(PIV_E_INVALID, "subpixel method");
+  return PIV_OK;
+}
+
+size_t dtype_size(int d) { return d == PIV_U8 ? 1 : (d == PIV_U16 ? 2 : 4); }
+
+// Grid node coordinates of a pass (gpu_process.py:940-947): x = j*sp + ss/2 + x_off.
+float grid_x0(const piv_pass_desc& p) { return (float)p.search_size / 2.0f + (float)p.x_offset; }
+float grid_y0(const piv_pass_desc& p) { return (float)p.search_size / 2.0f + (float)p.y_offset; }
+
+// smoothn p bounds for order 2 (gpu_smoothn.py:320-338).
+void p_bounds(int m, int n, double* pmin, double* pmax) {
+  const double h_min = 1e-3, h_max = 1 - h_min;
+  const int rank = (m != 1) + (n != 1);
+  const double e = 2.0 / rank;
+  auto f = [&](double h) {
+    const double he = pow(h, e);
+    const double t = (1 + sqrt(1 + 8 * he)) / 4 / he;
+    return log10((t * t - 1) / 16);
+  };
+  *pmin = f(h_max);
+  *pmax = f(h_min);
+}
+
+// Orthonormal DCT-II matrix C[k][i] = s_k cos(pi (2i+1) k / (2n)).
+std::vector<float> dct_matrix(int n) {
+  std::vector<float> c((size_t)n * n);
+  for (int k = 0; k < n; ++k) {
+    const double s = k == 0 ? sqrt(1.0 / n) : sqrt(2.0 / n);
+    for (int i = 0; i < n; ++i) c[(size_t)k * n + i] = (float)(s * cos(M_PI * (2 * i + 1) * k / (2.0 * n)));
+  }
+  return c;
+}
+
+// Eigenvalues of the difference operator (gpu_smoothn.py:340-351), squared
+// for smoothing order 2 (:165).
+std::vector<double> lambda_table(int m, int n) {
+  std::vector<double> l((size_t)m * n);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < n; ++j) {
+      const double v = 2.0 * (2.0 - (cos(M_PI * i / m) + cos(M_PI * j / n)));
+      l[(size_t)i * n + j] = v * v;
+    }
+  return l;
+}
+
+struct SmoothTables {
+  float* cos_m = nullptr;
+  float* cos_n = nullptr;
+  double* lambda = nullptr;
+};
+
+int make_tables(int m, int n, SmoothTables* t) {
+  auto cm = dct_matrix(m), cn = dct_matrix(n);
+  auto lam = lambda_table(m, n);
+  HIP_TRY(hipMalloc(&t->cos_m, cm.size() * sizeof(float)));
+  HIP_TRY(hipMalloc(&t->cos_n, cn.size() * sizeof(float)));
+  HIP_TRY(hipMalloc(&t->lambda, lam.size() * sizeof(double)));
+  HIP_TRY(hipMemcpy(t->cos_m, cm.data(), cm.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(t->cos_n, cn.data(), cn.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(t->lambda, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice));
+  return PIV_OK;
+}
+void free_tables(SmoothTables* t) {
+  if (t->cos_m) (void)hipFree(t->cos_m);
+  if (t->cos_n) (void)hipFree(t->cos_n);
+  if (t->lambda) (void)hipFree(t->lambda);
+  *t = SmoothTables{};
+}
+
+CorrArgs corr_args(const void* fa, const void* fb, int dtype, int ht, int wd, const piv_pass_desc& p) {
+  CorrArgs a{};
+  a.frame_a = fa;
+  a.frame_b = fb;
+  a.in_dtype = dtype;
+  a.ht = ht;
+  a.wd = wd;
+  a.ss = p.search_size;
+  a.fsize = p.fft_size;
+  a.spacing = p.spacing;
+  a.x_off = p.x_offset;
+  a.y_off = p.y_offset;
+  a.m = p.field_m;
+  a.n = p.field_n;
+  a.normalize = p.normalize;
+  a.subpixel = p.subpixel;
+  return a;
+}
+
+DeformArgs deform_args(const void* fa, const void* fb, int dtype, int ht, int wd, const piv_pass_desc& p,
+                       const float* up, const float* vp, float* oa, float* ob) {
+  DeformArgs d{};
+  d.frame_a = fa;
+  d.frame_b = fb;
+  d.in_dtype = dtype;
+  d.ht = ht;
+  d.wd = wd;
+  d.up = up;
+  d.vp = vp;
+  d.m = p.field_m;
+  d.n = p.field_n;
+  d.x0 = grid_x0(p);
+  d.y0 = grid_y0(p);
+  d.spacing = (float)p.spacing;
+  // gpu_process.py:1165-1171
+  const float dt = p.deforming_par;
+  if (dt > 0.f && dt < 1.f) {
+    d.warp_a = 1;
+    d.warp_b = 1;
+    d.dt_a = dt - 1.f;
+    d.dt_b = dt;
+  } else if (dt == 1.f) {
+    d.warp_a = 0;
+    d.warp_b = 1;
+    d.dt_b = 1.f;
+  } else {
+    d.warp_a = 1;
+    d.warp_b = 0;
+    d.dt_a = -1.f;
+  }
+  d.out_a = oa;
+  d.out_b = ob;
+  return d;
+}
+
+GridInterpArgs interp_args(const piv_pass_desc& from, const piv_pass_desc& to, const float* ui, const float* vi,
+                           float* uo, float* vo) {
+  GridInterpArgs g{};
+  g.u_in = ui;
+  g.v_in = vi;
+  g.m_in = from.field_m;
+  g.n_in = from.field_n;
+  g.x0_in = grid_x0(from);
+  g.y0_in = grid_y0(from);
+  g.sp_in = (float)from.spacing;
+  g.m_out = to.field_m;
+  g.n_out = to.field_n;
+  g.x0_out = grid_x0(to);
+  g.y0_out = grid_y0(to);
+  g.sp_out = (float)to.spacing;
+  g.u_out = uo;
+  g.v_out = vo;
+  return g;
+}
+
+}  // namespace
+
+struct PassBuffers {
+  piv_pass_desc d{};
+  int N = 0;
+  float* u = nullptr;
+  float* v = nullptr;
+  float* up = nullptr;
+  float* vp = nullptr;
+  uint8_t* flags = nullptr;
+  SmoothTables tab;
+  double p_min = 0, p_max = 0;
+};
+
+struct piv_plan {
+  int device = 0;
+  piv_plan_desc desc{};
+  hipStream_t stream = nullptr;
+  std::vector<PassBuffers> passes;
+  void* frame_a = nullptr;
+  void* frame_b = nullptr;
+  size_t frame_bytes = 0;
+  float* def_a = nullptr;
+  float* def_b = nullptr;
+  int32_t* counters = nullptr;  // 2 per pass
+  double* p_out = nullptr;      // 1 per pass
+  uint8_t* val_final = nullptr;
+  float* t0 = nullptr;
+  float* t1 = nullptr;
+  ReplaceWork rw{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int runs = 0;
+};
+
+namespace {
+
+int enqueue(piv_plan* P) {
+  hipStream_t s = P->stream;
+  const piv_plan_desc& D = P->desc;
+  const int np = (int)P->passes.size();
+  HIP_TRY(launch_fill_i32(P->counters, 0, 2 * np, s));
+  for (int k = 0; k < np; ++k) {
+    PassBuffers& B = P->passes[k];
+    const bool last = k == np - 1;
+    if (k == 0) {
+      CorrArgs a = corr_args(P->frame_a, P->frame_b, D.in_dtype, D.height, D.width, B.d);
+      a.u = B.u;
+      a.v = B.v;
+      HIP_TRY(launch_correlate(a, s));
+    } else {
+      PassBuffers& A = P->passes[k - 1];
+      // gpu_process.py:620-625 -- predictor on the current grid
+      if (A.d.search_size != B.d.search_size) {
+        HIP_TRY(launch_grid_interp(interp_args(A.d, B.d, A.u, A.v, B.up, B.vp), s));
+      } else {
+        HIP_TRY(launch_copy_f32(B.up, A.u, B.N, s));
+        HIP_TRY(launch_copy_f32(B.vp, A.v, B.N, s));
+      }
+      // gpu_process.py:1113-1114, Q13: the original frames are deformed by
+      // the total predictor
+      HIP_TRY(launch_deform(
+          deform_args(P->frame_a, P->frame_b, D.in_dtype, D.height, D.width, B.d, B.up, B.vp, P->def_a, P->def_b),
+          s));
+      CorrArgs a = corr_args(P->def_a, P->def_b, kF32, D.height, D.width, B.d);
+      a.up = B.up;
+      a.vp = B.vp;
+      a.u = B.u;
+      a.v = B.v;
+      HIP_TRY(launch_correlate(a, s));
+    }
+    if (B.d.validate) {
+      HIP_TRY(launch_median_validate(B.u, B.v, B.d.field_m, B.d.field_n, B.d.median_tol, B.flags,
+                                     P->counters + 2 * k, s));
+      if (B.d.num_replacing_iters > 0) {
+        HIP_TRY(launch_spring_replace(B.u, B.v, B.flags, P->counters + 2 * k, B.d.field_m, B.d.field_n, P->rw, s));
+      }
+      if (last) {
+        HIP_TRY(launch_median_validate(B.u, B.v, B.d.field_m, B.d.field_n, B.d.median_tol, P->val_final,
+                                       P->counters + 2 * k + 1, s));
+      }
+    }
+    if (!last && B.d.smooth) {
+      SmoothnWork w{};
+      w.cos_m = B.tab.cos_m;
+      w.cos_n = B.tab.cos_n;
+      w.lambda = B.tab.lambda;
+      w.t0 = P->t0;
+      w.t1 = P->t1;
+      w.p_out = P->p_out + k;
+      const int autos = !(B.d.smoothing_par > 0);
+      HIP_TRY(launch_smoothn(B.u, B.v, B.d.field_m, B.d.field_n, B.p_min, B.p_max,
+                             autos ? 0.0 : log10(B.d.smoothing_par), autos, w, s));
+    }
+  }
+  if (D.out_scale != 1.0f) {
+    PassBuffers& L = P->passes.back();
+    HIP_TRY(launch_scale(L.u, L.v, L.N, D.out_scale, s));
+  }
+  return PIV_OK;
+}
+
+void release(piv_plan* P) {
+  if (!P) return;
+  (void)hipSetDevice(P->device);
+  if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  for (auto& B : P->passes) {
+    for (void* p : {(void*)B.u, (void*)B.v, (void*)B.up, (void*)B.vp, (void*)B.flags})
+      if (p) (void)hipFree(p);
+    free_tables(&B.tab);
+  }
+  for (void* p : {P->frame_a, P->frame_b, (void*)P->def_a, (void*)P->def_b, (void*)P->counters, (void*)P->p_out,
+                  (void*)P->val_final, (void*)P->t0, (void*)P->t1, (void*)P->rw.slot, (void*)P->rw.list,
+                  (void*)P->rw.work})
+    if (p) (void)hipFree(p);
+  if (P->stream) (void)hipStreamDestroy(P->stream);
+  delete P;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* piv_last_error(void) { return g_err.c_str(); }
+
+int piv_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
+  if (!desc || !out) return fail(PIV_E_INVALID, "null argument");
+  *out = nullptr;
+  if (piv_device_count() <= 0) return fail(PIV_E_NO_DEVICE, "no HIP device visible");
+  const piv_plan_desc& D = *desc;
+  if (D.n_passes < 1 || D.n_passes > PIV_MAX_PASSES) return fail(PIV_E_INVALID, "n_passes out of range");
+  if (D.height < 2 || D.width < 2) return fail(PIV_E_INVALID, "frame too small");
+  if (D.in_dtype < PIV_U8 || D.in_dtype > PIV_F32) return fail(PIV_E_INVALID, "in_dtype");
+  for (int k = 0; k < D.n_passes; ++k) {
+    int r = check_pass(D.passes[k], D.height, D.width);
+    if (r) return r;
+    if (k > 0 && (D.passes[k].field_m < 2 || D.passes[k].field_n < 2 || D.passes[k - 1].field_m < 2 ||
+                  D.passes[k - 1].field_n < 2))
+      return fail(PIV_E_INVALID, "multi-pass schedules need fields of at least 2 x 2");
+  }
+  HIP_TRY(hipSetDevice(device));
+  piv_plan* P = new piv_plan();
+  P->device = device;
+  P->desc = D;
+  auto bail = [&](int code) {
+    release(P);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(PIV_E_HIP, "hipStreamCreate failed"));
+  const size_t npx = (size_t)D.height * D.width;
+  P->frame_bytes = npx * dtype_size(D.in_dtype);
+  int maxN = 0;
+  auto dmalloc = [&](void** p, size_t bytes) -> bool { return hipMalloc(p, bytes ? bytes : 16) == hipSuccess; };
+  bool ok = dmalloc(&P->frame_a, P->frame_bytes) && dmalloc(&P->frame_b, P->frame_bytes);
+  if (D.n_passes > 1) ok = ok && dmalloc((void**)&P->def_a, npx * 4) && dmalloc((void**)&P->def_b, npx * 4);
+  ok = ok && dmalloc((void**)&P->counters, 2 * D.n_passes * sizeof(int32_t)) &&
+       dmalloc((void**)&P->p_out, D.n_passes * sizeof(double));
+  if (!ok) return bail(fail(PIV_E_NOMEM, "device allocation failed"));
+  {
+    std::vector<double> nanv(D.n_passes, NAN);
+    if (hipMemcpy(P->p_out, nanv.data(), nanv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(PIV_E_HIP, "hipMemcpy failed"));
+  }
+  P->passes.resize(D.n_passes);
+  for (int k = 0; k < D.n_passes; ++k) {
+    PassBuffers& B = P->passes[k];
+    B.d = D.passes[k];
+    B.N = B.d.field_m * B.d.field_n;
+    if (B.N > maxN) maxN = B.N;
+    ok = dmalloc((void**)&B.u, B.N * 4) && dmalloc((void**)&B.v, B.N * 4) && dmalloc((void**)&B.up, B.N * 4) &&
+         dmalloc((void**)&B.vp, B.N * 4) && dmalloc((void**)&B.flags, B.N);
+    if (!ok) return bail(fail(PIV_E_NOMEM, "device allocation failed"));
+    if (hipMemset(B.flags, 0, B.N) != hipSuccess) return bail(fail(PIV_E_HIP, "hipMemset failed"));
+    if (k < D.n_passes - 1 && B.d.smooth) {
+      int r = make_tables(B.d.field_m, B.d.field_n, &B.tab);
+      if (r) return bail(r);
+      p_bounds(B.d.field_m, B.d.field_n, &B.p_min, &B.p_max);
+    }
+  }
+  const PassBuffers& L = P->passes.back();
+  ok = dmalloc((void**)&P->val_final, L.N) && dmalloc((void**)&P->t0, 2ull * maxN * 4) &&
+       dmalloc((void**)&P->t1, 2ull * maxN * 4) && dmalloc((void**)&P->rw.slot, maxN * 4) &&
+       dmalloc((void**)&P->rw.list, maxN * 4) && dmalloc((void**)&P->rw.work, 8ull * maxN * 8);
+  if (!ok) return bail(fail(PIV_E_NOMEM, "device allocation failed"));
+  if (hipMemset(P->val_final, 0, L.N) != hipSuccess) return bail(fail(PIV_E_HIP, "hipMemset failed"));
+  *out = P;
+  return PIV_OK;
+}
+
+void piv_plan_destroy(piv_plan* plan) { release(plan); }
+
+int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on_device, float* u, float* v,
+            uint8_t* val_init, uint8_t* val_final, int outputs_on_device) {
+  if (!P || !frame_a || !frame_b) return fail(PIV_E_INVALID, "null argument");
+  HIP_TRY(hipSetDevice(P->device));
+  hipStream_t s = P->stream;
+  const hipMemcpyKind in_kind = frames_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  HIP_TRY(hipMemcpyAsync(P->frame_a, frame_a, P->frame_bytes, in_kind, s));
+  HIP_TRY(hipMemcpyAsync(P->frame_b, frame_b, P->frame_bytes, in_kind, s));
+  // The first run is eager (it also sets kernel attributes); later runs
+  // replay a captured graph when requested.
+  if (P->desc.use_graph && P->runs > 0) {
+    if (!P->graph_exec) {
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int r = enqueue(P);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+      }
+      if (e != hipSuccess) return fail(PIV_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+      P->graph = g;
+      HIP_TRY(hipGraphInstantiate(&P->graph_exec, P->graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(P->graph_exec, s));
+  } else {
+    int r = enqueue(P);
+    if (r) return r;
+  }
+  P->runs++;
+  const PassBuffers& L = P->passes.back();
+  const hipMemcpyKind out_kind = outputs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (u) HIP_TRY(hipMemcpyAsync(u, L.u, L.N * sizeof(float), out_kind, s));
+  if (v) HIP_TRY(hipMemcpyAsync(v, L.v, L.N * sizeof(float), out_kind, s));
+  if (val_init) HIP_TRY(hipMemcpyAsync(val_init, L.flags, L.N, out_kind, s));
+  if (val_final) HIP_TRY(hipMemcpyAsync(val_final, P->val_final, L.N, out_kind, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return PIV_OK;
+}
+
+int piv_plan_last_smoothing(const piv_plan* P, int pass, double* log10_s) {
+  if (!P || !log10_s || pass < 0 || pass >= (int)P->passes.size()) return fail(PIV_E_INVALID, "bad argument");
+  HIP_TRY(hipSetDevice(P->device));
+  HIP_TRY(hipMemcpy(log10_s, P->p_out + pass, sizeof(double), hipMemcpyDeviceToHost));
+  return PIV_OK;
+}
+
+int piv_plan_last_counts(const piv_plan* P, int pass, int32_t* n_before, int32_t* n_after) {
+  if (!P || pass < 0 || pass >= (int)P->passes.size()) return fail(PIV_E_INVALID, "bad argument");
+  HIP_TRY(hipSetDevice(P->device));
+  int32_t c[2];
+  HIP_TRY(hipMemcpy(c, P->counters + 2 * pass, sizeof(c), hipMemcpyDeviceToHost));
+  if (n_before) *n_before = c[0];
+  if (n_after) *n_after = c[1];
+  return PIV_OK;
+}
+
+int piv_stage_correlate(const void* frame_a, const void* frame_b, int in_dtype, int height, int width,
+                        const piv_pass_desc* pass, const float* up, const float* vp, float* u, float* v,
+                        int32_t* i_peak, int32_t* j_peak, float* planes, void* stream) {
+  if (!frame_a || !frame_b || !pass || !u || !v) return fail(PIV_E_INVALID, "null argument");
+  if (in_dtype < PIV_U8 || in_dtype > PIV_F32) return fail(PIV_E_INVALID, "in_dtype");
+  int r = check_pass(*pass, height, width);
+  if (r) return r;
+  CorrArgs a = corr_args(frame_a, frame_b, in_dtype, height, width, *pass);
+  a.up = up;
+  a.vp = vp;
+  a.u = u;
+  a.v = v;
+  a.i_peak = i_peak;
+  a.j_peak = j_peak;
+  a.planes = planes;
+  HIP_TRY(launch_correlate(a, (hipStream_t)stream));
+  return PIV_OK;
+}
+
+int piv_stage_deform(const void* frame_a, const void* frame_b, int in_dtype, int height, int width,
+                     const piv_pass_desc* pass, const float* up, const float* vp, float* out_a, float* out_b,
+                     void* stream) {
+  if (!frame_a || !frame_b || !pass || !up || !vp || !out_a || !out_b) return fail(PIV_E_INVALID, "null argument");
+  int r = check_pass(*pass, height, width);
+  if (r) return r;
+  if (pass->field_m < 2 || pass->field_n < 2) return fail(PIV_E_INVALID, "deformation needs a 2 x 2 grid");
+  HIP_TRY(launch_deform(deform_args(frame_a, frame_b, in_dtype, height, width, *pass, up, vp, out_a, out_b),
+                        (hipStream_t)stream));
+  return PIV_OK;
+}
+
+int piv_stage_grid_interp(const piv_pass_desc* from, const piv_pass_desc* to, const float* u_in, const float* v_in,
+                          float* u_out, float* v_out, void* stream) {
+  if (!from || !to || !u_in || !v_in || !u_out || !v_out) return fail(PIV_E_INVALID, "null argument");
+  if (from->field_m < 2 || from->field_n < 2) return fail(PIV_E_INVALID, "source grid must be at least 2 x 2");
+  HIP_TRY(launch_grid_interp(interp_args(*from, *to, u_in, v_in, u_out, v_out), (hipStream_t)stream));
+  return PIV_OK;
+}
+
+int piv_stage_median_validate(const float* u, const float* v, int m, int n, float median_tol, uint8_t* flags,
+                              int32_t* count, void* stream) {
+  if (!u || !v || !flags || !count || m < 1 || n < 1) return fail(PIV_E_INVALID, "bad argument");
+  HIP_TRY(launch_median_validate(u, v, m, n, median_tol, flags, count, (hipStream_t)stream));
+  return PIV_OK;
+}
+
+int piv_stage_spring_replace(float* u, float* v, const uint8_t* flags, const int32_t* count, int m, int n,
+                             void* stream) {
+  if (!u || !v || !flags || !count || m < 1 || n < 1) return fail(PIV_E_INVALID, "bad argument");
+  const size_t N = (size_t)m * n;
+  ReplaceWork w{};
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMalloc((void**)&w.slot, N * 4));
+  HIP_TRY(hipMalloc((void**)&w.list, N * 4));
+  HIP_TRY(hipMalloc((void**)&w.work, 8 * N * 8));
+  hipError_t e = launch_spring_replace(u, v, flags, count, m, n, w, s);
+  hipError_t e2 = hipStreamSynchronize(s);
+  (void)hipFree(w.slot);
+  (void)hipFree(w.list);
+  (void)hipFree(w.work);
+  if (e != hipSuccess) return fail(PIV_E_HIP, "spring replace: %s", hipGetErrorString(e));
+  if (e2 != hipSuccess) return fail(PIV_E_HIP, "spring replace: %s", hipGetErrorString(e2));
+  return PIV_OK;
+}
+
+int piv_stage_smoothn(float* u, float* v, int m, int n, double smoothing_par, double* log10_s_out, void* stream) {
+  if (!u || !v || m < 1 || n < 1) return fail(PIV_E_INVALID, "bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  SmoothTables tab;
+  int r = make_tables(m, n, &tab);
+  if (r) {
+    free_tables(&tab);
+    return r;
+  }
+  const size_t N = (size_t)m * n;
+  float *t0 = nullptr, *t1 = nullptr;
+  HIP_TRY(hipMalloc(&t0, 2 * N * 4));
+  HIP_TRY(hipMalloc(&t1, 2 * N * 4));
+  double pmin, pmax;
+  p_bounds(m, n, &pmin, &pmax);
+  SmoothnWork w{tab.cos_m, tab.cos_n, tab.lambda, t0, t1, log10_s_out};
+  const int autos = !(smoothing_par > 0);
+  hipError_t e = launch_smoothn(u, v, m, n, pmin, pmax, autos ? 0.0 : log10(smoothing_par), autos, w, s);
+  hipError_t e2 = hipStreamSynchronize(s);
+  (void)hipFree(t0);
+  (void)hipFree(t1);
+  free_tables(&tab);
+  if (e != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e));
+  if (e2 != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e2));
+  return PIV_OK;
+}
+
+}  // extern "C"

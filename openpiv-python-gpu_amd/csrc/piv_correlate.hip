@@ -1,0 +1,478 @@
+// Fused window correlation for one PIV pass (gfx950, wave64).
+//
+// Semantics follow CorrelationGPU (gpu_process.py:1021-1310): windows are
+// extracted on the pass grid (cuda_get_stack, :1420-1449), mean-normalised
+// over ss x ss (:1201-1205, forced on whenever padding is nonzero, :1065),
+// zero-padded to F x F with (F-ss)/2 on each side (:1222-1235), correlated
+// as irfft2(conj(rfft2(a)) * rfft2(b)) (:1237-1246), fftshift-ed (:1218),
+// reduced to a first-occurrence argmax (:1248-1254) and a 3-point sub-pixel
+// fit (:1256-1294), and turned into u = j_sp - F/2, v = i_sp - F/2
+// (:1296-1310).  No plane ever touches HBM.
+//
+// Mapping: 2F lanes per window, 256 threads per workgroup.  One 1-D FFT of
+// length F lives in 2 lanes (F/2 points each, forward rows/columns) or in 4
+// lanes (F/4 points each, inverse columns/rows); LDS only carries the two
+// transposes and the final plane.
+//
+//   stage  what                                   lanes per FFT   LDS
+//   R      rows of z = a + i b (ss nonzero rows)  2               staging -> AB
+//   S      split z-row spectra into A and B rows  (in R's lanes)
+//   C      columns of A and B (one quad per kx)   2               AB -> regs
+//   P      P = conj(FA) FB, DC/Nyquist packed     (quad)
+//   IC     inverse columns of P                   4               regs -> R
+//   IR     inverse rows, two rows per transform   4               R -> plane
+//   peak   argmax + sub-pixel                                     plane
+#include "piv_fft.h"
+#include "piv_kernels.h"
+
+namespace piv {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ float load_px(const void* base, size_t idx) {
+  return (float)(reinterpret_cast<const T*>(base)[idx]);
+}
+
+struct Best {
+  float v;
+  int i;
+};
+__device__ __forceinline__ Best better(Best a, Best b) {
+  // larger value wins; equal values -> smaller flat index (first occurrence)
+  if (b.v > a.v || (b.v == a.v && b.i < a.i)) return b;
+  return a;
+}
+
+template <int F>
+struct Layout {
+  static constexpr int H = F / 2;
+  static constexpr int Q = F / 4;
+  static constexpr int LANES = 2 * F;
+  static constexpr int WPB = 256 / LANES;  // windows per workgroup
+  static constexpr int SF = F + 1;         // AB row stride (complex)
+  static constexpr int SR = H + 1;         // R row stride (complex)
+  static constexpr int SP = F + 1;         // plane row stride (float)
+};
+
+}  // namespace
+
+size_t correlate_lds_bytes(int fsize, int ss) {
+  size_t staging = 2ull * ss * (ss + 1) * sizeof(float);
+  size_t ab = (size_t)ss * (fsize + 1) * 8;
+  size_t r = (size_t)fsize * (fsize / 2 + 1) * 8;
+  size_t plane = (size_t)fsize * (fsize + 1) * sizeof(float);
+  size_t m = staging;
+  if (ab > m) m = ab;
+  if (r > m) m = r;
+  if (plane > m) m = plane;
+  return (m + 15) & ~size_t(15);
+}
+
+template <int F, typename InT>
+__global__ __launch_bounds__(256) void corr_kernel(CorrArgs a, int n_windows, int region_floats) {
+  using L = Layout<F>;
+  constexpr int H = L::H, Q = L::Q, LANES = L::LANES, WPB = L::WPB;
+  constexpr int SF = L::SF, SR = L::SR, SP = L::SP;
+
+  extern __shared__ float4 smem4[];
+  float* smem = reinterpret_cast<float*>(smem4);
+  __shared__ float red_f[8][2];
+  __shared__ Best red_b[8];
+
+  const int tid = threadIdx.x;
+  const int wl = tid / LANES;  // window slot in this workgroup
+  const int t = tid % LANES;   // lane within the window
+  const int wave = tid >> 6;
+
+  // XCD-aware block order: consecutive hardware blocks go round-robin over 8
+  // XCDs; give each XCD a contiguous band of windows so overlapping windows
+  // share an L2.
+  int bid = blockIdx.x;
+  {
+    const int nb = gridDim.x;
+    const int per = nb / 8;
+    if (per > 0 && bid < per * 8) bid = (bid & 7) * per + (bid >> 3);
+  }
+  const int win = bid * WPB + wl;
+  const bool valid = win < n_windows;
+  const int W = a.ss;
+  const int pad = (F - W) / 2;
+  const int SW = W + 1;
+
+  float* region = smem + (size_t)wl * region_floats;
+  float* stA = region;
+  float* stB = region + W * SW;
+  c32* AB = reinterpret_cast<c32*>(region);
+  c32* R = reinterpret_cast<c32*>(region);
+  float* plane = region;
+
+  const int wrow = valid ? (win / a.n) : 0;
+  const int wcol = valid ? (win % a.n) : 0;
+  const int row0 = wrow * a.spacing + a.y_off;
+  const int col0 = wcol * a.spacing + a.x_off;
+
+  // ---- staging: coalesced window load + partial sums --------------------
+  float sa = 0.f, sb = 0.f;
+  {
+    const int total = W * W;
+    int r = t / W, c = t % W;
+    const int dr = LANES / W, dc = LANES % W;
+    for (int e = t; e < total; e += LANES) {
+      float va = 0.f, vb = 0.f;
+      if (valid) {
+        size_t gi = (size_t)(row0 + r) * a.wd + (col0 + c);
+        va = load_px<InT>(a.frame_a, gi);
+        vb = load_px<InT>(a.frame_b, gi);
+      }
+      stA[r * SW + c] = va;
+      stB[r * SW + c] = vb;
+      sa += va;
+      sb += vb;
+      r += dr;
+      c += dc;
+      if (c >= W) {
+        c -= W;
+        r += 1;
+      }
+    }
+  }
+  // window-group sums
+  constexpr int WIDTH = LANES < 64 ? LANES : 64;
+#pragma unroll
+  for (int off = WIDTH / 2; off > 0; off >>= 1) {
+    sa += __shfl_xor(sa, off, 64);
+    sb += __shfl_xor(sb, off, 64);
+  }
+  if constexpr (LANES > 64) {
+    if ((tid & 63) == 0) {
+      red_f[wave][0] = sa;
+      red_f[wave][1] = sb;
+    }
+  }
+  __syncthreads();
+  if constexpr (LANES > 64) {
+    constexpr int WPW = LANES / 64;  // waves per window
+    const int w0 = (tid / LANES) * WPW;
+    sa = 0.f;
+    sb = 0.f;
+    for (int k = 0; k < WPW; ++k) {
+      sa += red_f[w0 + k][0];
+      sb += red_f[w0 + k][1];
+    }
+  }
+  float mean_a = 0.f, mean_b = 0.f;
+  if (a.normalize || pad != 0) {
+    const float cnt = (float)(W * W);
+    mean_a = sa / cnt;
+    mean_b = sb / cnt;
+  }
+
+  // ---- stage R: row FFTs of z = (a - mean_a) + i (b - mean_b) -----------
+  c32 x[H];
+  {
+    const int q = t >> 1;
+    const int h = t & 1;
+    const bool active = q < W;
+    const int rr = active ? q : 0;
+    static_for<0, H>([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      const int c = 2 * m + h - pad;
+      c32 zz{0.f, 0.f};
+      if (active && c >= 0 && c < W) {
+        zz.x = stA[rr * SW + c] - mean_a;
+        zz.y = stB[rr * SW + c] - mean_b;
+      }
+      x[m] = zz;
+    });
+  }
+  __syncthreads();  // staging (aliased by AB) fully consumed
+  {
+    const int q = t >> 1;
+    const bool up1 = t & 1;
+    fft_reg<H, -1>(x);
+    dit2_combine<F, -1>(x, up1);
+    // split: lane 0 -> A slots, lane 1 -> B slots; slot 0 packs (DC, Nyquist)
+    const bool active = q < W;
+    c32* row = AB + (size_t)(active ? q : 0) * SF + (up1 ? H : 0);
+    {
+      c32 o = x[0];
+      c32 w = swap1(o);
+      c32 z0 = up1 ? w : o;  // Zr[0]
+      c32 zn = up1 ? o : w;  // Zr[F/2]
+      c32 sv = up1 ? c32{z0.y, zn.y} : c32{z0.x, zn.x};
+      if (active) row[0] = sv;
+    }
+    static_for<1, H>([&](auto K) {
+      constexpr int kx = decltype(K)::value;
+      constexpr int r1 = bitrev(kx, H);
+      constexpr int r2 = bitrev(H - kx, H);
+      c32 o1 = x[r1], w1 = swap1(o1);
+      c32 o2 = x[r2], w2 = swap1(o2);
+      c32 zk = up1 ? w1 : o1;  // Zr[kx]    (lane 0, k = kx)
+      c32 zm = up1 ? o2 : w2;  // Zr[F-kx]  (lane 1, k = H-kx)
+      c32 sv = up1 ? c32{0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x)}
+                   : c32{0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y)};
+      if (active) row[kx] = sv;
+    });
+  }
+  __syncthreads();
+
+  // ---- stage C: column FFTs (quad = {A h0, A h1, B h0, B h1} per kx) ------
+  const int kx = t >> 2;
+  const int sel = (t >> 1) & 1;  // 0 -> A column, 1 -> B column
+  const bool upper = t & 1;
+  {
+    const int col = sel * H + kx;
+    static_for<0, H>([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      const int r = 2 * m + (upper ? 1 : 0) - pad;
+      c32 zz{0.f, 0.f};
+      if (r >= 0 && r < W) zz = AB[(size_t)r * SF + col];
+      x[m] = zz;
+    });
+  }
+  __syncthreads();  // AB fully consumed (R aliases it)
+  fft_reg<H, -1>(x);
+  dit2_combine<F, -1>(x, upper);
+
+  // ---- stage P: spectrum product -------------------------------------------
+  if (kx != 0) {
+    static_for<0, H>([&](auto RR) {
+      constexpr int r = decltype(RR)::value;
+      c32 own = x[r];
+      c32 oth = swap2(own);
+      c32 fa = sel ? oth : own;
+      c32 fb = sel ? own : oth;
+      x[r] = cconjmul(fa, fb);
+    });
+  } else {
+    // Column slot 0 holds FFT(col0 + i colN) of the real DC and Nyquist
+    // columns; unpack with the ky -> -ky mirror (partner lane), form both
+    // products and repack q = P0 + i PN (both columns invert to real data).
+    auto unpack = [](c32 qv, c32 qm, c32& f0, c32& fn) {
+      // f0 = (q + conj(qm)) / 2 ; fn = (q - conj(qm)) / (2i)
+      f0 = c32{0.5f * (qv.x + qm.x), 0.5f * (qv.y - qm.y)};
+      fn = c32{0.5f * (qv.y + qm.y), 0.5f * (qm.x - qv.x)};
+    };
+    auto product = [&](c32 f0, c32 fn) -> c32 {
+      c32 f0o = swap2(f0), fno = swap2(fn);
+      c32 a0 = sel ? f0o : f0, b0 = sel ? f0 : f0o;
+      c32 an = sel ? fno : fn, bn = sel ? fn : fno;
+      c32 p0 = cconjmul(a0, b0);
+      c32 pn = cconjmul(an, bn);
+      return c32{p0.x - pn.y, p0.y + pn.x};
+    };
+    c32 f0, fn;
+    unpack(x[0], x[0], f0, fn);  // local ky = 0 mirrors itself
+    c32 res0 = product(f0, fn);
+    static_for<1, H / 2 + 1>([&](auto KY) {
+      constexpr int ky = decltype(KY)::value;
+      constexpr int ky2 = H - ky;
+      constexpr int ra = bitrev(ky, H), rb = bitrev(ky2, H);
+      c32 qa = x[ra], qb = x[rb];
+      c32 ma = swap1(qb);  // partner local H-ky == mirror of local ky
+      c32 mb = swap1(qa);
+      c32 fa0, fan, fb0, fbn;
+      unpack(qa, ma, fa0, fan);
+      unpack(qb, mb, fb0, fbn);
+      c32 pa = product(fa0, fan);
+      c32 pb = product(fb0, fbn);
+      x[ra] = pa;
+      if constexpr (ky2 != ky) x[rb] = pb;
+    });
+    x[0] = res0;
+  }
+
+  // ---- stage IC: inverse column FFT of P over the quad (4-lane DIF) -------
+  c32 z[Q];
+  dif2_split<F, +1>(x, upper);
+  static_for<0, Q>([&](auto NN) {
+    constexpr int nn = decltype(NN)::value;
+    c32 y0 = x[bitrev(nn, H)];
+    c32 y1 = x[bitrev(nn + Q, H)];
+    z[nn] = sel ? twiddle<nn, H, +1>(csub(y0, y1)) : cadd(y0, y1);
+  });
+  fft_reg<Q, +1>(z);
+  // lane (kx, sel, h) now holds R[4k + 2 sel + h][kx] at z[bitrev(k, Q)]
+  static_for<0, Q>([&](auto KK) {
+    constexpr int k = decltype(KK)::value;
+    const int yy = 4 * k + 2 * sel + (upper ? 1 : 0);
+    R[(size_t)yy * SR + kx] = z[bitrev(k, Q)];
+  });
+  __syncthreads();
+
+  // ---- stage IR: inverse rows, rows (2j, 2j+1) packed as real/imag --------
+  const int jpair = t >> 2;
+  const int g = t & 3;
+  {
+    const int y1 = 2 * jpair, y2 = y1 + 1;
+    static_for<0, Q>([&](auto MM) {
+      constexpr int m = decltype(MM)::value;
+      const int kxx = 4 * m + g;
+      const int kk = kxx <= H ? kxx : F - kxx;
+      const int slot = kk == H ? 0 : kk;
+      c32 l1 = R[(size_t)y1 * SR + slot];
+      c32 l2 = R[(size_t)y2 * SR + slot];
+      c32 sv;
+      if (kxx == 0)
+        sv = c32{l1.x, l2.x};
+      else if (kxx == H)
+        sv = c32{l1.y, l2.y};
+      else if (kxx < H)
+        sv = c32{l1.x - l2.y, l1.y + l2.x};
+      else
+        sv = c32{l1.x + l2.y, l2.x - l1.y};
+      z[m] = sv;
+    });
+  }
+  fft_reg<Q, +1>(z);
+  {
+    const bool hi2 = g & 2;
+    const bool hi1 = g & 1;
+    // combine A (lanes g <-> g^2): half-length DIT butterflies
+    static_for<0, Q>([&](auto II) {
+      constexpr int i = decltype(II)::value;
+      constexpr int r = bitrev(i, Q);
+      c32 own = z[r];
+      c32 oth = swap2(own);
+      c32 lo = hi2 ? oth : own;
+      c32 hi = hi2 ? own : oth;
+      c32 wh = twiddle<i, H, +1>(hi);
+      z[r] = hi2 ? csub(lo, wh) : cadd(lo, wh);
+    });
+    // combine B (lanes g <-> g^1)
+    static_for<0, Q>([&](auto II) {
+      constexpr int i = decltype(II)::value;
+      constexpr int r = bitrev(i, Q);
+      c32 own = z[r];
+      c32 oth = swap1(own);
+      c32 e = hi1 ? oth : own;
+      c32 o = hi1 ? own : oth;
+      c32 wo = twiddle<i, F, +1>(o);
+      if (hi2) wo = c32{-wo.y, wo.x};  // extra factor w_F^{F/4} = i
+      z[r] = hi1 ? csub(e, wo) : cadd(e, wo);
+    });
+  }
+  __syncthreads();  // R fully consumed (plane aliases it)
+
+  // ---- peak search over the fftshift-ed plane -----------------------------
+  const float scale = 1.0f / (float)(F * F);
+  const int beta = (g == 0) ? 0 : (g == 1 ? 2 : (g == 2 ? 1 : 3));
+  Best best{-INFINITY, 0x7fffffff};
+  {
+    const int i1 = (2 * jpair) ^ H;
+    const int i2 = (2 * jpair + 1) ^ H;
+    static_for<0, Q>([&](auto KK) {
+      constexpr int k = decltype(KK)::value;
+      const int xc = k + beta * Q;
+      const int jj = xc ^ H;
+      c32 val = z[bitrev(k, Q)];
+      float c1 = val.x * scale, c2 = val.y * scale;
+      plane[i1 * SP + jj] = c1;
+      plane[i2 * SP + jj] = c2;
+      best = better(best, Best{c1, i1 * F + jj});
+      best = better(best, Best{c2, i2 * F + jj});
+      if (a.planes && valid) {
+        float* gp = a.planes + (size_t)win * F * F;
+        gp[i1 * F + jj] = c1;
+        gp[i2 * F + jj] = c2;
+      }
+    });
+  }
+#pragma unroll
+  for (int off = WIDTH / 2; off > 0; off >>= 1) {
+    Best o{__shfl_xor(best.v, off, 64), __shfl_xor(best.i, off, 64)};
+    best = better(best, o);
+  }
+  if constexpr (LANES > 64) {
+    if ((tid & 63) == 0) red_b[wave] = best;
+  }
+  __syncthreads();
+  if (t == 0 && valid) {
+    if constexpr (LANES > 64) {
+      constexpr int WPW = LANES / 64;
+      const int w0 = (tid / LANES) * WPW;
+      best = red_b[w0];
+      for (int k = 1; k < WPW; ++k) best = better(best, red_b[w0 + k]);
+    }
+    const int ip = best.i / F;
+    const int jp = best.i % F;
+    float isp = (float)ip, jsp = (float)jp;
+    if (ip > 0 && ip < F - 1 && jp > 0 && jp < F - 1) {
+      const float c = plane[ip * SP + jp];
+      const float cd = plane[(ip - 1) * SP + jp];
+      const float cu = plane[(ip + 1) * SP + jp];
+      const float cl = plane[ip * SP + jp - 1];
+      const float cr = plane[ip * SP + jp + 1];
+      const float eps = 1e-7f;
+      if (a.subpixel == kGaussian) {
+        if (c > 0.f && cd > 0.f && cu > 0.f && cl > 0.f && cr > 0.f) {
+          const float lc = logf(c), lcd = logf(cd), lcu = logf(cu), lcl = logf(cl), lcr = logf(cr);
+          isp = (float)ip + 0.5f * (lcd - lcu) / (lcd - 2.0f * lc + lcu + eps);
+          jsp = (float)jp + 0.5f * (lcl - lcr) / (lcl - 2.0f * lc + lcr + eps);
+        }
+      } else if (a.subpixel == kParabolic) {
+        isp = (float)ip + 0.5f * (cd - cu) / (cd - 2.0f * c + cu + eps);
+        jsp = (float)jp + 0.5f * (cl - cr) / (cl - 2.0f * c + cr + eps);
+      } else {
+        isp = (float)ip + (cu - cd) / (cd + c + cu + eps);
+        jsp = (float)jp + (cr - cl) / (cl + c + cr + eps);
+      }
+    }
+    float uu = jsp - (float)(F / 2);
+    float vv = isp - (float)(F / 2);
+    if (a.up) {
+      uu += a.up[win];
+      vv += a.vp[win];
+    }
+    a.u[win] = uu;
+    a.v[win] = vv;
+    if (a.i_peak) {
+      a.i_peak[win] = ip;
+      a.j_peak[win] = jp;
+    }
+  }
+}
+
+template <int F, typename InT>
+static hipError_t launch_f(const CorrArgs& a, hipStream_t stream) {
+  using L = Layout<F>;
+  const int n_windows = a.m * a.n;
+  const size_t region = correlate_lds_bytes(F, a.ss);
+  const size_t lds = region * L::WPB;
+  const int blocks = (n_windows + L::WPB - 1) / L::WPB;
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)corr_kernel<F, InT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds > 65536 ? 160 * 1024 : 65536);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((corr_kernel<F, InT>), dim3(blocks), dim3(256), lds, stream, a, n_windows,
+                     (int)(region / sizeof(float)));
+  return hipGetLastError();
+}
+
+template <int F>
+static hipError_t launch_t(const CorrArgs& a, hipStream_t stream) {
+  switch (a.in_dtype) {
+    case kU8: return launch_f<F, uint8_t>(a, stream);
+    case kU16: return launch_f<F, uint16_t>(a, stream);
+    case kF32: return launch_f<F, float>(a, stream);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_correlate(const CorrArgs& a, hipStream_t stream) {
+  if (a.ss < 2 || a.ss > a.fsize || (a.fsize - a.ss) % 2 != 0) return hipErrorInvalidValue;
+  switch (a.fsize) {
+    case 16: return launch_t<16>(a, stream);
+    case 32: return launch_t<32>(a, stream);
+    case 64: return launch_t<64>(a, stream);
+    case 128: return launch_t<128>(a, stream);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace piv

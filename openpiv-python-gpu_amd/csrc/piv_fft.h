@@ -1,0 +1,189 @@
+// Register-resident radix-2 FFT building blocks for gfx950 (wave64).
+//
+// A length-F 1-D transform is spread over 2 or 4 adjacent lanes; each lane
+// keeps F/2 or F/4 complex values in VGPRs, runs an in-register DIF FFT on
+// them (fully unrolled, compile-time twiddles) and exchanges data with its
+// partner lanes through DPP quad permutes -- no LDS traffic inside a 1-D
+// transform.  Register placement after an in-register DIF is bit-reversed;
+// all indices are compile-time constants, so the permutation costs nothing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+namespace piv {
+
+// Compile-time loop: fn(std::integral_constant<int, I>) for I in [B, E).
+template <int B, int E, typename Fn>
+__device__ __forceinline__ void static_for(Fn&& fn) {
+  if constexpr (B < E) {
+    fn(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(fn);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Compile-time trigonometry (double precision Taylor series after octant
+// reduction; the float casts are correctly rounded for every twiddle used).
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double taylor_sin(double x) {
+  double term = x, sum = x;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double taylor_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+    sum += term;
+  }
+  return sum;
+}
+// cos(2*pi*k/n) and sin(2*pi*k/n) for integers k, n (exact rational
+// reduction to the first octant, then a Taylor series).
+constexpr double cos_frac(long k, long n) {
+  k %= n;
+  if (k < 0) k += n;
+  if (k == 0) return 1.0;
+  if (2 * k > n) k = n - k;                           // cos(2pi - a) = cos(a)
+  if (4 * k == n) return 0.0;
+  if (4 * k > n) return -cos_frac(n - 2 * k, 2 * n);  // cos(a) = -cos(pi - a)
+  if (8 * k > n) return taylor_sin(2.0 * kPi * (double)(n - 4 * k) / (4.0 * (double)n));
+  return taylor_cos(2.0 * kPi * (double)k / (double)n);
+}
+constexpr double sin_frac(long k, long n) { return cos_frac(4 * k - n, 4 * n); }
+
+struct c32 {
+  float x, y;
+};
+
+__device__ __forceinline__ c32 cadd(c32 a, c32 b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ c32 csub(c32 a, c32 b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ c32 cmul(c32 a, c32 b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// conj(a) * b
+__device__ __forceinline__ c32 cconjmul(c32 a, c32 b) {
+  return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+
+// Multiply by exp(DIR * 2*pi*i * K / N) with the trivial cases folded.
+template <int K, int N, int DIR>
+__device__ __forceinline__ c32 twiddle(c32 v) {
+  constexpr int k = ((K % N) + N) % N;
+  if constexpr (k == 0) {
+    return v;
+  } else if constexpr (4 * k == N) {  // DIR * i
+    return DIR > 0 ? c32{-v.y, v.x} : c32{v.y, -v.x};
+  } else if constexpr (2 * k == N) {
+    return c32{-v.x, -v.y};
+  } else if constexpr (4 * k == 3 * N) {  // -DIR * i
+    return DIR > 0 ? c32{v.y, -v.x} : c32{-v.y, v.x};
+  } else {
+    constexpr float c = (float)cos_frac(k, N);
+    constexpr float s = (float)(DIR * sin_frac(k, N));
+    return c32{v.x * c - v.y * s, v.x * s + v.y * c};
+  }
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+constexpr int bitrev(int v, int n) {
+  int bits = ilog2(n), r = 0;
+  for (int i = 0; i < bits; ++i) r |= ((v >> i) & 1) << (bits - 1 - i);
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// In-register DIF FFT of length L: natural-order input x[n]; on return the
+// frequency-k value sits in x[bitrev(k, L)].
+// ---------------------------------------------------------------------------
+template <int L, int DIR, int S>
+struct DifStage {
+  // S = half block size at this stage.
+  template <int B, int J>
+  __device__ __forceinline__ static void bfly(c32 (&x)[L]) {
+    if constexpr (B < L) {
+      if constexpr (J < S) {
+        c32 u = x[B + J], v = x[B + J + S];
+        x[B + J] = cadd(u, v);
+        x[B + J + S] = twiddle<J * (L / (2 * S)), L, DIR>(csub(u, v));
+        bfly<B, J + 1>(x);
+      } else {
+        bfly<B + 2 * S, 0>(x);
+      }
+    }
+  }
+  __device__ __forceinline__ static void run(c32 (&x)[L]) {
+    bfly<0, 0>(x);
+    if constexpr (S > 1) DifStage<L, DIR, S / 2>::run(x);
+  }
+};
+
+template <int L, int DIR>
+__device__ __forceinline__ void fft_reg(c32 (&x)[L]) {
+  if constexpr (L > 1) DifStage<L, DIR, L / 2>::run(x);
+}
+
+// ---------------------------------------------------------------------------
+// Cross-lane moves (DPP quad permutes; all four lanes of a quad must be
+// active).
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ c32 dpp_c(c32 v) {
+  return c32{dpp_f<CTRL>(v.x), dpp_f<CTRL>(v.y)};
+}
+constexpr int kSwap1 = 0xB1;    // quad_perm [1,0,3,2]: lane ^ 1
+constexpr int kSwap2 = 0x4E;    // quad_perm [2,3,0,1]: lane ^ 2
+constexpr int kReverse = 0x1B;  // quad_perm [3,2,1,0]: lane ^ 3
+
+__device__ __forceinline__ c32 swap1(c32 v) { return dpp_c<kSwap1>(v); }
+__device__ __forceinline__ c32 swap2(c32 v) { return dpp_c<kSwap2>(v); }
+
+// ---------------------------------------------------------------------------
+// 2-lane DIT combine.  Before: lane h holds the length-F/2 transform of the
+// samples x[2m + h] (frequency k at register bitrev(k, F/2)).  After: lane h
+// holds X[k + h*F/2] at register bitrev(k, F/2).
+// ---------------------------------------------------------------------------
+template <int F, int DIR, int K = 0>
+__device__ __forceinline__ void dit2_combine(c32 (&x)[F / 2], bool upper) {
+  if constexpr (K < F / 2) {
+    constexpr int r = bitrev(K, F / 2);
+    c32 own = x[r];
+    c32 oth = swap1(own);
+    c32 e = upper ? oth : own;
+    c32 o = upper ? own : oth;
+    c32 wo = twiddle<K, F, DIR>(o);
+    x[r] = upper ? csub(e, wo) : cadd(e, wo);
+    dit2_combine<F, DIR, K + 1>(x, upper);
+  }
+}
+
+// 2-lane DIF first stage.  Before: lane h holds x[n + h*F/2] at register
+// slot(n) where slot(n) = bitrev(n, F/2) (the layout a forward transform
+// leaves behind).  After: lane 0 holds y[n] = x[n] + x[n+F/2], lane 1 holds
+// y'[n] = (x[n] - x[n+F/2]) w^n, still at register bitrev(n, F/2).
+template <int F, int DIR, int N = 0>
+__device__ __forceinline__ void dif2_split(c32 (&x)[F / 2], bool upper) {
+  if constexpr (N < F / 2) {
+    constexpr int r = bitrev(N, F / 2);
+    c32 own = x[r];
+    c32 oth = swap1(own);
+    c32 a = upper ? oth : own;  // x[n]
+    c32 b = upper ? own : oth;  // x[n + F/2]
+    x[r] = upper ? twiddle<N, F, DIR>(csub(a, b)) : cadd(a, b);
+    dif2_split<F, DIR, N + 1>(x, upper);
+  }
+}
+
+}  // namespace piv

@@ -1,0 +1,426 @@
+// Field-level kernels of one PIV pass: frame deformation, grid-to-grid
+// predictor interpolation, 3x3 median validation and spring replacement.
+#include <math.h>
+
+#include "piv_fft.h"
+#include "piv_kernels.h"
+
+namespace piv {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ float ld(const void* base, size_t idx) {
+  return (float)(reinterpret_cast<const T*>(base)[idx]);
+}
+
+// Bilinear value of a regular-grid field at (yq, xq) with linear
+// extrapolation outside the grid: the interval index is clamped to
+// [0, n-2] while the normalised distance is not (cupyx/scipy
+// RegularGridInterpolator(method="linear", fill_value=None) semantics used at
+// gpu_process.py:737-740 and :1159-1162).
+__device__ __forceinline__ void grid_bilinear(const float* fu, const float* fv, int m, int n, float y0, float x0,
+                                              float sp, float yq, float xq, float& ou, float& ov) {
+  int iy = (int)floorf((yq - y0) / sp);
+  int ix = (int)floorf((xq - x0) / sp);
+  iy = iy < 0 ? 0 : (iy > m - 2 ? m - 2 : iy);
+  ix = ix < 0 ? 0 : (ix > n - 2 ? n - 2 : ix);
+  const float ty = (yq - (y0 + (float)iy * sp)) / sp;
+  const float tx = (xq - (x0 + (float)ix * sp)) / sp;
+  const float wy0 = 1.f - ty, wx0 = 1.f - tx;
+  const size_t i00 = (size_t)iy * n + ix;
+  const size_t i10 = i00 + n;
+  ou = fu[i00] * wy0 * wx0 + fu[i00 + 1] * wy0 * tx + fu[i10] * ty * wx0 + fu[i10 + 1] * ty * tx;
+  ov = fv[i00] * wy0 * wx0 + fv[i00 + 1] * wy0 * tx + fv[i10] * ty * wx0 + fv[i10 + 1] * ty * tx;
+}
+
+// cuda_interpolate_frame (gpu_process.py:1528-1546): clamped base index,
+// unclamped fraction, so the border extrapolates linearly.
+template <typename T>
+__device__ __forceinline__ float warp_px(const void* f, int ht, int wd, float X, float Y, float up, float vp,
+                                         float dt) {
+  const float xi = fmaf(up, dt, X);
+  const float yi = fmaf(vp, dt, Y);
+  int xl = (int)floorf(xi);
+  int yu = (int)floorf(yi);
+  yu = yu < 0 ? 0 : (yu > ht - 2 ? ht - 2 : yu);
+  xl = xl < 0 ? 0 : (xl > wd - 2 ? wd - 2 : xl);
+  const int yd = yu + 1, xr = xl + 1;
+  const float ful = ld<T>(f, (size_t)yu * wd + xl);
+  const float fur = ld<T>(f, (size_t)yu * wd + xr);
+  const float fdl = ld<T>(f, (size_t)yd * wd + xl);
+  const float fdr = ld<T>(f, (size_t)yd * wd + xr);
+  const float a0 = yi - (float)yu, a1 = (float)yd - yi;
+  const float b0 = (float)xr - xi, b1 = xi - (float)xl;
+  return a0 * b0 * fdl + a0 * b1 * fdr + a1 * b0 * ful + a1 * b1 * fur;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void deform_kernel(DeformArgs a) {
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y;
+  if (X >= a.wd) return;
+  float up, vp;
+  grid_bilinear(a.up, a.vp, a.m, a.n, a.y0, a.x0, a.spacing, (float)Y, (float)X, up, vp);
+  const size_t idx = (size_t)Y * a.wd + X;
+  a.out_a[idx] = a.warp_a ? warp_px<T>(a.frame_a, a.ht, a.wd, (float)X, (float)Y, up, vp, a.dt_a)
+                          : ld<T>(a.frame_a, idx);
+  a.out_b[idx] = a.warp_b ? warp_px<T>(a.frame_b, a.ht, a.wd, (float)X, (float)Y, up, vp, a.dt_b)
+                          : ld<T>(a.frame_b, idx);
+}
+
+__global__ __launch_bounds__(256) void grid_interp_kernel(GridInterpArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.m_out * a.n_out) return;
+  const int i = k / a.n_out, j = k % a.n_out;
+  const float yq = a.y0_out + (float)i * a.sp_out;
+  const float xq = a.x0_out + (float)j * a.sp_out;
+  float ou, ov;
+  grid_bilinear(a.u_in, a.v_in, a.m_in, a.n_in, a.y0_in, a.x0_in, a.sp_in, yq, xq, ou, ov);
+  a.u_out[k] = ou;
+  a.v_out[k] = ov;
+}
+
+// Median of the valid entries of a[0..cnt) after sorting (cnt <= 8).
+__device__ __forceinline__ void cswap(float& x, float& y) {
+  const float lo = fminf(x, y), hi = fmaxf(x, y);
+  x = lo;
+  y = hi;
+}
+__device__ __forceinline__ float median8(float (&s)[8], int cnt) {
+  // Batcher odd-even merge sort network for 8 keys (19 comparators); missing
+  // entries are +inf and sort to the end.
+  cswap(s[0], s[1]); cswap(s[2], s[3]); cswap(s[4], s[5]); cswap(s[6], s[7]);
+  cswap(s[0], s[2]); cswap(s[1], s[3]); cswap(s[4], s[6]); cswap(s[5], s[7]);
+  cswap(s[1], s[2]); cswap(s[5], s[6]);
+  cswap(s[0], s[4]); cswap(s[1], s[5]); cswap(s[2], s[6]); cswap(s[3], s[7]);
+  cswap(s[2], s[4]); cswap(s[3], s[5]);
+  cswap(s[1], s[2]); cswap(s[3], s[4]); cswap(s[5], s[6]);
+  if (cnt <= 0) return NAN;
+  float r = 0.f;
+  switch (cnt) {
+    case 1: r = s[0]; break;
+    case 2: r = 0.5f * (s[0] + s[1]); break;
+    case 3: r = s[1]; break;
+    case 4: r = 0.5f * (s[1] + s[2]); break;
+    case 5: r = s[2]; break;
+    case 6: r = 0.5f * (s[2] + s[3]); break;
+    case 7: r = s[3]; break;
+    default: r = 0.5f * (s[3] + s[4]); break;
+  }
+  return r;
+}
+
+// ValidationGPU.median_validation (gpu_validation.py:136-141, 166-205):
+// NaN-padded 3x3 neighbourhood, centre excluded, nanmedian, absolute test,
+// flags OR-ed over u and v.
+__global__ __launch_bounds__(256) void median_kernel(const float* u, const float* v, int m, int n, float tol,
+                                                     uint8_t* flags, int32_t* count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  bool flag = false;
+  if (k < m * n) {
+    const int i = k / n, j = k % n;
+    float su[8], sv[8];
+    int cu = 0, cv = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      su[q] = INFINITY;
+      sv[q] = INFINITY;
+    }
+#pragma unroll
+    for (int di = -1; di <= 1; ++di) {
+#pragma unroll
+      for (int dj = -1; dj <= 1; ++dj) {
+        if (di == 0 && dj == 0) continue;
+        const int ii = i + di, jj = j + dj;
+        if (ii < 0 || ii >= m || jj < 0 || jj >= n) continue;
+        const float a = u[ii * n + jj], b = v[ii * n + jj];
+        if (!isnan(a)) {
+          // insert at position cu (register array indexed by a small runtime
+          // value -> keep it branch-free with a select chain)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) su[q] = (q == cu) ? a : su[q];
+          ++cu;
+        }
+        if (!isnan(b)) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) sv[q] = (q == cv) ? b : sv[q];
+          ++cv;
+        }
+      }
+    }
+    const float mu = median8(su, cu);
+    const float mv = median8(sv, cv);
+    flag = (fabsf(u[k] - mu) > tol) || (fabsf(v[k] - mv) > tol);
+    flags[k] = flag ? 1 : 0;
+  }
+  const unsigned long long b = __ballot(flag);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(count, (int32_t)__popcll(b));
+}
+
+// ---------------------------------------------------------------------------
+// Spring replacement (ReplacementGPU.spring_replacement,
+// gpu_validation.py:336-371, 381-415; PIVGPU.replace_outliers,
+// gpu_process.py:788-834).  The reference assembles
+//     x_k - (1/n_k) sum_{l in N4(k), l outlier} x_l
+//         = (1/n_k) sum_{l in N4(k), l valid} f_l
+// (n_k = in-bounds 4-neighbours) and solves it directly.  Scaling row k by
+// n_k gives a symmetric positive-definite graph Laplacian with Dirichlet
+// boundary (whenever some vector is valid), solved here by Jacobi-
+// preconditioned CG in FP64 for u and v together, in one workgroup.  When
+// every vector is an outlier the system is singular; the reference's NaN ->
+// reset path then leaves the field unchanged, and so does this kernel.
+// ---------------------------------------------------------------------------
+constexpr int kReplaceThreads = 1024;
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double (*scratch)[K]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) scratch[wave][k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += scratch[w][k];
+    v[k] = s;
+  }
+}
+
+__global__ __launch_bounds__(kReplaceThreads) void spring_kernel(float* u, float* v, const uint8_t* flags,
+                                                                 const int32_t* count, int m, int n,
+                                                                 ReplaceWork w) {
+  const int N = m * n;
+  const int nv = *count;
+  if (nv <= 0 || nv >= N) return;
+  __shared__ int32_t scan[kReplaceThreads];
+  __shared__ double red4[16][4];
+  const int tid = threadIdx.x;
+
+  // Deterministic compaction: outliers in row-major order.
+  const int chunk = (N + kReplaceThreads - 1) / kReplaceThreads;
+  const int b0 = tid * chunk;
+  const int b1 = min(N, b0 + chunk);
+  int local = 0;
+  for (int k = b0; k < b1; ++k) local += flags[k] ? 1 : 0;
+  scan[tid] = local;
+  __syncthreads();
+  for (int off = 1; off < kReplaceThreads; off <<= 1) {
+    int add = tid >= off ? scan[tid - off] : 0;
+    __syncthreads();
+    scan[tid] += add;
+    __syncthreads();
+  }
+  int pos = scan[tid] - local;
+  for (int k = b0; k < b1; ++k) {
+    if (flags[k]) {
+      w.slot[k] = pos;
+      w.list[pos] = k;
+      ++pos;
+    } else {
+      w.slot[k] = -1;
+    }
+  }
+  __syncthreads();
+  __threadfence_block();
+
+  double* xu = w.work;
+  double* xv = w.work + (size_t)nv;
+  double* ru = w.work + 2 * (size_t)nv;
+  double* rv = w.work + 3 * (size_t)nv;
+  double* pu = w.work + 4 * (size_t)nv;
+  double* pv = w.work + 5 * (size_t)nv;
+  double* qu = w.work + 6 * (size_t)nv;
+  double* qv = w.work + 7 * (size_t)nv;
+
+  // b = sum of valid in-bounds neighbours; diag = n_k.
+  double acc[4] = {0, 0, 0, 0};  // r.z (u), r.z (v), b.b (u), b.b (v)
+  for (int s = tid; s < nv; s += kReplaceThreads) {
+    const int k = w.list[s];
+    const int i = k / n, j = k % n;
+    double bu = 0.0, bv = 0.0;
+    int nk = 0;
+    const int ni[4] = {i, i, i + 1, i - 1};
+    const int nj[4] = {j - 1, j + 1, j, j};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (ni[e] < 0 || ni[e] >= m || nj[e] < 0 || nj[e] >= n) continue;
+      ++nk;
+      const int l = ni[e] * n + nj[e];
+      if (w.slot[l] < 0) {
+        bu += (double)u[l];
+        bv += (double)v[l];
+      }
+    }
+    xu[s] = 0.0;
+    xv[s] = 0.0;
+    ru[s] = bu;
+    rv[s] = bv;
+    const double zu = bu / nk, zv = bv / nk;
+    pu[s] = zu;
+    pv[s] = zv;
+    acc[0] += bu * zu;
+    acc[1] += bv * zv;
+    acc[2] += bu * bu;
+    acc[3] += bv * bv;
+  }
+  block_sum<4>(acc, red4);
+  double rz_u = acc[0], rz_v = acc[1];
+  const double tol2_u = 1e-26 * acc[2], tol2_v = 1e-26 * acc[3];
+  bool done_u = acc[2] == 0.0, done_v = acc[3] == 0.0;
+  const int max_it = nv + 200 < 20000 ? nv + 200 : 20000;
+  for (int it = 0; it < max_it && !(done_u && done_v); ++it) {
+    __syncthreads();
+    // q = A p
+    double pq[2] = {0, 0};
+    for (int s = tid; s < nv; s += kReplaceThreads) {
+      const int k = w.list[s];
+      const int i = k / n, j = k % n;
+      const int ni[4] = {i, i, i + 1, i - 1};
+      const int nj[4] = {j - 1, j + 1, j, j};
+      int nk = 0;
+      double au = 0.0, av = 0.0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (ni[e] < 0 || ni[e] >= m || nj[e] < 0 || nj[e] >= n) continue;
+        ++nk;
+        const int sl = w.slot[ni[e] * n + nj[e]];
+        if (sl >= 0) {
+          au += pu[sl];
+          av += pv[sl];
+        }
+      }
+      const double qau = nk * pu[s] - au;
+      const double qav = nk * pv[s] - av;
+      qu[s] = qau;
+      qv[s] = qav;
+      pq[0] += pu[s] * qau;
+      pq[1] += pv[s] * qav;
+    }
+    block_sum<2>(pq, reinterpret_cast<double(*)[2]>(red4));
+    const double alpha_u = (done_u || pq[0] == 0.0) ? 0.0 : rz_u / pq[0];
+    const double alpha_v = (done_v || pq[1] == 0.0) ? 0.0 : rz_v / pq[1];
+    double nr[4] = {0, 0, 0, 0};  // r.z u, r.z v, r.r u, r.r v
+    for (int s = tid; s < nv; s += kReplaceThreads) {
+      const int k = w.list[s];
+      const int i = k / n, j = k % n;
+      const int nk = (i > 0) + (i < m - 1) + (j > 0) + (j < n - 1);
+      xu[s] += alpha_u * pu[s];
+      xv[s] += alpha_v * pv[s];
+      const double r1 = ru[s] - alpha_u * qu[s];
+      const double r2 = rv[s] - alpha_v * qv[s];
+      ru[s] = r1;
+      rv[s] = r2;
+      nr[0] += r1 * r1 / nk;
+      nr[1] += r2 * r2 / nk;
+      nr[2] += r1 * r1;
+      nr[3] += r2 * r2;
+    }
+    block_sum<4>(nr, red4);
+    const double beta_u = rz_u > 0.0 ? nr[0] / rz_u : 0.0;
+    const double beta_v = rz_v > 0.0 ? nr[1] / rz_v : 0.0;
+    rz_u = nr[0];
+    rz_v = nr[1];
+    done_u = done_u || nr[2] <= tol2_u;
+    done_v = done_v || nr[3] <= tol2_v;
+    for (int s = tid; s < nv; s += kReplaceThreads) {
+      const int k = w.list[s];
+      const int i = k / n, j = k % n;
+      const int nk = (i > 0) + (i < m - 1) + (j > 0) + (j < n - 1);
+      pu[s] = ru[s] / nk + beta_u * pu[s];
+      pv[s] = rv[s] / nk + beta_v * pv[s];
+    }
+  }
+  __syncthreads();
+  for (int s = tid; s < nv; s += kReplaceThreads) {
+    const int k = w.list[s];
+    u[k] = (float)xu[s];
+    v[k] = (float)xv[s];
+  }
+}
+
+__global__ void fill_i32_kernel(int32_t* p, int32_t value, int count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count) p[k] = value;
+}
+__global__ void scale_kernel(float* u, float* v, int count, float factor) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count) {
+    u[k] *= factor;
+    v[k] *= factor;
+  }
+}
+__global__ void copy_f32_kernel(float* dst, const float* src, int count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count) dst[k] = src[k];
+}
+__global__ void copy_u8_kernel(uint8_t* dst, const uint8_t* src, int count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count) dst[k] = src[k];
+}
+
+}  // namespace
+
+hipError_t launch_deform(const DeformArgs& a, hipStream_t stream) {
+  if (a.m < 2 || a.n < 2) return hipErrorInvalidValue;
+  dim3 grid((a.wd + 255) / 256, a.ht);
+  switch (a.in_dtype) {
+    case kU8: hipLaunchKernelGGL(deform_kernel<uint8_t>, grid, dim3(256), 0, stream, a); break;
+    case kU16: hipLaunchKernelGGL(deform_kernel<uint16_t>, grid, dim3(256), 0, stream, a); break;
+    case kF32: hipLaunchKernelGGL(deform_kernel<float>, grid, dim3(256), 0, stream, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_grid_interp(const GridInterpArgs& a, hipStream_t stream) {
+  if (a.m_in < 2 || a.n_in < 2) return hipErrorInvalidValue;
+  const int total = a.m_out * a.n_out;
+  hipLaunchKernelGGL(grid_interp_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_median_validate(const float* u, const float* v, int m, int n, float tol, uint8_t* flags,
+                                  int32_t* count, hipStream_t stream) {
+  const int total = m * n;
+  hipLaunchKernelGGL(median_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, u, v, m, n, tol, flags,
+                     count);
+  return hipGetLastError();
+}
+
+hipError_t launch_spring_replace(float* u, float* v, const uint8_t* flags, const int32_t* count, int m, int n,
+                                 const ReplaceWork& w, hipStream_t stream) {
+  hipLaunchKernelGGL(spring_kernel, dim3(1), dim3(kReplaceThreads), 0, stream, u, v, flags, count, m, n, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int32_t* p, int32_t value, int count, hipStream_t stream) {
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, p, value, count);
+  return hipGetLastError();
+}
+hipError_t launch_scale(float* u, float* v, int count, float factor, hipStream_t stream) {
+  hipLaunchKernelGGL(scale_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, u, v, count, factor);
+  return hipGetLastError();
+}
+hipError_t launch_copy_f32(float* dst, const float* src, int count, hipStream_t stream) {
+  hipLaunchKernelGGL(copy_f32_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, dst, src, count);
+  return hipGetLastError();
+}
+hipError_t launch_copy_u8(uint8_t* dst, const uint8_t* src, int count, hipStream_t stream) {
+  hipLaunchKernelGGL(copy_u8_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, dst, src, count);
+  return hipGetLastError();
+}
+
+}  // namespace piv

@@ -1,0 +1,101 @@
+// Kernel launchers shared by the plan executor and the stage entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace piv {
+
+enum InDtype { kU8 = 0, kU16 = 1, kF32 = 2 };
+enum SubpixelMethod { kGaussian = 0, kParabolic = 1, kCentroid = 2 };
+
+// Correlation of one pass: every window of the field is extracted from the
+// two frames, mean-normalised, zero-padded to F x F, cross-correlated by FFT,
+// and reduced to an fftshift-ed argmax plus a 3-point sub-pixel fit.
+struct CorrArgs {
+  const void* frame_a;
+  const void* frame_b;
+  int in_dtype;            // InDtype of both frames
+  int ht, wd;              // frame shape
+  int ss;                  // window (search) size
+  int fsize;               // FFT size F (power of two, 16..128)
+  int spacing, x_off, y_off;
+  int m, n;                // field shape
+  int normalize;           // subtract window means (forced on when padded)
+  int subpixel;            // SubpixelMethod
+  const float* up;         // predictor on this grid added to the result, or null
+  const float* vp;
+  float* u;
+  float* v;
+  int32_t* i_peak;         // optional integer peak row (fftshift-ed plane)
+  int32_t* j_peak;         // optional integer peak column
+  float* planes;           // optional N x F x F dump of the fftshift-ed planes
+};
+
+hipError_t launch_correlate(const CorrArgs& a, hipStream_t stream);
+size_t correlate_lds_bytes(int fsize, int ss);
+
+// Frame deformation: predictor bilinearly interpolated (with linear
+// extrapolation) from the pass grid onto each pixel, then a backward bilinear
+// warp of each frame at its own time step.  Output frames are FP32.
+struct DeformArgs {
+  const void* frame_a;
+  const void* frame_b;
+  int in_dtype;
+  int ht, wd;
+  const float* up;  // m x n predictor on the grid
+  const float* vp;
+  int m, n;
+  float x0, y0;     // first grid node coordinates
+  float spacing;    // grid spacing (pixels)
+  float dt_a, dt_b;
+  int warp_a, warp_b;  // 0 -> copy the frame unchanged
+  float* out_a;
+  float* out_b;
+};
+hipError_t launch_deform(const DeformArgs& a, hipStream_t stream);
+
+// Linear interpolation with extrapolation between two regular grids.
+struct GridInterpArgs {
+  const float* u_in;
+  const float* v_in;
+  int m_in, n_in;
+  float x0_in, y0_in, sp_in;
+  int m_out, n_out;
+  float x0_out, y0_out, sp_out;
+  float* u_out;
+  float* v_out;
+};
+hipError_t launch_grid_interp(const GridInterpArgs& a, hipStream_t stream);
+
+// 3x3 NaN-aware neighbour-median test on u and v (flags OR-ed).
+hipError_t launch_median_validate(const float* u, const float* v, int m, int n, float tol, uint8_t* flags,
+                                  int32_t* count, hipStream_t stream);
+
+// Spring (Laplace) replacement of flagged vectors, solved on the device.
+struct ReplaceWork {
+  int32_t* slot;      // m*n
+  int32_t* list;      // m*n
+  double* work;       // 8 * m*n doubles
+};
+hipError_t launch_spring_replace(float* u, float* v, const uint8_t* flags, const int32_t* count, int m, int n,
+                                 const ReplaceWork& w, hipStream_t stream);
+
+// Unweighted DCT-PLS smoothing (GCV-chosen s) of u and v, in place.
+struct SmoothnWork {
+  const float* cos_m;   // m x m orthonormal DCT-II matrix (row k = frequency)
+  const float* cos_n;   // n x n
+  const double* lambda; // m x n
+  float* t0;            // 2 * m*n
+  float* t1;            // 2 * m*n
+  double* p_out;        // chosen log10(s)
+};
+hipError_t launch_smoothn(float* u, float* v, int m, int n, double p_min, double p_max, double fixed_p,
+                          int auto_s, const SmoothnWork& w, hipStream_t stream);
+
+hipError_t launch_fill_i32(int32_t* p, int32_t value, int count, hipStream_t stream);
+hipError_t launch_scale(float* u, float* v, int count, float factor, hipStream_t stream);
+hipError_t launch_copy_f32(float* dst, const float* src, int count, hipStream_t stream);
+hipError_t launch_copy_u8(uint8_t* dst, const uint8_t* src, int count, hipStream_t stream);
+
+}  // namespace piv

@@ -1,0 +1,169 @@
+"""ctypes binding of ``libopenpiv_hip.so`` (declared in include/openpiv_hip.h).
+
+There is no CPU fallback: importing a compute entry point without the built
+library, or calling one without a visible GPU, raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+MAX_PASSES = 16
+PIV_U8, PIV_U16, PIV_F32 = 0, 1, 2
+SUBPIXEL_CODES = {"gaussian": 0, "parabolic": 1, "centroid": 2}
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("PIV_HIP_LIB", os.path.join(_HERE, "_lib", "libopenpiv_hip.so"))
+
+# Every symbol include/openpiv_hip.h declares.
+EXPORTED_SYMBOLS = (
+    "piv_last_error", "piv_device_count", "piv_plan_create", "piv_plan_destroy", "piv_run",
+    "piv_plan_last_smoothing", "piv_plan_last_counts", "piv_stage_correlate", "piv_stage_deform",
+    "piv_stage_grid_interp", "piv_stage_median_validate", "piv_stage_spring_replace", "piv_stage_smoothn",
+)
+
+
+class PassDesc(ctypes.Structure):
+    _fields_ = [
+        ("search_size", ctypes.c_int32),
+        ("spacing", ctypes.c_int32),
+        ("x_offset", ctypes.c_int32),
+        ("y_offset", ctypes.c_int32),
+        ("field_m", ctypes.c_int32),
+        ("field_n", ctypes.c_int32),
+        ("fft_size", ctypes.c_int32),
+        ("normalize", ctypes.c_int32),
+        ("subpixel", ctypes.c_int32),
+        ("deforming_par", ctypes.c_float),
+        ("validate", ctypes.c_int32),
+        ("median_tol", ctypes.c_float),
+        ("num_replacing_iters", ctypes.c_int32),
+        ("smooth", ctypes.c_int32),
+        ("smoothing_par", ctypes.c_double),
+    ]
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("in_dtype", ctypes.c_int32),
+        ("n_passes", ctypes.c_int32),
+        ("passes", PassDesc * MAX_PASSES),
+        ("out_scale", ctypes.c_float),
+        ("use_graph", ctypes.c_int32),
+    ]
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def _bind(lib):
+    vp, i32, f32, f64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
+    sig = {
+        "piv_last_error": (ctypes.c_char_p, []),
+        "piv_device_count": (i32, []),
+        "piv_plan_create": (i32, [i32, ctypes.POINTER(PlanDesc), ctypes.POINTER(vp)]),
+        "piv_plan_destroy": (None, [vp]),
+        "piv_run": (i32, [vp, vp, vp, i32, vp, vp, vp, vp, i32]),
+        "piv_plan_last_smoothing": (i32, [vp, i32, ctypes.POINTER(f64)]),
+        "piv_plan_last_counts": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+        "piv_stage_correlate": (i32, [vp, vp, i32, i32, i32, ctypes.POINTER(PassDesc), vp, vp, vp, vp, vp, vp, vp,
+                                      vp]),
+        "piv_stage_deform": (i32, [vp, vp, i32, i32, i32, ctypes.POINTER(PassDesc), vp, vp, vp, vp, vp]),
+        "piv_stage_grid_interp": (i32, [ctypes.POINTER(PassDesc), ctypes.POINTER(PassDesc), vp, vp, vp, vp, vp]),
+        "piv_stage_median_validate": (i32, [vp, vp, i32, i32, f32, vp, vp, vp]),
+        "piv_stage_spring_replace": (i32, [vp, vp, vp, vp, i32, i32, vp]),
+        "piv_stage_smoothn": (i32, [vp, vp, i32, i32, f64, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def load_library(path: str = None) -> ctypes.CDLL:
+    """Loads (once) and returns the HIP library; raises if it is not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(f"HIP library not built: {p} (run build_native.py or __graft_entry__.build())")
+    lib = ctypes.CDLL(p)
+    _bind(lib)
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(code: int):
+    if code != 0:
+        msg = load_library().piv_last_error()
+        raise NativeError(msg.decode() if msg else f"openpiv_hip error {code}")
+
+
+def device_count() -> int:
+    return int(load_library().piv_device_count())
+
+
+def require_device():
+    if device_count() <= 0:
+        raise NativeError("no HIP device visible: the PIV hot path runs only on the GPU")
+
+
+def in_dtype_code(dtype) -> int:
+    dt = np.dtype(dtype)
+    if dt == np.uint8:
+        return PIV_U8
+    if dt == np.uint16:
+        return PIV_U16
+    if dt == np.float32:
+        return PIV_F32
+    raise ValueError(f"unsupported frame dtype {dt}")
+
+
+class Plan:
+    """Owns one native plan (frame shape + pass schedule)."""
+
+    def __init__(self, desc: PlanDesc, device: int = 0):
+        lib = load_library()
+        require_device()
+        handle = ctypes.c_void_p()
+        check(lib.piv_plan_create(device, ctypes.byref(desc), ctypes.byref(handle)))
+        self._lib = lib
+        self.handle = handle
+        self.desc = desc
+
+    def run(self, frame_a, frame_b, frames_on_device, u, v, val_init, val_final, outputs_on_device):
+        check(self._lib.piv_run(self.handle, frame_a, frame_b, int(frames_on_device), u, v, val_init, val_final,
+                                int(outputs_on_device)))
+
+    def last_smoothing(self, k: int) -> float:
+        out = ctypes.c_double()
+        check(self._lib.piv_plan_last_smoothing(self.handle, k, ctypes.byref(out)))
+        return out.value
+
+    def last_counts(self, k: int):
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        check(self._lib.piv_plan_last_counts(self.handle, k, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.piv_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

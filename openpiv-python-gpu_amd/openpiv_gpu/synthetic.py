@@ -1,0 +1,200 @@
+"""Seeded synthetic PIV image pairs.
+
+The reference ships no generator and no tests; this module defines the
+particle-image model the benchmarks and parity tests run on:
+
+* particles are isotropic Gaussian blobs ``I0 * exp(-r^2 / (2 sigma^2))``
+  point-sampled at integer pixel centres (pixel ``(r, c)`` sits at
+  ``x = c, y = r``, the pixel grid the reference warps on,
+  gpu_process.py:1153-1155), cut off at ``r <= 4 sigma``;
+* a fixed particle count ``round(density * area)`` is drawn over the frame
+  extended by a margin of ``ceil(max|d|) + ceil(4 sigma) + 1`` pixels;
+* placement is symmetric: frame A holds each particle at ``p - d(p)/2``,
+  frame B at ``p + d(p)/2``, so the displacement at a node is ``d(node)``;
+* independent additive Gaussian noise per frame, then ``rint``, clip, cast.
+
+All randomness comes from ``numpy.random.Generator(PCG64(seed))`` in a fixed
+draw order (positions x, positions y, peak intensities, noise A, noise B).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Callable, Optional, Tuple
+
+import numpy as np
+
+DisplacementFn = Callable[[np.ndarray, np.ndarray], Tuple[np.ndarray, np.ndarray]]
+
+
+def uniform_displacement(u: float, v: float) -> DisplacementFn:
+    """Constant displacement (u along x / columns, v along y / rows)."""
+    def fn(x, y):
+        return np.full_like(x, u, dtype=np.float64), np.full_like(y, v, dtype=np.float64)
+    fn.max_abs = max(abs(u), abs(v))
+    return fn
+
+
+def sine_displacement(amp_u: float = 4.0, amp_v: float = 3.0, period: float = 256.0) -> DisplacementFn:
+    """``u = amp_u sin(2 pi y / period)``, ``v = amp_v cos(2 pi x / period)``."""
+    def fn(x, y):
+        return (amp_u * np.sin(2.0 * np.pi * y / period),
+                amp_v * np.cos(2.0 * np.pi * x / period))
+    fn.max_abs = max(abs(amp_u), abs(amp_v))
+    return fn
+
+
+def lamb_oseen_displacement(rng: np.random.Generator, shape: Tuple[int, int]) -> DisplacementFn:
+    """Random superposition of 1-4 Lamb-Oseen vortices plus a uniform shear.
+
+    Core radius ``r_c ~ U(20, 200)`` px, peak swirl ``~ U(2, 10)`` px (the
+    swirl peaks at ``0.6382 * Gamma / (2 pi r_c)``), shear ``|gamma| <= 0.02``
+    px/px applied as ``u += gamma (y - H/2)``.
+    """
+    ht, wd = shape
+    n_vort = int(rng.integers(1, 5))
+    cx = rng.uniform(0, wd, n_vort)
+    cy = rng.uniform(0, ht, n_vort)
+    rc = rng.uniform(20.0, 200.0, n_vort)
+    peak = rng.uniform(2.0, 10.0, n_vort) * rng.choice([-1.0, 1.0], n_vort)
+    circ = peak * 2.0 * np.pi * rc / 0.6382
+    gamma = rng.uniform(-0.02, 0.02)
+
+    def fn(x, y):
+        u = gamma * (y - ht / 2.0)
+        v = np.zeros_like(x, dtype=np.float64)
+        for k in range(n_vort):
+            dx, dy = x - cx[k], y - cy[k]
+            r2 = dx * dx + dy * dy
+            r2s = np.where(r2 > 0, r2, 1.0)
+            ut = circ[k] / (2.0 * np.pi) * (1.0 - np.exp(-r2s / rc[k] ** 2)) / r2s
+            ut = np.where(r2 > 0, ut, 0.0)
+            u = u - ut * dy
+            v = v + ut * dx
+        return u, v
+    fn.max_abs = float(np.max(np.abs(peak))) + 0.02 * max(ht, wd) / 2.0
+    return fn
+
+
+@dataclass
+class ParticleModel:
+    sigma: float = 0.65
+    density: float = 0.05
+    i0_low: float = 150.0
+    i0_high: float = 255.0
+    noise_sigma: float = 2.0
+    max_value: float = 255.0
+    dtype: type = np.uint8
+
+
+def _render(ht, wd, xp, yp, i0, sigma):
+    """Point-samples Gaussian blobs at integer pixel centres (r <= 4 sigma)."""
+    rad = int(math.ceil(4.0 * sigma))
+    offs = np.arange(-rad, rad + 1)
+    oy, ox = np.meshgrid(offs, offs, indexing="ij")
+    oy = oy.reshape(-1)
+    ox = ox.reshape(-1)
+    img = np.zeros(ht * wd, dtype=np.float64)
+    cx = np.rint(xp).astype(np.int64)
+    cy = np.rint(yp).astype(np.int64)
+    # Chunk over particles to bound memory.
+    chunk = 65536
+    for s in range(0, xp.size, chunk):
+        sl = slice(s, s + chunk)
+        px = cx[sl, None] + ox[None, :]
+        py = cy[sl, None] + oy[None, :]
+        d2 = (px - xp[sl, None]) ** 2 + (py - yp[sl, None]) ** 2
+        ok = (d2 <= (4.0 * sigma) ** 2) & (px >= 0) & (px < wd) & (py >= 0) & (py < ht)
+        val = i0[sl, None] * np.exp(-d2 / (2.0 * sigma * sigma))
+        np.add.at(img, (py[ok] * wd + px[ok]), val[ok])
+    return img.reshape(ht, wd)
+
+
+def make_pair(shape: Tuple[int, int], disp: DisplacementFn, model: ParticleModel,
+              seed: int, rng: Optional[np.random.Generator] = None):
+    """Returns ``(frame_a, frame_b)`` of ``model.dtype`` for one seeded pair."""
+    ht, wd = shape
+    rng = np.random.Generator(np.random.PCG64(seed)) if rng is None else rng
+    margin = int(math.ceil(getattr(disp, "max_abs", 0.0))) + int(math.ceil(4.0 * model.sigma)) + 1
+    area = (ht + 2 * margin) * (wd + 2 * margin)
+    n = int(round(model.density * area))
+    px = rng.uniform(-margin, wd + margin, n)
+    py = rng.uniform(-margin, ht + margin, n)
+    i0 = rng.uniform(model.i0_low, model.i0_high, n)
+    du, dv = disp(px, py)
+    fa = _render(ht, wd, px - du / 2.0, py - dv / 2.0, i0, model.sigma)
+    fb = _render(ht, wd, px + du / 2.0, py + dv / 2.0, i0, model.sigma)
+    out = []
+    for f in (fa, fb):
+        if model.noise_sigma > 0:
+            f = f + rng.normal(0.0, model.noise_sigma, f.shape)
+        f = np.clip(np.rint(f), 0, model.max_value)
+        out.append(f.astype(model.dtype))
+    return out[0], out[1]
+
+
+# Named workloads -----------------------------------------------------------
+
+def cfg_512_uint8(seed: int = 1):
+    """512x512 uint8 pair, sigma 0.65, density 0.05, I0 ~ U(150,255), noise 2,
+    uniform shift (u, v) = (2.35, -1.60). Returns ``(a, b, disp_fn)``."""
+    disp = uniform_displacement(2.35, -1.60)
+    a, b = make_pair((512, 512), disp, ParticleModel(), seed)
+    return a, b, disp
+
+
+def cfg_2048_uint16(seed: int = 2, shape=(2048, 2048)):
+    """2048x2048 uint16 pair with 12-bit intensities (I0 ~ U(1000,4095),
+    noise 20, clip 0..4095) and ``u = 4 sin(2 pi y/256)``,
+    ``v = 3 cos(2 pi x/256)``. Returns ``(a, b, disp_fn)``."""
+    disp = sine_displacement(4.0, 3.0, 256.0)
+    model = ParticleModel(sigma=0.65, density=0.05, i0_low=1000.0, i0_high=4095.0,
+                          noise_sigma=20.0, max_value=4095.0, dtype=np.uint16)
+    a, b = make_pair(shape, disp, model, seed)
+    return a, b, disp
+
+
+def cfg_vortex_uint16(seed: int, shape: Tuple[int, int]):
+    """Uneven-batch pair: Lamb-Oseen vortices + shear, 12-bit uint16 model."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    disp = lamb_oseen_displacement(rng, shape)
+    model = ParticleModel(sigma=0.65, density=0.05, i0_low=1000.0, i0_high=4095.0,
+                          noise_sigma=20.0, max_value=4095.0, dtype=np.uint16)
+    a, b = make_pair(shape, disp, model, seed, rng=rng)
+    return a, b, disp
+
+
+def window_pairs(ws: int, n: int, seed: int, fft_size: Optional[int] = None):
+    """Independent ``ws x ws`` float32 window pairs with a uniform random shift.
+
+    Per window: shift ~ U[-8, 8]^2 (symmetric placement), sigma ~ U(0.5, 1.2),
+    I0 ~ U(0.5, 1), noise sigma ~ U(0, 0.05) of the unit peak, unquantised.
+    Returns ``(win_a, win_b, shift_uv)`` with ``shift_uv[:, 0] = u`` (x) and
+    ``shift_uv[:, 1] = v`` (y).
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shifts = rng.uniform(-8.0, 8.0, (n, 2))
+    sigmas = rng.uniform(0.5, 1.2, n)
+    noise = rng.uniform(0.0, 0.05, n)
+    wa = np.zeros((n, ws, ws), np.float32)
+    wb = np.zeros((n, ws, ws), np.float32)
+    margin = 8 + 6
+    area = (ws + 2 * margin) ** 2
+    npart = int(round(0.05 * area))
+    yy, xx = np.meshgrid(np.arange(ws), np.arange(ws), indexing="ij")
+    for k in range(n):
+        px = rng.uniform(-margin, ws + margin, npart)
+        py = rng.uniform(-margin, ws + margin, npart)
+        i0 = rng.uniform(0.5, 1.0, npart)
+        u, v = shifts[k]
+        s2 = 2.0 * sigmas[k] ** 2
+        for sign, dst in ((-0.5, wa), (0.5, wb)):
+            cx = px + sign * u
+            cy = py + sign * v
+            near = (cx > -6) & (cx < ws + 6) & (cy > -6) & (cy < ws + 6)
+            d2 = (xx[None] - cx[near, None, None]) ** 2 + (yy[None] - cy[near, None, None]) ** 2
+            img = np.sum(i0[near, None, None] * np.exp(-d2 / s2), axis=0)
+            img = img + rng.normal(0.0, noise[k], img.shape)
+            dst[k] = img.astype(np.float32)
+    return wa, wb, shifts

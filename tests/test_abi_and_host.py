@@ -1,0 +1,125 @@
+"""CPU-side checks: the C-ABI library loads and exports every declared
+symbol, the ctypes structs match the C layout, and the host-side settings
+logic mirrors the reference interface (no compute calls)."""
+
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from openpiv_gpu import _native
+from openpiv_gpu import gpu_process as gp
+from oracle import piv_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "openpiv_hip.h")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(piv_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declarations_match_binding():
+    assert _declared_functions() == sorted(_native.EXPORTED_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.skip("HIP library not built")
+    lib = _native.load_library()
+    for name in _declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_struct_layout_matches_c(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "openpiv_hip.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu\\n", sizeof(piv_pass_desc), sizeof(piv_plan_desc),'
+                   ' offsetof(piv_pass_desc, smoothing_par), offsetof(piv_plan_desc, out_scale),'
+                   ' offsetof(piv_plan_desc, use_graph)); return 0;}\n')
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    got = [int(t) for t in subprocess.check_output([str(exe)]).split()]
+    import ctypes
+    assert got == [ctypes.sizeof(_native.PassDesc), ctypes.sizeof(_native.PlanDesc),
+                   _native.PassDesc.smoothing_par.offset, _native.PlanDesc.out_scale.offset,
+                   _native.PlanDesc.use_graph.offset]
+
+
+def test_no_device_means_loud_failure():
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.skip("HIP library not built")
+    if _native.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    piv = gp.PIVGPU((128, 128), 32, dtype_f="float32")
+    with pytest.raises(_native.NativeError):
+        piv.run_host(np.zeros((128, 128), np.uint8), np.zeros((128, 128), np.uint8))
+
+
+def test_plan_desc_geometry_matches_oracle():
+    piv = gp.PIVGPU((2048, 2048), 16, search_size_iters=(1, 1, 1), dtype_f="float32")
+    d = piv._plan_desc(_native.PIV_U16)
+    assert d.n_passes == 3
+    for k, ss in enumerate((64, 32, 16)):
+        g = O.field_geometry((2048, 2048), ss, 0.5)
+        p = d.passes[k]
+        assert (p.search_size, p.spacing, p.x_offset, p.y_offset) == (ss, g.spacing, *g.offset)
+        assert (p.field_m, p.field_n) == g.field_shape
+        assert p.fft_size == O.fft_size(ss)
+        assert p.smooth == 1 and p.validate == 1 and p.num_replacing_iters == 1
+    x, y = piv.coords
+    np.testing.assert_array_equal(x.get(), O.field_geometry((2048, 2048), 16, 0.5).x)
+
+
+def test_per_size_settings_follow_size_groups():
+    piv = gp.PIVGPU((512, 512), 16, search_size_iters=(1, 2), n_fft=(2, 3), median_tol=(None, 1.5),
+                    deforming_par=(0.5, 1.0), dtype_f="float32")
+    d = piv._plan_desc(_native.PIV_U8)
+    assert [d.passes[k].search_size for k in range(3)] == [32, 16, 16]
+    assert [d.passes[k].fft_size for k in range(3)] == [64, 64, 64]
+    assert [d.passes[k].validate for k in range(3)] == [0, 1, 1]
+    assert d.passes[2].median_tol == pytest.approx(1.5)
+    assert d.passes[1].deforming_par == 1.0
+
+
+@pytest.mark.parametrize("kwargs,msg", [
+    (dict(search_size_iters=(0,)), "search_size_iters must be a tuple of positive int numbers."),
+    (dict(overlap_ratio=1.5), "overlap_ratio must be a tuple of real values between 0 and 1, defined for all passes."),
+    (dict(subpixel_method="bogus"), "subpixel_method for every pass must be one of"),
+    (dict(dt=-1.0), "dt must be a real number greater than 0."),
+    (dict(center=1), "center must have a bool value."),
+])
+def test_argument_assertions_match_reference_messages(kwargs, msg):
+    with pytest.raises(AssertionError, match=re.escape(msg)):
+        gp.piv_gpu((256, 256), 16, **kwargs)
+
+
+def test_min_search_size_and_frame_shape_assertions():
+    with pytest.raises(AssertionError, match="min_search_size must be an int number greater than 8."):
+        gp.piv_gpu((256, 256), 4)
+    with pytest.raises(AssertionError, match="frame_shape must be a tuple of int values greater than max_search_size"):
+        gp.piv_gpu((32, 256), 16, search_size_iters=(1, 1, 1))
+
+
+@pytest.mark.parametrize("kwargs,name", [
+    (dict(dtype_f="float64"), "dtype_f"), (dict(mask=np.zeros((256, 256), bool)), "mask"),
+    (dict(shrink_ratio=0.5), "shrink_ratio"), (dict(s2n_tol=1.2), "s2n_tol"), (dict(mad_tol=2.0), "mad_tol"),
+    (dict(replacing_method="median"), "replacing_method"), (dict(revalidate=True), "revalidate"),
+])
+def test_unimplemented_options_raise(kwargs, name):
+    with pytest.raises(NotImplementedError, match=name):
+        gp.piv_gpu((256, 256), 16, **kwargs)
+
+
+def test_gpu_piv_alias_and_frame_assertion():
+    assert gp.gpu_piv is gp.piv_gpu
+    piv = gp.piv_gpu((128, 128), 32, dtype_f="float32")
+    with pytest.raises(AssertionError, match="Both frames must be an ndarray of real values with shape"):
+        piv(np.zeros((64, 64)), np.zeros((64, 64)))
+    assert piv.field_shape == (7, 7)
+    assert not piv.val_locations.any() and not piv.outliers.any()
+    assert piv.field_mask.shape == (7, 7) and not piv.field_mask.any()

@@ -1,0 +1,166 @@
+"""Stage-level parity of the HIP kernels against the CPU oracle, through the C
+ABI stage entry points.  Tolerances:
+  * integer peak indices and the window grid bit-exact, except windows whose
+    two largest correlation values tie within 1e-5 (counted, asserted rare);
+  * u, v within 1e-3 px max-abs of the oracle's FP32 path;
+  * validation flags identical except vectors whose median-residual ratio is
+    within 1e-5 of the threshold.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import piv_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+gu = pytest.importorskip("gpu_util")
+
+
+def _frames(shape, dtype, seed, shift=(2.35, -1.6)):
+    from openpiv_gpu import synthetic
+    if dtype == np.float32:
+        rng = np.random.default_rng(seed)
+        a, b = synthetic.make_pair(shape, synthetic.uniform_displacement(*shift), synthetic.ParticleModel(), seed)
+        return (a.astype(np.float32) + rng.random(shape, dtype=np.float32),
+                b.astype(np.float32) + rng.random(shape, dtype=np.float32))
+    if dtype == np.uint16:
+        model = synthetic.ParticleModel(i0_low=1000, i0_high=4095, noise_sigma=20, max_value=4095, dtype=np.uint16)
+    else:
+        model = synthetic.ParticleModel()
+    return synthetic.make_pair(shape, synthetic.uniform_displacement(*shift), model, seed)
+
+
+@pytest.mark.parametrize("ss,n_fft", [(8, 2), (16, 2), (32, 2), (64, 2), (24, 2), (48, 2), (16, 3), (32, 1)])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32])
+def test_correlate_matches_oracle(ss, n_fft, dtype):
+    fsize = O.fft_size(ss, n_fft)
+    shape = (max(6 * ss, 96), max(5 * ss, 80))
+    fa, fb = _frames(shape, dtype, seed=ss + n_fft)
+    geom = O.field_geometry(shape, ss, 0.5)
+    ref = O.correlate_pass(fa, fb, geom, n_fft=n_fft, return_planes=True)
+    got = gu.stage_correlate(fa, fb, geom, fsize, planes=True)
+    planes_ref = ref["corr"]
+    scale = np.abs(planes_ref).reshape(len(planes_ref), -1).max(axis=1)[:, None, None]
+    err = np.abs(got["corr"] - planes_ref) / np.maximum(scale, 1e-30)
+    assert err.max() < 2e-5, f"plane rel err {err.max()}"
+    ties = gu.near_tie_mask(planes_ref).reshape(geom.field_shape)
+    assert ties.mean() < 0.01
+    ok = ~ties
+    assert np.array_equal(got["i_peak"][ok], ref["i_peak"][ok])
+    assert np.array_equal(got["j_peak"][ok], ref["j_peak"][ok])
+    assert np.abs(got["u"] - ref["u"])[ok].max() <= 1e-3
+    assert np.abs(got["v"] - ref["v"])[ok].max() <= 1e-3
+
+
+@pytest.mark.parametrize("method,code", [("gaussian", 0), ("parabolic", 1), ("centroid", 2)])
+def test_correlate_subpixel_methods_and_predictor(method, code):
+    shape = (256, 256)
+    fa, fb = _frames(shape, np.uint8, seed=11)
+    geom = O.field_geometry(shape, 32, 0.5)
+    ref = O.correlate_pass(fa, fb, geom, subpixel_method=method, return_planes=True)
+    rng = np.random.default_rng(0)
+    up = rng.uniform(-3, 3, geom.field_shape).astype(np.float32)
+    vpred = rng.uniform(-3, 3, geom.field_shape).astype(np.float32)
+    got = gu.stage_correlate(fa, fb, geom, 64, up=up, vpred=vpred, subpixel=code)
+    ok = ~gu.near_tie_mask(ref["corr"]).reshape(geom.field_shape)
+    assert np.abs(got["u"] - (ref["u"] + up))[ok].max() <= 1e-3
+    assert np.abs(got["v"] - (ref["v"] + vpred))[ok].max() <= 1e-3
+
+
+def test_correlate_blank_window_gives_corner_peak():
+    # all-constant windows correlate to exact zeros: the first occurrence
+    # (index 0) wins and the border keeps the integer peak -> u = v = -F/2
+    fa = np.full((64, 64), 7, np.uint8)
+    geom = O.field_geometry(fa.shape, 16, 0.5)
+    got = gu.stage_correlate(fa, fa, geom, 32)
+    ref = O.correlate_pass(fa, fa, geom)
+    assert np.array_equal(got["u"], ref["u"]) and np.all(got["u"] == -16)
+
+
+@pytest.mark.parametrize("deforming_par", [0.5, 1.0, 0.0, 0.3])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_deform_matches_oracle(deforming_par, dtype):
+    shape = (160, 200)
+    fa, fb = _frames(shape, dtype, seed=3)
+    geom = O.field_geometry(shape, 32, 0.5)
+    rng = np.random.default_rng(1)
+    up = rng.uniform(-4, 4, geom.field_shape)
+    vpred = rng.uniform(-4, 4, geom.field_shape)
+    ra, rb = O.deform_frames(fa, fb, geom, up, vpred, deforming_par)
+    ga, gb = gu.stage_deform(fa, fb, geom, up.astype(np.float32), vpred.astype(np.float32), deforming_par)
+    tol = 1e-5 * float(np.iinfo(dtype).max) * 40
+    assert np.abs(ga - ra).max() <= tol
+    assert np.abs(gb - rb).max() <= tol
+
+
+def test_grid_interp_matches_oracle():
+    shape = (512, 640)
+    g1 = O.field_geometry(shape, 64, 0.5)
+    g2 = O.field_geometry(shape, 32, 0.5)
+    rng = np.random.default_rng(2)
+    u = rng.uniform(-5, 5, g1.field_shape)
+    v = rng.uniform(-5, 5, g1.field_shape)
+    ru, rv = O.interpolate_displacement(g1, g2, u, v)
+    gu_, gv_ = gu.stage_grid_interp(g1, g2, u, v)
+    assert np.abs(gu_ - ru).max() < 1e-5 and np.abs(gv_ - rv).max() < 1e-5
+
+
+def _noisy_field(shape, seed, n_out):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), indexing="ij")
+    u = 3 * np.sin(xx / 9.0) + 0.3 * rng.standard_normal(shape)
+    v = 2 * np.cos(yy / 7.0) + 0.3 * rng.standard_normal(shape)
+    idx = rng.choice(u.size, n_out, replace=False)
+    u.flat[idx] += rng.uniform(-15, 15, n_out)
+    v.flat[idx[: n_out // 2]] += rng.uniform(-15, 15, n_out // 2)
+    return u.astype(np.float32), v.astype(np.float32)
+
+
+@pytest.mark.parametrize("shape,n_out", [((31, 31), 20), ((127, 127), 300), ((255, 255), 2000), ((1, 9), 2),
+                                         ((2, 2), 1)])
+def test_validate_matches_oracle(shape, n_out):
+    u, v = _noisy_field(shape, 4, n_out)
+    ref = O.median_validation(u, v, 2.0)
+    got, count = gu.stage_validate(u, v, 2.0)
+    ratio = O.median_residual_ratio(u, v, 2.0)
+    band = np.abs(ratio - 1.0) <= 1e-5
+    assert np.array_equal(got[~band], ref[~band])
+    assert count == int(got.sum())
+
+
+@pytest.mark.parametrize("shape,n_out", [((31, 31), 20), ((127, 127), 400), ((63, 63), 1500)])
+def test_spring_replace_matches_oracle(shape, n_out):
+    u, v = _noisy_field(shape, 5, n_out)
+    flags = O.median_validation(u, v, 2.0)
+    ru, rv = O.spring_replacement(u.astype(np.float64), v.astype(np.float64), flags)
+    gu_, gv_ = gu.stage_replace(u, v, flags)
+    assert np.abs(gu_ - ru).max() < 1e-4 and np.abs(gv_ - rv).max() < 1e-4
+    assert np.array_equal(gu_[~flags], u[~flags])
+
+
+def test_spring_replace_all_outliers_keeps_field():
+    u, v = _noisy_field((9, 9), 6, 0)
+    flags = np.ones((9, 9), bool)
+    gu_, gv_ = gu.stage_replace(u, v, flags)
+    assert np.array_equal(gu_, u) and np.array_equal(gv_, v)
+
+
+@pytest.mark.parametrize("shape", [(63, 63), (127, 127), (17, 23)])
+def test_smoothn_fixed_s(shape):
+    u, v = _noisy_field(shape, 7, 0)
+    rz = O.smoothn([u, v], s=2.5)
+    gz_u, gz_v, p = gu.stage_smoothn(u, v, s=2.5)
+    assert abs(p - np.log10(2.5)) < 1e-12
+    assert np.abs(gz_u - rz[0]).max() < 5e-5 and np.abs(gz_v - rz[1]).max() < 5e-5
+
+
+@pytest.mark.parametrize("shape", [(63, 63), (127, 127)])
+def test_smoothn_auto_s(shape):
+    u, v = _noisy_field(shape, 8, 0)
+    (ru, rv), p_ref = O.smoothn([u, v], return_p=True)
+    p_exact = O.gcv_exact_minimiser([u, v])
+    gz_u, gz_v, p = gu.stage_smoothn(u, v)
+    assert abs(p - p_exact) < 1e-6, (p, p_exact, p_ref)
+    assert abs(p - p_ref) < 5e-3
+    assert np.abs(gz_u - ru).max() < 1e-3 and np.abs(gz_v - rv).max() < 1e-3
