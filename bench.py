@@ -39,67 +39,55 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def stage_kernel_times(torch, fa_u16, piv, reps=20):
+def stage_kernel_times(fa_u16, piv, reps=20):
     """Average duration of each pass's correlation launch and of the
-    deformation launch, timed with events on the stream the kernels run on."""
+    deformation launch, timed with HIP events on the stream the kernels are
+    launched on."""
     from openpiv_gpu import _native
     lib = _native.load_library()
-    stream = torch.cuda.current_stream()
-    sh = ctypes.c_void_p(stream.cuda_stream)
+    stream = _native.Stream()
+    sh = stream.handle
     ht, wd = fa_u16.shape
-    d_u16 = torch.from_numpy(np.ascontiguousarray(fa_u16).view(np.int16)).to("cuda")
-    d_f32 = torch.from_numpy(fa_u16.astype(np.float32)).to("cuda")
+    d_u16 = _native.DeviceBuffer.from_array(fa_u16)
+    d_f32 = _native.DeviceBuffer.from_array(fa_u16.astype(np.float32))
     desc = piv._plan_desc(_native.PIV_U16)
-    out = []
-    for k in range(desc.n_passes):
-        p = desc.passes[k]
-        m, n = p.field_m, p.field_n
-        u = torch.empty((m, n), dtype=torch.float32, device="cuda")
-        v = torch.empty_like(u)
-        if k == 0:
-            src, code = d_u16, _native.PIV_U16
-        else:
-            src, code = d_f32, _native.PIV_F32
-        ptr = ctypes.c_void_p(src.data_ptr())
 
-        def launch():
-            _native.check(lib.piv_stage_correlate(ptr, ptr, code, ht, wd, ctypes.byref(p), None, None,
-                                                  ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(v.data_ptr()),
-                                                  None, None, None, sh))
+    def timed(launch):
         for _ in range(3):
             launch()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
+        e0, e1 = _native.Event(), _native.Event()
         e0.record(stream)
         for _ in range(reps):
             launch()
         e1.record(stream)
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
+        stream.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    out = []
+    for k in range(desc.n_passes):
+        p = desc.passes[k]
+        m, n = p.field_m, p.field_n
+        u = _native.DeviceBuffer((m, n), np.float32)
+        v = _native.DeviceBuffer((m, n), np.float32)
+        src, code = (d_u16, _native.PIV_U16) if k == 0 else (d_f32, _native.PIV_F32)
+
+        def launch():
+            _native.check(lib.piv_stage_correlate(src.ptr, src.ptr, code, ht, wd, ctypes.byref(p), None, None,
+                                                  u.ptr, v.ptr, None, None, None, sh))
+        ms = timed(launch)
         flops = fft_flops_per_window(p.fft_size) * m * n
         out.append({"pass": k, "kernel": f"corr_kernel<F={p.fft_size}>", "windows": m * n, "fft": p.fft_size,
                     "avg_ms": ms, "flop": flops, "tflops": flops / (ms * 1e-3) / 1e12})
-    # deformation (pass 2 geometry): reads both uint16 frames, writes two FP32 frames
+    # deformation (second pass grid): reads both uint16 frames, writes two FP32 frames
     p = desc.passes[min(1, desc.n_passes - 1)]
-    up = torch.zeros((p.field_m, p.field_n), dtype=torch.float32, device="cuda") + 1.5
-    oa = torch.empty((ht, wd), dtype=torch.float32, device="cuda")
-    ob = torch.empty_like(oa)
-    ptr = ctypes.c_void_p(d_u16.data_ptr())
+    up = _native.DeviceBuffer.from_array(np.full((p.field_m, p.field_n), 1.5, np.float32))
+    oa = _native.DeviceBuffer((ht, wd), np.float32)
+    ob = _native.DeviceBuffer((ht, wd), np.float32)
 
     def dlaunch():
-        _native.check(lib.piv_stage_deform(ptr, ptr, _native.PIV_U16, ht, wd, ctypes.byref(p),
-                                           ctypes.c_void_p(up.data_ptr()), ctypes.c_void_p(up.data_ptr()),
-                                           ctypes.c_void_p(oa.data_ptr()), ctypes.c_void_p(ob.data_ptr()), sh))
-    for _ in range(3):
-        dlaunch()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        dlaunch()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
+        _native.check(lib.piv_stage_deform(d_u16.ptr, d_u16.ptr, _native.PIV_U16, ht, wd, ctypes.byref(p),
+                                           up.ptr, up.ptr, oa.ptr, ob.ptr, sh))
+    ms = timed(dlaunch)
     nbytes = ht * wd * (2 * 2 + 2 * 4)
     deform = {"kernel": "deform_kernel<uint16>", "avg_ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9}
     return out, deform
@@ -132,27 +120,22 @@ def main():
               "so several GPUs would each run this same loop on their share of the pair stream.", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-    from openpiv_gpu import synthetic
+    from openpiv_gpu import _native, synthetic
     from openpiv_gpu.gpu_process import PIVGPU
 
     log("generating the 2048x2048 uint16 pair (seed 2)")
     fa, fb, disp = synthetic.cfg_2048_uint16(seed=2)
     piv = PIVGPU((2048, 2048), 16, search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2,
                  dtype_f="float32", use_graph=not args.no_graph)
-    from openpiv_gpu import _native
     plan = piv._get_plan(_native.PIV_U16)
     m, n = piv.piv_fields[-1].field_shape
-    dev = torch.device("cuda", 0)
-    # uint16 bits carried in an int16 tensor (torch is only the allocator)
-    da = torch.from_numpy(np.ascontiguousarray(fa).view(np.int16)).to(dev)
-    db = torch.from_numpy(np.ascontiguousarray(fb).view(np.int16)).to(dev)
-    u = torch.empty((m, n), dtype=torch.float32, device=dev)
-    v = torch.empty_like(u)
-    vi = torch.empty((m, n), dtype=torch.uint8, device=dev)
-    vf = torch.empty_like(vi)
-    torch.cuda.synchronize()
-    ptrs = [t.data_ptr() for t in (da, db, u, v, vi, vf)]
+    da = _native.DeviceBuffer.from_array(fa)
+    db = _native.DeviceBuffer.from_array(fb)
+    u = _native.DeviceBuffer((m, n), np.float32)
+    v = _native.DeviceBuffer((m, n), np.float32)
+    vi = _native.DeviceBuffer((m, n), np.uint8)
+    vf = _native.DeviceBuffer((m, n), np.uint8)
+    ptrs = [b.value for b in (da, db, u, v, vi, vf)]
 
     def step():
         plan.run(ptrs[0], ptrs[1], True, ptrs[2], ptrs[3], ptrs[4], ptrs[5], True)
@@ -160,12 +143,12 @@ def main():
     log(f"warmup {args.warmup}")
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    _native.synchronize()
     log(f"timing {args.steps} steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
-    torch.cuda.synchronize()
+    _native.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = 1e3 / ms_per_step
@@ -173,7 +156,7 @@ def main():
     # accuracy of this run against the known displacement
     x, y = piv.coords
     ut, vt = disp(np.asarray(x, np.float64), np.asarray(y, np.float64))
-    uh, vh = u.cpu().numpy(), v.cpu().numpy()
+    uh, vh = u.get(), v.get()
     rms = float(np.sqrt(np.mean((uh - ut) ** 2 + (vh - vt) ** 2) / 2))
 
     # host-to-host rate (NumPy in/out, includes PCIe both ways)
@@ -188,7 +171,7 @@ def main():
     deform = None
     if not args.skip_kernel_timing:
         log("per-kernel timing")
-        kernels, deform = stage_kernel_times(torch, fa, piv)
+        kernels, deform = stage_kernel_times(fa, piv)
         dom = max(kernels, key=lambda r: r["avg_ms"])
         roof = {"bound": "mfma", "compute": "fp32-valu (no matrix cores; FP32 vector peak)",
                 "kernel": dom["kernel"], "achieved": round(dom["tflops"], 3), "peak": 157.3, "unit": "TFLOP/s",

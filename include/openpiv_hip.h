@@ -147,6 +147,26 @@ int piv_stage_spring_replace(float *u, float *v, const uint8_t *flags, const int
  * Replaces SmoothnGPU.__call__ (gpu_smoothn.py:123-231, 284-372). */
 int piv_stage_smoothn(float *u, float *v, int m, int n, double smoothing_par, double *log10_s_out, void *stream);
 
+/* ---- device memory, streams and events --------------------------------
+ * The reference allocates and moves data with CuPy (cp.asarray / cp.empty /
+ * .get(), gpu_process.py:582-583, 1127, 316).  These entry points give a
+ * caller without another GPU runtime the same services on the current
+ * device (device 0 unless piv_set_device was called). */
+int piv_set_device(int device);
+int piv_malloc(void **ptr, size_t bytes);
+int piv_free(void *ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device */
+int piv_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int piv_memset(void *ptr, int value, size_t bytes);
+int piv_synchronize(void);
+int piv_stream_create(void **stream);
+int piv_stream_destroy(void *stream);
+int piv_stream_synchronize(void *stream);
+int piv_event_create(void **event);
+int piv_event_destroy(void *event);
+int piv_event_record(void *event, void *stream);
+int piv_event_elapsed_ms(void *start, void *stop, float *ms);
+
 #ifdef __cplusplus
 }
 #endif

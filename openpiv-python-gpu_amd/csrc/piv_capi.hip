@@ -26,10 +26,12 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                                           \
-  do {                                                                                          \
-    hipError_t _e = (expr);                                                                     \
-    if (_e != hipSuccess) return fail(PIV_E_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+#define HIP_TRY(expr)                                                                               \
+  do {                                                                                              \
+    piv::set_launch_detail("", hipSuccess);                                                          \
+    hipError_t _e = (expr);                                                                         \
+    if (_e != hipSuccess)                                                                           \
+      return fail(PIV_E_HIP, "%s failed: %s [%s]", #expr, hipGetErrorString(_e), piv::launch_detail()); \
   } while (0)
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
@@ -534,6 +536,75 @@ int piv_stage_smoothn(float* u, float* v, int m, int n, double smoothing_par, do
   free_tables(&tab);
   if (e != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e));
   if (e2 != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e2));
+  return PIV_OK;
+}
+
+int piv_set_device(int device) {
+  HIP_TRY(hipSetDevice(device));
+  return PIV_OK;
+}
+int piv_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return fail(PIV_E_INVALID, "null argument");
+  *ptr = nullptr;
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
+  if (e != hipSuccess) return fail(PIV_E_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return PIV_OK;
+}
+int piv_free(void* ptr) {
+  if (ptr) HIP_TRY(hipFree(ptr));
+  return PIV_OK;
+}
+int piv_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  if ((!dst || !src) && bytes) return fail(PIV_E_INVALID, "null argument");
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                                    : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+  if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, k));
+  return PIV_OK;
+}
+int piv_memset(void* ptr, int value, size_t bytes) {
+  if (!ptr && bytes) return fail(PIV_E_INVALID, "null argument");
+  if (bytes) HIP_TRY(hipMemset(ptr, value, bytes));
+  return PIV_OK;
+}
+int piv_synchronize(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return PIV_OK;
+}
+int piv_stream_create(void** stream) {
+  if (!stream) return fail(PIV_E_INVALID, "null argument");
+  hipStream_t s = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void*)s;
+  return PIV_OK;
+}
+int piv_stream_destroy(void* stream) {
+  if (stream) HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+  return PIV_OK;
+}
+int piv_stream_synchronize(void* stream) {
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  return PIV_OK;
+}
+int piv_event_create(void** event) {
+  if (!event) return fail(PIV_E_INVALID, "null argument");
+  hipEvent_t e = nullptr;
+  HIP_TRY(hipEventCreate(&e));
+  *event = (void*)e;
+  return PIV_OK;
+}
+int piv_event_destroy(void* event) {
+  if (event) HIP_TRY(hipEventDestroy((hipEvent_t)event));
+  return PIV_OK;
+}
+int piv_event_record(void* event, void* stream) {
+  if (!event) return fail(PIV_E_INVALID, "null argument");
+  HIP_TRY(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+  return PIV_OK;
+}
+int piv_event_elapsed_ms(void* start, void* stop, float* ms) {
+  if (!start || !stop || !ms) return fail(PIV_E_INVALID, "null argument");
+  HIP_TRY(hipEventSynchronize((hipEvent_t)stop));
+  HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
   return PIV_OK;
 }
 

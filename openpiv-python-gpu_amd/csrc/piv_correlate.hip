@@ -442,16 +442,24 @@ static hipError_t launch_f(const CorrArgs& a, hipStream_t stream) {
   const size_t region = correlate_lds_bytes(F, a.ss);
   const size_t lds = region * L::WPB;
   const int blocks = (n_windows + L::WPB - 1) / L::WPB;
-  static bool attr_set = false;
-  if (!attr_set) {
+  (void)hipGetLastError();  // do not report an earlier, unrelated error
+  // Dynamic LDS above 64 KiB must be opted into; request exactly what this
+  // launch needs (gfx950 allows up to 160 KiB per workgroup).
+  static thread_local size_t attr_bytes = 0;
+  if (lds > 65536 && lds > attr_bytes) {
     hipError_t e = hipFuncSetAttribute((const void*)corr_kernel<F, InT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds > 65536 ? 160 * 1024 : 65536);
-    if (e != hipSuccess) return e;
-    attr_set = true;
+                                       (int)lds);
+    if (e != hipSuccess) {
+      set_launch_detail("hipFuncSetAttribute(MaxDynamicSharedMemorySize)", e);
+      return e;
+    }
+    attr_bytes = lds;
   }
   hipLaunchKernelGGL((corr_kernel<F, InT>), dim3(blocks), dim3(256), lds, stream, a, n_windows,
                      (int)(region / sizeof(float)));
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) set_launch_detail("corr_kernel launch", e);
+  return e;
 }
 
 template <int F>
@@ -465,7 +473,10 @@ static hipError_t launch_t(const CorrArgs& a, hipStream_t stream) {
 }
 
 hipError_t launch_correlate(const CorrArgs& a, hipStream_t stream) {
-  if (a.ss < 2 || a.ss > a.fsize || (a.fsize - a.ss) % 2 != 0) return hipErrorInvalidValue;
+  if (a.ss < 2 || a.ss > a.fsize || (a.fsize - a.ss) % 2 != 0) {
+    set_launch_detail("correlate: search_size / fft_size mismatch", hipErrorInvalidValue);
+    return hipErrorInvalidValue;
+  }
   switch (a.fsize) {
     case 16: return launch_t<16>(a, stream);
     case 32: return launch_t<32>(a, stream);

@@ -1,6 +1,7 @@
 // Field-level kernels of one PIV pass: frame deformation, grid-to-grid
 // predictor interpolation, 3x3 median validation and spring replacement.
 #include <math.h>
+#include <stdio.h>
 
 #include "piv_fft.h"
 #include "piv_kernels.h"
@@ -371,7 +372,14 @@ __global__ void copy_u8_kernel(uint8_t* dst, const uint8_t* src, int count) {
   if (k < count) dst[k] = src[k];
 }
 
+thread_local char g_detail[256] = "";
+
 }  // namespace
+
+const char* launch_detail() { return g_detail; }
+void set_launch_detail(const char* what, hipError_t e) {
+  snprintf(g_detail, sizeof(g_detail), "%s: %s", what, hipGetErrorString(e));
+}
 
 hipError_t launch_deform(const DeformArgs& a, hipStream_t stream) {
   if (a.m < 2 || a.n < 2) return hipErrorInvalidValue;

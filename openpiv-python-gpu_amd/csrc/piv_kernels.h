@@ -93,6 +93,11 @@ struct SmoothnWork {
 hipError_t launch_smoothn(float* u, float* v, int m, int n, double p_min, double p_max, double fixed_p,
                           int auto_s, const SmoothnWork& w, hipStream_t stream);
 
+// Text describing the last failing launcher call on this thread ("" when the
+// failure came from the kernel launch itself).
+const char* launch_detail();
+void set_launch_detail(const char* what, hipError_t e);
+
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int count, hipStream_t stream);
 hipError_t launch_scale(float* u, float* v, int count, float factor, hipStream_t stream);
 hipError_t launch_copy_f32(float* dst, const float* src, int count, hipStream_t stream);

@@ -24,6 +24,9 @@ EXPORTED_SYMBOLS = (
     "piv_last_error", "piv_device_count", "piv_plan_create", "piv_plan_destroy", "piv_run",
     "piv_plan_last_smoothing", "piv_plan_last_counts", "piv_stage_correlate", "piv_stage_deform",
     "piv_stage_grid_interp", "piv_stage_median_validate", "piv_stage_spring_replace", "piv_stage_smoothn",
+    "piv_set_device", "piv_malloc", "piv_free", "piv_memcpy", "piv_memset", "piv_synchronize",
+    "piv_stream_create", "piv_stream_destroy", "piv_stream_synchronize", "piv_event_create", "piv_event_destroy",
+    "piv_event_record", "piv_event_elapsed_ms",
 )
 
 
@@ -83,6 +86,19 @@ def _bind(lib):
         "piv_stage_median_validate": (i32, [vp, vp, i32, i32, f32, vp, vp, vp]),
         "piv_stage_spring_replace": (i32, [vp, vp, vp, vp, i32, i32, vp]),
         "piv_stage_smoothn": (i32, [vp, vp, i32, i32, f64, vp, vp]),
+        "piv_set_device": (i32, [i32]),
+        "piv_malloc": (i32, [ctypes.POINTER(vp), ctypes.c_size_t]),
+        "piv_free": (i32, [vp]),
+        "piv_memcpy": (i32, [vp, vp, ctypes.c_size_t, i32]),
+        "piv_memset": (i32, [vp, i32, ctypes.c_size_t]),
+        "piv_synchronize": (i32, []),
+        "piv_stream_create": (i32, [ctypes.POINTER(vp)]),
+        "piv_stream_destroy": (i32, [vp]),
+        "piv_stream_synchronize": (i32, [vp]),
+        "piv_event_create": (i32, [ctypes.POINTER(vp)]),
+        "piv_event_destroy": (i32, [vp]),
+        "piv_event_record": (i32, [vp, vp]),
+        "piv_event_elapsed_ms": (i32, [vp, vp, ctypes.POINTER(ctypes.c_float)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -129,6 +145,101 @@ def in_dtype_code(dtype) -> int:
     if dt == np.float32:
         return PIV_F32
     raise ValueError(f"unsupported frame dtype {dt}")
+
+
+class DeviceBuffer:
+    """Device allocation owned by Python (freed with the object)."""
+
+    def __init__(self, shape, dtype):
+        lib = load_library()
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(lib.piv_malloc(ctypes.byref(p), self.nbytes))
+        self._lib = lib
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        buf = cls(a.shape, a.dtype)
+        buf.upload(a)
+        return buf
+
+    @classmethod
+    def zeros(cls, shape, dtype):
+        buf = cls(shape, dtype)
+        check(buf._lib.piv_memset(buf.ptr, 0, buf.nbytes))
+        return buf
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(self._lib.piv_memcpy(self.ptr, a.ctypes.data, self.nbytes, 0))
+
+    def get(self):
+        out = np.empty(self.shape, self.dtype)
+        check(self._lib.piv_memcpy(out.ctypes.data, self.ptr, self.nbytes, 1))
+        return out
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+    def free(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            self._lib.piv_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        check(self._lib.piv_stream_create(ctypes.byref(h)))
+        self.handle = h
+
+    def synchronize(self):
+        check(self._lib.piv_stream_synchronize(self.handle))
+
+    def __del__(self):
+        try:
+            self._lib.piv_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        check(self._lib.piv_event_create(ctypes.byref(h)))
+        self.handle = h
+
+    def record(self, stream=None):
+        check(self._lib.piv_event_record(self.handle, stream.handle if stream is not None else None))
+
+    def elapsed_ms(self, stop):
+        ms = ctypes.c_float()
+        check(self._lib.piv_event_elapsed_ms(self.handle, stop.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            self._lib.piv_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def synchronize():
+    check(load_library().piv_synchronize())
 
 
 class Plan:

@@ -88,19 +88,19 @@ def _host(a):
 
 class DeviceArray:
     """Device-resident result (``PIVGPU`` outputs); ``.get()`` copies to the
-    host like a CuPy array.  ``.tensor`` exposes the underlying torch tensor
-    (torch is only the allocator here)."""
+    host like a CuPy array.  ``.buffer`` is the underlying device allocation
+    (``.buffer.ptr`` its address)."""
 
-    def __init__(self, tensor, dtype):
-        self.tensor = tensor
+    def __init__(self, buffer, dtype):
+        self.buffer = buffer
         self.dtype = np.dtype(dtype)
 
     @property
     def shape(self):
-        return tuple(self.tensor.shape)
+        return tuple(self.buffer.shape)
 
     def get(self):
-        return self.tensor.detach().cpu().numpy().astype(self.dtype, copy=False)
+        return self.buffer.get().astype(self.dtype, copy=False)
 
     def __array__(self, dtype=None, copy=None):
         a = self.get()
@@ -340,14 +340,10 @@ class PIVGPU:
         plan = self._get_plan(_native.in_dtype_code(fa.dtype))
         m, n = self.piv_fields[-1].field_shape
         if outputs_on_device:
-            import torch
-            dev = torch.device("cuda", self.device)
-            u = torch.empty((m, n), dtype=torch.float32, device=dev)
-            v = torch.empty((m, n), dtype=torch.float32, device=dev)
-            vi = torch.empty((m, n), dtype=torch.uint8, device=dev)
-            vf = torch.empty((m, n), dtype=torch.uint8, device=dev)
-            torch.cuda.synchronize(dev)
-            ptrs = [t.data_ptr() for t in (u, v, vi, vf)]
+            outs = [_native.DeviceBuffer((m, n), np.float32), _native.DeviceBuffer((m, n), np.float32),
+                    _native.DeviceBuffer((m, n), np.uint8), _native.DeviceBuffer((m, n), np.uint8)]
+            u, v, vi, vf = outs
+            ptrs = [b.value for b in outs]
         else:
             u = np.empty((m, n), np.float32)
             v = np.empty((m, n), np.float32)
@@ -415,8 +411,8 @@ class PIVGPU:
 def _flags_to_bool(a):
     if a is None:
         return None
-    if hasattr(a, "detach"):
-        a = a.detach().cpu().numpy()
+    if isinstance(a, _native.DeviceBuffer):
+        a = a.get()
     return np.asarray(a).astype(bool)
 
 

@@ -1,5 +1,5 @@
-"""Helpers for GPU tests: device buffers (torch as the allocator) and thin
-wrappers that call the C ABI stage entry points."""
+"""Helpers for GPU tests: device buffers from the library's own allocator and
+thin wrappers that call the C ABI stage entry points."""
 
 import ctypes
 
@@ -9,29 +9,20 @@ from openpiv_gpu import _native
 from oracle import piv_oracle as O
 
 
-def torch_mod():
-    import torch
-    return torch
-
-
 def dev(a):
-    torch = torch_mod()
-    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    return _native.DeviceBuffer.from_array(np.ascontiguousarray(a))
 
 
 def empty(shape, dtype):
-    torch = torch_mod()
-    return torch.empty(shape, dtype=dtype, device="cuda")
+    return _native.DeviceBuffer(tuple(shape), dtype)
 
 
-def vp(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+def vp(b):
+    return ctypes.c_void_p(b.value) if b is not None else None
 
 
-def host(t):
-    torch = torch_mod()
-    torch.cuda.synchronize()
-    return t.cpu().numpy()
+def host(b):
+    return b.get()
 
 
 def pass_desc(geom: O.Geometry, fsize: int, normalize=True, subpixel=0, deforming_par=0.5):
@@ -54,18 +45,16 @@ def pass_desc(geom: O.Geometry, fsize: int, normalize=True, subpixel=0, deformin
 
 def stage_correlate(fa, fb, geom, fsize, up=None, vpred=None, planes=False, subpixel=0):
     lib = _native.load_library()
-    torch = torch_mod()
     m, n = geom.field_shape
     da, db = dev(fa), dev(fb)
-    u = empty((m, n), torch.float32)
-    v = empty((m, n), torch.float32)
-    ip = empty((m, n), torch.int32)
-    jp = empty((m, n), torch.int32)
-    pl = empty((m * n, fsize, fsize), torch.float32) if planes else None
+    u = empty((m, n), np.float32)
+    v = empty((m, n), np.float32)
+    ip = empty((m, n), np.int32)
+    jp = empty((m, n), np.int32)
+    pl = empty((m * n, fsize, fsize), np.float32) if planes else None
     dup = dev(np.asarray(up, np.float32)) if up is not None else None
     dvp = dev(np.asarray(vpred, np.float32)) if vpred is not None else None
     desc = pass_desc(geom, fsize, subpixel=subpixel)
-    torch.cuda.synchronize()
     _native.check(lib.piv_stage_correlate(vp(da), vp(db), _native.in_dtype_code(fa.dtype), fa.shape[0], fa.shape[1],
                                           ctypes.byref(desc), vp(dup), vp(dvp), vp(u), vp(v), vp(ip), vp(jp), vp(pl),
                                           None))
@@ -77,13 +66,11 @@ def stage_correlate(fa, fb, geom, fsize, up=None, vpred=None, planes=False, subp
 
 def stage_deform(fa, fb, geom, up, vpred, deforming_par=0.5):
     lib = _native.load_library()
-    torch = torch_mod()
     da, db = dev(fa), dev(fb)
-    oa = empty(fa.shape, torch.float32)
-    ob = empty(fa.shape, torch.float32)
+    oa = empty(fa.shape, np.float32)
+    ob = empty(fa.shape, np.float32)
     desc = pass_desc(geom, O.fft_size(geom.search_size), deforming_par=deforming_par)
     dup, dvp = dev(np.asarray(up, np.float32)), dev(np.asarray(vpred, np.float32))
-    torch.cuda.synchronize()
     _native.check(lib.piv_stage_deform(vp(da), vp(db), _native.in_dtype_code(fa.dtype), fa.shape[0], fa.shape[1],
                                        ctypes.byref(desc), vp(dup), vp(dvp), vp(oa), vp(ob), None))
     return host(oa), host(ob)
@@ -91,13 +78,11 @@ def stage_deform(fa, fb, geom, up, vpred, deforming_par=0.5):
 
 def stage_grid_interp(g_from, g_to, u, v):
     lib = _native.load_library()
-    torch = torch_mod()
     d_from = pass_desc(g_from, O.fft_size(g_from.search_size))
     d_to = pass_desc(g_to, O.fft_size(g_to.search_size))
     du, dv = dev(np.asarray(u, np.float32)), dev(np.asarray(v, np.float32))
-    uo = empty(g_to.field_shape, torch.float32)
-    vo = empty(g_to.field_shape, torch.float32)
-    torch.cuda.synchronize()
+    uo = empty(g_to.field_shape, np.float32)
+    vo = empty(g_to.field_shape, np.float32)
     _native.check(lib.piv_stage_grid_interp(ctypes.byref(d_from), ctypes.byref(d_to), vp(du), vp(dv), vp(uo), vp(vo),
                                             None))
     return host(uo), host(vo)
@@ -105,35 +90,29 @@ def stage_grid_interp(g_from, g_to, u, v):
 
 def stage_validate(u, v, tol=2.0):
     lib = _native.load_library()
-    torch = torch_mod()
     m, n = u.shape
     du, dv = dev(np.asarray(u, np.float32)), dev(np.asarray(v, np.float32))
-    flags = empty((m, n), torch.uint8)
-    count = torch.zeros(1, dtype=torch.int32, device="cuda")
-    torch.cuda.synchronize()
+    flags = empty((m, n), np.uint8)
+    count = _native.DeviceBuffer.zeros((1,), np.int32)
     _native.check(lib.piv_stage_median_validate(vp(du), vp(dv), m, n, float(tol), vp(flags), vp(count), None))
     return host(flags).astype(bool), int(host(count)[0])
 
 
 def stage_replace(u, v, flags):
     lib = _native.load_library()
-    torch = torch_mod()
     m, n = u.shape
     du, dv = dev(np.asarray(u, np.float32)), dev(np.asarray(v, np.float32))
     df = dev(np.asarray(flags, np.uint8))
     count = dev(np.array([int(np.count_nonzero(flags))], np.int32))
-    torch.cuda.synchronize()
     _native.check(lib.piv_stage_spring_replace(vp(du), vp(dv), vp(df), vp(count), m, n, None))
     return host(du), host(dv)
 
 
 def stage_smoothn(u, v, s=None):
     lib = _native.load_library()
-    torch = torch_mod()
     m, n = u.shape
     du, dv = dev(np.asarray(u, np.float32)), dev(np.asarray(v, np.float32))
-    p = torch.zeros(1, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
+    p = _native.DeviceBuffer.zeros((1,), np.float64)
     _native.check(lib.piv_stage_smoothn(vp(du), vp(dv), m, n, float(s) if s else 0.0, vp(p), None))
     return host(du), host(dv), float(host(p)[0])
 
@@ -145,3 +124,20 @@ def near_tie_mask(planes, rel=1e-5):
     part = np.partition(flat, -2, axis=1)
     top, second = part[:, -1], part[:, -2]
     return (top - second) <= rel * np.maximum(np.abs(top), 1e-30)
+
+
+def fit_sign_ambiguous(planes, i_peak, j_peak, rel=1e-5):
+    """Windows where one of the five samples of the 3-point Gaussian fit is
+    within ``rel`` (of the peak) of zero: whether the fit runs (all five > 0)
+    then depends on rounding, and the result can jump by up to 0.5 px."""
+    n, F, _ = planes.shape
+    ip = i_peak.reshape(-1)
+    jp = j_peak.reshape(-1)
+    out = np.zeros(n, bool)
+    for k in range(n):
+        i, j = ip[k], jp[k]
+        if 0 < i < F - 1 and 0 < j < F - 1:
+            c = planes[k, i, j]
+            vals = np.array([c, planes[k, i - 1, j], planes[k, i + 1, j], planes[k, i, j - 1], planes[k, i, j + 1]])
+            out[k] = np.any(np.abs(vals) <= rel * max(abs(c), 1e-30))
+    return out

@@ -73,7 +73,7 @@ def cfg2_runs():
     runs = []
     for _ in range(3):  # eager, graph capture + replay, replay
         u, v = piv(a, b)
-        runs.append((u.get(), v.get(), piv.init_val_locations.cpu().numpy().copy(),
+        runs.append((u.get(), v.get(), piv.init_val_locations.get().copy(),
                      [piv.smoothing_log10_s(k) for k in range(2)]))
     s = O.OracleSettings(min_search_size=16, search_size_iters=(1, 1, 1))
     ref = O.piv(a, b, s, keep_passes=True)

@@ -49,6 +49,9 @@ def test_correlate_matches_oracle(ss, n_fft, dtype):
     ok = ~ties
     assert np.array_equal(got["i_peak"][ok], ref["i_peak"][ok])
     assert np.array_equal(got["j_peak"][ok], ref["j_peak"][ok])
+    amb = gu.fit_sign_ambiguous(planes_ref, ref["i_peak"], ref["j_peak"]).reshape(geom.field_shape)
+    assert amb.mean() < 0.02
+    ok &= ~amb
     assert np.abs(got["u"] - ref["u"])[ok].max() <= 1e-3
     assert np.abs(got["v"] - ref["v"])[ok].max() <= 1e-3
 
@@ -89,7 +92,7 @@ def test_deform_matches_oracle(deforming_par, dtype):
     vpred = rng.uniform(-4, 4, geom.field_shape)
     ra, rb = O.deform_frames(fa, fb, geom, up, vpred, deforming_par)
     ga, gb = gu.stage_deform(fa, fb, geom, up.astype(np.float32), vpred.astype(np.float32), deforming_par)
-    tol = 1e-5 * float(np.iinfo(dtype).max) * 40
+    tol = 1e-3 * float(max(fa.max(), fb.max()))  # FP32 warp vs FP64-interpolated predictor
     assert np.abs(ga - ra).max() <= tol
     assert np.abs(gb - rb).max() <= tol
 
