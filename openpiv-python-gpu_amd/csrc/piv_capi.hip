@@ -220,6 +220,7 @@ struct piv_plan {
   uint8_t* val_final = nullptr;
   float* t0 = nullptr;
   float* t1 = nullptr;
+  void* smooth_state = nullptr;
   ReplaceWork rw{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -281,6 +282,7 @@ int enqueue(piv_plan* P) {
       w.t0 = P->t0;
       w.t1 = P->t1;
       w.p_out = P->p_out + k;
+      w.state = P->smooth_state;
       const int autos = !(B.d.smoothing_par > 0);
       HIP_TRY(launch_smoothn(B.u, B.v, B.d.field_m, B.d.field_n, B.p_min, B.p_max,
                              autos ? 0.0 : log10(B.d.smoothing_par), autos, w, s));
@@ -305,7 +307,7 @@ void release(piv_plan* P) {
   }
   for (void* p : {P->frame_a, P->frame_b, (void*)P->def_a, (void*)P->def_b, (void*)P->counters, (void*)P->p_out,
                   (void*)P->val_final, (void*)P->t0, (void*)P->t1, (void*)P->rw.slot, (void*)P->rw.list,
-                  (void*)P->rw.work})
+                  (void*)P->rw.work, P->smooth_state})
     if (p) (void)hipFree(p);
   if (P->stream) (void)hipStreamDestroy(P->stream);
   delete P;
@@ -382,8 +384,10 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
   ok = dmalloc((void**)&P->val_final, L.N) && dmalloc((void**)&P->t0, 2ull * maxN * 4) &&
        dmalloc((void**)&P->t1, 2ull * maxN * 4) && dmalloc((void**)&P->rw.slot, maxN * 4) &&
        dmalloc((void**)&P->rw.list, maxN * 4) && dmalloc((void**)&P->rw.work, 8ull * maxN * 8);
+  ok = ok && dmalloc(&P->smooth_state, smoothn_state_bytes());
   if (!ok) return bail(fail(PIV_E_NOMEM, "device allocation failed"));
-  if (hipMemset(P->val_final, 0, L.N) != hipSuccess) return bail(fail(PIV_E_HIP, "hipMemset failed"));
+  if (hipMemset(P->val_final, 0, L.N) != hipSuccess || hipMemset(P->smooth_state, 0, smoothn_state_bytes()) != hipSuccess)
+    return bail(fail(PIV_E_HIP, "hipMemset failed"));
   *out = P;
   return PIV_OK;
 }
@@ -523,16 +527,20 @@ int piv_stage_smoothn(float* u, float* v, int m, int n, double smoothing_par, do
   }
   const size_t N = (size_t)m * n;
   float *t0 = nullptr, *t1 = nullptr;
+  void* state = nullptr;
   HIP_TRY(hipMalloc(&t0, 2 * N * 4));
   HIP_TRY(hipMalloc(&t1, 2 * N * 4));
+  HIP_TRY(hipMalloc(&state, smoothn_state_bytes()));
+  HIP_TRY(hipMemset(state, 0, smoothn_state_bytes()));
   double pmin, pmax;
   p_bounds(m, n, &pmin, &pmax);
-  SmoothnWork w{tab.cos_m, tab.cos_n, tab.lambda, t0, t1, log10_s_out};
+  SmoothnWork w{tab.cos_m, tab.cos_n, tab.lambda, t0, t1, log10_s_out, state};
   const int autos = !(smoothing_par > 0);
   hipError_t e = launch_smoothn(u, v, m, n, pmin, pmax, autos ? 0.0 : log10(smoothing_par), autos, w, s);
   hipError_t e2 = hipStreamSynchronize(s);
   (void)hipFree(t0);
   (void)hipFree(t1);
+  (void)hipFree(state);
   free_tables(&tab);
   if (e != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e));
   if (e2 != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e2));

@@ -53,6 +53,8 @@ struct Layout {
   static constexpr int SF = F + 1;         // AB row stride (complex)
   static constexpr int SR = H + 1;         // R row stride (complex)
   static constexpr int SP = F + 1;         // plane row stride (float)
+  // occupancy target (waves per SIMD) the register allocation is held to
+  static constexpr int WAVES = F <= 16 ? 6 : (F == 32 ? 4 : (F == 64 ? 2 : 1));
 };
 
 }  // namespace
@@ -70,7 +72,7 @@ size_t correlate_lds_bytes(int fsize, int ss) {
 }
 
 template <int F, typename InT>
-__global__ __launch_bounds__(256) void corr_kernel(CorrArgs a, int n_windows, int region_floats) {
+__global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a, int n_windows, int region_floats) {
   using L = Layout<F>;
   constexpr int H = L::H, Q = L::Q, LANES = L::LANES, WPB = L::WPB;
   constexpr int SF = L::SF, SR = L::SR, SP = L::SP;
@@ -200,20 +202,32 @@ __global__ __launch_bounds__(256) void corr_kernel(CorrArgs a, int n_windows, in
       c32 w = swap1(o);
       c32 z0 = up1 ? w : o;  // Zr[0]
       c32 zn = up1 ? o : w;  // Zr[F/2]
-      c32 sv = up1 ? c32{z0.y, zn.y} : c32{z0.x, zn.x};
+      c32 sv = up1 ? c32{2.f * z0.y, 2.f * zn.y} : c32{2.f * z0.x, 2.f * zn.x};
       if (active) row[0] = sv;
     }
-    static_for<1, H>([&](auto K) {
+    // slots kx and H-kx read the same two registers: one pair of swaps
+    // serves both.  Lane 1 turns its inputs by -i so that both lanes use
+    // the A formula: B(zk, zm) = A(-i zk, -i zm).  The 1/2 of the split is
+    // folded into the final plane scale.
+    static_for<1, H / 2 + 1>([&](auto K) {
       constexpr int kx = decltype(K)::value;
+      constexpr int kx2 = H - kx;
       constexpr int r1 = bitrev(kx, H);
-      constexpr int r2 = bitrev(H - kx, H);
-      c32 o1 = x[r1], w1 = swap1(o1);
-      c32 o2 = x[r2], w2 = swap1(o2);
-      c32 zk = up1 ? w1 : o1;  // Zr[kx]    (lane 0, k = kx)
-      c32 zm = up1 ? o2 : w2;  // Zr[F-kx]  (lane 1, k = H-kx)
-      c32 sv = up1 ? c32{0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x)}
-                   : c32{0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y)};
-      if (active) row[kx] = sv;
+      constexpr int r2 = bitrev(kx2, H);
+      const c32 o1 = x[r1], w1 = swap1(o1);
+      const c32 o2 = x[r2], w2 = swap1(o2);
+      // slot kx : Zr[kx] = (o1 | w1), Zr[F-kx] = (w2 | o2)
+      // slot kx2: Zr[kx2] = (o2 | w2), Zr[F-kx2] = (w1 | o1)
+      const c32 zk1 = up1 ? c32{w1.y, -w1.x} : o1;
+      const c32 zm1 = up1 ? c32{o2.y, -o2.x} : w2;
+      const c32 s1{zk1.x + zm1.x, zk1.y - zm1.y};
+      if (active) row[kx] = s1;
+      if constexpr (kx2 != kx) {
+        const c32 zk2 = up1 ? c32{w2.y, -w2.x} : o2;
+        const c32 zm2 = up1 ? c32{o1.y, -o1.x} : w1;
+        const c32 s2{zk2.x + zm2.x, zk2.y - zm2.y};
+        if (active) row[kx2] = s2;
+      }
     });
   }
   __syncthreads();
@@ -238,13 +252,14 @@ __global__ __launch_bounds__(256) void corr_kernel(CorrArgs a, int n_windows, in
 
   // ---- stage P: spectrum product -------------------------------------------
   if (kx != 0) {
+    // A lanes: conj(own) * oth = conj(FA) FB; B lanes get its conjugate
+    const float sgn = sel ? -1.f : 1.f;
     static_for<0, H>([&](auto RR) {
       constexpr int r = decltype(RR)::value;
-      c32 own = x[r];
-      c32 oth = swap2(own);
-      c32 fa = sel ? oth : own;
-      c32 fb = sel ? own : oth;
-      x[r] = cconjmul(fa, fb);
+      const c32 own = x[r];
+      const c32 oth = swap2(own);
+      const c32 c = cconjmul(own, oth);
+      x[r] = c32{c.x, sgn * c.y};
     });
   } else {
     // Column slot 0 holds FFT(col0 + i colN) of the real DC and Nyquist
@@ -291,7 +306,8 @@ __global__ __launch_bounds__(256) void corr_kernel(CorrArgs a, int n_windows, in
     constexpr int nn = decltype(NN)::value;
     c32 y0 = x[bitrev(nn, H)];
     c32 y1 = x[bitrev(nn + Q, H)];
-    z[nn] = sel ? twiddle<nn, H, +1>(csub(y0, y1)) : cadd(y0, y1);
+    const c32 t = sel ? csub(y0, y1) : cadd(y0, y1);
+    z[nn] = sel ? twiddle<nn, H, +1>(t) : t;
   });
   fft_reg<Q, +1>(z);
   // lane (kx, sel, h) now holds R[4k + 2 sel + h][kx] at z[bitrev(k, Q)]
@@ -334,30 +350,29 @@ __global__ __launch_bounds__(256) void corr_kernel(CorrArgs a, int n_windows, in
     static_for<0, Q>([&](auto II) {
       constexpr int i = decltype(II)::value;
       constexpr int r = bitrev(i, Q);
-      c32 own = z[r];
-      c32 oth = swap2(own);
-      c32 lo = hi2 ? oth : own;
-      c32 hi = hi2 ? own : oth;
-      c32 wh = twiddle<i, H, +1>(hi);
-      z[r] = hi2 ? csub(lo, wh) : cadd(lo, wh);
+      const c32 own = z[r];
+      const c32 v = hi2 ? twiddle<i, H, +1>(own) : own;
+      const c32 recv = swap2(v);
+      const c32 sv = hi2 ? c32{-v.x, -v.y} : v;
+      z[r] = c32{recv.x + sv.x, recv.y + sv.y};
     });
-    // combine B (lanes g <-> g^1)
+    // combine B (lanes g <-> g^1); lanes with g & 2 carry the extra w_F^{F/4} = i
     static_for<0, Q>([&](auto II) {
       constexpr int i = decltype(II)::value;
       constexpr int r = bitrev(i, Q);
-      c32 own = z[r];
-      c32 oth = swap1(own);
-      c32 e = hi1 ? oth : own;
-      c32 o = hi1 ? own : oth;
-      c32 wo = twiddle<i, F, +1>(o);
-      if (hi2) wo = c32{-wo.y, wo.x};  // extra factor w_F^{F/4} = i
-      z[r] = hi1 ? csub(e, wo) : cadd(e, wo);
+      const c32 own = z[r];
+      const c32 t = twiddle<i, F, +1>(own);
+      const c32 ti = hi2 ? c32{-t.y, t.x} : t;
+      const c32 v = hi1 ? ti : own;
+      const c32 recv = swap1(v);
+      const c32 sv = hi1 ? c32{-v.x, -v.y} : v;
+      z[r] = c32{recv.x + sv.x, recv.y + sv.y};
     });
   }
   __syncthreads();  // R fully consumed (plane aliases it)
 
   // ---- peak search over the fftshift-ed plane -----------------------------
-  const float scale = 1.0f / (float)(F * F);
+  const float scale = 1.0f / (float)(4 * F * F);  // 4: the row split's factors of 2
   const int beta = (g == 0) ? 0 : (g == 1 ? 2 : (g == 2 ? 1 : 3));
   Best best{-INFINITY, 0x7fffffff};
   {

@@ -151,39 +151,44 @@ __device__ __forceinline__ c32 swap1(c32 v) { return dpp_c<kSwap1>(v); }
 __device__ __forceinline__ c32 swap2(c32 v) { return dpp_c<kSwap2>(v); }
 
 // ---------------------------------------------------------------------------
-// 2-lane DIT combine.  Before: lane h holds the length-F/2 transform of the
-// samples x[2m + h] (frequency k at register bitrev(k, F/2)).  After: lane h
-// holds X[k + h*F/2] at register bitrev(k, F/2).
+// Cross-lane radix-2 butterflies.  The lane that holds the odd half
+// pre-multiplies it by the twiddle, both lanes swap, and each lane finishes
+// with one signed add:  lower: own + recv,  upper: recv - own_twiddled.
 // ---------------------------------------------------------------------------
-template <int F, int DIR, int K = 0>
+template <int CTRL>
+__device__ __forceinline__ c32 xlane(c32 v) { return dpp_c<CTRL>(v); }
+
+// Before: lane h (h = upper) holds the length-F/2 transform of the samples
+// x[2m + h] (frequency k at register bitrev(k, F/2)).  After: lane h holds
+// X[k + h*F/2] at register bitrev(k, F/2).
+template <int F, int DIR, int CTRL = kSwap1>
 __device__ __forceinline__ void dit2_combine(c32 (&x)[F / 2], bool upper) {
-  if constexpr (K < F / 2) {
+  static_for<0, F / 2>([&](auto KK) {
+    constexpr int K = decltype(KK)::value;
     constexpr int r = bitrev(K, F / 2);
-    c32 own = x[r];
-    c32 oth = swap1(own);
-    c32 e = upper ? oth : own;
-    c32 o = upper ? own : oth;
-    c32 wo = twiddle<K, F, DIR>(o);
-    x[r] = upper ? csub(e, wo) : cadd(e, wo);
-    dit2_combine<F, DIR, K + 1>(x, upper);
-  }
+    const c32 own = x[r];
+    const c32 tw = twiddle<K, F, DIR>(own);
+    const c32 v = upper ? tw : own;
+    const c32 recv = xlane<CTRL>(v);
+    const c32 sv = upper ? c32{-v.x, -v.y} : v;
+    x[r] = c32{recv.x + sv.x, recv.y + sv.y};
+  });
 }
 
 // 2-lane DIF first stage.  Before: lane h holds x[n + h*F/2] at register
-// slot(n) where slot(n) = bitrev(n, F/2) (the layout a forward transform
-// leaves behind).  After: lane 0 holds y[n] = x[n] + x[n+F/2], lane 1 holds
-// y'[n] = (x[n] - x[n+F/2]) w^n, still at register bitrev(n, F/2).
-template <int F, int DIR, int N = 0>
+// bitrev(n, F/2).  After: lane 0 holds y[n] = x[n] + x[n+F/2], lane 1 holds
+// y'[n] = (x[n] - x[n+F/2]) w^n, at the same registers.
+template <int F, int DIR, int CTRL = kSwap1>
 __device__ __forceinline__ void dif2_split(c32 (&x)[F / 2], bool upper) {
-  if constexpr (N < F / 2) {
+  static_for<0, F / 2>([&](auto NN) {
+    constexpr int N = decltype(NN)::value;
     constexpr int r = bitrev(N, F / 2);
-    c32 own = x[r];
-    c32 oth = swap1(own);
-    c32 a = upper ? oth : own;  // x[n]
-    c32 b = upper ? own : oth;  // x[n + F/2]
-    x[r] = upper ? twiddle<N, F, DIR>(csub(a, b)) : cadd(a, b);
-    dif2_split<F, DIR, N + 1>(x, upper);
-  }
+    const c32 own = x[r];
+    const c32 recv = xlane<CTRL>(own);
+    // lower: x[n] + x[n+H];  upper: x[n] - x[n+H] = recv - own
+    const c32 t = upper ? c32{recv.x - own.x, recv.y - own.y} : c32{own.x + recv.x, own.y + recv.y};
+    x[r] = upper ? twiddle<N, F, DIR>(t) : t;
+  });
 }
 
 }  // namespace piv

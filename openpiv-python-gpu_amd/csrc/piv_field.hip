@@ -173,6 +173,7 @@ __global__ __launch_bounds__(256) void median_kernel(const float* u, const float
 // reset path then leaves the field unchanged, and so does this kernel.
 // ---------------------------------------------------------------------------
 constexpr int kReplaceThreads = 1024;
+constexpr int kSpringLds = 1024;  // outlier count handled by the LDS-resident solver
 
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
@@ -200,43 +201,27 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double (*scratch)[K]) 
   }
 }
 
-__global__ __launch_bounds__(kReplaceThreads) void spring_kernel(float* u, float* v, const uint8_t* flags,
-                                                                 const int32_t* count, int m, int n,
-                                                                 ReplaceWork w) {
-  const int N = m * n;
-  const int nv = *count;
-  if (nv <= 0 || nv >= N) return;
-  __shared__ int32_t scan[kReplaceThreads];
-  __shared__ double red4[16][4];
+// CG over global-memory vectors for large outlier sets, run by the 256-thread
+// workgroup after the compaction (slot map and list already written).
+template <int K>
+__device__ __forceinline__ void sum256(double (&v)[K], double (*scratch)[4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) scratch[wave][k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = scratch[0][k] + scratch[1][k] + scratch[2][k] + scratch[3][k];
+  __syncthreads();
+}
+
+__device__ void spring_cg_global(float* u, float* v, int m, int n, int nv, const ReplaceWork& w,
+                                 double (*red)[4]) {
+  constexpr int T = 256;
   const int tid = threadIdx.x;
-
-  // Deterministic compaction: outliers in row-major order.
-  const int chunk = (N + kReplaceThreads - 1) / kReplaceThreads;
-  const int b0 = tid * chunk;
-  const int b1 = min(N, b0 + chunk);
-  int local = 0;
-  for (int k = b0; k < b1; ++k) local += flags[k] ? 1 : 0;
-  scan[tid] = local;
-  __syncthreads();
-  for (int off = 1; off < kReplaceThreads; off <<= 1) {
-    int add = tid >= off ? scan[tid - off] : 0;
-    __syncthreads();
-    scan[tid] += add;
-    __syncthreads();
-  }
-  int pos = scan[tid] - local;
-  for (int k = b0; k < b1; ++k) {
-    if (flags[k]) {
-      w.slot[k] = pos;
-      w.list[pos] = k;
-      ++pos;
-    } else {
-      w.slot[k] = -1;
-    }
-  }
-  __syncthreads();
-  __threadfence_block();
-
   double* xu = w.work;
   double* xv = w.work + (size_t)nv;
   double* ru = w.work + 2 * (size_t)nv;
@@ -245,16 +230,20 @@ __global__ __launch_bounds__(kReplaceThreads) void spring_kernel(float* u, float
   double* pv = w.work + 5 * (size_t)nv;
   double* qu = w.work + 6 * (size_t)nv;
   double* qv = w.work + 7 * (size_t)nv;
-
-  // b = sum of valid in-bounds neighbours; diag = n_k.
-  double acc[4] = {0, 0, 0, 0};  // r.z (u), r.z (v), b.b (u), b.b (v)
-  for (int s = tid; s < nv; s += kReplaceThreads) {
+  auto nbrs = [&](int s, int (&ni)[4], int (&nj)[4]) {
     const int k = w.list[s];
     const int i = k / n, j = k % n;
+    ni[0] = i; nj[0] = j - 1;
+    ni[1] = i; nj[1] = j + 1;
+    ni[2] = i + 1; nj[2] = j;
+    ni[3] = i - 1; nj[3] = j;
+  };
+  double acc[4] = {0, 0, 0, 0};
+  for (int s = tid; s < nv; s += T) {
+    int ni[4], nj[4];
+    nbrs(s, ni, nj);
     double bu = 0.0, bv = 0.0;
     int nk = 0;
-    const int ni[4] = {i, i, i + 1, i - 1};
-    const int nj[4] = {j - 1, j + 1, j, j};
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       if (ni[e] < 0 || ni[e] >= m || nj[e] < 0 || nj[e] >= n) continue;
@@ -277,20 +266,16 @@ __global__ __launch_bounds__(kReplaceThreads) void spring_kernel(float* u, float
     acc[2] += bu * bu;
     acc[3] += bv * bv;
   }
-  block_sum<4>(acc, red4);
+  sum256<4>(acc, red);
   double rz_u = acc[0], rz_v = acc[1];
   const double tol2_u = 1e-26 * acc[2], tol2_v = 1e-26 * acc[3];
   bool done_u = acc[2] == 0.0, done_v = acc[3] == 0.0;
   const int max_it = nv + 200 < 20000 ? nv + 200 : 20000;
   for (int it = 0; it < max_it && !(done_u && done_v); ++it) {
-    __syncthreads();
-    // q = A p
     double pq[2] = {0, 0};
-    for (int s = tid; s < nv; s += kReplaceThreads) {
-      const int k = w.list[s];
-      const int i = k / n, j = k % n;
-      const int ni[4] = {i, i, i + 1, i - 1};
-      const int nj[4] = {j - 1, j + 1, j, j};
+    for (int s = tid; s < nv; s += T) {
+      int ni[4], nj[4];
+      nbrs(s, ni, nj);
       int nk = 0;
       double au = 0.0, av = 0.0;
 #pragma unroll
@@ -303,25 +288,24 @@ __global__ __launch_bounds__(kReplaceThreads) void spring_kernel(float* u, float
           av += pv[sl];
         }
       }
-      const double qau = nk * pu[s] - au;
-      const double qav = nk * pv[s] - av;
-      qu[s] = qau;
-      qv[s] = qav;
-      pq[0] += pu[s] * qau;
-      pq[1] += pv[s] * qav;
+      const double qa = nk * pu[s] - au, qb = nk * pv[s] - av;
+      qu[s] = qa;
+      qv[s] = qb;
+      pq[0] += pu[s] * qa;
+      pq[1] += pv[s] * qb;
     }
-    block_sum<2>(pq, reinterpret_cast<double(*)[2]>(red4));
-    const double alpha_u = (done_u || pq[0] == 0.0) ? 0.0 : rz_u / pq[0];
-    const double alpha_v = (done_v || pq[1] == 0.0) ? 0.0 : rz_v / pq[1];
-    double nr[4] = {0, 0, 0, 0};  // r.z u, r.z v, r.r u, r.r v
-    for (int s = tid; s < nv; s += kReplaceThreads) {
+    double pq4[4] = {pq[0], pq[1], 0.0, 0.0};
+    sum256<4>(pq4, red);
+    const double alpha_u = (done_u || pq4[0] == 0.0) ? 0.0 : rz_u / pq4[0];
+    const double alpha_v = (done_v || pq4[1] == 0.0) ? 0.0 : rz_v / pq4[1];
+    double nr[4] = {0, 0, 0, 0};
+    for (int s = tid; s < nv; s += T) {
       const int k = w.list[s];
       const int i = k / n, j = k % n;
       const int nk = (i > 0) + (i < m - 1) + (j > 0) + (j < n - 1);
       xu[s] += alpha_u * pu[s];
       xv[s] += alpha_v * pv[s];
-      const double r1 = ru[s] - alpha_u * qu[s];
-      const double r2 = rv[s] - alpha_v * qv[s];
+      const double r1 = ru[s] - alpha_u * qu[s], r2 = rv[s] - alpha_v * qv[s];
       ru[s] = r1;
       rv[s] = r2;
       nr[0] += r1 * r1 / nk;
@@ -329,26 +313,264 @@ __global__ __launch_bounds__(kReplaceThreads) void spring_kernel(float* u, float
       nr[2] += r1 * r1;
       nr[3] += r2 * r2;
     }
-    block_sum<4>(nr, red4);
+    sum256<4>(nr, red);
     const double beta_u = rz_u > 0.0 ? nr[0] / rz_u : 0.0;
     const double beta_v = rz_v > 0.0 ? nr[1] / rz_v : 0.0;
     rz_u = nr[0];
     rz_v = nr[1];
     done_u = done_u || nr[2] <= tol2_u;
     done_v = done_v || nr[3] <= tol2_v;
-    for (int s = tid; s < nv; s += kReplaceThreads) {
+    for (int s = tid; s < nv; s += T) {
       const int k = w.list[s];
       const int i = k / n, j = k % n;
       const int nk = (i > 0) + (i < m - 1) + (j > 0) + (j < n - 1);
       pu[s] = ru[s] / nk + beta_u * pu[s];
       pv[s] = rv[s] / nk + beta_v * pv[s];
     }
+    __syncthreads();
   }
   __syncthreads();
-  for (int s = tid; s < nv; s += kReplaceThreads) {
+  for (int s = tid; s < nv; s += T) {
     const int k = w.list[s];
     u[k] = (float)xu[s];
     v[k] = (float)xv[s];
+  }
+}
+
+// LDS-resident variant of the same solve for up to kSpringLds outliers (the
+// common case): all CG vectors and the outlier neighbour lists live in LDS, so
+// an iteration touches no global memory.  The flags are compacted by the whole
+// 256-thread workgroup with 16-byte loads; up to 256 outliers are then solved
+// by wave 0 alone (shuffle reductions, no workgroup barriers).  CG starts
+// from x0 = D^-1 b (the mean of the valid neighbours), exact for isolated
+// outliers.
+constexpr int kSpringLdsThreads = 256;
+
+struct SpringLds {
+  double xu[kSpringLds], xv[kSpringLds], ru[kSpringLds], rv[kSpringLds];
+  double pu[kSpringLds], pv[kSpringLds], qu[kSpringLds], qv[kSpringLds];
+  int16_t nb[kSpringLds][4];
+  uint8_t nk[kSpringLds];
+  int32_t scan[kSpringLdsThreads];
+  double red[4][4];
+};
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int K, int T>
+__device__ __forceinline__ void group_sum(double (&v)[K], double (*scratch)[4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  if constexpr (T > 64) {
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) scratch[wave][k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = scratch[0][k] + scratch[1][k] + scratch[2][k] + scratch[3][k];
+    __syncthreads();
+  }
+}
+
+// CG iterations over the LDS system with T active threads (64: wave 0 only).
+template <int T>
+__device__ void spring_cg(SpringLds& L, int nv) {
+  const int tid = threadIdx.x;
+  auto sync = [] {
+    if constexpr (T > 64) __syncthreads(); else wave_sync_lds();
+  };
+  double acc[4] = {0, 0, 0, 0};
+  for (int s = tid; s < nv; s += T) {
+    double au = 0.0, av = 0.0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int c = L.nb[s][e];
+      if (c >= 0) {
+        au += L.xu[c];
+        av += L.xv[c];
+      }
+    }
+    L.ru[s] = au;  // r0 = b - A x0 = sum of x0 over outlier neighbours
+    L.rv[s] = av;
+    const double inv = 1.0 / L.nk[s];
+    L.pu[s] = au * inv;
+    L.pv[s] = av * inv;
+    acc[0] += au * au * inv;
+    acc[1] += av * av * inv;
+    const double bu = L.xu[s] * L.nk[s], bv = L.xv[s] * L.nk[s];  // |b|^2 as the scale
+    acc[2] += bu * bu;
+    acc[3] += bv * bv;
+  }
+  group_sum<4, T>(acc, L.red);
+  double rz_u = acc[0], rz_v = acc[1];
+  const double tol2_u = 1e-26 * acc[2], tol2_v = 1e-26 * acc[3];
+  bool done_u = rz_u <= tol2_u, done_v = rz_v <= tol2_v;
+  const int max_it = nv + 200;
+  for (int it = 0; it < max_it && !(done_u && done_v); ++it) {
+    sync();
+    double pq[2] = {0, 0};
+    for (int s = tid; s < nv; s += T) {
+      double au = 0.0, av = 0.0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int c = L.nb[s][e];
+        if (c >= 0) {
+          au += L.pu[c];
+          av += L.pv[c];
+        }
+      }
+      const double qa = L.nk[s] * L.pu[s] - au, qb = L.nk[s] * L.pv[s] - av;
+      L.qu[s] = qa;
+      L.qv[s] = qb;
+      pq[0] += L.pu[s] * qa;
+      pq[1] += L.pv[s] * qb;
+    }
+    group_sum<2, T>(pq, L.red);
+    const double alpha_u = (done_u || pq[0] == 0.0) ? 0.0 : rz_u / pq[0];
+    const double alpha_v = (done_v || pq[1] == 0.0) ? 0.0 : rz_v / pq[1];
+    double nr[4] = {0, 0, 0, 0};
+    for (int s = tid; s < nv; s += T) {
+      L.xu[s] += alpha_u * L.pu[s];
+      L.xv[s] += alpha_v * L.pv[s];
+      const double r1 = L.ru[s] - alpha_u * L.qu[s], r2 = L.rv[s] - alpha_v * L.qv[s];
+      L.ru[s] = r1;
+      L.rv[s] = r2;
+      const double inv = 1.0 / L.nk[s];
+      nr[0] += r1 * r1 * inv;
+      nr[1] += r2 * r2 * inv;
+      nr[2] += r1 * r1;
+      nr[3] += r2 * r2;
+    }
+    group_sum<4, T>(nr, L.red);
+    const double beta_u = rz_u > 0.0 ? nr[0] / rz_u : 0.0;
+    const double beta_v = rz_v > 0.0 ? nr[1] / rz_v : 0.0;
+    rz_u = nr[0];
+    rz_v = nr[1];
+    done_u = done_u || nr[2] <= tol2_u;
+    done_v = done_v || nr[3] <= tol2_v;
+    for (int s = tid; s < nv; s += T) {
+      const double inv = 1.0 / L.nk[s];
+      L.pu[s] = L.ru[s] * inv + beta_u * L.pu[s];
+      L.pv[s] = L.rv[s] * inv + beta_v * L.pv[s];
+    }
+  }
+  sync();
+}
+
+__global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(float* u, float* v, const uint8_t* flags,
+                                                                       const int32_t* count, int m, int n,
+                                                                       ReplaceWork w) {
+  constexpr int T = kSpringLdsThreads;
+  const int N = m * n;
+  const int nv = *count;
+  if (nv <= 0 || nv >= N) return;
+  __shared__ SpringLds L;
+  const int tid = threadIdx.x;
+
+  // row-major compaction: thread tid owns a contiguous run of 16-flag chunks
+  const int nchunk = (N + 15) / 16;
+  const int per = (nchunk + T - 1) / T;
+  const int c0 = tid * per, c1 = min(nchunk, c0 + per);
+  const uint4* f4 = reinterpret_cast<const uint4*>(flags);
+  int local = 0;
+  for (int c = c0; c < c1; ++c) {
+    if (16 * c + 16 <= N) {
+      const uint4 q = f4[c];
+      local += __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);  // flags are 0/1 bytes
+    } else {
+      for (int k = 16 * c; k < N; ++k) local += flags[k] ? 1 : 0;
+    }
+  }
+  L.scan[tid] = local;
+  __syncthreads();
+  for (int off = 1; off < T; off <<= 1) {
+    const int add = tid >= off ? L.scan[tid - off] : 0;
+    __syncthreads();
+    L.scan[tid] += add;
+    __syncthreads();
+  }
+  int pos = L.scan[tid] - local;
+  for (int c = c0; c < c1; ++c) {
+    uint32_t words[4] = {0, 0, 0, 0};
+    if (16 * c + 16 <= N) {
+      const uint4 q = f4[c];
+      words[0] = q.x;
+      words[1] = q.y;
+      words[2] = q.z;
+      words[3] = q.w;
+    } else {
+      for (int k = 16 * c; k < N; ++k) words[(k - 16 * c) >> 2] |= (uint32_t)flags[k] << (8 * ((k - 16 * c) & 3));
+    }
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      const int k = 16 * c + b;
+      if (k < N) {
+        if ((words[b >> 2] >> (8 * (b & 3))) & 0xff) {
+          w.slot[k] = pos;
+          w.list[pos] = k;
+          ++pos;
+        } else {
+          w.slot[k] = -1;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (nv > kSpringLds) {
+    spring_cg_global(u, v, m, n, nv, w, L.red);
+    return;
+  }
+
+  // neighbour codes (slot, -1 valid, -2 outside) and x0 = b / n_k
+  for (int s = tid; s < nv; s += T) {
+    const int k = w.list[s];
+    const int i = k / n, j = k % n;
+    const int ni[4] = {i, i, i + 1, i - 1};
+    const int nj[4] = {j - 1, j + 1, j, j};
+    double bu = 0.0, bv = 0.0;
+    int cnt = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      int16_t code = -2;
+      if (ni[e] >= 0 && ni[e] < m && nj[e] >= 0 && nj[e] < n) {
+        ++cnt;
+        const int l = ni[e] * n + nj[e];
+        const int sl = w.slot[l];
+        if (sl >= 0) {
+          code = (int16_t)sl;
+        } else {
+          code = -1;
+          bu += (double)u[l];
+          bv += (double)v[l];
+        }
+      }
+      L.nb[s][e] = code;
+    }
+    L.nk[s] = (uint8_t)cnt;
+    L.xu[s] = bu / cnt;
+    L.xv[s] = bv / cnt;
+  }
+  __syncthreads();
+  if (nv <= 256) {
+    if (tid >= 64) return;  // wave 0 finishes alone
+    spring_cg<64>(L, nv);
+    for (int s = tid; s < nv; s += 64) {
+      const int k = w.list[s];
+      u[k] = (float)L.xu[s];
+      v[k] = (float)L.xv[s];
+    }
+  } else {
+    spring_cg<T>(L, nv);
+    for (int s = tid; s < nv; s += T) {
+      const int k = w.list[s];
+      u[k] = (float)L.xu[s];
+      v[k] = (float)L.xv[s];
+    }
   }
 }
 
@@ -410,7 +632,9 @@ hipError_t launch_median_validate(const float* u, const float* v, int m, int n, 
 
 hipError_t launch_spring_replace(float* u, float* v, const uint8_t* flags, const int32_t* count, int m, int n,
                                  const ReplaceWork& w, hipStream_t stream) {
-  hipLaunchKernelGGL(spring_kernel, dim3(1), dim3(kReplaceThreads), 0, stream, u, v, flags, count, m, n, w);
+  // the kernel reads the outlier count on the device (graph-capturable, no
+  // host sync) and returns at once when there is nothing to solve
+  hipLaunchKernelGGL(spring_lds_kernel, dim3(1), dim3(kSpringLdsThreads), 0, stream, u, v, flags, count, m, n, w);
   return hipGetLastError();
 }
 

@@ -89,7 +89,10 @@ struct SmoothnWork {
   float* t0;            // 2 * m*n
   float* t1;            // 2 * m*n
   double* p_out;        // chosen log10(s)
+  void* state;          // smoothn_state_bytes() of device scratch; its
+                        // counter word must start (and stays) zero
 };
+size_t smoothn_state_bytes();
 hipError_t launch_smoothn(float* u, float* v, int m, int n, double p_min, double p_max, double fixed_p,
                           int auto_s, const SmoothnWork& w, hipStream_t stream);
 
