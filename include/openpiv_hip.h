@@ -147,6 +147,18 @@ int piv_stage_spring_replace(float *u, float *v, const uint8_t *flags, const int
  * Replaces SmoothnGPU.__call__ (gpu_smoothn.py:123-231, 284-372). */
 int piv_stage_smoothn(float *u, float *v, int m, int n, double smoothing_par, double *log10_s_out, void *stream);
 
+/* Precision mode: correlation of window stacks by dense DFT matrix
+ * products on the MFMA units (FP16/BF16 inputs, FP32 accumulation), beside
+ * the FP32 FFT path of piv_stage_correlate.  win_a/win_b: n x ws x ws FP32
+ * windows (device).  precision: 0 FP16 split hi/lo, 1 BF16 split hi/lo,
+ * 2 FP16, 3 BF16.  Same normalisation, zero padding, fftshift, peak and
+ * sub-pixel rules as gpu_process.py:1065-1310.  Allocates its workspace;
+ * synchronizes the stream.  No counterpart in the reference (its only
+ * correlation is the FP32/FP64 cuFFT path, gpu_process.py:1237-1246). */
+int piv_stage_correlate_dft(const float *win_a, const float *win_b, int n_windows, int window_size, int fft_size,
+                            int precision, int subpixel, float *u, float *v, int32_t *i_peak, int32_t *j_peak,
+                            float *planes, void *stream);
+
 /* ---- device memory, streams and events --------------------------------
  * The reference allocates and moves data with CuPy (cp.asarray / cp.empty /
  * .get(), gpu_process.py:582-583, 1127, 316).  These entry points give a

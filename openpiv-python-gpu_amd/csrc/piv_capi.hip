@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -544,6 +545,76 @@ int piv_stage_smoothn(float* u, float* v, int m, int n, double smoothing_par, do
   free_tables(&tab);
   if (e != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e));
   if (e2 != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e2));
+  return PIV_OK;
+}
+
+int piv_stage_correlate_dft(const float* win_a, const float* win_b, int n_windows, int window_size, int fft_size,
+                            int precision, int subpixel, float* u, float* v, int32_t* i_peak, int32_t* j_peak,
+                            float* planes, void* stream) {
+  if (!win_a || !win_b || !u || !v || n_windows < 0) return fail(PIV_E_INVALID, "bad argument");
+  const int W = window_size, F = fft_size;
+  if (W < 16 || W % 16 || !is_pow2(F) || F < W || (F - W) % 2 || F > 256)
+    return fail(PIV_E_INVALID, "window_size must be a multiple of 16 and fft_size a power of two >= it");
+  if (precision < 0 || precision > 3 || subpixel < 0 || subpixel > 2) return fail(PIV_E_INVALID, "mode");
+  if (n_windows == 0) return PIV_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int pad = (F - W) / 2;
+  std::vector<float> er((size_t)F * W), ei((size_t)F * W), gr((size_t)F * F), gi((size_t)F * F);
+  for (int k = 0; k < F; ++k) {
+    for (int n = 0; n < W; ++n) {
+      const double a = -2.0 * M_PI * (double)((long)k * (n + pad) % F) / F;
+      er[(size_t)k * W + n] = (float)cos(a);
+      ei[(size_t)k * W + n] = (float)sin(a);
+    }
+    for (int x = 0; x < F; ++x) {
+      const double a = 2.0 * M_PI * (double)((long)k * x % F) / F;
+      gr[(size_t)x * F + k] = (float)cos(a);
+      gi[(size_t)x * F + k] = (float)sin(a);
+    }
+  }
+  // process in chunks that keep the workspace near 1 GiB
+  const size_t per = dft_workspace_bytes(1, W, F);
+  const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)n_windows, ((size_t)1 << 30) / per));
+  float *d_er = nullptr, *d_ei = nullptr, *d_gr = nullptr, *d_gi = nullptr, *work = nullptr;
+  auto cleanup = [&] {
+    for (void* p : {(void*)d_er, (void*)d_ei, (void*)d_gr, (void*)d_gi, (void*)work})
+      if (p) (void)hipFree(p);
+  };
+  if (hipMalloc(&d_er, er.size() * 4) != hipSuccess || hipMalloc(&d_ei, ei.size() * 4) != hipSuccess ||
+      hipMalloc(&d_gr, gr.size() * 4) != hipSuccess || hipMalloc(&d_gi, gi.size() * 4) != hipSuccess ||
+      hipMalloc((void**)&work, per * chunk) != hipSuccess) {
+    cleanup();
+    return fail(PIV_E_NOMEM, "DFT workspace allocation failed");
+  }
+  hipError_t e = hipMemcpy(d_er, er.data(), er.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_ei, ei.data(), ei.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_gr, gr.data(), gr.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_gi, gi.data(), gi.size() * 4, hipMemcpyHostToDevice);
+  for (int b0 = 0; b0 < n_windows && e == hipSuccess; b0 += chunk) {
+    const int nb = std::min(chunk, n_windows - b0);
+    DftArgs d{};
+    d.win_a = win_a + (size_t)b0 * W * W;
+    d.win_b = win_b + (size_t)b0 * W * W;
+    d.n = nb;
+    d.ws = W;
+    d.fsize = F;
+    d.precision = precision;
+    d.subpixel = subpixel;
+    d.er = d_er;
+    d.ei = d_ei;
+    d.gr = d_gr;
+    d.gi = d_gi;
+    d.workspace = work;
+    d.u = u + b0;
+    d.v = v + b0;
+    d.i_peak = i_peak ? i_peak + b0 : nullptr;
+    d.j_peak = j_peak ? j_peak + b0 : nullptr;
+    d.planes = planes ? planes + (size_t)b0 * F * F : nullptr;
+    e = launch_dft_correlate(d, s);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  cleanup();
+  if (e != hipSuccess) return fail(PIV_E_HIP, "DFT correlation: %s", hipGetErrorString(e));
   return PIV_OK;
 }
 

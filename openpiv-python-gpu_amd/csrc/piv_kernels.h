@@ -96,6 +96,27 @@ size_t smoothn_state_bytes();
 hipError_t launch_smoothn(float* u, float* v, int m, int n, double p_min, double p_max, double fixed_p,
                           int auto_s, const SmoothnWork& w, hipStream_t stream);
 
+// Window correlation by DFT matrix products on MFMA (precision mode).
+struct DftArgs {
+  const float* win_a;   // n x ws x ws windows (FP32)
+  const float* win_b;
+  int n, ws, fsize;
+  int precision;        // 0 fp16 split, 1 bf16 split, 2 fp16, 3 bf16
+  int subpixel;
+  const float* er;      // E = D_F[:, pad:pad+ws]: F x ws, real and imaginary parts
+  const float* ei;
+  const float* gr;      // G[x][k] = exp(+2 pi i k x / F): F x F
+  const float* gi;
+  float* workspace;     // dft_workspace_bytes(n, ws, fsize)
+  float* u;
+  float* v;
+  int32_t* i_peak;
+  int32_t* j_peak;
+  float* planes;        // optional n x F x F fftshift-ed planes
+};
+size_t dft_workspace_bytes(int n, int ws, int fsize);
+hipError_t launch_dft_correlate(const DftArgs& d, hipStream_t stream);
+
 // Text describing the last failing launcher call on this thread ("" when the
 // failure came from the kernel launch itself).
 const char* launch_detail();

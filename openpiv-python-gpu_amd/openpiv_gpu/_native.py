@@ -26,7 +26,7 @@ EXPORTED_SYMBOLS = (
     "piv_stage_grid_interp", "piv_stage_median_validate", "piv_stage_spring_replace", "piv_stage_smoothn",
     "piv_set_device", "piv_malloc", "piv_free", "piv_memcpy", "piv_memset", "piv_synchronize",
     "piv_stream_create", "piv_stream_destroy", "piv_stream_synchronize", "piv_event_create", "piv_event_destroy",
-    "piv_event_record", "piv_event_elapsed_ms",
+    "piv_event_record", "piv_event_elapsed_ms", "piv_stage_correlate_dft",
 )
 
 
@@ -86,6 +86,7 @@ def _bind(lib):
         "piv_stage_median_validate": (i32, [vp, vp, i32, i32, f32, vp, vp, vp]),
         "piv_stage_spring_replace": (i32, [vp, vp, vp, vp, i32, i32, vp]),
         "piv_stage_smoothn": (i32, [vp, vp, i32, i32, f64, vp, vp]),
+        "piv_stage_correlate_dft": (i32, [vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
         "piv_set_device": (i32, [i32]),
         "piv_malloc": (i32, [ctypes.POINTER(vp), ctypes.c_size_t]),
         "piv_free": (i32, [vp]),

@@ -141,3 +141,35 @@ def fit_sign_ambiguous(planes, i_peak, j_peak, rel=1e-5):
             vals = np.array([c, planes[k, i - 1, j], planes[k, i + 1, j], planes[k, i, j - 1], planes[k, i, j + 1]])
             out[k] = np.any(np.abs(vals) <= rel * max(abs(c), 1e-30))
     return out
+
+
+def stack_geometry(n, ws):
+    """Geometry that reads an (n, ws, ws) window stack laid out as an
+    (n*ws, ws) frame: window k at rows k*ws .. k*ws+ws-1."""
+    g = O.Geometry((n * ws, ws), ws, 0, ws, (n, 1), (0, 0), np.zeros((n, 1), np.float32),
+                   np.zeros((n, 1), np.float32))
+    return g
+
+
+def stack_correlate_fft(wa, wb, fsize):
+    n, ws, _ = wa.shape
+    g = stack_geometry(n, ws)
+    out = stage_correlate(wa.reshape(n * ws, ws), wb.reshape(n * ws, ws), g, fsize)
+    return {k: v.reshape(-1) for k, v in out.items()}
+
+
+def stack_correlate_dft(wa, wb, fsize, precision, subpixel=0, planes=False):
+    lib = _native.load_library()
+    n, ws, _ = wa.shape
+    da, db = dev(np.ascontiguousarray(wa, np.float32)), dev(np.ascontiguousarray(wb, np.float32))
+    u = empty((n,), np.float32)
+    v = empty((n,), np.float32)
+    ip = empty((n,), np.int32)
+    jp = empty((n,), np.int32)
+    pl = empty((n, fsize, fsize), np.float32) if planes else None
+    _native.check(lib.piv_stage_correlate_dft(vp(da), vp(db), n, ws, fsize, precision, subpixel, vp(u), vp(v),
+                                              vp(ip), vp(jp), vp(pl), None))
+    out = {"u": host(u), "v": host(v), "i_peak": host(ip), "j_peak": host(jp)}
+    if planes:
+        out["corr"] = host(pl)
+    return out
