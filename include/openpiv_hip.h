@@ -159,6 +159,40 @@ int piv_stage_correlate_dft(const float *win_a, const float *win_b, int n_window
                             int precision, int subpixel, float *u, float *v, int32_t *i_peak, int32_t *j_peak,
                             float *planes, void *stream);
 
+/* ---- synthetic particle-image pairs ------------------------------------
+ * No counterpart in the reference (it ships no generator or tests).  The
+ * device twin of openpiv_gpu/synthetic.py's particle model, used to build
+ * training and evaluation streams without a host bottleneck: particles
+ * uniform over the frame plus `margin` px, Gaussian blobs point-sampled at
+ * pixel centres out to 4 sigma, frame A at p - d(p)/2 and frame B at
+ * p + d(p)/2, additive Gaussian noise, rint, clip to [0, max_value].
+ * Random numbers come from a counter-based hash of (seed, stream, index).
+ * accum: 2 * height * width int32 device workspace, zero on entry (the call
+ * leaves it zeroed).  particles: optional n_particles x 5 FP32 device
+ * output (x, y, I0, du, dv). */
+#define PIV_SYNTH_MAX_VORTICES 8
+enum piv_field_kind { PIV_FIELD_UNIFORM = 0, PIV_FIELD_SINE = 1, PIV_FIELD_VORTICES = 2 };
+
+typedef struct piv_synth_field {
+  int32_t kind;               /* enum piv_field_kind                          */
+  int32_t n_vortices;         /* <= PIV_SYNTH_MAX_VORTICES                    */
+  double u0, v0;              /* uniform: (u, v)                              */
+  double amp_u, amp_v, period;/* sine: u = amp_u sin(2 pi y / period), v = amp_v cos(2 pi x / period) */
+  double gamma;               /* vortices: shear u += gamma (y - height / 2)  */
+  double cx[PIV_SYNTH_MAX_VORTICES], cy[PIV_SYNTH_MAX_VORTICES];
+  double rc[PIV_SYNTH_MAX_VORTICES], circ[PIV_SYNTH_MAX_VORTICES];  /* Lamb-Oseen core radius, circulation */
+} piv_synth_field;
+
+typedef struct piv_synth_model {
+  double sigma, i0_low, i0_high, noise_sigma, max_value;
+  int32_t out_dtype;          /* enum piv_in_dtype                            */
+  int32_t margin;             /* px beyond each frame edge                    */
+  int64_t n_particles;
+} piv_synth_model;
+
+int piv_synth_pair(const piv_synth_field *field, const piv_synth_model *model, uint64_t seed, int height, int width,
+                   void *frame_a, void *frame_b, int32_t *accum, float *particles, void *stream);
+
 /* ---- device memory, streams and events --------------------------------
  * The reference allocates and moves data with CuPy (cp.asarray / cp.empty /
  * .get(), gpu_process.py:582-583, 1127, 316).  These entry points give a

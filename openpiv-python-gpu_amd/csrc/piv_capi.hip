@@ -14,10 +14,11 @@
 using namespace piv;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const char* fmt, ...) {
+namespace piv {
+int capi_fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -26,6 +27,11 @@ int fail(int code, const char* fmt, ...) {
   g_err = buf;
   return code;
 }
+}  // namespace piv
+
+namespace {
+
+#define fail piv::capi_fail
 
 #define HIP_TRY(expr)                                                                               \
   do {                                                                                              \

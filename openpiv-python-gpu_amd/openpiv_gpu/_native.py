@@ -26,8 +26,11 @@ EXPORTED_SYMBOLS = (
     "piv_stage_grid_interp", "piv_stage_median_validate", "piv_stage_spring_replace", "piv_stage_smoothn",
     "piv_set_device", "piv_malloc", "piv_free", "piv_memcpy", "piv_memset", "piv_synchronize",
     "piv_stream_create", "piv_stream_destroy", "piv_stream_synchronize", "piv_event_create", "piv_event_destroy",
-    "piv_event_record", "piv_event_elapsed_ms", "piv_stage_correlate_dft",
+    "piv_event_record", "piv_event_elapsed_ms", "piv_stage_correlate_dft", "piv_synth_pair",
 )
+
+SYNTH_MAX_VORTICES = 8
+FIELD_UNIFORM, FIELD_SINE, FIELD_VORTICES = 0, 1, 2
 
 
 class PassDesc(ctypes.Structure):
@@ -62,6 +65,36 @@ class PlanDesc(ctypes.Structure):
     ]
 
 
+class SynthField(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("n_vortices", ctypes.c_int32),
+        ("u0", ctypes.c_double),
+        ("v0", ctypes.c_double),
+        ("amp_u", ctypes.c_double),
+        ("amp_v", ctypes.c_double),
+        ("period", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("cx", ctypes.c_double * SYNTH_MAX_VORTICES),
+        ("cy", ctypes.c_double * SYNTH_MAX_VORTICES),
+        ("rc", ctypes.c_double * SYNTH_MAX_VORTICES),
+        ("circ", ctypes.c_double * SYNTH_MAX_VORTICES),
+    ]
+
+
+class SynthModel(ctypes.Structure):
+    _fields_ = [
+        ("sigma", ctypes.c_double),
+        ("i0_low", ctypes.c_double),
+        ("i0_high", ctypes.c_double),
+        ("noise_sigma", ctypes.c_double),
+        ("max_value", ctypes.c_double),
+        ("out_dtype", ctypes.c_int32),
+        ("margin", ctypes.c_int32),
+        ("n_particles", ctypes.c_int64),
+    ]
+
+
 class NativeError(RuntimeError):
     pass
 
@@ -87,6 +120,8 @@ def _bind(lib):
         "piv_stage_spring_replace": (i32, [vp, vp, vp, vp, i32, i32, vp]),
         "piv_stage_smoothn": (i32, [vp, vp, i32, i32, f64, vp, vp]),
         "piv_stage_correlate_dft": (i32, [vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
+        "piv_synth_pair": (i32, [ctypes.POINTER(SynthField), ctypes.POINTER(SynthModel), ctypes.c_uint64, i32, i32,
+                                 vp, vp, vp, vp, vp]),
         "piv_set_device": (i32, [i32]),
         "piv_malloc": (i32, [ctypes.POINTER(vp), ctypes.c_size_t]),
         "piv_free": (i32, [vp]),

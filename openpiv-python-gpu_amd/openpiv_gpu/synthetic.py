@@ -33,6 +33,7 @@ def uniform_displacement(u: float, v: float) -> DisplacementFn:
     def fn(x, y):
         return np.full_like(x, u, dtype=np.float64), np.full_like(y, v, dtype=np.float64)
     fn.max_abs = max(abs(u), abs(v))
+    fn.params = {"kind": "uniform", "u0": u, "v0": v}
     return fn
 
 
@@ -42,6 +43,7 @@ def sine_displacement(amp_u: float = 4.0, amp_v: float = 3.0, period: float = 25
         return (amp_u * np.sin(2.0 * np.pi * y / period),
                 amp_v * np.cos(2.0 * np.pi * x / period))
     fn.max_abs = max(abs(amp_u), abs(amp_v))
+    fn.params = {"kind": "sine", "amp_u": amp_u, "amp_v": amp_v, "period": period}
     return fn
 
 
@@ -74,6 +76,7 @@ def lamb_oseen_displacement(rng: np.random.Generator, shape: Tuple[int, int]) ->
             v = v + ut * dx
         return u, v
     fn.max_abs = float(np.max(np.abs(peak))) + 0.02 * max(ht, wd) / 2.0
+    fn.params = {"kind": "vortices", "gamma": gamma, "cx": cx, "cy": cy, "rc": rc, "circ": circ}
     return fn
 
 
@@ -222,3 +225,91 @@ def window_pairs(ws: int, n: int, seed: int, chunk: int = 1024, workers: int = 1
         for job in jobs:
             run(job)
     return wa, wb, shifts
+
+
+# Device renderer ------------------------------------------------------------
+
+def particle_budget(shape, disp, model: ParticleModel):
+    """``(margin, n_particles)`` of :func:`make_pair` for this field and model."""
+    ht, wd = shape
+    margin = int(math.ceil(getattr(disp, "max_abs", 0.0))) + int(math.ceil(4.0 * model.sigma)) + 1
+    return margin, int(round(model.density * (ht + 2 * margin) * (wd + 2 * margin)))
+
+
+def synth_field(disp):
+    """C descriptor (``piv_synth_field``) of a displacement made by this module."""
+    from . import _native
+    p = disp.params
+    f = _native.SynthField()
+    if p["kind"] == "uniform":
+        f.kind, f.u0, f.v0 = _native.FIELD_UNIFORM, p["u0"], p["v0"]
+    elif p["kind"] == "sine":
+        f.kind, f.amp_u, f.amp_v, f.period = _native.FIELD_SINE, p["amp_u"], p["amp_v"], p["period"]
+    else:
+        k = len(p["cx"])
+        if k > _native.SYNTH_MAX_VORTICES:
+            raise ValueError("too many vortices")
+        f.kind, f.n_vortices, f.gamma = _native.FIELD_VORTICES, k, p["gamma"]
+        for i in range(k):
+            f.cx[i], f.cy[i], f.rc[i], f.circ[i] = p["cx"][i], p["cy"][i], p["rc"][i], p["circ"][i]
+    return f
+
+
+class DevicePairGenerator:
+    """Renders seeded pairs of one frame shape directly into device buffers.
+
+    Same particle model, margin, particle count, placement, noise, rounding
+    and clipping as :func:`make_pair`; particles and noise come from a
+    counter-based hash of ``seed`` instead of the host PCG64 stream, so a
+    seed gives a different (equally distributed) realisation than
+    :func:`make_pair`, identical on every run.
+    """
+
+    def __init__(self, shape, model: ParticleModel = None):
+        from . import _native
+        self._native = _native
+        self.shape = tuple(int(t) for t in shape)
+        self.model = model or ParticleModel()
+        ht, wd = self.shape
+        self.accum = _native.DeviceBuffer.zeros((2, ht, wd), np.int32)
+        self.dtype = np.dtype(self.model.dtype)
+
+    def new_frames(self):
+        d = self._native.DeviceBuffer
+        return d(self.shape, self.dtype), d(self.shape, self.dtype)
+
+    def render(self, disp, seed: int, frame_a=None, frame_b=None, particles=None, stream=None):
+        """Renders one pair; returns the two device buffers (allocated when
+        not given).  ``particles``: optional device buffer (n, 5) FP32 that
+        receives x, y, I0, du, dv of every particle."""
+        import ctypes
+        nat = self._native
+        if frame_a is None or frame_b is None:
+            frame_a, frame_b = self.new_frames()
+        margin, n = particle_budget(self.shape, disp, self.model)
+        m = nat.SynthModel()
+        m.sigma, m.i0_low, m.i0_high = self.model.sigma, self.model.i0_low, self.model.i0_high
+        m.noise_sigma, m.max_value = self.model.noise_sigma, self.model.max_value
+        m.out_dtype, m.margin, m.n_particles = nat.in_dtype_code(self.dtype), margin, n
+        f = synth_field(disp)
+        ht, wd = self.shape
+        nat.check(nat.load_library().piv_synth_pair(
+            ctypes.byref(f), ctypes.byref(m), ctypes.c_uint64(seed & 0xFFFFFFFFFFFFFFFF), ht, wd, frame_a.value,
+            frame_b.value, self.accum.value, None if particles is None else particles.value,
+            None if stream is None else stream.handle))
+        return frame_a, frame_b
+
+    @staticmethod
+    def n_particles(shape, disp, model: ParticleModel):
+        return particle_budget(shape, disp, model)[1]
+
+
+def vortex_field(seed: int, shape):
+    """The uneven-batch displacement field of seed ``seed`` (1-4 Lamb-Oseen
+    vortices + shear), drawn exactly as :func:`cfg_vortex_uint16` draws it."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return lamb_oseen_displacement(rng, shape)
+
+
+MODEL_12BIT = ParticleModel(sigma=0.65, density=0.05, i0_low=1000.0, i0_high=4095.0, noise_sigma=20.0,
+                            max_value=4095.0, dtype=np.uint16)

@@ -35,19 +35,28 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
+STRUCTS = (("piv_pass_desc", "PassDesc"), ("piv_plan_desc", "PlanDesc"), ("piv_synth_field", "SynthField"),
+           ("piv_synth_model", "SynthModel"))
+
+
 def test_struct_layout_matches_c(tmp_path):
+    """sizeof and every field offset of each C struct equal the ctypes mirror."""
+    import ctypes
+    lines, expect = [], []
+    for cname, pyname in STRUCTS:
+        cls = getattr(_native, pyname)
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        expect.append(ctypes.sizeof(cls))
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {fname}));')
+            expect.append(getattr(cls, fname).offset)
     src = tmp_path / "layout.c"
-    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "openpiv_hip.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu\\n", sizeof(piv_pass_desc), sizeof(piv_plan_desc),'
-                   ' offsetof(piv_pass_desc, smoothing_par), offsetof(piv_plan_desc, out_scale),'
-                   ' offsetof(piv_plan_desc, use_graph)); return 0;}\n')
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "openpiv_hip.h"\nint main(void){'
+                   + "".join(lines) + "return 0;}\n")
     exe = tmp_path / "layout"
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
     got = [int(t) for t in subprocess.check_output([str(exe)]).split()]
-    import ctypes
-    assert got == [ctypes.sizeof(_native.PassDesc), ctypes.sizeof(_native.PlanDesc),
-                   _native.PassDesc.smoothing_par.offset, _native.PlanDesc.out_scale.offset,
-                   _native.PlanDesc.use_graph.offset]
+    assert got == expect
 
 
 def test_no_device_means_loud_failure():
