@@ -159,6 +159,53 @@ int piv_stage_correlate_dft(const float *win_a, const float *win_b, int n_window
                             int precision, int subpixel, float *u, float *v, int32_t *i_peak, int32_t *j_peak,
                             float *planes, void *stream);
 
+/* ---- learned coarse-displacement predictor -----------------------------
+ * No counterpart in the reference: there the deformation predictor of a
+ * pass is always the previous pass's field (gpu_process.py:620-625).  A
+ * small encoder-decoder CNN maps a pair, box-averaged 8 x 8 and standardised
+ * (piv_cnn_prepare_input), to a displacement field on nodes
+ * (16 j + 8, 16 i + 8) of the frame; it can stand in for the coarse passes
+ * of a plan (piv_plan_set_predictor).  Training and inference run on the
+ * library's own kernels: BF16 weights and activations, FP32 accumulation,
+ * FP32 gradients, Adam.  in_height/in_width: network input = frame / 8
+ * (multiples of 8).  All calls on a network are ordered on the given stream;
+ * the get_* calls synchronise the device. */
+typedef struct piv_cnn piv_cnn;
+int piv_cnn_create(int in_height, int in_width, int max_batch, piv_cnn **out);
+void piv_cnn_destroy(piv_cnn *net);
+/* desc: n_layers x 8 int32 (source buffer 0, source buffer 1 or -1, x2
+ * upsampling of source 0, input channels from source 0 and 1, output
+ * channels, stride, ReLU); offsets: n_layers x 2 (weight [co][3][3][ci] and
+ * bias offsets in the flat FP32 parameter vector).  Buffer 0 is the input,
+ * buffer l+1 the output of layer l. */
+int piv_cnn_layout(const piv_cnn *net, int32_t *n_layers, int32_t *desc, int64_t *offsets, int64_t *n_params);
+int piv_cnn_set_params(piv_cnn *net, const float *host_params);   /* also resets Adam */
+int piv_cnn_get_params(const piv_cnn *net, float *host_params);
+int piv_cnn_get_grads(const piv_cnn *net, float *host_grads);
+/* activation (gradient = 0, BF16 widened; the last buffer is the FP32
+ * planar output) or its gradient (NHWC FP32) of buffer 1..n_layers */
+int piv_cnn_get_buffer(const piv_cnn *net, int buffer, int gradient, int batch, float *host_out);
+/* input: device BF16 [batch][in_h][in_w][2]; out: device FP32
+ * [batch][2][in_h/2][in_w/2] (u then v) or NULL */
+int piv_cnn_forward(piv_cnn *net, const uint16_t *input, int batch, float *out, void *stream);
+/* forward + weighted mean-square loss + backward; target: device FP32
+ * [batch][2][in_h/2][in_w/2]; weight: device FP32 [batch][in_h/2][in_w/2]
+ * or NULL; loss: device FP32 scalar or NULL.  Gradients stay in the net. */
+int piv_cnn_gradients(piv_cnn *net, const uint16_t *input, const float *target, const float *weight, int batch,
+                      float *loss, void *stream);
+int piv_cnn_adam_step(piv_cnn *net, float lr, float beta1, float beta2, float eps, void *stream);
+int piv_cnn_prepare_input(piv_cnn *net, const void *frame_a, const void *frame_b, int in_dtype, int height, int width,
+                          uint16_t *out, void *stream);
+
+/* Learned predictor for a plan: piv_run then evaluates the network on the
+ * pair, interpolates its field to the last pass's grid as that pass's
+ * deformation predictor and runs the last pass only.  When the last pass's
+ * median test rejects more than max_reject_fraction of the vectors, the same
+ * call runs the plan's full schedule instead (fallback).  net = NULL
+ * removes the predictor.  The frame must be 8 x (network input). */
+int piv_plan_set_predictor(piv_plan *plan, piv_cnn *net, float max_reject_fraction);
+int piv_plan_last_fallback(const piv_plan *plan, int32_t *fell_back, int32_t *rejected);
+
 /* ---- synthetic particle-image pairs ------------------------------------
  * No counterpart in the reference (it ships no generator or tests).  The
  * device twin of openpiv_gpu/synthetic.py's particle model, used to build

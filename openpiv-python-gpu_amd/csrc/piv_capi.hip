@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/openpiv_hip.h"
+#include "piv_cnn.h"
 #include "piv_kernels.h"
 
 using namespace piv;
@@ -232,9 +233,67 @@ struct piv_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int runs = 0;
+  // learned predictor (piv_plan_set_predictor)
+  piv_cnn* net = nullptr;
+  float max_reject = 0.05f;
+  piv_pass_desc net_grid{};     // the network's output nodes as a pass grid
+  uint16_t* net_in = nullptr;   // network input of the current pair
+  void* net_scratch = nullptr;
+  hipGraph_t lgraph = nullptr;
+  hipGraphExec_t lgraph_exec = nullptr;
+  int lruns = 0;
+  int32_t last_fallback = 0;
+  int32_t last_rejected = 0;
 };
 
 namespace {
+
+// Pass k > 0 (or the learned path's single pass): deform the original
+// frames by the total predictor B.up/B.vp (gpu_process.py:1113-1114, Q13)
+// and correlate the residual, adding the predictor back (:628-632).
+int deform_and_correlate(piv_plan* P, PassBuffers& B, hipStream_t s) {
+  const piv_plan_desc& D = P->desc;
+  HIP_TRY(launch_deform(
+      deform_args(P->frame_a, P->frame_b, D.in_dtype, D.height, D.width, B.d, B.up, B.vp, P->def_a, P->def_b), s));
+  CorrArgs a = corr_args(P->def_a, P->def_b, kF32, D.height, D.width, B.d);
+  a.up = B.up;
+  a.vp = B.vp;
+  a.u = B.u;
+  a.v = B.v;
+  HIP_TRY(launch_correlate(a, s));
+  return PIV_OK;
+}
+
+// Validation, replacement, last-pass revalidation and smoothing of pass k
+// (gpu_process.py:635-639, 744-845).
+int validate_and_smooth(piv_plan* P, int k, bool last, hipStream_t s) {
+  PassBuffers& B = P->passes[k];
+  if (B.d.validate) {
+    HIP_TRY(launch_median_validate(B.u, B.v, B.d.field_m, B.d.field_n, B.d.median_tol, B.flags, P->counters + 2 * k,
+                                   s));
+    if (B.d.num_replacing_iters > 0) {
+      HIP_TRY(launch_spring_replace(B.u, B.v, B.flags, P->counters + 2 * k, B.d.field_m, B.d.field_n, P->rw, s));
+    }
+    if (last) {
+      HIP_TRY(launch_median_validate(B.u, B.v, B.d.field_m, B.d.field_n, B.d.median_tol, P->val_final,
+                                     P->counters + 2 * k + 1, s));
+    }
+  }
+  if (!last && B.d.smooth) {
+    SmoothnWork w{};
+    w.cos_m = B.tab.cos_m;
+    w.cos_n = B.tab.cos_n;
+    w.lambda = B.tab.lambda;
+    w.t0 = P->t0;
+    w.t1 = P->t1;
+    w.p_out = P->p_out + k;
+    w.state = P->smooth_state;
+    const int autos = !(B.d.smoothing_par > 0);
+    HIP_TRY(launch_smoothn(B.u, B.v, B.d.field_m, B.d.field_n, B.p_min, B.p_max,
+                           autos ? 0.0 : log10(B.d.smoothing_par), autos, w, s));
+  }
+  return PIV_OK;
+}
 
 int enqueue(piv_plan* P) {
   hipStream_t s = P->stream;
@@ -258,42 +317,11 @@ int enqueue(piv_plan* P) {
         HIP_TRY(launch_copy_f32(B.up, A.u, B.N, s));
         HIP_TRY(launch_copy_f32(B.vp, A.v, B.N, s));
       }
-      // gpu_process.py:1113-1114, Q13: the original frames are deformed by
-      // the total predictor
-      HIP_TRY(launch_deform(
-          deform_args(P->frame_a, P->frame_b, D.in_dtype, D.height, D.width, B.d, B.up, B.vp, P->def_a, P->def_b),
-          s));
-      CorrArgs a = corr_args(P->def_a, P->def_b, kF32, D.height, D.width, B.d);
-      a.up = B.up;
-      a.vp = B.vp;
-      a.u = B.u;
-      a.v = B.v;
-      HIP_TRY(launch_correlate(a, s));
+      int r = deform_and_correlate(P, B, s);
+      if (r) return r;
     }
-    if (B.d.validate) {
-      HIP_TRY(launch_median_validate(B.u, B.v, B.d.field_m, B.d.field_n, B.d.median_tol, B.flags,
-                                     P->counters + 2 * k, s));
-      if (B.d.num_replacing_iters > 0) {
-        HIP_TRY(launch_spring_replace(B.u, B.v, B.flags, P->counters + 2 * k, B.d.field_m, B.d.field_n, P->rw, s));
-      }
-      if (last) {
-        HIP_TRY(launch_median_validate(B.u, B.v, B.d.field_m, B.d.field_n, B.d.median_tol, P->val_final,
-                                       P->counters + 2 * k + 1, s));
-      }
-    }
-    if (!last && B.d.smooth) {
-      SmoothnWork w{};
-      w.cos_m = B.tab.cos_m;
-      w.cos_n = B.tab.cos_n;
-      w.lambda = B.tab.lambda;
-      w.t0 = P->t0;
-      w.t1 = P->t1;
-      w.p_out = P->p_out + k;
-      w.state = P->smooth_state;
-      const int autos = !(B.d.smoothing_par > 0);
-      HIP_TRY(launch_smoothn(B.u, B.v, B.d.field_m, B.d.field_n, B.p_min, B.p_max,
-                             autos ? 0.0 : log10(B.d.smoothing_par), autos, w, s));
-    }
+    int r = validate_and_smooth(P, k, last, s);
+    if (r) return r;
   }
   if (D.out_scale != 1.0f) {
     PassBuffers& L = P->passes.back();
@@ -302,11 +330,72 @@ int enqueue(piv_plan* P) {
   return PIV_OK;
 }
 
+// Learned path: the network's field, interpolated to the last pass's grid,
+// is that pass's deformation predictor; the coarse passes do not run.
+int enqueue_learned(piv_plan* P) {
+  hipStream_t s = P->stream;
+  const piv_plan_desc& D = P->desc;
+  const int np = (int)P->passes.size();
+  PassBuffers& B = P->passes.back();
+  HIP_TRY(launch_fill_i32(P->counters, 0, 2 * np, s));
+  HIP_TRY(launch_prepare_input(P->frame_a, P->frame_b, D.in_dtype, D.height, D.width, kCnnFactor, P->net_scratch,
+                               P->net_in, s));
+  int r = cnn_enqueue_forward(P->net, P->net_in, 1, s);
+  if (r) return r;
+  const float* o = cnn_output(P->net);
+  const CnnInfo ni = cnn_info(P->net);
+  HIP_TRY(launch_grid_interp(interp_args(P->net_grid, B.d, o, o + (size_t)ni.out_h * ni.out_w, B.up, B.vp), s));
+  r = deform_and_correlate(P, B, s);
+  if (r) return r;
+  r = validate_and_smooth(P, np - 1, true, s);
+  if (r) return r;
+  if (D.out_scale != 1.0f) HIP_TRY(launch_scale(B.u, B.v, B.N, D.out_scale, s));
+  return PIV_OK;
+}
+
+// Runs an enqueue function eagerly on its first use and as a captured graph
+// afterwards (when the plan asks for graphs).
+int run_captured(piv_plan* P, int (*fn)(piv_plan*), hipGraph_t* graph, hipGraphExec_t* exec, int* runs) {
+  hipStream_t s = P->stream;
+  if (P->desc.use_graph && *runs > 0) {
+    if (!*exec) {
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int r = fn(P);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+      }
+      if (e != hipSuccess) return fail(PIV_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+      *graph = g;
+      HIP_TRY(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(*exec, s));
+  } else {
+    int r = fn(P);
+    if (r) return r;
+  }
+  (*runs)++;
+  return PIV_OK;
+}
+
+void drop_learned_graph(piv_plan* P) {
+  if (P->lgraph_exec) (void)hipGraphExecDestroy(P->lgraph_exec);
+  if (P->lgraph) (void)hipGraphDestroy(P->lgraph);
+  P->lgraph_exec = nullptr;
+  P->lgraph = nullptr;
+  P->lruns = 0;
+}
+
 void release(piv_plan* P) {
   if (!P) return;
   (void)hipSetDevice(P->device);
   if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
   if (P->graph) (void)hipGraphDestroy(P->graph);
+  drop_learned_graph(P);
+  if (P->net_in) (void)hipFree(P->net_in);
+  if (P->net_scratch) (void)hipFree(P->net_scratch);
   for (auto& B : P->passes) {
     for (void* p : {(void*)B.u, (void*)B.v, (void*)B.up, (void*)B.vp, (void*)B.flags})
       if (p) (void)hipFree(p);
@@ -409,28 +498,28 @@ int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on
   const hipMemcpyKind in_kind = frames_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   HIP_TRY(hipMemcpyAsync(P->frame_a, frame_a, P->frame_bytes, in_kind, s));
   HIP_TRY(hipMemcpyAsync(P->frame_b, frame_b, P->frame_bytes, in_kind, s));
-  // The first run is eager (it also sets kernel attributes); later runs
-  // replay a captured graph when requested.
-  if (P->desc.use_graph && P->runs > 0) {
-    if (!P->graph_exec) {
-      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int r = enqueue(P);
-      hipGraph_t g = nullptr;
-      hipError_t e = hipStreamEndCapture(s, &g);
-      if (r) {
-        if (g) (void)hipGraphDestroy(g);
-        return r;
-      }
-      if (e != hipSuccess) return fail(PIV_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
-      P->graph = g;
-      HIP_TRY(hipGraphInstantiate(&P->graph_exec, P->graph, nullptr, nullptr, 0));
+  // The first run of each path is eager (it also sets kernel attributes);
+  // later runs replay a captured graph when requested.
+  P->last_fallback = 0;
+  P->last_rejected = 0;
+  if (P->net) {
+    int r = run_captured(P, enqueue_learned, &P->lgraph, &P->lgraph_exec, &P->lruns);
+    if (r) return r;
+    const PassBuffers& L = P->passes.back();
+    int32_t rejected = 0;
+    HIP_TRY(hipMemcpyAsync(&rejected, P->counters + 2 * (P->passes.size() - 1), sizeof(int32_t),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    P->last_rejected = rejected;
+    if ((double)rejected > (double)P->max_reject * L.N) {
+      P->last_fallback = 1;
+      r = run_captured(P, enqueue, &P->graph, &P->graph_exec, &P->runs);
+      if (r) return r;
     }
-    HIP_TRY(hipGraphLaunch(P->graph_exec, s));
   } else {
-    int r = enqueue(P);
+    int r = run_captured(P, enqueue, &P->graph, &P->graph_exec, &P->runs);
     if (r) return r;
   }
-  P->runs++;
   const PassBuffers& L = P->passes.back();
   const hipMemcpyKind out_kind = outputs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   if (u) HIP_TRY(hipMemcpyAsync(u, L.u, L.N * sizeof(float), out_kind, s));
@@ -438,6 +527,46 @@ int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on
   if (val_init) HIP_TRY(hipMemcpyAsync(val_init, L.flags, L.N, out_kind, s));
   if (val_final) HIP_TRY(hipMemcpyAsync(val_final, P->val_final, L.N, out_kind, s));
   HIP_TRY(hipStreamSynchronize(s));
+  return PIV_OK;
+}
+
+int piv_plan_set_predictor(piv_plan* P, piv_cnn* net, float max_reject_fraction) {
+  if (!P) return fail(PIV_E_INVALID, "null plan");
+  HIP_TRY(hipSetDevice(P->device));
+  HIP_TRY(hipStreamSynchronize(P->stream));
+  drop_learned_graph(P);
+  if (!net) {
+    P->net = nullptr;
+    return PIV_OK;
+  }
+  if (!(max_reject_fraction >= 0.f && max_reject_fraction <= 1.f))
+    return fail(PIV_E_INVALID, "max_reject_fraction must lie in [0, 1]");
+  const piv_plan_desc& D = P->desc;
+  const CnnInfo ni = cnn_info(net);
+  if (D.height != ni.in_h * kCnnFactor || D.width != ni.in_w * kCnnFactor)
+    return fail(PIV_E_INVALID, "predictor input %d x %d does not match the frame %d x %d / %d", ni.in_h, ni.in_w,
+                D.height, D.width, kCnnFactor);
+  if (P->passes.size() < 2) return fail(PIV_E_INVALID, "a learned predictor needs a multi-pass schedule");
+  if (!P->net_in) {
+    if (hipMalloc((void**)&P->net_in, (size_t)ni.in_h * ni.in_w * 2 * sizeof(uint16_t)) != hipSuccess ||
+        hipMalloc(&P->net_scratch, prepare_scratch_bytes(D.height, D.width, kCnnFactor)) != hipSuccess)
+      return fail(PIV_E_NOMEM, "device allocation failed");
+  }
+  piv_pass_desc g{};
+  g.search_size = 2 * kCnnFactor;  // node (i, j) at (2f j + f, 2f i + f)
+  g.spacing = 2 * kCnnFactor;
+  g.field_m = ni.out_h;
+  g.field_n = ni.out_w;
+  P->net_grid = g;
+  P->net = net;
+  P->max_reject = max_reject_fraction;
+  return PIV_OK;
+}
+
+int piv_plan_last_fallback(const piv_plan* P, int32_t* fell_back, int32_t* rejected) {
+  if (!P) return fail(PIV_E_INVALID, "null plan");
+  if (fell_back) *fell_back = P->last_fallback;
+  if (rejected) *rejected = P->last_rejected;
   return PIV_OK;
 }
 

@@ -27,6 +27,9 @@ EXPORTED_SYMBOLS = (
     "piv_set_device", "piv_malloc", "piv_free", "piv_memcpy", "piv_memset", "piv_synchronize",
     "piv_stream_create", "piv_stream_destroy", "piv_stream_synchronize", "piv_event_create", "piv_event_destroy",
     "piv_event_record", "piv_event_elapsed_ms", "piv_stage_correlate_dft", "piv_synth_pair",
+    "piv_cnn_create", "piv_cnn_destroy", "piv_cnn_layout", "piv_cnn_set_params", "piv_cnn_get_params",
+    "piv_cnn_get_grads", "piv_cnn_get_buffer", "piv_cnn_forward", "piv_cnn_gradients", "piv_cnn_adam_step",
+    "piv_cnn_prepare_input", "piv_plan_set_predictor", "piv_plan_last_fallback",
 )
 
 SYNTH_MAX_VORTICES = 8
@@ -122,6 +125,19 @@ def _bind(lib):
         "piv_stage_correlate_dft": (i32, [vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
         "piv_synth_pair": (i32, [ctypes.POINTER(SynthField), ctypes.POINTER(SynthModel), ctypes.c_uint64, i32, i32,
                                  vp, vp, vp, vp, vp]),
+        "piv_cnn_create": (i32, [i32, i32, i32, ctypes.POINTER(vp)]),
+        "piv_cnn_destroy": (None, [vp]),
+        "piv_cnn_layout": (i32, [vp, vp, vp, vp, vp]),
+        "piv_cnn_set_params": (i32, [vp, vp]),
+        "piv_cnn_get_params": (i32, [vp, vp]),
+        "piv_cnn_get_grads": (i32, [vp, vp]),
+        "piv_cnn_get_buffer": (i32, [vp, i32, i32, i32, vp]),
+        "piv_cnn_forward": (i32, [vp, vp, i32, vp, vp]),
+        "piv_cnn_gradients": (i32, [vp, vp, vp, vp, i32, vp, vp]),
+        "piv_cnn_adam_step": (i32, [vp, f32, f32, f32, f32, vp]),
+        "piv_cnn_prepare_input": (i32, [vp, vp, vp, i32, i32, i32, vp, vp]),
+        "piv_plan_set_predictor": (i32, [vp, vp, f32]),
+        "piv_plan_last_fallback": (i32, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
         "piv_set_device": (i32, [i32]),
         "piv_malloc": (i32, [ctypes.POINTER(vp), ctypes.c_size_t]),
         "piv_free": (i32, [vp]),
@@ -303,6 +319,18 @@ class Plan:
         a, b = ctypes.c_int32(), ctypes.c_int32()
         check(self._lib.piv_plan_last_counts(self.handle, k, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def set_predictor(self, net, max_reject_fraction: float):
+        """Attach (or with ``net=None`` detach) a learned predictor network."""
+        self._net = net  # keep the network alive while the plan uses it
+        check(self._lib.piv_plan_set_predictor(self.handle, None if net is None else net.handle,
+                                               float(max_reject_fraction)))
+
+    def last_fallback(self):
+        """``(fell_back, rejected)`` of the last run with a predictor."""
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        check(self._lib.piv_plan_last_fallback(self.handle, ctypes.byref(a), ctypes.byref(b)))
+        return bool(a.value), b.value
 
     def close(self):
         if getattr(self, "handle", None):

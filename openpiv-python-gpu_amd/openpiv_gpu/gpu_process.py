@@ -198,7 +198,7 @@ class PIVGPU:
                  mean_tol=MEAN_TOL, rms_tol=RMS_TOL, num_replacing_iters=NUM_REPLACING_ITERS,
                  replacing_method=REPLACING_METHOD, replacing_size=REPLACING_SIZE, revalidate=REVALIDATE,
                  smooth=SMOOTH, smoothing_par=SMOOTHING_PAR, dt=1, scaling_par=1, mask=None, dtype_f="float32",
-                 device=0, use_graph=True):
+                 device=0, use_graph=True, predictor=None, max_reject_fraction=0.05):
         if not (dtype_f == "float32" or dtype_f is np.float32):
             raise NotImplementedError("dtype_f: only 'float32' processing is implemented on the MI355X path")
         if mask is not None:
@@ -240,6 +240,11 @@ class PIVGPU:
         self.dtype_f = np.float32
         self.device = device
         self.use_graph = bool(use_graph)
+        # learned coarse predictor (openpiv_gpu.learned.CoarseFlowNet): replaces
+        # every pass but the last; a pair whose last-pass median test rejects
+        # more than max_reject_fraction of the vectors reruns the full schedule
+        self.predictor = predictor
+        self.max_reject_fraction = float(max_reject_fraction)
 
         for i in range(n):
             for name, seq in (("s2n_tol", self.s2n_tol), ("mad_tol", self.mad_tol), ("mean_tol", self.mean_tol),
@@ -321,7 +326,15 @@ class PIVGPU:
     def _get_plan(self, in_dtype: int):
         if self._plan is None or self._plan.desc.in_dtype != in_dtype:
             self._plan = _native.Plan(self._plan_desc(in_dtype), self.device)
+            if self.predictor is not None:
+                self._plan.set_predictor(self.predictor, self.max_reject_fraction)
         return self._plan
+
+    @property
+    def last_fallback(self):
+        """Whether the last pair fell back to the full schedule (learned
+        predictor only), and the last pass's rejected-vector count."""
+        return self._plan.last_fallback() if self._plan is not None else (False, 0)
 
     # -- the pass loop ---------------------------------------------------------
     @staticmethod
@@ -435,7 +448,7 @@ class piv_gpu:
     """
 
     def __init__(self, frame_shape, min_search_size, **kwargs):
-        unknown = set(kwargs) - set(_KWARGS) - {"device", "use_graph"}
+        unknown = set(kwargs) - set(_KWARGS) - {"device", "use_graph", "predictor", "max_reject_fraction"}
         if unknown:
             raise TypeError(f"unexpected keyword arguments: {sorted(unknown)}")
         cfg = dict(_DEFAULTS)
