@@ -93,6 +93,89 @@ def stage_kernel_times(fa_u16, piv, reps=20):
     return out, deform
 
 
+def learned_vs_plain(n_pairs, weights_path):
+    """Plain 64->32->16 schedule vs the learned predictor (last pass only,
+    full-schedule fallback above 5 % rejections) on a stream of held-out
+    2048x2048 vortex pairs (seeds 200000+), rendered into HBM before timing.
+    Both rates include each call's end-of-pair synchronisation."""
+    from openpiv_gpu import _native, synthetic
+    from openpiv_gpu.gpu_process import PIVGPU
+    from openpiv_gpu.learned import CoarseFlowNet
+    if not os.path.exists(weights_path):
+        return {"skipped": f"no trained weights at {os.path.relpath(weights_path, ROOT)}"}
+    shape = (2048, 2048)
+    net = CoarseFlowNet.load(weights_path)
+    kw = dict(search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2, dtype_f="float32")
+    plain = PIVGPU(shape, 16, **kw)
+    learned = PIVGPU(shape, 16, predictor=net, max_reject_fraction=0.05, **kw)
+    # the network's field feeding the 32 px pass instead (only the 64 px pass skipped)
+    learned32 = PIVGPU(shape, 16, predictor=net, max_reject_fraction=0.05, predictor_pass=1, **kw)
+    pp = plain._get_plan(_native.PIV_U16)
+    lp = learned._get_plan(_native.PIV_U16)
+    lp32 = learned32._get_plan(_native.PIV_U16)
+    m, n = plain.piv_fields[-1].field_shape
+    x, y = (np.asarray(c, np.float64) for c in plain.coords)
+    frames = _native.DeviceBuffer((n_pairs, 2) + shape, np.uint16)
+    outs = {k: _native.DeviceBuffer((n_pairs, 2, m, n), np.float32) for k in ("plain", "learned", "learned32")}
+    gen = synthetic.DevicePairGenerator(shape, synthetic.MODEL_12BIT)
+    seeds = [200000 + i for i in range(n_pairs)]
+    log(f"rendering {n_pairs} held-out vortex pairs")
+    for i, sd in enumerate(seeds):
+        f = frames.view(i)
+        gen.render(synthetic.vortex_field(sd, shape), sd, f.view(0), f.view(1))
+    _native.synchronize()
+
+    def run(plan, key, i):
+        f = frames.view(i)
+        o = outs[key].view(i)
+        plan.run(f.view(0).value, f.view(1).value, True, o.view(0).value, o.view(1).value, None, None, True)
+
+    res = {}
+    for key, plan in (("plain", pp), ("learned", lp), ("learned32", lp32)):
+        for w in range(3):
+            run(plan, key, w % n_pairs)
+        fallbacks = 0
+        t0 = time.perf_counter()
+        for i in range(n_pairs):
+            run(plan, key, i)
+            if key != "plain":
+                fallbacks += plan.last_fallback()[0]
+        dt = time.perf_counter() - t0
+        res[key] = {"pairs_per_s": n_pairs / dt, "fallbacks": fallbacks}
+    for key in res:
+        se = 0.0
+        cnt = 0
+        per_pair = []
+        for i, sd in enumerate(seeds):
+            uv = outs[key].view(i).get()
+            ut, vt = synthetic.vortex_field(sd, shape)(x, y)
+            e = float(np.sum((uv[0] - ut) ** 2 + (uv[1] - vt) ** 2))
+            se += e
+            cnt += 2 * ut.size
+            per_pair.append(np.sqrt(e / (2 * ut.size)))
+        res[key]["rms_px_vs_truth"] = float(np.sqrt(se / cnt))
+        res[key]["worst_pair_rms_px"] = float(np.max(per_pair))
+    return {
+        "workload": f"{n_pairs} held-out 2048x2048 uint16 pairs (1-4 Lamb-Oseen vortices + shear, seeds 200000+), "
+                    "device-rendered, resident in HBM",
+        "plain_pairs_per_s": round(res["plain"]["pairs_per_s"], 2),
+        "learned_pairs_per_s": round(res["learned"]["pairs_per_s"], 2),
+        "speedup": round(res["learned"]["pairs_per_s"] / res["plain"]["pairs_per_s"], 3),
+        "plain_rms_px_vs_truth": round(res["plain"]["rms_px_vs_truth"], 5),
+        "learned_rms_px_vs_truth": round(res["learned"]["rms_px_vs_truth"], 5),
+        "learned_worst_pair_rms_px": round(res["learned"]["worst_pair_rms_px"], 5),
+        "fallback_fraction": round(res["learned"]["fallbacks"] / n_pairs, 4),
+        "max_reject_fraction": 0.05,
+        "skip_64_only": {
+            "pairs_per_s": round(res["learned32"]["pairs_per_s"], 2),
+            "speedup": round(res["learned32"]["pairs_per_s"] / res["plain"]["pairs_per_s"], 3),
+            "rms_px_vs_truth": round(res["learned32"]["rms_px_vs_truth"], 5),
+            "fallback_fraction": round(res["learned32"]["fallbacks"] / n_pairs, 4),
+        },
+        "weights": os.path.relpath(weights_path, ROOT),
+    }
+
+
 def cpu_baseline(fa, fb, pairs=1):
     from oracle import piv_oracle as O
     s = O.OracleSettings(min_search_size=16, search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2.0)
@@ -113,6 +196,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--skip-cpu-baseline", action="store_true")
     ap.add_argument("--skip-kernel-timing", action="store_true")
+    ap.add_argument("--learned-pairs", type=int, default=1000)
+    ap.add_argument("--skip-learned", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -178,6 +263,12 @@ def main():
                 "frac": round(dom["tflops"] / 157.3, 4), "traffic": None,
                 "per_launch_flop": dom["flop"], "avg_launch_ms": round(dom["avg_ms"], 5)}
 
+    learned = None
+    if not args.skip_learned and args.learned_pairs > 0:
+        from openpiv_gpu.learned import default_weights_path
+        log("learned predictor vs plain")
+        learned = learned_vs_plain(args.learned_pairs, default_weights_path((2048, 2048)))
+
     cpu = None
     if not args.skip_cpu_baseline:
         log("CPU baseline (oracle, 1 pair)")
@@ -198,6 +289,7 @@ def main():
             "bound": "hbm", "achieved": round(deform["gbs"], 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(deform["gbs"] / 8000.0, 4), "avg_launch_ms": round(deform["avg_ms"], 5),
             "per_launch_bytes": deform["bytes"]},
+        "learned_vs_plain": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -92,7 +92,8 @@ void piv_plan_destroy(piv_plan *plan);
  * before replacement (PIVGPU.init_val_locations, :770-771); val_final =
  * flags after replacement (PIVGPU.val_locations, :783-784).  Any output
  * pointer may be NULL.  outputs_on_device selects host or device outputs.
- * Synchronous: returns after the outputs are written. */
+ * Device-resident frames are read after all work queued before the call on
+ * the null stream.  Synchronous: returns after the outputs are written. */
 int piv_run(piv_plan *plan, const void *frame_a, const void *frame_b, int frames_on_device, float *u, float *v,
             uint8_t *val_init, uint8_t *val_final, int outputs_on_device);
 
@@ -198,12 +199,13 @@ int piv_cnn_prepare_input(piv_cnn *net, const void *frame_a, const void *frame_b
                           uint16_t *out, void *stream);
 
 /* Learned predictor for a plan: piv_run then evaluates the network on the
- * pair, interpolates its field to the last pass's grid as that pass's
- * deformation predictor and runs the last pass only.  When the last pass's
- * median test rejects more than max_reject_fraction of the vectors, the same
- * call runs the plan's full schedule instead (fallback).  net = NULL
- * removes the predictor.  The frame must be 8 x (network input). */
-int piv_plan_set_predictor(piv_plan *plan, piv_cnn *net, float max_reject_fraction);
+ * pair, interpolates its field to the grid of pass first_pass as that pass's
+ * deformation predictor and runs passes first_pass..last only (first_pass
+ * < 0: the last pass alone).  When the last pass's median test rejects more
+ * than max_reject_fraction of the vectors, the same call runs the plan's
+ * full schedule instead (fallback).  net = NULL removes the predictor.  The
+ * frame must be 8 x (network input). */
+int piv_plan_set_predictor(piv_plan *plan, piv_cnn *net, int first_pass, float max_reject_fraction);
 int piv_plan_last_fallback(const piv_plan *plan, int32_t *fell_back, int32_t *rejected);
 
 /* ---- synthetic particle-image pairs ------------------------------------

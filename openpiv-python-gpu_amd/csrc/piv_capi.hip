@@ -235,12 +235,14 @@ struct piv_plan {
   int runs = 0;
   // learned predictor (piv_plan_set_predictor)
   piv_cnn* net = nullptr;
+  int net_first = 0;            // first pass run on the learned path (its predictor = the network's field)
   float max_reject = 0.05f;
   piv_pass_desc net_grid{};     // the network's output nodes as a pass grid
   uint16_t* net_in = nullptr;   // network input of the current pair
   void* net_scratch = nullptr;
   hipGraph_t lgraph = nullptr;
   hipGraphExec_t lgraph_exec = nullptr;
+  hipEvent_t inputs_ready = nullptr;  // null-stream work that produced device-resident frames
   int lruns = 0;
   int32_t last_fallback = 0;
   int32_t last_rejected = 0;
@@ -295,32 +297,39 @@ int validate_and_smooth(piv_plan* P, int k, bool last, hipStream_t s) {
   return PIV_OK;
 }
 
-int enqueue(piv_plan* P) {
+// Pass k with the predictor from pass k-1 (gpu_process.py:620-625).
+int predictor_from_previous(piv_plan* P, int k, hipStream_t s) {
+  PassBuffers& A = P->passes[k - 1];
+  PassBuffers& B = P->passes[k];
+  if (A.d.search_size != B.d.search_size) {
+    HIP_TRY(launch_grid_interp(interp_args(A.d, B.d, A.u, A.v, B.up, B.vp), s));
+  } else {
+    HIP_TRY(launch_copy_f32(B.up, A.u, B.N, s));
+    HIP_TRY(launch_copy_f32(B.vp, A.v, B.N, s));
+  }
+  return PIV_OK;
+}
+
+// Passes first..np-1; pass `first` takes its predictor from B.up/B.vp as
+// already written (first > 0) or correlates the raw frames (first == 0).
+int enqueue_passes(piv_plan* P, int first) {
   hipStream_t s = P->stream;
   const piv_plan_desc& D = P->desc;
   const int np = (int)P->passes.size();
-  HIP_TRY(launch_fill_i32(P->counters, 0, 2 * np, s));
-  for (int k = 0; k < np; ++k) {
+  for (int k = first; k < np; ++k) {
     PassBuffers& B = P->passes[k];
     const bool last = k == np - 1;
+    int r = PIV_OK;
     if (k == 0) {
       CorrArgs a = corr_args(P->frame_a, P->frame_b, D.in_dtype, D.height, D.width, B.d);
       a.u = B.u;
       a.v = B.v;
       HIP_TRY(launch_correlate(a, s));
     } else {
-      PassBuffers& A = P->passes[k - 1];
-      // gpu_process.py:620-625 -- predictor on the current grid
-      if (A.d.search_size != B.d.search_size) {
-        HIP_TRY(launch_grid_interp(interp_args(A.d, B.d, A.u, A.v, B.up, B.vp), s));
-      } else {
-        HIP_TRY(launch_copy_f32(B.up, A.u, B.N, s));
-        HIP_TRY(launch_copy_f32(B.vp, A.v, B.N, s));
-      }
-      int r = deform_and_correlate(P, B, s);
-      if (r) return r;
+      if (k > first) r = predictor_from_previous(P, k, s);
+      if (!r) r = deform_and_correlate(P, B, s);
     }
-    int r = validate_and_smooth(P, k, last, s);
+    if (!r) r = validate_and_smooth(P, k, last, s);
     if (r) return r;
   }
   if (D.out_scale != 1.0f) {
@@ -330,14 +339,18 @@ int enqueue(piv_plan* P) {
   return PIV_OK;
 }
 
-// Learned path: the network's field, interpolated to the last pass's grid,
-// is that pass's deformation predictor; the coarse passes do not run.
+int enqueue(piv_plan* P) {
+  HIP_TRY(launch_fill_i32(P->counters, 0, 2 * (int)P->passes.size(), P->stream));
+  return enqueue_passes(P, 0);
+}
+
+// Learned path: the network's field, interpolated to the grid of pass
+// net_first, is that pass's deformation predictor; earlier passes do not run.
 int enqueue_learned(piv_plan* P) {
   hipStream_t s = P->stream;
   const piv_plan_desc& D = P->desc;
-  const int np = (int)P->passes.size();
-  PassBuffers& B = P->passes.back();
-  HIP_TRY(launch_fill_i32(P->counters, 0, 2 * np, s));
+  PassBuffers& B = P->passes[P->net_first];
+  HIP_TRY(launch_fill_i32(P->counters, 0, 2 * (int)P->passes.size(), s));
   HIP_TRY(launch_prepare_input(P->frame_a, P->frame_b, D.in_dtype, D.height, D.width, kCnnFactor, P->net_scratch,
                                P->net_in, s));
   int r = cnn_enqueue_forward(P->net, P->net_in, 1, s);
@@ -345,12 +358,7 @@ int enqueue_learned(piv_plan* P) {
   const float* o = cnn_output(P->net);
   const CnnInfo ni = cnn_info(P->net);
   HIP_TRY(launch_grid_interp(interp_args(P->net_grid, B.d, o, o + (size_t)ni.out_h * ni.out_w, B.up, B.vp), s));
-  r = deform_and_correlate(P, B, s);
-  if (r) return r;
-  r = validate_and_smooth(P, np - 1, true, s);
-  if (r) return r;
-  if (D.out_scale != 1.0f) HIP_TRY(launch_scale(B.u, B.v, B.N, D.out_scale, s));
-  return PIV_OK;
+  return enqueue_passes(P, P->net_first);
 }
 
 // Runs an enqueue function eagerly on its first use and as a captured graph
@@ -405,6 +413,7 @@ void release(piv_plan* P) {
                   (void*)P->val_final, (void*)P->t0, (void*)P->t1, (void*)P->rw.slot, (void*)P->rw.list,
                   (void*)P->rw.work, P->smooth_state})
     if (p) (void)hipFree(p);
+  if (P->inputs_ready) (void)hipEventDestroy(P->inputs_ready);
   if (P->stream) (void)hipStreamDestroy(P->stream);
   delete P;
 }
@@ -444,8 +453,9 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
     release(P);
     return code;
   };
-  if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess)
-    return bail(fail(PIV_E_HIP, "hipStreamCreate failed"));
+  if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&P->inputs_ready, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(PIV_E_HIP, "hipStreamCreate/hipEventCreate failed"));
   const size_t npx = (size_t)D.height * D.width;
   P->frame_bytes = npx * dtype_size(D.in_dtype);
   int maxN = 0;
@@ -496,6 +506,12 @@ int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on
   HIP_TRY(hipSetDevice(P->device));
   hipStream_t s = P->stream;
   const hipMemcpyKind in_kind = frames_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  if (frames_on_device) {
+    // device frames may come from work queued on the null stream, which a
+    // non-blocking plan stream does not wait for by itself
+    HIP_TRY(hipEventRecord(P->inputs_ready, nullptr));
+    HIP_TRY(hipStreamWaitEvent(s, P->inputs_ready, 0));
+  }
   HIP_TRY(hipMemcpyAsync(P->frame_a, frame_a, P->frame_bytes, in_kind, s));
   HIP_TRY(hipMemcpyAsync(P->frame_b, frame_b, P->frame_bytes, in_kind, s));
   // The first run of each path is eager (it also sets kernel attributes);
@@ -530,7 +546,7 @@ int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on
   return PIV_OK;
 }
 
-int piv_plan_set_predictor(piv_plan* P, piv_cnn* net, float max_reject_fraction) {
+int piv_plan_set_predictor(piv_plan* P, piv_cnn* net, int first_pass, float max_reject_fraction) {
   if (!P) return fail(PIV_E_INVALID, "null plan");
   HIP_TRY(hipSetDevice(P->device));
   HIP_TRY(hipStreamSynchronize(P->stream));
@@ -547,6 +563,9 @@ int piv_plan_set_predictor(piv_plan* P, piv_cnn* net, float max_reject_fraction)
     return fail(PIV_E_INVALID, "predictor input %d x %d does not match the frame %d x %d / %d", ni.in_h, ni.in_w,
                 D.height, D.width, kCnnFactor);
   if (P->passes.size() < 2) return fail(PIV_E_INVALID, "a learned predictor needs a multi-pass schedule");
+  if (first_pass < 0) first_pass = (int)P->passes.size() - 1;
+  if (first_pass < 1 || first_pass >= (int)P->passes.size())
+    return fail(PIV_E_INVALID, "first_pass %d outside 1..%d", first_pass, (int)P->passes.size() - 1);
   if (!P->net_in) {
     if (hipMalloc((void**)&P->net_in, (size_t)ni.in_h * ni.in_w * 2 * sizeof(uint16_t)) != hipSuccess ||
         hipMalloc(&P->net_scratch, prepare_scratch_bytes(D.height, D.width, kCnnFactor)) != hipSuccess)
@@ -559,6 +578,7 @@ int piv_plan_set_predictor(piv_plan* P, piv_cnn* net, float max_reject_fraction)
   g.field_n = ni.out_w;
   P->net_grid = g;
   P->net = net;
+  P->net_first = first_pass;
   P->max_reject = max_reject_fraction;
   return PIV_OK;
 }

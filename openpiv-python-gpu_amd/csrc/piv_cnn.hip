@@ -102,20 +102,108 @@ __device__ __forceinline__ float in_value(const InView& v, bool ok, long o0, lon
 }
 
 // ---------------------------------------------------------------- forward
-// grid: x = ceil(pixels / 256), y = ceil(co / 32); block 256 (4 waves, one
-// 64-pixel strip each).  VEC: ci0 and ci1 multiples of 16 (4-channel vector
-// loads); otherwise element-wise loads over the flat (tap, ci) index.
-template <bool VEC>
+// One block computes one (16 TM) x 64 tile of D[co][pixel]; its four waves
+// split the K = 9 * ci reduction (16-channel chunks of one tap, round robin)
+// and are summed in LDS, so even the 32 x 32 layers of a single image spread
+// over enough blocks.  ci0 and ci1 are multiples of 16 (4-channel vector loads).
+// grid: x = ceil(pixels / 64), y = ceil(co / (16 TM)).
+template <int TM>
 __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvGeom g, InView in, const uint16_t* __restrict__ w,
                                                        const float* __restrict__ bias, void* out, int out_mode,
                                                        int relu) {
+  __shared__ float red[3][TM][4][64][4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = lane & 15, gq = lane >> 4;
+  const long total = (long)g.batch * g.ho * g.wo;
+  const long pbase = (long)blockIdx.x * 64;
+  const int cobase = blockIdx.y * (16 * TM);
+  const int ci = in.c0 + in.c1, nc = ci / 16;
+  Pix px[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) px[t] = pix_of(pbase + t * 16 + r, total, g.ho, g.wo);
+  int coA[TM];
+  bool okA[TM];
+#pragma unroll
+  for (int m = 0; m < TM; ++m) {
+    coA[m] = cobase + m * 16 + r;
+    okA[m] = coA[m] < g.co;
+  }
+  float4_t acc[TM][4];
+#pragma unroll
+  for (int m = 0; m < TM; ++m)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[m][t] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const bf16x4_t zero4 = {0, 0, 0, 0};
+  for (int kc = wv; kc < 9 * nc; kc += 4) {
+    const int tap = kc / nc, c = (kc - tap * nc) * 16;
+    const int ky = tap / 3, kx = tap - 3 * ky;
+    const int cc = c + 4 * gq;
+    bf16x4_t a[TM], b[4];
+#pragma unroll
+    for (int m = 0; m < TM; ++m) a[m] = okA[m] ? load_bf4(w + ((long)coA[m] * 9 + tap) * ci + cc) : zero4;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      long o0, o1;
+      const bool ok = px[t].ok && in_offsets(in, px[t].n, px[t].y * g.stride + ky - 1,
+                                             px[t].x * g.stride + kx - 1, &o0, &o1);
+      b[t] = !ok ? zero4 : (c < in.c0 ? load_bf4(in.s0 + o0 + cc) : load_bf4(in.s1 + o1 + cc - in.c0));
+    }
+#pragma unroll
+    for (int m = 0; m < TM; ++m)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[m][t] = mfma(a[m], b[t], acc[m][t]);
+  }
+  if (wv > 0) {
+#pragma unroll
+    for (int m = 0; m < TM; ++m)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) red[wv - 1][m][t][lane][e] = acc[m][t][e];
+  }
+  __syncthreads();
+  if (wv != 0) return;
+  // epilogue: rows co = cobase + 16m + 4gq + e of pixel column r
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    if (!px[t].ok) continue;
+    const long p = pbase + t * 16 + r;
+#pragma unroll
+    for (int m = 0; m < TM; ++m) {
+      const int co0 = cobase + m * 16 + 4 * gq;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int co = co0 + e;
+        if (co >= g.co) continue;
+        float val = acc[m][t][e] + red[0][m][t][lane][e] + red[1][m][t][lane][e] + red[2][m][t][lane][e] + bias[co];
+        if (relu) val = fmaxf(val, 0.f);
+        if (out_mode == kOutBf16) {
+          static_cast<uint16_t*>(out)[p * g.co + co] = f2bf(val);
+        } else if (out_mode == kOutF32) {
+          static_cast<float*>(out)[p * g.co + co] = val;
+        } else {  // per-image planar [n][co][y][x]
+          const long hw = (long)g.ho * g.wo;
+          static_cast<float*>(out)[((long)px[t].n * g.co + co) * hw + p % hw] = val;
+        }
+      }
+    }
+  }
+}
+
+// Layers whose input channels are not multiples of 16 (the 2-channel input
+// layer): each wave owns a 32 x 64 tile and walks the flat (tap, ci) index
+// element by element.  grid: x = ceil(pixels / 256), y = ceil(co / 32).
+template <int CI>  // CI > 0: channel count known at compile time
+__global__ __launch_bounds__(256) void conv_fwd_small_kernel(ConvGeom g, InView in, const uint16_t* __restrict__ w,
+                                                             const float* __restrict__ bias, void* out, int out_mode,
+                                                             int relu) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = lane & 15, gq = lane >> 4;
   const long total = (long)g.batch * g.ho * g.wo;
   const long pbase = ((long)blockIdx.x * 4 + wv) * 64;
   if (pbase >= total) return;
   const int cobase = blockIdx.y * 32;
-  const int ci = in.c0 + in.c1, K = 9 * ci;
+  const int ci = CI > 0 ? CI : in.c0 + in.c1, K = 9 * ci;
   Pix px[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) px[t] = pix_of(pbase + t * 16 + r, total, g.ho, g.wo);
@@ -131,62 +219,35 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvGeom g, InView in, co
   for (int m = 0; m < 2; ++m)
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[m][t] = float4_t{0.f, 0.f, 0.f, 0.f};
-  const bf16x4_t zero4 = {0, 0, 0, 0};
-  if constexpr (VEC) {
-    for (int tap = 0; tap < 9; ++tap) {
-      const int ky = tap / 3, kx = tap % 3;
-      long o0[4], o1[4];
-      bool ok[4];
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    bf16x4_t a[2], b[4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
-        ok[t] = px[t].ok &&
-                in_offsets(in, px[t].n, px[t].y * g.stride + ky - 1, px[t].x * g.stride + kx - 1, &o0[t], &o1[t]);
-      for (int c = 0; c < ci; c += 16) {
-        const int cc = c + 4 * gq;
-        bf16x4_t a[2], b[4];
+    for (int m = 0; m < 2; ++m)
 #pragma unroll
-        for (int m = 0; m < 2; ++m) a[m] = okA[m] ? load_bf4(w + ((long)coA[m] * 9 + tap) * ci + cc) : zero4;
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-          b[t] = !ok[t] ? zero4 : (c < in.c0 ? load_bf4(in.s0 + o0[t] + cc) : load_bf4(in.s1 + o1[t] + cc - in.c0));
-#pragma unroll
-        for (int m = 0; m < 2; ++m)
-#pragma unroll
-          for (int t = 0; t < 4; ++t) acc[m][t] = mfma(a[m], b[t], acc[m][t]);
+      for (int e = 0; e < 4; ++e) {
+        const int k = k0 + 4 * gq + e;
+        a[m][e] = (okA[m] && k < K) ? (short)w[(long)coA[m] * K + k] : (short)0;
       }
-    }
-  } else {
-    for (int k0 = 0; k0 < K; k0 += 16) {
-      bf16x4_t a[2], b[4];
 #pragma unroll
-      for (int m = 0; m < 2; ++m)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int k = k0 + 4 * gq + e;
-          a[m][e] = (okA[m] && k < K) ? (short)w[(long)coA[m] * K + k] : (short)0;
+      for (int e = 0; e < 4; ++e) {
+        const int k = k0 + 4 * gq + e;
+        float val = 0.f;
+        if (k < K && px[t].ok) {
+          const int tap = k / ci, cch = k - tap * ci;
+          long o0, o1;
+          const bool ok = in_offsets(in, px[t].n, px[t].y * g.stride + tap / 3 - 1,
+                                     px[t].x * g.stride + tap % 3 - 1, &o0, &o1);
+          val = in_value(in, ok, o0, o1, cch);
         }
+        b[t][e] = (short)f2bf(val);
+      }
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
+    for (int m = 0; m < 2; ++m)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int k = k0 + 4 * gq + e;
-          float val = 0.f;
-          if (k < K && px[t].ok) {
-            const int tap = k / ci, cch = k % ci;
-            long o0, o1;
-            const bool ok = in_offsets(in, px[t].n, px[t].y * g.stride + tap / 3 - 1,
-                                       px[t].x * g.stride + tap % 3 - 1, &o0, &o1);
-            val = in_value(in, ok, o0, o1, cch);
-          }
-          b[t][e] = (short)f2bf(val);
-        }
-#pragma unroll
-      for (int m = 0; m < 2; ++m)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) acc[m][t] = mfma(a[m], b[t], acc[m][t]);
-    }
+      for (int t = 0; t < 4; ++t) acc[m][t] = mfma(a[m], b[t], acc[m][t]);
   }
-  // epilogue: rows co = cobase + 16m + 4gq + e of pixel column r
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     if (!px[t].ok) continue;
@@ -204,7 +265,7 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvGeom g, InView in, co
           static_cast<uint16_t*>(out)[p * g.co + co] = f2bf(val);
         } else if (out_mode == kOutF32) {
           static_cast<float*>(out)[p * g.co + co] = val;
-        } else {  // per-image planar [n][co][y][x]
+        } else {
           const long hw = (long)g.ho * g.wo;
           static_cast<float*>(out)[((long)px[t].n * g.co + co) * hw + p % hw] = val;
         }
@@ -544,9 +605,25 @@ __global__ __launch_bounds__(256) void box_mean_kernel(const T* __restrict__ fa,
   if (i < (long)hh * ww) {
     const int y = (int)(i / ww), x = (int)(i % ww);
     float acc = 0.f;
-    for (int dy = 0; dy < f; ++dy) {
-      const T* row = src + (long)(y * f + dy) * wd + x * f;
-      for (int dx = 0; dx < f; ++dx) acc += (float)row[dx];
+    if constexpr (sizeof(T) == 2) {
+      if (f == 8 && (wd & 7) == 0) {  // one 16-byte load per row
+        for (int dy = 0; dy < 8; ++dy) {
+          const uint4 q = *reinterpret_cast<const uint4*>(src + (long)(y * 8 + dy) * wd + x * 8);
+          const uint32_t wds[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+          for (int k = 0; k < 4; ++k) acc += (float)(wds[k] & 0xffffu) + (float)(wds[k] >> 16);
+        }
+      } else {
+        for (int dy = 0; dy < f; ++dy) {
+          const T* row = src + (long)(y * f + dy) * wd + x * f;
+          for (int dx = 0; dx < f; ++dx) acc += (float)row[dx];
+        }
+      }
+    } else {
+      for (int dy = 0; dy < f; ++dy) {
+        const T* row = src + (long)(y * f + dy) * wd + x * f;
+        for (int dx = 0; dx < f; ++dx) acc += (float)row[dx];
+      }
     }
     acc /= (float)(f * f);
     tmp[(long)frame * hh * ww + i] = acc;
@@ -573,16 +650,29 @@ __global__ __launch_bounds__(256) void box_mean_kernel(const T* __restrict__ fa,
 // BF16 NHWC input [y][x][frame].
 __global__ __launch_bounds__(256) void standardise_kernel(const float* __restrict__ tmp, const double* __restrict__ part,
                                                           int nparts, long npix, uint16_t* out) {
+  __shared__ double red[4][256];
   __shared__ float stat[2][2];
+  double a[4] = {0.0, 0.0, 0.0, 0.0};  // sum, sum of squares of frame 0 and frame 1
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+    a[0] += part[(long)k * 2];
+    a[1] += part[(long)k * 2 + 1];
+    a[2] += part[((long)nparts + k) * 2];
+    a[3] += part[((long)nparts + k) * 2 + 1];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = a[q];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
   if (threadIdx.x < 2) {
     const int frame = threadIdx.x;
-    double s = 0.0, s2 = 0.0;
-    for (int k = 0; k < nparts; ++k) {
-      s += part[((long)frame * nparts + k) * 2];
-      s2 += part[((long)frame * nparts + k) * 2 + 1];
-    }
-    const double mean = s / (double)npix;
-    const double var = fmax(s2 / (double)npix - mean * mean, 0.0);
+    const double mean = red[2 * frame][0] / (double)npix;
+    const double var = fmax(red[2 * frame + 1][0] / (double)npix - mean * mean, 0.0);
     stat[frame][0] = (float)mean;
     stat[frame][1] = (float)(var > 0.0 ? 1.0 / sqrt(var) : 1.0);
   }
@@ -604,11 +694,23 @@ hipError_t launch_conv_fwd(const ConvGeom& g, const uint16_t* s0, int c0, int up
                            const uint16_t* w, const float* bias, void* out, int out_mode, int relu, hipStream_t s) {
   const InView in{s0, s1, c0, c1, up0, g.hi, g.wi};
   const long pixels = (long)g.batch * g.ho * g.wo;
-  const dim3 grid(blocks_for(pixels), (unsigned)((g.co + 31) / 32));
-  if (c0 % 16 == 0 && c1 % 16 == 0)
-    hipLaunchKernelGGL(conv_fwd_kernel<true>, grid, dim3(256), 0, s, g, in, w, bias, out, out_mode, relu);
-  else
-    hipLaunchKernelGGL(conv_fwd_kernel<false>, grid, dim3(256), 0, s, g, in, w, bias, out, out_mode, relu);
+  if (c0 % 16 == 0 && c1 % 16 == 0) {
+    // 32-row tiles once there are enough 64-pixel columns to fill the chip
+    const long cols = (pixels + 63) / 64;
+    if (g.co > 16 && cols * ((g.co + 31) / 32) >= 512) {
+      const dim3 grid((unsigned)cols, (unsigned)((g.co + 31) / 32));
+      hipLaunchKernelGGL(conv_fwd_kernel<2>, grid, dim3(256), 0, s, g, in, w, bias, out, out_mode, relu);
+    } else {
+      const dim3 grid((unsigned)cols, (unsigned)((g.co + 15) / 16));
+      hipLaunchKernelGGL(conv_fwd_kernel<1>, grid, dim3(256), 0, s, g, in, w, bias, out, out_mode, relu);
+    }
+  } else {
+    const dim3 grid(blocks_for(pixels), (unsigned)((g.co + 31) / 32));
+    if (c0 + c1 == 2)
+      hipLaunchKernelGGL(conv_fwd_small_kernel<2>, grid, dim3(256), 0, s, g, in, w, bias, out, out_mode, relu);
+    else
+      hipLaunchKernelGGL(conv_fwd_small_kernel<0>, grid, dim3(256), 0, s, g, in, w, bias, out, out_mode, relu);
+  }
   return hipGetLastError();
 }
 

@@ -268,7 +268,7 @@ __device__ void spring_cg_global(float* u, float* v, int m, int n, int nv, const
   }
   sum256<4>(acc, red);
   double rz_u = acc[0], rz_v = acc[1];
-  const double tol2_u = 1e-26 * acc[2], tol2_v = 1e-26 * acc[3];
+  const double tol2_u = 1e-20 * acc[2], tol2_v = 1e-20 * acc[3];
   bool done_u = acc[2] == 0.0, done_v = acc[3] == 0.0;
   const int max_it = nv + 200 < 20000 ? nv + 200 : 20000;
   for (int it = 0; it < max_it && !(done_u && done_v); ++it) {
@@ -408,7 +408,7 @@ __device__ void spring_cg(SpringLds& L, int nv) {
   }
   group_sum<4, T>(acc, L.red);
   double rz_u = acc[0], rz_v = acc[1];
-  const double tol2_u = 1e-26 * acc[2], tol2_v = 1e-26 * acc[3];
+  const double tol2_u = 1e-20 * acc[2], tol2_v = 1e-20 * acc[3];
   bool done_u = rz_u <= tol2_u, done_v = rz_v <= tol2_v;
   const int max_it = nv + 200;
   for (int it = 0; it < max_it && !(done_u && done_v); ++it) {

@@ -136,7 +136,7 @@ def _bind(lib):
         "piv_cnn_gradients": (i32, [vp, vp, vp, vp, i32, vp, vp]),
         "piv_cnn_adam_step": (i32, [vp, f32, f32, f32, f32, vp]),
         "piv_cnn_prepare_input": (i32, [vp, vp, vp, i32, i32, i32, vp, vp]),
-        "piv_plan_set_predictor": (i32, [vp, vp, f32]),
+        "piv_plan_set_predictor": (i32, [vp, vp, i32, f32]),
         "piv_plan_last_fallback": (i32, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
         "piv_set_device": (i32, [i32]),
         "piv_malloc": (i32, [ctypes.POINTER(vp), ctypes.c_size_t]),
@@ -239,6 +239,12 @@ class DeviceBuffer:
     def value(self):
         return self.ptr.value
 
+    def view(self, index):
+        """Sub-array ``index`` along the first axis (no copy)."""
+        sub = self.shape[1:]
+        step = int(np.prod(sub)) * self.dtype.itemsize
+        return DeviceView(self.value + index * step, sub, self.dtype, self)
+
     def free(self):
         if getattr(self, "ptr", None) is not None and self.ptr.value:
             self._lib.piv_free(self.ptr)
@@ -249,6 +255,33 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+class DeviceView:
+    """A typed window into device memory owned elsewhere (no ownership)."""
+
+    def __init__(self, ptr: int, shape, dtype, owner=None):
+        self.ptr = ctypes.c_void_p(int(ptr))
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self._owner = owner
+        self._lib = load_library()
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+    def get(self):
+        out = np.empty(self.shape, self.dtype)
+        check(self._lib.piv_memcpy(out.ctypes.data, self.ptr, self.nbytes, 1))
+        return out
+
+    def view(self, index):
+        """Sub-array ``index`` along the first axis."""
+        sub = self.shape[1:]
+        step = int(np.prod(sub)) * self.dtype.itemsize
+        return DeviceView(self.value + index * step, sub, self.dtype, self._owner or self)
 
 
 class Stream:
@@ -320,10 +353,11 @@ class Plan:
         check(self._lib.piv_plan_last_counts(self.handle, k, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
-    def set_predictor(self, net, max_reject_fraction: float):
-        """Attach (or with ``net=None`` detach) a learned predictor network."""
+    def set_predictor(self, net, max_reject_fraction: float, first_pass: int = -1):
+        """Attach (or with ``net=None`` detach) a learned predictor network
+        feeding pass ``first_pass`` (-1: the last pass)."""
         self._net = net  # keep the network alive while the plan uses it
-        check(self._lib.piv_plan_set_predictor(self.handle, None if net is None else net.handle,
+        check(self._lib.piv_plan_set_predictor(self.handle, None if net is None else net.handle, int(first_pass),
                                                float(max_reject_fraction)))
 
     def last_fallback(self):

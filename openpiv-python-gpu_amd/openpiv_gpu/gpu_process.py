@@ -198,7 +198,7 @@ class PIVGPU:
                  mean_tol=MEAN_TOL, rms_tol=RMS_TOL, num_replacing_iters=NUM_REPLACING_ITERS,
                  replacing_method=REPLACING_METHOD, replacing_size=REPLACING_SIZE, revalidate=REVALIDATE,
                  smooth=SMOOTH, smoothing_par=SMOOTHING_PAR, dt=1, scaling_par=1, mask=None, dtype_f="float32",
-                 device=0, use_graph=True, predictor=None, max_reject_fraction=0.05):
+                 device=0, use_graph=True, predictor=None, max_reject_fraction=0.05, predictor_pass=-1):
         if not (dtype_f == "float32" or dtype_f is np.float32):
             raise NotImplementedError("dtype_f: only 'float32' processing is implemented on the MI355X path")
         if mask is not None:
@@ -245,6 +245,7 @@ class PIVGPU:
         # more than max_reject_fraction of the vectors reruns the full schedule
         self.predictor = predictor
         self.max_reject_fraction = float(max_reject_fraction)
+        self.predictor_pass = int(predictor_pass)  # first pass fed by the network (-1: the last)
 
         for i in range(n):
             for name, seq in (("s2n_tol", self.s2n_tol), ("mad_tol", self.mad_tol), ("mean_tol", self.mean_tol),
@@ -327,7 +328,7 @@ class PIVGPU:
         if self._plan is None or self._plan.desc.in_dtype != in_dtype:
             self._plan = _native.Plan(self._plan_desc(in_dtype), self.device)
             if self.predictor is not None:
-                self._plan.set_predictor(self.predictor, self.max_reject_fraction)
+                self._plan.set_predictor(self.predictor, self.max_reject_fraction, self.predictor_pass)
         return self._plan
 
     @property
@@ -448,7 +449,8 @@ class piv_gpu:
     """
 
     def __init__(self, frame_shape, min_search_size, **kwargs):
-        unknown = set(kwargs) - set(_KWARGS) - {"device", "use_graph", "predictor", "max_reject_fraction"}
+        unknown = set(kwargs) - set(_KWARGS) - {"device", "use_graph", "predictor", "max_reject_fraction",
+                                                "predictor_pass"}
         if unknown:
             raise TypeError(f"unexpected keyword arguments: {sorted(unknown)}")
         cfg = dict(_DEFAULTS)

@@ -67,3 +67,23 @@ def test_gpu_rendered_shift_is_recovered():
     piv = PIVGPU(shape, 32, overlap_ratio=0.5, median_tol=2, dtype_f="float32")
     u, v = piv.run_host(fa.get(), fb.get())
     assert np.sqrt(np.mean((u - 2.35) ** 2)) < 0.05 and np.sqrt(np.mean((v + 1.6) ** 2)) < 0.05
+
+
+def test_plan_reads_device_frames_after_null_stream_producers():
+    """Frames rendered on the null stream feed piv_run directly: the plan's
+    own stream must wait for them (first, eager run and graph replays)."""
+    from openpiv_gpu import _native, synthetic as S
+    from openpiv_gpu.gpu_process import PIVGPU
+    shape = (1024, 1024)
+    gen = _gen(shape, S.MODEL_12BIT)
+    piv = PIVGPU(shape, 16, search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2, dtype_f="float32")
+    plan = piv._get_plan(_native.PIV_U16)
+    m, n = piv.piv_fields[-1].field_shape
+    fa, fb = gen.new_frames()
+    for seed in (4100, 4101, 4102):
+        gen.render(S.vortex_field(seed, shape), seed, fa, fb)
+        u = np.empty((m, n), np.float32)
+        v = np.empty((m, n), np.float32)
+        plan.run(fa.value, fb.value, True, u.ctypes.data, v.ctypes.data, None, None, False)
+        uh, vh = piv.run_host(fa.get(), fb.get())
+        assert np.array_equal(u, uh) and np.array_equal(v, vh), seed
