@@ -73,7 +73,7 @@ def stage_kernel_times(fa_u16, piv, reps=20):
 
         def launch():
             _native.check(lib.piv_stage_correlate(src.ptr, src.ptr, code, ht, wd, ctypes.byref(p), None, None,
-                                                  u.ptr, v.ptr, None, None, None, sh))
+                                                  u.ptr, v.ptr, None, None, None, None, sh))
         ms = timed(launch)
         flops = fft_flops_per_window(p.fft_size) * m * n
         out.append({"pass": k, "kernel": f"corr_kernel<F={p.fft_size}>", "windows": m * n, "fft": p.fft_size,

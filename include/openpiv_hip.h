@@ -55,10 +55,25 @@ typedef struct piv_pass_desc {
   float deforming_par;        /* gpu_process.py:1165-1171                */
   int32_t validate;           /* 1 -> median test with median_tol        */
   float median_tol;
-  int32_t num_replacing_iters;/* 0 -> no spring replacement              */
+  int32_t num_replacing_iters;/* 0 -> no replacement (at most 16)       */
   int32_t smooth;             /* smoothn after this pass                 */
   double smoothing_par;       /* <= 0 -> GCV-chosen                      */
+  /* the other validation tests and replacement methods
+   * (gpu_validation.py:84-164, 294-415; gpu_process.py:744-834); a NaN
+   * tolerance (median_tol included) turns that test off */
+  int32_t s2n_method;         /* enum piv_s2n (gpu_process.py:1312-1365)  */
+  int32_t s2n_size;           /* peak2peak exclusion half-width           */
+  float s2n_tol;              /* flag s2n < s2n_tol                       */
+  float mad_tol;              /* flag |f - med| > mad_tol * MAD           */
+  float mean_tol;             /* flag |f - mean| > mean_tol               */
+  float rms_tol;              /* flag |f - mean(f)| > rms_tol * std(f)    */
+  int32_t replacing_method;   /* enum piv_replacing                       */
+  int32_t revalidate;         /* re-test replaced vectors each iteration  */
 } piv_pass_desc;
+
+#define PIV_MAX_REPLACING_ITERS 16
+enum piv_s2n { PIV_PEAK2PEAK = 0, PIV_PEAK2MEAN = 1, PIV_PEAK2ENERGY = 2 };
+enum piv_replacing { PIV_SPRING = 0, PIV_MEDIAN = 1, PIV_MEAN = 2 };
 
 typedef struct piv_plan_desc {
   int32_t height, width;      /* frame shape                             */
@@ -107,14 +122,16 @@ int piv_plan_last_counts(const piv_plan *plan, int pass, int32_t *n_before, int3
  * NULL for the null stream) -------------------------------------------- */
 
 /* Window extraction + normalisation + zero-padded FFT correlation + peak +
- * sub-pixel fit for every window of one pass.  Replaces cuda_get_stack
+ * sub-pixel fit (+ the signal-to-noise ratio of pass->s2n_method into s2n
+ * when s2n is non-NULL, CorrelationGPU.get_s2n_ratio, :1312-1365) for every
+ * window of one pass.  Replaces cuda_get_stack
  * (gpu_process.py:1122-1147, 1420-1449) and CorrelationGPU.__call__'s
  * correlation chain (:1051-1097, 1201-1310).  up/vp (field_m x field_n) are
  * added to the result when non-NULL (:631-632).  i_peak/j_peak/planes are
  * optional (planes: N x F x F fftshift-ed correlation, float32). */
 int piv_stage_correlate(const void *frame_a, const void *frame_b, int in_dtype, int height, int width,
                         const piv_pass_desc *pass, const float *up, const float *vp, float *u, float *v,
-                        int32_t *i_peak, int32_t *j_peak, float *planes, void *stream);
+                        int32_t *i_peak, int32_t *j_peak, float *planes, float *s2n, void *stream);
 
 /* Frame deformation: predictor (field_m x field_n on the pass grid)
  * interpolated onto every pixel with linear extrapolation, then the backward
@@ -135,6 +152,23 @@ int piv_stage_grid_interp(const piv_pass_desc *from, const piv_pass_desc *to, co
  * incremented by the number of flagged vectors. */
 int piv_stage_median_validate(const float *u, const float *v, int m, int n, float median_tol, uint8_t *flags,
                               int32_t *count, void *stream);
+
+/* Every validation test of the pass (median, MAD, mean, RMS; S2N when s2n
+ * is non-NULL), OR-ed.  Replaces ValidationGPU.__call__
+ * (gpu_validation.py:84-164).  only (optional): validate where only[k] != 0
+ * (the complement of the reference's mask argument, :119, 222-226).
+ * count (device int32) is incremented by the number of flags set. */
+int piv_stage_validate(const float *u, const float *v, int m, int n, const float *s2n, const piv_pass_desc *pass,
+                       const uint8_t *only, uint8_t *flags, int32_t *count, void *stream);
+
+/* The replacement loop of one pass on u, v in place: pass->replacing_method,
+ * pass->num_replacing_iters iterations, revalidation with the pass's tests
+ * (s2n optional).  flags: in, the vectors to replace; out, the flags after
+ * the loop (changed only when revalidating).  Replaces
+ * PIVGPU.replace_outliers + ReplacementGPU (gpu_process.py:788-834;
+ * gpu_validation.py:294-415).  Allocates its workspace; synchronises. */
+int piv_stage_replace(float *u, float *v, uint8_t *flags, int m, int n, const piv_pass_desc *pass, const float *s2n,
+                      void *stream);
 
 /* Spring replacement of the flagged vectors in place.  Replaces
  * ReplacementGPU spring path + PIVGPU.replace_outliers

@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,18 @@ namespace {
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
+// Validation / replacement settings of a pass.
+int check_tests(const piv_pass_desc& p) {
+  if (p.s2n_method < PIV_PEAK2PEAK || p.s2n_method > PIV_PEAK2ENERGY) return fail(PIV_E_INVALID, "s2n_method");
+  if (p.s2n_size < 1) return fail(PIV_E_INVALID, "s2n_size must be positive");
+  if (p.replacing_method < PIV_SPRING || p.replacing_method > PIV_MEAN) return fail(PIV_E_INVALID, "replacing_method");
+  if (p.num_replacing_iters < 0 || p.num_replacing_iters > PIV_MAX_REPLACING_ITERS)
+    return fail(PIV_E_INVALID, "num_replacing_iters outside 0..%d", PIV_MAX_REPLACING_ITERS);
+  return PIV_OK;
+}
+
+bool needs_s2n(const piv_pass_desc& p) { return p.validate && !std::isnan(p.s2n_tol); }
+
 int check_pass(const piv_pass_desc& p, int ht, int wd) {
   if (p.search_size < 2 || p.spacing < 1) return fail(PIV_E_INVALID, "search_size/spacing out of range");
   if (!is_pow2(p.fft_size) || p.fft_size < 16 || p.fft_size > 128)
@@ -55,7 +68,7 @@ int check_pass(const piv_pass_desc& p, int ht, int wd) {
       (p.field_n - 1) * p.spacing + p.search_size + p.x_offset > wd)
     return fail(PIV_E_INVALID, "window grid exceeds the frame");
   if (p.subpixel < 0 || p.subpixel > 2) return fail(PIV_E_INVALID, "subpixel method");
-  return PIV_OK;
+  return check_tests(p);
 }
 
 size_t dtype_size(int d) { return d == PIV_U8 ? 1 : (d == PIV_U16 ? 2 : 4); }
@@ -140,6 +153,8 @@ CorrArgs corr_args(const void* fa, const void* fb, int dtype, int ht, int wd, co
   a.n = p.field_n;
   a.normalize = p.normalize;
   a.subpixel = p.subpixel;
+  a.s2n_method = p.s2n_method;
+  a.s2n_size = p.s2n_size;
   return a;
 }
 
@@ -209,8 +224,22 @@ struct PassBuffers {
   float* up = nullptr;
   float* vp = nullptr;
   uint8_t* flags = nullptr;
+  float* s2n = nullptr;         // when the pass has an S2N test
   SmoothTables tab;
   double p_min = 0, p_max = 0;
+};
+
+// Work buffers of the validation / replacement loop (field-sized).
+struct LoopBufs {
+  float* u0 = nullptr;        // originals for the unresolved reset
+  float* v0 = nullptr;
+  float* tu = nullptr;        // inputs of a simultaneous (Jacobi) replacement
+  float* tv = nullptr;
+  uint8_t* wf[2] = {nullptr, nullptr};  // flags after each revalidation (ping-pong)
+  uint8_t* nanf = nullptr;    // unresolved vectors of the last replacement
+  int32_t* it_counts = nullptr;  // PIV_MAX_REPLACING_ITERS counters
+  double* stats = nullptr;    // mean_u, std_u, mean_v, std_v
+  ReplaceWork rw{};
 };
 
 struct piv_plan {
@@ -230,6 +259,7 @@ struct piv_plan {
   float* t1 = nullptr;
   void* smooth_state = nullptr;
   ReplaceWork rw{};
+  LoopBufs lb{};                // shares rw
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int runs = 0;
@@ -262,7 +292,82 @@ int deform_and_correlate(piv_plan* P, PassBuffers& B, hipStream_t s) {
   a.vp = B.vp;
   a.u = B.u;
   a.v = B.v;
+  a.s2n = B.s2n;
   HIP_TRY(launch_correlate(a, s));
+  return PIV_OK;
+}
+
+ValidateArgs validate_args(const piv_pass_desc& d, const float* u, const float* v, int m, int n, const float* s2n,
+                           const double* stats, const uint8_t* only, const uint8_t* extra, uint8_t* flags,
+                           int32_t* count) {
+  ValidateArgs a{};
+  a.u = u;
+  a.v = v;
+  a.m = m;
+  a.n = n;
+  a.s2n = s2n;
+  a.s2n_tol = s2n ? d.s2n_tol : NAN;
+  a.median_tol = d.median_tol;
+  a.mad_tol = d.mad_tol;
+  a.mean_tol = d.mean_tol;
+  a.rms_tol = d.rms_tol;
+  a.rms = stats;
+  a.only = only;
+  a.extra = extra;
+  a.flags = flags;
+  a.count = count;
+  return a;
+}
+
+// ValidationGPU.__call__ (gpu_validation.py:84-118): every enabled test.
+int run_validate(const piv_pass_desc& d, const float* u, const float* v, int m, int n, const float* s2n,
+                 double* stats, const uint8_t* only, const uint8_t* extra, uint8_t* flags, int32_t* count,
+                 hipStream_t s) {
+  if (!std::isnan(d.rms_tol)) HIP_TRY(launch_field_stats(u, v, m * n, stats, s));
+  HIP_TRY(launch_validate(validate_args(d, u, v, m, n, s2n, stats, only, extra, flags, count), s));
+  return PIV_OK;
+}
+
+// PIVGPU.replace_outliers (gpu_process.py:788-834) on the vectors flagged in
+// flags0 (count0 on the device): num_replacing_iters iterations of the
+// pass's method; iteration 1 leaves the flagged neighbours out, later ones
+// use them unless revalidating (:824-826); revalidation resets unresolved
+// vectors and re-tests only the replaced ones (:809-822); without it the
+// unresolved ones are reset at the end (:829-832).  it_counts must be zero.
+// No host synchronisation: an empty flag set turns every step into a no-op.
+int replace_loop(float* u, float* v, const uint8_t* flags0, int32_t* count0, int m, int n, const piv_pass_desc& d,
+                 const float* s2n, LoopBufs& L, hipStream_t s) {
+  const int N = m * n;
+  const int iters = d.num_replacing_iters;
+  if (iters <= 0) return PIV_OK;
+  const bool spring = d.replacing_method == PIV_SPRING;
+  const bool reval = d.revalidate != 0;
+  if (!spring || reval) {
+    HIP_TRY(launch_copy_f32(L.u0, u, N, s));
+    HIP_TRY(launch_copy_f32(L.v0, v, N, s));
+  }
+  const uint8_t* cur = flags0;
+  int32_t* cur_count = count0;
+  for (int it = 0; it < iters; ++it) {
+    if (spring) {
+      HIP_TRY(launch_spring_replace(u, v, cur, cur_count, m, n, L.rw, s));
+    } else {
+      HIP_TRY(launch_copy_f32(L.tu, u, N, s));
+      HIP_TRY(launch_copy_f32(L.tv, v, N, s));
+      HIP_TRY(launch_neighbour_replace(L.tu, L.tv, u, v, cur, m, n, d.replacing_method, (it == 0 || reval) ? 1 : 0,
+                                       s));
+    }
+    if (reval) {
+      HIP_TRY(launch_reset_unresolved(u, v, L.u0, L.v0, cur, L.nanf, N, s));
+      uint8_t* nxt = L.wf[it & 1];
+      int32_t* nxt_count = L.it_counts + it;
+      int r = run_validate(d, u, v, m, n, s2n, L.stats, cur, L.nanf, nxt, nxt_count, s);
+      if (r) return r;
+      cur = nxt;
+      cur_count = nxt_count;
+    }
+  }
+  if (!reval && !spring) HIP_TRY(launch_reset_unresolved(u, v, L.u0, L.v0, cur, nullptr, N, s));
   return PIV_OK;
 }
 
@@ -270,16 +375,14 @@ int deform_and_correlate(piv_plan* P, PassBuffers& B, hipStream_t s) {
 // (gpu_process.py:635-639, 744-845).
 int validate_and_smooth(piv_plan* P, int k, bool last, hipStream_t s) {
   PassBuffers& B = P->passes[k];
+  const int m = B.d.field_m, n = B.d.field_n;
   if (B.d.validate) {
-    HIP_TRY(launch_median_validate(B.u, B.v, B.d.field_m, B.d.field_n, B.d.median_tol, B.flags, P->counters + 2 * k,
-                                   s));
-    if (B.d.num_replacing_iters > 0) {
-      HIP_TRY(launch_spring_replace(B.u, B.v, B.flags, P->counters + 2 * k, B.d.field_m, B.d.field_n, P->rw, s));
-    }
-    if (last) {
-      HIP_TRY(launch_median_validate(B.u, B.v, B.d.field_m, B.d.field_n, B.d.median_tol, P->val_final,
-                                     P->counters + 2 * k + 1, s));
-    }
+    int r = run_validate(B.d, B.u, B.v, m, n, B.s2n, P->lb.stats, nullptr, nullptr, B.flags, P->counters + 2 * k, s);
+    if (!r) r = replace_loop(B.u, B.v, B.flags, P->counters + 2 * k, m, n, B.d, B.s2n, P->lb, s);
+    if (!r && last)
+      r = run_validate(B.d, B.u, B.v, m, n, B.s2n, P->lb.stats, nullptr, nullptr, P->val_final,
+                       P->counters + 2 * k + 1, s);
+    if (r) return r;
   }
   if (!last && B.d.smooth) {
     SmoothnWork w{};
@@ -324,6 +427,7 @@ int enqueue_passes(piv_plan* P, int first) {
       CorrArgs a = corr_args(P->frame_a, P->frame_b, D.in_dtype, D.height, D.width, B.d);
       a.u = B.u;
       a.v = B.v;
+      a.s2n = B.s2n;
       HIP_TRY(launch_correlate(a, s));
     } else {
       if (k > first) r = predictor_from_previous(P, k, s);
@@ -341,6 +445,7 @@ int enqueue_passes(piv_plan* P, int first) {
 
 int enqueue(piv_plan* P) {
   HIP_TRY(launch_fill_i32(P->counters, 0, 2 * (int)P->passes.size(), P->stream));
+  HIP_TRY(launch_fill_i32(P->lb.it_counts, 0, PIV_MAX_REPLACING_ITERS, P->stream));
   return enqueue_passes(P, 0);
 }
 
@@ -351,6 +456,7 @@ int enqueue_learned(piv_plan* P) {
   const piv_plan_desc& D = P->desc;
   PassBuffers& B = P->passes[P->net_first];
   HIP_TRY(launch_fill_i32(P->counters, 0, 2 * (int)P->passes.size(), s));
+  HIP_TRY(launch_fill_i32(P->lb.it_counts, 0, PIV_MAX_REPLACING_ITERS, s));
   HIP_TRY(launch_prepare_input(P->frame_a, P->frame_b, D.in_dtype, D.height, D.width, kCnnFactor, P->net_scratch,
                                P->net_in, s));
   int r = cnn_enqueue_forward(P->net, P->net_in, 1, s);
@@ -405,10 +511,13 @@ void release(piv_plan* P) {
   if (P->net_in) (void)hipFree(P->net_in);
   if (P->net_scratch) (void)hipFree(P->net_scratch);
   for (auto& B : P->passes) {
-    for (void* p : {(void*)B.u, (void*)B.v, (void*)B.up, (void*)B.vp, (void*)B.flags})
+    for (void* p : {(void*)B.u, (void*)B.v, (void*)B.up, (void*)B.vp, (void*)B.flags, (void*)B.s2n})
       if (p) (void)hipFree(p);
     free_tables(&B.tab);
   }
+  for (void* p : {(void*)P->lb.u0, (void*)P->lb.v0, (void*)P->lb.tu, (void*)P->lb.tv, (void*)P->lb.wf[0],
+                  (void*)P->lb.wf[1], (void*)P->lb.nanf, (void*)P->lb.it_counts, (void*)P->lb.stats})
+    if (p) (void)hipFree(p);
   for (void* p : {P->frame_a, P->frame_b, (void*)P->def_a, (void*)P->def_b, (void*)P->counters, (void*)P->p_out,
                   (void*)P->val_final, (void*)P->t0, (void*)P->t1, (void*)P->rw.slot, (void*)P->rw.list,
                   (void*)P->rw.work, P->smooth_state})
@@ -478,6 +587,7 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
     if (B.N > maxN) maxN = B.N;
     ok = dmalloc((void**)&B.u, B.N * 4) && dmalloc((void**)&B.v, B.N * 4) && dmalloc((void**)&B.up, B.N * 4) &&
          dmalloc((void**)&B.vp, B.N * 4) && dmalloc((void**)&B.flags, B.N);
+    if (ok && needs_s2n(B.d)) ok = dmalloc((void**)&B.s2n, B.N * 4);
     if (!ok) return bail(fail(PIV_E_NOMEM, "device allocation failed"));
     if (hipMemset(B.flags, 0, B.N) != hipSuccess) return bail(fail(PIV_E_HIP, "hipMemset failed"));
     if (k < D.n_passes - 1 && B.d.smooth) {
@@ -491,6 +601,14 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
        dmalloc((void**)&P->t1, 2ull * maxN * 4) && dmalloc((void**)&P->rw.slot, maxN * 4) &&
        dmalloc((void**)&P->rw.list, maxN * 4) && dmalloc((void**)&P->rw.work, 8ull * maxN * 8);
   ok = ok && dmalloc(&P->smooth_state, smoothn_state_bytes());
+  {
+    LoopBufs& lb = P->lb;
+    ok = ok && dmalloc((void**)&lb.u0, maxN * 4) && dmalloc((void**)&lb.v0, maxN * 4) &&
+         dmalloc((void**)&lb.tu, maxN * 4) && dmalloc((void**)&lb.tv, maxN * 4) && dmalloc((void**)&lb.wf[0], maxN) &&
+         dmalloc((void**)&lb.wf[1], maxN) && dmalloc((void**)&lb.nanf, maxN) &&
+         dmalloc((void**)&lb.it_counts, PIV_MAX_REPLACING_ITERS * 4) && dmalloc((void**)&lb.stats, 4 * 8);
+    lb.rw = P->rw;
+  }
   if (!ok) return bail(fail(PIV_E_NOMEM, "device allocation failed"));
   if (hipMemset(P->val_final, 0, L.N) != hipSuccess || hipMemset(P->smooth_state, 0, smoothn_state_bytes()) != hipSuccess)
     return bail(fail(PIV_E_HIP, "hipMemset failed"));
@@ -609,7 +727,7 @@ int piv_plan_last_counts(const piv_plan* P, int pass, int32_t* n_before, int32_t
 
 int piv_stage_correlate(const void* frame_a, const void* frame_b, int in_dtype, int height, int width,
                         const piv_pass_desc* pass, const float* up, const float* vp, float* u, float* v,
-                        int32_t* i_peak, int32_t* j_peak, float* planes, void* stream) {
+                        int32_t* i_peak, int32_t* j_peak, float* planes, float* s2n, void* stream) {
   if (!frame_a || !frame_b || !pass || !u || !v) return fail(PIV_E_INVALID, "null argument");
   if (in_dtype < PIV_U8 || in_dtype > PIV_F32) return fail(PIV_E_INVALID, "in_dtype");
   int r = check_pass(*pass, height, width);
@@ -622,6 +740,7 @@ int piv_stage_correlate(const void* frame_a, const void* frame_b, int in_dtype, 
   a.i_peak = i_peak;
   a.j_peak = j_peak;
   a.planes = planes;
+  a.s2n = s2n;
   HIP_TRY(launch_correlate(a, (hipStream_t)stream));
   return PIV_OK;
 }
@@ -651,6 +770,73 @@ int piv_stage_median_validate(const float* u, const float* v, int m, int n, floa
   if (!u || !v || !flags || !count || m < 1 || n < 1) return fail(PIV_E_INVALID, "bad argument");
   HIP_TRY(launch_median_validate(u, v, m, n, median_tol, flags, count, (hipStream_t)stream));
   return PIV_OK;
+}
+
+int piv_stage_validate(const float* u, const float* v, int m, int n, const float* s2n, const piv_pass_desc* pass,
+                       const uint8_t* only, uint8_t* flags, int32_t* count, void* stream) {
+  if (!u || !v || !pass || !flags || !count || m < 1 || n < 1) return fail(PIV_E_INVALID, "bad argument");
+  int r = check_tests(*pass);
+  if (r) return r;
+  double* stats = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  if (!std::isnan(pass->rms_tol)) HIP_TRY(hipMalloc((void**)&stats, 4 * sizeof(double)));
+  r = run_validate(*pass, u, v, m, n, s2n, stats, only, nullptr, flags, count, s);
+  if (stats) {
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(stats);
+  }
+  return r;
+}
+
+int piv_stage_replace(float* u, float* v, uint8_t* flags, int m, int n, const piv_pass_desc* pass, const float* s2n,
+                      void* stream) {
+  if (!u || !v || !flags || !pass || m < 1 || n < 1) return fail(PIV_E_INVALID, "bad argument");
+  int r = check_tests(*pass);
+  if (r) return r;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t N = (size_t)m * n;
+  LoopBufs L{};
+  int32_t* count0 = nullptr;
+  std::vector<void*> owned;
+  auto grab = [&](void** p, size_t bytes) -> bool {
+    if (hipMalloc(p, bytes) != hipSuccess) return false;
+    owned.push_back(*p);
+    return true;
+  };
+  bool ok = grab((void**)&L.u0, N * 4) && grab((void**)&L.v0, N * 4) && grab((void**)&L.tu, N * 4) &&
+            grab((void**)&L.tv, N * 4) && grab((void**)&L.wf[0], N) && grab((void**)&L.wf[1], N) &&
+            grab((void**)&L.nanf, N) && grab((void**)&L.it_counts, PIV_MAX_REPLACING_ITERS * 4) &&
+            grab((void**)&L.stats, 32) && grab((void**)&L.rw.slot, N * 4) && grab((void**)&L.rw.list, N * 4) &&
+            grab((void**)&L.rw.work, 8 * N * 8) && grab((void**)&count0, 4);
+  auto cleanup = [&] {
+    (void)hipStreamSynchronize(s);
+    for (void* p : owned) (void)hipFree(p);
+  };
+  if (!ok) {
+    cleanup();
+    return fail(PIV_E_NOMEM, "device allocation failed");
+  }
+  hipError_t e = launch_fill_i32(L.it_counts, 0, PIV_MAX_REPLACING_ITERS, s);
+  if (e == hipSuccess) e = launch_fill_i32(count0, 0, 1, s);
+  // the count of the incoming flags (the validation kernel's counter)
+  if (e == hipSuccess) {
+    piv_pass_desc none = *pass;
+    none.median_tol = none.mad_tol = none.mean_tol = none.rms_tol = none.s2n_tol = NAN;
+    e = launch_validate(validate_args(none, u, v, m, n, nullptr, nullptr, nullptr, flags, L.wf[1], count0), s);
+  }
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(PIV_E_HIP, "replace: %s", hipGetErrorString(e));
+  }
+  r = replace_loop(u, v, L.wf[1], count0, m, n, *pass, s2n, L, s);
+  if (!r && pass->revalidate && pass->num_replacing_iters > 0) {
+    // final flags of the loop: the last revalidation's output
+    const uint8_t* last = L.wf[(pass->num_replacing_iters - 1) & 1];
+    e = hipMemcpyAsync(flags, last, N, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) r = fail(PIV_E_HIP, "replace: %s", hipGetErrorString(e));
+  }
+  cleanup();
+  return r;
 }
 
 int piv_stage_spring_replace(float* u, float* v, const uint8_t* flags, const int32_t* count, int m, int n,

@@ -81,6 +81,7 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a,
   float* smem = reinterpret_cast<float*>(smem4);
   __shared__ float red_f[8][2];
   __shared__ Best red_b[8];
+  __shared__ Best red_s[8];
 
   const int tid = threadIdx.x;
   const int wl = tid / LANES;  // window slot in this workgroup
@@ -404,13 +405,88 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a,
     if ((tid & 63) == 0) red_b[wave] = best;
   }
   __syncthreads();
-  if (t == 0 && valid) {
-    if constexpr (LANES > 64) {
-      constexpr int WPW = LANES / 64;
-      const int w0 = (tid / LANES) * WPW;
-      best = red_b[w0];
-      for (int k = 1; k < WPW; ++k) best = better(best, red_b[w0 + k]);
+  if constexpr (LANES > 64) {  // every lane of the window learns the first peak
+    constexpr int WPW = LANES / 64;
+    const int w0 = (tid / LANES) * WPW;
+    best = red_b[w0];
+    for (int k = 1; k < WPW; ++k) best = better(best, red_b[w0 + k]);
+  }
+
+  // ---- signal-to-noise ratio (CorrelationGPU.get_s2n_ratio,
+  // gpu_process.py:1312-1365) on the same fftshift-ed plane ---------------
+  float s2n = 0.f;
+  if (a.s2n) {  // uniform over the launch
+    const float c1 = best.v;
+    const int ip1 = best.i / F, jp1 = best.i % F;
+    float c2;
+    if (a.s2n_method == 0) {
+      // second peak: first-occurrence argmax outside the (2w+1)^2 block
+      // around the first peak, the block wrapping around the plane edges
+      // (fill_kernel through CuPy's wrapping fancy indices, gpu_misc.py:52-85)
+      const int w = a.s2n_size;
+      Best b2{-INFINITY, 0x7fffffff};
+      for (int idx = t; idx < F * F; idx += LANES) {
+        const int i = idx / F, j = idx % F;
+        int di = i - ip1, dj = j - jp1;
+        di = di < 0 ? -di : di;
+        dj = dj < 0 ? -dj : dj;
+        di = di < F - di ? di : F - di;
+        dj = dj < F - dj ? dj : F - dj;
+        if (di <= w && dj <= w) continue;
+        b2 = better(b2, Best{plane[i * SP + j], idx});
+      }
+#pragma unroll
+      for (int off = WIDTH / 2; off > 0; off >>= 1) {
+        Best o{__shfl_xor(b2.v, off, 64), __shfl_xor(b2.i, off, 64)};
+        b2 = better(b2, o);
+      }
+      if constexpr (LANES > 64) {
+        if ((tid & 63) == 0) red_s[wave] = b2;
+        __syncthreads();
+        constexpr int WPW = LANES / 64;
+        const int w0 = (tid / LANES) * WPW;
+        b2 = red_s[w0];
+        for (int k = 1; k < WPW; ++k) b2 = better(b2, red_s[w0 + k]);
+      }
+      c2 = b2.i == 0x7fffffff ? -INFINITY : b2.v;
+    } else {
+      // mean of the positive values (peak2energy), of those <= peak / 2 (peak2mean)
+      const float lim = 0.5f * c1;
+      float sum = 0.f;
+      int cnt = 0;
+      for (int idx = t; idx < F * F; idx += LANES) {
+        const float val = plane[(idx / F) * SP + idx % F];
+        if (val > 0.f && (a.s2n_method == 2 || val <= lim)) {
+          sum += val;
+          ++cnt;
+        }
+      }
+#pragma unroll
+      for (int off = WIDTH / 2; off > 0; off >>= 1) {
+        sum += __shfl_xor(sum, off, 64);
+        cnt += __shfl_xor(cnt, off, 64);
+      }
+      if constexpr (LANES > 64) {
+        if ((tid & 63) == 0) red_s[wave] = Best{sum, cnt};
+        __syncthreads();
+        constexpr int WPW = LANES / 64;
+        const int w0 = (tid / LANES) * WPW;
+        sum = 0.f;
+        cnt = 0;
+        for (int k = 0; k < WPW; ++k) {
+          sum += red_s[w0 + k].v;
+          cnt += red_s[w0 + k].i;
+        }
+      }
+      c2 = cnt > 0 ? sum / (float)cnt : NAN;
     }
+    // :1329-1333 -- 0 where the first peak is not positive, inf where the
+    // second is not, NaN when the mean is empty
+    s2n = c1 > 0.f ? (c2 <= 0.f ? INFINITY : c1 / c2) : 0.f;
+  }
+
+  if (t == 0 && valid) {
+    if (a.s2n) a.s2n[win] = s2n;
     const int ip = best.i / F;
     const int jp = best.i % F;
     float isp = (float)ip, jsp = (float)jp;

@@ -596,6 +596,160 @@ __global__ void copy_u8_kernel(uint8_t* dst, const uint8_t* src, int count) {
 
 thread_local char g_detail[256] = "";
 
+
+// ---------------------------------------------------------------------------
+// Full validation and median/mean replacement (ValidationGPU.__call__,
+// gpu_validation.py:84-164; ReplacementGPU, :294-415).
+// ---------------------------------------------------------------------------
+
+// Gathers the up-to-8 in-bounds, non-NaN 3x3 neighbours of (i, j) of f
+// (optionally leaving out flagged ones) into s[0..cnt), +inf after.
+__device__ __forceinline__ int gather8(const float* f, int m, int n, int i, int j, const uint8_t* skip, float (&s)[8]) {
+  int c = 0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) s[q] = INFINITY;
+#pragma unroll
+  for (int di = -1; di <= 1; ++di) {
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj) {
+      if (di == 0 && dj == 0) continue;
+      const int ii = i + di, jj = j + dj;
+      if (ii < 0 || ii >= m || jj < 0 || jj >= n) continue;
+      const int idx = ii * n + jj;
+      if (skip && skip[idx]) continue;
+      const float a = f[idx];
+      if (isnan(a)) continue;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) s[q] = (q == c) ? a : s[q];
+      ++c;
+    }
+  }
+  return c;
+}
+
+// Median, MAD and mean of the gathered neighbours (NaN when there are none).
+__device__ __forceinline__ void stats8(float (&s)[8], int cnt, float* med, float* mad, float* mean) {
+  float sum = 0.f;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) sum += (q < cnt) ? s[q] : 0.f;
+  *mean = cnt > 0 ? sum / (float)cnt : NAN;
+  const float md = median8(s, cnt);
+  *med = md;
+  if (mad) {
+    float d[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) d[q] = (q < cnt) ? fabsf(s[q] - md) : INFINITY;
+    *mad = median8(d, cnt);
+  }
+}
+
+__global__ __launch_bounds__(256) void validate_kernel(ValidateArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  bool flag = false;
+  if (k < a.m * a.n) {
+    if (!a.only || a.only[k]) {
+      const int i = k / a.n, j = k % a.n;
+      const float uk = a.u[k], vk = a.v[k];
+      if (a.s2n && !isnan(a.s2n_tol)) flag = flag || (a.s2n[k] < a.s2n_tol);
+      const bool nb = !isnan(a.median_tol) || !isnan(a.mad_tol) || !isnan(a.mean_tol);
+      if (nb) {
+        float s[8];
+        float medu, madu, meanu, medv, madv, meanv;
+        int c = gather8(a.u, a.m, a.n, i, j, nullptr, s);
+        stats8(s, c, &medu, isnan(a.mad_tol) ? nullptr : &madu, &meanu);
+        c = gather8(a.v, a.m, a.n, i, j, nullptr, s);
+        stats8(s, c, &medv, isnan(a.mad_tol) ? nullptr : &madv, &meanv);
+        if (!isnan(a.median_tol))
+          flag = flag || (fabsf(uk - medu) > a.median_tol) || (fabsf(vk - medv) > a.median_tol);
+        if (!isnan(a.mad_tol))
+          flag = flag || (fabsf(uk - medu) > a.mad_tol * madu) || (fabsf(vk - medv) > a.mad_tol * madv);
+        if (!isnan(a.mean_tol))
+          flag = flag || (fabsf(uk - meanu) > a.mean_tol) || (fabsf(vk - meanv) > a.mean_tol);
+      }
+      if (!isnan(a.rms_tol) && a.rms) {
+        flag = flag || (fabs((double)uk - a.rms[0]) > (double)a.rms_tol * a.rms[1]) ||
+               (fabs((double)vk - a.rms[2]) > (double)a.rms_tol * a.rms[3]);
+      }
+    }
+    if (a.extra && a.extra[k]) flag = true;
+    a.flags[k] = flag ? 1 : 0;
+  }
+  const unsigned long long b = __ballot(flag);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(a.count, (int32_t)__popcll(b));
+}
+
+// One 1024-thread block: NaN-mean and population std of u and v in FP64,
+// two passes (mean, then sum of squared deviations).
+__global__ __launch_bounds__(1024) void field_stats_kernel(const float* u, const float* v, int count,
+                                                           double* stats) {
+  __shared__ double red[2][32];
+  __shared__ double mean[2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double s[2] = {0.0, 0.0}, c[2] = {0.0, 0.0};
+  for (int k = tid; k < count; k += 1024) {
+    const float a = u[k], b = v[k];
+    if (!isnan(a)) { s[0] += a; c[0] += 1.0; }
+    if (!isnan(b)) { s[1] += b; c[1] += 1.0; }
+  }
+  auto block_sum = [&](double x, int slot) -> double {
+    x = wave_sum(x);
+    if (lane == 0) red[slot][wave] = x;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t += red[slot][w];
+    __syncthreads();
+    return t;
+  };
+  const double su = block_sum(s[0], 0), sv = block_sum(s[1], 1);
+  const double cu = block_sum(c[0], 0), cv = block_sum(c[1], 1);
+  if (tid == 0) {
+    mean[0] = cu > 0 ? su / cu : NAN;
+    mean[1] = cv > 0 ? sv / cv : NAN;
+  }
+  __syncthreads();
+  double q[2] = {0.0, 0.0};
+  for (int k = tid; k < count; k += 1024) {
+    const float a = u[k], b = v[k];
+    if (!isnan(a)) q[0] += ((double)a - mean[0]) * ((double)a - mean[0]);
+    if (!isnan(b)) q[1] += ((double)b - mean[1]) * ((double)b - mean[1]);
+  }
+  const double qu = block_sum(q[0], 0), qv = block_sum(q[1], 1);
+  if (tid == 0) {
+    stats[0] = mean[0];
+    stats[1] = cu > 0 ? sqrt(qu / cu) : NAN;
+    stats[2] = mean[1];
+    stats[3] = cv > 0 ? sqrt(qv / cv) : NAN;
+  }
+}
+
+__global__ __launch_bounds__(256) void neighbour_replace_kernel(const float* u_in, const float* v_in, float* u,
+                                                                float* v, const uint8_t* flags, int m, int n,
+                                                                int method, int exclude) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m * n || !flags[k]) return;
+  const int i = k / n, j = k % n;
+  float s[8];
+  float med, mean;
+  int c = gather8(u_in, m, n, i, j, exclude ? flags : nullptr, s);
+  stats8(s, c, &med, nullptr, &mean);
+  u[k] = method == 1 ? med : mean;
+  c = gather8(v_in, m, n, i, j, exclude ? flags : nullptr, s);
+  stats8(s, c, &med, nullptr, &mean);
+  v[k] = method == 1 ? med : mean;
+}
+
+__global__ __launch_bounds__(256) void reset_unresolved_kernel(float* u, float* v, const float* u0, const float* v0,
+                                                               const uint8_t* flags, uint8_t* nan_flags, int count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const bool bad = flags[k] && (isnan(u[k]) || isnan(v[k]));
+  if (bad) {
+    u[k] = u0[k];
+    v[k] = v0[k];
+  }
+  if (nan_flags) nan_flags[k] = bad ? 1 : 0;
+}
+
 }  // namespace
 
 const char* launch_detail() { return g_detail; }
@@ -627,6 +781,32 @@ hipError_t launch_median_validate(const float* u, const float* v, int m, int n, 
   const int total = m * n;
   hipLaunchKernelGGL(median_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, u, v, m, n, tol, flags,
                      count);
+  return hipGetLastError();
+}
+
+hipError_t launch_validate(const ValidateArgs& a, hipStream_t stream) {
+  const int total = a.m * a.n;
+  hipLaunchKernelGGL(validate_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_field_stats(const float* u, const float* v, int count, double* stats, hipStream_t stream) {
+  hipLaunchKernelGGL(field_stats_kernel, dim3(1), dim3(1024), 0, stream, u, v, count, stats);
+  return hipGetLastError();
+}
+
+hipError_t launch_neighbour_replace(const float* u_in, const float* v_in, float* u, float* v, const uint8_t* flags,
+                                    int m, int n, int method, int exclude, hipStream_t stream) {
+  const int total = m * n;
+  hipLaunchKernelGGL(neighbour_replace_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, u_in, v_in, u, v,
+                     flags, m, n, method, exclude);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_unresolved(float* u, float* v, const float* u0, const float* v0, const uint8_t* flags,
+                                   uint8_t* nan_flags, int count, hipStream_t stream) {
+  hipLaunchKernelGGL(reset_unresolved_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, u, v, u0, v0, flags,
+                     nan_flags, count);
   return hipGetLastError();
 }
 

@@ -30,6 +30,9 @@ struct CorrArgs {
   int32_t* i_peak;         // optional integer peak row (fftshift-ed plane)
   int32_t* j_peak;         // optional integer peak column
   float* planes;           // optional N x F x F dump of the fftshift-ed planes
+  int s2n_method;          // 0 peak2peak, 1 peak2mean, 2 peak2energy (gpu_process.py:1312-1365)
+  int s2n_size;            // peak2peak exclusion half-width
+  float* s2n;              // per-window signal-to-noise ratio, or null (not computed)
 };
 
 hipError_t launch_correlate(const CorrArgs& a, hipStream_t stream);
@@ -71,6 +74,39 @@ hipError_t launch_grid_interp(const GridInterpArgs& a, hipStream_t stream);
 // 3x3 NaN-aware neighbour-median test on u and v (flags OR-ed).
 hipError_t launch_median_validate(const float* u, const float* v, int m, int n, float tol, uint8_t* flags,
                                   int32_t* count, hipStream_t stream);
+
+// Every validation test of ValidationGPU (gpu_validation.py:84-164), OR-ed:
+// s2n < s2n_tol; |f - med| > median_tol; |f - med| > mad_tol * mad;
+// |f - mean| > mean_tol over the 3x3 neighbourhood (centre excluded); and
+// |f - mean(f)| > rms_tol * std(f) with the field statistics in rms
+// (mean_u, std_u, mean_v, std_v, FP64).  A NaN tolerance turns a test off.
+// only: validate where only[k] != 0 (others 0); extra: OR-ed in.
+struct ValidateArgs {
+  const float* u;
+  const float* v;
+  int m, n;
+  const float* s2n;
+  float s2n_tol, median_tol, mad_tol, mean_tol, rms_tol;
+  const double* rms;
+  const uint8_t* only;
+  const uint8_t* extra;
+  uint8_t* flags;
+  int32_t* count;
+};
+hipError_t launch_validate(const ValidateArgs& a, hipStream_t stream);
+// Field mean and population standard deviation of u and v (two-pass, FP64).
+hipError_t launch_field_stats(const float* u, const float* v, int count, double* stats, hipStream_t stream);
+
+// Median (method 1) or mean (2) replacement of the flagged vectors from the
+// 3x3 neighbourhood of (u_in, v_in) (ReplacementGPU, gpu_validation.py:
+// 373-398); exclude: flagged neighbours left out (first iteration).  Writes
+// u, v at flagged positions only; NaN where no neighbour is usable.
+hipError_t launch_neighbour_replace(const float* u_in, const float* v_in, float* u, float* v, const uint8_t* flags,
+                                    int m, int n, int method, int exclude, hipStream_t stream);
+// Flagged vectors that came out NaN go back to (u0, v0) (ReplacementGPU.reset,
+// gpu_validation.py:401-415); nan_flags[k] = 1 for them, else 0.
+hipError_t launch_reset_unresolved(float* u, float* v, const float* u0, const float* v0, const uint8_t* flags,
+                                   uint8_t* nan_flags, int count, hipStream_t stream);
 
 // Spring (Laplace) replacement of flagged vectors, solved on the device.
 struct ReplaceWork {

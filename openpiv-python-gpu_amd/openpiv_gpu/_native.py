@@ -15,6 +15,9 @@ import numpy as np
 MAX_PASSES = 16
 PIV_U8, PIV_U16, PIV_F32 = 0, 1, 2
 SUBPIXEL_CODES = {"gaussian": 0, "parabolic": 1, "centroid": 2}
+S2N_CODES = {"peak2peak": 0, "peak2mean": 1, "peak2energy": 2}
+REPLACING_CODES = {"spring": 0, "median": 1, "mean": 2}
+MAX_REPLACING_ITERS = 16
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("PIV_HIP_LIB", os.path.join(_HERE, "_lib", "libopenpiv_hip.so"))
@@ -23,6 +26,7 @@ LIB_PATH = os.environ.get("PIV_HIP_LIB", os.path.join(_HERE, "_lib", "libopenpiv
 EXPORTED_SYMBOLS = (
     "piv_last_error", "piv_device_count", "piv_plan_create", "piv_plan_destroy", "piv_run",
     "piv_plan_last_smoothing", "piv_plan_last_counts", "piv_stage_correlate", "piv_stage_deform",
+    "piv_stage_validate", "piv_stage_replace",
     "piv_stage_grid_interp", "piv_stage_median_validate", "piv_stage_spring_replace", "piv_stage_smoothn",
     "piv_set_device", "piv_malloc", "piv_free", "piv_memcpy", "piv_memset", "piv_synchronize",
     "piv_stream_create", "piv_stream_destroy", "piv_stream_synchronize", "piv_event_create", "piv_event_destroy",
@@ -53,6 +57,14 @@ class PassDesc(ctypes.Structure):
         ("num_replacing_iters", ctypes.c_int32),
         ("smooth", ctypes.c_int32),
         ("smoothing_par", ctypes.c_double),
+        ("s2n_method", ctypes.c_int32),
+        ("s2n_size", ctypes.c_int32),
+        ("s2n_tol", ctypes.c_float),
+        ("mad_tol", ctypes.c_float),
+        ("mean_tol", ctypes.c_float),
+        ("rms_tol", ctypes.c_float),
+        ("replacing_method", ctypes.c_int32),
+        ("revalidate", ctypes.c_int32),
     ]
 
 
@@ -116,7 +128,9 @@ def _bind(lib):
         "piv_plan_last_smoothing": (i32, [vp, i32, ctypes.POINTER(f64)]),
         "piv_plan_last_counts": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
         "piv_stage_correlate": (i32, [vp, vp, i32, i32, i32, ctypes.POINTER(PassDesc), vp, vp, vp, vp, vp, vp, vp,
-                                      vp]),
+                                      vp, vp]),
+        "piv_stage_validate": (i32, [vp, vp, i32, i32, vp, ctypes.POINTER(PassDesc), vp, vp, vp, vp]),
+        "piv_stage_replace": (i32, [vp, vp, vp, i32, i32, ctypes.POINTER(PassDesc), vp, vp]),
         "piv_stage_deform": (i32, [vp, vp, i32, i32, i32, ctypes.POINTER(PassDesc), vp, vp, vp, vp, vp]),
         "piv_stage_grid_interp": (i32, [ctypes.POINTER(PassDesc), ctypes.POINTER(PassDesc), vp, vp, vp, vp, vp]),
         "piv_stage_median_validate": (i32, [vp, vp, i32, i32, f32, vp, vp, vp]),

@@ -248,18 +248,10 @@ class PIVGPU:
         self.predictor_pass = int(predictor_pass)  # first pass fed by the network (-1: the last)
 
         for i in range(n):
-            for name, seq in (("s2n_tol", self.s2n_tol), ("mad_tol", self.mad_tol), ("mean_tol", self.mean_tol),
-                              ("rms_tol", self.rms_tol)):
-                if seq[i] is not None:
-                    raise NotImplementedError(f"{name}: only median validation is implemented on the MI355X path")
             if self.validation_size[i] != 1:
                 raise NotImplementedError("validation_size: only 1 (3x3 kernels) is implemented")
-            if self.replacing_method[i] != "spring" and self.num_replacing_iters[i] > 0:
-                raise NotImplementedError("replacing_method: only 'spring' replacement is implemented")
             if self.replacing_size[i] != 1:
                 raise NotImplementedError("replacing_size: only 1 is implemented")
-            if self.is_revalidated[i]:
-                raise NotImplementedError("revalidate: only revalidate=False is implemented")
 
         self.piv_fields = None
         self.init_val_locations = self.val_locations = None
@@ -309,9 +301,21 @@ class PIVGPU:
             p.normalize = int(bool(self.is_normalized[i]) or fft != f.search_size)   # gpu_process.py:1065
             p.subpixel = _native.SUBPIXEL_CODES[self.subpixel_method[i]]
             p.deforming_par = float(self.deforming_par[i])
-            p.validate = int(self.median_tol[i] is not None)
-            p.median_tol = float(self.median_tol[i]) if self.median_tol[i] is not None else 0.0
+            tol = lambda t: float(t) if t is not None else float("nan")
+            p.median_tol = tol(self.median_tol[i])
+            p.s2n_tol = tol(self.s2n_tol[i])
+            p.mad_tol = tol(self.mad_tol[i])
+            p.mean_tol = tol(self.mean_tol[i])
+            p.rms_tol = tol(self.rms_tol[i])
+            p.validate = int(any(t is not None for t in (self.median_tol[i], self.s2n_tol[i], self.mad_tol[i],
+                                                         self.mean_tol[i], self.rms_tol[i])))
+            p.s2n_method = _native.S2N_CODES[self.s2n_method[i]]
+            p.s2n_size = int(self.s2n_size[i])
+            p.replacing_method = _native.REPLACING_CODES[self.replacing_method[i]]
+            p.revalidate = int(bool(self.is_revalidated[i]))
             p.num_replacing_iters = int(self.num_replacing_iters[i])
+            if p.num_replacing_iters > _native.MAX_REPLACING_ITERS:
+                raise NotImplementedError(f"num_replacing_iters above {_native.MAX_REPLACING_ITERS}")
             # gpu_process.py:841 tests the whole tuple, so every non-final pass
             # is smoothed whatever the per-pass flag says.
             p.smooth = int(bool(self.is_smoothed))

@@ -364,6 +364,176 @@ def median_residual_ratio(u, v, median_tol=2.0, size=1):
 
 
 # ---------------------------------------------------------------------------
+# Signal-to-noise ratio -- CorrelationGPU.get_s2n_ratio / get_second_peak /
+# get_correlation_energy (gpu_process.py:1312-1365; gpu_misc.py:52-85).
+# ---------------------------------------------------------------------------
+
+def s2n_ratio(corr, ip, jp, method="peak2peak", size=2):
+    """Per-window S2N from the fftshift-ed planes (N, F, F).
+
+    * peak2peak: the (2 size + 1)^2 block around the first peak is set to
+      -inf (``fill_kernel``; CuPy wraps out-of-range fancy indices, so the
+      block wraps around the plane edges) and the first-occurrence argmax of
+      what is left is the second peak (:1318-1320, 1345-1354);
+    * peak2mean: the mean of the values in (0, peak / 2] (:1322-1324, 1356-1360);
+    * peak2energy: the mean of the positive values (:1326-1327).
+    ``s2n = peak1 / peak2`` where peak1 > 0, +inf where additionally
+    peak2 <= 0, 0 where peak1 <= 0 (:1329-1333); an empty mean is NaN and so
+    is the ratio.
+    """
+    corr = np.asarray(corr)
+    dtype = corr.dtype.type
+    nw, fh, fw = corr.shape
+    k = np.arange(nw)
+    ip = np.asarray(ip).reshape(-1)
+    jp = np.asarray(jp).reshape(-1)
+    c1 = corr[k, ip, jp]
+    with np.errstate(all="ignore"):
+        if method == "peak2peak":
+            work = corr.copy()
+            sh = np.arange(-size, size + 1)
+            ii = (ip[:, None, None] + sh[None, :, None]) % fh
+            jj = (jp[:, None, None] + sh[None, None, :]) % fw
+            work[k[:, None, None], ii, jj] = -np.inf
+            i2, j2 = first_peak(work)
+            c2 = work[k, i2, j2]
+        else:
+            c = np.where(corr > dtype(0.5) * c1[:, None, None], np.nan, corr) if method == "peak2mean" else corr
+            c = np.where(c <= 0, np.nan, c)
+            import warnings
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore", RuntimeWarning)
+                c2 = np.nanmean(c, axis=(1, 2)).astype(dtype)
+        s2n = np.where(c1 > 0, c1 / c2, dtype(0))
+        s2n = np.where((c2 <= 0) & (c1 > 0), dtype(np.inf), s2n)
+    return s2n.astype(dtype)
+
+
+# ---------------------------------------------------------------------------
+# Full validation -- ValidationGPU.__call__ (gpu_validation.py:84-164,
+# 166-227) with every test, and the replacement loop
+# (gpu_validation.py:294-415; gpu_process.py:744-834).
+# ---------------------------------------------------------------------------
+
+def _stacks(f, size=1, centre_nan=True):
+    ks = 2 * size + 1
+    f = np.asarray(f)
+    if not np.issubdtype(f.dtype, np.floating):
+        f = f.astype(np.float64)
+    fp = np.pad(f, size, mode="constant", constant_values=np.nan)
+    win = np.lib.stride_tricks.sliding_window_view(fp, (ks, ks)).copy()
+    if centre_nan:
+        win[:, :, size, size] = np.nan
+    m, n = np.shape(f)
+    return win.reshape(m, n, ks * ks)
+
+
+def _nan_reduce(fn, a):
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        return fn(a, axis=-1)
+
+
+def neighbour_stats(f, size=1):
+    """Median, MAD (median of |x - median|) and mean of each NaN-padded
+    neighbourhood with the centre excluded (gpu_validation.py:166-205)."""
+    st = _stacks(f, size)
+    med = _nan_reduce(np.nanmedian, st)
+    mad = _nan_reduce(np.nanmedian, np.abs(st - med[..., None]))
+    mean = _nan_reduce(np.nanmean, st)
+    return med, mad, mean
+
+
+def validate(u, v, s2n=None, s2n_tol=None, median_tol=2.0, mad_tol=None, mean_tol=None, rms_tol=None, size=1,
+             only=None):
+    """OR of the enabled tests (gpu_validation.py:103-118), per field OR-ed
+    over u and v: s2n < s2n_tol; |f - med| > median_tol; |f - med| >
+    mad_tol * mad; |f - mean| > mean_tol; |f - mean(f)| > rms_tol * std(f)
+    (global NaN-mean and population std, :143-148).  ``only``: positions to
+    validate (the reference's mask complement, :119, 222-226); others are
+    False.  With no test enabled, nothing is flagged."""
+    u = np.asarray(u, np.float64)
+    v = np.asarray(v, np.float64)
+    flags = np.zeros(u.shape, bool)
+    with np.errstate(invalid="ignore"):
+        if s2n_tol is not None and s2n is not None:
+            flags |= np.asarray(s2n).reshape(u.shape) < s2n_tol
+        if median_tol is not None or mad_tol is not None or mean_tol is not None:
+            su, sv = neighbour_stats(u, size), neighbour_stats(v, size)
+            if median_tol is not None:
+                flags |= (np.abs(u - su[0]) > median_tol) | (np.abs(v - sv[0]) > median_tol)
+            if mad_tol is not None:
+                flags |= (np.abs(u - su[0]) > mad_tol * su[1]) | (np.abs(v - sv[0]) > mad_tol * sv[1])
+            if mean_tol is not None:
+                flags |= (np.abs(u - su[2]) > mean_tol) | (np.abs(v - sv[2]) > mean_tol)
+        if rms_tol is not None:
+            for f in (u, v):
+                mu = _nan_reduce(np.nanmean, f.reshape(-1))
+                sd = _nan_reduce(np.nanstd, f.reshape(-1))
+                flags |= np.abs(f - mu) > rms_tol * sd
+    if only is not None:
+        flags &= only
+    return flags
+
+
+def neighbour_replacement(u, v, val, method="median", exclude_flagged=True, size=1):
+    """ReplacementGPU median/mean replacement (gpu_validation.py:373-398):
+    every flagged vector takes the nanmedian / nanmean of its neighbourhood
+    (centre excluded), with the flagged vectors themselves excluded
+    (``fill_value=NaN``) or included at their current values (``None``), all
+    from the same input (simultaneous update)."""
+    out = []
+    for f in (u, v):
+        f = np.asarray(f)
+        g = np.where(val, f.dtype.type(np.nan), f) if exclude_flagged else f
+        st = _stacks(g, size)[val]
+        stat = _nan_reduce(np.nanmedian if method == "median" else np.nanmean, st)
+        h = f.copy()
+        h[val] = stat  # written into the field's own dtype, as the reference assigns (:328-329)
+        out.append(h)
+    return out[0], out[1]
+
+
+def replace_outliers(u, v, val, method="spring", n_iters=1, revalidate=False, validate_fn=None, size=1):
+    """PIVGPU.replace_outliers (gpu_process.py:788-834).  Returns (u, v,
+    val): the replaced fields and the flags after the loop (updated only when
+    revalidating).  Iteration 1 excludes the flagged vectors; without
+    revalidation later iterations include them at their replaced values
+    (:824-826); with it, unresolved (NaN) vectors are reset to the originals
+    and only the previously flagged vectors are re-tested (:809-822).
+    Remaining NaN vectors are reset at the end (:829-832)."""
+    u = np.asarray(u)
+    v = np.asarray(v)
+    val = np.asarray(val, bool)
+    if not val.any() or n_iters == 0:
+        return u, v, val
+    u0, v0 = u.copy(), v.copy()
+    exclude = True
+    last_val = val
+    for _ in range(n_iters):
+        if not val.any():
+            break
+        if method == "spring":
+            u, v = spring_replacement(u, v, val)
+        else:
+            u, v = neighbour_replacement(u, v, val, method, exclude, size)
+        last_val = val
+        if revalidate:
+            nan = (np.isnan(u) | np.isnan(v)) & val
+            u = np.where(nan, u0, u)
+            v = np.where(nan, v0, v)
+            val = validate_fn(u, v, only=val) | nan
+        else:
+            exclude = False
+    if not revalidate:
+        nan = (np.isnan(u) | np.isnan(v)) & last_val
+        u = np.where(nan, u0, u)
+        v = np.where(nan, v0, v)
+    return u, v, val
+
+
+# ---------------------------------------------------------------------------
 # Spring replacement -- ReplacementGPU.spring_replacement
 # (gpu_validation.py:336-371, 381-415) and PIVGPU.replace_outliers
 # (gpu_process.py:788-834).
@@ -534,6 +704,14 @@ class OracleSettings:
     median_tol: Optional[float] = 2.0
     num_replacing_iters: int = 1
     smoothing_par: Optional[float] = None
+    s2n_method: str = "peak2peak"
+    s2n_size: int = 2
+    s2n_tol: Optional[float] = None
+    mad_tol: Optional[float] = None
+    mean_tol: Optional[float] = None
+    rms_tol: Optional[float] = None
+    replacing_method: str = "spring"
+    revalidate: bool = False
     dt: float = 1.0
     scaling_par: float = 1.0
 
@@ -565,23 +743,28 @@ def run_pass(frame_a, frame_b, settings: OracleSettings, geom: Geometry, prev_ge
     if prev_geom is not None:
         up, vp = interpolate_displacement(prev_geom, geom, u_prev, v_prev)   # :620-625
         fa, fb = deform_frames(fa, fb, geom, up, vp, settings.deforming_par, dtype)
-    corr = correlate_pass(fa, fb, geom, settings.n_fft, settings.normalize, settings.subpixel_method, dtype)
+    corr = correlate_pass(fa, fb, geom, settings.n_fft, settings.normalize, settings.subpixel_method, dtype,
+                          return_planes=settings.s2n_tol is not None)
     u, v = corr["u"], corr["v"]
     if up is not None:                                                       # :631-632
         u, v = u + up, v + vp
     u_pre, v_pre = u, v
-    if settings.median_tol is not None:
-        val = median_validation(u, v, settings.median_tol)                   # :769
-    else:
-        val = np.zeros(u.shape, bool)
-    if settings.num_replacing_iters > 0 and val.any():
-        u_rep, v_rep = spring_replacement(u, v, val)                         # :788-834
-    else:
-        u_rep, v_rep = u, v
+    s2n = None
+    if settings.s2n_tol is not None:                                         # :755
+        s2n = s2n_ratio(corr["corr"], corr["i_peak"], corr["j_peak"], settings.s2n_method,
+                        settings.s2n_size).reshape(u.shape)
+        corr["s2n"] = s2n
+
+    def validate_fn(a, b, only=None):                                        # :758-769
+        return validate(a, b, s2n, settings.s2n_tol, settings.median_tol, settings.mad_tol, settings.mean_tol,
+                        settings.rms_tol, only=only)
+
+    val = validate_fn(u, v)
+    u_rep, v_rep, _ = replace_outliers(u, v, val, settings.replacing_method, settings.num_replacing_iters,
+                                       settings.revalidate, validate_fn)     # :788-834
     outliers = None
     if is_last:
-        outliers = (median_validation(u_rep, v_rep, settings.median_tol)
-                    if settings.median_tol is not None else np.zeros(u.shape, bool))   # :783-784
+        outliers = validate_fn(u_rep, v_rep)                                 # :783-784
     p = None
     if not is_last:                                                          # :637-639, Q2
         (u_out, v_out), p = smoothn([u_rep, v_rep], s=settings.smoothing_par, return_p=True)

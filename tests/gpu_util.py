@@ -40,10 +40,55 @@ def pass_desc(geom: O.Geometry, fsize: int, normalize=True, subpixel=0, deformin
     p.num_replacing_iters = 1
     p.smooth = 1
     p.smoothing_par = 0.0
+    nan = float("nan")
+    p.s2n_method, p.s2n_size = 0, 2
+    p.s2n_tol = p.mad_tol = p.mean_tol = p.rms_tol = nan
+    p.replacing_method = 0
+    p.revalidate = 0
     return p
 
 
-def stage_correlate(fa, fb, geom, fsize, up=None, vpred=None, planes=False, subpixel=0):
+def tests_desc(median_tol=2.0, s2n_tol=None, mad_tol=None, mean_tol=None, rms_tol=None, method="spring", n_iters=1,
+               revalidate=False, s2n_method="peak2peak", s2n_size=2):
+    """A pass descriptor carrying only validation / replacement settings."""
+    p = _native.PassDesc()
+    nan = float("nan")
+    f = lambda t: nan if t is None else float(t)
+    p.validate = 1
+    p.median_tol, p.s2n_tol, p.mad_tol, p.mean_tol, p.rms_tol = (f(median_tol), f(s2n_tol), f(mad_tol), f(mean_tol),
+                                                                 f(rms_tol))
+    p.s2n_method, p.s2n_size = _native.S2N_CODES[s2n_method], s2n_size
+    p.replacing_method = _native.REPLACING_CODES[method]
+    p.num_replacing_iters = n_iters
+    p.revalidate = int(revalidate)
+    return p
+
+
+def stage_validate_all(u, v, desc, s2n=None, only=None):
+    lib = _native.load_library()
+    m, n = u.shape
+    du, dv = dev(np.asarray(u, np.float32)), dev(np.asarray(v, np.float32))
+    ds = dev(np.asarray(s2n, np.float32)) if s2n is not None else None
+    do = dev(np.asarray(only, np.uint8)) if only is not None else None
+    fl = empty((m, n), np.uint8)
+    cnt = _native.DeviceBuffer.zeros((1,), np.int32)
+    _native.check(lib.piv_stage_validate(vp(du), vp(dv), m, n, vp(ds), ctypes.byref(desc), vp(do), vp(fl), vp(cnt),
+                                         None))
+    return host(fl).astype(bool), int(host(cnt)[0])
+
+
+def stage_replace_all(u, v, flags, desc, s2n=None):
+    lib = _native.load_library()
+    m, n = u.shape
+    du, dv = dev(np.asarray(u, np.float32)), dev(np.asarray(v, np.float32))
+    df = dev(np.asarray(flags, np.uint8))
+    ds = dev(np.asarray(s2n, np.float32)) if s2n is not None else None
+    _native.check(lib.piv_stage_replace(vp(du), vp(dv), vp(df), m, n, ctypes.byref(desc), vp(ds), None))
+    return host(du), host(dv), host(df).astype(bool)
+
+
+def stage_correlate(fa, fb, geom, fsize, up=None, vpred=None, planes=False, subpixel=0, s2n_method=None,
+                    s2n_size=2):
     lib = _native.load_library()
     m, n = geom.field_shape
     da, db = dev(fa), dev(fb)
@@ -52,15 +97,21 @@ def stage_correlate(fa, fb, geom, fsize, up=None, vpred=None, planes=False, subp
     ip = empty((m, n), np.int32)
     jp = empty((m, n), np.int32)
     pl = empty((m * n, fsize, fsize), np.float32) if planes else None
+    sn = empty((m, n), np.float32) if s2n_method is not None else None
     dup = dev(np.asarray(up, np.float32)) if up is not None else None
     dvp = dev(np.asarray(vpred, np.float32)) if vpred is not None else None
     desc = pass_desc(geom, fsize, subpixel=subpixel)
+    if s2n_method is not None:
+        desc.s2n_method = _native.S2N_CODES[s2n_method]
+        desc.s2n_size = s2n_size
     _native.check(lib.piv_stage_correlate(vp(da), vp(db), _native.in_dtype_code(fa.dtype), fa.shape[0], fa.shape[1],
                                           ctypes.byref(desc), vp(dup), vp(dvp), vp(u), vp(v), vp(ip), vp(jp), vp(pl),
-                                          None))
+                                          vp(sn), None))
     out = {"u": host(u), "v": host(v), "i_peak": host(ip), "j_peak": host(jp)}
     if planes:
         out["corr"] = host(pl)
+    if sn is not None:
+        out["s2n"] = host(sn)
     return out
 
 

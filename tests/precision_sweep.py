@@ -153,7 +153,7 @@ def main():
         def fft_call():
             _native.check(lib.piv_stage_correlate(gu.vp(da), gu.vp(db), _native.PIV_F32, n * ws, ws,
                                                   ctypes.byref(desc), None, None, gu.vp(u), gu.vp(v), gu.vp(ipd),
-                                                  gu.vp(jpd), None, None))
+                                                  gu.vp(jpd), None, None, None))
             _native.synchronize()
 
         def dft_call(prec):

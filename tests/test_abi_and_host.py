@@ -116,12 +116,29 @@ def test_min_search_size_and_frame_shape_assertions():
 
 @pytest.mark.parametrize("kwargs,name", [
     (dict(dtype_f="float64"), "dtype_f"), (dict(mask=np.zeros((256, 256), bool)), "mask"),
-    (dict(shrink_ratio=0.5), "shrink_ratio"), (dict(s2n_tol=1.2), "s2n_tol"), (dict(mad_tol=2.0), "mad_tol"),
-    (dict(replacing_method="median"), "replacing_method"), (dict(revalidate=True), "revalidate"),
+    (dict(shrink_ratio=0.5), "shrink_ratio"), (dict(validation_size=2), "validation_size"),
+    (dict(replacing_size=2), "replacing_size"),
 ])
 def test_unimplemented_options_raise(kwargs, name):
     with pytest.raises(NotImplementedError, match=name):
         gp.piv_gpu((256, 256), 16, **kwargs)
+
+
+def test_plan_desc_carries_every_validation_setting():
+    piv = gp.PIVGPU((256, 256), 16, search_size_iters=(1, 2), s2n_method="peak2mean", s2n_size=3, s2n_tol=1.3,
+                    median_tol=None, mad_tol=2.5, mean_tol=(1.0, None), rms_tol=3.0, replacing_method="median",
+                    num_replacing_iters=3, revalidate=True, dtype_f="float32")
+    d = piv._plan_desc(_native.PIV_U8)
+    assert d.n_passes == 3
+    for k, p in enumerate(d.passes[:3]):
+        assert p.validate == 1 and np.isnan(p.median_tol)
+        assert p.s2n_method == 1 and p.s2n_size == 3 and np.isclose(p.s2n_tol, 1.3) and np.isclose(p.mad_tol, 2.5)
+        assert np.isclose(p.rms_tol, 3.0) and p.replacing_method == 1 and p.num_replacing_iters == 3
+        assert p.revalidate == 1
+    # per size group (gpu_process.py:702-711): passes 1 and 2 share the second entry
+    assert np.isclose(d.passes[0].mean_tol, 1.0) and np.isnan(d.passes[1].mean_tol) and np.isnan(d.passes[2].mean_tol)
+    none = gp.PIVGPU((256, 256), 16, median_tol=None, dtype_f="float32")._plan_desc(_native.PIV_U8)
+    assert none.passes[0].validate == 0
 
 
 def test_gpu_piv_alias_and_frame_assertion():

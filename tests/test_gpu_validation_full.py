@@ -1,0 +1,170 @@
+"""The rest of validation and replacement on the GPU against the oracle:
+signal-to-noise ratios (peak2peak / peak2mean / peak2energy,
+gpu_process.py:1312-1365), the S2N / median / MAD / mean / RMS tests
+(gpu_validation.py:84-164), median and mean replacement with iterations and
+revalidation (gpu_process.py:788-834; gpu_validation.py:294-415), and a
+multi-pass run with all of them.
+
+Tolerances: S2N within 1e-5 relative of the oracle evaluated on the kernel's
+own planes; against the oracle's FP32 planes 1e-4 for peak2peak and ~1 %
+for the two mean methods, whose value sets move with the sign of near-zero
+plane values (FFT rounding) -- windows whose second-peak candidates tie
+within 1e-5 are left out; flags identical except vectors within 1e-4 (relative) of a test's
+threshold; replaced vectors within 1e-5 px."""
+
+import numpy as np
+import pytest
+
+from oracle import piv_oracle as O
+
+pytestmark = pytest.mark.gpu
+gu = pytest.importorskip("gpu_util")
+
+
+def _frames(shape, seed, shift=(2.35, -1.6)):
+    from openpiv_gpu import synthetic
+    return synthetic.make_pair(shape, synthetic.uniform_displacement(*shift), synthetic.ParticleModel(), seed)
+
+
+def _second_peak_tie(planes, ip, jp, size, rel=1e-5):
+    out = np.zeros(len(planes), bool)
+    F = planes.shape[1]
+    for k in range(len(planes)):
+        p = planes[k].copy()
+        sh = np.arange(-size, size + 1)
+        p[np.ix_((ip[k] + sh) % F, (jp[k] + sh) % F)] = -np.inf
+        flat = np.sort(p.reshape(-1))
+        out[k] = flat[-1] - flat[-2] <= rel * max(abs(flat[-1]), 1e-30)
+    return out
+
+
+@pytest.mark.parametrize("method,size", [("peak2peak", 2), ("peak2peak", 4), ("peak2mean", 2), ("peak2energy", 2)])
+def test_s2n_matches_oracle(method, size):
+    shape = (256, 320)
+    fa, fb = _frames(shape, 31)
+    geom = O.field_geometry(shape, 32, 0.5)
+    got = gu.stage_correlate(fa, fb, geom, 64, planes=True, s2n_method=method, s2n_size=size)
+    ip, jp = got["i_peak"].reshape(-1), got["j_peak"].reshape(-1)
+    have = got["s2n"].reshape(-1)
+    # exact semantics: the oracle's S2N of the very planes the kernel produced
+    # (only the summation order of the two means differs)
+    want = O.s2n_ratio(got["corr"], ip, jp, method, size)
+    ok = ~gu.near_tie_mask(got["corr"])
+    if method == "peak2peak":
+        ok &= ~_second_peak_tie(got["corr"], ip, jp, size)
+    assert ok.mean() > 0.97
+    fin = ok & np.isfinite(want)
+    assert np.all(np.abs(have[fin] - want[fin]) <= 1e-5 * np.abs(want[fin]))
+    assert np.array_equal(np.isinf(have[ok]), np.isinf(want[ok]))
+    # against the oracle's own FP32 planes: peak2peak to 1e-4; the two means
+    # move with the sign of near-zero plane values (FFT rounding), ~1 %
+    ref = O.correlate_pass(fa, fb, geom, return_planes=True)
+    rp = O.s2n_ratio(ref["corr"], ref["i_peak"].reshape(-1), ref["j_peak"].reshape(-1), method, size)
+    same = (ip == ref["i_peak"].reshape(-1)) & (jp == ref["j_peak"].reshape(-1)) & ok & np.isfinite(rp)
+    if method == "peak2peak":
+        same &= ~_second_peak_tie(ref["corr"], ip, jp, size)
+    rel = np.abs(have[same] - rp[same]) / np.abs(rp[same])
+    if method == "peak2peak":
+        assert rel.max() <= 1e-4
+    else:
+        assert np.median(rel) <= 1e-2 and rel.max() <= 5e-2
+
+
+def _field(seed, shape=(40, 48), n_out=60):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), indexing="ij")
+    u = (2 * np.sin(xx / 7.0) + 0.05 * rng.standard_normal(shape)).astype(np.float32)
+    v = (1.5 * np.cos(yy / 5.0) + 0.05 * rng.standard_normal(shape)).astype(np.float32)
+    idx = rng.choice(u.size, n_out, replace=False)
+    u.flat[idx] += rng.choice([-1, 1], n_out) * rng.uniform(3, 9, n_out)
+    v.flat[idx[::2]] += rng.uniform(-8, 8, idx[::2].size)
+    return u, v
+
+
+def _band(u, v, s2n, tests, rel=1e-4):
+    """Vectors within rel of any enabled test's threshold."""
+    med_u, mad_u, mean_u = O.neighbour_stats(u)
+    med_v, mad_v, mean_v = O.neighbour_stats(v)
+    near = np.zeros(u.shape, bool)
+
+    def close(x, t):
+        with np.errstate(invalid="ignore", divide="ignore"):
+            return np.abs(x - t) <= rel * np.maximum(np.abs(t), 1e-12)
+
+    for f, med, mad, mean in ((u, med_u, mad_u, mean_u), (v, med_v, mad_v, mean_v)):
+        if tests.get("median_tol") is not None:
+            near |= close(np.abs(f - med), tests["median_tol"])
+        if tests.get("mad_tol") is not None:
+            near |= close(np.abs(f - med), tests["mad_tol"] * mad)
+        if tests.get("mean_tol") is not None:
+            near |= close(np.abs(f - mean), tests["mean_tol"])
+        if tests.get("rms_tol") is not None:
+            near |= close(np.abs(f - f.mean(dtype=np.float64)), tests["rms_tol"] * f.std(dtype=np.float64))
+    if tests.get("s2n_tol") is not None:
+        near |= close(s2n, tests["s2n_tol"])
+    return near
+
+
+@pytest.mark.parametrize("tests", [
+    dict(median_tol=2.0), dict(median_tol=None, mad_tol=3.0), dict(median_tol=None, mean_tol=1.5),
+    dict(median_tol=None, rms_tol=2.0), dict(median_tol=None, s2n_tol=1.3),
+    dict(median_tol=1.0, mad_tol=4.0, mean_tol=2.0, rms_tol=2.5, s2n_tol=1.2),
+])
+def test_validation_tests_match_oracle(tests):
+    u, v = _field(3)
+    rng = np.random.default_rng(4)
+    s2n = rng.uniform(1.0, 2.0, u.shape).astype(np.float32)
+    desc = gu.tests_desc(**tests)
+    got, count = gu.stage_validate_all(u, v, desc, s2n=s2n if "s2n_tol" in tests else None)
+    ref = O.validate(u, v, s2n if "s2n_tol" in tests else None, tests.get("s2n_tol"), tests.get("median_tol"),
+                     tests.get("mad_tol"), tests.get("mean_tol"), tests.get("rms_tol"))
+    ok = ~_band(u.astype(np.float64), v.astype(np.float64), s2n, tests)
+    assert np.array_equal(got[ok], ref[ok]) and count == int(got.sum()) and got.sum() > 0
+    # restricted to a subset
+    only = rng.random(u.shape) < 0.3
+    got_o, _ = gu.stage_validate_all(u, v, desc, s2n=s2n if "s2n_tol" in tests else None, only=only)
+    assert not got_o[~only].any() and np.array_equal(got_o[ok & only], ref[ok & only])
+
+
+@pytest.mark.parametrize("method", ["median", "mean", "spring"])
+@pytest.mark.parametrize("n_iters,revalidate", [(1, False), (3, False), (2, True), (4, True)])
+def test_replacement_loop_matches_oracle(method, n_iters, revalidate):
+    u, v = _field(5)
+    tests = dict(median_tol=1.0)
+    flags = O.validate(u, v, median_tol=1.0)
+    desc = gu.tests_desc(method=method, n_iters=n_iters, revalidate=revalidate, **tests)
+    gu_, gv_, gflags = gu.stage_replace_all(u, v, flags, desc)
+    fn = lambda a, b, only=None: O.validate(a, b, median_tol=1.0, only=only)
+    ru, rv, rflags = O.replace_outliers(u, v, flags, method, n_iters, revalidate, fn)
+    assert np.abs(gu_ - ru).max() <= 1e-4 and np.abs(gv_ - rv).max() <= 1e-4
+    assert np.array_equal(gflags, rflags)
+    assert np.array_equal(gu_[~flags], u[~flags])
+
+
+def test_unresolved_block_is_reset_on_gpu():
+    u = np.ones((10, 10), np.float32)
+    v = np.ones((10, 10), np.float32)
+    flags = np.zeros((10, 10), bool)
+    flags[2:8, 2:8] = True
+    u[flags] = 7.0
+    for method in ("median", "mean"):
+        gu_, gv_, _ = gu.stage_replace_all(u, v, flags, gu.tests_desc(method=method))
+        ru, rv, _ = O.replace_outliers(u, v, flags, method, 1)
+        assert np.array_equal(gu_, ru.astype(np.float32)) and not np.isnan(gu_).any()
+
+
+def test_multipass_with_every_test_matches_oracle():
+    from openpiv_gpu.gpu_process import PIVGPU
+    shape = (320, 384)
+    fa, fb = _frames(shape, 41, shift=(3.2, 1.7))
+    kw = dict(search_size_iters=(1, 1), overlap_ratio=0.5, s2n_method="peak2energy", s2n_tol=1.05, median_tol=2.0,
+              mad_tol=6.0, mean_tol=3.0, replacing_method="median", num_replacing_iters=2, revalidate=True)
+    piv = PIVGPU(shape, 16, dtype_f="float32", **kw)
+    u, v = piv.run_host(fa, fb)
+    s = O.OracleSettings(min_search_size=16, search_size_iters=(1, 1), overlap_ratio=0.5, s2n_method="peak2energy",
+                         s2n_tol=1.05, median_tol=2.0, mad_tol=6.0, mean_tol=3.0, replacing_method="median",
+                         num_replacing_iters=2, revalidate=True)
+    r = O.piv(fa, fb, s)
+    err = np.maximum(np.abs(u - r["u"]), np.abs(v - r["v"]))
+    assert np.mean(err <= 1e-3) > 0.995, np.sort(err.reshape(-1))[-10:]
+    assert np.mean(piv.init_val_locations == r["val_locations"]) > 0.995

@@ -1,0 +1,149 @@
+"""Pins of the oracle's S2N, full validation and replacement loop
+(gpu_process.py:744-834, 1312-1365; gpu_validation.py:84-415) by hand-built
+planes, brute-force neighbourhood loops and hand-stepped iterations."""
+
+import numpy as np
+
+from oracle import piv_oracle as O
+
+
+def _plane(F=16):
+    p = np.full((1, F, F), -0.5, np.float32)
+    return p
+
+
+def test_s2n_peak2peak_known_answer_and_wrapped_exclusion():
+    p = _plane()
+    p[0, 8, 8] = 10.0
+    p[0, 9, 9] = 9.0    # inside the 5x5 block around the peak: ignored
+    p[0, 3, 12] = 4.0   # the second peak
+    s = O.s2n_ratio(p, np.array([8]), np.array([8]), "peak2peak", 2)
+    assert np.isclose(s[0], 2.5)
+    # peak in the corner: the excluded block wraps to the opposite edges
+    q = _plane()
+    q[0, 0, 0] = 10.0
+    q[0, 15, 15] = 8.0   # wrapped neighbour (-1, -1): excluded
+    q[0, 14, 1] = 7.0    # wrapped (-2, +1): excluded
+    q[0, 5, 5] = 2.0
+    s = O.s2n_ratio(q, np.array([0]), np.array([0]), "peak2peak", 2)
+    assert np.isclose(s[0], 5.0)
+
+
+def test_s2n_mean_and_energy_by_hand():
+    p = np.zeros((1, 4, 4), np.float64)
+    p[0] = [[8, 1, 2, -1], [3, 5, 0, 0], [4, 0, -2, 0], [0, 0, 0, 0]]
+    # peak2energy: mean of positive values {8, 1, 2, 3, 5, 4}
+    s = O.s2n_ratio(p, np.array([0]), np.array([0]), "peak2energy")
+    assert np.isclose(s[0], 8.0 / (23.0 / 6.0))
+    # peak2mean: mean of values in (0, 4] = {1, 2, 3, 4}
+    s = O.s2n_ratio(p, np.array([0]), np.array([0]), "peak2mean")
+    assert np.isclose(s[0], 8.0 / 2.5)
+
+
+def test_s2n_special_cases():
+    p = np.full((3, 4, 4), -1.0)
+    p[0, 0, 0] = -0.5          # first peak not positive -> 0
+    p[1, 1, 1] = 2.0           # no positive second value -> inf (peak2peak: second peak -1)
+    p[2, 2, 2] = 2.0           # peak2mean: nothing in (0, 1] -> NaN
+    ip = np.array([0, 1, 2])
+    jp = np.array([0, 1, 2])
+    s = O.s2n_ratio(p, ip, jp, "peak2peak", 1)
+    assert s[0] == 0 and np.isinf(s[1])
+    s = O.s2n_ratio(p, ip, jp, "peak2mean")
+    assert s[0] == 0 and np.isnan(s[2])
+
+
+def _brute_stats(f, i, j):
+    m, n = f.shape
+    nb = [f[a, b] for a in range(i - 1, i + 2) for b in range(j - 1, j + 2)
+          if (a, b) != (i, j) and 0 <= a < m and 0 <= b < n]
+    srt = sorted(nb)
+    k = len(srt)
+    med = srt[k // 2] if k % 2 else 0.5 * (srt[k // 2 - 1] + srt[k // 2])
+    dev = sorted(abs(x - med) for x in nb)
+    mad = dev[k // 2] if k % 2 else 0.5 * (dev[k // 2 - 1] + dev[k // 2])
+    return med, mad, sum(nb) / k
+
+
+def test_neighbour_stats_match_brute_force():
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal((7, 9))
+    med, mad, mean = O.neighbour_stats(f)
+    for i in range(7):
+        for j in range(9):
+            bm, bd, bn = _brute_stats(f, i, j)
+            assert np.isclose(med[i, j], bm) and np.isclose(mad[i, j], bd) and np.isclose(mean[i, j], bn)
+
+
+def test_validate_each_test_and_only_mask():
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal((12, 12)) * 0.1
+    v = rng.standard_normal((12, 12)) * 0.1
+    u[5, 5] = 3.0
+    v[2, 8] = -1.0
+    s2n = np.full((12, 12), 5.0)
+    s2n[9, 1] = 1.1
+    flags = O.validate(u, v, s2n, s2n_tol=1.3, median_tol=None)
+    assert flags.sum() == 1 and flags[9, 1]
+    flags = O.validate(u, v, median_tol=2.0)
+    assert flags[5, 5] and not flags[2, 8]
+    flags = O.validate(u, v, median_tol=None, mean_tol=0.8)
+    assert flags[5, 5] and flags[2, 8]
+    # RMS: global mean/std per field
+    fl = O.validate(u, v, median_tol=None, rms_tol=3.0)
+    ref = (np.abs(u - u.mean()) > 3 * u.std()) | (np.abs(v - v.mean()) > 3 * v.std())
+    assert np.array_equal(fl, ref)
+    # MAD: |f - med| > tol * mad by brute force
+    fl = O.validate(u, v, median_tol=None, mad_tol=4.0)
+    for i in range(12):
+        for j in range(12):
+            mu, du, _ = _brute_stats(u, i, j)
+            mv, dv, _ = _brute_stats(v, i, j)
+            assert fl[i, j] == ((abs(u[i, j] - mu) > 4 * du) or (abs(v[i, j] - mv) > 4 * dv))
+    only = np.zeros((12, 12), bool)
+    only[2, 8] = True
+    assert O.validate(u, v, median_tol=None, mean_tol=0.8, only=only).sum() == 1
+
+
+def test_neighbour_replacement_exclusion():
+    u = np.arange(25, dtype=np.float64).reshape(5, 5)
+    v = -u
+    val = np.zeros((5, 5), bool)
+    val[2, 2] = val[2, 3] = True
+    ru, rv = O.neighbour_replacement(u, v, val, "mean", exclude_flagged=True)
+    nb = [6, 7, 8, 11, 16, 17, 18]          # (2,2)'s neighbours without the flagged (2,3)
+    assert np.isclose(ru[2, 2], np.mean(nb)) and np.isclose(rv[2, 2], -np.mean(nb))
+    ru, _ = O.neighbour_replacement(u, v, val, "median", exclude_flagged=False)
+    assert np.isclose(ru[2, 2], np.median([6, 7, 8, 11, 13, 16, 17, 18]))
+    assert np.array_equal(ru[~val], u[~val])
+
+
+def test_replacement_iterations_by_hand():
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((8, 8))
+    v = rng.standard_normal((8, 8))
+    val = np.zeros((8, 8), bool)
+    val[3, 3] = val[3, 4] = val[4, 3] = True
+    # two plain iterations: the second includes the replaced values
+    a1 = O.neighbour_replacement(u, v, val, "median", True)
+    a2 = O.neighbour_replacement(*a1, val, "median", False)
+    ru, rv, rval = O.replace_outliers(u, v, val, "median", n_iters=2)
+    assert np.allclose(ru, a2[0]) and np.allclose(rv, a2[1]) and np.array_equal(rval, val)
+    # revalidation: the replaced vectors pass the test and are cleared
+    fn = lambda a, b, only=None: O.validate(a, b, median_tol=0.5, only=only)
+    u2 = np.zeros((8, 8))
+    v2 = np.zeros((8, 8))
+    u2[3, 3] = 9.0
+    ru, rv, rval = O.replace_outliers(u2, v2, fn(u2, v2), "mean", n_iters=3, revalidate=True, validate_fn=fn)
+    assert not rval.any() and ru[3, 3] == 0.0
+
+
+def test_unresolved_replacements_are_reset():
+    u = np.ones((6, 6))
+    v = np.ones((6, 6))
+    val = np.zeros((6, 6), bool)
+    val[1:5, 1:5] = True                    # the inner 2x2 has only flagged neighbours
+    u[1:5, 1:5] = 7.0
+    ru, rv, _ = O.replace_outliers(u, v, val, "median", n_iters=1)
+    assert np.all(ru[2:4, 2:4] == 7.0)      # NaN -> reset to the original
+    assert np.all(ru[1, 1:5] == 1.0)        # border of the block resolved from valid neighbours
