@@ -69,6 +69,10 @@ typedef struct piv_pass_desc {
   float rms_tol;              /* flag |f - mean(f)| > rms_tol * std(f)    */
   int32_t replacing_method;   /* enum piv_replacing                       */
   int32_t revalidate;         /* re-test replaced vectors each iteration  */
+  int32_t window_size;        /* interrogation window inside the search
+                               * window (shrink_ratio, gpu_process.py:654-662,
+                               * 1066-1081); 0 = search_size            */
+  int32_t reserved;
 } piv_pass_desc;
 
 #define PIV_MAX_REPLACING_ITERS 16

@@ -68,6 +68,9 @@ int check_pass(const piv_pass_desc& p, int ht, int wd) {
       (p.field_n - 1) * p.spacing + p.search_size + p.x_offset > wd)
     return fail(PIV_E_INVALID, "window grid exceeds the frame");
   if (p.subpixel < 0 || p.subpixel > 2) return fail(PIV_E_INVALID, "subpixel method");
+  if (p.window_size != 0 && (p.window_size < 2 || p.window_size > p.search_size ||
+                             (p.search_size - p.window_size) % 2))
+    return fail(PIV_E_INVALID, "window_size %d incompatible with search_size %d", p.window_size, p.search_size);
   return check_tests(p);
 }
 
@@ -155,6 +158,7 @@ CorrArgs corr_args(const void* fa, const void* fb, int dtype, int ht, int wd, co
   a.subpixel = p.subpixel;
   a.s2n_method = p.s2n_method;
   a.s2n_size = p.s2n_size;
+  a.win_pad = p.window_size > 0 ? (p.search_size - p.window_size) / 2 : 0;
   return a;
 }
 

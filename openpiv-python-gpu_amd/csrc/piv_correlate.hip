@@ -102,6 +102,7 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a,
   const int W = a.ss;
   const int pad = (F - W) / 2;
   const int SW = W + 1;
+  const int wp = a.win_pad;  // (search - interrogation window) / 2
 
   float* region = smem + (size_t)wl * region_floats;
   float* stA = region;
@@ -125,7 +126,9 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a,
       float va = 0.f, vb = 0.f;
       if (valid) {
         size_t gi = (size_t)(row0 + r) * a.wd + (col0 + c);
-        va = load_px<InT>(a.frame_a, gi);
+        // frame A keeps only the interrogation window inside the search
+        // window (first pass with shrink_ratio, gpu_process.py:1066-1070)
+        if (r >= wp && r < W - wp && c >= wp && c < W - wp) va = load_px<InT>(a.frame_a, gi);
         vb = load_px<InT>(a.frame_b, gi);
       }
       stA[r * SW + c] = va;
@@ -166,9 +169,11 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a,
   }
   float mean_a = 0.f, mean_b = 0.f;
   if (a.normalize || pad != 0) {
-    const float cnt = (float)(W * W);
-    mean_a = sa / cnt;
-    mean_b = sb / cnt;
+    // means over the interrogation window (A) and the search window (B)
+    // (gpu_process.py:1072-1081, 1201-1205)
+    const int wi = W - 2 * wp;
+    mean_a = sa / (float)(wi * wi);
+    mean_b = sb / (float)(W * W);
   }
 
   // ---- stage R: row FFTs of z = (a - mean_a) + i (b - mean_b) -----------
@@ -183,7 +188,8 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a,
       const int c = 2 * m + h - pad;
       c32 zz{0.f, 0.f};
       if (active && c >= 0 && c < W) {
-        zz.x = stA[rr * SW + c] - mean_a;
+        const bool in_a = rr >= wp && rr < W - wp && c >= wp && c < W - wp;
+        zz.x = in_a ? stA[rr * SW + c] - mean_a : 0.f;
         zz.y = stB[rr * SW + c] - mean_b;
       }
       x[m] = zz;
@@ -564,6 +570,10 @@ static hipError_t launch_t(const CorrArgs& a, hipStream_t stream) {
 }
 
 hipError_t launch_correlate(const CorrArgs& a, hipStream_t stream) {
+  if (a.win_pad < 0 || 2 * a.win_pad >= a.ss) {
+    set_launch_detail("correlate: window_size / search_size mismatch", hipErrorInvalidValue);
+    return hipErrorInvalidValue;
+  }
   if (a.ss < 2 || a.ss > a.fsize || (a.fsize - a.ss) % 2 != 0) {
     set_launch_detail("correlate: search_size / fft_size mismatch", hipErrorInvalidValue);
     return hipErrorInvalidValue;

@@ -17,7 +17,8 @@ struct CorrArgs {
   const void* frame_b;
   int in_dtype;            // InDtype of both frames
   int ht, wd;              // frame shape
-  int ss;                  // window (search) size
+  int ss;                  // search window size
+  int win_pad;             // (ss - interrogation window) / 2; frame A is zero outside
   int fsize;               // FFT size F (power of two, 16..128)
   int spacing, x_off, y_off;
   int m, n;                // field shape

@@ -65,6 +65,8 @@ class PassDesc(ctypes.Structure):
         ("rms_tol", ctypes.c_float),
         ("replacing_method", ctypes.c_int32),
         ("revalidate", ctypes.c_int32),
+        ("window_size", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

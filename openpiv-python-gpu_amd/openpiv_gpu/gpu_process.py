@@ -203,8 +203,6 @@ class PIVGPU:
             raise NotImplementedError("dtype_f: only 'float32' processing is implemented on the MI355X path")
         if mask is not None:
             raise NotImplementedError("mask: masked processing is not implemented on the MI355X path")
-        if shrink_ratio != 1:
-            raise NotImplementedError("shrink_ratio: extended search is not implemented on the MI355X path")
         self.min_search_size = min_search_size
         self.ss_iters = (search_size_iters,) if isinstance(search_size_iters, int) else tuple(search_size_iters)
         self.n_passes = len(self.ss_iters)
@@ -272,10 +270,18 @@ class PIVGPU:
                 yield i
 
     def init_piv_fields(self):
-        """Creates field objects for all iterations (gpu_process.py:649-673)."""
-        self.piv_fields = [PIVFieldGPU(self.frame_shape, window_size=ss, search_size=ss, overlap_ratio=r,
-                                       center=self.center, mask=None, dtype_f=self.dtype_f)
-                           for ss, r in self.get_geom_settings()]
+        """Creates field objects for all iterations (gpu_process.py:649-673):
+        with shrink_ratio != 1 the first pass correlates an interrogation
+        window of even(int(ss * shrink_ratio)) >= 8 px inside its search window."""
+        self.piv_fields = []
+        for k, (ss, r) in enumerate(self.get_geom_settings()):
+            ws = ss
+            if k == 0 and self.shrink_ratio != 1:
+                ws = int(ss * self.shrink_ratio)
+                ws = ws if ws % 2 == 0 else ws + 1
+                ws = max(ws, 8)
+            self.piv_fields.append(PIVFieldGPU(self.frame_shape, window_size=ws, search_size=ss, overlap_ratio=r,
+                                               center=self.center, mask=None, dtype_f=self.dtype_f))
 
     def _plan_desc(self, in_dtype: int) -> _native.PlanDesc:
         fields = self.piv_fields
@@ -294,6 +300,9 @@ class PIVGPU:
                 raise NotImplementedError("odd padding: search_size must be even")
             p = d.passes[k]
             p.search_size = f.search_size
+            if f.window_size > f.search_size or (f.search_size - f.window_size) % 2:
+                raise NotImplementedError("window_size must be an even size not above search_size")
+            p.window_size = f.window_size if f.window_size != f.search_size else 0
             p.spacing = f.spacing
             p.x_offset, p.y_offset = f.offset
             p.field_m, p.field_n = f.field_shape

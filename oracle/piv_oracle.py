@@ -233,7 +233,7 @@ def deform_frames(frame_a, frame_b, geom: Geometry, up, vp, deforming_par=0.5, d
 # (gpu_process.py:1065, 1201-1246).
 # ---------------------------------------------------------------------------
 
-def correlate_windows(win_a, win_b, fsize, normalize=True):
+def correlate_windows(win_a, win_b, fsize, normalize=True, win_pad=0):
     """Returns the fftshift-ed (N, F, F) correlation planes.
 
     Normalisation is forced on whenever padding is nonzero
@@ -246,7 +246,17 @@ def correlate_windows(win_a, win_b, fsize, normalize=True):
     dtype = win_a.dtype
     n, ss, _ = win_a.shape
     pad = (fsize - ss) // 2
-    if pad != 0 or normalize:
+    if win_pad:
+        # interrogation window inside the search window: A is zero outside
+        # it and normalised over it only (gpu_process.py:1066-1081, 1198-1199)
+        inner = (slice(None), slice(win_pad, ss - win_pad), slice(win_pad, ss - win_pad))
+        a = np.zeros_like(win_a)
+        a[inner] = win_a[inner]
+        if pad != 0 or normalize:
+            a[inner] = a[inner] - a[inner].mean(axis=(1, 2), keepdims=True, dtype=dtype)
+            win_b = win_b - win_b.mean(axis=(1, 2), keepdims=True, dtype=dtype)
+        win_a = a
+    elif pad != 0 or normalize:
         win_a = win_a - win_a.mean(axis=(1, 2), keepdims=True, dtype=dtype)
         win_b = win_b - win_b.mean(axis=(1, 2), keepdims=True, dtype=dtype)
     if pad != 0:
@@ -308,14 +318,15 @@ def subpixel_peak(corr, ip, jp, method="gaussian"):
 
 
 def correlate_pass(frame_a, frame_b, geom: Geometry, n_fft=2, normalize=True,
-                   subpixel_method="gaussian", dtype=np.float32, return_planes=False):
+                   subpixel_method="gaussian", dtype=np.float32, return_planes=False, window_size=None):
     """CorrelationGPU.__call__ (gpu_process.py:1021-1097) on already deformed
     frames: stack, normalise, correlate, peak, sub-pixel, displacement
     ``u = j_sp - F//2``, ``v = i_sp - F//2`` (:1296-1310)."""
     fsize = fft_size(geom.search_size, n_fft)
     wa = get_stack(np.asarray(frame_a, dtype), geom)
     wb = get_stack(np.asarray(frame_b, dtype), geom)
-    corr = correlate_windows(wa, wb, fsize, normalize)
+    win_pad = (geom.search_size - window_size) // 2 if window_size else 0
+    corr = correlate_windows(wa, wb, fsize, normalize, win_pad)
     ip, jp = first_peak(corr)
     i_sp, j_sp = subpixel_peak(corr, ip, jp, subpixel_method)
     m, n = geom.field_shape
@@ -712,6 +723,7 @@ class OracleSettings:
     rms_tol: Optional[float] = None
     replacing_method: str = "spring"
     revalidate: bool = False
+    shrink_ratio: float = 1.0
     dt: float = 1.0
     scaling_par: float = 1.0
 
@@ -733,8 +745,15 @@ class PassState:
     smooth_p: Optional[float] = None
 
 
+def shrunk_window(search_size, shrink_ratio):
+    """gpu_process.py:654-662: the first pass's interrogation window."""
+    ws = int(search_size * shrink_ratio)
+    ws = ws if ws % 2 == 0 else ws + 1
+    return max(ws, 8)
+
+
 def run_pass(frame_a, frame_b, settings: OracleSettings, geom: Geometry, prev_geom: Optional[Geometry],
-             u_prev, v_prev, is_last: bool, mode="faithful32") -> PassState:
+             u_prev, v_prev, is_last: bool, mode="faithful32", window_size=None) -> PassState:
     """One iteration of the main loop (gpu_process.py:596-639)."""
     dtype = np.float32 if mode == "faithful32" else np.float64
     fa = np.asarray(frame_a, dtype)
@@ -744,7 +763,7 @@ def run_pass(frame_a, frame_b, settings: OracleSettings, geom: Geometry, prev_ge
         up, vp = interpolate_displacement(prev_geom, geom, u_prev, v_prev)   # :620-625
         fa, fb = deform_frames(fa, fb, geom, up, vp, settings.deforming_par, dtype)
     corr = correlate_pass(fa, fb, geom, settings.n_fft, settings.normalize, settings.subpixel_method, dtype,
-                          return_planes=settings.s2n_tol is not None)
+                          return_planes=settings.s2n_tol is not None, window_size=window_size)
     u, v = corr["u"], corr["v"]
     if up is not None:                                                       # :631-632
         u, v = u + up, v + vp
@@ -782,8 +801,9 @@ def piv(frame_a, frame_b, settings: OracleSettings, mode="faithful32", keep_pass
     u = v = None
     passes = []
     for k, g in enumerate(geoms):
+        ws = shrunk_window(g.search_size, settings.shrink_ratio) if k == 0 and settings.shrink_ratio != 1 else None
         st = run_pass(frame_a, frame_b, settings, g, geoms[k - 1] if k > 0 else None, u, v,
-                      k == len(geoms) - 1, mode)
+                      k == len(geoms) - 1, mode, window_size=ws)
         u, v = st.u_out, st.v_out
         if keep_passes:
             passes.append(st)

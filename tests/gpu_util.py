@@ -88,7 +88,7 @@ def stage_replace_all(u, v, flags, desc, s2n=None):
 
 
 def stage_correlate(fa, fb, geom, fsize, up=None, vpred=None, planes=False, subpixel=0, s2n_method=None,
-                    s2n_size=2):
+                    s2n_size=2, window_size=0):
     lib = _native.load_library()
     m, n = geom.field_shape
     da, db = dev(fa), dev(fb)
@@ -101,6 +101,7 @@ def stage_correlate(fa, fb, geom, fsize, up=None, vpred=None, planes=False, subp
     dup = dev(np.asarray(up, np.float32)) if up is not None else None
     dvp = dev(np.asarray(vpred, np.float32)) if vpred is not None else None
     desc = pass_desc(geom, fsize, subpixel=subpixel)
+    desc.window_size = window_size
     if s2n_method is not None:
         desc.s2n_method = _native.S2N_CODES[s2n_method]
         desc.s2n_size = s2n_size

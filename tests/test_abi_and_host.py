@@ -116,7 +116,7 @@ def test_min_search_size_and_frame_shape_assertions():
 
 @pytest.mark.parametrize("kwargs,name", [
     (dict(dtype_f="float64"), "dtype_f"), (dict(mask=np.zeros((256, 256), bool)), "mask"),
-    (dict(shrink_ratio=0.5), "shrink_ratio"), (dict(validation_size=2), "validation_size"),
+    (dict(validation_size=2), "validation_size"),
     (dict(replacing_size=2), "replacing_size"),
 ])
 def test_unimplemented_options_raise(kwargs, name):
@@ -139,6 +139,17 @@ def test_plan_desc_carries_every_validation_setting():
     assert np.isclose(d.passes[0].mean_tol, 1.0) and np.isnan(d.passes[1].mean_tol) and np.isnan(d.passes[2].mean_tol)
     none = gp.PIVGPU((256, 256), 16, median_tol=None, dtype_f="float32")._plan_desc(_native.PIV_U8)
     assert none.passes[0].validate == 0
+
+
+def test_shrink_ratio_sets_first_pass_window():
+    piv = gp.PIVGPU((256, 256), 16, search_size_iters=(1, 1), shrink_ratio=0.6, dtype_f="float32")
+    f0, f1 = piv.piv_fields
+    assert (f0.search_size, f0.window_size) == (32, 20) and (f1.search_size, f1.window_size) == (16, 16)
+    d = piv._plan_desc(_native.PIV_U8)
+    assert d.passes[0].window_size == 20 and d.passes[1].window_size == 0
+    # int(64 * 0.1) = 6 -> clipped to 8; odd sizes round up
+    assert gp.PIVGPU((256, 256), 64, shrink_ratio=0.1, dtype_f="float32").piv_fields[0].window_size == 8
+    assert gp.PIVGPU((256, 256), 32, shrink_ratio=0.55, dtype_f="float32").piv_fields[0].window_size == 18
 
 
 def test_gpu_piv_alias_and_frame_assertion():

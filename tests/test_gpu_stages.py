@@ -167,3 +167,36 @@ def test_smoothn_auto_s(shape):
     assert abs(p - p_exact) < 1e-6, (p, p_exact, p_ref)
     assert abs(p - p_ref) < 5e-3
     assert np.abs(gz_u - ru).max() < 1e-3 and np.abs(gz_v - rv).max() < 1e-3
+
+
+@pytest.mark.parametrize("ss,ws", [(32, 20), (64, 32), (16, 8)])
+def test_correlate_shrunk_interrogation_window(ss, ws):
+    """First pass with shrink_ratio: frame A's interrogation window inside the
+    search window (gpu_process.py:654-662, 1066-1081)."""
+    shape = (6 * ss, 5 * ss)
+    fa, fb = _frames(shape, np.uint8, seed=ss + ws)
+    geom = O.field_geometry(shape, ss, 0.5)
+    fsize = O.fft_size(ss)
+    ref = O.correlate_pass(fa, fb, geom, return_planes=True, window_size=ws)
+    got = gu.stage_correlate(fa, fb, geom, fsize, planes=True, window_size=ws)
+    scale = np.abs(ref["corr"]).reshape(len(ref["corr"]), -1).max(axis=1)[:, None, None]
+    assert (np.abs(got["corr"] - ref["corr"]) / scale).max() < 2e-5
+    ok = ~gu.near_tie_mask(ref["corr"]).reshape(geom.field_shape)
+    ok &= ~gu.fit_sign_ambiguous(ref["corr"], ref["i_peak"], ref["j_peak"]).reshape(geom.field_shape)
+    assert np.array_equal(got["i_peak"][ok], ref["i_peak"][ok])
+    assert np.abs(got["u"] - ref["u"])[ok].max() <= 1e-3 and np.abs(got["v"] - ref["v"])[ok].max() <= 1e-3
+
+
+def test_shrink_ratio_end_to_end():
+    from openpiv_gpu.gpu_process import PIVGPU
+    shape = (384, 384)
+    fa, fb = _frames(shape, np.uint8, seed=77, shift=(5.5, -4.2))
+    # fixed smoothing parameter: both sides smooth the first pass identically
+    # (the GCV search itself is checked elsewhere)
+    piv = PIVGPU(shape, 16, search_size_iters=(1, 1), shrink_ratio=0.6, median_tol=2, smoothing_par=0.8,
+                 dtype_f="float32")
+    u, v = piv.run_host(fa, fb)
+    s = O.OracleSettings(min_search_size=16, search_size_iters=(1, 1), shrink_ratio=0.6, smoothing_par=0.8)
+    r = O.piv(fa, fb, s)
+    err = np.maximum(np.abs(u - r["u"]), np.abs(v - r["v"]))
+    assert np.mean(err <= 1e-3) > 0.999, np.sort(err.reshape(-1))[-8:]

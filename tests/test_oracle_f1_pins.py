@@ -147,3 +147,25 @@ def test_unresolved_replacements_are_reset():
     ru, rv, _ = O.replace_outliers(u, v, val, "median", n_iters=1)
     assert np.all(ru[2:4, 2:4] == 7.0)      # NaN -> reset to the original
     assert np.all(ru[1, 1:5] == 1.0)        # border of the block resolved from valid neighbours
+
+
+def test_shrunk_window_correlation_by_direct_sum():
+    """Interrogation window (ws = 4) inside an 8 px search window, F = 16:
+    A zero outside its window and centred over it, B centred over the whole
+    window, then the direct circular correlation c[d] = sum_x a[x] b[x + d]."""
+    rng = np.random.default_rng(9)
+    wa = rng.random((2, 8, 8))
+    wb = rng.random((2, 8, 8))
+    got = O.correlate_windows(wa, wb, 16, True, win_pad=2)
+    for k in range(2):
+        a = np.zeros((16, 16))
+        b = np.zeros((16, 16))
+        inner = wa[k, 2:6, 2:6]
+        a[6:10, 6:10] = inner - inner.mean()
+        b[4:12, 4:12] = wb[k] - wb[k].mean()
+        c = np.zeros((16, 16))
+        for dy in range(16):
+            for dx in range(16):
+                c[dy, dx] = np.sum(a * np.roll(np.roll(b, -dy, 0), -dx, 1))
+        assert np.allclose(got[k], np.fft.fftshift(c), atol=1e-10)
+    assert O.shrunk_window(64, 0.1) == 8 and O.shrunk_window(32, 0.55) == 18 and O.shrunk_window(32, 0.6) == 20
