@@ -76,7 +76,8 @@ def stage_kernel_times(fa_u16, piv, reps=20):
                                                   u.ptr, v.ptr, None, None, None, None, sh))
         ms = timed(launch)
         flops = fft_flops_per_window(p.fft_size) * m * n
-        out.append({"pass": k, "kernel": f"corr_kernel<F={p.fft_size}>", "windows": m * n, "fft": p.fft_size,
+        out.append({"pass": k, "kernel": f"corr_kernel<F={p.fft_size},{'u16' if k == 0 else 'f32'}>",
+                    "windows": m * n, "fft": p.fft_size,
                     "avg_ms": ms, "flop": flops, "tflops": flops / (ms * 1e-3) / 1e12})
     # deformation (second pass grid): reads both uint16 frames, writes two FP32 frames
     p = desc.passes[min(1, desc.n_passes - 1)]
@@ -258,9 +259,16 @@ def main():
         log("per-kernel timing")
         kernels, deform = stage_kernel_times(fa, piv)
         dom = max(kernels, key=lambda r: r["avg_ms"])
+        traffic = None
+        tp = os.path.join(ROOT, "profiles", "hbm_traffic_2048_pipeline.json")
+        if os.path.exists(tp):
+            with open(tp) as f:
+                tk = json.load(f)["kernels"].get(dom["kernel"])
+            traffic = None if tk is None else tk["traffic_bytes"]
         roof = {"bound": "mfma", "compute": "fp32-valu (no matrix cores; FP32 vector peak)",
                 "kernel": dom["kernel"], "achieved": round(dom["tflops"], 3), "peak": 157.3, "unit": "TFLOP/s",
-                "frac": round(dom["tflops"] / 157.3, 4), "traffic": None,
+                "frac": round(dom["tflops"] / 157.3, 4), "traffic": traffic,
+                "traffic_source": "profiles/hbm_traffic_2048_pipeline.md (FETCH_SIZE x2 + WRITE_SIZE, copy-calibrated)",
                 "per_launch_flop": dom["flop"], "avg_launch_ms": round(dom["avg_ms"], 5)}
 
     learned = None
