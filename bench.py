@@ -177,6 +177,80 @@ def learned_vs_plain(n_pairs, weights_path):
     }
 
 
+def uneven_batch(n_pairs, reps):
+    """The uneven-batch workload: n_pairs device-rendered uint16 pairs (seed
+    4000+i) of sizes {512^2, 1024x768, 2048^2, 3072x2048, 4096^2}, 1-4
+    Lamb-Oseen vortices + shear, schedule 64->32->16 or 48->24 px, overlap
+    U(0.5, 0.75); batched (one deformation / correlation launch per pass level,
+    graph-captured, autotuned) vs one plan per pair."""
+    from openpiv_gpu import _native, synthetic
+    from openpiv_gpu.batch import PIVBatch, uneven_pair_spec
+    from openpiv_gpu.gpu_process import PIVGPU
+    specs = [uneven_pair_spec(4000 + i) for i in range(n_pairs)]
+    gens = {}
+    frames = []
+    truth = []
+    log(f"rendering {n_pairs} uneven pairs")
+    for i, (shape, mss, kw) in enumerate(specs):
+        if shape not in gens:
+            gens[shape] = synthetic.DevicePairGenerator(shape, synthetic.MODEL_12BIT)
+        disp = synthetic.vortex_field(4000 + i, shape)
+        frames.append(gens[shape].render(disp, 4000 + i))
+        truth.append(disp)
+    _native.synchronize()
+    log("building the batch")
+    t0 = time.perf_counter()
+    batch = PIVBatch(specs)
+    batch.set_frames(frames)
+    t_build = time.perf_counter() - t0
+    batch.run()                    # eager
+    choices = batch.tune(repeats=2)
+    batch.run()                    # capture + first replay
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.run()
+    dt_batch = (time.perf_counter() - t0) / reps
+    log("one plan per pair")
+    pivs = [PIVGPU(shape, mss, dtype_f="float32", **kw) for shape, mss, kw in specs]
+    plans = [p._get_plan(_native.PIV_U16) for p in pivs]
+    outs = []
+    for p, piv in zip(plans, pivs):
+        m, n = piv.piv_fields[-1].field_shape
+        outs.append((_native.DeviceBuffer((m, n), np.float32), _native.DeviceBuffer((m, n), np.float32)))
+
+    def seq():
+        for pl, (fa, fb), (u, v) in zip(plans, frames, outs):
+            pl.run(fa.value, fb.value, True, u.value, v.value, None, None, True)
+    seq()
+    seq()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        seq()
+    dt_seq = (time.perf_counter() - t0) / reps
+    se = cnt = 0.0
+    for i, piv in enumerate(pivs):
+        u, v, _, _ = batch.result(i)
+        x, y = (np.asarray(c, np.float64) for c in piv.coords)
+        ut, vt = truth[i](x, y)
+        se += float(np.sum((u - ut) ** 2 + (v - vt) ** 2))
+        cnt += 2 * u.size
+    px = sum(s[0][0] * s[0][1] for s in specs)
+    return {
+        "metric": "pairs/s (uneven batch)", "value": round(n_pairs / dt_batch, 2), "unit": "pairs/s",
+        "higher_is_better": True, "n_gpus": 1, "steps": reps, "data": "synthetic", "dtype": "f32",
+        "config": {"workload": f"{n_pairs} uneven uint16 pairs (512^2..4096^2, 64->32->16 or 48->24 px, "
+                               "overlap 0.5-0.75, Lamb-Oseen vortices + shear), frames resident in HBM",
+                   "megapixels_per_batch": round(px / 1e6, 1)},
+        "ms_per_batch": round(dt_batch * 1e3, 2),
+        "one_plan_per_pair_pairs_per_s": round(n_pairs / dt_seq, 2),
+        "batched_vs_per_pair": round(dt_seq / dt_batch, 3),
+        "megapixel_pairs_per_s": round(px / 1e6 / dt_batch, 1),
+        "rms_px_vs_truth": round(float(np.sqrt(se / cnt)), 5),
+        "autotuned_choices": [int(c) for c in choices[:12]],
+        "batch_build_s": round(t_build, 2),
+    }
+
+
 def cpu_baseline(fa, fb, pairs=1):
     from oracle import piv_oracle as O
     s = O.OracleSettings(min_search_size=16, search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2.0)
@@ -199,12 +273,18 @@ def main():
     ap.add_argument("--skip-kernel-timing", action="store_true")
     ap.add_argument("--learned-pairs", type=int, default=1000)
     ap.add_argument("--skip-learned", action="store_true")
+    ap.add_argument("--workload", choices=("pair2048", "uneven"), default="pair2048")
+    ap.add_argument("--pairs", type=int, default=256)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for {max(args.gpus, world)}): image pairs are independent, "
               "so several GPUs would each run this same loop on their share of the pair stream.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "uneven":
+        print(json.dumps(uneven_batch(args.pairs, max(1, args.steps))), flush=True)
+        return
 
     from openpiv_gpu import _native, synthetic
     from openpiv_gpu.gpu_process import PIVGPU

@@ -198,6 +198,28 @@ int piv_stage_correlate_dft(const float *win_a, const float *win_b, int n_window
                             int precision, int subpixel, float *u, float *v, int32_t *i_peak, int32_t *j_peak,
                             float *planes, void *stream);
 
+/* ---- batches of uneven pairs -------------------------------------------
+ * No counterpart in the reference, which processes one pair per
+ * PIVGPU.__call__ (gpu_process.py:564-647).  A batch holds n_pairs plans
+ * (each with its own frame shape and schedule, one shared input dtype).
+ * Every pass level runs one deformation launch and one correlation launch
+ * per FFT size over all pairs, then each pair's field stages; the loop is
+ * one hipGraph after the first run.  Frames are caller-owned device
+ * buffers registered once (read in place; re-register after moving them).
+ * piv_batch_tune times the launch-shape candidates of every level on the
+ * registered frames and keeps the fastest; choices (optional) receives, per
+ * level, the deformation block width then each correlation group's
+ * XCD-banded flag. */
+typedef struct piv_batch piv_batch;
+int piv_batch_create(int device, int n_pairs, const piv_plan_desc *descs, piv_batch **out);
+void piv_batch_destroy(piv_batch *batch);
+int piv_batch_set_frames(piv_batch *batch, const void *const *frames_a, const void *const *frames_b);
+int piv_batch_run(piv_batch *batch);
+int piv_batch_tune(piv_batch *batch, int repeats, int32_t *choices);
+int piv_batch_pair_shape(const piv_batch *batch, int pair, int32_t *m, int32_t *n);
+int piv_batch_get(const piv_batch *batch, int pair, float *u, float *v, uint8_t *val_init, uint8_t *val_final,
+                  int outputs_on_device);
+
 /* ---- learned coarse-displacement predictor -----------------------------
  * No counterpart in the reference: there the deformation predictor of a
  * pass is always the previous pass's field (gpu_process.py:620-625).  A

@@ -277,6 +277,8 @@ struct piv_plan {
   hipGraph_t lgraph = nullptr;
   hipGraphExec_t lgraph_exec = nullptr;
   hipEvent_t inputs_ready = nullptr;  // null-stream work that produced device-resident frames
+  const void* ext_a = nullptr;  // caller-owned device frames read in place (batches)
+  const void* ext_b = nullptr;
   int lruns = 0;
   int32_t last_fallback = 0;
   int32_t last_rejected = 0;
@@ -284,13 +286,16 @@ struct piv_plan {
 
 namespace {
 
+const void* src_a(const piv_plan* P) { return P->ext_a ? P->ext_a : P->frame_a; }
+const void* src_b(const piv_plan* P) { return P->ext_b ? P->ext_b : P->frame_b; }
+
 // Pass k > 0 (or the learned path's single pass): deform the original
 // frames by the total predictor B.up/B.vp (gpu_process.py:1113-1114, Q13)
 // and correlate the residual, adding the predictor back (:628-632).
 int deform_and_correlate(piv_plan* P, PassBuffers& B, hipStream_t s) {
   const piv_plan_desc& D = P->desc;
   HIP_TRY(launch_deform(
-      deform_args(P->frame_a, P->frame_b, D.in_dtype, D.height, D.width, B.d, B.up, B.vp, P->def_a, P->def_b), s));
+      deform_args(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d, B.up, B.vp, P->def_a, P->def_b), s));
   CorrArgs a = corr_args(P->def_a, P->def_b, kF32, D.height, D.width, B.d);
   a.up = B.up;
   a.vp = B.vp;
@@ -428,7 +433,7 @@ int enqueue_passes(piv_plan* P, int first) {
     const bool last = k == np - 1;
     int r = PIV_OK;
     if (k == 0) {
-      CorrArgs a = corr_args(P->frame_a, P->frame_b, D.in_dtype, D.height, D.width, B.d);
+      CorrArgs a = corr_args(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d);
       a.u = B.u;
       a.v = B.v;
       a.s2n = B.s2n;
@@ -461,7 +466,7 @@ int enqueue_learned(piv_plan* P) {
   PassBuffers& B = P->passes[P->net_first];
   HIP_TRY(launch_fill_i32(P->counters, 0, 2 * (int)P->passes.size(), s));
   HIP_TRY(launch_fill_i32(P->lb.it_counts, 0, PIV_MAX_REPLACING_ITERS, s));
-  HIP_TRY(launch_prepare_input(P->frame_a, P->frame_b, D.in_dtype, D.height, D.width, kCnnFactor, P->net_scratch,
+  HIP_TRY(launch_prepare_input(src_a(P), src_b(P), D.in_dtype, D.height, D.width, kCnnFactor, P->net_scratch,
                                P->net_in, s));
   int r = cnn_enqueue_forward(P->net, P->net_in, 1, s);
   if (r) return r;
@@ -1029,6 +1034,384 @@ int piv_event_elapsed_ms(void* start, void* stop, float* ms) {
   if (!start || !stop || !ms) return fail(PIV_E_INVALID, "null argument");
   HIP_TRY(hipEventSynchronize((hipEvent_t)stop));
   HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return PIV_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches of pairs with their own frame shapes and schedules.  Pass level L
+// of every pair runs as: per-pair predictor interpolation, ONE deformation
+// launch over all pairs' frames, ONE correlation launch per (FFT size, input
+// dtype) over all pairs' windows, then the per-pair field stages
+// (validation, replacement, smoothn).  The whole loop is one hipGraph after
+// the first (eager) run.  Launch shapes per level (XCD-banded or linear
+// block order for the correlation, block width of the deformation) come from
+// piv_batch_tune, which times the candidates on the batch's own data.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct BatchGroup {  // one batched correlation launch
+  int fsize = 0, in_dtype = 0, total_blocks = 0, max_ss = 0, xcd = 1;
+  std::vector<int> plans;
+  CorrArgs* d_fields = nullptr;
+  int* d_prefix = nullptr;
+};
+
+struct BatchLevel {
+  std::vector<int> plans;  // pairs with a pass at this level
+  std::vector<BatchGroup> groups;
+  DeformArgs* d_deform = nullptr;  // pairs with L > 0
+  int* d_rows = nullptr;
+  int n_deform = 0, total_rows = 0, max_wd = 0, deform_threads = 256;
+};
+
+}  // namespace
+
+struct piv_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<piv_plan*> plans;
+  std::vector<BatchLevel> levels;
+  bool frames_set = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int runs = 0;
+};
+
+namespace {
+
+void batch_free_tables(piv_batch* b) {
+  for (auto& L : b->levels) {
+    for (auto& g : L.groups) {
+      if (g.d_fields) (void)hipFree(g.d_fields);
+      if (g.d_prefix) (void)hipFree(g.d_prefix);
+    }
+    if (L.d_deform) (void)hipFree(L.d_deform);
+    if (L.d_rows) (void)hipFree(L.d_rows);
+  }
+}
+
+void batch_drop_graph(piv_batch* b) {
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+  b->runs = 0;
+}
+
+void batch_release(piv_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  batch_drop_graph(b);
+  batch_free_tables(b);
+  for (auto* P : b->plans) release(P);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// Device tables of every level for the current frames (called once the
+// frame pointers are known; the plans' own buffers never move).
+int batch_build_tables(piv_batch* b) {
+  batch_free_tables(b);
+  for (size_t l = 0; l < b->levels.size(); ++l) {
+    BatchLevel& L = b->levels[l];
+    std::vector<int> keep_xcd;
+    for (auto& g : L.groups) keep_xcd.push_back(g.xcd);
+    const int threads = L.deform_threads;
+    L = BatchLevel{};
+    L.deform_threads = threads;
+    for (int p = 0; p < (int)b->plans.size(); ++p)
+      if ((int)b->plans[p]->passes.size() > (int)l) L.plans.push_back(p);
+    // correlation groups by (FFT size, input dtype)
+    for (int p : L.plans) {
+      piv_plan* P = b->plans[p];
+      const PassBuffers& B = P->passes[l];
+      const int dtype = l == 0 ? P->desc.in_dtype : (int)kF32;
+      BatchGroup* g = nullptr;
+      for (auto& q : L.groups)
+        if (q.fsize == B.d.fft_size && q.in_dtype == dtype) g = &q;
+      if (!g) {
+        L.groups.push_back(BatchGroup{});
+        g = &L.groups.back();
+        g->fsize = B.d.fft_size;
+        g->in_dtype = dtype;
+        g->xcd = L.groups.size() <= keep_xcd.size() ? keep_xcd[L.groups.size() - 1] : 1;
+      }
+      g->plans.push_back(p);
+    }
+    for (auto& g : L.groups) {
+      std::vector<CorrArgs> fields;
+      std::vector<int> prefix;
+      const int wpb = correlate_windows_per_block(g.fsize);
+      g.total_blocks = 0;
+      for (int p : g.plans) {
+        piv_plan* P = b->plans[p];
+        PassBuffers& B = P->passes[l];
+        const piv_plan_desc& D = P->desc;
+        CorrArgs a = l == 0 ? corr_args(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d)
+                            : corr_args(P->def_a, P->def_b, kF32, D.height, D.width, B.d);
+        if (l > 0) {
+          a.up = B.up;
+          a.vp = B.vp;
+        }
+        a.u = B.u;
+        a.v = B.v;
+        a.s2n = B.s2n;
+        fields.push_back(a);
+        prefix.push_back(g.total_blocks);
+        g.total_blocks += (B.N + wpb - 1) / wpb;
+        g.max_ss = std::max(g.max_ss, B.d.search_size);
+      }
+      if (hipMalloc((void**)&g.d_fields, fields.size() * sizeof(CorrArgs)) != hipSuccess ||
+          hipMalloc((void**)&g.d_prefix, prefix.size() * sizeof(int)) != hipSuccess)
+        return fail(PIV_E_NOMEM, "batch: table allocation failed");
+      HIP_TRY(hipMemcpy(g.d_fields, fields.data(), fields.size() * sizeof(CorrArgs), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(g.d_prefix, prefix.data(), prefix.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (l > 0) {
+      std::vector<DeformArgs> dfs;
+      std::vector<int> rows;
+      for (int p : L.plans) {
+        piv_plan* P = b->plans[p];
+        const PassBuffers& B = P->passes[l];
+        const piv_plan_desc& D = P->desc;
+        dfs.push_back(deform_args(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d, B.up, B.vp, P->def_a,
+                                  P->def_b));
+        rows.push_back(L.total_rows);
+        L.total_rows += D.height;
+        L.max_wd = std::max(L.max_wd, D.width);
+      }
+      L.n_deform = (int)dfs.size();
+      if (L.n_deform > 0) {
+        if (hipMalloc((void**)&L.d_deform, dfs.size() * sizeof(DeformArgs)) != hipSuccess ||
+            hipMalloc((void**)&L.d_rows, rows.size() * sizeof(int)) != hipSuccess)
+          return fail(PIV_E_NOMEM, "batch: table allocation failed");
+        HIP_TRY(hipMemcpy(L.d_deform, dfs.data(), dfs.size() * sizeof(DeformArgs), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(L.d_rows, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice));
+      }
+    }
+  }
+  return PIV_OK;
+}
+
+int batch_enqueue_corr(piv_batch* b, BatchGroup& g, hipStream_t s) {
+  CorrBatch cb{g.d_fields, g.d_prefix, (int)g.plans.size(), g.xcd};
+  HIP_TRY(launch_correlate_batch(cb, g.fsize, g.in_dtype, g.total_blocks, g.max_ss, s));
+  return PIV_OK;
+}
+
+int batch_enqueue_deform(piv_batch* b, BatchLevel& L, hipStream_t s) {
+  if (L.n_deform == 0) return PIV_OK;
+  const int in_dtype = b->plans[L.plans[0]]->desc.in_dtype;
+  HIP_TRY(launch_deform_batch(L.d_deform, L.d_rows, L.n_deform, L.total_rows, L.max_wd, in_dtype, L.deform_threads,
+                              s));
+  return PIV_OK;
+}
+
+int batch_enqueue(piv_batch* b) {
+  hipStream_t s = b->stream;
+  for (auto* P : b->plans) {
+    HIP_TRY(launch_fill_i32(P->counters, 0, 2 * (int)P->passes.size(), s));
+    HIP_TRY(launch_fill_i32(P->lb.it_counts, 0, PIV_MAX_REPLACING_ITERS, s));
+  }
+  for (size_t l = 0; l < b->levels.size(); ++l) {
+    BatchLevel& L = b->levels[l];
+    if (l > 0) {
+      for (int p : L.plans) {
+        int r = predictor_from_previous(b->plans[p], (int)l, s);
+        if (r) return r;
+      }
+      int r = batch_enqueue_deform(b, L, s);
+      if (r) return r;
+    }
+    for (auto& g : L.groups) {
+      int r = batch_enqueue_corr(b, g, s);
+      if (r) return r;
+    }
+    for (int p : L.plans) {
+      piv_plan* P = b->plans[p];
+      int r = validate_and_smooth(P, (int)l, l + 1 == P->passes.size(), s);
+      if (r) return r;
+    }
+  }
+  for (auto* P : b->plans) {
+    if (P->desc.out_scale != 1.0f) {
+      PassBuffers& Lp = P->passes.back();
+      HIP_TRY(launch_scale(Lp.u, Lp.v, Lp.N, P->desc.out_scale, s));
+    }
+  }
+  return PIV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int piv_batch_create(int device, int n_pairs, const piv_plan_desc* descs, piv_batch** out) {
+  if (!out || !descs || n_pairs < 1) return fail(PIV_E_INVALID, "batch: bad argument");
+  *out = nullptr;
+  if (piv_device_count() <= 0) return fail(PIV_E_NO_DEVICE, "no HIP device visible");
+  HIP_TRY(hipSetDevice(device));
+  piv_batch* b = new piv_batch();
+  b->device = device;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    batch_release(b);
+    return fail(PIV_E_HIP, "batch: hipStreamCreate failed");
+  }
+  size_t max_passes = 0;
+  for (int i = 0; i < n_pairs; ++i) {
+    if (descs[i].in_dtype != descs[0].in_dtype) {
+      batch_release(b);
+      return fail(PIV_E_INVALID, "batch: every pair must share the input dtype");
+    }
+    piv_plan* P = nullptr;
+    int r = piv_plan_create(device, &descs[i], &P);
+    if (r) {
+      batch_release(b);
+      return r;
+    }
+    b->plans.push_back(P);
+    max_passes = std::max(max_passes, P->passes.size());
+  }
+  b->levels.resize(max_passes);
+  *out = b;
+  return PIV_OK;
+}
+
+void piv_batch_destroy(piv_batch* b) { batch_release(b); }
+
+int piv_batch_set_frames(piv_batch* b, const void* const* frames_a, const void* const* frames_b) {
+  if (!b || !frames_a || !frames_b) return fail(PIV_E_INVALID, "batch: bad argument");
+  HIP_TRY(hipSetDevice(b->device));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < b->plans.size(); ++i) {
+    if (!frames_a[i] || !frames_b[i]) return fail(PIV_E_INVALID, "batch: null frame for pair %zu", i);
+    b->plans[i]->ext_a = frames_a[i];
+    b->plans[i]->ext_b = frames_b[i];
+  }
+  batch_drop_graph(b);
+  int r = batch_build_tables(b);
+  if (r) return r;
+  b->frames_set = true;
+  return PIV_OK;
+}
+
+int piv_batch_run(piv_batch* b) {
+  if (!b || !b->frames_set) return fail(PIV_E_INVALID, "batch: frames not set");
+  HIP_TRY(hipSetDevice(b->device));
+  hipStream_t s = b->stream;
+  // the frames may come from null-stream work (see piv_run)
+  HIP_TRY(hipEventRecord(b->plans[0]->inputs_ready, nullptr));
+  HIP_TRY(hipStreamWaitEvent(s, b->plans[0]->inputs_ready, 0));
+  if (b->runs > 0) {
+    if (!b->graph_exec) {
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int r = batch_enqueue(b);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+      }
+      if (e != hipSuccess) return fail(PIV_E_HIP, "batch: graph capture failed: %s", hipGetErrorString(e));
+      b->graph = g;
+      HIP_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(b->graph_exec, s));
+  } else {
+    int r = batch_enqueue(b);
+    if (r) return r;
+  }
+  b->runs++;
+  HIP_TRY(hipStreamSynchronize(s));
+  return PIV_OK;
+}
+
+int piv_batch_tune(piv_batch* b, int repeats, int32_t* choices) {
+  if (!b || !b->frames_set) return fail(PIV_E_INVALID, "batch: frames not set");
+  HIP_TRY(hipSetDevice(b->device));
+  if (repeats < 1) repeats = 1;
+  hipStream_t s = b->stream;
+  // one eager run fills every buffer the candidates read
+  int r = batch_enqueue(b);
+  if (r) return r;
+  HIP_TRY(hipStreamSynchronize(s));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  auto time_it = [&](auto&& launch) -> float {
+    float best = 1e30f;
+    for (int k = 0; k < repeats + 1; ++k) {
+      (void)hipEventRecord(e0, s);
+      if (launch()) return 1e30f;
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (k > 0) best = std::min(best, ms);  // the first repetition warms up
+    }
+    return best;
+  };
+  int ci = 0;
+  for (size_t l = 0; l < b->levels.size(); ++l) {
+    BatchLevel& L = b->levels[l];
+    if (l > 0 && L.n_deform > 0) {
+      int best_t = L.deform_threads;
+      float best = 1e30f;
+      for (int t : {128, 256, 512}) {
+        L.deform_threads = t;
+        const float ms = time_it([&] { return batch_enqueue_deform(b, L, s); });
+        if (ms < best) {
+          best = ms;
+          best_t = t;
+        }
+      }
+      L.deform_threads = best_t;
+    }
+    if (choices) choices[ci++] = L.deform_threads;
+    for (auto& g : L.groups) {
+      int best_x = g.xcd;
+      float best = 1e30f;
+      for (int x : {1, 0}) {
+        g.xcd = x;
+        const float ms = time_it([&] { return batch_enqueue_corr(b, g, s); });
+        if (ms < best) {
+          best = ms;
+          best_x = x;
+        }
+      }
+      g.xcd = best_x;
+      if (choices) choices[ci++] = g.xcd;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  batch_drop_graph(b);  // the next run captures the tuned shapes
+  return PIV_OK;
+}
+
+int piv_batch_pair_shape(const piv_batch* b, int pair, int32_t* m, int32_t* n) {
+  if (!b || pair < 0 || pair >= (int)b->plans.size()) return fail(PIV_E_INVALID, "batch: pair out of range");
+  const PassBuffers& L = b->plans[pair]->passes.back();
+  if (m) *m = L.d.field_m;
+  if (n) *n = L.d.field_n;
+  return PIV_OK;
+}
+
+int piv_batch_get(const piv_batch* b, int pair, float* u, float* v, uint8_t* val_init, uint8_t* val_final,
+                  int outputs_on_device) {
+  if (!b || pair < 0 || pair >= (int)b->plans.size()) return fail(PIV_E_INVALID, "batch: pair out of range");
+  HIP_TRY(hipSetDevice(b->device));
+  const piv_plan* P = b->plans[pair];
+  const PassBuffers& L = P->passes.back();
+  const hipMemcpyKind k = outputs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (u) HIP_TRY(hipMemcpyAsync(u, L.u, L.N * sizeof(float), k, b->stream));
+  if (v) HIP_TRY(hipMemcpyAsync(v, L.v, L.N * sizeof(float), k, b->stream));
+  if (val_init) HIP_TRY(hipMemcpyAsync(val_init, L.flags, L.N, k, b->stream));
+  if (val_final) HIP_TRY(hipMemcpyAsync(val_final, P->val_final, L.N, k, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
   return PIV_OK;
 }
 

@@ -71,8 +71,12 @@ size_t correlate_lds_bytes(int fsize, int ss) {
   return (m + 15) & ~size_t(15);
 }
 
-template <int F, typename InT>
-__global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a, int n_windows, int region_floats) {
+// BATCHED: one launch over the windows of many fields (pairs); block b
+// belongs to the last field whose block_prefix entry is <= b, so every block
+// works on one field and the field's arguments stay wave-uniform.
+template <int F, typename InT, bool BATCHED>
+__global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_in, CorrBatch batch,
+                                                                     int n_windows_in, int region_floats) {
   using L = Layout<F>;
   constexpr int H = L::H, Q = L::Q, LANES = L::LANES, WPB = L::WPB;
   constexpr int SF = L::SF, SR = L::SR, SP = L::SP;
@@ -92,10 +96,26 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a,
   // XCDs; give each XCD a contiguous band of windows so overlapping windows
   // share an L2.
   int bid = blockIdx.x;
-  {
+  if (!BATCHED || batch.xcd) {
     const int nb = gridDim.x;
     const int per = nb / 8;
     if (per > 0 && bid < per * 8) bid = (bid & 7) * per + (bid >> 3);
+  }
+  CorrArgs a;
+  int n_windows;
+  if constexpr (BATCHED) {
+    int lo = 0, hi = batch.n_fields - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (batch.block_prefix[mid] <= bid) lo = mid;
+      else hi = mid - 1;
+    }
+    a = batch.fields[lo];
+    bid -= batch.block_prefix[lo];
+    n_windows = a.m * a.n;
+  } else {
+    a = a_in;
+    n_windows = n_windows_in;
   }
   const int win = bid * WPB + wl;
   const bool valid = win < n_windows;
@@ -532,6 +552,23 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a,
   }
 }
 
+// Dynamic LDS above 64 KiB must be opted into; request exactly what a
+// launch needs (gfx950 allows up to 160 KiB per workgroup).
+template <int F, typename InT, bool BATCHED>
+static hipError_t lds_attr(size_t lds) {
+  static thread_local size_t attr_bytes = 0;
+  if (lds > 65536 && lds > attr_bytes) {
+    hipError_t e = hipFuncSetAttribute((const void*)corr_kernel<F, InT, BATCHED>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) {
+      set_launch_detail("hipFuncSetAttribute(MaxDynamicSharedMemorySize)", e);
+      return e;
+    }
+    attr_bytes = lds;
+  }
+  return hipSuccess;
+}
+
 template <int F, typename InT>
 static hipError_t launch_f(const CorrArgs& a, hipStream_t stream) {
   using L = Layout<F>;
@@ -540,23 +577,38 @@ static hipError_t launch_f(const CorrArgs& a, hipStream_t stream) {
   const size_t lds = region * L::WPB;
   const int blocks = (n_windows + L::WPB - 1) / L::WPB;
   (void)hipGetLastError();  // do not report an earlier, unrelated error
-  // Dynamic LDS above 64 KiB must be opted into; request exactly what this
-  // launch needs (gfx950 allows up to 160 KiB per workgroup).
-  static thread_local size_t attr_bytes = 0;
-  if (lds > 65536 && lds > attr_bytes) {
-    hipError_t e = hipFuncSetAttribute((const void*)corr_kernel<F, InT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds);
-    if (e != hipSuccess) {
-      set_launch_detail("hipFuncSetAttribute(MaxDynamicSharedMemorySize)", e);
-      return e;
-    }
-    attr_bytes = lds;
-  }
-  hipLaunchKernelGGL((corr_kernel<F, InT>), dim3(blocks), dim3(256), lds, stream, a, n_windows,
+  hipError_t e = lds_attr<F, InT, false>(lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((corr_kernel<F, InT, false>), dim3(blocks), dim3(256), lds, stream, a, CorrBatch{}, n_windows,
                      (int)(region / sizeof(float)));
-  hipError_t e = hipGetLastError();
+  e = hipGetLastError();
   if (e != hipSuccess) set_launch_detail("corr_kernel launch", e);
   return e;
+}
+
+template <int F, typename InT>
+static hipError_t launch_fb(const CorrBatch& b, int total_blocks, int max_ss, hipStream_t stream) {
+  using L = Layout<F>;
+  const size_t region = correlate_lds_bytes(F, max_ss);
+  const size_t lds = region * L::WPB;
+  (void)hipGetLastError();
+  hipError_t e = lds_attr<F, InT, true>(lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((corr_kernel<F, InT, true>), dim3(total_blocks), dim3(256), lds, stream, CorrArgs{}, b, 0,
+                     (int)(region / sizeof(float)));
+  e = hipGetLastError();
+  if (e != hipSuccess) set_launch_detail("corr_kernel (batched) launch", e);
+  return e;
+}
+
+template <int F>
+static hipError_t launch_tb(const CorrBatch& b, int in_dtype, int total_blocks, int max_ss, hipStream_t stream) {
+  switch (in_dtype) {
+    case kU8: return launch_fb<F, uint8_t>(b, total_blocks, max_ss, stream);
+    case kU16: return launch_fb<F, uint16_t>(b, total_blocks, max_ss, stream);
+    case kF32: return launch_fb<F, float>(b, total_blocks, max_ss, stream);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <int F>
@@ -565,6 +617,20 @@ static hipError_t launch_t(const CorrArgs& a, hipStream_t stream) {
     case kU8: return launch_f<F, uint8_t>(a, stream);
     case kU16: return launch_f<F, uint16_t>(a, stream);
     case kF32: return launch_f<F, float>(a, stream);
+  }
+  return hipErrorInvalidValue;
+}
+
+int correlate_windows_per_block(int fsize) { return 256 / (2 * fsize); }
+
+hipError_t launch_correlate_batch(const CorrBatch& b, int fsize, int in_dtype, int total_blocks, int max_ss,
+                                  hipStream_t stream) {
+  if (total_blocks <= 0) return hipSuccess;
+  switch (fsize) {
+    case 16: return launch_tb<16>(b, in_dtype, total_blocks, max_ss, stream);
+    case 32: return launch_tb<32>(b, in_dtype, total_blocks, max_ss, stream);
+    case 64: return launch_tb<64>(b, in_dtype, total_blocks, max_ss, stream);
+    case 128: return launch_tb<128>(b, in_dtype, total_blocks, max_ss, stream);
   }
   return hipErrorInvalidValue;
 }

@@ -57,10 +57,7 @@ __device__ __forceinline__ float warp_px(const void* f, int ht, int wd, float X,
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void deform_kernel(DeformArgs a) {
-  const int X = blockIdx.x * blockDim.x + threadIdx.x;
-  const int Y = blockIdx.y;
-  if (X >= a.wd) return;
+__device__ __forceinline__ void deform_px(const DeformArgs& a, int X, int Y) {
   float up, vp;
   grid_bilinear(a.up, a.vp, a.m, a.n, a.y0, a.x0, a.spacing, (float)Y, (float)X, up, vp);
   const size_t idx = (size_t)Y * a.wd + X;
@@ -68,6 +65,31 @@ __global__ __launch_bounds__(256) void deform_kernel(DeformArgs a) {
                           : ld<T>(a.frame_a, idx);
   a.out_b[idx] = a.warp_b ? warp_px<T>(a.frame_b, a.ht, a.wd, (float)X, (float)Y, up, vp, a.dt_b)
                           : ld<T>(a.frame_b, idx);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void deform_kernel(DeformArgs a) {
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y;
+  if (X >= a.wd) return;
+  deform_px<T>(a, X, Y);
+}
+
+// Many frames in one launch: grid.y runs over the concatenated rows.
+template <typename T>
+__global__ __launch_bounds__(512) void deform_batch_kernel(const DeformArgs* fields, const int* row_prefix,
+                                                           int n_fields) {
+  const int r = blockIdx.y;
+  int lo = 0, hi = n_fields - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (row_prefix[mid] <= r) lo = mid;
+    else hi = mid - 1;
+  }
+  const DeformArgs a = fields[lo];
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  if (X >= a.wd) return;
+  deform_px<T>(a, X, r - row_prefix[lo]);
 }
 
 __global__ __launch_bounds__(256) void grid_interp_kernel(GridInterpArgs a) {
@@ -764,6 +786,20 @@ hipError_t launch_deform(const DeformArgs& a, hipStream_t stream) {
     case kU8: hipLaunchKernelGGL(deform_kernel<uint8_t>, grid, dim3(256), 0, stream, a); break;
     case kU16: hipLaunchKernelGGL(deform_kernel<uint16_t>, grid, dim3(256), 0, stream, a); break;
     case kF32: hipLaunchKernelGGL(deform_kernel<float>, grid, dim3(256), 0, stream, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_deform_batch(const DeformArgs* fields, const int* row_prefix, int n_fields, int total_rows,
+                               int max_wd, int in_dtype, int threads, hipStream_t stream) {
+  if (total_rows <= 0) return hipSuccess;
+  if (threads != 128 && threads != 256 && threads != 512) return hipErrorInvalidValue;
+  dim3 grid((max_wd + threads - 1) / threads, total_rows);
+  switch (in_dtype) {
+    case kU8: hipLaunchKernelGGL(deform_batch_kernel<uint8_t>, grid, dim3(threads), 0, stream, fields, row_prefix, n_fields); break;
+    case kU16: hipLaunchKernelGGL(deform_batch_kernel<uint16_t>, grid, dim3(threads), 0, stream, fields, row_prefix, n_fields); break;
+    case kF32: hipLaunchKernelGGL(deform_batch_kernel<float>, grid, dim3(threads), 0, stream, fields, row_prefix, n_fields); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -37,6 +37,19 @@ struct CorrArgs {
 };
 
 hipError_t launch_correlate(const CorrArgs& a, hipStream_t stream);
+
+// One launch over the windows of many fields of the same FFT size and input
+// dtype (device tables; every field's windows padded to whole blocks of
+// correlate_windows_per_block(F)).  xcd: XCD-banded block order.
+struct CorrBatch {
+  const CorrArgs* fields;
+  const int* block_prefix;  // n_fields entries: first block of each field
+  int n_fields;
+  int xcd;
+};
+int correlate_windows_per_block(int fsize);
+hipError_t launch_correlate_batch(const CorrBatch& b, int fsize, int in_dtype, int total_blocks, int max_ss,
+                                  hipStream_t stream);
 size_t correlate_lds_bytes(int fsize, int ss);
 
 // Frame deformation: predictor bilinearly interpolated (with linear
@@ -58,6 +71,10 @@ struct DeformArgs {
   float* out_b;
 };
 hipError_t launch_deform(const DeformArgs& a, hipStream_t stream);
+// One launch over the rows of many frames (device tables); row r belongs to
+// the last frame whose row_prefix entry is <= r.  threads: block width.
+hipError_t launch_deform_batch(const DeformArgs* fields, const int* row_prefix, int n_fields, int total_rows,
+                               int max_wd, int in_dtype, int threads, hipStream_t stream);
 
 // Linear interpolation with extrapolation between two regular grids.
 struct GridInterpArgs {

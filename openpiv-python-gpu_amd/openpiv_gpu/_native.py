@@ -34,6 +34,8 @@ EXPORTED_SYMBOLS = (
     "piv_cnn_create", "piv_cnn_destroy", "piv_cnn_layout", "piv_cnn_set_params", "piv_cnn_get_params",
     "piv_cnn_get_grads", "piv_cnn_get_buffer", "piv_cnn_forward", "piv_cnn_gradients", "piv_cnn_adam_step",
     "piv_cnn_prepare_input", "piv_plan_set_predictor", "piv_plan_last_fallback",
+    "piv_batch_create", "piv_batch_destroy", "piv_batch_set_frames", "piv_batch_run", "piv_batch_tune",
+    "piv_batch_pair_shape", "piv_batch_get",
 )
 
 SYNTH_MAX_VORTICES = 8
@@ -154,6 +156,13 @@ def _bind(lib):
         "piv_cnn_prepare_input": (i32, [vp, vp, vp, i32, i32, i32, vp, vp]),
         "piv_plan_set_predictor": (i32, [vp, vp, i32, f32]),
         "piv_plan_last_fallback": (i32, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+        "piv_batch_create": (i32, [i32, i32, ctypes.POINTER(PlanDesc), ctypes.POINTER(vp)]),
+        "piv_batch_destroy": (None, [vp]),
+        "piv_batch_set_frames": (i32, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+        "piv_batch_run": (i32, [vp]),
+        "piv_batch_tune": (i32, [vp, i32, vp]),
+        "piv_batch_pair_shape": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+        "piv_batch_get": (i32, [vp, i32, vp, vp, vp, vp, i32]),
         "piv_set_device": (i32, [i32]),
         "piv_malloc": (i32, [ctypes.POINTER(vp), ctypes.c_size_t]),
         "piv_free": (i32, [vp]),
