@@ -251,6 +251,44 @@ def uneven_batch(n_pairs, reps):
     }
 
 
+def tiff_stream(fa_dev_pairs, n_pairs, workdir):
+    """Host-to-host rate of a stream of 2048x2048 uint16 TIFF pairs on disk:
+    library TIFF decoding into pinned buffers, upload on a copy stream
+    overlapped with the previous pair's compute, results back to the host;
+    beside it the sequential host path (decode, piv_gpu on NumPy frames)."""
+    from openpiv_gpu.gpu_process import PIVGPU
+    from openpiv_gpu.io import PairStream, read_tiff
+    try:
+        from PIL import Image
+    except ImportError:
+        return {"skipped": "PIL not importable (needed to write the TIFF files)"}
+    os.makedirs(workdir, exist_ok=True)
+    files = []
+    for k, (a, b) in enumerate(fa_dev_pairs):
+        pa, pb = os.path.join(workdir, f"{k}_a.tif"), os.path.join(workdir, f"{k}_b.tif")
+        Image.frombytes("I;16", a.shape[::-1], a.tobytes()).save(pa)
+        Image.frombytes("I;16", b.shape[::-1], b.tobytes()).save(pb)
+        files.append((pa, pb))
+    seq = [files[i % len(files)] for i in range(n_pairs)]
+    kw = dict(search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2, dtype_f="float32")
+    stream = PairStream(PIVGPU((2048, 2048), 16, **kw), depth=3, workers=4)
+    for _ in stream.run(seq[:4]):
+        pass
+    t0 = time.perf_counter()
+    for _ in stream.run(seq):
+        pass
+    t_stream = time.perf_counter() - t0
+    piv = PIVGPU((2048, 2048), 16, **kw)
+    piv.run_host(read_tiff(seq[0][0]), read_tiff(seq[0][1]))
+    t0 = time.perf_counter()
+    for pa, pb in seq:
+        piv.run_host(read_tiff(pa), read_tiff(pb))
+    t_seq = time.perf_counter() - t0
+    return {"pairs": n_pairs, "stream_pairs_per_s": round(n_pairs / t_stream, 2),
+            "sequential_host_pairs_per_s": round(n_pairs / t_seq, 2),
+            "note": "2048x2048 uint16 TIFF pairs from disk (page cache), decode + PCIe + compute + results to host"}
+
+
 def cpu_baseline(fa, fb, pairs=1):
     from oracle import piv_oracle as O
     s = O.OracleSettings(min_search_size=16, search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2.0)
@@ -274,6 +312,7 @@ def main():
     ap.add_argument("--learned-pairs", type=int, default=1000)
     ap.add_argument("--skip-learned", action="store_true")
     ap.add_argument("--workload", choices=("pair2048", "uneven"), default="pair2048")
+    ap.add_argument("--stream-pairs", type=int, default=48)
     ap.add_argument("--pairs", type=int, default=256)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -351,6 +390,13 @@ def main():
                 "traffic_source": "profiles/hbm_traffic_2048_pipeline.md (FETCH_SIZE x2 + WRITE_SIZE, copy-calibrated)",
                 "per_launch_flop": dom["flop"], "avg_launch_ms": round(dom["avg_ms"], 5)}
 
+    stream = None
+    if args.stream_pairs > 0:
+        log("TIFF pair stream")
+        import tempfile
+        with tempfile.TemporaryDirectory() as tmp:
+            stream = tiff_stream([(fa, fb)], args.stream_pairs, tmp)
+
     learned = None
     if not args.skip_learned and args.learned_pairs > 0:
         from openpiv_gpu.learned import default_weights_path
@@ -378,6 +424,7 @@ def main():
             "frac": round(deform["gbs"] / 8000.0, 4), "avg_launch_ms": round(deform["avg_ms"], 5),
             "per_launch_bytes": deform["bytes"]},
         "learned_vs_plain": learned,
+        "tiff_stream": stream,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -302,6 +302,23 @@ typedef struct piv_synth_model {
 int piv_synth_pair(const piv_synth_field *field, const piv_synth_model *model, uint64_t seed, int height, int width,
                    void *frame_a, void *frame_b, int32_t *accum, float *particles, void *stream);
 
+/* ---- host input path for pair streams ----------------------------------
+ * The reference's notebooks read frames with OpenCV imread(...,
+ * IMREAD_ANYDEPTH) and upload them with cp.asarray (gpu_process.py:582-583).
+ * piv_tiff_read decodes baseline uncompressed grayscale TIFF (8/16-bit
+ * unsigned or 32-bit float samples, strips, either byte order) into caller
+ * memory, e.g. a pinned buffer from piv_host_alloc; piv_memcpy_async moves
+ * it on a copy stream; piv_plan_wait_event makes the plan's next piv_run
+ * wait for an event recorded after that copy, so decoding, upload and
+ * compute of consecutive pairs overlap. */
+int piv_tiff_info(const char *path, int32_t *height, int32_t *width, int32_t *bits);
+int piv_tiff_read(const char *path, void *out, size_t out_bytes);
+int piv_host_alloc(void **ptr, size_t bytes);
+int piv_host_free(void *ptr);
+int piv_memcpy_async(void *dst, const void *src, size_t bytes, int kind, void *stream);
+int piv_stream_wait_event(void *stream, void *event);
+int piv_plan_wait_event(piv_plan *plan, void *event);
+
 /* ---- device memory, streams and events --------------------------------
  * The reference allocates and moves data with CuPy (cp.asarray / cp.empty /
  * .get(), gpu_process.py:582-583, 1127, 316).  These entry points give a

@@ -277,6 +277,7 @@ struct piv_plan {
   hipGraph_t lgraph = nullptr;
   hipGraphExec_t lgraph_exec = nullptr;
   hipEvent_t inputs_ready = nullptr;  // null-stream work that produced device-resident frames
+  hipEvent_t wait_next = nullptr;  // the next piv_run waits for this event (piv_plan_wait_event)
   const void* ext_a = nullptr;  // caller-owned device frames read in place (batches)
   const void* ext_b = nullptr;
   int lruns = 0;
@@ -639,6 +640,10 @@ int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on
     HIP_TRY(hipEventRecord(P->inputs_ready, nullptr));
     HIP_TRY(hipStreamWaitEvent(s, P->inputs_ready, 0));
   }
+  if (P->wait_next) {
+    HIP_TRY(hipStreamWaitEvent(s, P->wait_next, 0));
+    P->wait_next = nullptr;
+  }
   HIP_TRY(hipMemcpyAsync(P->frame_a, frame_a, P->frame_bytes, in_kind, s));
   HIP_TRY(hipMemcpyAsync(P->frame_b, frame_b, P->frame_bytes, in_kind, s));
   // The first run of each path is eager (it also sets kernel attributes);
@@ -707,6 +712,12 @@ int piv_plan_set_predictor(piv_plan* P, piv_cnn* net, int first_pass, float max_
   P->net = net;
   P->net_first = first_pass;
   P->max_reject = max_reject_fraction;
+  return PIV_OK;
+}
+
+int piv_plan_wait_event(piv_plan* P, void* event) {
+  if (!P || !event) return fail(PIV_E_INVALID, "null argument");
+  P->wait_next = (hipEvent_t)event;
   return PIV_OK;
 }
 

@@ -35,7 +35,8 @@ EXPORTED_SYMBOLS = (
     "piv_cnn_get_grads", "piv_cnn_get_buffer", "piv_cnn_forward", "piv_cnn_gradients", "piv_cnn_adam_step",
     "piv_cnn_prepare_input", "piv_plan_set_predictor", "piv_plan_last_fallback",
     "piv_batch_create", "piv_batch_destroy", "piv_batch_set_frames", "piv_batch_run", "piv_batch_tune",
-    "piv_batch_pair_shape", "piv_batch_get",
+    "piv_batch_pair_shape", "piv_batch_get", "piv_tiff_info", "piv_tiff_read", "piv_host_alloc", "piv_host_free",
+    "piv_memcpy_async", "piv_stream_wait_event", "piv_plan_wait_event",
 )
 
 SYNTH_MAX_VORTICES = 8
@@ -163,6 +164,14 @@ def _bind(lib):
         "piv_batch_tune": (i32, [vp, i32, vp]),
         "piv_batch_pair_shape": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
         "piv_batch_get": (i32, [vp, i32, vp, vp, vp, vp, i32]),
+        "piv_tiff_info": (i32, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                ctypes.POINTER(ctypes.c_int32)]),
+        "piv_tiff_read": (i32, [ctypes.c_char_p, vp, ctypes.c_size_t]),
+        "piv_host_alloc": (i32, [ctypes.POINTER(vp), ctypes.c_size_t]),
+        "piv_host_free": (i32, [vp]),
+        "piv_memcpy_async": (i32, [vp, vp, ctypes.c_size_t, i32, vp]),
+        "piv_stream_wait_event": (i32, [vp, vp]),
+        "piv_plan_wait_event": (i32, [vp, vp]),
         "piv_set_device": (i32, [i32]),
         "piv_malloc": (i32, [ctypes.POINTER(vp), ctypes.c_size_t]),
         "piv_free": (i32, [vp]),

@@ -120,13 +120,16 @@ def evaluate(net: CoarseFlowNet, data: Dataset, batch: int):
 
 
 def train(pairs=20000, holdout=256, epochs=8, batch=32, lr=1e-3, lr_min=2e-5, warmup=200, seed=0, out=None,
-          report=None):
+          report=None, init=None, seed0=TRAIN_SEED0):
     _native.require_device()
     net = CoarseFlowNet(SHAPE, max_batch=batch, seed=seed)
+    if init:
+        net.set_params(CoarseFlowNet.load(init).get_params())
+        log(f"resuming from {init}")
     log(f"network: {net.n_params} parameters, input {net.in_shape}, output {net.out_shape}")
     t0 = time.perf_counter()
-    log(f"building {pairs} training pairs (seeds {TRAIN_SEED0}+)")
-    train_set = Dataset(net, pairs, TRAIN_SEED0)
+    log(f"building {pairs} training pairs (seeds {seed0}+)")
+    train_set = Dataset(net, pairs, seed0)
     log(f"building {holdout} held-out pairs (seeds {HOLDOUT_SEED0}+)")
     hold = Dataset(net, holdout, HOLDOUT_SEED0, progress_every=0)
     t_data = time.perf_counter() - t0
@@ -183,8 +186,13 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--init", default=None, help="start from these weights (.npz)")
+    ap.add_argument("--lr-min", type=float, default=2e-5)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--seed0", type=int, default=TRAIN_SEED0, help="first training seed")
     a = ap.parse_args(argv)
-    train(a.pairs, a.holdout, a.epochs, a.batch, a.lr, seed=a.seed, out=a.out, report=a.report)
+    train(a.pairs, a.holdout, a.epochs, a.batch, a.lr, a.lr_min, a.warmup, seed=a.seed, out=a.out, report=a.report,
+          init=a.init, seed0=a.seed0)
     return 0
 
 
