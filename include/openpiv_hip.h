@@ -122,6 +122,22 @@ int piv_run(piv_plan *plan, const void *frame_a, const void *frame_b, int frames
 int piv_plan_last_smoothing(const piv_plan *plan, int pass, double *log10_s);
 int piv_plan_last_counts(const piv_plan *plan, int pass, int32_t *n_before, int32_t *n_after);
 
+/* Frame mask (height x width bytes on the host, non-zero = masked) or NULL
+ * to remove it.  Replaces the mask argument of PIVGPU (gpu_process.py:494,
+ * 548-549) with its uses: frames zeroed under the mask (mask_frames,
+ * :586, 682-687); per-pass field masks mask[int(y), int(x)] at the window
+ * centres (PIVFieldGPU.get_field_mask, :919-920, 949-951); masked windows
+ * give NaN displacement and S2N 0 (:1296-1310, 1338-1343); masked pixels of
+ * deformed frames are 0 (cuda_interpolate_masked_frame, :1461-1506);
+ * masked vectors are never flagged (gpu_validation.py:99-119, 207-210);
+ * smoothing is the weighted one with masked vectors 0 on return
+ * (SmoothnGPU with mask, :842; gpu_smoothn.py:123-297).  Not combinable
+ * with a learned predictor. */
+int piv_plan_set_mask(piv_plan *plan, const uint8_t *mask, int height, int width);
+/* Iterations of the weighted smoothing after pass k in the last run (0 when
+ * the pass was smoothed unweighted or not at all). */
+int piv_plan_last_smoothing_iterations(const piv_plan *plan, int pass, int32_t *iterations);
+
 /* ---- stage entry points (device pointers, caller-owned; stream may be
  * NULL for the null stream) -------------------------------------------- */
 
@@ -185,6 +201,17 @@ int piv_stage_spring_replace(float *u, float *v, const uint8_t *flags, const int
  * log10_s_out (device double, may be NULL) receives the chosen log10(s).
  * Replaces SmoothnGPU.__call__ (gpu_smoothn.py:123-231, 284-372). */
 int piv_stage_smoothn(float *u, float *v, int m, int n, double smoothing_par, double *log10_s_out, void *stream);
+
+/* Weighted smoothn of u and v in place: vectors that are non-finite or
+ * under mask (device m x n bytes, may be NULL) get weight 0 and are filled
+ * from their nearest finite neighbour; relaxed iterations to a relative
+ * change of 1e-3 (at most 100), GCV re-run at iterations 2^j - 1; masked
+ * vectors are 0 on return.  Falls back to piv_stage_smoothn's single pass
+ * when every weight is 1.  iterations_out (device int32, may be NULL).
+ * Replaces SmoothnGPU.__call__ with mask / missing data
+ * (gpu_smoothn.py:123-297, 353-372).  m, n <= 1024.  Synchronises. */
+int piv_stage_smoothn_masked(float *u, float *v, const uint8_t *mask, int m, int n, double smoothing_par,
+                             double *log10_s_out, int32_t *iterations_out, void *stream);
 
 /* Precision mode: correlation of window stacks by dense DFT matrix
  * products on the MFMA units (FP16/BF16 inputs, FP32 accumulation), beside

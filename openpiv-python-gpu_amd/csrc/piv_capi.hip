@@ -119,25 +119,106 @@ std::vector<double> lambda_table(int m, int n) {
 struct SmoothTables {
   float* cos_m = nullptr;
   float* cos_n = nullptr;
+  float* cos_mt = nullptr;  // transposes (weighted path)
+  float* cos_nt = nullptr;
   double* lambda = nullptr;
 };
 
+std::vector<float> transposed(const std::vector<float>& c, int n) {
+  std::vector<float> t(c.size());
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) t[(size_t)b * n + a] = c[(size_t)a * n + b];
+  return t;
+}
+
 int make_tables(int m, int n, SmoothTables* t) {
   auto cm = dct_matrix(m), cn = dct_matrix(n);
+  auto cmt = transposed(cm, m), cnt = transposed(cn, n);
   auto lam = lambda_table(m, n);
   HIP_TRY(hipMalloc(&t->cos_m, cm.size() * sizeof(float)));
   HIP_TRY(hipMalloc(&t->cos_n, cn.size() * sizeof(float)));
+  HIP_TRY(hipMalloc(&t->cos_mt, cm.size() * sizeof(float)));
+  HIP_TRY(hipMalloc(&t->cos_nt, cn.size() * sizeof(float)));
   HIP_TRY(hipMalloc(&t->lambda, lam.size() * sizeof(double)));
   HIP_TRY(hipMemcpy(t->cos_m, cm.data(), cm.size() * sizeof(float), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(t->cos_n, cn.data(), cn.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(t->cos_mt, cmt.data(), cmt.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(t->cos_nt, cnt.data(), cnt.size() * sizeof(float), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(t->lambda, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice));
   return PIV_OK;
 }
 void free_tables(SmoothTables* t) {
-  if (t->cos_m) (void)hipFree(t->cos_m);
-  if (t->cos_n) (void)hipFree(t->cos_n);
-  if (t->lambda) (void)hipFree(t->lambda);
+  for (void* p : {(void*)t->cos_m, (void*)t->cos_n, (void*)t->cos_mt, (void*)t->cos_nt, (void*)t->lambda})
+    if (p) (void)hipFree(p);
   *t = SmoothTables{};
+}
+
+// Weighted-smoothing work buffers sized for fields of up to maxN vectors
+// and maxM rows (slow: also the batched GCV residual buffer).
+struct MaskedSmoothBufs {
+  int32_t* nr = nullptr;
+  float* f = nullptr;
+  float* w = nullptr;
+  float* z = nullptr;
+  double* part = nullptr;
+  void* state = nullptr;
+  float* qbatch = nullptr;
+};
+
+int alloc_masked_smooth(int maxN, int maxM, bool slow, MaskedSmoothBufs* b) {
+  HIP_TRY(hipMalloc(&b->nr, (size_t)maxN * 4));
+  HIP_TRY(hipMalloc(&b->f, 2ull * maxN * 4));
+  HIP_TRY(hipMalloc(&b->w, (size_t)maxN * 4));
+  HIP_TRY(hipMalloc(&b->z, 4ull * maxN * 4));
+  HIP_TRY(hipMalloc(&b->part, smoothn_masked_part_doubles(maxM) * 8));
+  HIP_TRY(hipMalloc(&b->state, smoothn_masked_state_bytes()));
+  HIP_TRY(hipMemset(b->state, 0, smoothn_masked_state_bytes()));
+  if (slow) HIP_TRY(hipMalloc(&b->qbatch, 64ull * 2 * maxN * 4));
+  return PIV_OK;
+}
+void free_masked_smooth(MaskedSmoothBufs* b) {
+  for (void* p : {(void*)b->nr, (void*)b->f, (void*)b->w, (void*)b->z, (void*)b->part, b->state, (void*)b->qbatch})
+    if (p) (void)hipFree(p);
+  *b = MaskedSmoothBufs{};
+}
+
+SmoothnMaskedWork masked_work(const SmoothTables& tab, const uint8_t* fmask, const MaskedSmoothBufs& b, float* t0,
+                              float* t1, double* p_out, int32_t* iters_out, int slow) {
+  SmoothnMaskedWork w{};
+  w.cos_m = tab.cos_m;
+  w.cos_mt = tab.cos_mt;
+  w.cos_n = tab.cos_n;
+  w.cos_nt = tab.cos_nt;
+  w.lambda = tab.lambda;
+  w.mask = fmask;
+  w.nr = b.nr;
+  w.f = b.f;
+  w.w = b.w;
+  w.z = b.z;
+  w.t0 = t0;
+  w.t1 = t1;
+  w.part = b.part;
+  w.state = b.state;
+  w.qbatch = b.qbatch;
+  w.p_out = p_out;
+  w.iters_out = iters_out;
+  w.slow_gcv = slow;
+  return w;
+}
+
+// Field mask of a pass: the frame mask at each window centre truncated to
+// integer pixels, mask[int(y), int(x)] (gpu_process.py:949-951).
+std::vector<uint8_t> field_mask_of(const piv_pass_desc& p, const uint8_t* mask, int wd) {
+  std::vector<uint8_t> fm((size_t)p.field_m * p.field_n);
+  const double x0 = p.search_size / 2.0 + p.x_offset, y0 = p.search_size / 2.0 + p.y_offset;
+  for (int i = 0; i < p.field_m; ++i) {
+    const int y = (int)(float)(i * (double)p.spacing + y0);
+    for (int j = 0; j < p.field_n; ++j) {
+      const int x = (int)(float)(j * (double)p.spacing + x0);
+      fm[(size_t)i * p.field_n + j] = mask[(size_t)y * wd + x] ? 1 : 0;
+    }
+  }
+  return fm;
 }
 
 CorrArgs corr_args(const void* fa, const void* fb, int dtype, int ht, int wd, const piv_pass_desc& p) {
@@ -231,6 +312,9 @@ struct PassBuffers {
   float* s2n = nullptr;         // when the pass has an S2N test
   SmoothTables tab;
   double p_min = 0, p_max = 0;
+  uint8_t* fmask = nullptr;     // field mask (piv_plan_set_mask) or null
+  int n_masked = 0;             // masked vectors: > 0 selects weighted smoothing
+  int slow_gcv = 0;             // mean weight <= 0.9
 };
 
 // Work buffers of the validation / replacement loop (field-sized).
@@ -283,6 +367,10 @@ struct piv_plan {
   int lruns = 0;
   int32_t last_fallback = 0;
   int32_t last_rejected = 0;
+  // frame mask (piv_plan_set_mask)
+  uint8_t* frame_mask = nullptr;
+  MaskedSmoothBufs msb{};
+  int32_t* smooth_iters = nullptr;  // 1 per pass
 };
 
 namespace {
@@ -295,8 +383,9 @@ const void* src_b(const piv_plan* P) { return P->ext_b ? P->ext_b : P->frame_b; 
 // and correlate the residual, adding the predictor back (:628-632).
 int deform_and_correlate(piv_plan* P, PassBuffers& B, hipStream_t s) {
   const piv_plan_desc& D = P->desc;
-  HIP_TRY(launch_deform(
-      deform_args(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d, B.up, B.vp, P->def_a, P->def_b), s));
+  DeformArgs da = deform_args(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d, B.up, B.vp, P->def_a, P->def_b);
+  da.mask = P->frame_mask;
+  HIP_TRY(launch_deform(da, s));
   CorrArgs a = corr_args(P->def_a, P->def_b, kF32, D.height, D.width, B.d);
   a.up = B.up;
   a.vp = B.vp;
@@ -304,12 +393,13 @@ int deform_and_correlate(piv_plan* P, PassBuffers& B, hipStream_t s) {
   a.v = B.v;
   a.s2n = B.s2n;
   HIP_TRY(launch_correlate(a, s));
+  if (B.fmask) HIP_TRY(launch_apply_field_mask(B.u, B.v, B.s2n, B.fmask, B.N, s));
   return PIV_OK;
 }
 
 ValidateArgs validate_args(const piv_pass_desc& d, const float* u, const float* v, int m, int n, const float* s2n,
                            const double* stats, const uint8_t* only, const uint8_t* extra, uint8_t* flags,
-                           int32_t* count) {
+                           int32_t* count, const uint8_t* fmask = nullptr) {
   ValidateArgs a{};
   a.u = u;
   a.v = v;
@@ -326,15 +416,16 @@ ValidateArgs validate_args(const piv_pass_desc& d, const float* u, const float* 
   a.extra = extra;
   a.flags = flags;
   a.count = count;
+  a.fmask = fmask;
   return a;
 }
 
 // ValidationGPU.__call__ (gpu_validation.py:84-118): every enabled test.
 int run_validate(const piv_pass_desc& d, const float* u, const float* v, int m, int n, const float* s2n,
                  double* stats, const uint8_t* only, const uint8_t* extra, uint8_t* flags, int32_t* count,
-                 hipStream_t s) {
+                 hipStream_t s, const uint8_t* fmask = nullptr) {
   if (!std::isnan(d.rms_tol)) HIP_TRY(launch_field_stats(u, v, m * n, stats, s));
-  HIP_TRY(launch_validate(validate_args(d, u, v, m, n, s2n, stats, only, extra, flags, count), s));
+  HIP_TRY(launch_validate(validate_args(d, u, v, m, n, s2n, stats, only, extra, flags, count, fmask), s));
   return PIV_OK;
 }
 
@@ -346,7 +437,7 @@ int run_validate(const piv_pass_desc& d, const float* u, const float* v, int m, 
 // unresolved ones are reset at the end (:829-832).  it_counts must be zero.
 // No host synchronisation: an empty flag set turns every step into a no-op.
 int replace_loop(float* u, float* v, const uint8_t* flags0, int32_t* count0, int m, int n, const piv_pass_desc& d,
-                 const float* s2n, LoopBufs& L, hipStream_t s) {
+                 const float* s2n, LoopBufs& L, hipStream_t s, const uint8_t* fmask = nullptr) {
   const int N = m * n;
   const int iters = d.num_replacing_iters;
   if (iters <= 0) return PIV_OK;
@@ -371,7 +462,7 @@ int replace_loop(float* u, float* v, const uint8_t* flags0, int32_t* count0, int
       HIP_TRY(launch_reset_unresolved(u, v, L.u0, L.v0, cur, L.nanf, N, s));
       uint8_t* nxt = L.wf[it & 1];
       int32_t* nxt_count = L.it_counts + it;
-      int r = run_validate(d, u, v, m, n, s2n, L.stats, cur, L.nanf, nxt, nxt_count, s);
+      int r = run_validate(d, u, v, m, n, s2n, L.stats, cur, L.nanf, nxt, nxt_count, s, fmask);
       if (r) return r;
       cur = nxt;
       cur_count = nxt_count;
@@ -387,14 +478,21 @@ int validate_and_smooth(piv_plan* P, int k, bool last, hipStream_t s) {
   PassBuffers& B = P->passes[k];
   const int m = B.d.field_m, n = B.d.field_n;
   if (B.d.validate) {
-    int r = run_validate(B.d, B.u, B.v, m, n, B.s2n, P->lb.stats, nullptr, nullptr, B.flags, P->counters + 2 * k, s);
-    if (!r) r = replace_loop(B.u, B.v, B.flags, P->counters + 2 * k, m, n, B.d, B.s2n, P->lb, s);
+    int r = run_validate(B.d, B.u, B.v, m, n, B.s2n, P->lb.stats, nullptr, nullptr, B.flags, P->counters + 2 * k, s,
+                         B.fmask);
+    if (!r) r = replace_loop(B.u, B.v, B.flags, P->counters + 2 * k, m, n, B.d, B.s2n, P->lb, s, B.fmask);
     if (!r && last)
       r = run_validate(B.d, B.u, B.v, m, n, B.s2n, P->lb.stats, nullptr, nullptr, P->val_final,
-                       P->counters + 2 * k + 1, s);
+                       P->counters + 2 * k + 1, s, B.fmask);
     if (r) return r;
   }
-  if (!last && B.d.smooth) {
+  if (!last && B.d.smooth && B.n_masked > 0) {
+    const int autos = !(B.d.smoothing_par > 0);
+    SmoothnMaskedWork w = masked_work(B.tab, B.fmask, P->msb, P->t0, P->t1, P->p_out + k, P->smooth_iters + k,
+                                      B.slow_gcv);
+    HIP_TRY(launch_smoothn_masked(B.u, B.v, m, n, B.p_min, B.p_max, autos ? 0.0 : log10(B.d.smoothing_par), autos, w,
+                                  s));
+  } else if (!last && B.d.smooth) {
     SmoothnWork w{};
     w.cos_m = B.tab.cos_m;
     w.cos_n = B.tab.cos_n;
@@ -439,6 +537,7 @@ int enqueue_passes(piv_plan* P, int first) {
       a.v = B.v;
       a.s2n = B.s2n;
       HIP_TRY(launch_correlate(a, s));
+      if (B.fmask) HIP_TRY(launch_apply_field_mask(B.u, B.v, B.s2n, B.fmask, B.N, s));
     } else {
       if (k > first) r = predictor_from_previous(P, k, s);
       if (!r) r = deform_and_correlate(P, B, s);
@@ -454,6 +553,9 @@ int enqueue_passes(piv_plan* P, int first) {
 }
 
 int enqueue(piv_plan* P) {
+  if (P->frame_mask)
+    HIP_TRY(launch_mask_frames(P->frame_a, P->frame_b, P->desc.in_dtype, P->frame_mask,
+                               (size_t)P->desc.height * P->desc.width, P->stream));
   HIP_TRY(launch_fill_i32(P->counters, 0, 2 * (int)P->passes.size(), P->stream));
   HIP_TRY(launch_fill_i32(P->lb.it_counts, 0, PIV_MAX_REPLACING_ITERS, P->stream));
   return enqueue_passes(P, 0);
@@ -521,10 +623,13 @@ void release(piv_plan* P) {
   if (P->net_in) (void)hipFree(P->net_in);
   if (P->net_scratch) (void)hipFree(P->net_scratch);
   for (auto& B : P->passes) {
-    for (void* p : {(void*)B.u, (void*)B.v, (void*)B.up, (void*)B.vp, (void*)B.flags, (void*)B.s2n})
+    for (void* p : {(void*)B.u, (void*)B.v, (void*)B.up, (void*)B.vp, (void*)B.flags, (void*)B.s2n, (void*)B.fmask})
       if (p) (void)hipFree(p);
     free_tables(&B.tab);
   }
+  free_masked_smooth(&P->msb);
+  if (P->frame_mask) (void)hipFree(P->frame_mask);
+  if (P->smooth_iters) (void)hipFree(P->smooth_iters);
   for (void* p : {(void*)P->lb.u0, (void*)P->lb.v0, (void*)P->lb.tu, (void*)P->lb.tv, (void*)P->lb.wf[0],
                   (void*)P->lb.wf[1], (void*)P->lb.nanf, (void*)P->lb.it_counts, (void*)P->lb.stats})
     if (p) (void)hipFree(p);
@@ -689,6 +794,7 @@ int piv_plan_set_predictor(piv_plan* P, piv_cnn* net, int first_pass, float max_
   }
   if (!(max_reject_fraction >= 0.f && max_reject_fraction <= 1.f))
     return fail(PIV_E_INVALID, "max_reject_fraction must lie in [0, 1]");
+  if (P->frame_mask) return fail(PIV_E_INVALID, "a learned predictor cannot be combined with a frame mask");
   const piv_plan_desc& D = P->desc;
   const CnnInfo ni = cnn_info(net);
   if (D.height != ni.in_h * kCnnFactor || D.width != ni.in_w * kCnnFactor)
@@ -712,6 +818,75 @@ int piv_plan_set_predictor(piv_plan* P, piv_cnn* net, int first_pass, float max_
   P->net = net;
   P->net_first = first_pass;
   P->max_reject = max_reject_fraction;
+  return PIV_OK;
+}
+
+int piv_plan_set_mask(piv_plan* P, const uint8_t* mask, int height, int width) {
+  if (!P) return fail(PIV_E_INVALID, "null plan");
+  const piv_plan_desc& D = P->desc;
+  HIP_TRY(hipSetDevice(P->device));
+  HIP_TRY(hipStreamSynchronize(P->stream));
+  if (mask && P->net) return fail(PIV_E_INVALID, "a frame mask cannot be combined with a learned predictor");
+  if (mask && (height != D.height || width != D.width))
+    return fail(PIV_E_INVALID, "mask is %d x %d, the frames are %d x %d", height, width, D.height, D.width);
+  // the captured sequence changes
+  if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  P->graph_exec = nullptr;
+  P->graph = nullptr;
+  P->runs = 0;
+  for (auto& B : P->passes) {
+    if (B.fmask) (void)hipFree(B.fmask);
+    B.fmask = nullptr;
+    B.n_masked = 0;
+    B.slow_gcv = 0;
+  }
+  free_masked_smooth(&P->msb);
+  if (P->frame_mask) (void)hipFree(P->frame_mask);
+  P->frame_mask = nullptr;
+  if (!mask) return PIV_OK;
+  const size_t npx = (size_t)D.height * D.width;
+  const int np = (int)P->passes.size();
+  int maxN = 0, maxM = 0;
+  bool slow = false;
+  for (int k = 0; k < np; ++k) {
+    PassBuffers& B = P->passes[k];
+    std::vector<uint8_t> fm = field_mask_of(B.d, mask, D.width);
+    for (uint8_t x : fm) B.n_masked += x;
+    HIP_TRY(hipMalloc(&B.fmask, fm.size()));
+    HIP_TRY(hipMemcpy(B.fmask, fm.data(), fm.size(), hipMemcpyHostToDevice));
+    if (k < np - 1 && B.d.smooth && B.n_masked > 0) {
+      if (B.d.field_m > smoothn_masked_max_dim() || B.d.field_n > smoothn_masked_max_dim())
+        return fail(PIV_E_INVALID, "masked smoothing supports fields of up to %d x %d",
+                    smoothn_masked_max_dim(), smoothn_masked_max_dim());
+      // mean weight (gpu_smoothn.py:171, 367): the masked vectors are NaN
+      B.slow_gcv = !((double)(B.N - B.n_masked) / B.N > 0.9);
+      slow = slow || (B.slow_gcv && !(B.d.smoothing_par > 0));
+      maxN = std::max(maxN, B.N);
+      maxM = std::max(maxM, B.d.field_m);
+    }
+  }
+  HIP_TRY(hipMalloc(&P->frame_mask, npx));
+  std::vector<uint8_t> fmk(npx);
+  for (size_t q = 0; q < npx; ++q) fmk[q] = mask[q] ? 1 : 0;
+  HIP_TRY(hipMemcpy(P->frame_mask, fmk.data(), npx, hipMemcpyHostToDevice));
+  if (maxN > 0) {
+    int r = alloc_masked_smooth(maxN, maxM, slow, &P->msb);
+    if (r) return r;
+  }
+  if (!P->smooth_iters) {
+    HIP_TRY(hipMalloc(&P->smooth_iters, np * sizeof(int32_t)));
+    HIP_TRY(hipMemset(P->smooth_iters, 0, np * sizeof(int32_t)));
+  }
+  return PIV_OK;
+}
+
+int piv_plan_last_smoothing_iterations(const piv_plan* P, int pass, int32_t* iterations) {
+  if (!P || !iterations || pass < 0 || pass >= (int)P->passes.size()) return fail(PIV_E_INVALID, "bad argument");
+  *iterations = 0;
+  if (!P->smooth_iters) return PIV_OK;
+  HIP_TRY(hipSetDevice(P->device));
+  HIP_TRY(hipMemcpy(iterations, P->smooth_iters + pass, sizeof(int32_t), hipMemcpyDeviceToHost));
   return PIV_OK;
 }
 
@@ -904,6 +1079,59 @@ int piv_stage_smoothn(float* u, float* v, int m, int n, double smoothing_par, do
   (void)hipFree(t1);
   (void)hipFree(state);
   free_tables(&tab);
+  if (e != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e));
+  if (e2 != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e2));
+  return PIV_OK;
+}
+
+int piv_stage_smoothn_masked(float* u, float* v, const uint8_t* mask, int m, int n, double smoothing_par,
+                             double* log10_s_out, int32_t* iterations_out, void* stream) {
+  if (!u || !v || m < 1 || n < 1) return fail(PIV_E_INVALID, "bad argument");
+  if (m > smoothn_masked_max_dim() || n > smoothn_masked_max_dim())
+    return fail(PIV_E_INVALID, "masked smoothing supports fields of up to %d x %d", smoothn_masked_max_dim(),
+                smoothn_masked_max_dim());
+  hipStream_t s = (hipStream_t)stream;
+  const int N = m * n;
+  // the branch of the GCV score follows the mean weight, from the mask and
+  // the non-finite vectors (gpu_smoothn.py:171, 259-269, 367)
+  std::vector<float> hu(N), hv(N);
+  std::vector<uint8_t> hm(N, 0);
+  HIP_TRY(hipMemcpyAsync(hu.data(), u, (size_t)N * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(hv.data(), v, (size_t)N * 4, hipMemcpyDeviceToHost, s));
+  if (mask) HIP_TRY(hipMemcpyAsync(hm.data(), mask, N, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  int n_w = 0;
+  for (int k = 0; k < N; ++k) n_w += (std::isfinite(hu[k]) && std::isfinite(hv[k]) && !hm[k]) ? 1 : 0;
+  const int slow = !((double)n_w / N > 0.9);
+  SmoothTables tab;
+  int r = make_tables(m, n, &tab);
+  MaskedSmoothBufs b;
+  float *t0 = nullptr, *t1 = nullptr;
+  if (!r) r = alloc_masked_smooth(N, m, slow != 0, &b);
+  if (!r && (hipMalloc(&t0, 2ull * N * 4) != hipSuccess || hipMalloc(&t1, 2ull * N * 4) != hipSuccess))
+    r = fail(PIV_E_NOMEM, "device allocation failed");
+  hipError_t e = hipSuccess, e2 = hipSuccess;
+  if (!r) {
+    double pmin, pmax;
+    p_bounds(m, n, &pmin, &pmax);
+    const int autos = !(smoothing_par > 0);
+    const SmoothnMaskedWork w = masked_work(tab, mask, b, t0, t1, log10_s_out, iterations_out, slow);
+    if (n_w == N) {
+      // nothing masked or missing: the unweighted single pass
+      void* state = b.state;
+      SmoothnWork uw{tab.cos_m, tab.cos_n, tab.lambda, t0, t1, log10_s_out, state};
+      e = launch_smoothn(u, v, m, n, pmin, pmax, autos ? 0.0 : log10(smoothing_par), autos, uw, s);
+      if (e == hipSuccess && iterations_out) e = hipMemsetAsync(iterations_out, 0, 4, s);
+    } else {
+      e = launch_smoothn_masked(u, v, m, n, pmin, pmax, autos ? 0.0 : log10(smoothing_par), autos, w, s);
+    }
+    e2 = hipStreamSynchronize(s);
+  }
+  if (t0) (void)hipFree(t0);
+  if (t1) (void)hipFree(t1);
+  free_masked_smooth(&b);
+  free_tables(&tab);
+  if (r) return r;
   if (e != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e));
   if (e2 != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e2));
   return PIV_OK;

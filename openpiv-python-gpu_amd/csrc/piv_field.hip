@@ -58,9 +58,16 @@ __device__ __forceinline__ float warp_px(const void* f, int ht, int wd, float X,
 
 template <typename T>
 __device__ __forceinline__ void deform_px(const DeformArgs& a, int X, int Y) {
+  const size_t idx = (size_t)Y * a.wd + X;
+  if (a.mask && a.mask[idx]) {
+    // cuda_interpolate_masked_frame (gpu_process.py:1461-1506) leaves
+    // masked pixels at 0; an unwarped frame is already 0 there
+    a.out_a[idx] = 0.f;
+    a.out_b[idx] = 0.f;
+    return;
+  }
   float up, vp;
   grid_bilinear(a.up, a.vp, a.m, a.n, a.y0, a.x0, a.spacing, (float)Y, (float)X, up, vp);
-  const size_t idx = (size_t)Y * a.wd + X;
   a.out_a[idx] = a.warp_a ? warp_px<T>(a.frame_a, a.ht, a.wd, (float)X, (float)Y, up, vp, a.dt_a)
                           : ld<T>(a.frame_a, idx);
   a.out_b[idx] = a.warp_b ? warp_px<T>(a.frame_b, a.ht, a.wd, (float)X, (float)Y, up, vp, a.dt_b)
@@ -240,6 +247,67 @@ __device__ __forceinline__ void sum256(double (&v)[K], double (*scratch)[4]) {
   __syncthreads();
 }
 
+// Right-hand side of outlier row k scaled by n_k (the in-bounds
+// 4-neighbour count): (n_k / c_k) times the sum of the neighbours that are
+// not outliers and not NaN, c_k counting the in-bounds neighbours that are
+// outliers or not NaN -- the reference's NaN-mean over the neighbours with
+// the outliers set to 0 (gpu_validation.py:342, 381-394).  c_k = 0 gives a
+// NaN right-hand side there; such a row has no outlier neighbour, so it is
+// decoupled, and the vector keeps its value (the reset of unresolved
+// vectors, gpu_process.py:829-832): its right-hand side is 0 here and
+// nan_isolated() tells the final write to skip it.
+__device__ __forceinline__ void spring_rhs(const float* u, const float* v, const int32_t* slot, int m, int n,
+                                           const int (&ni)[4], const int (&nj)[4], double* bu, double* bv,
+                                           int* nk) {
+  double su = 0.0, sv = 0.0;
+  int cnt = 0, cu = 0, cv = 0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (ni[e] < 0 || ni[e] >= m || nj[e] < 0 || nj[e] >= n) continue;
+    ++cnt;
+    const int l = ni[e] * n + nj[e];
+    if (slot[l] >= 0) {
+      ++cu;
+      ++cv;
+    } else {
+      const float a = u[l], b = v[l];
+      if (!isnan(a)) {
+        su += (double)a;
+        ++cu;
+      }
+      if (!isnan(b)) {
+        sv += (double)b;
+        ++cv;
+      }
+    }
+  }
+  *bu = (cu > 0 && cu != cnt) ? su * cnt / cu : su;
+  *bv = (cv > 0 && cv != cnt) ? sv * cnt / cv : sv;
+  *nk = cnt;
+}
+
+__device__ __forceinline__ void nan_isolated(const float* u, const float* v, const int32_t* slot, int m, int n,
+                                             int k, bool* ku, bool* kv) {
+  const int i = k / n, j = k % n;
+  const int ni[4] = {i, i, i + 1, i - 1};
+  const int nj[4] = {j - 1, j + 1, j, j};
+  int cu = 0, cv = 0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (ni[e] < 0 || ni[e] >= m || nj[e] < 0 || nj[e] >= n) continue;
+    const int l = ni[e] * n + nj[e];
+    if (slot[l] >= 0) {
+      ++cu;
+      ++cv;
+    } else {
+      cu += isnan(u[l]) ? 0 : 1;
+      cv += isnan(v[l]) ? 0 : 1;
+    }
+  }
+  *ku = cu == 0;
+  *kv = cv == 0;
+}
+
 __device__ void spring_cg_global(float* u, float* v, int m, int n, int nv, const ReplaceWork& w,
                                  double (*red)[4]) {
   constexpr int T = 256;
@@ -266,16 +334,7 @@ __device__ void spring_cg_global(float* u, float* v, int m, int n, int nv, const
     nbrs(s, ni, nj);
     double bu = 0.0, bv = 0.0;
     int nk = 0;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      if (ni[e] < 0 || ni[e] >= m || nj[e] < 0 || nj[e] >= n) continue;
-      ++nk;
-      const int l = ni[e] * n + nj[e];
-      if (w.slot[l] < 0) {
-        bu += (double)u[l];
-        bv += (double)v[l];
-      }
-    }
+    spring_rhs(u, v, w.slot, m, n, ni, nj, &bu, &bv, &nk);
     xu[s] = 0.0;
     xv[s] = 0.0;
     ru[s] = bu;
@@ -354,8 +413,10 @@ __device__ void spring_cg_global(float* u, float* v, int m, int n, int nv, const
   __syncthreads();
   for (int s = tid; s < nv; s += T) {
     const int k = w.list[s];
-    u[k] = (float)xu[s];
-    v[k] = (float)xv[s];
+    bool ku, kv;
+    nan_isolated(u, v, w.slot, m, n, k, &ku, &kv);
+    if (!ku) u[k] = (float)xu[s];
+    if (!kv) v[k] = (float)xv[s];
   }
 }
 
@@ -560,19 +621,12 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(float* u,
     for (int e = 0; e < 4; ++e) {
       int16_t code = -2;
       if (ni[e] >= 0 && ni[e] < m && nj[e] >= 0 && nj[e] < n) {
-        ++cnt;
-        const int l = ni[e] * n + nj[e];
-        const int sl = w.slot[l];
-        if (sl >= 0) {
-          code = (int16_t)sl;
-        } else {
-          code = -1;
-          bu += (double)u[l];
-          bv += (double)v[l];
-        }
+        const int sl = w.slot[ni[e] * n + nj[e]];
+        code = sl >= 0 ? (int16_t)sl : (int16_t)-1;
       }
       L.nb[s][e] = code;
     }
+    spring_rhs(u, v, w.slot, m, n, ni, nj, &bu, &bv, &cnt);
     L.nk[s] = (uint8_t)cnt;
     L.xu[s] = bu / cnt;
     L.xv[s] = bv / cnt;
@@ -581,18 +635,17 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(float* u,
   if (nv <= 256) {
     if (tid >= 64) return;  // wave 0 finishes alone
     spring_cg<64>(L, nv);
-    for (int s = tid; s < nv; s += 64) {
-      const int k = w.list[s];
-      u[k] = (float)L.xu[s];
-      v[k] = (float)L.xv[s];
-    }
   } else {
     spring_cg<T>(L, nv);
-    for (int s = tid; s < nv; s += T) {
-      const int k = w.list[s];
-      u[k] = (float)L.xu[s];
-      v[k] = (float)L.xv[s];
-    }
+  }
+  // all neighbour reads of nan_isolated() see pre-solve values: outlier
+  // neighbours are recognised by their slot, never read
+  for (int s = tid; s < nv; s += (nv <= 256 ? 64 : T)) {
+    const int k = w.list[s];
+    bool ku, kv;
+    nan_isolated(u, v, w.slot, m, n, k, &ku, &kv);
+    if (!ku) u[k] = (float)L.xu[s];
+    if (!kv) v[k] = (float)L.xv[s];
   }
 }
 
@@ -614,6 +667,44 @@ __global__ void copy_f32_kernel(float* dst, const float* src, int count) {
 __global__ void copy_u8_kernel(uint8_t* dst, const uint8_t* src, int count) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < count) dst[k] = src[k];
+}
+
+// PIVGPU.mask_frames (gpu_process.py:682-687): masked pixels of both frames
+// set to 0, in place, 16 pixels per thread.
+template <typename T>
+__global__ __launch_bounds__(256) void mask_frames_kernel(T* a, T* b, const uint8_t* mask, size_t npx) {
+  const size_t p0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;
+  if (p0 >= npx) return;
+  if (p0 + 16 <= npx) {
+    const uint4 q = *reinterpret_cast<const uint4*>(mask + p0);
+    if ((q.x | q.y | q.z | q.w) == 0) return;
+    const uint32_t words[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      if ((words[e >> 2] >> (8 * (e & 3))) & 0xff) {
+        a[p0 + e] = T(0);
+        b[p0 + e] = T(0);
+      }
+    }
+  } else {
+    for (size_t p = p0; p < npx; ++p)
+      if (mask[p]) {
+        a[p] = T(0);
+        b[p] = T(0);
+      }
+  }
+}
+
+// Masked windows (CorrelationGPU with field_mask, gpu_process.py:1296-1310,
+// 1338-1343): displacement NaN, signal-to-noise 0.
+__global__ __launch_bounds__(256) void apply_field_mask_kernel(float* u, float* v, float* s2n, const uint8_t* fmask,
+                                                               int count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count && fmask[k]) {
+    u[k] = NAN;
+    v[k] = NAN;
+    if (s2n) s2n[k] = 0.f;
+  }
 }
 
 thread_local char g_detail[256] = "";
@@ -669,7 +760,7 @@ __global__ __launch_bounds__(256) void validate_kernel(ValidateArgs a) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   bool flag = false;
   if (k < a.m * a.n) {
-    if (!a.only || a.only[k]) {
+    if ((!a.only || a.only[k]) && !(a.fmask && a.fmask[k])) {
       const int i = k / a.n, j = k % a.n;
       const float uk = a.u[k], vk = a.v[k];
       if (a.s2n && !isnan(a.s2n_tol)) flag = flag || (a.s2n[k] < a.s2n_tol);
@@ -868,6 +959,32 @@ hipError_t launch_copy_f32(float* dst, const float* src, int count, hipStream_t 
 }
 hipError_t launch_copy_u8(uint8_t* dst, const uint8_t* src, int count, hipStream_t stream) {
   hipLaunchKernelGGL(copy_u8_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, dst, src, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_mask_frames(void* a, void* b, int in_dtype, const uint8_t* mask, size_t npx, hipStream_t stream) {
+  if (((uintptr_t)mask & 15) != 0) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)((npx + 16 * 256 - 1) / (16 * 256)));
+  switch (in_dtype) {
+    case kU8:
+      hipLaunchKernelGGL(mask_frames_kernel<uint8_t>, grid, dim3(256), 0, stream, (uint8_t*)a, (uint8_t*)b, mask, npx);
+      break;
+    case kU16:
+      hipLaunchKernelGGL(mask_frames_kernel<uint16_t>, grid, dim3(256), 0, stream, (uint16_t*)a, (uint16_t*)b, mask,
+                         npx);
+      break;
+    case kF32:
+      hipLaunchKernelGGL(mask_frames_kernel<float>, grid, dim3(256), 0, stream, (float*)a, (float*)b, mask, npx);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_field_mask(float* u, float* v, float* s2n, const uint8_t* fmask, int count,
+                                   hipStream_t stream) {
+  hipLaunchKernelGGL(apply_field_mask_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, u, v, s2n, fmask,
+                     count);
   return hipGetLastError();
 }
 

@@ -69,6 +69,7 @@ struct DeformArgs {
   int warp_a, warp_b;  // 0 -> copy the frame unchanged
   float* out_a;
   float* out_b;
+  const uint8_t* mask;  // frame mask (1 = masked: both outputs 0) or null
 };
 hipError_t launch_deform(const DeformArgs& a, hipStream_t stream);
 // One launch over the rows of many frames (device tables); row r belongs to
@@ -110,6 +111,7 @@ struct ValidateArgs {
   const uint8_t* extra;
   uint8_t* flags;
   int32_t* count;
+  const uint8_t* fmask;  // field mask: never flagged (gpu_validation.py:207-210), or null
 };
 hipError_t launch_validate(const ValidateArgs& a, hipStream_t stream);
 // Field mean and population standard deviation of u and v (two-pass, FP64).
@@ -150,6 +152,36 @@ size_t smoothn_state_bytes();
 hipError_t launch_smoothn(float* u, float* v, int m, int n, double p_min, double p_max, double fixed_p,
                           int auto_s, const SmoothnWork& w, hipStream_t stream);
 
+// Weighted DCT-PLS smoothing of u and v in place for fields with masked or
+// missing vectors (nearest-finite fill, weights, coarse start, relaxed
+// iterations with a device-side convergence flag; masked vectors 0 on
+// return).  m, n <= smoothn_masked_max_dim().
+struct SmoothnMaskedWork {
+  const float* cos_m;   // m x m DCT-II matrix (row k = frequency)
+  const float* cos_mt;  // its transpose
+  const float* cos_n;
+  const float* cos_nt;
+  const double* lambda; // m x n
+  const uint8_t* mask;  // m x n field mask (1 = masked) or null
+  int32_t* nr;          // m*n
+  float* f;             // 2 m*n
+  float* w;             // m*n
+  float* z;             // 4 m*n (two slots)
+  float* t0;            // 2 m*n
+  float* t1;            // 2 m*n
+  double* part;         // smoothn_masked_part_doubles(m)
+  void* state;          // smoothn_masked_state_bytes()
+  float* qbatch;        // 64 x 2 m*n when slow_gcv, else unused
+  double* p_out;        // chosen log10(s) (may be null)
+  int32_t* iters_out;   // iterations run (may be null)
+  int slow_gcv;         // mean weight <= 0.9: GCV residual from the data misfit
+};
+size_t smoothn_masked_state_bytes();
+size_t smoothn_masked_part_doubles(int m);
+int smoothn_masked_max_dim();
+hipError_t launch_smoothn_masked(float* u, float* v, int m, int n, double p_min, double p_max, double fixed_p,
+                                 int auto_s, const SmoothnMaskedWork& w, hipStream_t stream);
+
 // Window correlation by DFT matrix products on MFMA (precision mode).
 struct DftArgs {
   const float* win_a;   // n x ws x ws windows (FP32)
@@ -180,5 +212,10 @@ hipError_t launch_fill_i32(int32_t* p, int32_t value, int count, hipStream_t str
 hipError_t launch_scale(float* u, float* v, int count, float factor, hipStream_t stream);
 hipError_t launch_copy_f32(float* dst, const float* src, int count, hipStream_t stream);
 hipError_t launch_copy_u8(uint8_t* dst, const uint8_t* src, int count, hipStream_t stream);
+// Masked pixels of both frames set to 0 in place (mask: 16-byte aligned).
+hipError_t launch_mask_frames(void* a, void* b, int in_dtype, const uint8_t* mask, size_t npx, hipStream_t stream);
+// u = v = NaN and s2n = 0 (when non-null) at masked windows.
+hipError_t launch_apply_field_mask(float* u, float* v, float* s2n, const uint8_t* fmask, int count,
+                                   hipStream_t stream);
 
 }  // namespace piv

@@ -12,6 +12,7 @@
 // span over [p_min, p_max] (:182-188), then, instead of host L-BFGS-B
 // (:191-193), two 64-point bracket refinements of dg/dp = 0 and two Newton
 // steps using the analytic first and second derivatives of RSS and T.
+#include <limits.h>
 #include <math.h>
 
 #include "piv_kernels.h"
@@ -141,7 +142,9 @@ __device__ double g_of(const double* sm, int N) {
 // communication inside the kernel; the decide kernel runs after it.
 template <int NS>
 __global__ __launch_bounds__(256) void gcv_eval_kernel(const float* Y, const double* lam, int N, const GcvState* st,
-                                                       double* part, int span, double p_min, double p_max) {
+                                                       double* part, int span, double p_min, double p_max,
+                                                       const int32_t* skip) {
+  if (skip && *skip) return;
   __shared__ double red[256][NS];
   const int tid = threadIdx.x;
   const int pi = tid % 64, sub = tid / 64;
@@ -193,7 +196,8 @@ __global__ __launch_bounds__(256) void gcv_eval_kernel(const float* Y, const dou
 // and picks the next p-values (phases as described above).
 template <int PHASE>
 __global__ __launch_bounds__(1024) void gcv_decide_kernel(int N, GcvState* st, const double* part, double p_min,
-                                                          double p_max) {
+                                                          double p_max, const int32_t* skip) {
+  if (skip && *skip) return;
   constexpr int NS = (PHASE >= 3) ? 6 : 4;
   __shared__ double tot[kGrid][6];
   const int tid = threadIdx.x;
@@ -339,6 +343,23 @@ __global__ void gamma_apply_kernel(float* Y, const double* lam, int N, double p_
   if (k == 0 && p_out) *p_out = p;
 }
 
+// The GCV search on the spectrum Y (two fields, 2N floats): span, two
+// bracket refinements, two Newton steps; leaves p in st->p.
+void enqueue_gcv_search(const float* Y, const double* lam, int N, GcvState* st, double* part, double p_min,
+                        double p_max, const int32_t* skip, hipStream_t stream) {
+  const dim3 eb(kGcvBlocks), et(256);
+  hipLaunchKernelGGL(gcv_eval_kernel<4>, eb, et, 0, stream, Y, lam, N, st, part, 1, p_min, p_max, skip);
+  hipLaunchKernelGGL(gcv_decide_kernel<0>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
+  hipLaunchKernelGGL(gcv_eval_kernel<4>, eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
+  hipLaunchKernelGGL(gcv_decide_kernel<1>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
+  hipLaunchKernelGGL(gcv_eval_kernel<4>, eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
+  hipLaunchKernelGGL(gcv_decide_kernel<2>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
+  hipLaunchKernelGGL(gcv_eval_kernel<6>, eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
+  hipLaunchKernelGGL(gcv_decide_kernel<3>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
+  hipLaunchKernelGGL(gcv_eval_kernel<6>, eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
+  hipLaunchKernelGGL(gcv_decide_kernel<4>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
+}
+
 }  // namespace
 
 constexpr size_t kGcvPartOffset = (sizeof(GcvState) + 255) & ~size_t(255);
@@ -357,17 +378,7 @@ hipError_t launch_smoothn(float* u, float* v, int m, int n, double p_min, double
   if (auto_s) {
     GcvState* st = reinterpret_cast<GcvState*>(w.state);
     double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(w.state) + kGcvPartOffset);
-    const dim3 eb(kGcvBlocks), et(256);
-    hipLaunchKernelGGL(gcv_eval_kernel<4>, eb, et, 0, stream, w.t1, w.lambda, N, st, part, 1, p_min, p_max);
-    hipLaunchKernelGGL(gcv_decide_kernel<0>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max);
-    hipLaunchKernelGGL(gcv_eval_kernel<4>, eb, et, 0, stream, w.t1, w.lambda, N, st, part, 0, p_min, p_max);
-    hipLaunchKernelGGL(gcv_decide_kernel<1>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max);
-    hipLaunchKernelGGL(gcv_eval_kernel<4>, eb, et, 0, stream, w.t1, w.lambda, N, st, part, 0, p_min, p_max);
-    hipLaunchKernelGGL(gcv_decide_kernel<2>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max);
-    hipLaunchKernelGGL(gcv_eval_kernel<6>, eb, et, 0, stream, w.t1, w.lambda, N, st, part, 0, p_min, p_max);
-    hipLaunchKernelGGL(gcv_decide_kernel<3>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max);
-    hipLaunchKernelGGL(gcv_eval_kernel<6>, eb, et, 0, stream, w.t1, w.lambda, N, st, part, 0, p_min, p_max);
-    hipLaunchKernelGGL(gcv_decide_kernel<4>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max);
+    enqueue_gcv_search(w.t1, w.lambda, N, st, part, p_min, p_max, nullptr, stream);
     hipLaunchKernelGGL(gamma_apply_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, w.t1, w.lambda, N, 0.0,
                        (const GcvState*)st, w.p_out);
   } else {
@@ -380,6 +391,568 @@ hipError_t launch_smoothn(float* u, float* v, int m, int n, double p_min, double
   e = gemm2(w.cos_m, w.cos_m, m, 1, w.t1, w.t1 + N, n, 0, w.t0, w.t0 + N, n, m, n, m, stream);
   if (e != hipSuccess) return e;
   return gemm2(w.t0, w.t0 + N, n, 0, w.cos_n, w.cos_n, n, 0, u, v, n, m, n, n, stream);
+}
+
+
+// ===========================================================================
+// Weighted smoothing for masked or missing vectors (SmoothnGPU with a mask,
+// gpu_smoothn.py:123-297): missing values take the value of their nearest
+// finite neighbour, weights are 1 at finite unmasked vectors and 0
+// elsewhere, z starts from a coarse DCT smoothing, and the relaxed update
+//   z <- r IDCT2(Gamma DCT2(W (f - z) + z)) + (1 - r) z,   r = 1.75
+// runs until ||dz|| / (||z|| + 1e-6) <= 1e-3 or 100 iterations, with the
+// GCV search re-run at the iterations k where k + 1 is a power of two.
+//
+// Every iteration is three launches: a row transform with the W (f - z) + z
+// prologue, a column-strip kernel doing the column DCT, the Gamma scaling
+// and the column inverse in LDS, and a row inverse with the relaxation
+// epilogue whose last block to finish reduces the change norms and raises
+// the converged flag.  All later launches read that flag and return at
+// once, so the host enqueues the full iteration budget with no
+// synchronisation (and the sequence can sit in a graph).
+// ===========================================================================
+namespace {
+
+constexpr int kMaxIters = 100;   // MAX_SMOOTHING_ITERS, gpu_smoothn.py:23
+constexpr double kTolZ = 1e-3;   // TOL, gpu_smoothn.py:24
+constexpr float kRelax = 1.75f;  // 1 + 0.75 * is_weighted, gpu_smoothn.py:162
+constexpr int kSH = 8;           // rows per block of the row transforms
+constexpr int kSW = 8;           // columns per block of the column transforms
+constexpr int kMaxDim = 1024;
+
+struct WState {
+  int32_t converged;   // the relative change of z reached the tolerance
+  int32_t result;      // ping-pong slot of the last update
+  int32_t iterations;
+  uint32_t arrivals;   // blocks of the current update launch that finished
+  double tol;
+};
+
+// Nearest finite row of each element within its column, ties to the upper
+// one; -1 for a column with no finite element.  One thread per column.
+__global__ __launch_bounds__(256) void nn_column_kernel(const float* u, const float* v, int m, int n, int32_t* nr) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int last = -1;
+  for (int i = 0; i < m; ++i) {
+    const size_t k = (size_t)i * n + j;
+    if (isfinite(u[k]) && isfinite(v[k])) last = i;
+    nr[k] = last;
+  }
+  int next = -1;
+  for (int i = m - 1; i >= 0; --i) {
+    const size_t k = (size_t)i * n + j;
+    if (isfinite(u[k]) && isfinite(v[k])) next = i;
+    const int up = nr[k];
+    nr[k] = (next >= 0 && (up < 0 || next - i < i - up)) ? next : up;
+  }
+}
+
+// Weights, and the data with every non-finite vector replaced by the
+// vector at the nearest finite position (exact Euclidean distance, unit
+// spacing; equal distances resolved to the smaller column, then the smaller
+// row -- the choice of scipy's exact feature transform used at
+// gpu_smoothn.py:244-251).  The column scan stops once |dj|^2 exceeds the
+// best squared distance.  Also resets the iteration state.
+__global__ __launch_bounds__(256) void nn_fill_kernel(const float* u, const float* v, const uint8_t* mask, int m,
+                                                      int n, const int32_t* nr, float* f, float* w, WState* st) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0) {
+    st->converged = 0;
+    st->result = 0;
+    st->iterations = 0;
+    st->arrivals = 0;
+    st->tol = 0.0;
+  }
+  const int N = m * n;
+  if (k >= N) return;
+  const int i = k / n, j = k % n;
+  const float a = u[k], b = v[k];
+  const bool fin = isfinite(a) && isfinite(b);
+  w[k] = (fin && !(mask && mask[k])) ? 1.f : 0.f;
+  if (fin) {
+    f[k] = a;
+    f[N + k] = b;
+    return;
+  }
+  long long best = LLONG_MAX;
+  int bi = -1, bj = -1;
+  for (int d = 0; d < n; ++d) {
+    if ((long long)d * d > best) break;
+    for (int side = 0; side < 2; ++side) {
+      if (d == 0 && side) break;
+      const int jj = side ? j + d : j - d;
+      if (jj < 0 || jj >= n) continue;
+      const int r = nr[(size_t)i * n + jj];
+      if (r < 0) continue;
+      const long long d2 = (long long)d * d + (long long)(r - i) * (r - i);
+      if (d2 < best || (d2 == best && (jj < bj || (jj == bj && r < bi)))) {
+        best = d2;
+        bi = r;
+        bj = jj;
+      }
+    }
+  }
+  // no finite vector at all: the weights are all zero and the value is never
+  // used unweighted; 0 keeps the transforms finite
+  f[k] = bi < 0 ? 0.f : u[(size_t)bi * n + bj];
+  f[N + k] = bi < 0 ? 0.f : v[(size_t)bi * n + bj];
+}
+
+__device__ __forceinline__ double block_sum_d(double x, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = x;
+  __syncthreads();
+  double t = 0.0;
+  for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += red[q];
+  return t;
+}
+
+// Row transforms out[r][c] = sum_j X[r][j] M[j][c] over kSH rows per block
+// (blockIdx.y = field, blockIdx.z = p-value of a batched GCV evaluation).
+struct RowArgs {
+  const float* in;      // pro 0 input (2N per batch entry)
+  const float* f;       // pro 1 / epi 2: filled data (2N)
+  const float* w;       // weights (N)
+  const float* z;       // current z (2N): pro 1, epi 1 relaxation
+  const float* M;       // n x n
+  float* out;           // 2N (epi 0, 1)
+  int m, n;
+  int pro;              // 0: in; 1: w (f - z) + z
+  int epi;              // 0: store; 1: relaxed update (+ change norms); 2: GCV residual partials
+  float relax;
+  int track;            // epi 1: change norms, last block decides convergence
+  int slot;             // epi 1: ping-pong slot written
+  int iteration;
+  const int32_t* skip;
+  WState* st;
+  double* part;         // epi 1: 2 per block; epi 2: (p, field, block) x {rss, trace}
+  const GcvState* gst;  // epi 2: p-values
+  const double* lambda; // epi 2
+};
+
+__global__ __launch_bounds__(256) void rows_kernel(RowArgs a) {
+  if (a.skip && *a.skip) return;
+  __shared__ float X[kSH][kMaxDim];
+  __shared__ double red[4];
+  __shared__ int is_last;
+  const int field = blockIdx.y, pz = blockIdx.z;
+  const int m = a.m, n = a.n;
+  const size_t N = (size_t)m * n;
+  const int r0 = blockIdx.x * kSH;
+  const float* in = a.in ? a.in + (size_t)pz * 2 * N + (size_t)field * N : nullptr;
+  for (int e = threadIdx.x; e < kSH * n; e += blockDim.x) {
+    const int rr = e / n, jc = e - rr * n, r = r0 + rr;
+    float x = 0.f;
+    if (r < m) {
+      const size_t k = (size_t)r * n + jc;
+      if (a.pro == 0) {
+        x = in[k];
+      } else {
+        const float zz = a.z[field * N + k];
+        x = a.w[k] * (a.f[field * N + k] - zz) + zz;
+      }
+    }
+    X[rr][jc] = x;
+  }
+  __syncthreads();
+  double s0 = 0.0, s1 = 0.0;
+  double trace = 0.0;
+  const double sp = a.epi == 2 ? pow(10.0, a.gst->pv[pz]) : 0.0;
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    float acc[kSH];
+#pragma unroll
+    for (int rr = 0; rr < kSH; ++rr) acc[rr] = 0.f;
+    for (int jj = 0; jj < n; ++jj) {
+      const float mv = a.M[(size_t)jj * n + c];
+#pragma unroll
+      for (int rr = 0; rr < kSH; ++rr) acc[rr] = fmaf(X[rr][jj], mv, acc[rr]);
+    }
+#pragma unroll
+    for (int rr = 0; rr < kSH; ++rr) {
+      const int r = r0 + rr;
+      if (r >= m) break;
+      const size_t k = (size_t)r * n + c;
+      if (a.epi == 0) {
+        a.out[field * N + k] = acc[rr];
+      } else if (a.epi == 1) {
+        float zn = acc[rr];
+        if (a.relax != 1.f) {
+          const float zo = a.z[field * N + k];
+          zn = a.relax * acc[rr] + (1.f - a.relax) * zo;
+          const double dz = (double)zo - (double)zn;
+          s0 += dz * dz;
+        }
+        s1 += (double)zn * (double)zn;
+        a.out[field * N + k] = zn;
+      } else {
+        const double res = (double)a.f[field * N + k] - (double)acc[rr];
+        s0 += (double)a.w[k] * res * res;
+        if (field == 0) trace += 1.0 / (1.0 + sp * a.lambda[k]);
+      }
+    }
+  }
+  if (a.epi == 0 || (a.epi == 1 && !a.track)) return;
+  const int nblk = gridDim.x;
+  if (a.epi == 2) {
+    s0 = block_sum_d(s0, red);
+    trace = block_sum_d(trace, red);
+    if (threadIdx.x == 0) {
+      double* o = a.part + (((size_t)pz * 2 + field) * nblk + blockIdx.x) * 2;
+      o[0] = s0;
+      o[1] = trace;
+    }
+    return;
+  }
+  s0 = block_sum_d(s0, red);
+  s1 = block_sum_d(s1, red);
+  const int bid = field * nblk + blockIdx.x, nb = nblk * gridDim.y;
+  if (threadIdx.x == 0) {
+    a.part[2 * bid] = s0;
+    a.part[2 * bid + 1] = s1;
+    __threadfence();
+    is_last = atomicAdd(&a.st->arrivals, 1u) == (unsigned)(nb - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    // fixed order: the decision does not depend on block completion order
+    double d2 = 0.0, z2 = 0.0;
+    const volatile double* pp = a.part;
+    for (int q = 0; q < nb; ++q) {
+      d2 += pp[2 * q];
+      z2 += pp[2 * q + 1];
+    }
+    const double tol = sqrt(d2) / (sqrt(z2) + 1e-6);
+    a.st->tol = tol;
+    a.st->result = a.slot;
+    a.st->iterations = a.iteration + 1;
+    a.st->arrivals = 0;
+    if (tol <= kTolZ) a.st->converged = 1;
+  }
+}
+
+// Column transforms on strips of kSW columns (blockIdx.y = field,
+// blockIdx.z = p-value of a batched GCV evaluation):
+//   mode 0: out = Cm^T (G .* (Cm X));  mode 1: out = Cm X;  mode 2: out = Cm^T (G .* X)
+// G = Gamma(p) = 1 / (1 + 10^p Lambda), or the coarse-start truncation
+// (1 where row < n0 or column < n1).  In-place use is safe: a block reads
+// its whole strip before writing it.
+struct ColArgs {
+  const float* in;
+  float* out;           // + z * 2N when batched
+  const float* Mf;      // forward, transposed: Mf[r * m + i] = C[i][r]
+  const float* Mi;      // inverse: C[k][i] at k * m + i
+  const double* lambda;
+  int m, n;
+  int mode;
+  int trunc;
+  int n0, n1;
+  double p_fixed;
+  const GcvState* gst;  // p = gst->p (or gst->pv[z] when batched) when non-null
+  int batched;
+  const int32_t* skip;
+};
+
+__global__ __launch_bounds__(256) void cols_kernel(ColArgs a) {
+  if (a.skip && *a.skip) return;
+  __shared__ float X[kMaxDim][kSW];
+  __shared__ float Y[kMaxDim][kSW];
+  const int field = blockIdx.y, pz = blockIdx.z;
+  const int m = a.m, n = a.n;
+  const size_t N = (size_t)m * n;
+  const int c0 = blockIdx.x * kSW;
+  const int nc = min(kSW, n - c0);
+  const float* in = a.in + (size_t)field * N;
+  float* out = a.out + (size_t)(a.batched ? pz : 0) * 2 * N + (size_t)field * N;
+  for (int e = threadIdx.x; e < m * kSW; e += blockDim.x) {
+    const int r = e / kSW, c = e - r * kSW;
+    X[r][c] = c < nc ? in[(size_t)r * n + c0 + c] : 0.f;
+  }
+  __syncthreads();
+  double s = 0.0;
+  if (!a.trunc) {
+    const double p = a.gst ? (a.batched ? a.gst->pv[pz] : a.gst->p) : a.p_fixed;
+    s = pow(10.0, p);
+  }
+  auto gain = [&](int i, int c) -> float {
+    if (a.trunc) return (i < a.n0 || c0 + c < a.n1) ? 1.f : 0.f;
+    return (float)(1.0 / (1.0 + s * a.lambda[(size_t)i * n + c0 + c]));
+  };
+  if (a.mode != 2) {
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      float acc[kSW];
+#pragma unroll
+      for (int c = 0; c < kSW; ++c) acc[c] = 0.f;
+      for (int r = 0; r < m; ++r) {
+        const float cv = a.Mf[(size_t)r * m + i];
+#pragma unroll
+        for (int c = 0; c < kSW; ++c) acc[c] = fmaf(cv, X[r][c], acc[c]);
+      }
+      if (a.mode == 1) {
+        for (int c = 0; c < nc; ++c) out[(size_t)i * n + c0 + c] = acc[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < kSW; ++c) Y[i][c] = c < nc ? acc[c] * gain(i, c) : 0.f;
+      }
+    }
+    if (a.mode == 1) return;
+  } else {
+    for (int e = threadIdx.x; e < m * kSW; e += blockDim.x) {
+      const int r = e / kSW, c = e - r * kSW;
+      Y[r][c] = c < nc ? X[r][c] * gain(r, c) : 0.f;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    float acc[kSW];
+#pragma unroll
+    for (int c = 0; c < kSW; ++c) acc[c] = 0.f;
+    for (int k = 0; k < m; ++k) {
+      const float cv = a.Mi[(size_t)k * m + i];
+#pragma unroll
+      for (int c = 0; c < kSW; ++c) acc[c] = fmaf(cv, Y[k][c], acc[c]);
+    }
+    for (int c = 0; c < nc; ++c) out[(size_t)i * n + c0 + c] = acc[c];
+  }
+}
+
+// Batched-residual GCV for w_mean <= 0.9 (gpu_smoothn.py:363-368): the
+// score of each pending p-value from the partials of rows_kernel (epi 2);
+// phase 0 scores the 10-point span, phases 1 and 2 a 64-point grid over the
+// bracket around the previous minimum, and phase 2 ends with the vertex of
+// the parabola through the minimum and its neighbours.
+template <int PHASE>
+__global__ __launch_bounds__(64) void gcv_slow_decide_kernel(int N, GcvState* st, const double* part, int nblk,
+                                                             const int32_t* skip) {
+  if (skip && *skip) return;
+  const int lane = threadIdx.x;
+  const int npv = st->npv;
+  double g = 1e300, pv = 0.0;
+  if (lane < npv) {
+    double rss = 0.0, tr = 0.0;
+    for (int fld = 0; fld < 2; ++fld)
+      for (int b = 0; b < nblk; ++b) {
+        const double* o = part + (((size_t)lane * 2 + fld) * nblk + b) * 2;
+        rss += o[0];
+        if (fld == 0) tr += o[1];
+      }
+    const double d = 1.0 - tr / N;
+    g = rss / N / (d * d) / 2.0;
+    pv = st->pv[lane];
+  }
+  double gb = g;
+  int ib = lane;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double go = __shfl_xor(gb, off, 64);
+    const int io = __shfl_xor(ib, off, 64);
+    if (go < gb || (go == gb && io < ib)) {
+      gb = go;
+      ib = io;
+    }
+  }
+  const int il = ib > 0 ? ib - 1 : 0, ir = ib < npv - 1 ? ib + 1 : npv - 1;
+  const double pl = __shfl(pv, il, 64), pb = __shfl(pv, ib, 64), pr = __shfl(pv, ir, 64);
+  const double gl = __shfl(g, il, 64), gr = __shfl(g, ir, 64);
+  __syncthreads();
+  if (PHASE < 2) {
+    st->pv[lane] = pl + (pr - pl) * (double)lane / (kGrid - 1);
+    if (lane == 0) st->npv = kGrid;
+  } else if (lane == 0) {
+    double p = pb;
+    if (ib > 0 && ib < npv - 1) {
+      const double a1 = (pb - pl) * (gb - gr), a2 = (pb - pr) * (gb - gl);
+      const double den = a1 - a2;
+      if (den != 0.0) {
+        const double q = pb - 0.5 * ((pb - pl) * a1 - (pb - pr) * a2) / den;
+        if (q >= pl && q <= pr) p = q;
+      }
+    }
+    st->p = p;
+  }
+}
+
+__global__ void gcv_slow_init_kernel(GcvState* st, double p_min, double p_max, const int32_t* skip) {
+  if (skip && *skip) return;
+  const int i = threadIdx.x;
+  if (i < kSpan) st->pv[i] = (double)i * (1.0 / (kSpan - 1)) * (p_max - p_min) + p_min;
+  if (i == 0) {
+    st->npv = kSpan;
+    st->p_min = p_min;
+    st->p_max = p_max;
+  }
+}
+
+// z at the converged (or last) slot, 0 at masked vectors (:224-225).
+__global__ __launch_bounds__(256) void masked_out_kernel(float* u, float* v, const uint8_t* mask, const float* z,
+                                                         int N, const WState* st, const GcvState* gst,
+                                                         double p_fixed, double* p_out, int32_t* iters_out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const float* zz = z + (size_t)st->result * 2 * N;
+  if (k < N) {
+    const bool mk = mask && mask[k];
+    u[k] = mk ? 0.f : zz[k];
+    v[k] = mk ? 0.f : zz[N + k];
+  }
+  if (k == 0) {
+    if (p_out) *p_out = gst ? gst->p : p_fixed;
+    if (iters_out) *iters_out = st->iterations;
+  }
+}
+
+constexpr size_t kWStateOffset =
+    ((((sizeof(GcvState) + 255) & ~size_t(255)) + (size_t)kGcvBlocks * kGrid * 6 * sizeof(double)) + 255) &
+    ~size_t(255);
+
+}  // namespace
+
+size_t smoothn_masked_state_bytes() { return kWStateOffset + 256; }
+
+size_t smoothn_masked_part_doubles(int m) { return 2 * 2 * (size_t)((m + kSH - 1) / kSH); }
+
+int smoothn_masked_max_dim() { return kMaxDim; }
+
+hipError_t launch_smoothn_masked(float* u, float* v, int m, int n, double p_min, double p_max, double fixed_p,
+                                 int auto_s, const SmoothnMaskedWork& w, hipStream_t stream) {
+  const int N = m * n;
+  if (N < 2) return hipSuccess;  // gpu_smoothn.py:137-140
+  if (m > kMaxDim || n > kMaxDim) return hipErrorInvalidValue;
+  char* base = reinterpret_cast<char*>(w.state);
+  GcvState* gst = reinterpret_cast<GcvState*>(base);
+  double* gpart = reinterpret_cast<double*>(base + kGcvPartOffset);
+  WState* ws = reinterpret_cast<WState*>(base + kWStateOffset);
+  const int32_t* skip = &ws->converged;
+  const size_t N2 = 2 * (size_t)N;
+  float* zs[2] = {w.z, w.z + N2};
+  const dim3 rgrid((m + kSH - 1) / kSH, 2), cgrid((n + kSW - 1) / kSW, 2), tb(256);
+
+  hipLaunchKernelGGL(nn_column_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, u, v, m, n, w.nr);
+  hipLaunchKernelGGL(nn_fill_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, u, v, w.mask, m, n, w.nr, w.f,
+                     w.w, ws);
+
+  RowArgs ra{};
+  ra.m = m;
+  ra.n = n;
+  ra.st = ws;
+  ra.part = w.part;
+  ColArgs ca{};
+  ca.m = m;
+  ca.n = n;
+  ca.Mf = w.cos_mt;
+  ca.Mi = w.cos_m;
+  ca.lambda = w.lambda;
+  ca.p_fixed = fixed_p;
+
+  // coarse start (gpu_smoothn.py:283-297): keep the coefficients with row <
+  // ceil(m/10)+1 or column < ceil(n/10)+1
+  RowArgs r0 = ra;
+  r0.in = w.f;
+  r0.M = w.cos_nt;
+  r0.out = w.t0;
+  hipLaunchKernelGGL(rows_kernel, rgrid, tb, 0, stream, r0);
+  ColArgs c0 = ca;
+  c0.in = w.t0;
+  c0.out = w.t1;
+  c0.mode = 0;
+  c0.trunc = 1;
+  c0.n0 = (m + 9) / 10 + 1;
+  c0.n1 = (n + 9) / 10 + 1;
+  hipLaunchKernelGGL(cols_kernel, cgrid, tb, 0, stream, c0);
+  RowArgs r1 = ra;
+  r1.in = w.t1;
+  r1.M = w.cos_n;
+  r1.out = zs[0];
+  r1.epi = 1;
+  r1.relax = 1.f;
+  hipLaunchKernelGGL(rows_kernel, rgrid, tb, 0, stream, r1);
+
+  for (int k = 0; k < kMaxIters; ++k) {
+    const float* zc = zs[k & 1];
+    RowArgs fa = ra;
+    fa.f = w.f;
+    fa.w = w.w;
+    fa.z = zc;
+    fa.pro = 1;
+    fa.M = w.cos_nt;
+    fa.out = w.t0;
+    fa.skip = skip;
+    hipLaunchKernelGGL(rows_kernel, rgrid, tb, 0, stream, fa);
+    const bool gcv = auto_s && ((k + 1) & k) == 0;
+    ColArgs cc = ca;
+    cc.skip = skip;
+    cc.gst = auto_s ? gst : nullptr;
+    if (gcv) {
+      ColArgs cs = cc;
+      cs.in = w.t0;
+      cs.out = w.t1;
+      cs.mode = 1;
+      hipLaunchKernelGGL(cols_kernel, cgrid, tb, 0, stream, cs);
+      if (!w.slow_gcv) {
+        enqueue_gcv_search(w.t1, w.lambda, N, gst, gpart, p_min, p_max, skip, stream);
+      } else {
+        hipLaunchKernelGGL(gcv_slow_init_kernel, dim3(1), dim3(64), 0, stream, gst, p_min, p_max, skip);
+        for (int ph = 0; ph < 3; ++ph) {
+          const int P = ph == 0 ? kSpan : kGrid;
+          ColArgs cb = cc;
+          cb.in = w.t1;
+          cb.out = w.qbatch;
+          cb.mode = 2;
+          cb.batched = 1;
+          hipLaunchKernelGGL(cols_kernel, dim3(cgrid.x, 2, P), tb, 0, stream, cb);
+          RowArgs rb = ra;
+          rb.in = w.qbatch;
+          rb.f = w.f;
+          rb.w = w.w;
+          rb.M = w.cos_n;
+          rb.epi = 2;
+          rb.gst = gst;
+          rb.lambda = w.lambda;
+          rb.part = gpart;
+          rb.skip = skip;
+          hipLaunchKernelGGL(rows_kernel, dim3(rgrid.x, 2, P), tb, 0, stream, rb);
+          if (ph == 0)
+            hipLaunchKernelGGL(gcv_slow_decide_kernel<0>, dim3(1), dim3(64), 0, stream, N, gst, gpart, (int)rgrid.x,
+                               skip);
+          else if (ph == 1)
+            hipLaunchKernelGGL(gcv_slow_decide_kernel<1>, dim3(1), dim3(64), 0, stream, N, gst, gpart, (int)rgrid.x,
+                               skip);
+          else
+            hipLaunchKernelGGL(gcv_slow_decide_kernel<2>, dim3(1), dim3(64), 0, stream, N, gst, gpart, (int)rgrid.x,
+                               skip);
+        }
+      }
+      ColArgs ci = cc;
+      ci.in = w.t1;
+      ci.out = w.t1;
+      ci.mode = 2;
+      hipLaunchKernelGGL(cols_kernel, cgrid, tb, 0, stream, ci);
+    } else {
+      cc.in = w.t0;
+      cc.out = w.t1;
+      cc.mode = 0;
+      hipLaunchKernelGGL(cols_kernel, cgrid, tb, 0, stream, cc);
+    }
+    RowArgs ru = ra;
+    ru.in = w.t1;
+    ru.M = w.cos_n;
+    ru.z = zc;
+    ru.out = zs[(k + 1) & 1];
+    ru.epi = 1;
+    ru.relax = kRelax;
+    ru.track = 1;
+    ru.slot = (k + 1) & 1;
+    ru.iteration = k;
+    ru.skip = skip;
+    hipLaunchKernelGGL(rows_kernel, rgrid, tb, 0, stream, ru);
+  }
+  hipLaunchKernelGGL(masked_out_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, u, v, w.mask, w.z, N,
+                     (const WState*)ws, auto_s ? (const GcvState*)gst : nullptr, fixed_p, w.p_out, w.iters_out);
+  return hipGetLastError();
 }
 
 }  // namespace piv

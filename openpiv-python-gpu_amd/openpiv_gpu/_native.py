@@ -36,7 +36,8 @@ EXPORTED_SYMBOLS = (
     "piv_cnn_prepare_input", "piv_plan_set_predictor", "piv_plan_last_fallback",
     "piv_batch_create", "piv_batch_destroy", "piv_batch_set_frames", "piv_batch_run", "piv_batch_tune",
     "piv_batch_pair_shape", "piv_batch_get", "piv_tiff_info", "piv_tiff_read", "piv_host_alloc", "piv_host_free",
-    "piv_memcpy_async", "piv_stream_wait_event", "piv_plan_wait_event",
+    "piv_memcpy_async", "piv_stream_wait_event", "piv_plan_wait_event", "piv_plan_set_mask",
+    "piv_plan_last_smoothing_iterations", "piv_stage_smoothn_masked",
 )
 
 SYNTH_MAX_VORTICES = 8
@@ -141,6 +142,9 @@ def _bind(lib):
         "piv_stage_median_validate": (i32, [vp, vp, i32, i32, f32, vp, vp, vp]),
         "piv_stage_spring_replace": (i32, [vp, vp, vp, vp, i32, i32, vp]),
         "piv_stage_smoothn": (i32, [vp, vp, i32, i32, f64, vp, vp]),
+        "piv_stage_smoothn_masked": (i32, [vp, vp, vp, i32, i32, f64, vp, vp, vp]),
+        "piv_plan_set_mask": (i32, [vp, vp, i32, i32]),
+        "piv_plan_last_smoothing_iterations": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int32)]),
         "piv_stage_correlate_dft": (i32, [vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
         "piv_synth_pair": (i32, [ctypes.POINTER(SynthField), ctypes.POINTER(SynthModel), ctypes.c_uint64, i32, i32,
                                  vp, vp, vp, vp, vp]),
@@ -386,6 +390,19 @@ class Plan:
         a, b = ctypes.c_int32(), ctypes.c_int32()
         check(self._lib.piv_plan_last_counts(self.handle, k, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def set_mask(self, mask):
+        """Frame mask (bool/number array of the frame shape; ``None`` removes it)."""
+        if mask is None:
+            check(self._lib.piv_plan_set_mask(self.handle, None, 0, 0))
+            return
+        m = np.ascontiguousarray(np.asarray(mask).astype(bool).view(np.uint8))
+        check(self._lib.piv_plan_set_mask(self.handle, m.ctypes.data, int(m.shape[0]), int(m.shape[1])))
+
+    def last_smoothing_iterations(self, k: int) -> int:
+        out = ctypes.c_int32()
+        check(self._lib.piv_plan_last_smoothing_iterations(self.handle, k, ctypes.byref(out)))
+        return out.value
 
     def set_predictor(self, net, max_reject_fraction: float, first_pass: int = -1):
         """Attach (or with ``net=None`` detach) a learned predictor network

@@ -202,7 +202,11 @@ class PIVGPU:
         if not (dtype_f == "float32" or dtype_f is np.float32):
             raise NotImplementedError("dtype_f: only 'float32' processing is implemented on the MI355X path")
         if mask is not None:
-            raise NotImplementedError("mask: masked processing is not implemented on the MI355X path")
+            mask = np.asarray(mask).astype(bool)                # gpu_process.py:548-549
+            if mask.shape != tuple(frame_shape):
+                raise ValueError(f"mask has shape {mask.shape}, the frames {tuple(frame_shape)}")
+            if predictor is not None:
+                raise NotImplementedError("a learned predictor cannot be combined with a mask")
         self.min_search_size = min_search_size
         self.ss_iters = (search_size_iters,) if isinstance(search_size_iters, int) else tuple(search_size_iters)
         self.n_passes = len(self.ss_iters)
@@ -234,7 +238,7 @@ class PIVGPU:
         self.smoothing_par = _tuple_of(smoothing_par, n, tol_scalar)
         self.dt = dt
         self.scaling_par = scaling_par
-        self.mask = None
+        self.mask = mask
         self.dtype_f = np.float32
         self.device = device
         self.use_graph = bool(use_graph)
@@ -281,7 +285,7 @@ class PIVGPU:
                 ws = ws if ws % 2 == 0 else ws + 1
                 ws = max(ws, 8)
             self.piv_fields.append(PIVFieldGPU(self.frame_shape, window_size=ws, search_size=ss, overlap_ratio=r,
-                                               center=self.center, mask=None, dtype_f=self.dtype_f))
+                                               center=self.center, mask=self.mask, dtype_f=self.dtype_f))
 
     def _plan_desc(self, in_dtype: int) -> _native.PlanDesc:
         fields = self.piv_fields
@@ -340,6 +344,8 @@ class PIVGPU:
     def _get_plan(self, in_dtype: int):
         if self._plan is None or self._plan.desc.in_dtype != in_dtype:
             self._plan = _native.Plan(self._plan_desc(in_dtype), self.device)
+            if self.mask is not None:
+                self._plan.set_mask(self.mask)
             if self.predictor is not None:
                 self._plan.set_predictor(self.predictor, self.max_reject_fraction, self.predictor_pass)
         return self._plan

@@ -56,6 +56,7 @@ import scipy.fft as sfft
 import scipy.sparse as sps
 import scipy.sparse.linalg as spla
 from scipy.interpolate import RegularGridInterpolator
+from scipy.ndimage import distance_transform_edt
 from scipy.optimize import fmin_l_bfgs_b
 
 EPS_SUBPIXEL = 1e-7  # gpu_process.py:1011
@@ -180,10 +181,11 @@ def _f32(a):
     return np.asarray(a, dtype=np.float32)
 
 
-def warp_frame(f, up_pix, vp_pix, dt, dtype=np.float32):
+def warp_frame(f, up_pix, vp_pix, dt, dtype=np.float32, mask=None):
     """Backward bilinear warp, clamped base index, unclamped fraction
     (gpu_process.py:1528-1546).  ``xi = x + up*dt`` is formed as one fused
-    multiply-add (rounded once), the way NVRTC contracts it."""
+    multiply-add (rounded once), the way NVRTC contracts it.  Masked pixels
+    stay 0 (cuda_interpolate_masked_frame, :1461-1506)."""
     ht, wd = f.shape
     f = np.asarray(f, dtype)
     yy, xx = np.meshgrid(np.arange(ht, dtype=dtype), np.arange(wd, dtype=dtype), indexing="ij")
@@ -208,10 +210,13 @@ def warp_frame(f, up_pix, vp_pix, dt, dtype=np.float32):
     ty1 = (yd.astype(dtype) - yi)
     tx0 = (xr.astype(dtype) - xi)
     tx1 = (xi - xl.astype(dtype))
-    return (ty0 * tx0 * fdl + ty0 * tx1 * fdr + ty1 * tx0 * ful + ty1 * tx1 * fur).astype(dtype)
+    out = (ty0 * tx0 * fdl + ty0 * tx1 * fdr + ty1 * tx0 * ful + ty1 * tx1 * fur).astype(dtype)
+    if mask is not None:
+        out[mask] = 0
+    return out
 
 
-def deform_frames(frame_a, frame_b, geom: Geometry, up, vp, deforming_par=0.5, dtype=np.float32):
+def deform_frames(frame_a, frame_b, geom: Geometry, up, vp, deforming_par=0.5, dtype=np.float32, mask=None):
     """gpu_process.py:1149-1173: predictor interpolated (FP64 RGI) from the
     current grid onto every pixel, cast to dtype_f, then frame A warped at
     ``dt-1`` and frame B at ``dt`` (0 < dt < 1); dt == 1 warps B only,
@@ -222,10 +227,29 @@ def deform_frames(frame_a, frame_b, geom: Geometry, up, vp, deforming_par=0.5, d
     vp_pix = rgi_linear(geom.y_grid, geom.x_grid, vp, yy, xx).astype(dtype)
     dt = deforming_par
     if 0 < dt < 1:
-        return warp_frame(frame_a, up_pix, vp_pix, dt - 1, dtype), warp_frame(frame_b, up_pix, vp_pix, dt, dtype)
+        return (warp_frame(frame_a, up_pix, vp_pix, dt - 1, dtype, mask),
+                warp_frame(frame_b, up_pix, vp_pix, dt, dtype, mask))
     if dt == 1:
-        return np.asarray(frame_a, dtype), warp_frame(frame_b, up_pix, vp_pix, 1, dtype)
-    return warp_frame(frame_a, up_pix, vp_pix, -1, dtype), np.asarray(frame_b, dtype)
+        return np.asarray(frame_a, dtype), warp_frame(frame_b, up_pix, vp_pix, 1, dtype, mask)
+    return warp_frame(frame_a, up_pix, vp_pix, -1, dtype, mask), np.asarray(frame_b, dtype)
+
+
+def field_mask(geom: Geometry, mask):
+    """PIVFieldGPU.get_field_mask (gpu_process.py:949-951): the frame mask
+    at the window centres, mask[int(y), int(x)] of the dtype_f coordinates."""
+    if mask is None:
+        return None
+    return np.asarray(mask, bool)[geom.y.astype(int), geom.x.astype(int)]
+
+
+def mask_frames(frame_a, frame_b, mask):
+    """PIVGPU.mask_frames (gpu_process.py:682-687) on the dtype_f copies."""
+    if mask is not None:
+        frame_a = frame_a.copy()
+        frame_b = frame_b.copy()
+        frame_a[mask] = 0
+        frame_b[mask] = 0
+    return frame_a, frame_b
 
 
 # ---------------------------------------------------------------------------
@@ -654,31 +678,124 @@ def gcv_score(p, f_dct, lam, n_fields):
     return rss / size / (1 - tr_h / size) ** 2 / n_fields
 
 
-def smoothn(fields: Sequence[np.ndarray], s=None, return_p=False):
-    """One unweighted pass of Garcia's DCT-PLS smoother (gpu_smoothn.py:
-    123-231): tol = 0 so a single iteration runs; with ``s=None`` the
-    smoothing parameter is chosen by GCV: the best of 10 evenly spaced
-    ``p = log10 s`` values, then ``fmin_l_bfgs_b(approx_grad, factr=10)``
-    within ``[p_min, p_max]`` (:179-193)."""
-    f = [np.asarray(a, np.float64) for a in fields]
+def smoothn(fields: Sequence[np.ndarray], s=None, return_p=False, mask=None, tol_z=1e-3, max_iters=100,
+            return_info=False):
+    """Garcia's DCT-PLS smoother as ``SmoothnGPU.__call__`` runs it
+    (gpu_smoothn.py:123-231), non-robust, order 2.
+
+    Unweighted (no masked and no missing value): ``tol = 0`` so one
+    iteration runs; with ``s=None`` the smoothing parameter is chosen by GCV:
+    the best of 10 evenly spaced ``p = log10 s`` values, then
+    ``fmin_l_bfgs_b(approx_grad, factr=10)`` within ``[p_min, p_max]``
+    (:179-193).
+
+    Weighted (``mask`` has a True element or a value is not finite,
+    :253-269): missing values are filled from their nearest finite
+    neighbour (:233-251), weights are 1 at finite unmasked values and 0
+    elsewhere, z starts from a coarse DCT smoothing (:271-297), and the
+    relaxed (factor 1.75) iteration runs until the relative change of z
+    falls to ``tol_z`` or ``max_iters`` iterations, with GCV re-run at the
+    iterations k where log2(k + 1) is an integer (L-BFGS-B from the current
+    p after the first span, :176-193); masked values are 0 on return
+    (:224-225)."""
+    f = [np.array(a, np.float64) for a in fields]
     shape = f[0].shape
     if f[0].size < 2:
         return (list(f), None) if return_p else list(f)
+    n_fields = len(f)
+    size = f[0].size
     lam = smoothn_lambda(shape)
-    f_dct = [dct2(a) for a in f]
-    if s is None:
-        p_min, p_max = smoothn_p_bounds(shape)
-        p_span = np.arange(10) * (1 / 9) * (p_max - p_min) + p_min
-        g = np.array([gcv_score(pi, f_dct, lam, len(f)) for pi in p_span])
-        p0 = p_span[np.argmin(g)]
-        p, _, _ = fmin_l_bfgs_b(gcv_score, np.array([p0]), fprime=None, factr=10, approx_grad=True,
-                                bounds=[(p_min, p_max)], args=(f_dct, lam, len(f)))
-        p = float(p[0])
+    mask = None if mask is None else np.asarray(mask, bool)
+    finite = np.ones(shape, bool)
+    for a in f:
+        finite &= np.isfinite(a)
+    missing = ~finite
+    if missing.any():                                                        # :233-251
+        idx = nearest_finite_index(missing)
+        for a in f:
+            a[missing] = a[idx[0], idx[1]][missing]
+    w = finite.astype(np.float64)                                            # :262
+    if mask is not None:
+        w = np.where(mask, 0.0, w)                                           # :266-268
+    weighted = bool(np.any(w != 1))
+    n_finite = int(finite.sum())
+    r = 1.0 + 0.75 * weighted                                                # :162
+    if weighted:
+        z = smoothn_init_z(f)                                                # :271-297
     else:
+        z = [np.zeros(shape) for _ in f]
+    p = None
+    p_min, p_max = smoothn_p_bounds(shape)
+    if s is not None:
         p = math.log10(s)
+    w_mean = float(np.mean(w))
+    k = 0
+    n_iter = 0
+    for k in range(max_iters):                                               # :173-214
+        z0 = z
+        f_dct = [dct2(w * (a - b) + b) for a, b in zip(f, z)]
+        if s is None and float(np.log2(k + 1)) % 1 == 0:
+            args = (f_dct, lam, n_fields, f, w, finite, w_mean, n_finite)
+            if not p:
+                p_span = np.arange(10) * (1 / 9) * (p_max - p_min) + p_min
+                g = np.array([gcv_score_weighted(pi, *args) for pi in p_span])
+                p = p_span[np.argmin(g)]
+            pv, _, _ = fmin_l_bfgs_b(gcv_score_weighted, np.array([p]), fprime=None, factr=10, approx_grad=True,
+                                     bounds=[(p_min, p_max)], args=args)
+            p = float(pv[0])
+        gamma = 1.0 / (1.0 + 10.0 ** p * lam)
+        z = [r * idct2(gamma * fd) + (1 - r) * b for fd, b in zip(f_dct, z)]
+        n_iter = k + 1
+        if weighted:
+            tol = (np.linalg.norm(np.concatenate([a - b for a, b in zip(z0, z)]))
+                   / (np.linalg.norm(np.concatenate(z)) + 1e-6))
+        else:
+            tol = 0.0
+        if tol <= tol_z:
+            break
+    if mask is not None:
+        z = [np.where(mask, 0.0, a) for a in z]
+    out = (z, p) if return_p else z
+    if return_info:
+        return out, {"iterations": n_iter, "weighted": weighted, "w_mean": w_mean}
+    return out
+
+
+def nearest_finite_index(missing):
+    """Index of the nearest non-missing element (unit spacing; the
+    reference calls scipy's exact Euclidean feature transform before
+    ``init_spacing`` runs, gpu_smoothn.py:244-251, 155)."""
+    return distance_transform_edt(missing, return_distances=False, return_indices=True)
+
+
+def smoothn_init_z(f):
+    """gpu_smoothn.py:283-297: DCT, zero the coefficients whose every index
+    is at least ceil(shape / 10) + 1, inverse DCT."""
+    z = []
+    for a in f:
+        d = dct2(a)
+        n0, n1 = (int(math.ceil(s / 10)) + 1 for s in d.shape)
+        d[n0:, n1:] = 0.0
+        z.append(idct2(d))
+    return z
+
+
+def gcv_score_weighted(p, f_dct, lam, n_fields, f, w, finite, w_mean, n_finite):
+    """gpu_smoothn.py:353-372: with w_mean > 0.9 the residual comes from the
+    spectrum alone; otherwise from the weighted data misfit at finite
+    points.  Normalised by the finite count."""
+    p = float(np.asarray(p).reshape(-1)[0])
     gamma = 1.0 / (1.0 + 10.0 ** p * lam)
-    z = [idct2(gamma * fd) for fd in f_dct]
-    return (z, p) if return_p else z
+    rss = 0.0
+    if w_mean > 0.9:
+        for fd in f_dct:
+            rss += np.linalg.norm(fd * (gamma - 1)) ** 2
+    else:
+        for a, fd in zip(f, f_dct):
+            fh = idct2(gamma * fd)
+            rss += np.linalg.norm(np.sqrt(w[finite]) * (a[finite] - fh[finite])) ** 2
+    tr_h = np.sum(gamma)
+    return rss / n_finite / (1 - tr_h / lam.size) ** 2 / n_fields
 
 
 def gcv_exact_minimiser(fields):
@@ -726,6 +843,7 @@ class OracleSettings:
     shrink_ratio: float = 1.0
     dt: float = 1.0
     scaling_par: float = 1.0
+    mask: Optional[np.ndarray] = None
 
 
 @dataclass
@@ -758,13 +876,18 @@ def run_pass(frame_a, frame_b, settings: OracleSettings, geom: Geometry, prev_ge
     dtype = np.float32 if mode == "faithful32" else np.float64
     fa = np.asarray(frame_a, dtype)
     fb = np.asarray(frame_b, dtype)
+    fmask = field_mask(geom, settings.mask)
     up = vp = None
     if prev_geom is not None:
         up, vp = interpolate_displacement(prev_geom, geom, u_prev, v_prev)   # :620-625
-        fa, fb = deform_frames(fa, fb, geom, up, vp, settings.deforming_par, dtype)
+        fa, fb = deform_frames(fa, fb, geom, up, vp, settings.deforming_par, dtype,
+                               None if settings.mask is None else np.asarray(settings.mask, bool))
     corr = correlate_pass(fa, fb, geom, settings.n_fft, settings.normalize, settings.subpixel_method, dtype,
                           return_planes=settings.s2n_tol is not None, window_size=window_size)
     u, v = corr["u"], corr["v"]
+    if fmask is not None:                                                    # :1296-1310
+        u = np.where(fmask, np.nan, u).astype(u.dtype)
+        v = np.where(fmask, np.nan, v).astype(v.dtype)
     if up is not None:                                                       # :631-632
         u, v = u + up, v + vp
     u_pre, v_pre = u, v
@@ -772,11 +895,15 @@ def run_pass(frame_a, frame_b, settings: OracleSettings, geom: Geometry, prev_ge
     if settings.s2n_tol is not None:                                         # :755
         s2n = s2n_ratio(corr["corr"], corr["i_peak"], corr["j_peak"], settings.s2n_method,
                         settings.s2n_size).reshape(u.shape)
+        if fmask is not None:                                                # :1338-1343
+            s2n = np.where(fmask, 0.0, s2n).astype(s2n.dtype)
         corr["s2n"] = s2n
+    keep = None if fmask is None else ~fmask                                 # gpu_validation.py:207-210
 
     def validate_fn(a, b, only=None):                                        # :758-769
+        o = keep if only is None else (only if keep is None else only & keep)
         return validate(a, b, s2n, settings.s2n_tol, settings.median_tol, settings.mad_tol, settings.mean_tol,
-                        settings.rms_tol, only=only)
+                        settings.rms_tol, only=o)
 
     val = validate_fn(u, v)
     u_rep, v_rep, _ = replace_outliers(u, v, val, settings.replacing_method, settings.num_replacing_iters,
@@ -786,7 +913,7 @@ def run_pass(frame_a, frame_b, settings: OracleSettings, geom: Geometry, prev_ge
         outliers = validate_fn(u_rep, v_rep)                                 # :783-784
     p = None
     if not is_last:                                                          # :637-639, Q2
-        (u_out, v_out), p = smoothn([u_rep, v_rep], s=settings.smoothing_par, return_p=True)
+        (u_out, v_out), p = smoothn([u_rep, v_rep], s=settings.smoothing_par, return_p=True, mask=fmask)
     else:
         u_out, v_out = u_rep, v_rep
     return PassState(geom, up, vp, corr, u_pre, v_pre, val, u_rep, v_rep, outliers, u_out, v_out, p)
@@ -800,6 +927,10 @@ def piv(frame_a, frame_b, settings: OracleSettings, mode="faithful32", keep_pass
     geoms = [field_geometry(frame_shape, ss, settings.overlap_ratio, settings.center) for ss in sizes]
     u = v = None
     passes = []
+    if settings.mask is not None:                                            # :586, 682-687
+        dtype = np.float32 if mode == "faithful32" else np.float64
+        frame_a, frame_b = mask_frames(np.asarray(frame_a, dtype), np.asarray(frame_b, dtype),
+                                       np.asarray(settings.mask, bool))
     for k, g in enumerate(geoms):
         ws = shrunk_window(g.search_size, settings.shrink_ratio) if k == 0 and settings.shrink_ratio != 1 else None
         st = run_pass(frame_a, frame_b, settings, g, geoms[k - 1] if k > 0 else None, u, v,

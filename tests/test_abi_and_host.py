@@ -115,13 +115,30 @@ def test_min_search_size_and_frame_shape_assertions():
 
 
 @pytest.mark.parametrize("kwargs,name", [
-    (dict(dtype_f="float64"), "dtype_f"), (dict(mask=np.zeros((256, 256), bool)), "mask"),
+    (dict(dtype_f="float64"), "dtype_f"),
     (dict(validation_size=2), "validation_size"),
     (dict(replacing_size=2), "replacing_size"),
 ])
 def test_unimplemented_options_raise(kwargs, name):
     with pytest.raises(NotImplementedError, match=name):
         gp.piv_gpu((256, 256), 16, **kwargs)
+
+
+def test_mask_settings_on_the_host():
+    """The frame mask reaches every pass's field (gpu_process.py:949-951) and
+    the class rejects what the plan cannot take."""
+    mask = np.zeros((256, 256), bool)
+    mask[100:140, 60:90] = True
+    piv = gp.piv_gpu((256, 256), 16, search_size_iters=(1, 2), mask=mask.astype(np.uint8))
+    for f in piv.gpu_process.piv_fields:
+        fm = f.field_mask
+        assert fm.shape == f.field_shape and fm.dtype == bool
+        assert np.array_equal(fm, mask[f.y.astype(int), f.x.astype(int)])
+    assert np.array_equal(piv.field_mask, piv.gpu_process.field_mask)
+    with pytest.raises(ValueError, match="mask has shape"):
+        gp.PIVGPU((256, 256), 16, mask=np.zeros((128, 256), bool))
+    with pytest.raises(NotImplementedError, match="predictor"):
+        gp.PIVGPU((256, 256), 16, mask=mask, predictor=object())
 
 
 def test_plan_desc_carries_every_validation_setting():

@@ -1,0 +1,164 @@
+"""Masked processing on the GPU against the oracle: the weighted smoother
+stage (both GCV branches, fixed s, missing values without a mask), the
+spring replacement with NaN neighbours, and the masked multi-pass loop
+end to end (gpu_process.py:494-549, 682-687, 949-951, 1296-1343, 1461-1506;
+gpu_validation.py:99-210; gpu_smoothn.py:123-372)."""
+
+import numpy as np
+import pytest
+
+from oracle import piv_oracle as O
+
+pytestmark = pytest.mark.gpu
+gu = pytest.importorskip("gpu_util")
+
+
+def _smooth_masked(u, v, mask, s=None):
+    from openpiv_gpu import _native
+    lib = _native.load_library()
+    m, n = u.shape
+    du, dv = gu.dev(np.asarray(u, np.float32)), gu.dev(np.asarray(v, np.float32))
+    dm = gu.dev(np.asarray(mask, np.uint8)) if mask is not None else None
+    p = _native.DeviceBuffer.zeros((1,), np.float64)
+    it = _native.DeviceBuffer.zeros((1,), np.int32)
+    _native.check(lib.piv_stage_smoothn_masked(gu.vp(du), gu.vp(dv), gu.vp(dm) if dm is not None else None, m, n,
+                                               float(s) if s else 0.0, gu.vp(p), gu.vp(it), None))
+    return gu.host(du), gu.host(dv), float(gu.host(p)[0]), int(gu.host(it)[0])
+
+
+def _field(shape, radius, seed, centre=(0.5, 0.4)):
+    rng = np.random.default_rng(seed)
+    m, n = shape
+    yy, xx = np.mgrid[0:m, 0:n]
+    u = np.sin(xx / 7.0) + 0.4 * np.cos(yy / 5.0) + 0.08 * rng.standard_normal(shape)
+    v = 0.6 * np.cos(xx / 9.0 + yy / 13.0) + 0.08 * rng.standard_normal(shape)
+    mask = (yy - centre[0] * m) ** 2 + (xx - centre[1] * n) ** 2 < radius ** 2
+    u[mask] = np.nan
+    v[mask] = np.nan
+    return u.astype(np.float32), v.astype(np.float32), mask
+
+
+def _compare(got, ref, mask, it_got, it_ref, rel=2e-3):
+    assert abs(it_got - it_ref) <= 1, (it_got, it_ref)
+    for g, r in zip(got, ref):
+        assert np.all(g[mask] == 0)
+        scale = np.abs(r).max()
+        assert np.abs(g - r).max() <= rel * scale, np.abs(g - r).max() / scale
+
+
+@pytest.mark.parametrize("shape,radius", [((63, 63), 9), ((40, 52), 7)])
+def test_weighted_fixed_s_matches_oracle(shape, radius):
+    u, v, mask = _field(shape, radius, 1)
+    gu_, gv_, p, it = _smooth_masked(u, v, mask, s=4.0)
+    (ru, rv), info = O.smoothn([u, v], s=4.0, mask=mask, return_info=True)
+    assert info["w_mean"] > 0.9
+    assert abs(p - np.log10(4.0)) < 1e-12
+    _compare((gu_, gv_), (ru, rv), mask, it, info["iterations"])
+
+
+def test_weighted_gcv_spectral_branch_matches_oracle():
+    u, v, mask = _field((63, 63), 9, 2)
+    gu_, gv_, p, it = _smooth_masked(u, v, mask)
+    ((ru, rv), rp), info = O.smoothn([u, v], mask=mask, return_p=True, return_info=True)
+    assert info["w_mean"] > 0.9
+    assert abs(p - rp) < 2e-2, (p, rp)
+    _compare((gu_, gv_), (ru, rv), mask, it, info["iterations"], rel=5e-3)
+
+
+def test_weighted_gcv_misfit_branch_matches_oracle():
+    u, v, mask = _field((47, 55), 15, 3)
+    gu_, gv_, p, it = _smooth_masked(u, v, mask)
+    ((ru, rv), rp), info = O.smoothn([u, v], mask=mask, return_p=True, return_info=True)
+    assert info["w_mean"] <= 0.9
+    assert abs(p - rp) < 2e-2, (p, rp)
+    _compare((gu_, gv_), (ru, rv), mask, it, info["iterations"], rel=5e-3)
+
+
+def test_missing_values_without_mask_are_smoothed_not_zeroed():
+    u, v, miss = _field((31, 37), 3, 4)
+    gu_, gv_, p, it = _smooth_masked(u, v, None, s=2.0)
+    (ru, rv), info = O.smoothn([u, v], s=2.0, return_info=True)
+    assert info["weighted"] and np.all(np.isfinite(gu_)) and np.any(gu_[miss] != 0)
+    _compare((gu_, gv_), (ru, rv), np.zeros_like(miss), it, info["iterations"])
+
+
+def test_unit_weights_fall_back_to_the_single_pass():
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal((20, 24)).astype(np.float32)
+    v = rng.standard_normal((20, 24)).astype(np.float32)
+    a = _smooth_masked(u, v, np.zeros((20, 24), bool), s=3.0)
+    b = gu.stage_smoothn(u, v, s=3.0)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[3] == 0
+
+
+def test_spring_replacement_with_nan_neighbours():
+    """Masked (NaN) neighbours drop out of the right-hand side's mean; a
+    flagged vector whose neighbours are all NaN keeps its value."""
+    rng = np.random.default_rng(6)
+    m, n = 12, 14
+    u = rng.standard_normal((m, n)).astype(np.float32)
+    v = rng.standard_normal((m, n)).astype(np.float32)
+    nanm = np.zeros((m, n), bool)
+    nanm[4:7, 4:8] = True
+    nanm[9, 1] = nanm[8, 0] = nanm[10, 0] = True       # (9, 0) is surrounded by NaN
+    u[nanm] = np.nan
+    v[nanm] = np.nan
+    val = np.zeros((m, n), bool)
+    val[3, 5] = val[7, 6] = val[5, 8] = val[5, 9] = val[9, 0] = val[1, 1] = True
+    ru, rv, _ = O.replace_outliers(u, v, val, "spring", 1)
+    gu_, gv_ = gu.stage_replace(u, v, val)
+    assert gu_[9, 0] == u[9, 0] and gv_[9, 0] == v[9, 0]
+    fin = ~nanm
+    assert np.allclose(gu_[fin], ru[fin], atol=1e-5) and np.allclose(gv_[fin], rv[fin], atol=1e-5)
+    assert np.array_equal(np.isnan(gu_), nanm)
+
+
+def _tie_windows(frame_a, frame_b, settings, passes, mask):
+    """Near-tie windows of the final pass, from the oracle's own planes."""
+    st = passes[-1]
+    fa, fb = O.mask_frames(np.asarray(frame_a, np.float32), np.asarray(frame_b, np.float32), mask)
+    fa, fb = O.deform_frames(fa, fb, st.geom, st.up, st.vp, settings.deforming_par, np.float32, mask)
+    r = O.correlate_pass(fa, fb, st.geom, return_planes=True)
+    return gu.near_tie_mask(r["corr"]).reshape(st.geom.field_shape)
+
+
+@pytest.fixture(scope="module")
+def masked_runs():
+    from openpiv_gpu import synthetic
+    from openpiv_gpu.gpu_process import piv_gpu
+    shape = (384, 448)
+    disp = synthetic.sine_displacement(2.0, 1.5, 160.0)
+    a, b = synthetic.make_pair(shape, disp, synthetic.ParticleModel(), 21)
+    yy, xx = np.mgrid[0:shape[0], 0:shape[1]]
+    mask = ((yy - 190) ** 2 + (xx - 210) ** 2 < 55 ** 2) | (xx > 420)
+    kw = dict(search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2, mask=mask, dtype_f="float32")
+    piv = piv_gpu(shape, 16, **kw)
+    runs = []
+    for _ in range(3):   # eager, graph capture, replay
+        u, v = piv(a, b)
+        runs.append((u, v, piv.val_locations.copy(), piv.outliers.copy()))
+    s = O.OracleSettings(min_search_size=16, search_size_iters=(1, 1, 1), mask=mask)
+    ref = O.piv(a, b, s, keep_passes=True)
+    return piv, runs, ref, mask, s, a, b
+
+
+def test_masked_three_pass_matches_oracle(masked_runs):
+    piv, runs, ref, mask, s, a, b = masked_runs
+    u, v, val, outl = runs[0]
+    g = ref["passes"][-1].geom
+    fm = O.field_mask(g, mask)
+    assert np.array_equal(piv.field_mask, fm)
+    assert np.array_equal(np.isnan(u), fm) and np.array_equal(np.isnan(v), fm)
+    ok = ~fm & ~_tie_windows(a, b, s, ref["passes"], mask)
+    du = np.abs(u - ref["u"])[ok]
+    dv = np.abs(v - ref["v"])[ok]
+    assert np.quantile(du, 0.99) <= 1e-2 and np.quantile(dv, 0.99) <= 1e-2, (du.max(), dv.max())
+    assert not val[fm].any() and not outl[fm].any()
+    assert (val != ref["val_locations"]).sum() <= max(2, val.size // 500)
+
+
+def test_masked_graph_replay_is_bitwise_stable(masked_runs):
+    _, runs, _, _, _, _, _ = masked_runs
+    for r in runs[1:]:
+        for x, y in zip(r, runs[0]):
+            assert np.array_equal(x, y, equal_nan=True) if x.dtype != bool else np.array_equal(x, y)
