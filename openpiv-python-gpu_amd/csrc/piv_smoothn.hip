@@ -429,22 +429,33 @@ struct WState {
 };
 
 // Nearest finite row of each element within its column, ties to the upper
-// one; -1 for a column with no finite element.  One thread per column.
+// one; -1 for a column with no finite element.  One 256-thread block per
+// column: the last finite row at or above (max-scan) and the first at or
+// below (min-scan) by Hillis-Steele scans in LDS (m <= kMaxDim).
 __global__ __launch_bounds__(256) void nn_column_kernel(const float* u, const float* v, int m, int n, int32_t* nr) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  int last = -1;
-  for (int i = 0; i < m; ++i) {
+  __shared__ int up[2][1024];
+  __shared__ int dn[2][1024];
+  const int j = blockIdx.x;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
     const size_t k = (size_t)i * n + j;
-    if (isfinite(u[k]) && isfinite(v[k])) last = i;
-    nr[k] = last;
+    const bool fin = isfinite(u[k]) && isfinite(v[k]);
+    up[0][i] = fin ? i : -1;
+    dn[0][i] = fin ? i : INT_MAX;
   }
-  int next = -1;
-  for (int i = m - 1; i >= 0; --i) {
-    const size_t k = (size_t)i * n + j;
-    if (isfinite(u[k]) && isfinite(v[k])) next = i;
-    const int up = nr[k];
-    nr[k] = (next >= 0 && (up < 0 || next - i < i - up)) ? next : up;
+  __syncthreads();
+  int cur = 0;
+  for (int off = 1; off < m; off <<= 1) {
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const int a = up[cur][i], b = dn[cur][i];
+      up[cur ^ 1][i] = i >= off ? max(a, up[cur][i - off]) : a;
+      dn[cur ^ 1][i] = i + off < m ? min(b, dn[cur][i + off]) : b;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const int a = up[cur][i], b = dn[cur][i];
+    nr[(size_t)i * n + j] = (b != INT_MAX && (a < 0 || b - i < i - a)) ? b : a;
   }
 }
 
@@ -831,7 +842,7 @@ hipError_t launch_smoothn_masked(float* u, float* v, int m, int n, double p_min,
   float* zs[2] = {w.z, w.z + N2};
   const dim3 rgrid((m + kSH - 1) / kSH, 2), cgrid((n + kSW - 1) / kSW, 2), tb(256);
 
-  hipLaunchKernelGGL(nn_column_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, u, v, m, n, w.nr);
+  hipLaunchKernelGGL(nn_column_kernel, dim3(n), dim3(256), 0, stream, u, v, m, n, w.nr);
   hipLaunchKernelGGL(nn_fill_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, u, v, w.mask, m, n, w.nr, w.f,
                      w.w, ws);
 

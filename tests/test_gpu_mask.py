@@ -38,8 +38,11 @@ def _field(shape, radius, seed, centre=(0.5, 0.4)):
     return u.astype(np.float32), v.astype(np.float32), mask
 
 
-def _compare(got, ref, mask, it_got, it_ref, rel=2e-3):
-    assert abs(it_got - it_ref) <= 1, (it_got, it_ref)
+def _compare(got, ref, mask, it_got, it_ref, rel, d_iter=0):
+    """Masked entries exactly 0; elsewhere within rel of the field's largest
+    magnitude; iteration counts equal (or within d_iter where the stopping
+    test sits near its threshold)."""
+    assert abs(it_got - it_ref) <= d_iter, (it_got, it_ref)
     for g, r in zip(got, ref):
         assert np.all(g[mask] == 0)
         scale = np.abs(r).max()
@@ -53,7 +56,8 @@ def test_weighted_fixed_s_matches_oracle(shape, radius):
     (ru, rv), info = O.smoothn([u, v], s=4.0, mask=mask, return_info=True)
     assert info["w_mean"] > 0.9
     assert abs(p - np.log10(4.0)) < 1e-12
-    _compare((gu_, gv_), (ru, rv), mask, it, info["iterations"])
+    # FP32 transforms against the FP64 oracle: measured 2e-6
+    _compare((gu_, gv_), (ru, rv), mask, it, info["iterations"], rel=2e-5)
 
 
 def test_weighted_gcv_spectral_branch_matches_oracle():
@@ -61,17 +65,22 @@ def test_weighted_gcv_spectral_branch_matches_oracle():
     gu_, gv_, p, it = _smooth_masked(u, v, mask)
     ((ru, rv), rp), info = O.smoothn([u, v], mask=mask, return_p=True, return_info=True)
     assert info["w_mean"] > 0.9
-    assert abs(p - rp) < 2e-2, (p, rp)
-    _compare((gu_, gv_), (ru, rv), mask, it, info["iterations"], rel=5e-3)
+    # spectral GCV branch: the device search and L-BFGS-B agree to 1e-5 in p here
+    assert abs(p - rp) < 1e-3, (p, rp)
+    _compare((gu_, gv_), (ru, rv), mask, it, info["iterations"], rel=5e-4)
 
 
 def test_weighted_gcv_misfit_branch_matches_oracle():
+    """w_mean <= 0.9: the GCV residual comes from the data misfit.  The
+    reference's L-BFGS-B often stops at its starting p here (its projected
+    gradient test passes at once), up to ~0.01 from the GCV minimum, while
+    the device search goes to the minimum; p and z agree to that level."""
     u, v, mask = _field((47, 55), 15, 3)
     gu_, gv_, p, it = _smooth_masked(u, v, mask)
     ((ru, rv), rp), info = O.smoothn([u, v], mask=mask, return_p=True, return_info=True)
     assert info["w_mean"] <= 0.9
     assert abs(p - rp) < 2e-2, (p, rp)
-    _compare((gu_, gv_), (ru, rv), mask, it, info["iterations"], rel=5e-3)
+    _compare((gu_, gv_), (ru, rv), mask, it, info["iterations"], rel=3e-3, d_iter=1)
 
 
 def test_missing_values_without_mask_are_smoothed_not_zeroed():
@@ -79,7 +88,7 @@ def test_missing_values_without_mask_are_smoothed_not_zeroed():
     gu_, gv_, p, it = _smooth_masked(u, v, None, s=2.0)
     (ru, rv), info = O.smoothn([u, v], s=2.0, return_info=True)
     assert info["weighted"] and np.all(np.isfinite(gu_)) and np.any(gu_[miss] != 0)
-    _compare((gu_, gv_), (ru, rv), np.zeros_like(miss), it, info["iterations"])
+    _compare((gu_, gv_), (ru, rv), np.zeros_like(miss), it, info["iterations"], rel=2e-5)
 
 
 def test_unit_weights_fall_back_to_the_single_pass():
@@ -152,9 +161,12 @@ def test_masked_three_pass_matches_oracle(masked_runs):
     ok = ~fm & ~_tie_windows(a, b, s, ref["passes"], mask)
     du = np.abs(u - ref["u"])[ok]
     dv = np.abs(v - ref["v"])[ok]
-    assert np.quantile(du, 0.99) <= 1e-2 and np.quantile(dv, 0.99) <= 1e-2, (du.max(), dv.max())
+    assert du.max() <= 1e-3 and dv.max() <= 1e-3, (du.max(), dv.max())
     assert not val[fm].any() and not outl[fm].any()
-    assert (val != ref["val_locations"]).sum() <= max(2, val.size // 500)
+    assert np.array_equal(val, ref["val_locations"]) and np.array_equal(outl, ref["outliers"])
+    for k in range(2):
+        assert abs(piv.gpu_process._plan.last_smoothing(k) - ref["passes"][k].smooth_p) < 5e-3
+        assert piv.gpu_process._plan.last_smoothing_iterations(k) > 1
 
 
 def test_masked_graph_replay_is_bitwise_stable(masked_runs):
