@@ -38,7 +38,10 @@ enum piv_status {
   PIV_E_NOMEM = -4
 };
 
-enum piv_in_dtype { PIV_U8 = 0, PIV_U16 = 1, PIV_F32 = 2 };
+enum piv_in_dtype { PIV_U8 = 0, PIV_U16 = 1, PIV_F32 = 2, PIV_F64 = 3 /* FP64 plans only */ };
+/* Processing precision, the reference's dtype_f (gpu_process.py:552-562):
+ * FP64 computes every frame, window, plane and field value in double. */
+enum piv_precision { PIV_FP32 = 0, PIV_FP64 = 1 };
 enum piv_subpixel { PIV_GAUSSIAN = 0, PIV_PARABOLIC = 1, PIV_CENTROID = 2 };
 
 /* One pass of the schedule.  Geometry is computed by the caller with the
@@ -86,6 +89,10 @@ typedef struct piv_plan_desc {
   piv_pass_desc passes[PIV_MAX_PASSES];
   float out_scale;            /* scaling_par / dt (gpu_process.py:847-849) */
   int32_t use_graph;          /* capture the pass loop in a hipGraph     */
+  int32_t precision;          /* enum piv_precision                      */
+  int32_t reserved;
+  double scaling_par;         /* FP64 plans: (u * scaling_par) / dt when */
+  double dt;                  /* either is not 1 (out_scale is FP32's)   */
 } piv_plan_desc;
 
 typedef struct piv_plan piv_plan;
@@ -107,13 +114,14 @@ void piv_plan_destroy(piv_plan *plan);
  * (gpu_process.py:564-647) including the host-to-device upload (:582-583).
  * frame_a/frame_b: height*width elements of in_dtype, on the host
  * (frames_on_device == 0) or the device.  Outputs (final pass, field_m x
- * field_n of the last pass, row-major): u, v as float32; val_init = flags
+ * field_n of the last pass, row-major): u, v as float32 (float64 for FP64
+ * plans); val_init = flags
  * before replacement (PIVGPU.init_val_locations, :770-771); val_final =
  * flags after replacement (PIVGPU.val_locations, :783-784).  Any output
  * pointer may be NULL.  outputs_on_device selects host or device outputs.
  * Device-resident frames are read after all work queued before the call on
  * the null stream.  Synchronous: returns after the outputs are written. */
-int piv_run(piv_plan *plan, const void *frame_a, const void *frame_b, int frames_on_device, float *u, float *v,
+int piv_run(piv_plan *plan, const void *frame_a, const void *frame_b, int frames_on_device, void *u, void *v,
             uint8_t *val_init, uint8_t *val_final, int outputs_on_device);
 
 /* Diagnostics of the last run: smoothing parameter log10(s) chosen after
@@ -152,6 +160,14 @@ int piv_plan_last_smoothing_iterations(const piv_plan *plan, int pass, int32_t *
 int piv_stage_correlate(const void *frame_a, const void *frame_b, int in_dtype, int height, int width,
                         const piv_pass_desc *pass, const float *up, const float *vp, float *u, float *v,
                         int32_t *i_peak, int32_t *j_peak, float *planes, float *s2n, void *stream);
+
+/* FP64 variant of piv_stage_correlate (dtype_f = float64, USE_LONG,
+ * gpu_process.py:561-562, 1378-1449): frames of any in_dtype (PIV_F64
+ * included), predictor and outputs in double, planes N x F x F double.
+ * Allocates its twiddles and scratch; synchronises the stream. */
+int piv_stage_correlate64(const void *frame_a, const void *frame_b, int in_dtype, int height, int width,
+                          const piv_pass_desc *pass, const double *up, const double *vp, double *u, double *v,
+                          int32_t *i_peak, int32_t *j_peak, double *planes, double *s2n, void *stream);
 
 /* Frame deformation: predictor (field_m x field_n on the pass grid)
  * interpolated onto every pixel with linear extrapolation, then the backward

@@ -74,7 +74,7 @@ int check_pass(const piv_pass_desc& p, int ht, int wd) {
   return check_tests(p);
 }
 
-size_t dtype_size(int d) { return d == PIV_U8 ? 1 : (d == PIV_U16 ? 2 : 4); }
+size_t dtype_size(int d) { return d == PIV_U8 ? 1 : (d == PIV_U16 ? 2 : (d == PIV_F32 ? 4 : 8)); }
 
 // Grid node coordinates of a pass (gpu_process.py:940-947): x = j*sp + ss/2 + x_off.
 float grid_x0(const piv_pass_desc& p) { return (float)p.search_size / 2.0f + (float)p.x_offset; }
@@ -95,11 +95,19 @@ void p_bounds(int m, int n, double* pmin, double* pmax) {
 }
 
 // Orthonormal DCT-II matrix C[k][i] = s_k cos(pi (2i+1) k / (2n)).
-std::vector<float> dct_matrix(int n) {
-  std::vector<float> c((size_t)n * n);
+// FP32 tables round the FP64 value; FP64 tables round an extended-precision one.
+template <typename R>
+std::vector<R> dct_matrix(int n) {
+  std::vector<R> c((size_t)n * n);
   for (int k = 0; k < n; ++k) {
-    const double s = k == 0 ? sqrt(1.0 / n) : sqrt(2.0 / n);
-    for (int i = 0; i < n; ++i) c[(size_t)k * n + i] = (float)(s * cos(M_PI * (2 * i + 1) * k / (2.0 * n)));
+    if (sizeof(R) == 4) {
+      const double s = k == 0 ? sqrt(1.0 / n) : sqrt(2.0 / n);
+      for (int i = 0; i < n; ++i) c[(size_t)k * n + i] = (R)(s * cos(M_PI * (2 * i + 1) * k / (2.0 * n)));
+    } else {
+      const long double s = k == 0 ? sqrtl(1.0L / n) : sqrtl(2.0L / n);
+      for (int i = 0; i < n; ++i)
+        c[(size_t)k * n + i] = (R)(s * cosl(3.14159265358979323846264338327950288L * (2 * i + 1) * k / (2.0L * n)));
+    }
   }
   return c;
 }
@@ -116,36 +124,42 @@ std::vector<double> lambda_table(int m, int n) {
   return l;
 }
 
+// DCT matrices in the plan's value type (void*: float or double).
 struct SmoothTables {
-  float* cos_m = nullptr;
-  float* cos_n = nullptr;
-  float* cos_mt = nullptr;  // transposes (weighted path)
-  float* cos_nt = nullptr;
+  void* cos_m = nullptr;
+  void* cos_n = nullptr;
+  void* cos_mt = nullptr;  // transposes (weighted path)
+  void* cos_nt = nullptr;
   double* lambda = nullptr;
 };
 
-std::vector<float> transposed(const std::vector<float>& c, int n) {
-  std::vector<float> t(c.size());
+template <typename R>
+std::vector<R> transposed(const std::vector<R>& c, int n) {
+  std::vector<R> t(c.size());
   for (int a = 0; a < n; ++a)
     for (int b = 0; b < n; ++b) t[(size_t)b * n + a] = c[(size_t)a * n + b];
   return t;
 }
 
-int make_tables(int m, int n, SmoothTables* t) {
-  auto cm = dct_matrix(m), cn = dct_matrix(n);
-  auto cmt = transposed(cm, m), cnt = transposed(cn, n);
-  auto lam = lambda_table(m, n);
-  HIP_TRY(hipMalloc(&t->cos_m, cm.size() * sizeof(float)));
-  HIP_TRY(hipMalloc(&t->cos_n, cn.size() * sizeof(float)));
-  HIP_TRY(hipMalloc(&t->cos_mt, cm.size() * sizeof(float)));
-  HIP_TRY(hipMalloc(&t->cos_nt, cn.size() * sizeof(float)));
-  HIP_TRY(hipMalloc(&t->lambda, lam.size() * sizeof(double)));
-  HIP_TRY(hipMemcpy(t->cos_m, cm.data(), cm.size() * sizeof(float), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(t->cos_n, cn.data(), cn.size() * sizeof(float), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(t->cos_mt, cmt.data(), cmt.size() * sizeof(float), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(t->cos_nt, cnt.data(), cnt.size() * sizeof(float), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(t->lambda, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice));
+template <typename R>
+int upload(void** dst, const std::vector<R>& v) {
+  HIP_TRY(hipMalloc(dst, v.size() * sizeof(R)));
+  HIP_TRY(hipMemcpy(*dst, v.data(), v.size() * sizeof(R), hipMemcpyHostToDevice));
   return PIV_OK;
+}
+
+template <typename R>
+int make_tables_t(int m, int n, SmoothTables* t) {
+  auto cm = dct_matrix<R>(m), cn = dct_matrix<R>(n);
+  int r = upload(&t->cos_m, cm);
+  if (!r) r = upload(&t->cos_n, cn);
+  if (!r) r = upload(&t->cos_mt, transposed(cm, m));
+  if (!r) r = upload(&t->cos_nt, transposed(cn, n));
+  if (!r) r = upload((void**)&t->lambda, lambda_table(m, n));
+  return r;
+}
+int make_tables(int m, int n, SmoothTables* t, bool fp64 = false) {
+  return fp64 ? make_tables_t<double>(m, n, t) : make_tables_t<float>(m, n, t);
 }
 void free_tables(SmoothTables* t) {
   for (void* p : {(void*)t->cos_m, (void*)t->cos_n, (void*)t->cos_mt, (void*)t->cos_nt, (void*)t->lambda})
@@ -157,23 +171,23 @@ void free_tables(SmoothTables* t) {
 // and maxM rows (slow: also the batched GCV residual buffer).
 struct MaskedSmoothBufs {
   int32_t* nr = nullptr;
-  float* f = nullptr;
-  float* w = nullptr;
-  float* z = nullptr;
+  void* f = nullptr;       // value-type arrays (float or double)
+  void* w = nullptr;
+  void* z = nullptr;
   double* part = nullptr;
   void* state = nullptr;
-  float* qbatch = nullptr;
+  void* qbatch = nullptr;
 };
 
-int alloc_masked_smooth(int maxN, int maxM, bool slow, MaskedSmoothBufs* b) {
+int alloc_masked_smooth(int maxN, int maxM, bool slow, MaskedSmoothBufs* b, size_t esize = 4) {
   HIP_TRY(hipMalloc(&b->nr, (size_t)maxN * 4));
-  HIP_TRY(hipMalloc(&b->f, 2ull * maxN * 4));
-  HIP_TRY(hipMalloc(&b->w, (size_t)maxN * 4));
-  HIP_TRY(hipMalloc(&b->z, 4ull * maxN * 4));
+  HIP_TRY(hipMalloc(&b->f, 2ull * maxN * esize));
+  HIP_TRY(hipMalloc(&b->w, (size_t)maxN * esize));
+  HIP_TRY(hipMalloc(&b->z, 4ull * maxN * esize));
   HIP_TRY(hipMalloc(&b->part, smoothn_masked_part_doubles(maxM) * 8));
   HIP_TRY(hipMalloc(&b->state, smoothn_masked_state_bytes()));
   HIP_TRY(hipMemset(b->state, 0, smoothn_masked_state_bytes()));
-  if (slow) HIP_TRY(hipMalloc(&b->qbatch, 64ull * 2 * maxN * 4));
+  if (slow) HIP_TRY(hipMalloc(&b->qbatch, 64ull * 2 * maxN * esize));
   return PIV_OK;
 }
 void free_masked_smooth(MaskedSmoothBufs* b) {
@@ -182,24 +196,25 @@ void free_masked_smooth(MaskedSmoothBufs* b) {
   *b = MaskedSmoothBufs{};
 }
 
-SmoothnMaskedWork masked_work(const SmoothTables& tab, const uint8_t* fmask, const MaskedSmoothBufs& b, float* t0,
-                              float* t1, double* p_out, int32_t* iters_out, int slow) {
-  SmoothnMaskedWork w{};
-  w.cos_m = tab.cos_m;
-  w.cos_mt = tab.cos_mt;
-  w.cos_n = tab.cos_n;
-  w.cos_nt = tab.cos_nt;
+template <typename R>
+SmoothnMaskedWorkT<R> masked_work(const SmoothTables& tab, const uint8_t* fmask, const MaskedSmoothBufs& b, void* t0,
+                                  void* t1, double* p_out, int32_t* iters_out, int slow) {
+  SmoothnMaskedWorkT<R> w{};
+  w.cos_m = (const R*)tab.cos_m;
+  w.cos_mt = (const R*)tab.cos_mt;
+  w.cos_n = (const R*)tab.cos_n;
+  w.cos_nt = (const R*)tab.cos_nt;
   w.lambda = tab.lambda;
   w.mask = fmask;
   w.nr = b.nr;
-  w.f = b.f;
-  w.w = b.w;
-  w.z = b.z;
-  w.t0 = t0;
-  w.t1 = t1;
+  w.f = (R*)b.f;
+  w.w = (R*)b.w;
+  w.z = (R*)b.z;
+  w.t0 = (R*)t0;
+  w.t1 = (R*)t1;
   w.part = b.part;
   w.state = b.state;
-  w.qbatch = b.qbatch;
+  w.qbatch = (R*)b.qbatch;
   w.p_out = p_out;
   w.iters_out = iters_out;
   w.slow_gcv = slow;
@@ -219,6 +234,18 @@ std::vector<uint8_t> field_mask_of(const piv_pass_desc& p, const uint8_t* mask, 
     }
   }
   return fm;
+}
+
+// exp(-2 pi i k / F) for k < F/2 as (re, im) pairs, rounded from extended
+// precision.
+std::vector<double> fft_twiddles(int F) {
+  std::vector<double> t((size_t)F);
+  for (int k = 0; k < F / 2; ++k) {
+    const long double ang = -2.0L * 3.14159265358979323846264338327950288L * k / F;
+    t[2 * k] = (double)cosl(ang);
+    t[2 * k + 1] = (double)sinl(ang);
+  }
+  return t;
 }
 
 CorrArgs corr_args(const void* fa, const void* fb, int dtype, int ht, int wd, const piv_pass_desc& p) {
@@ -243,9 +270,10 @@ CorrArgs corr_args(const void* fa, const void* fb, int dtype, int ht, int wd, co
   return a;
 }
 
-DeformArgs deform_args(const void* fa, const void* fb, int dtype, int ht, int wd, const piv_pass_desc& p,
-                       const float* up, const float* vp, float* oa, float* ob) {
-  DeformArgs d{};
+template <typename R>
+DeformArgsT<R> deform_args(const void* fa, const void* fb, int dtype, int ht, int wd, const piv_pass_desc& p,
+                           const R* up, const R* vp, R* oa, R* ob) {
+  DeformArgsT<R> d{};
   d.frame_a = fa;
   d.frame_b = fb;
   d.in_dtype = dtype;
@@ -255,45 +283,46 @@ DeformArgs deform_args(const void* fa, const void* fb, int dtype, int ht, int wd
   d.vp = vp;
   d.m = p.field_m;
   d.n = p.field_n;
-  d.x0 = grid_x0(p);
-  d.y0 = grid_y0(p);
-  d.spacing = (float)p.spacing;
-  // gpu_process.py:1165-1171
-  const float dt = p.deforming_par;
-  if (dt > 0.f && dt < 1.f) {
+  d.x0 = (R)grid_x0(p);
+  d.y0 = (R)grid_y0(p);
+  d.spacing = (R)p.spacing;
+  // gpu_process.py:1165-1171 (dt - 1 formed in the value type)
+  const R dt = (R)p.deforming_par;
+  if (dt > R(0) && dt < R(1)) {
     d.warp_a = 1;
     d.warp_b = 1;
-    d.dt_a = dt - 1.f;
+    d.dt_a = dt - R(1);
     d.dt_b = dt;
-  } else if (dt == 1.f) {
+  } else if (dt == R(1)) {
     d.warp_a = 0;
     d.warp_b = 1;
-    d.dt_b = 1.f;
+    d.dt_b = R(1);
   } else {
     d.warp_a = 1;
     d.warp_b = 0;
-    d.dt_a = -1.f;
+    d.dt_a = R(-1);
   }
   d.out_a = oa;
   d.out_b = ob;
   return d;
 }
 
-GridInterpArgs interp_args(const piv_pass_desc& from, const piv_pass_desc& to, const float* ui, const float* vi,
-                           float* uo, float* vo) {
-  GridInterpArgs g{};
+template <typename R>
+GridInterpArgsT<R> interp_args(const piv_pass_desc& from, const piv_pass_desc& to, const R* ui, const R* vi, R* uo,
+                               R* vo) {
+  GridInterpArgsT<R> g{};
   g.u_in = ui;
   g.v_in = vi;
   g.m_in = from.field_m;
   g.n_in = from.field_n;
-  g.x0_in = grid_x0(from);
-  g.y0_in = grid_y0(from);
-  g.sp_in = (float)from.spacing;
+  g.x0_in = (R)grid_x0(from);
+  g.y0_in = (R)grid_y0(from);
+  g.sp_in = (R)from.spacing;
   g.m_out = to.field_m;
   g.n_out = to.field_n;
-  g.x0_out = grid_x0(to);
-  g.y0_out = grid_y0(to);
-  g.sp_out = (float)to.spacing;
+  g.x0_out = (R)grid_x0(to);
+  g.y0_out = (R)grid_y0(to);
+  g.sp_out = (R)to.spacing;
   g.u_out = uo;
   g.v_out = vo;
   return g;
@@ -301,6 +330,8 @@ GridInterpArgs interp_args(const piv_pass_desc& from, const piv_pass_desc& to, c
 
 }  // namespace
 
+// Field arrays hold the plan's value type: float, or double for FP64 plans
+// (the executor's templates cast them; FP32-only paths use them as float).
 struct PassBuffers {
   piv_pass_desc d{};
   int N = 0;
@@ -319,10 +350,10 @@ struct PassBuffers {
 
 // Work buffers of the validation / replacement loop (field-sized).
 struct LoopBufs {
-  float* u0 = nullptr;        // originals for the unresolved reset
-  float* v0 = nullptr;
-  float* tu = nullptr;        // inputs of a simultaneous (Jacobi) replacement
-  float* tv = nullptr;
+  void* u0 = nullptr;         // originals for the unresolved reset (value type)
+  void* v0 = nullptr;
+  void* tu = nullptr;         // inputs of a simultaneous (Jacobi) replacement
+  void* tv = nullptr;
   uint8_t* wf[2] = {nullptr, nullptr};  // flags after each revalidation (ping-pong)
   uint8_t* nanf = nullptr;    // unresolved vectors of the last replacement
   int32_t* it_counts = nullptr;  // PIV_MAX_REPLACING_ITERS counters
@@ -338,13 +369,17 @@ struct piv_plan {
   void* frame_a = nullptr;
   void* frame_b = nullptr;
   size_t frame_bytes = 0;
-  float* def_a = nullptr;
-  float* def_b = nullptr;
+  int fp64 = 0;                 // dtype_f = float64: every field and deformed frame in double
+  size_t esize = 4;             // bytes per value
+  void* def_a = nullptr;        // deformed frames (value type)
+  void* def_b = nullptr;
+  double* twiddle[8] = {};      // FP64 FFT twiddles per log2(F) (F = 16..128)
+  double2* corr_scratch = nullptr;  // FP64 planes of F = 128
   int32_t* counters = nullptr;  // 2 per pass
   double* p_out = nullptr;      // 1 per pass
   uint8_t* val_final = nullptr;
-  float* t0 = nullptr;
-  float* t1 = nullptr;
+  void* t0 = nullptr;           // smoothing transforms (value type)
+  void* t1 = nullptr;
   void* smooth_state = nullptr;
   ReplaceWork rw{};
   LoopBufs lb{};                // shares rw
@@ -378,39 +413,109 @@ namespace {
 const void* src_a(const piv_plan* P) { return P->ext_a ? P->ext_a : P->frame_a; }
 const void* src_b(const piv_plan* P) { return P->ext_b ? P->ext_b : P->frame_b; }
 
+template <typename R>
+R* as(void* p) {
+  return reinterpret_cast<R*>(p);
+}
+
+CorrArgs64 corr_args64_for(const void* fa, const void* fb, int dtype, int ht, int wd, const piv_pass_desc& p,
+                           const double* twiddle, double2* scratch) {
+  CorrArgs64 a{};
+  a.frame_a = fa;
+  a.frame_b = fb;
+  a.in_dtype = dtype;
+  a.ht = ht;
+  a.wd = wd;
+  a.ss = p.search_size;
+  a.fsize = p.fft_size;
+  a.spacing = p.spacing;
+  a.x_off = p.x_offset;
+  a.y_off = p.y_offset;
+  a.m = p.field_m;
+  a.n = p.field_n;
+  a.normalize = p.normalize;
+  a.subpixel = p.subpixel;
+  a.s2n_method = p.s2n_method;
+  a.s2n_size = p.s2n_size;
+  a.win_pad = p.window_size > 0 ? (p.search_size - p.window_size) / 2 : 0;
+  a.twiddle = twiddle;
+  a.scratch = scratch;
+  return a;
+}
+
+CorrArgs64 corr_args64(const piv_plan* P, const void* fa, const void* fb, int dtype, const piv_pass_desc& p) {
+  int lg = 0;
+  while ((1 << lg) < p.fft_size) ++lg;
+  return corr_args64_for(fa, fb, dtype, P->desc.height, P->desc.width, p, P->twiddle[lg], P->corr_scratch);
+}
+
 // Pass k > 0 (or the learned path's single pass): deform the original
 // frames by the total predictor B.up/B.vp (gpu_process.py:1113-1114, Q13)
 // and correlate the residual, adding the predictor back (:628-632).
+template <typename R>
 int deform_and_correlate(piv_plan* P, PassBuffers& B, hipStream_t s) {
   const piv_plan_desc& D = P->desc;
-  DeformArgs da = deform_args(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d, B.up, B.vp, P->def_a, P->def_b);
+  DeformArgsT<R> da = deform_args<R>(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d, as<R>(B.up),
+                                     as<R>(B.vp), as<R>(P->def_a), as<R>(P->def_b));
   da.mask = P->frame_mask;
   HIP_TRY(launch_deform(da, s));
-  CorrArgs a = corr_args(P->def_a, P->def_b, kF32, D.height, D.width, B.d);
-  a.up = B.up;
-  a.vp = B.vp;
-  a.u = B.u;
-  a.v = B.v;
-  a.s2n = B.s2n;
-  HIP_TRY(launch_correlate(a, s));
-  if (B.fmask) HIP_TRY(launch_apply_field_mask(B.u, B.v, B.s2n, B.fmask, B.N, s));
+  if constexpr (sizeof(R) == 8) {
+    CorrArgs64 a = corr_args64(P, P->def_a, P->def_b, kF64, B.d);
+    a.up = as<double>(B.up);
+    a.vp = as<double>(B.vp);
+    a.u = as<double>(B.u);
+    a.v = as<double>(B.v);
+    a.s2n = as<double>(B.s2n);
+    HIP_TRY(launch_correlate64(a, s));
+  } else {
+    CorrArgs a = corr_args(P->def_a, P->def_b, kF32, D.height, D.width, B.d);
+    a.up = B.up;
+    a.vp = B.vp;
+    a.u = B.u;
+    a.v = B.v;
+    a.s2n = B.s2n;
+    HIP_TRY(launch_correlate(a, s));
+  }
+  if (B.fmask) HIP_TRY(launch_apply_field_mask(as<R>(B.u), as<R>(B.v), as<R>(B.s2n), B.fmask, B.N, s));
   return PIV_OK;
 }
 
-ValidateArgs validate_args(const piv_pass_desc& d, const float* u, const float* v, int m, int n, const float* s2n,
-                           const double* stats, const uint8_t* only, const uint8_t* extra, uint8_t* flags,
-                           int32_t* count, const uint8_t* fmask = nullptr) {
-  ValidateArgs a{};
+// Pass 0: correlate the raw (masked) frames.
+template <typename R>
+int correlate_first(piv_plan* P, PassBuffers& B, hipStream_t s) {
+  const piv_plan_desc& D = P->desc;
+  if constexpr (sizeof(R) == 8) {
+    CorrArgs64 a = corr_args64(P, src_a(P), src_b(P), D.in_dtype, B.d);
+    a.u = as<double>(B.u);
+    a.v = as<double>(B.v);
+    a.s2n = as<double>(B.s2n);
+    HIP_TRY(launch_correlate64(a, s));
+  } else {
+    CorrArgs a = corr_args(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d);
+    a.u = B.u;
+    a.v = B.v;
+    a.s2n = B.s2n;
+    HIP_TRY(launch_correlate(a, s));
+  }
+  if (B.fmask) HIP_TRY(launch_apply_field_mask(as<R>(B.u), as<R>(B.v), as<R>(B.s2n), B.fmask, B.N, s));
+  return PIV_OK;
+}
+
+template <typename R>
+ValidateArgsT<R> validate_args(const piv_pass_desc& d, const R* u, const R* v, int m, int n, const R* s2n,
+                               const double* stats, const uint8_t* only, const uint8_t* extra, uint8_t* flags,
+                               int32_t* count, const uint8_t* fmask = nullptr) {
+  ValidateArgsT<R> a{};
   a.u = u;
   a.v = v;
   a.m = m;
   a.n = n;
   a.s2n = s2n;
-  a.s2n_tol = s2n ? d.s2n_tol : NAN;
-  a.median_tol = d.median_tol;
-  a.mad_tol = d.mad_tol;
-  a.mean_tol = d.mean_tol;
-  a.rms_tol = d.rms_tol;
+  a.s2n_tol = s2n ? (R)d.s2n_tol : (R)NAN;
+  a.median_tol = (R)d.median_tol;
+  a.mad_tol = (R)d.mad_tol;
+  a.mean_tol = (R)d.mean_tol;
+  a.rms_tol = (R)d.rms_tol;
   a.rms = stats;
   a.only = only;
   a.extra = extra;
@@ -421,9 +526,10 @@ ValidateArgs validate_args(const piv_pass_desc& d, const float* u, const float* 
 }
 
 // ValidationGPU.__call__ (gpu_validation.py:84-118): every enabled test.
-int run_validate(const piv_pass_desc& d, const float* u, const float* v, int m, int n, const float* s2n,
-                 double* stats, const uint8_t* only, const uint8_t* extra, uint8_t* flags, int32_t* count,
-                 hipStream_t s, const uint8_t* fmask = nullptr) {
+template <typename R>
+int run_validate(const piv_pass_desc& d, const R* u, const R* v, int m, int n, const R* s2n, double* stats,
+                 const uint8_t* only, const uint8_t* extra, uint8_t* flags, int32_t* count, hipStream_t s,
+                 const uint8_t* fmask = nullptr) {
   if (!std::isnan(d.rms_tol)) HIP_TRY(launch_field_stats(u, v, m * n, stats, s));
   HIP_TRY(launch_validate(validate_args(d, u, v, m, n, s2n, stats, only, extra, flags, count, fmask), s));
   return PIV_OK;
@@ -436,16 +542,21 @@ int run_validate(const piv_pass_desc& d, const float* u, const float* v, int m, 
 // vectors and re-tests only the replaced ones (:809-822); without it the
 // unresolved ones are reset at the end (:829-832).  it_counts must be zero.
 // No host synchronisation: an empty flag set turns every step into a no-op.
-int replace_loop(float* u, float* v, const uint8_t* flags0, int32_t* count0, int m, int n, const piv_pass_desc& d,
-                 const float* s2n, LoopBufs& L, hipStream_t s, const uint8_t* fmask = nullptr) {
+template <typename R>
+int replace_loop(R* u, R* v, const uint8_t* flags0, int32_t* count0, int m, int n, const piv_pass_desc& d,
+                 const R* s2n, LoopBufs& L, hipStream_t s, const uint8_t* fmask = nullptr) {
   const int N = m * n;
   const int iters = d.num_replacing_iters;
   if (iters <= 0) return PIV_OK;
   const bool spring = d.replacing_method == PIV_SPRING;
   const bool reval = d.revalidate != 0;
+  R* u0 = as<R>(L.u0);
+  R* v0 = as<R>(L.v0);
+  R* tu = as<R>(L.tu);
+  R* tv = as<R>(L.tv);
   if (!spring || reval) {
-    HIP_TRY(launch_copy_f32(L.u0, u, N, s));
-    HIP_TRY(launch_copy_f32(L.v0, v, N, s));
+    HIP_TRY(launch_copy(u0, (const R*)u, N, s));
+    HIP_TRY(launch_copy(v0, (const R*)v, N, s));
   }
   const uint8_t* cur = flags0;
   int32_t* cur_count = count0;
@@ -453,77 +564,84 @@ int replace_loop(float* u, float* v, const uint8_t* flags0, int32_t* count0, int
     if (spring) {
       HIP_TRY(launch_spring_replace(u, v, cur, cur_count, m, n, L.rw, s));
     } else {
-      HIP_TRY(launch_copy_f32(L.tu, u, N, s));
-      HIP_TRY(launch_copy_f32(L.tv, v, N, s));
-      HIP_TRY(launch_neighbour_replace(L.tu, L.tv, u, v, cur, m, n, d.replacing_method, (it == 0 || reval) ? 1 : 0,
-                                       s));
+      HIP_TRY(launch_copy(tu, (const R*)u, N, s));
+      HIP_TRY(launch_copy(tv, (const R*)v, N, s));
+      HIP_TRY(launch_neighbour_replace((const R*)tu, (const R*)tv, u, v, cur, m, n, d.replacing_method,
+                                       (it == 0 || reval) ? 1 : 0, s));
     }
     if (reval) {
-      HIP_TRY(launch_reset_unresolved(u, v, L.u0, L.v0, cur, L.nanf, N, s));
+      HIP_TRY(launch_reset_unresolved(u, v, (const R*)u0, (const R*)v0, cur, L.nanf, N, s));
       uint8_t* nxt = L.wf[it & 1];
       int32_t* nxt_count = L.it_counts + it;
-      int r = run_validate(d, u, v, m, n, s2n, L.stats, cur, L.nanf, nxt, nxt_count, s, fmask);
+      int r = run_validate<R>(d, u, v, m, n, s2n, L.stats, cur, L.nanf, nxt, nxt_count, s, fmask);
       if (r) return r;
       cur = nxt;
       cur_count = nxt_count;
     }
   }
-  if (!reval && !spring) HIP_TRY(launch_reset_unresolved(u, v, L.u0, L.v0, cur, nullptr, N, s));
+  if (!reval && !spring) HIP_TRY(launch_reset_unresolved(u, v, (const R*)u0, (const R*)v0, cur, nullptr, N, s));
   return PIV_OK;
 }
 
 // Validation, replacement, last-pass revalidation and smoothing of pass k
 // (gpu_process.py:635-639, 744-845).
+template <typename R>
 int validate_and_smooth(piv_plan* P, int k, bool last, hipStream_t s) {
   PassBuffers& B = P->passes[k];
   const int m = B.d.field_m, n = B.d.field_n;
+  R* u = as<R>(B.u);
+  R* v = as<R>(B.v);
+  const R* s2n = as<R>(B.s2n);
   if (B.d.validate) {
-    int r = run_validate(B.d, B.u, B.v, m, n, B.s2n, P->lb.stats, nullptr, nullptr, B.flags, P->counters + 2 * k, s,
-                         B.fmask);
-    if (!r) r = replace_loop(B.u, B.v, B.flags, P->counters + 2 * k, m, n, B.d, B.s2n, P->lb, s, B.fmask);
+    int r = run_validate<R>(B.d, u, v, m, n, s2n, P->lb.stats, nullptr, nullptr, B.flags, P->counters + 2 * k, s,
+                            B.fmask);
+    if (!r) r = replace_loop<R>(u, v, B.flags, P->counters + 2 * k, m, n, B.d, s2n, P->lb, s, B.fmask);
     if (!r && last)
-      r = run_validate(B.d, B.u, B.v, m, n, B.s2n, P->lb.stats, nullptr, nullptr, P->val_final,
-                       P->counters + 2 * k + 1, s, B.fmask);
+      r = run_validate<R>(B.d, u, v, m, n, s2n, P->lb.stats, nullptr, nullptr, P->val_final, P->counters + 2 * k + 1,
+                          s, B.fmask);
     if (r) return r;
   }
   if (!last && B.d.smooth && B.n_masked > 0) {
     const int autos = !(B.d.smoothing_par > 0);
-    SmoothnMaskedWork w = masked_work(B.tab, B.fmask, P->msb, P->t0, P->t1, P->p_out + k, P->smooth_iters + k,
-                                      B.slow_gcv);
-    HIP_TRY(launch_smoothn_masked(B.u, B.v, m, n, B.p_min, B.p_max, autos ? 0.0 : log10(B.d.smoothing_par), autos, w,
+    SmoothnMaskedWorkT<R> w = masked_work<R>(B.tab, B.fmask, P->msb, P->t0, P->t1, P->p_out + k,
+                                             P->smooth_iters + k, B.slow_gcv);
+    HIP_TRY(launch_smoothn_masked(u, v, m, n, B.p_min, B.p_max, autos ? 0.0 : log10(B.d.smoothing_par), autos, w,
                                   s));
   } else if (!last && B.d.smooth) {
-    SmoothnWork w{};
-    w.cos_m = B.tab.cos_m;
-    w.cos_n = B.tab.cos_n;
+    SmoothnWorkT<R> w{};
+    w.cos_m = (const R*)B.tab.cos_m;
+    w.cos_n = (const R*)B.tab.cos_n;
     w.lambda = B.tab.lambda;
-    w.t0 = P->t0;
-    w.t1 = P->t1;
+    w.t0 = as<R>(P->t0);
+    w.t1 = as<R>(P->t1);
     w.p_out = P->p_out + k;
     w.state = P->smooth_state;
     const int autos = !(B.d.smoothing_par > 0);
-    HIP_TRY(launch_smoothn(B.u, B.v, B.d.field_m, B.d.field_n, B.p_min, B.p_max,
+    HIP_TRY(launch_smoothn(u, v, B.d.field_m, B.d.field_n, B.p_min, B.p_max,
                            autos ? 0.0 : log10(B.d.smoothing_par), autos, w, s));
   }
   return PIV_OK;
 }
 
 // Pass k with the predictor from pass k-1 (gpu_process.py:620-625).
+template <typename R>
 int predictor_from_previous(piv_plan* P, int k, hipStream_t s) {
   PassBuffers& A = P->passes[k - 1];
   PassBuffers& B = P->passes[k];
   if (A.d.search_size != B.d.search_size) {
-    HIP_TRY(launch_grid_interp(interp_args(A.d, B.d, A.u, A.v, B.up, B.vp), s));
+    HIP_TRY(launch_grid_interp(
+        interp_args<R>(A.d, B.d, as<R>(A.u), as<R>(A.v), as<R>(B.up), as<R>(B.vp)), s));
   } else {
-    HIP_TRY(launch_copy_f32(B.up, A.u, B.N, s));
-    HIP_TRY(launch_copy_f32(B.vp, A.v, B.N, s));
+    HIP_TRY(launch_copy(as<R>(B.up), (const R*)as<R>(A.u), B.N, s));
+    HIP_TRY(launch_copy(as<R>(B.vp), (const R*)as<R>(A.v), B.N, s));
   }
   return PIV_OK;
 }
 
 // Passes first..np-1; pass `first` takes its predictor from B.up/B.vp as
 // already written (first > 0) or correlates the raw frames (first == 0).
-int enqueue_passes(piv_plan* P, int first) {
+template <typename R>
+int enqueue_passes_t(piv_plan* P, int first) {
   hipStream_t s = P->stream;
   const piv_plan_desc& D = P->desc;
   const int np = (int)P->passes.size();
@@ -532,24 +650,27 @@ int enqueue_passes(piv_plan* P, int first) {
     const bool last = k == np - 1;
     int r = PIV_OK;
     if (k == 0) {
-      CorrArgs a = corr_args(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d);
-      a.u = B.u;
-      a.v = B.v;
-      a.s2n = B.s2n;
-      HIP_TRY(launch_correlate(a, s));
-      if (B.fmask) HIP_TRY(launch_apply_field_mask(B.u, B.v, B.s2n, B.fmask, B.N, s));
+      r = correlate_first<R>(P, B, s);
     } else {
-      if (k > first) r = predictor_from_previous(P, k, s);
-      if (!r) r = deform_and_correlate(P, B, s);
+      if (k > first) r = predictor_from_previous<R>(P, k, s);
+      if (!r) r = deform_and_correlate<R>(P, B, s);
     }
-    if (!r) r = validate_and_smooth(P, k, last, s);
+    if (!r) r = validate_and_smooth<R>(P, k, last, s);
     if (r) return r;
   }
-  if (D.out_scale != 1.0f) {
-    PassBuffers& L = P->passes.back();
-    HIP_TRY(launch_scale(L.u, L.v, L.N, D.out_scale, s));
+  PassBuffers& L = P->passes.back();
+  if constexpr (sizeof(R) == 8) {
+    // (u * scaling_par) / dt in double (gpu_process.py:642-643, 847-849)
+    if (D.scaling_par != 1.0 || D.dt != 1.0)
+      HIP_TRY(launch_scale_div(as<double>(L.u), as<double>(L.v), L.N, D.scaling_par, D.dt, s));
+  } else if (D.out_scale != 1.0f) {
+    HIP_TRY(launch_scale(as<R>(L.u), as<R>(L.v), L.N, (R)D.out_scale, s));
   }
   return PIV_OK;
+}
+
+int enqueue_passes(piv_plan* P, int first) {
+  return P->fp64 ? enqueue_passes_t<double>(P, first) : enqueue_passes_t<float>(P, first);
 }
 
 int enqueue(piv_plan* P) {
@@ -635,8 +756,10 @@ void release(piv_plan* P) {
     if (p) (void)hipFree(p);
   for (void* p : {P->frame_a, P->frame_b, (void*)P->def_a, (void*)P->def_b, (void*)P->counters, (void*)P->p_out,
                   (void*)P->val_final, (void*)P->t0, (void*)P->t1, (void*)P->rw.slot, (void*)P->rw.list,
-                  (void*)P->rw.work, P->smooth_state})
+                  (void*)P->rw.work, P->smooth_state, (void*)P->corr_scratch})
     if (p) (void)hipFree(p);
+  for (double* t : P->twiddle)
+    if (t) (void)hipFree(t);
   if (P->inputs_ready) (void)hipEventDestroy(P->inputs_ready);
   if (P->stream) (void)hipStreamDestroy(P->stream);
   delete P;
@@ -661,7 +784,10 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
   const piv_plan_desc& D = *desc;
   if (D.n_passes < 1 || D.n_passes > PIV_MAX_PASSES) return fail(PIV_E_INVALID, "n_passes out of range");
   if (D.height < 2 || D.width < 2) return fail(PIV_E_INVALID, "frame too small");
-  if (D.in_dtype < PIV_U8 || D.in_dtype > PIV_F32) return fail(PIV_E_INVALID, "in_dtype");
+  if (D.in_dtype < PIV_U8 || D.in_dtype > PIV_F64) return fail(PIV_E_INVALID, "in_dtype");
+  if (D.precision != PIV_FP32 && D.precision != PIV_FP64) return fail(PIV_E_INVALID, "precision");
+  if (D.in_dtype == PIV_F64 && D.precision != PIV_FP64)
+    return fail(PIV_E_INVALID, "float64 frames need an FP64 plan (convert to float32 for FP32 processing)");
   for (int k = 0; k < D.n_passes; ++k) {
     int r = check_pass(D.passes[k], D.height, D.width);
     if (r) return r;
@@ -673,6 +799,9 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
   piv_plan* P = new piv_plan();
   P->device = device;
   P->desc = D;
+  P->fp64 = D.precision == PIV_FP64;
+  P->esize = P->fp64 ? 8 : 4;
+  const size_t es = P->esize;
   auto bail = [&](int code) {
     release(P);
     return code;
@@ -685,7 +814,7 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
   int maxN = 0;
   auto dmalloc = [&](void** p, size_t bytes) -> bool { return hipMalloc(p, bytes ? bytes : 16) == hipSuccess; };
   bool ok = dmalloc(&P->frame_a, P->frame_bytes) && dmalloc(&P->frame_b, P->frame_bytes);
-  if (D.n_passes > 1) ok = ok && dmalloc((void**)&P->def_a, npx * 4) && dmalloc((void**)&P->def_b, npx * 4);
+  if (D.n_passes > 1) ok = ok && dmalloc((void**)&P->def_a, npx * es) && dmalloc((void**)&P->def_b, npx * es);
   ok = ok && dmalloc((void**)&P->counters, 2 * D.n_passes * sizeof(int32_t)) &&
        dmalloc((void**)&P->p_out, D.n_passes * sizeof(double));
   if (!ok) return bail(fail(PIV_E_NOMEM, "device allocation failed"));
@@ -700,26 +829,37 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
     B.d = D.passes[k];
     B.N = B.d.field_m * B.d.field_n;
     if (B.N > maxN) maxN = B.N;
-    ok = dmalloc((void**)&B.u, B.N * 4) && dmalloc((void**)&B.v, B.N * 4) && dmalloc((void**)&B.up, B.N * 4) &&
-         dmalloc((void**)&B.vp, B.N * 4) && dmalloc((void**)&B.flags, B.N);
-    if (ok && needs_s2n(B.d)) ok = dmalloc((void**)&B.s2n, B.N * 4);
+    ok = dmalloc((void**)&B.u, B.N * es) && dmalloc((void**)&B.v, B.N * es) && dmalloc((void**)&B.up, B.N * es) &&
+         dmalloc((void**)&B.vp, B.N * es) && dmalloc((void**)&B.flags, B.N);
+    if (ok && needs_s2n(B.d)) ok = dmalloc((void**)&B.s2n, B.N * es);
     if (!ok) return bail(fail(PIV_E_NOMEM, "device allocation failed"));
     if (hipMemset(B.flags, 0, B.N) != hipSuccess) return bail(fail(PIV_E_HIP, "hipMemset failed"));
+    if (P->fp64) {
+      int lg = 0;
+      while ((1 << lg) < B.d.fft_size) ++lg;
+      if (!P->twiddle[lg]) {
+        int r = upload((void**)&P->twiddle[lg], fft_twiddles(B.d.fft_size));
+        if (r) return bail(r);
+      }
+      const size_t sb = correlate64_scratch_bytes(B.d.fft_size);
+      if (sb && !P->corr_scratch && hipMalloc((void**)&P->corr_scratch, sb) != hipSuccess)
+        return bail(fail(PIV_E_NOMEM, "device allocation failed"));
+    }
     if (k < D.n_passes - 1 && B.d.smooth) {
-      int r = make_tables(B.d.field_m, B.d.field_n, &B.tab);
+      int r = make_tables(B.d.field_m, B.d.field_n, &B.tab, P->fp64 != 0);
       if (r) return bail(r);
       p_bounds(B.d.field_m, B.d.field_n, &B.p_min, &B.p_max);
     }
   }
   const PassBuffers& L = P->passes.back();
-  ok = dmalloc((void**)&P->val_final, L.N) && dmalloc((void**)&P->t0, 2ull * maxN * 4) &&
-       dmalloc((void**)&P->t1, 2ull * maxN * 4) && dmalloc((void**)&P->rw.slot, maxN * 4) &&
+  ok = dmalloc((void**)&P->val_final, L.N) && dmalloc((void**)&P->t0, 2ull * maxN * es) &&
+       dmalloc((void**)&P->t1, 2ull * maxN * es) && dmalloc((void**)&P->rw.slot, maxN * 4) &&
        dmalloc((void**)&P->rw.list, maxN * 4) && dmalloc((void**)&P->rw.work, 8ull * maxN * 8);
   ok = ok && dmalloc(&P->smooth_state, smoothn_state_bytes());
   {
     LoopBufs& lb = P->lb;
-    ok = ok && dmalloc((void**)&lb.u0, maxN * 4) && dmalloc((void**)&lb.v0, maxN * 4) &&
-         dmalloc((void**)&lb.tu, maxN * 4) && dmalloc((void**)&lb.tv, maxN * 4) && dmalloc((void**)&lb.wf[0], maxN) &&
+    ok = ok && dmalloc((void**)&lb.u0, maxN * es) && dmalloc((void**)&lb.v0, maxN * es) &&
+         dmalloc((void**)&lb.tu, maxN * es) && dmalloc((void**)&lb.tv, maxN * es) && dmalloc((void**)&lb.wf[0], maxN) &&
          dmalloc((void**)&lb.wf[1], maxN) && dmalloc((void**)&lb.nanf, maxN) &&
          dmalloc((void**)&lb.it_counts, PIV_MAX_REPLACING_ITERS * 4) && dmalloc((void**)&lb.stats, 4 * 8);
     lb.rw = P->rw;
@@ -733,7 +873,7 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
 
 void piv_plan_destroy(piv_plan* plan) { release(plan); }
 
-int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on_device, float* u, float* v,
+int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on_device, void* u, void* v,
             uint8_t* val_init, uint8_t* val_final, int outputs_on_device) {
   if (!P || !frame_a || !frame_b) return fail(PIV_E_INVALID, "null argument");
   HIP_TRY(hipSetDevice(P->device));
@@ -775,8 +915,8 @@ int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on
   }
   const PassBuffers& L = P->passes.back();
   const hipMemcpyKind out_kind = outputs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-  if (u) HIP_TRY(hipMemcpyAsync(u, L.u, L.N * sizeof(float), out_kind, s));
-  if (v) HIP_TRY(hipMemcpyAsync(v, L.v, L.N * sizeof(float), out_kind, s));
+  if (u) HIP_TRY(hipMemcpyAsync(u, L.u, L.N * P->esize, out_kind, s));
+  if (v) HIP_TRY(hipMemcpyAsync(v, L.v, L.N * P->esize, out_kind, s));
   if (val_init) HIP_TRY(hipMemcpyAsync(val_init, L.flags, L.N, out_kind, s));
   if (val_final) HIP_TRY(hipMemcpyAsync(val_final, P->val_final, L.N, out_kind, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -795,6 +935,7 @@ int piv_plan_set_predictor(piv_plan* P, piv_cnn* net, int first_pass, float max_
   if (!(max_reject_fraction >= 0.f && max_reject_fraction <= 1.f))
     return fail(PIV_E_INVALID, "max_reject_fraction must lie in [0, 1]");
   if (P->frame_mask) return fail(PIV_E_INVALID, "a learned predictor cannot be combined with a frame mask");
+  if (P->fp64) return fail(PIV_E_INVALID, "the learned predictor runs FP32 plans only");
   const piv_plan_desc& D = P->desc;
   const CnnInfo ni = cnn_info(net);
   if (D.height != ni.in_h * kCnnFactor || D.width != ni.in_w * kCnnFactor)
@@ -871,7 +1012,7 @@ int piv_plan_set_mask(piv_plan* P, const uint8_t* mask, int height, int width) {
   for (size_t q = 0; q < npx; ++q) fmk[q] = mask[q] ? 1 : 0;
   HIP_TRY(hipMemcpy(P->frame_mask, fmk.data(), npx, hipMemcpyHostToDevice));
   if (maxN > 0) {
-    int r = alloc_masked_smooth(maxN, maxM, slow, &P->msb);
+    int r = alloc_masked_smooth(maxN, maxM, slow, &P->msb, P->esize);
     if (r) return r;
   }
   if (!P->smooth_iters) {
@@ -1017,7 +1158,7 @@ int piv_stage_replace(float* u, float* v, uint8_t* flags, int m, int n, const pi
   if (e == hipSuccess) {
     piv_pass_desc none = *pass;
     none.median_tol = none.mad_tol = none.mean_tol = none.rms_tol = none.s2n_tol = NAN;
-    e = launch_validate(validate_args(none, u, v, m, n, nullptr, nullptr, nullptr, flags, L.wf[1], count0), s);
+    e = launch_validate(validate_args<float>(none, u, v, m, n, nullptr, nullptr, nullptr, flags, L.wf[1], count0), s);
   }
   if (e != hipSuccess) {
     cleanup();
@@ -1071,7 +1212,7 @@ int piv_stage_smoothn(float* u, float* v, int m, int n, double smoothing_par, do
   HIP_TRY(hipMemset(state, 0, smoothn_state_bytes()));
   double pmin, pmax;
   p_bounds(m, n, &pmin, &pmax);
-  SmoothnWork w{tab.cos_m, tab.cos_n, tab.lambda, t0, t1, log10_s_out, state};
+  SmoothnWork w{(const float*)tab.cos_m, (const float*)tab.cos_n, tab.lambda, t0, t1, log10_s_out, state};
   const int autos = !(smoothing_par > 0);
   hipError_t e = launch_smoothn(u, v, m, n, pmin, pmax, autos ? 0.0 : log10(smoothing_par), autos, w, s);
   hipError_t e2 = hipStreamSynchronize(s);
@@ -1081,6 +1222,41 @@ int piv_stage_smoothn(float* u, float* v, int m, int n, double smoothing_par, do
   free_tables(&tab);
   if (e != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e));
   if (e2 != hipSuccess) return fail(PIV_E_HIP, "smoothn: %s", hipGetErrorString(e2));
+  return PIV_OK;
+}
+
+int piv_stage_correlate64(const void* frame_a, const void* frame_b, int in_dtype, int height, int width,
+                          const piv_pass_desc* pass, const double* up, const double* vp, double* u, double* v,
+                          int32_t* i_peak, int32_t* j_peak, double* planes, double* s2n, void* stream) {
+  if (!frame_a || !frame_b || !pass || !u || !v) return fail(PIV_E_INVALID, "null argument");
+  if (in_dtype < PIV_U8 || in_dtype > PIV_F64) return fail(PIV_E_INVALID, "in_dtype");
+  int r = check_pass(*pass, height, width);
+  if (r) return r;
+  hipStream_t s = (hipStream_t)stream;
+  double* tw = nullptr;
+  double2* scratch = nullptr;
+  r = upload((void**)&tw, fft_twiddles(pass->fft_size));
+  const size_t sb = correlate64_scratch_bytes(pass->fft_size);
+  if (!r && sb && hipMalloc((void**)&scratch, sb) != hipSuccess) r = fail(PIV_E_NOMEM, "device allocation failed");
+  hipError_t e = hipSuccess, e2 = hipSuccess;
+  if (!r) {
+    CorrArgs64 a = corr_args64_for(frame_a, frame_b, in_dtype, height, width, *pass, tw, scratch);
+    a.up = up;
+    a.vp = vp;
+    a.u = u;
+    a.v = v;
+    a.i_peak = i_peak;
+    a.j_peak = j_peak;
+    a.planes = planes;
+    a.s2n = s2n;
+    e = launch_correlate64(a, s);
+    e2 = hipStreamSynchronize(s);
+  }
+  if (tw) (void)hipFree(tw);
+  if (scratch) (void)hipFree(scratch);
+  if (r) return r;
+  if (e != hipSuccess) return fail(PIV_E_HIP, "correlate64: %s [%s]", hipGetErrorString(e), launch_detail());
+  if (e2 != hipSuccess) return fail(PIV_E_HIP, "correlate64: %s", hipGetErrorString(e2));
   return PIV_OK;
 }
 
@@ -1115,11 +1291,11 @@ int piv_stage_smoothn_masked(float* u, float* v, const uint8_t* mask, int m, int
     double pmin, pmax;
     p_bounds(m, n, &pmin, &pmax);
     const int autos = !(smoothing_par > 0);
-    const SmoothnMaskedWork w = masked_work(tab, mask, b, t0, t1, log10_s_out, iterations_out, slow);
+    const SmoothnMaskedWork w = masked_work<float>(tab, mask, b, t0, t1, log10_s_out, iterations_out, slow);
     if (n_w == N) {
       // nothing masked or missing: the unweighted single pass
       void* state = b.state;
-      SmoothnWork uw{tab.cos_m, tab.cos_n, tab.lambda, t0, t1, log10_s_out, state};
+      SmoothnWork uw{(const float*)tab.cos_m, (const float*)tab.cos_n, tab.lambda, t0, t1, log10_s_out, state};
       e = launch_smoothn(u, v, m, n, pmin, pmax, autos ? 0.0 : log10(smoothing_par), autos, uw, s);
       if (e == hipSuccess && iterations_out) e = hipMemsetAsync(iterations_out, 0, 4, s);
     } else {
@@ -1416,8 +1592,8 @@ int batch_build_tables(piv_batch* b) {
         piv_plan* P = b->plans[p];
         const PassBuffers& B = P->passes[l];
         const piv_plan_desc& D = P->desc;
-        dfs.push_back(deform_args(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d, B.up, B.vp, P->def_a,
-                                  P->def_b));
+        dfs.push_back(deform_args<float>(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d, B.up, B.vp,
+                                         as<float>(P->def_a), as<float>(P->def_b)));
         rows.push_back(L.total_rows);
         L.total_rows += D.height;
         L.max_wd = std::max(L.max_wd, D.width);
@@ -1459,7 +1635,7 @@ int batch_enqueue(piv_batch* b) {
     BatchLevel& L = b->levels[l];
     if (l > 0) {
       for (int p : L.plans) {
-        int r = predictor_from_previous(b->plans[p], (int)l, s);
+        int r = predictor_from_previous<float>(b->plans[p], (int)l, s);
         if (r) return r;
       }
       int r = batch_enqueue_deform(b, L, s);
@@ -1471,7 +1647,7 @@ int batch_enqueue(piv_batch* b) {
     }
     for (int p : L.plans) {
       piv_plan* P = b->plans[p];
-      int r = validate_and_smooth(P, (int)l, l + 1 == P->passes.size(), s);
+      int r = validate_and_smooth<float>(P, (int)l, l + 1 == P->passes.size(), s);
       if (r) return r;
     }
   }
@@ -1491,6 +1667,8 @@ extern "C" {
 int piv_batch_create(int device, int n_pairs, const piv_plan_desc* descs, piv_batch** out) {
   if (!out || !descs || n_pairs < 1) return fail(PIV_E_INVALID, "batch: bad argument");
   *out = nullptr;
+  for (int i = 0; i < n_pairs; ++i)
+    if (descs[i].precision != PIV_FP32) return fail(PIV_E_INVALID, "batch: FP32 plans only");
   if (piv_device_count() <= 0) return fail(PIV_E_NO_DEVICE, "no HIP device visible");
   HIP_TRY(hipSetDevice(device));
   piv_batch* b = new piv_batch();

@@ -14,21 +14,41 @@ template <typename T>
 __device__ __forceinline__ float ld(const void* base, size_t idx) {
   return (float)(reinterpret_cast<const T*>(base)[idx]);
 }
+template <typename R, typename T>
+__device__ __forceinline__ R ldr(const void* base, size_t idx) {
+  return (R)(reinterpret_cast<const T*>(base)[idx]);
+}
+
+// Precision-generic scalar helpers (float and double).
+__device__ __forceinline__ float vfma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double vfma(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float vfloor(float a) { return floorf(a); }
+__device__ __forceinline__ double vfloor(double a) { return floor(a); }
+__device__ __forceinline__ float vabs(float a) { return fabsf(a); }
+__device__ __forceinline__ double vabs(double a) { return fabs(a); }
+__device__ __forceinline__ float vmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double vmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float vmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double vmax(double a, double b) { return fmax(a, b); }
+template <typename R> __device__ __forceinline__ R vnan();
+template <> __device__ __forceinline__ float vnan<float>() { return __builtin_nanf(""); }
+template <> __device__ __forceinline__ double vnan<double>() { return __builtin_nan(""); }
 
 // Bilinear value of a regular-grid field at (yq, xq) with linear
 // extrapolation outside the grid: the interval index is clamped to
 // [0, n-2] while the normalised distance is not (cupyx/scipy
 // RegularGridInterpolator(method="linear", fill_value=None) semantics used at
 // gpu_process.py:737-740 and :1159-1162).
-__device__ __forceinline__ void grid_bilinear(const float* fu, const float* fv, int m, int n, float y0, float x0,
-                                              float sp, float yq, float xq, float& ou, float& ov) {
-  int iy = (int)floorf((yq - y0) / sp);
-  int ix = (int)floorf((xq - x0) / sp);
+template <typename R>
+__device__ __forceinline__ void grid_bilinear(const R* fu, const R* fv, int m, int n, R y0, R x0, R sp, R yq, R xq,
+                                              R& ou, R& ov) {
+  int iy = (int)vfloor((yq - y0) / sp);
+  int ix = (int)vfloor((xq - x0) / sp);
   iy = iy < 0 ? 0 : (iy > m - 2 ? m - 2 : iy);
   ix = ix < 0 ? 0 : (ix > n - 2 ? n - 2 : ix);
-  const float ty = (yq - (y0 + (float)iy * sp)) / sp;
-  const float tx = (xq - (x0 + (float)ix * sp)) / sp;
-  const float wy0 = 1.f - ty, wx0 = 1.f - tx;
+  const R ty = (yq - (y0 + (R)iy * sp)) / sp;
+  const R tx = (xq - (x0 + (R)ix * sp)) / sp;
+  const R wy0 = R(1) - ty, wx0 = R(1) - tx;
   const size_t i00 = (size_t)iy * n + ix;
   const size_t i10 = i00 + n;
   ou = fu[i00] * wy0 * wx0 + fu[i00 + 1] * wy0 * tx + fu[i10] * ty * wx0 + fu[i10 + 1] * ty * tx;
@@ -37,49 +57,48 @@ __device__ __forceinline__ void grid_bilinear(const float* fu, const float* fv, 
 
 // cuda_interpolate_frame (gpu_process.py:1528-1546): clamped base index,
 // unclamped fraction, so the border extrapolates linearly.
-template <typename T>
-__device__ __forceinline__ float warp_px(const void* f, int ht, int wd, float X, float Y, float up, float vp,
-                                         float dt) {
-  const float xi = fmaf(up, dt, X);
-  const float yi = fmaf(vp, dt, Y);
-  int xl = (int)floorf(xi);
-  int yu = (int)floorf(yi);
+template <typename R, typename T>
+__device__ __forceinline__ R warp_px(const void* f, int ht, int wd, R X, R Y, R up, R vp, R dt) {
+  const R xi = vfma(up, dt, X);
+  const R yi = vfma(vp, dt, Y);
+  int xl = (int)vfloor(xi);
+  int yu = (int)vfloor(yi);
   yu = yu < 0 ? 0 : (yu > ht - 2 ? ht - 2 : yu);
   xl = xl < 0 ? 0 : (xl > wd - 2 ? wd - 2 : xl);
   const int yd = yu + 1, xr = xl + 1;
-  const float ful = ld<T>(f, (size_t)yu * wd + xl);
-  const float fur = ld<T>(f, (size_t)yu * wd + xr);
-  const float fdl = ld<T>(f, (size_t)yd * wd + xl);
-  const float fdr = ld<T>(f, (size_t)yd * wd + xr);
-  const float a0 = yi - (float)yu, a1 = (float)yd - yi;
-  const float b0 = (float)xr - xi, b1 = xi - (float)xl;
+  const R ful = ldr<R, T>(f, (size_t)yu * wd + xl);
+  const R fur = ldr<R, T>(f, (size_t)yu * wd + xr);
+  const R fdl = ldr<R, T>(f, (size_t)yd * wd + xl);
+  const R fdr = ldr<R, T>(f, (size_t)yd * wd + xr);
+  const R a0 = yi - (R)yu, a1 = (R)yd - yi;
+  const R b0 = (R)xr - xi, b1 = xi - (R)xl;
   return a0 * b0 * fdl + a0 * b1 * fdr + a1 * b0 * ful + a1 * b1 * fur;
 }
 
-template <typename T>
-__device__ __forceinline__ void deform_px(const DeformArgs& a, int X, int Y) {
+template <typename R, typename T>
+__device__ __forceinline__ void deform_px(const DeformArgsT<R>& a, int X, int Y) {
   const size_t idx = (size_t)Y * a.wd + X;
   if (a.mask && a.mask[idx]) {
     // cuda_interpolate_masked_frame (gpu_process.py:1461-1506) leaves
     // masked pixels at 0; an unwarped frame is already 0 there
-    a.out_a[idx] = 0.f;
-    a.out_b[idx] = 0.f;
+    a.out_a[idx] = R(0);
+    a.out_b[idx] = R(0);
     return;
   }
-  float up, vp;
-  grid_bilinear(a.up, a.vp, a.m, a.n, a.y0, a.x0, a.spacing, (float)Y, (float)X, up, vp);
-  a.out_a[idx] = a.warp_a ? warp_px<T>(a.frame_a, a.ht, a.wd, (float)X, (float)Y, up, vp, a.dt_a)
-                          : ld<T>(a.frame_a, idx);
-  a.out_b[idx] = a.warp_b ? warp_px<T>(a.frame_b, a.ht, a.wd, (float)X, (float)Y, up, vp, a.dt_b)
-                          : ld<T>(a.frame_b, idx);
+  R up, vp;
+  grid_bilinear<R>(a.up, a.vp, a.m, a.n, a.y0, a.x0, a.spacing, (R)Y, (R)X, up, vp);
+  a.out_a[idx] = a.warp_a ? warp_px<R, T>(a.frame_a, a.ht, a.wd, (R)X, (R)Y, up, vp, a.dt_a)
+                          : ldr<R, T>(a.frame_a, idx);
+  a.out_b[idx] = a.warp_b ? warp_px<R, T>(a.frame_b, a.ht, a.wd, (R)X, (R)Y, up, vp, a.dt_b)
+                          : ldr<R, T>(a.frame_b, idx);
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void deform_kernel(DeformArgs a) {
+template <typename R, typename T>
+__global__ __launch_bounds__(256) void deform_kernel(DeformArgsT<R> a) {
   const int X = blockIdx.x * blockDim.x + threadIdx.x;
   const int Y = blockIdx.y;
   if (X >= a.wd) return;
-  deform_px<T>(a, X, Y);
+  deform_px<R, T>(a, X, Y);
 }
 
 // Many frames in one launch: grid.y runs over the concatenated rows.
@@ -96,28 +115,31 @@ __global__ __launch_bounds__(512) void deform_batch_kernel(const DeformArgs* fie
   const DeformArgs a = fields[lo];
   const int X = blockIdx.x * blockDim.x + threadIdx.x;
   if (X >= a.wd) return;
-  deform_px<T>(a, X, r - row_prefix[lo]);
+  deform_px<float, T>(a, X, r - row_prefix[lo]);
 }
 
-__global__ __launch_bounds__(256) void grid_interp_kernel(GridInterpArgs a) {
+template <typename R>
+__global__ __launch_bounds__(256) void grid_interp_kernel(GridInterpArgsT<R> a) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.m_out * a.n_out) return;
   const int i = k / a.n_out, j = k % a.n_out;
-  const float yq = a.y0_out + (float)i * a.sp_out;
-  const float xq = a.x0_out + (float)j * a.sp_out;
-  float ou, ov;
-  grid_bilinear(a.u_in, a.v_in, a.m_in, a.n_in, a.y0_in, a.x0_in, a.sp_in, yq, xq, ou, ov);
+  const R yq = a.y0_out + (R)i * a.sp_out;
+  const R xq = a.x0_out + (R)j * a.sp_out;
+  R ou, ov;
+  grid_bilinear<R>(a.u_in, a.v_in, a.m_in, a.n_in, a.y0_in, a.x0_in, a.sp_in, yq, xq, ou, ov);
   a.u_out[k] = ou;
   a.v_out[k] = ov;
 }
 
 // Median of the valid entries of a[0..cnt) after sorting (cnt <= 8).
-__device__ __forceinline__ void cswap(float& x, float& y) {
-  const float lo = fminf(x, y), hi = fmaxf(x, y);
+template <typename R>
+__device__ __forceinline__ void cswap(R& x, R& y) {
+  const R lo = vmin(x, y), hi = vmax(x, y);
   x = lo;
   y = hi;
 }
-__device__ __forceinline__ float median8(float (&s)[8], int cnt) {
+template <typename R>
+__device__ __forceinline__ R median8(R (&s)[8], int cnt) {
   // Batcher odd-even merge sort network for 8 keys (19 comparators); missing
   // entries are +inf and sort to the end.
   cswap(s[0], s[1]); cswap(s[2], s[3]); cswap(s[4], s[5]); cswap(s[6], s[7]);
@@ -126,17 +148,18 @@ __device__ __forceinline__ float median8(float (&s)[8], int cnt) {
   cswap(s[0], s[4]); cswap(s[1], s[5]); cswap(s[2], s[6]); cswap(s[3], s[7]);
   cswap(s[2], s[4]); cswap(s[3], s[5]);
   cswap(s[1], s[2]); cswap(s[3], s[4]); cswap(s[5], s[6]);
-  if (cnt <= 0) return NAN;
-  float r = 0.f;
+  if (cnt <= 0) return vnan<R>();
+  R r = R(0);
+  const R h = R(0.5);
   switch (cnt) {
     case 1: r = s[0]; break;
-    case 2: r = 0.5f * (s[0] + s[1]); break;
+    case 2: r = h * (s[0] + s[1]); break;
     case 3: r = s[1]; break;
-    case 4: r = 0.5f * (s[1] + s[2]); break;
+    case 4: r = h * (s[1] + s[2]); break;
     case 5: r = s[2]; break;
-    case 6: r = 0.5f * (s[2] + s[3]); break;
+    case 6: r = h * (s[2] + s[3]); break;
     case 7: r = s[3]; break;
-    default: r = 0.5f * (s[3] + s[4]); break;
+    default: r = h * (s[3] + s[4]); break;
   }
   return r;
 }
@@ -256,7 +279,8 @@ __device__ __forceinline__ void sum256(double (&v)[K], double (*scratch)[4]) {
 // decoupled, and the vector keeps its value (the reset of unresolved
 // vectors, gpu_process.py:829-832): its right-hand side is 0 here and
 // nan_isolated() tells the final write to skip it.
-__device__ __forceinline__ void spring_rhs(const float* u, const float* v, const int32_t* slot, int m, int n,
+template <typename R>
+__device__ __forceinline__ void spring_rhs(const R* u, const R* v, const int32_t* slot, int m, int n,
                                            const int (&ni)[4], const int (&nj)[4], double* bu, double* bv,
                                            int* nk) {
   double su = 0.0, sv = 0.0;
@@ -270,7 +294,7 @@ __device__ __forceinline__ void spring_rhs(const float* u, const float* v, const
       ++cu;
       ++cv;
     } else {
-      const float a = u[l], b = v[l];
+      const R a = u[l], b = v[l];
       if (!isnan(a)) {
         su += (double)a;
         ++cu;
@@ -286,7 +310,8 @@ __device__ __forceinline__ void spring_rhs(const float* u, const float* v, const
   *nk = cnt;
 }
 
-__device__ __forceinline__ void nan_isolated(const float* u, const float* v, const int32_t* slot, int m, int n,
+template <typename R>
+__device__ __forceinline__ void nan_isolated(const R* u, const R* v, const int32_t* slot, int m, int n,
                                              int k, bool* ku, bool* kv) {
   const int i = k / n, j = k % n;
   const int ni[4] = {i, i, i + 1, i - 1};
@@ -308,7 +333,8 @@ __device__ __forceinline__ void nan_isolated(const float* u, const float* v, con
   *kv = cv == 0;
 }
 
-__device__ void spring_cg_global(float* u, float* v, int m, int n, int nv, const ReplaceWork& w,
+template <typename R>
+__device__ void spring_cg_global(R* u, R* v, int m, int n, int nv, const ReplaceWork& w,
                                  double (*red)[4]) {
   constexpr int T = 256;
   const int tid = threadIdx.x;
@@ -415,8 +441,8 @@ __device__ void spring_cg_global(float* u, float* v, int m, int n, int nv, const
     const int k = w.list[s];
     bool ku, kv;
     nan_isolated(u, v, w.slot, m, n, k, &ku, &kv);
-    if (!ku) u[k] = (float)xu[s];
-    if (!kv) v[k] = (float)xv[s];
+    if (!ku) u[k] = (R)xu[s];
+    if (!kv) v[k] = (R)xv[s];
   }
 }
 
@@ -545,7 +571,8 @@ __device__ void spring_cg(SpringLds& L, int nv) {
   sync();
 }
 
-__global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(float* u, float* v, const uint8_t* flags,
+template <typename R>
+__global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* v, const uint8_t* flags,
                                                                        const int32_t* count, int m, int n,
                                                                        ReplaceWork w) {
   constexpr int T = kSpringLdsThreads;
@@ -644,8 +671,8 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(float* u,
     const int k = w.list[s];
     bool ku, kv;
     nan_isolated(u, v, w.slot, m, n, k, &ku, &kv);
-    if (!ku) u[k] = (float)L.xu[s];
-    if (!kv) v[k] = (float)L.xv[s];
+    if (!ku) u[k] = (R)L.xu[s];
+    if (!kv) v[k] = (R)L.xv[s];
   }
 }
 
@@ -653,14 +680,23 @@ __global__ void fill_i32_kernel(int32_t* p, int32_t value, int count) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < count) p[k] = value;
 }
-__global__ void scale_kernel(float* u, float* v, int count, float factor) {
+template <typename R>
+__global__ void scale_kernel(R* u, R* v, int count, R factor) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < count) {
     u[k] *= factor;
     v[k] *= factor;
   }
 }
-__global__ void copy_f32_kernel(float* dst, const float* src, int count) {
+__global__ void scale_div_kernel(double* u, double* v, int count, double mul, double div) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count) {
+    u[k] = u[k] * mul / div;
+    v[k] = v[k] * mul / div;
+  }
+}
+template <typename R>
+__global__ void copy_kernel(R* dst, const R* src, int count) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < count) dst[k] = src[k];
 }
@@ -697,13 +733,13 @@ __global__ __launch_bounds__(256) void mask_frames_kernel(T* a, T* b, const uint
 
 // Masked windows (CorrelationGPU with field_mask, gpu_process.py:1296-1310,
 // 1338-1343): displacement NaN, signal-to-noise 0.
-__global__ __launch_bounds__(256) void apply_field_mask_kernel(float* u, float* v, float* s2n, const uint8_t* fmask,
-                                                               int count) {
+template <typename R>
+__global__ __launch_bounds__(256) void apply_field_mask_kernel(R* u, R* v, R* s2n, const uint8_t* fmask, int count) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < count && fmask[k]) {
-    u[k] = NAN;
-    v[k] = NAN;
-    if (s2n) s2n[k] = 0.f;
+    u[k] = vnan<R>();
+    v[k] = vnan<R>();
+    if (s2n) s2n[k] = R(0);
   }
 }
 
@@ -717,10 +753,11 @@ thread_local char g_detail[256] = "";
 
 // Gathers the up-to-8 in-bounds, non-NaN 3x3 neighbours of (i, j) of f
 // (optionally leaving out flagged ones) into s[0..cnt), +inf after.
-__device__ __forceinline__ int gather8(const float* f, int m, int n, int i, int j, const uint8_t* skip, float (&s)[8]) {
+template <typename R>
+__device__ __forceinline__ int gather8(const R* f, int m, int n, int i, int j, const uint8_t* skip, R (&s)[8]) {
   int c = 0;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) s[q] = INFINITY;
+  for (int q = 0; q < 8; ++q) s[q] = (R)INFINITY;
 #pragma unroll
   for (int di = -1; di <= 1; ++di) {
 #pragma unroll
@@ -730,7 +767,7 @@ __device__ __forceinline__ int gather8(const float* f, int m, int n, int i, int 
       if (ii < 0 || ii >= m || jj < 0 || jj >= n) continue;
       const int idx = ii * n + jj;
       if (skip && skip[idx]) continue;
-      const float a = f[idx];
+      const R a = f[idx];
       if (isnan(a)) continue;
 #pragma unroll
       for (int q = 0; q < 8; ++q) s[q] = (q == c) ? a : s[q];
@@ -741,43 +778,45 @@ __device__ __forceinline__ int gather8(const float* f, int m, int n, int i, int 
 }
 
 // Median, MAD and mean of the gathered neighbours (NaN when there are none).
-__device__ __forceinline__ void stats8(float (&s)[8], int cnt, float* med, float* mad, float* mean) {
-  float sum = 0.f;
+template <typename R>
+__device__ __forceinline__ void stats8(R (&s)[8], int cnt, R* med, R* mad, R* mean) {
+  R sum = R(0);
 #pragma unroll
-  for (int q = 0; q < 8; ++q) sum += (q < cnt) ? s[q] : 0.f;
-  *mean = cnt > 0 ? sum / (float)cnt : NAN;
-  const float md = median8(s, cnt);
+  for (int q = 0; q < 8; ++q) sum += (q < cnt) ? s[q] : R(0);
+  *mean = cnt > 0 ? sum / (R)cnt : vnan<R>();
+  const R md = median8(s, cnt);
   *med = md;
   if (mad) {
-    float d[8];
+    R d[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) d[q] = (q < cnt) ? fabsf(s[q] - md) : INFINITY;
+    for (int q = 0; q < 8; ++q) d[q] = (q < cnt) ? vabs(s[q] - md) : (R)INFINITY;
     *mad = median8(d, cnt);
   }
 }
 
-__global__ __launch_bounds__(256) void validate_kernel(ValidateArgs a) {
+template <typename R>
+__global__ __launch_bounds__(256) void validate_kernel(ValidateArgsT<R> a) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   bool flag = false;
   if (k < a.m * a.n) {
     if ((!a.only || a.only[k]) && !(a.fmask && a.fmask[k])) {
       const int i = k / a.n, j = k % a.n;
-      const float uk = a.u[k], vk = a.v[k];
+      const R uk = a.u[k], vk = a.v[k];
       if (a.s2n && !isnan(a.s2n_tol)) flag = flag || (a.s2n[k] < a.s2n_tol);
       const bool nb = !isnan(a.median_tol) || !isnan(a.mad_tol) || !isnan(a.mean_tol);
       if (nb) {
-        float s[8];
-        float medu, madu, meanu, medv, madv, meanv;
+        R s[8];
+        R medu, madu, meanu, medv, madv, meanv;
         int c = gather8(a.u, a.m, a.n, i, j, nullptr, s);
         stats8(s, c, &medu, isnan(a.mad_tol) ? nullptr : &madu, &meanu);
         c = gather8(a.v, a.m, a.n, i, j, nullptr, s);
         stats8(s, c, &medv, isnan(a.mad_tol) ? nullptr : &madv, &meanv);
         if (!isnan(a.median_tol))
-          flag = flag || (fabsf(uk - medu) > a.median_tol) || (fabsf(vk - medv) > a.median_tol);
+          flag = flag || (vabs(uk - medu) > a.median_tol) || (vabs(vk - medv) > a.median_tol);
         if (!isnan(a.mad_tol))
-          flag = flag || (fabsf(uk - medu) > a.mad_tol * madu) || (fabsf(vk - medv) > a.mad_tol * madv);
+          flag = flag || (vabs(uk - medu) > a.mad_tol * madu) || (vabs(vk - medv) > a.mad_tol * madv);
         if (!isnan(a.mean_tol))
-          flag = flag || (fabsf(uk - meanu) > a.mean_tol) || (fabsf(vk - meanv) > a.mean_tol);
+          flag = flag || (vabs(uk - meanu) > a.mean_tol) || (vabs(vk - meanv) > a.mean_tol);
       }
       if (!isnan(a.rms_tol) && a.rms) {
         flag = flag || (fabs((double)uk - a.rms[0]) > (double)a.rms_tol * a.rms[1]) ||
@@ -793,14 +832,14 @@ __global__ __launch_bounds__(256) void validate_kernel(ValidateArgs a) {
 
 // One 1024-thread block: NaN-mean and population std of u and v in FP64,
 // two passes (mean, then sum of squared deviations).
-__global__ __launch_bounds__(1024) void field_stats_kernel(const float* u, const float* v, int count,
-                                                           double* stats) {
+template <typename R>
+__global__ __launch_bounds__(1024) void field_stats_kernel(const R* u, const R* v, int count, double* stats) {
   __shared__ double red[2][32];
   __shared__ double mean[2];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double s[2] = {0.0, 0.0}, c[2] = {0.0, 0.0};
   for (int k = tid; k < count; k += 1024) {
-    const float a = u[k], b = v[k];
+    const R a = u[k], b = v[k];
     if (!isnan(a)) { s[0] += a; c[0] += 1.0; }
     if (!isnan(b)) { s[1] += b; c[1] += 1.0; }
   }
@@ -822,7 +861,7 @@ __global__ __launch_bounds__(1024) void field_stats_kernel(const float* u, const
   __syncthreads();
   double q[2] = {0.0, 0.0};
   for (int k = tid; k < count; k += 1024) {
-    const float a = u[k], b = v[k];
+    const R a = u[k], b = v[k];
     if (!isnan(a)) q[0] += ((double)a - mean[0]) * ((double)a - mean[0]);
     if (!isnan(b)) q[1] += ((double)b - mean[1]) * ((double)b - mean[1]);
   }
@@ -835,23 +874,25 @@ __global__ __launch_bounds__(1024) void field_stats_kernel(const float* u, const
   }
 }
 
-__global__ __launch_bounds__(256) void neighbour_replace_kernel(const float* u_in, const float* v_in, float* u,
-                                                                float* v, const uint8_t* flags, int m, int n,
-                                                                int method, int exclude) {
+template <typename R>
+__global__ __launch_bounds__(256) void neighbour_replace_kernel(const R* u_in, const R* v_in, R* u, R* v,
+                                                                const uint8_t* flags, int m, int n, int method,
+                                                                int exclude) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m * n || !flags[k]) return;
   const int i = k / n, j = k % n;
-  float s[8];
-  float med, mean;
+  R s[8];
+  R med, mean;
   int c = gather8(u_in, m, n, i, j, exclude ? flags : nullptr, s);
-  stats8(s, c, &med, nullptr, &mean);
+  stats8<R>(s, c, &med, nullptr, &mean);
   u[k] = method == 1 ? med : mean;
   c = gather8(v_in, m, n, i, j, exclude ? flags : nullptr, s);
-  stats8(s, c, &med, nullptr, &mean);
+  stats8<R>(s, c, &med, nullptr, &mean);
   v[k] = method == 1 ? med : mean;
 }
 
-__global__ __launch_bounds__(256) void reset_unresolved_kernel(float* u, float* v, const float* u0, const float* v0,
+template <typename R>
+__global__ __launch_bounds__(256) void reset_unresolved_kernel(R* u, R* v, const R* u0, const R* v0,
                                                                const uint8_t* flags, uint8_t* nan_flags, int count) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
@@ -870,13 +911,15 @@ void set_launch_detail(const char* what, hipError_t e) {
   snprintf(g_detail, sizeof(g_detail), "%s: %s", what, hipGetErrorString(e));
 }
 
-hipError_t launch_deform(const DeformArgs& a, hipStream_t stream) {
+template <typename R>
+hipError_t launch_deform(const DeformArgsT<R>& a, hipStream_t stream) {
   if (a.m < 2 || a.n < 2) return hipErrorInvalidValue;
   dim3 grid((a.wd + 255) / 256, a.ht);
   switch (a.in_dtype) {
-    case kU8: hipLaunchKernelGGL(deform_kernel<uint8_t>, grid, dim3(256), 0, stream, a); break;
-    case kU16: hipLaunchKernelGGL(deform_kernel<uint16_t>, grid, dim3(256), 0, stream, a); break;
-    case kF32: hipLaunchKernelGGL(deform_kernel<float>, grid, dim3(256), 0, stream, a); break;
+    case kU8: hipLaunchKernelGGL((deform_kernel<R, uint8_t>), grid, dim3(256), 0, stream, a); break;
+    case kU16: hipLaunchKernelGGL((deform_kernel<R, uint16_t>), grid, dim3(256), 0, stream, a); break;
+    case kF32: hipLaunchKernelGGL((deform_kernel<R, float>), grid, dim3(256), 0, stream, a); break;
+    case kF64: hipLaunchKernelGGL((deform_kernel<R, double>), grid, dim3(256), 0, stream, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -896,10 +939,11 @@ hipError_t launch_deform_batch(const DeformArgs* fields, const int* row_prefix, 
   return hipGetLastError();
 }
 
-hipError_t launch_grid_interp(const GridInterpArgs& a, hipStream_t stream) {
+template <typename R>
+hipError_t launch_grid_interp(const GridInterpArgsT<R>& a, hipStream_t stream) {
   if (a.m_in < 2 || a.n_in < 2) return hipErrorInvalidValue;
   const int total = a.m_out * a.n_out;
-  hipLaunchKernelGGL(grid_interp_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(grid_interp_kernel<R>, dim3((total + 255) / 256), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
@@ -911,37 +955,42 @@ hipError_t launch_median_validate(const float* u, const float* v, int m, int n, 
   return hipGetLastError();
 }
 
-hipError_t launch_validate(const ValidateArgs& a, hipStream_t stream) {
+template <typename R>
+hipError_t launch_validate(const ValidateArgsT<R>& a, hipStream_t stream) {
   const int total = a.m * a.n;
-  hipLaunchKernelGGL(validate_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(validate_kernel<R>, dim3((total + 255) / 256), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_field_stats(const float* u, const float* v, int count, double* stats, hipStream_t stream) {
-  hipLaunchKernelGGL(field_stats_kernel, dim3(1), dim3(1024), 0, stream, u, v, count, stats);
+template <typename R>
+hipError_t launch_field_stats(const R* u, const R* v, int count, double* stats, hipStream_t stream) {
+  hipLaunchKernelGGL(field_stats_kernel<R>, dim3(1), dim3(1024), 0, stream, u, v, count, stats);
   return hipGetLastError();
 }
 
-hipError_t launch_neighbour_replace(const float* u_in, const float* v_in, float* u, float* v, const uint8_t* flags,
-                                    int m, int n, int method, int exclude, hipStream_t stream) {
+template <typename R>
+hipError_t launch_neighbour_replace(const R* u_in, const R* v_in, R* u, R* v, const uint8_t* flags, int m, int n,
+                                    int method, int exclude, hipStream_t stream) {
   const int total = m * n;
-  hipLaunchKernelGGL(neighbour_replace_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, u_in, v_in, u, v,
+  hipLaunchKernelGGL(neighbour_replace_kernel<R>, dim3((total + 255) / 256), dim3(256), 0, stream, u_in, v_in, u, v,
                      flags, m, n, method, exclude);
   return hipGetLastError();
 }
 
-hipError_t launch_reset_unresolved(float* u, float* v, const float* u0, const float* v0, const uint8_t* flags,
-                                   uint8_t* nan_flags, int count, hipStream_t stream) {
-  hipLaunchKernelGGL(reset_unresolved_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, u, v, u0, v0, flags,
-                     nan_flags, count);
+template <typename R>
+hipError_t launch_reset_unresolved(R* u, R* v, const R* u0, const R* v0, const uint8_t* flags, uint8_t* nan_flags,
+                                   int count, hipStream_t stream) {
+  hipLaunchKernelGGL(reset_unresolved_kernel<R>, dim3((count + 255) / 256), dim3(256), 0, stream, u, v, u0, v0,
+                     flags, nan_flags, count);
   return hipGetLastError();
 }
 
-hipError_t launch_spring_replace(float* u, float* v, const uint8_t* flags, const int32_t* count, int m, int n,
+template <typename R>
+hipError_t launch_spring_replace(R* u, R* v, const uint8_t* flags, const int32_t* count, int m, int n,
                                  const ReplaceWork& w, hipStream_t stream) {
   // the kernel reads the outlier count on the device (graph-capturable, no
   // host sync) and returns at once when there is nothing to solve
-  hipLaunchKernelGGL(spring_lds_kernel, dim3(1), dim3(kSpringLdsThreads), 0, stream, u, v, flags, count, m, n, w);
+  hipLaunchKernelGGL(spring_lds_kernel<R>, dim3(1), dim3(kSpringLdsThreads), 0, stream, u, v, flags, count, m, n, w);
   return hipGetLastError();
 }
 
@@ -949,12 +998,18 @@ hipError_t launch_fill_i32(int32_t* p, int32_t value, int count, hipStream_t str
   hipLaunchKernelGGL(fill_i32_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, p, value, count);
   return hipGetLastError();
 }
-hipError_t launch_scale(float* u, float* v, int count, float factor, hipStream_t stream) {
-  hipLaunchKernelGGL(scale_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, u, v, count, factor);
+template <typename R>
+hipError_t launch_scale(R* u, R* v, int count, R factor, hipStream_t stream) {
+  hipLaunchKernelGGL(scale_kernel<R>, dim3((count + 255) / 256), dim3(256), 0, stream, u, v, count, factor);
   return hipGetLastError();
 }
-hipError_t launch_copy_f32(float* dst, const float* src, int count, hipStream_t stream) {
-  hipLaunchKernelGGL(copy_f32_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, dst, src, count);
+hipError_t launch_scale_div(double* u, double* v, int count, double mul, double div, hipStream_t stream) {
+  hipLaunchKernelGGL(scale_div_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, u, v, count, mul, div);
+  return hipGetLastError();
+}
+template <typename R>
+hipError_t launch_copy(R* dst, const R* src, int count, hipStream_t stream) {
+  hipLaunchKernelGGL(copy_kernel<R>, dim3((count + 255) / 256), dim3(256), 0, stream, dst, src, count);
   return hipGetLastError();
 }
 hipError_t launch_copy_u8(uint8_t* dst, const uint8_t* src, int count, hipStream_t stream) {
@@ -976,16 +1031,37 @@ hipError_t launch_mask_frames(void* a, void* b, int in_dtype, const uint8_t* mas
     case kF32:
       hipLaunchKernelGGL(mask_frames_kernel<float>, grid, dim3(256), 0, stream, (float*)a, (float*)b, mask, npx);
       break;
+    case kF64:
+      hipLaunchKernelGGL(mask_frames_kernel<double>, grid, dim3(256), 0, stream, (double*)a, (double*)b, mask, npx);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_apply_field_mask(float* u, float* v, float* s2n, const uint8_t* fmask, int count,
-                                   hipStream_t stream) {
-  hipLaunchKernelGGL(apply_field_mask_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, u, v, s2n, fmask,
+template <typename R>
+hipError_t launch_apply_field_mask(R* u, R* v, R* s2n, const uint8_t* fmask, int count, hipStream_t stream) {
+  hipLaunchKernelGGL(apply_field_mask_kernel<R>, dim3((count + 255) / 256), dim3(256), 0, stream, u, v, s2n, fmask,
                      count);
   return hipGetLastError();
 }
+
+#define PIV_FIELD_INSTANTIATE(R)                                                                                  \
+  template hipError_t launch_deform<R>(const DeformArgsT<R>&, hipStream_t);                                      \
+  template hipError_t launch_grid_interp<R>(const GridInterpArgsT<R>&, hipStream_t);                             \
+  template hipError_t launch_validate<R>(const ValidateArgsT<R>&, hipStream_t);                                  \
+  template hipError_t launch_field_stats<R>(const R*, const R*, int, double*, hipStream_t);                      \
+  template hipError_t launch_neighbour_replace<R>(const R*, const R*, R*, R*, const uint8_t*, int, int, int, int, \
+                                                  hipStream_t);                                                  \
+  template hipError_t launch_reset_unresolved<R>(R*, R*, const R*, const R*, const uint8_t*, uint8_t*, int,       \
+                                                 hipStream_t);                                                   \
+  template hipError_t launch_spring_replace<R>(R*, R*, const uint8_t*, const int32_t*, int, int, const ReplaceWork&, \
+                                               hipStream_t);                                                     \
+  template hipError_t launch_scale<R>(R*, R*, int, R, hipStream_t);                                              \
+  template hipError_t launch_copy<R>(R*, const R*, int, hipStream_t);                                            \
+  template hipError_t launch_apply_field_mask<R>(R*, R*, R*, const uint8_t*, int, hipStream_t);
+PIV_FIELD_INSTANTIATE(float)
+PIV_FIELD_INSTANTIATE(double)
+#undef PIV_FIELD_INSTANTIATE
 
 }  // namespace piv

@@ -6,7 +6,7 @@
 
 namespace piv {
 
-enum InDtype { kU8 = 0, kU16 = 1, kF32 = 2 };
+enum InDtype { kU8 = 0, kU16 = 1, kF32 = 2, kF64 = 3 };
 enum SubpixelMethod { kGaussian = 0, kParabolic = 1, kCentroid = 2 };
 
 // Correlation of one pass: every window of the field is extracted from the
@@ -38,6 +38,37 @@ struct CorrArgs {
 
 hipError_t launch_correlate(const CorrArgs& a, hipStream_t stream);
 
+// FP64 correlation (dtype_f = float64): the same semantics with every value
+// in double; frames of any InDtype (deformed frames are kF64).  scratch:
+// correlate64_scratch_bytes(fsize) of device memory (planes of F = 128 live
+// there instead of LDS).
+struct CorrArgs64 {
+  const void* frame_a;
+  const void* frame_b;
+  int in_dtype;
+  int ht, wd;
+  int ss, win_pad, fsize;
+  int spacing, x_off, y_off;
+  int m, n;
+  int normalize;
+  int subpixel;
+  const double* up;
+  const double* vp;
+  double* u;
+  double* v;
+  int32_t* i_peak;
+  int32_t* j_peak;
+  double* planes;          // optional N x F x F fftshift-ed planes
+  int s2n_method;
+  int s2n_size;
+  double* s2n;
+  const double* twiddle;   // F/2 entries exp(-2 pi i k / F), (re, im) pairs
+  double2* scratch;
+};
+size_t correlate64_scratch_bytes(int fsize);
+int correlate64_blocks(int fsize, int n_windows);
+hipError_t launch_correlate64(const CorrArgs64& a, hipStream_t stream);
+
 // One launch over the windows of many fields of the same FFT size and input
 // dtype (device tables; every field's windows padded to whole blocks of
 // correlate_windows_per_block(F)).  xcd: XCD-banded block order.
@@ -55,40 +86,47 @@ size_t correlate_lds_bytes(int fsize, int ss);
 // Frame deformation: predictor bilinearly interpolated (with linear
 // extrapolation) from the pass grid onto each pixel, then a backward bilinear
 // warp of each frame at its own time step.  Output frames are FP32.
-struct DeformArgs {
+// R: the field / frame value type (float, or double for dtype_f = float64).
+template <typename R>
+struct DeformArgsT {
   const void* frame_a;
   const void* frame_b;
   int in_dtype;
   int ht, wd;
-  const float* up;  // m x n predictor on the grid
-  const float* vp;
+  const R* up;      // m x n predictor on the grid
+  const R* vp;
   int m, n;
-  float x0, y0;     // first grid node coordinates
-  float spacing;    // grid spacing (pixels)
-  float dt_a, dt_b;
+  R x0, y0;         // first grid node coordinates
+  R spacing;        // grid spacing (pixels)
+  R dt_a, dt_b;
   int warp_a, warp_b;  // 0 -> copy the frame unchanged
-  float* out_a;
-  float* out_b;
+  R* out_a;
+  R* out_b;
   const uint8_t* mask;  // frame mask (1 = masked: both outputs 0) or null
 };
-hipError_t launch_deform(const DeformArgs& a, hipStream_t stream);
+using DeformArgs = DeformArgsT<float>;
+template <typename R>
+hipError_t launch_deform(const DeformArgsT<R>& a, hipStream_t stream);
 // One launch over the rows of many frames (device tables); row r belongs to
 // the last frame whose row_prefix entry is <= r.  threads: block width.
 hipError_t launch_deform_batch(const DeformArgs* fields, const int* row_prefix, int n_fields, int total_rows,
                                int max_wd, int in_dtype, int threads, hipStream_t stream);
 
 // Linear interpolation with extrapolation between two regular grids.
-struct GridInterpArgs {
-  const float* u_in;
-  const float* v_in;
+template <typename R>
+struct GridInterpArgsT {
+  const R* u_in;
+  const R* v_in;
   int m_in, n_in;
-  float x0_in, y0_in, sp_in;
+  R x0_in, y0_in, sp_in;
   int m_out, n_out;
-  float x0_out, y0_out, sp_out;
-  float* u_out;
-  float* v_out;
+  R x0_out, y0_out, sp_out;
+  R* u_out;
+  R* v_out;
 };
-hipError_t launch_grid_interp(const GridInterpArgs& a, hipStream_t stream);
+using GridInterpArgs = GridInterpArgsT<float>;
+template <typename R>
+hipError_t launch_grid_interp(const GridInterpArgsT<R>& a, hipStream_t stream);
 
 // 3x3 NaN-aware neighbour-median test on u and v (flags OR-ed).
 hipError_t launch_median_validate(const float* u, const float* v, int m, int n, float tol, uint8_t* flags,
@@ -100,12 +138,13 @@ hipError_t launch_median_validate(const float* u, const float* v, int m, int n, 
 // |f - mean(f)| > rms_tol * std(f) with the field statistics in rms
 // (mean_u, std_u, mean_v, std_v, FP64).  A NaN tolerance turns a test off.
 // only: validate where only[k] != 0 (others 0); extra: OR-ed in.
-struct ValidateArgs {
-  const float* u;
-  const float* v;
+template <typename R>
+struct ValidateArgsT {
+  const R* u;
+  const R* v;
   int m, n;
-  const float* s2n;
-  float s2n_tol, median_tol, mad_tol, mean_tol, rms_tol;
+  const R* s2n;
+  R s2n_tol, median_tol, mad_tol, mean_tol, rms_tol;
   const double* rms;
   const uint8_t* only;
   const uint8_t* extra;
@@ -113,20 +152,25 @@ struct ValidateArgs {
   int32_t* count;
   const uint8_t* fmask;  // field mask: never flagged (gpu_validation.py:207-210), or null
 };
-hipError_t launch_validate(const ValidateArgs& a, hipStream_t stream);
+using ValidateArgs = ValidateArgsT<float>;
+template <typename R>
+hipError_t launch_validate(const ValidateArgsT<R>& a, hipStream_t stream);
 // Field mean and population standard deviation of u and v (two-pass, FP64).
-hipError_t launch_field_stats(const float* u, const float* v, int count, double* stats, hipStream_t stream);
+template <typename R>
+hipError_t launch_field_stats(const R* u, const R* v, int count, double* stats, hipStream_t stream);
 
 // Median (method 1) or mean (2) replacement of the flagged vectors from the
 // 3x3 neighbourhood of (u_in, v_in) (ReplacementGPU, gpu_validation.py:
 // 373-398); exclude: flagged neighbours left out (first iteration).  Writes
 // u, v at flagged positions only; NaN where no neighbour is usable.
-hipError_t launch_neighbour_replace(const float* u_in, const float* v_in, float* u, float* v, const uint8_t* flags,
-                                    int m, int n, int method, int exclude, hipStream_t stream);
+template <typename R>
+hipError_t launch_neighbour_replace(const R* u_in, const R* v_in, R* u, R* v, const uint8_t* flags, int m, int n,
+                                    int method, int exclude, hipStream_t stream);
 // Flagged vectors that came out NaN go back to (u0, v0) (ReplacementGPU.reset,
 // gpu_validation.py:401-415); nan_flags[k] = 1 for them, else 0.
-hipError_t launch_reset_unresolved(float* u, float* v, const float* u0, const float* v0, const uint8_t* flags,
-                                   uint8_t* nan_flags, int count, hipStream_t stream);
+template <typename R>
+hipError_t launch_reset_unresolved(R* u, R* v, const R* u0, const R* v0, const uint8_t* flags, uint8_t* nan_flags,
+                                   int count, hipStream_t stream);
 
 // Spring (Laplace) replacement of flagged vectors, solved on the device.
 struct ReplaceWork {
@@ -134,53 +178,60 @@ struct ReplaceWork {
   int32_t* list;      // m*n
   double* work;       // 8 * m*n doubles
 };
-hipError_t launch_spring_replace(float* u, float* v, const uint8_t* flags, const int32_t* count, int m, int n,
+template <typename R>
+hipError_t launch_spring_replace(R* u, R* v, const uint8_t* flags, const int32_t* count, int m, int n,
                                  const ReplaceWork& w, hipStream_t stream);
 
 // Unweighted DCT-PLS smoothing (GCV-chosen s) of u and v, in place.
-struct SmoothnWork {
-  const float* cos_m;   // m x m orthonormal DCT-II matrix (row k = frequency)
-  const float* cos_n;   // n x n
+template <typename R>
+struct SmoothnWorkT {
+  const R* cos_m;       // m x m orthonormal DCT-II matrix (row k = frequency)
+  const R* cos_n;       // n x n
   const double* lambda; // m x n
-  float* t0;            // 2 * m*n
-  float* t1;            // 2 * m*n
+  R* t0;                // 2 * m*n
+  R* t1;                // 2 * m*n
   double* p_out;        // chosen log10(s)
   void* state;          // smoothn_state_bytes() of device scratch; its
                         // counter word must start (and stays) zero
 };
+using SmoothnWork = SmoothnWorkT<float>;
 size_t smoothn_state_bytes();
-hipError_t launch_smoothn(float* u, float* v, int m, int n, double p_min, double p_max, double fixed_p,
-                          int auto_s, const SmoothnWork& w, hipStream_t stream);
+template <typename R>
+hipError_t launch_smoothn(R* u, R* v, int m, int n, double p_min, double p_max, double fixed_p, int auto_s,
+                          const SmoothnWorkT<R>& w, hipStream_t stream);
 
 // Weighted DCT-PLS smoothing of u and v in place for fields with masked or
 // missing vectors (nearest-finite fill, weights, coarse start, relaxed
 // iterations with a device-side convergence flag; masked vectors 0 on
 // return).  m, n <= smoothn_masked_max_dim().
-struct SmoothnMaskedWork {
-  const float* cos_m;   // m x m DCT-II matrix (row k = frequency)
-  const float* cos_mt;  // its transpose
-  const float* cos_n;
-  const float* cos_nt;
+template <typename R>
+struct SmoothnMaskedWorkT {
+  const R* cos_m;       // m x m DCT-II matrix (row k = frequency)
+  const R* cos_mt;      // its transpose
+  const R* cos_n;
+  const R* cos_nt;
   const double* lambda; // m x n
   const uint8_t* mask;  // m x n field mask (1 = masked) or null
   int32_t* nr;          // m*n
-  float* f;             // 2 m*n
-  float* w;             // m*n
-  float* z;             // 4 m*n (two slots)
-  float* t0;            // 2 m*n
-  float* t1;            // 2 m*n
+  R* f;                 // 2 m*n
+  R* w;                 // m*n
+  R* z;                 // 4 m*n (two slots)
+  R* t0;                // 2 m*n
+  R* t1;                // 2 m*n
   double* part;         // smoothn_masked_part_doubles(m)
   void* state;          // smoothn_masked_state_bytes()
-  float* qbatch;        // 64 x 2 m*n when slow_gcv, else unused
+  R* qbatch;            // 64 x 2 m*n when slow_gcv, else unused
   double* p_out;        // chosen log10(s) (may be null)
   int32_t* iters_out;   // iterations run (may be null)
   int slow_gcv;         // mean weight <= 0.9: GCV residual from the data misfit
 };
+using SmoothnMaskedWork = SmoothnMaskedWorkT<float>;
 size_t smoothn_masked_state_bytes();
 size_t smoothn_masked_part_doubles(int m);
 int smoothn_masked_max_dim();
-hipError_t launch_smoothn_masked(float* u, float* v, int m, int n, double p_min, double p_max, double fixed_p,
-                                 int auto_s, const SmoothnMaskedWork& w, hipStream_t stream);
+template <typename R>
+hipError_t launch_smoothn_masked(R* u, R* v, int m, int n, double p_min, double p_max, double fixed_p, int auto_s,
+                                 const SmoothnMaskedWorkT<R>& w, hipStream_t stream);
 
 // Window correlation by DFT matrix products on MFMA (precision mode).
 struct DftArgs {
@@ -209,13 +260,20 @@ const char* launch_detail();
 void set_launch_detail(const char* what, hipError_t e);
 
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int count, hipStream_t stream);
-hipError_t launch_scale(float* u, float* v, int count, float factor, hipStream_t stream);
-hipError_t launch_copy_f32(float* dst, const float* src, int count, hipStream_t stream);
+template <typename R>
+hipError_t launch_scale(R* u, R* v, int count, R factor, hipStream_t stream);
+// u = (u * mul) / div, v likewise (FP64 output scaling).
+hipError_t launch_scale_div(double* u, double* v, int count, double mul, double div, hipStream_t stream);
+template <typename R>
+hipError_t launch_copy(R* dst, const R* src, int count, hipStream_t stream);
+inline hipError_t launch_copy_f32(float* dst, const float* src, int count, hipStream_t stream) {
+  return launch_copy<float>(dst, src, count, stream);
+}
 hipError_t launch_copy_u8(uint8_t* dst, const uint8_t* src, int count, hipStream_t stream);
 // Masked pixels of both frames set to 0 in place (mask: 16-byte aligned).
 hipError_t launch_mask_frames(void* a, void* b, int in_dtype, const uint8_t* mask, size_t npx, hipStream_t stream);
 // u = v = NaN and s2n = 0 (when non-null) at masked windows.
-hipError_t launch_apply_field_mask(float* u, float* v, float* s2n, const uint8_t* fmask, int count,
-                                   hipStream_t stream);
+template <typename R>
+hipError_t launch_apply_field_mask(R* u, R* v, R* s2n, const uint8_t* fmask, int count, hipStream_t stream);
 
 }  // namespace piv

@@ -21,6 +21,9 @@ namespace piv {
 
 namespace {
 
+__device__ __forceinline__ float sfma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double sfma(double a, double b, double c) { return fma(a, b, c); }
+
 // ---------------------------------------------------------------------------
 // Small batched GEMM for the 2-D DCTs: for f in {0, 1}
 //   C_f(r, c) = sum_k A_f(r, k) B_f(k, c),  A(r,k) = ta ? A[k*lda + r] : A[r*lda + k]
@@ -28,54 +31,57 @@ namespace {
 // 16 x 16 output tile per 256-thread workgroup (one output per thread), K
 // staged through LDS in chunks of 16; FP32 with FMA accumulation.
 // ---------------------------------------------------------------------------
+template <typename R>
 struct GemmArgs {
-  const float* a[2];
-  const float* b[2];
-  float* c[2];
+  const R* a[2];
+  const R* b[2];
+  R* c[2];
   int lda, ldb, ldc;
   int ta, tb;
   int M, N, K;
 };
 
-__global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g) {
-  __shared__ float As[16][17];
-  __shared__ float Bs[16][17];
+template <typename R>
+__global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs<R> g) {
+  __shared__ R As[16][17];
+  __shared__ R Bs[16][17];
   const int f = blockIdx.z;
-  const float* A = g.a[f];
-  const float* B = g.b[f];
+  const R* A = g.a[f];
+  const R* B = g.b[f];
   const int r0 = blockIdx.y * 16, c0 = blockIdx.x * 16;
   const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
-  float acc = 0.f;
+  R acc = R(0);
   for (int k0 = 0; k0 < g.K; k0 += 16) {
     {
       // As[row][k]: the thread index runs along the contiguous direction
       const int rr = g.ta ? tx : ty, kk = g.ta ? ty : tx;
       const int r = r0 + rr, k = k0 + kk;
-      float val = 0.f;
+      R val = R(0);
       if (r < g.M && k < g.K) val = g.ta ? A[(size_t)k * g.lda + r] : A[(size_t)r * g.lda + k];
       As[rr][kk] = val;
     }
     {
       const int kk = g.tb ? tx : ty, cc = g.tb ? ty : tx;
       const int k = k0 + kk, c = c0 + cc;
-      float val = 0.f;
+      R val = R(0);
       if (k < g.K && c < g.N) val = g.tb ? B[(size_t)c * g.ldb + k] : B[(size_t)k * g.ldb + c];
       Bs[kk][cc] = val;
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) acc = fmaf(As[ty][kk], Bs[kk][tx], acc);
+    for (int kk = 0; kk < 16; ++kk) acc = sfma(As[ty][kk], Bs[kk][tx], acc);
     __syncthreads();
   }
   const int r = r0 + ty, c = c0 + tx;
   if (r < g.M && c < g.N) g.c[f][(size_t)r * g.ldc + c] = acc;
 }
 
-hipError_t gemm2(const float* a0, const float* a1, int lda, int ta, const float* b0, const float* b1, int ldb, int tb,
-                 float* c0, float* c1, int ldc, int M, int N, int K, hipStream_t s) {
-  GemmArgs g{{a0, a1}, {b0, b1}, {c0, c1}, lda, ldb, ldc, ta, tb, M, N, K};
+template <typename R>
+hipError_t gemm2(const R* a0, const R* a1, int lda, int ta, const R* b0, const R* b1, int ldb, int tb, R* c0, R* c1,
+                 int ldc, int M, int N, int K, hipStream_t s) {
+  GemmArgs<R> g{{a0, a1}, {b0, b1}, {c0, c1}, lda, ldb, ldc, ta, tb, M, N, K};
   dim3 grid((N + 15) / 16, (M + 15) / 16, 2);
-  hipLaunchKernelGGL(gemm_small_kernel, grid, dim3(256), 0, s, g);
+  hipLaunchKernelGGL(gemm_small_kernel<R>, grid, dim3(256), 0, s, g);
   return hipGetLastError();
 }
 
@@ -140,8 +146,8 @@ __device__ double g_of(const double* sm, int N) {
 // One evaluation: every block scores the pending p-values over its slice of
 // the spectrum and writes per-block partials (p-major).  No cross-block
 // communication inside the kernel; the decide kernel runs after it.
-template <int NS>
-__global__ __launch_bounds__(256) void gcv_eval_kernel(const float* Y, const double* lam, int N, const GcvState* st,
+template <int NS, typename R>
+__global__ __launch_bounds__(256) void gcv_eval_kernel(const R* Y, const double* lam, int N, const GcvState* st,
                                                        double* part, int span, double p_min, double p_max,
                                                        const int32_t* skip) {
   if (skip && *skip) return;
@@ -331,32 +337,34 @@ __global__ __launch_bounds__(1024) void gcv_decide_kernel(int N, GcvState* st, c
 
 // z_dct = Gamma * f_dct (gpu_smoothn.py:196-199); p from the search state
 // (p_state) or fixed (p_state == null).
-__global__ void gamma_apply_kernel(float* Y, const double* lam, int N, double p_fixed, const GcvState* p_state,
+template <typename R>
+__global__ void gamma_apply_kernel(R* Y, const double* lam, int N, double p_fixed, const GcvState* p_state,
                                    double* p_out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const double p = p_state ? p_state->p : p_fixed;
   if (k < N) {
     const double G = 1.0 / (1.0 + pow(10.0, p) * lam[k]);
-    Y[k] = (float)(G * (double)Y[k]);
-    Y[N + k] = (float)(G * (double)Y[N + k]);
+    Y[k] = (R)(G * (double)Y[k]);
+    Y[N + k] = (R)(G * (double)Y[N + k]);
   }
   if (k == 0 && p_out) *p_out = p;
 }
 
 // The GCV search on the spectrum Y (two fields, 2N floats): span, two
 // bracket refinements, two Newton steps; leaves p in st->p.
-void enqueue_gcv_search(const float* Y, const double* lam, int N, GcvState* st, double* part, double p_min,
+template <typename R>
+void enqueue_gcv_search(const R* Y, const double* lam, int N, GcvState* st, double* part, double p_min,
                         double p_max, const int32_t* skip, hipStream_t stream) {
   const dim3 eb(kGcvBlocks), et(256);
-  hipLaunchKernelGGL(gcv_eval_kernel<4>, eb, et, 0, stream, Y, lam, N, st, part, 1, p_min, p_max, skip);
+  hipLaunchKernelGGL((gcv_eval_kernel<4, R>), eb, et, 0, stream, Y, lam, N, st, part, 1, p_min, p_max, skip);
   hipLaunchKernelGGL(gcv_decide_kernel<0>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
-  hipLaunchKernelGGL(gcv_eval_kernel<4>, eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
+  hipLaunchKernelGGL((gcv_eval_kernel<4, R>), eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
   hipLaunchKernelGGL(gcv_decide_kernel<1>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
-  hipLaunchKernelGGL(gcv_eval_kernel<4>, eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
+  hipLaunchKernelGGL((gcv_eval_kernel<4, R>), eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
   hipLaunchKernelGGL(gcv_decide_kernel<2>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
-  hipLaunchKernelGGL(gcv_eval_kernel<6>, eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
+  hipLaunchKernelGGL((gcv_eval_kernel<6, R>), eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
   hipLaunchKernelGGL(gcv_decide_kernel<3>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
-  hipLaunchKernelGGL(gcv_eval_kernel<6>, eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
+  hipLaunchKernelGGL((gcv_eval_kernel<6, R>), eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
   hipLaunchKernelGGL(gcv_decide_kernel<4>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
 }
 
@@ -365,8 +373,9 @@ void enqueue_gcv_search(const float* Y, const double* lam, int N, GcvState* st, 
 constexpr size_t kGcvPartOffset = (sizeof(GcvState) + 255) & ~size_t(255);
 size_t smoothn_state_bytes() { return kGcvPartOffset + (size_t)kGcvBlocks * kGrid * 6 * sizeof(double); }
 
-hipError_t launch_smoothn(float* u, float* v, int m, int n, double p_min, double p_max, double fixed_p, int auto_s,
-                          const SmoothnWork& w, hipStream_t stream) {
+template <typename R>
+hipError_t launch_smoothn(R* u, R* v, int m, int n, double p_min, double p_max, double fixed_p, int auto_s,
+                          const SmoothnWorkT<R>& w, hipStream_t stream) {
   const int N = m * n;
   if (N < 2) return hipSuccess;  // gpu_smoothn.py:137-140
   hipError_t e;
@@ -379,10 +388,10 @@ hipError_t launch_smoothn(float* u, float* v, int m, int n, double p_min, double
     GcvState* st = reinterpret_cast<GcvState*>(w.state);
     double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(w.state) + kGcvPartOffset);
     enqueue_gcv_search(w.t1, w.lambda, N, st, part, p_min, p_max, nullptr, stream);
-    hipLaunchKernelGGL(gamma_apply_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, w.t1, w.lambda, N, 0.0,
+    hipLaunchKernelGGL(gamma_apply_kernel<R>, dim3((N + 255) / 256), dim3(256), 0, stream, w.t1, w.lambda, N, 0.0,
                        (const GcvState*)st, w.p_out);
   } else {
-    hipLaunchKernelGGL(gamma_apply_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, w.t1, w.lambda, N, fixed_p,
+    hipLaunchKernelGGL(gamma_apply_kernel<R>, dim3((N + 255) / 256), dim3(256), 0, stream, w.t1, w.lambda, N, fixed_p,
                        (const GcvState*)nullptr, w.p_out);
   }
   e = hipGetLastError();
@@ -416,8 +425,10 @@ namespace {
 constexpr int kMaxIters = 100;   // MAX_SMOOTHING_ITERS, gpu_smoothn.py:23
 constexpr double kTolZ = 1e-3;   // TOL, gpu_smoothn.py:24
 constexpr float kRelax = 1.75f;  // 1 + 0.75 * is_weighted, gpu_smoothn.py:162
-constexpr int kSH = 8;           // rows per block of the row transforms
-constexpr int kSW = 8;           // columns per block of the column transforms
+// rows per block of the row transforms and columns per block of the column
+// transforms (LDS: kSH x kMaxDim and 2 x kMaxDim x kSW values)
+template <typename R> constexpr int kSH = sizeof(R) == 8 ? 4 : 8;
+template <typename R> constexpr int kSW = sizeof(R) == 8 ? 2 : 8;
 constexpr int kMaxDim = 1024;
 
 struct WState {
@@ -432,7 +443,8 @@ struct WState {
 // one; -1 for a column with no finite element.  One 256-thread block per
 // column: the last finite row at or above (max-scan) and the first at or
 // below (min-scan) by Hillis-Steele scans in LDS (m <= kMaxDim).
-__global__ __launch_bounds__(256) void nn_column_kernel(const float* u, const float* v, int m, int n, int32_t* nr) {
+template <typename R>
+__global__ __launch_bounds__(256) void nn_column_kernel(const R* u, const R* v, int m, int n, int32_t* nr) {
   __shared__ int up[2][1024];
   __shared__ int dn[2][1024];
   const int j = blockIdx.x;
@@ -465,8 +477,9 @@ __global__ __launch_bounds__(256) void nn_column_kernel(const float* u, const fl
 // row -- the choice of scipy's exact feature transform used at
 // gpu_smoothn.py:244-251).  The column scan stops once |dj|^2 exceeds the
 // best squared distance.  Also resets the iteration state.
-__global__ __launch_bounds__(256) void nn_fill_kernel(const float* u, const float* v, const uint8_t* mask, int m,
-                                                      int n, const int32_t* nr, float* f, float* w, WState* st) {
+template <typename R>
+__global__ __launch_bounds__(256) void nn_fill_kernel(const R* u, const R* v, const uint8_t* mask, int m, int n,
+                                                      const int32_t* nr, R* f, R* w, WState* st) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k == 0) {
     st->converged = 0;
@@ -478,9 +491,9 @@ __global__ __launch_bounds__(256) void nn_fill_kernel(const float* u, const floa
   const int N = m * n;
   if (k >= N) return;
   const int i = k / n, j = k % n;
-  const float a = u[k], b = v[k];
+  const R a = u[k], b = v[k];
   const bool fin = isfinite(a) && isfinite(b);
-  w[k] = (fin && !(mask && mask[k])) ? 1.f : 0.f;
+  w[k] = (fin && !(mask && mask[k])) ? R(1) : R(0);
   if (fin) {
     f[k] = a;
     f[N + k] = b;
@@ -506,8 +519,8 @@ __global__ __launch_bounds__(256) void nn_fill_kernel(const float* u, const floa
   }
   // no finite vector at all: the weights are all zero and the value is never
   // used unweighted; 0 keeps the transforms finite
-  f[k] = bi < 0 ? 0.f : u[(size_t)bi * n + bj];
-  f[N + k] = bi < 0 ? 0.f : v[(size_t)bi * n + bj];
+  f[k] = bi < 0 ? R(0) : u[(size_t)bi * n + bj];
+  f[N + k] = bi < 0 ? R(0) : v[(size_t)bi * n + bj];
 }
 
 __device__ __forceinline__ double block_sum_d(double x, double* red) {
@@ -524,17 +537,18 @@ __device__ __forceinline__ double block_sum_d(double x, double* red) {
 
 // Row transforms out[r][c] = sum_j X[r][j] M[j][c] over kSH rows per block
 // (blockIdx.y = field, blockIdx.z = p-value of a batched GCV evaluation).
+template <typename R>
 struct RowArgs {
-  const float* in;      // pro 0 input (2N per batch entry)
-  const float* f;       // pro 1 / epi 2: filled data (2N)
-  const float* w;       // weights (N)
-  const float* z;       // current z (2N): pro 1, epi 1 relaxation
-  const float* M;       // n x n
-  float* out;           // 2N (epi 0, 1)
+  const R* in;          // pro 0 input (2N per batch entry)
+  const R* f;           // pro 1 / epi 2: filled data (2N)
+  const R* w;           // weights (N)
+  const R* z;           // current z (2N): pro 1, epi 1 relaxation
+  const R* M;           // n x n
+  R* out;               // 2N (epi 0, 1)
   int m, n;
   int pro;              // 0: in; 1: w (f - z) + z
   int epi;              // 0: store; 1: relaxed update (+ change norms); 2: GCV residual partials
-  float relax;
+  R relax;
   int track;            // epi 1: change norms, last block decides convergence
   int slot;             // epi 1: ping-pong slot written
   int iteration;
@@ -545,25 +559,27 @@ struct RowArgs {
   const double* lambda; // epi 2
 };
 
-__global__ __launch_bounds__(256) void rows_kernel(RowArgs a) {
+template <typename R>
+__global__ __launch_bounds__(256) void rows_kernel(RowArgs<R> a) {
+  constexpr int SH = kSH<R>;
   if (a.skip && *a.skip) return;
-  __shared__ float X[kSH][kMaxDim];
+  __shared__ R X[SH][kMaxDim];
   __shared__ double red[4];
   __shared__ int is_last;
   const int field = blockIdx.y, pz = blockIdx.z;
   const int m = a.m, n = a.n;
   const size_t N = (size_t)m * n;
-  const int r0 = blockIdx.x * kSH;
-  const float* in = a.in ? a.in + (size_t)pz * 2 * N + (size_t)field * N : nullptr;
-  for (int e = threadIdx.x; e < kSH * n; e += blockDim.x) {
+  const int r0 = blockIdx.x * SH;
+  const R* in = a.in ? a.in + (size_t)pz * 2 * N + (size_t)field * N : nullptr;
+  for (int e = threadIdx.x; e < SH * n; e += blockDim.x) {
     const int rr = e / n, jc = e - rr * n, r = r0 + rr;
-    float x = 0.f;
+    R x = R(0);
     if (r < m) {
       const size_t k = (size_t)r * n + jc;
       if (a.pro == 0) {
         x = in[k];
       } else {
-        const float zz = a.z[field * N + k];
+        const R zz = a.z[field * N + k];
         x = a.w[k] * (a.f[field * N + k] - zz) + zz;
       }
     }
@@ -574,26 +590,26 @@ __global__ __launch_bounds__(256) void rows_kernel(RowArgs a) {
   double trace = 0.0;
   const double sp = a.epi == 2 ? pow(10.0, a.gst->pv[pz]) : 0.0;
   for (int c = threadIdx.x; c < n; c += blockDim.x) {
-    float acc[kSH];
+    R acc[SH];
 #pragma unroll
-    for (int rr = 0; rr < kSH; ++rr) acc[rr] = 0.f;
+    for (int rr = 0; rr < SH; ++rr) acc[rr] = R(0);
     for (int jj = 0; jj < n; ++jj) {
-      const float mv = a.M[(size_t)jj * n + c];
+      const R mv = a.M[(size_t)jj * n + c];
 #pragma unroll
-      for (int rr = 0; rr < kSH; ++rr) acc[rr] = fmaf(X[rr][jj], mv, acc[rr]);
+      for (int rr = 0; rr < SH; ++rr) acc[rr] = sfma(X[rr][jj], mv, acc[rr]);
     }
 #pragma unroll
-    for (int rr = 0; rr < kSH; ++rr) {
+    for (int rr = 0; rr < SH; ++rr) {
       const int r = r0 + rr;
       if (r >= m) break;
       const size_t k = (size_t)r * n + c;
       if (a.epi == 0) {
         a.out[field * N + k] = acc[rr];
       } else if (a.epi == 1) {
-        float zn = acc[rr];
-        if (a.relax != 1.f) {
-          const float zo = a.z[field * N + k];
-          zn = a.relax * acc[rr] + (1.f - a.relax) * zo;
+        R zn = acc[rr];
+        if (a.relax != R(1)) {
+          const R zo = a.z[field * N + k];
+          zn = a.relax * acc[rr] + (R(1) - a.relax) * zo;
           const double dz = (double)zo - (double)zn;
           s0 += dz * dz;
         }
@@ -653,11 +669,12 @@ __global__ __launch_bounds__(256) void rows_kernel(RowArgs a) {
 // G = Gamma(p) = 1 / (1 + 10^p Lambda), or the coarse-start truncation
 // (1 where row < n0 or column < n1).  In-place use is safe: a block reads
 // its whole strip before writing it.
+template <typename R>
 struct ColArgs {
-  const float* in;
-  float* out;           // + z * 2N when batched
-  const float* Mf;      // forward, transposed: Mf[r * m + i] = C[i][r]
-  const float* Mi;      // inverse: C[k][i] at k * m + i
+  const R* in;
+  R* out;               // + z * 2N when batched
+  const R* Mf;          // forward, transposed: Mf[r * m + i] = C[i][r]
+  const R* Mi;          // inverse: C[k][i] at k * m + i
   const double* lambda;
   int m, n;
   int mode;
@@ -669,20 +686,22 @@ struct ColArgs {
   const int32_t* skip;
 };
 
-__global__ __launch_bounds__(256) void cols_kernel(ColArgs a) {
+template <typename R>
+__global__ __launch_bounds__(256) void cols_kernel(ColArgs<R> a) {
+  constexpr int SW = kSW<R>;
   if (a.skip && *a.skip) return;
-  __shared__ float X[kMaxDim][kSW];
-  __shared__ float Y[kMaxDim][kSW];
+  __shared__ R X[kMaxDim][SW];
+  __shared__ R Y[kMaxDim][SW];
   const int field = blockIdx.y, pz = blockIdx.z;
   const int m = a.m, n = a.n;
   const size_t N = (size_t)m * n;
-  const int c0 = blockIdx.x * kSW;
-  const int nc = min(kSW, n - c0);
-  const float* in = a.in + (size_t)field * N;
-  float* out = a.out + (size_t)(a.batched ? pz : 0) * 2 * N + (size_t)field * N;
-  for (int e = threadIdx.x; e < m * kSW; e += blockDim.x) {
-    const int r = e / kSW, c = e - r * kSW;
-    X[r][c] = c < nc ? in[(size_t)r * n + c0 + c] : 0.f;
+  const int c0 = blockIdx.x * SW;
+  const int nc = min(SW, n - c0);
+  const R* in = a.in + (size_t)field * N;
+  R* out = a.out + (size_t)(a.batched ? pz : 0) * 2 * N + (size_t)field * N;
+  for (int e = threadIdx.x; e < m * SW; e += blockDim.x) {
+    const int r = e / SW, c = e - r * SW;
+    X[r][c] = c < nc ? in[(size_t)r * n + c0 + c] : R(0);
   }
   __syncthreads();
   double s = 0.0;
@@ -690,43 +709,43 @@ __global__ __launch_bounds__(256) void cols_kernel(ColArgs a) {
     const double p = a.gst ? (a.batched ? a.gst->pv[pz] : a.gst->p) : a.p_fixed;
     s = pow(10.0, p);
   }
-  auto gain = [&](int i, int c) -> float {
-    if (a.trunc) return (i < a.n0 || c0 + c < a.n1) ? 1.f : 0.f;
-    return (float)(1.0 / (1.0 + s * a.lambda[(size_t)i * n + c0 + c]));
+  auto gain = [&](int i, int c) -> R {
+    if (a.trunc) return (i < a.n0 || c0 + c < a.n1) ? R(1) : R(0);
+    return (R)(1.0 / (1.0 + s * a.lambda[(size_t)i * n + c0 + c]));
   };
   if (a.mode != 2) {
     for (int i = threadIdx.x; i < m; i += blockDim.x) {
-      float acc[kSW];
+      R acc[SW];
 #pragma unroll
-      for (int c = 0; c < kSW; ++c) acc[c] = 0.f;
+      for (int c = 0; c < SW; ++c) acc[c] = R(0);
       for (int r = 0; r < m; ++r) {
-        const float cv = a.Mf[(size_t)r * m + i];
+        const R cv = a.Mf[(size_t)r * m + i];
 #pragma unroll
-        for (int c = 0; c < kSW; ++c) acc[c] = fmaf(cv, X[r][c], acc[c]);
+        for (int c = 0; c < SW; ++c) acc[c] = sfma(cv, X[r][c], acc[c]);
       }
       if (a.mode == 1) {
         for (int c = 0; c < nc; ++c) out[(size_t)i * n + c0 + c] = acc[c];
       } else {
 #pragma unroll
-        for (int c = 0; c < kSW; ++c) Y[i][c] = c < nc ? acc[c] * gain(i, c) : 0.f;
+        for (int c = 0; c < SW; ++c) Y[i][c] = c < nc ? acc[c] * gain(i, c) : R(0);
       }
     }
     if (a.mode == 1) return;
   } else {
-    for (int e = threadIdx.x; e < m * kSW; e += blockDim.x) {
-      const int r = e / kSW, c = e - r * kSW;
-      Y[r][c] = c < nc ? X[r][c] * gain(r, c) : 0.f;
+    for (int e = threadIdx.x; e < m * SW; e += blockDim.x) {
+      const int r = e / SW, c = e - r * SW;
+      Y[r][c] = c < nc ? X[r][c] * gain(r, c) : R(0);
     }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    float acc[kSW];
+    R acc[SW];
 #pragma unroll
-    for (int c = 0; c < kSW; ++c) acc[c] = 0.f;
+    for (int c = 0; c < SW; ++c) acc[c] = R(0);
     for (int k = 0; k < m; ++k) {
-      const float cv = a.Mi[(size_t)k * m + i];
+      const R cv = a.Mi[(size_t)k * m + i];
 #pragma unroll
-      for (int c = 0; c < kSW; ++c) acc[c] = fmaf(cv, Y[k][c], acc[c]);
+      for (int c = 0; c < SW; ++c) acc[c] = sfma(cv, Y[k][c], acc[c]);
     }
     for (int c = 0; c < nc; ++c) out[(size_t)i * n + c0 + c] = acc[c];
   }
@@ -800,15 +819,16 @@ __global__ void gcv_slow_init_kernel(GcvState* st, double p_min, double p_max, c
 }
 
 // z at the converged (or last) slot, 0 at masked vectors (:224-225).
-__global__ __launch_bounds__(256) void masked_out_kernel(float* u, float* v, const uint8_t* mask, const float* z,
-                                                         int N, const WState* st, const GcvState* gst,
-                                                         double p_fixed, double* p_out, int32_t* iters_out) {
+template <typename R>
+__global__ __launch_bounds__(256) void masked_out_kernel(R* u, R* v, const uint8_t* mask, const R* z, int N,
+                                                         const WState* st, const GcvState* gst, double p_fixed,
+                                                         double* p_out, int32_t* iters_out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const float* zz = z + (size_t)st->result * 2 * N;
+  const R* zz = z + (size_t)st->result * 2 * N;
   if (k < N) {
     const bool mk = mask && mask[k];
-    u[k] = mk ? 0.f : zz[k];
-    v[k] = mk ? 0.f : zz[N + k];
+    u[k] = mk ? R(0) : zz[k];
+    v[k] = mk ? R(0) : zz[N + k];
   }
   if (k == 0) {
     if (p_out) *p_out = gst ? gst->p : p_fixed;
@@ -816,20 +836,27 @@ __global__ __launch_bounds__(256) void masked_out_kernel(float* u, float* v, con
   }
 }
 
+// GCV partials: the spectral search's (kGcvBlocks x kGrid x 6) or the
+// batched-residual search's (kGrid p-values x 2 fields x up to 256 row
+// blocks x 2), whichever is larger
+constexpr size_t kGcvPartDoubles =
+    (size_t)kGcvBlocks * kGrid * 6 > (size_t)kGrid * 2 * 256 * 2 ? (size_t)kGcvBlocks * kGrid * 6
+                                                                  : (size_t)kGrid * 2 * 256 * 2;
 constexpr size_t kWStateOffset =
-    ((((sizeof(GcvState) + 255) & ~size_t(255)) + (size_t)kGcvBlocks * kGrid * 6 * sizeof(double)) + 255) &
-    ~size_t(255);
+    ((((sizeof(GcvState) + 255) & ~size_t(255)) + kGcvPartDoubles * sizeof(double)) + 255) & ~size_t(255);
 
 }  // namespace
 
 size_t smoothn_masked_state_bytes() { return kWStateOffset + 256; }
 
-size_t smoothn_masked_part_doubles(int m) { return 2 * 2 * (size_t)((m + kSH - 1) / kSH); }
+size_t smoothn_masked_part_doubles(int m) { return 2 * 2 * (size_t)((m + kSH<double> - 1) / kSH<double>); }
 
 int smoothn_masked_max_dim() { return kMaxDim; }
 
-hipError_t launch_smoothn_masked(float* u, float* v, int m, int n, double p_min, double p_max, double fixed_p,
-                                 int auto_s, const SmoothnMaskedWork& w, hipStream_t stream) {
+template <typename R>
+hipError_t launch_smoothn_masked(R* u, R* v, int m, int n, double p_min, double p_max, double fixed_p, int auto_s,
+                                 const SmoothnMaskedWorkT<R>& w, hipStream_t stream) {
+  constexpr int SH = kSH<R>, SW = kSW<R>;
   const int N = m * n;
   if (N < 2) return hipSuccess;  // gpu_smoothn.py:137-140
   if (m > kMaxDim || n > kMaxDim) return hipErrorInvalidValue;
@@ -839,19 +866,19 @@ hipError_t launch_smoothn_masked(float* u, float* v, int m, int n, double p_min,
   WState* ws = reinterpret_cast<WState*>(base + kWStateOffset);
   const int32_t* skip = &ws->converged;
   const size_t N2 = 2 * (size_t)N;
-  float* zs[2] = {w.z, w.z + N2};
-  const dim3 rgrid((m + kSH - 1) / kSH, 2), cgrid((n + kSW - 1) / kSW, 2), tb(256);
+  R* zs[2] = {w.z, w.z + N2};
+  const dim3 rgrid((m + SH - 1) / SH, 2), cgrid((n + SW - 1) / SW, 2), tb(256);
 
-  hipLaunchKernelGGL(nn_column_kernel, dim3(n), dim3(256), 0, stream, u, v, m, n, w.nr);
-  hipLaunchKernelGGL(nn_fill_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, u, v, w.mask, m, n, w.nr, w.f,
+  hipLaunchKernelGGL(nn_column_kernel<R>, dim3(n), dim3(256), 0, stream, u, v, m, n, w.nr);
+  hipLaunchKernelGGL(nn_fill_kernel<R>, dim3((N + 255) / 256), dim3(256), 0, stream, u, v, w.mask, m, n, w.nr, w.f,
                      w.w, ws);
 
-  RowArgs ra{};
+  RowArgs<R> ra{};
   ra.m = m;
   ra.n = n;
   ra.st = ws;
   ra.part = w.part;
-  ColArgs ca{};
+  ColArgs<R> ca{};
   ca.m = m;
   ca.n = n;
   ca.Mf = w.cos_mt;
@@ -861,30 +888,30 @@ hipError_t launch_smoothn_masked(float* u, float* v, int m, int n, double p_min,
 
   // coarse start (gpu_smoothn.py:283-297): keep the coefficients with row <
   // ceil(m/10)+1 or column < ceil(n/10)+1
-  RowArgs r0 = ra;
+  RowArgs<R> r0 = ra;
   r0.in = w.f;
   r0.M = w.cos_nt;
   r0.out = w.t0;
-  hipLaunchKernelGGL(rows_kernel, rgrid, tb, 0, stream, r0);
-  ColArgs c0 = ca;
+  hipLaunchKernelGGL(rows_kernel<R>, rgrid, tb, 0, stream, r0);
+  ColArgs<R> c0 = ca;
   c0.in = w.t0;
   c0.out = w.t1;
   c0.mode = 0;
   c0.trunc = 1;
   c0.n0 = (m + 9) / 10 + 1;
   c0.n1 = (n + 9) / 10 + 1;
-  hipLaunchKernelGGL(cols_kernel, cgrid, tb, 0, stream, c0);
-  RowArgs r1 = ra;
+  hipLaunchKernelGGL(cols_kernel<R>, cgrid, tb, 0, stream, c0);
+  RowArgs<R> r1 = ra;
   r1.in = w.t1;
   r1.M = w.cos_n;
   r1.out = zs[0];
   r1.epi = 1;
-  r1.relax = 1.f;
-  hipLaunchKernelGGL(rows_kernel, rgrid, tb, 0, stream, r1);
+  r1.relax = R(1);
+  hipLaunchKernelGGL(rows_kernel<R>, rgrid, tb, 0, stream, r1);
 
   for (int k = 0; k < kMaxIters; ++k) {
-    const float* zc = zs[k & 1];
-    RowArgs fa = ra;
+    const R* zc = zs[k & 1];
+    RowArgs<R> fa = ra;
     fa.f = w.f;
     fa.w = w.w;
     fa.z = zc;
@@ -892,30 +919,30 @@ hipError_t launch_smoothn_masked(float* u, float* v, int m, int n, double p_min,
     fa.M = w.cos_nt;
     fa.out = w.t0;
     fa.skip = skip;
-    hipLaunchKernelGGL(rows_kernel, rgrid, tb, 0, stream, fa);
+    hipLaunchKernelGGL(rows_kernel<R>, rgrid, tb, 0, stream, fa);
     const bool gcv = auto_s && ((k + 1) & k) == 0;
-    ColArgs cc = ca;
+    ColArgs<R> cc = ca;
     cc.skip = skip;
     cc.gst = auto_s ? gst : nullptr;
     if (gcv) {
-      ColArgs cs = cc;
+      ColArgs<R> cs = cc;
       cs.in = w.t0;
       cs.out = w.t1;
       cs.mode = 1;
-      hipLaunchKernelGGL(cols_kernel, cgrid, tb, 0, stream, cs);
+      hipLaunchKernelGGL(cols_kernel<R>, cgrid, tb, 0, stream, cs);
       if (!w.slow_gcv) {
         enqueue_gcv_search(w.t1, w.lambda, N, gst, gpart, p_min, p_max, skip, stream);
       } else {
         hipLaunchKernelGGL(gcv_slow_init_kernel, dim3(1), dim3(64), 0, stream, gst, p_min, p_max, skip);
         for (int ph = 0; ph < 3; ++ph) {
           const int P = ph == 0 ? kSpan : kGrid;
-          ColArgs cb = cc;
+          ColArgs<R> cb = cc;
           cb.in = w.t1;
           cb.out = w.qbatch;
           cb.mode = 2;
           cb.batched = 1;
-          hipLaunchKernelGGL(cols_kernel, dim3(cgrid.x, 2, P), tb, 0, stream, cb);
-          RowArgs rb = ra;
+          hipLaunchKernelGGL(cols_kernel<R>, dim3(cgrid.x, 2, P), tb, 0, stream, cb);
+          RowArgs<R> rb = ra;
           rb.in = w.qbatch;
           rb.f = w.f;
           rb.w = w.w;
@@ -925,7 +952,7 @@ hipError_t launch_smoothn_masked(float* u, float* v, int m, int n, double p_min,
           rb.lambda = w.lambda;
           rb.part = gpart;
           rb.skip = skip;
-          hipLaunchKernelGGL(rows_kernel, dim3(rgrid.x, 2, P), tb, 0, stream, rb);
+          hipLaunchKernelGGL(rows_kernel<R>, dim3(rgrid.x, 2, P), tb, 0, stream, rb);
           if (ph == 0)
             hipLaunchKernelGGL(gcv_slow_decide_kernel<0>, dim3(1), dim3(64), 0, stream, N, gst, gpart, (int)rgrid.x,
                                skip);
@@ -937,33 +964,42 @@ hipError_t launch_smoothn_masked(float* u, float* v, int m, int n, double p_min,
                                skip);
         }
       }
-      ColArgs ci = cc;
+      ColArgs<R> ci = cc;
       ci.in = w.t1;
       ci.out = w.t1;
       ci.mode = 2;
-      hipLaunchKernelGGL(cols_kernel, cgrid, tb, 0, stream, ci);
+      hipLaunchKernelGGL(cols_kernel<R>, cgrid, tb, 0, stream, ci);
     } else {
       cc.in = w.t0;
       cc.out = w.t1;
       cc.mode = 0;
-      hipLaunchKernelGGL(cols_kernel, cgrid, tb, 0, stream, cc);
+      hipLaunchKernelGGL(cols_kernel<R>, cgrid, tb, 0, stream, cc);
     }
-    RowArgs ru = ra;
+    RowArgs<R> ru = ra;
     ru.in = w.t1;
     ru.M = w.cos_n;
     ru.z = zc;
     ru.out = zs[(k + 1) & 1];
     ru.epi = 1;
-    ru.relax = kRelax;
+    ru.relax = (R)kRelax;
     ru.track = 1;
     ru.slot = (k + 1) & 1;
     ru.iteration = k;
     ru.skip = skip;
-    hipLaunchKernelGGL(rows_kernel, rgrid, tb, 0, stream, ru);
+    hipLaunchKernelGGL(rows_kernel<R>, rgrid, tb, 0, stream, ru);
   }
-  hipLaunchKernelGGL(masked_out_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, u, v, w.mask, w.z, N,
+  hipLaunchKernelGGL(masked_out_kernel<R>, dim3((N + 255) / 256), dim3(256), 0, stream, u, v, w.mask, w.z, N,
                      (const WState*)ws, auto_s ? (const GcvState*)gst : nullptr, fixed_p, w.p_out, w.iters_out);
   return hipGetLastError();
 }
+
+template hipError_t launch_smoothn<float>(float*, float*, int, int, double, double, double, int,
+                                         const SmoothnWorkT<float>&, hipStream_t);
+template hipError_t launch_smoothn<double>(double*, double*, int, int, double, double, double, int,
+                                          const SmoothnWorkT<double>&, hipStream_t);
+template hipError_t launch_smoothn_masked<float>(float*, float*, int, int, double, double, double, int,
+                                                const SmoothnMaskedWorkT<float>&, hipStream_t);
+template hipError_t launch_smoothn_masked<double>(double*, double*, int, int, double, double, double, int,
+                                                 const SmoothnMaskedWorkT<double>&, hipStream_t);
 
 }  // namespace piv

@@ -13,7 +13,8 @@ from typing import Optional
 import numpy as np
 
 MAX_PASSES = 16
-PIV_U8, PIV_U16, PIV_F32 = 0, 1, 2
+PIV_U8, PIV_U16, PIV_F32, PIV_F64 = 0, 1, 2, 3
+PIV_FP32, PIV_FP64 = 0, 1
 SUBPIXEL_CODES = {"gaussian": 0, "parabolic": 1, "centroid": 2}
 S2N_CODES = {"peak2peak": 0, "peak2mean": 1, "peak2energy": 2}
 REPLACING_CODES = {"spring": 0, "median": 1, "mean": 2}
@@ -37,7 +38,7 @@ EXPORTED_SYMBOLS = (
     "piv_batch_create", "piv_batch_destroy", "piv_batch_set_frames", "piv_batch_run", "piv_batch_tune",
     "piv_batch_pair_shape", "piv_batch_get", "piv_tiff_info", "piv_tiff_read", "piv_host_alloc", "piv_host_free",
     "piv_memcpy_async", "piv_stream_wait_event", "piv_plan_wait_event", "piv_plan_set_mask",
-    "piv_plan_last_smoothing_iterations", "piv_stage_smoothn_masked",
+    "piv_plan_last_smoothing_iterations", "piv_stage_smoothn_masked", "piv_stage_correlate64",
 )
 
 SYNTH_MAX_VORTICES = 8
@@ -83,6 +84,10 @@ class PlanDesc(ctypes.Structure):
         ("passes", PassDesc * MAX_PASSES),
         ("out_scale", ctypes.c_float),
         ("use_graph", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("scaling_par", ctypes.c_double),
+        ("dt", ctypes.c_double),
     ]
 
 
@@ -143,6 +148,8 @@ def _bind(lib):
         "piv_stage_spring_replace": (i32, [vp, vp, vp, vp, i32, i32, vp]),
         "piv_stage_smoothn": (i32, [vp, vp, i32, i32, f64, vp, vp]),
         "piv_stage_smoothn_masked": (i32, [vp, vp, vp, i32, i32, f64, vp, vp, vp]),
+        "piv_stage_correlate64": (i32, [vp, vp, i32, i32, i32, ctypes.POINTER(PassDesc), vp, vp, vp, vp, vp, vp, vp,
+                                        vp, vp]),
         "piv_plan_set_mask": (i32, [vp, vp, i32, i32]),
         "piv_plan_last_smoothing_iterations": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int32)]),
         "piv_stage_correlate_dft": (i32, [vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
@@ -234,6 +241,8 @@ def in_dtype_code(dtype) -> int:
         return PIV_U16
     if dt == np.float32:
         return PIV_F32
+    if dt == np.float64:
+        return PIV_F64
     raise ValueError(f"unsupported frame dtype {dt}")
 
 

@@ -199,8 +199,11 @@ class PIVGPU:
                  replacing_method=REPLACING_METHOD, replacing_size=REPLACING_SIZE, revalidate=REVALIDATE,
                  smooth=SMOOTH, smoothing_par=SMOOTHING_PAR, dt=1, scaling_par=1, mask=None, dtype_f="float32",
                  device=0, use_graph=True, predictor=None, max_reject_fraction=0.05, predictor_pass=-1):
-        if not (dtype_f == "float32" or dtype_f is np.float32):
-            raise NotImplementedError("dtype_f: only 'float32' processing is implemented on the MI355X path")
+        # gpu_process.py:552: anything but the string "float32" selects FP64
+        # processing (the default here is "float32")
+        self.fp64 = not (isinstance(dtype_f, str) and dtype_f == "float32")
+        if self.fp64 and predictor is not None:
+            raise NotImplementedError("the learned predictor runs FP32 processing only")
         if mask is not None:
             mask = np.asarray(mask).astype(bool)                # gpu_process.py:548-549
             if mask.shape != tuple(frame_shape):
@@ -239,7 +242,7 @@ class PIVGPU:
         self.dt = dt
         self.scaling_par = scaling_par
         self.mask = mask
-        self.dtype_f = np.float32
+        self.dtype_f = np.float64 if self.fp64 else np.float32
         self.device = device
         self.use_graph = bool(use_graph)
         # learned coarse predictor (openpiv_gpu.learned.CoarseFlowNet): replaces
@@ -339,6 +342,9 @@ class PIVGPU:
             scale = float(self.scaling_par) / float(self.dt)
         d.out_scale = scale
         d.use_graph = int(self.use_graph)
+        d.precision = _native.PIV_FP64 if self.fp64 else _native.PIV_FP32
+        d.scaling_par = float(self.scaling_par)
+        d.dt = float(self.dt)
         return d
 
     def _get_plan(self, in_dtype: int):
@@ -357,29 +363,33 @@ class PIVGPU:
         return self._plan.last_fallback() if self._plan is not None else (False, 0)
 
     # -- the pass loop ---------------------------------------------------------
-    @staticmethod
-    def _prepare_frame(frame):
+    def _prepare_frame(self, frame):
+        # the library reads uint8 / uint16 / float32 (and float64 when processing
+        # in FP64) frames directly and converts exactly; anything else is cast
+        # like gpu_process.py:582-583 (astype(dtype_f))
         frame = np.asarray(frame)
-        if frame.dtype not in (np.uint8, np.uint16, np.float32):
-            frame = frame.astype(np.float32)   # gpu_process.py:582-583 (astype(dtype_f))
+        raw = (np.uint8, np.uint16, np.float32) + ((np.float64,) if self.fp64 else ())
+        if frame.dtype not in raw:
+            frame = frame.astype(self.dtype_f)
         return np.ascontiguousarray(frame)
 
     def _run(self, frame_a, frame_b, outputs_on_device):
         fa = self._prepare_frame(frame_a)
         fb = self._prepare_frame(frame_b)
         if fa.dtype != fb.dtype:
-            fa = fa.astype(np.float32)
-            fb = fb.astype(np.float32)
+            fa = fa.astype(self.dtype_f)
+            fb = fb.astype(self.dtype_f)
         plan = self._get_plan(_native.in_dtype_code(fa.dtype))
         m, n = self.piv_fields[-1].field_shape
+        fd = self.dtype_f
         if outputs_on_device:
-            outs = [_native.DeviceBuffer((m, n), np.float32), _native.DeviceBuffer((m, n), np.float32),
+            outs = [_native.DeviceBuffer((m, n), fd), _native.DeviceBuffer((m, n), fd),
                     _native.DeviceBuffer((m, n), np.uint8), _native.DeviceBuffer((m, n), np.uint8)]
             u, v, vi, vf = outs
             ptrs = [b.value for b in outs]
         else:
-            u = np.empty((m, n), np.float32)
-            v = np.empty((m, n), np.float32)
+            u = np.empty((m, n), fd)
+            v = np.empty((m, n), fd)
             vi = np.empty((m, n), np.uint8)
             vf = np.empty((m, n), np.uint8)
             ptrs = [a.ctypes.data for a in (u, v, vi, vf)]
@@ -388,8 +398,9 @@ class PIVGPU:
 
     @property
     def _out_dtype(self):
-        # one pass: FP32; more: FP64 through smoothn's promotion (gpu_process.py:631-632, 842)
-        return np.float32 if len(self.piv_fields) == 1 else np.float64
+        # FP32 processing: one pass FP32, more FP64 through smoothn's promotion
+        # (gpu_process.py:631-632, 842); FP64 processing: FP64
+        return np.float32 if len(self.piv_fields) == 1 and not self.fp64 else np.float64
 
     def __call__(self, frame_a, frame_b):
         """Computes the velocity field from an image pair; returns device

@@ -115,13 +115,31 @@ def test_min_search_size_and_frame_shape_assertions():
 
 
 @pytest.mark.parametrize("kwargs,name", [
-    (dict(dtype_f="float64"), "dtype_f"),
     (dict(validation_size=2), "validation_size"),
     (dict(replacing_size=2), "replacing_size"),
 ])
 def test_unimplemented_options_raise(kwargs, name):
     with pytest.raises(NotImplementedError, match=name):
         gp.piv_gpu((256, 256), 16, **kwargs)
+
+
+def test_fp64_settings_on_the_host():
+    """dtype_f other than the string "float32" selects FP64 processing
+    (gpu_process.py:552): double coordinates, an FP64 plan description with
+    the exact scaling factors, float64 frames accepted raw."""
+    piv = gp.PIVGPU((256, 256), 16, search_size_iters=(1, 1), dtype_f="float64", dt=0.3, scaling_par=2.0)
+    assert piv.fp64 and piv.dtype_f == np.float64
+    assert piv.piv_fields[-1].x.dtype == np.float64
+    d = piv._plan_desc(_native.PIV_F64)
+    assert d.precision == _native.PIV_FP64 and d.scaling_par == 2.0 and d.dt == 0.3
+    assert piv._prepare_frame(np.zeros((256, 256), np.float64)).dtype == np.float64
+    assert piv._prepare_frame(np.zeros((256, 256), np.int32)).dtype == np.float64
+    assert gp.PIVGPU((256, 256), 16, dtype_f=np.float32).fp64        # not the string: FP64, as the reference
+    p32 = gp.PIVGPU((256, 256), 16, dtype_f="float32")
+    assert not p32.fp64 and p32._prepare_frame(np.zeros((256, 256), np.float64)).dtype == np.float32
+    assert p32._plan_desc(_native.PIV_U8).precision == _native.PIV_FP32
+    with pytest.raises(NotImplementedError, match="predictor"):
+        gp.PIVGPU((256, 256), 16, dtype_f="float64", predictor=object())
 
 
 def test_mask_settings_on_the_host():
