@@ -145,6 +145,13 @@ int piv_plan_set_mask(piv_plan *plan, const uint8_t *mask, int height, int width
 /* Iterations of the weighted smoothing after pass k in the last run (0 when
  * the pass was smoothed unweighted or not at all). */
 int piv_plan_last_smoothing_iterations(const piv_plan *plan, int pass, int32_t *iterations);
+/* Signal-to-noise ratio of the last pass of the last run (field_m x
+ * field_n host values, float32 or float64 per the plan's precision) with
+ * that pass's s2n_method / s2n_size, 0 at masked vectors.  Replaces
+ * PIVGPU.s2n_ratio -> CorrelationGPU.s2n_ratio (gpu_process.py:864-867,
+ * 1312-1376): recomputed on request from the frames the last pass read,
+ * which the plan still holds.  Synchronises. */
+int piv_plan_last_s2n(piv_plan *plan, void *s2n_out);
 
 /* ---- stage entry points (device pointers, caller-owned; stream may be
  * NULL for the null stream) -------------------------------------------- */

@@ -1031,6 +1031,63 @@ int piv_plan_last_smoothing_iterations(const piv_plan* P, int pass, int32_t* ite
   return PIV_OK;
 }
 
+int piv_plan_last_s2n(piv_plan* P, void* s2n_out) {
+  if (!P || !s2n_out) return fail(PIV_E_INVALID, "null argument");
+  if (P->runs == 0 && P->lruns == 0) return fail(PIV_E_INVALID, "no pair has been processed yet");
+  HIP_TRY(hipSetDevice(P->device));
+  hipStream_t s = P->stream;
+  HIP_TRY(hipStreamSynchronize(s));
+  const int k = (int)P->passes.size() - 1;
+  PassBuffers& B = P->passes[k];
+  const piv_plan_desc& D = P->desc;
+  // the last pass read the raw (masked) frames when it is the only one, the
+  // deformed frames otherwise; both are still in the plan's buffers
+  const bool raw = k == 0;
+  const void* fa = raw ? src_a(P) : P->def_a;
+  const void* fb = raw ? src_b(P) : P->def_b;
+  const int dtype = raw ? D.in_dtype : (P->fp64 ? kF64 : kF32);
+  void *tu = nullptr, *tv = nullptr, *ts = nullptr;
+  const size_t bytes = (size_t)B.N * P->esize;
+  int r = PIV_OK;
+  if (hipMalloc(&tu, bytes) != hipSuccess || hipMalloc(&tv, bytes) != hipSuccess || hipMalloc(&ts, bytes) != hipSuccess)
+    r = fail(PIV_E_NOMEM, "device allocation failed");
+  hipError_t e = hipSuccess;
+  if (!r) {
+    if (P->fp64) {
+      CorrArgs64 a = corr_args64(P, fa, fb, dtype, B.d);
+      if (!raw) {
+        a.up = as<double>(B.up);
+        a.vp = as<double>(B.vp);
+      }
+      a.u = as<double>(tu);
+      a.v = as<double>(tv);
+      a.s2n = as<double>(ts);
+      e = launch_correlate64(a, s);
+      if (e == hipSuccess && B.fmask)
+        e = launch_apply_field_mask(as<double>(tu), as<double>(tv), as<double>(ts), B.fmask, B.N, s);
+    } else {
+      CorrArgs a = corr_args(fa, fb, dtype, D.height, D.width, B.d);
+      if (!raw) {
+        a.up = B.up;
+        a.vp = B.vp;
+      }
+      a.u = as<float>(tu);
+      a.v = as<float>(tv);
+      a.s2n = as<float>(ts);
+      e = launch_correlate(a, s);
+      if (e == hipSuccess && B.fmask)
+        e = launch_apply_field_mask(as<float>(tu), as<float>(tv), as<float>(ts), B.fmask, B.N, s);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(s2n_out, ts, bytes, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  for (void* p : {tu, tv, ts})
+    if (p) (void)hipFree(p);
+  if (r) return r;
+  if (e != hipSuccess) return fail(PIV_E_HIP, "last_s2n: %s [%s]", hipGetErrorString(e), launch_detail());
+  return PIV_OK;
+}
+
 int piv_plan_wait_event(piv_plan* P, void* event) {
   if (!P || !event) return fail(PIV_E_INVALID, "null argument");
   P->wait_next = (hipEvent_t)event;

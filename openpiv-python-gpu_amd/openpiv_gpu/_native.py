@@ -39,6 +39,7 @@ EXPORTED_SYMBOLS = (
     "piv_batch_pair_shape", "piv_batch_get", "piv_tiff_info", "piv_tiff_read", "piv_host_alloc", "piv_host_free",
     "piv_memcpy_async", "piv_stream_wait_event", "piv_plan_wait_event", "piv_plan_set_mask",
     "piv_plan_last_smoothing_iterations", "piv_stage_smoothn_masked", "piv_stage_correlate64",
+    "piv_plan_last_s2n",
 )
 
 SYNTH_MAX_VORTICES = 8
@@ -151,6 +152,7 @@ def _bind(lib):
         "piv_stage_correlate64": (i32, [vp, vp, i32, i32, i32, ctypes.POINTER(PassDesc), vp, vp, vp, vp, vp, vp, vp,
                                         vp, vp]),
         "piv_plan_set_mask": (i32, [vp, vp, i32, i32]),
+        "piv_plan_last_s2n": (i32, [vp, vp]),
         "piv_plan_last_smoothing_iterations": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int32)]),
         "piv_stage_correlate_dft": (i32, [vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
         "piv_synth_pair": (i32, [ctypes.POINTER(SynthField), ctypes.POINTER(SynthModel), ctypes.c_uint64, i32, i32,
@@ -407,6 +409,12 @@ class Plan:
             return
         m = np.ascontiguousarray(np.asarray(mask).astype(bool).view(np.uint8))
         check(self._lib.piv_plan_set_mask(self.handle, m.ctypes.data, int(m.shape[0]), int(m.shape[1])))
+
+    def last_s2n(self, out):
+        """Signal-to-noise ratio of the last pass of the last run into ``out``
+        (a C-contiguous host array of the field shape and plan precision)."""
+        check(self._lib.piv_plan_last_s2n(self.handle, out.ctypes.data))
+        return out
 
     def last_smoothing_iterations(self, k: int) -> int:
         out = ctypes.c_int32()

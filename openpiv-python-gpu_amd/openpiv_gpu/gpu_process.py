@@ -441,7 +441,14 @@ class PIVGPU:
 
     @property
     def s2n_ratio(self):
-        raise NotImplementedError("s2n_ratio: signal-to-noise output is not implemented on the MI355X path")
+        """Signal-to-noise ratio of the last pass of the last call with that
+        pass's ``s2n_method`` / ``s2n_size``, 0 inside the mask
+        (gpu_process.py:864-867, 1312-1376; unlike the reference it does not
+        turn into ``None`` on a second access)."""
+        if self._plan is None:
+            return None
+        m, n = self.piv_fields[-1].field_shape
+        return self._plan.last_s2n(np.empty((m, n), self.dtype_f))
 
     def smoothing_log10_s(self, k):
         """log10 of the smoothing parameter chosen after pass ``k`` of the last call."""
