@@ -111,13 +111,19 @@ def learned_vs_plain(n_pairs, weights_path):
     learned = PIVGPU(shape, 16, predictor=net, max_reject_fraction=0.05, **kw)
     # the network's field feeding the 32 px pass instead (only the 64 px pass skipped)
     learned32 = PIVGPU(shape, 16, predictor=net, max_reject_fraction=0.05, predictor_pass=1, **kw)
+    # the network's field feeding two 16 px passes (schedule 64->32->16->16, the
+    # first two skipped; the fallback runs all four)
+    kw2 = dict(kw, search_size_iters=(1, 1, 2))
+    learned16x2 = PIVGPU(shape, 16, predictor=net, max_reject_fraction=0.05, predictor_pass=2, **kw2)
     pp = plain._get_plan(_native.PIV_U16)
     lp = learned._get_plan(_native.PIV_U16)
     lp32 = learned32._get_plan(_native.PIV_U16)
+    lp16x2 = learned16x2._get_plan(_native.PIV_U16)
     m, n = plain.piv_fields[-1].field_shape
     x, y = (np.asarray(c, np.float64) for c in plain.coords)
     frames = _native.DeviceBuffer((n_pairs, 2) + shape, np.uint16)
-    outs = {k: _native.DeviceBuffer((n_pairs, 2, m, n), np.float32) for k in ("plain", "learned", "learned32")}
+    outs = {k: _native.DeviceBuffer((n_pairs, 2, m, n), np.float32)
+            for k in ("plain", "learned", "learned32", "learned16x2")}
     gen = synthetic.DevicePairGenerator(shape, synthetic.MODEL_12BIT)
     seeds = [200000 + i for i in range(n_pairs)]
     log(f"rendering {n_pairs} held-out vortex pairs")
@@ -132,7 +138,7 @@ def learned_vs_plain(n_pairs, weights_path):
         plan.run(f.view(0).value, f.view(1).value, True, o.view(0).value, o.view(1).value, None, None, True)
 
     res = {}
-    for key, plan in (("plain", pp), ("learned", lp), ("learned32", lp32)):
+    for key, plan in (("plain", pp), ("learned", lp), ("learned32", lp32), ("learned16x2", lp16x2)):
         for w in range(3):
             run(plan, key, w % n_pairs)
         fallbacks = 0
@@ -167,6 +173,13 @@ def learned_vs_plain(n_pairs, weights_path):
         "learned_worst_pair_rms_px": round(res["learned"]["worst_pair_rms_px"], 5),
         "fallback_fraction": round(res["learned"]["fallbacks"] / n_pairs, 4),
         "max_reject_fraction": 0.05,
+        "two_16px_passes": {
+            "pairs_per_s": round(res["learned16x2"]["pairs_per_s"], 2),
+            "speedup": round(res["learned16x2"]["pairs_per_s"] / res["plain"]["pairs_per_s"], 3),
+            "rms_px_vs_truth": round(res["learned16x2"]["rms_px_vs_truth"], 5),
+            "worst_pair_rms_px": round(res["learned16x2"]["worst_pair_rms_px"], 5),
+            "fallback_fraction": round(res["learned16x2"]["fallbacks"] / n_pairs, 4),
+        },
         "skip_64_only": {
             "pairs_per_s": round(res["learned32"]["pairs_per_s"], 2),
             "speedup": round(res["learned32"]["pairs_per_s"] / res["plain"]["pairs_per_s"], 3),
@@ -289,6 +302,44 @@ def tiff_stream(fa_dev_pairs, n_pairs, workdir):
             "note": "2048x2048 uint16 TIFF pairs from disk (page cache), decode + PCIe + compute + results to host"}
 
 
+def variant_rates(fa, fb, disp, steps):
+    """Device-resident pairs/s of the same 2048x2048 pair with a frame mask
+    (disk of radius 300 px, 6.7 % of the frame) and with FP64 processing,
+    plus their RMS against the known displacement (outside the mask)."""
+    from openpiv_gpu import _native
+    from openpiv_gpu.gpu_process import PIVGPU
+    yy, xx = np.mgrid[0:2048, 0:2048]
+    mask = (yy - 1000) ** 2 + (xx - 900) ** 2 < 300 ** 2
+    out = {}
+    for name, kw in (("masked", dict(mask=mask)), ("fp64", dict(dtype_f="float64"))):
+        piv = PIVGPU((2048, 2048), 16, search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2,
+                     **({"dtype_f": "float32"} | kw))
+        plan = piv._get_plan(_native.PIV_U16)
+        m, n = piv.piv_fields[-1].field_shape
+        fd = piv.dtype_f
+        da, db = _native.DeviceBuffer.from_array(fa), _native.DeviceBuffer.from_array(fb)
+        u, v = _native.DeviceBuffer((m, n), fd), _native.DeviceBuffer((m, n), fd)
+        for _ in range(3):
+            plan.run(da.value, db.value, True, u.value, v.value, None, None, True)
+        _native.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan.run(da.value, db.value, True, u.value, v.value, None, None, True)
+        _native.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        x, y = piv.coords
+        ut, vt = disp(np.asarray(x, np.float64), np.asarray(y, np.float64))
+        uh, vh = u.get().astype(np.float64), v.get().astype(np.float64)
+        ok = np.isfinite(uh) & np.isfinite(vh)
+        rms = float(np.sqrt(np.mean((uh[ok] - ut[ok]) ** 2 + (vh[ok] - vt[ok]) ** 2) / 2))
+        rec = {"pairs_per_s": round(1e3 / ms, 2), "ms_per_pair": round(ms, 4), "rms_px_vs_truth": round(rms, 5)}
+        if name == "masked":
+            rec["masked_vectors"] = int((~ok).sum())
+            rec["smoothing_iterations"] = [plan.last_smoothing_iterations(k) for k in range(2)]
+        out[name] = rec
+    return out
+
+
 def cpu_baseline(fa, fb, pairs=1):
     from oracle import piv_oracle as O
     s = O.OracleSettings(min_search_size=16, search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2.0)
@@ -314,6 +365,7 @@ def main():
     ap.add_argument("--workload", choices=("pair2048", "uneven"), default="pair2048")
     ap.add_argument("--stream-pairs", type=int, default=48)
     ap.add_argument("--pairs", type=int, default=256)
+    ap.add_argument("--variant-steps", type=int, default=20)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -397,6 +449,11 @@ def main():
         with tempfile.TemporaryDirectory() as tmp:
             stream = tiff_stream([(fa, fb)], args.stream_pairs, tmp)
 
+    variants = None
+    if args.variant_steps > 0:
+        log("masked and FP64 variants")
+        variants = variant_rates(fa, fb, disp, args.variant_steps)
+
     learned = None
     if not args.skip_learned and args.learned_pairs > 0:
         from openpiv_gpu.learned import default_weights_path
@@ -424,6 +481,7 @@ def main():
             "frac": round(deform["gbs"] / 8000.0, 4), "avg_launch_ms": round(deform["avg_ms"], 5),
             "per_launch_bytes": deform["bytes"]},
         "learned_vs_plain": learned,
+        "variants": variants,
         "tiff_stream": stream,
         "cpu_baseline": cpu,
     }

@@ -134,11 +134,11 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvGeom g, InView in, co
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[m][t] = float4_t{0.f, 0.f, 0.f, 0.f};
   const bf16x4_t zero4 = {0, 0, 0, 0};
-  for (int kc = wv; kc < 9 * nc; kc += 4) {
+  // operands of K chunk kc: TM weight fragments and 4 input fragments
+  auto load_chunk = [&](int kc, bf16x4_t (&a)[TM], bf16x4_t (&b)[4]) {
     const int tap = kc / nc, c = (kc - tap * nc) * 16;
     const int ky = tap / 3, kx = tap - 3 * ky;
     const int cc = c + 4 * gq;
-    bf16x4_t a[TM], b[4];
 #pragma unroll
     for (int m = 0; m < TM; ++m) a[m] = okA[m] ? load_bf4(w + ((long)coA[m] * 9 + tap) * ci + cc) : zero4;
 #pragma unroll
@@ -148,10 +148,31 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvGeom g, InView in, co
                                              px[t].x * g.stride + kx - 1, &o0, &o1);
       b[t] = !ok ? zero4 : (c < in.c0 ? load_bf4(in.s0 + o0 + cc) : load_bf4(in.s1 + o1 + cc - in.c0));
     }
+  };
+  // chunks kc = wv, wv + 4, ... of this wave, in groups of G whose loads are
+  // all issued before the group's MFMAs (G chunks of latency in flight at once)
+  constexpr int G = 4;
+  const int K = 9 * nc;
+  for (int kc0 = wv; kc0 < K; kc0 += 4 * G) {
+    bf16x4_t a[G][TM], b[G][4];
 #pragma unroll
-    for (int m = 0; m < TM; ++m)
+    for (int q = 0; q < G; ++q) {
+      const int kc = kc0 + 4 * q;
+      if (kc < K) {
+        load_chunk(kc, a[q], b[q]);
+      } else {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) acc[m][t] = mfma(a[m], b[t], acc[m][t]);
+        for (int m = 0; m < TM; ++m) a[q][m] = zero4;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) b[q][t] = zero4;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q)
+#pragma unroll
+      for (int m = 0; m < TM; ++m)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[m][t] = mfma(a[q][m], b[q][t], acc[m][t]);
   }
   if (wv > 0) {
 #pragma unroll

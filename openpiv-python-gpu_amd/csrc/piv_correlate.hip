@@ -74,7 +74,9 @@ size_t correlate_lds_bytes(int fsize, int ss) {
 // BATCHED: one launch over the windows of many fields (pairs); block b
 // belongs to the last field whose block_prefix entry is <= b, so every block
 // works on one field and the field's arguments stay wave-uniform.
-template <int F, typename InT, bool BATCHED>
+// EXTRA: the S2N epilogue and the interrogation window inside the search
+// window are compiled in (launches that need neither use the lean variant).
+template <int F, typename InT, bool BATCHED, bool EXTRA>
 __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_in, CorrBatch batch,
                                                                      int n_windows_in, int region_floats) {
   using L = Layout<F>;
@@ -122,7 +124,7 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
   const int W = a.ss;
   const int pad = (F - W) / 2;
   const int SW = W + 1;
-  const int wp = a.win_pad;  // (search - interrogation window) / 2
+  const int wp = EXTRA ? a.win_pad : 0;  // (search - interrogation window) / 2
 
   float* region = smem + (size_t)wl * region_floats;
   float* stA = region;
@@ -441,7 +443,7 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
   // ---- signal-to-noise ratio (CorrelationGPU.get_s2n_ratio,
   // gpu_process.py:1312-1365) on the same fftshift-ed plane ---------------
   float s2n = 0.f;
-  if (a.s2n) {  // uniform over the launch
+  if (EXTRA && a.s2n) {  // uniform over the launch
     const float c1 = best.v;
     const int ip1 = best.i / F, jp1 = best.i % F;
     float c2;
@@ -512,7 +514,7 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
   }
 
   if (t == 0 && valid) {
-    if (a.s2n) a.s2n[win] = s2n;
+    if (EXTRA && a.s2n) a.s2n[win] = s2n;
     const int ip = best.i / F;
     const int jp = best.i % F;
     float isp = (float)ip, jsp = (float)jp;
@@ -554,11 +556,11 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
 
 // Dynamic LDS above 64 KiB must be opted into; request exactly what a
 // launch needs (gfx950 allows up to 160 KiB per workgroup).
-template <int F, typename InT, bool BATCHED>
+template <int F, typename InT, bool BATCHED, bool EXTRA>
 static hipError_t lds_attr(size_t lds) {
   static thread_local size_t attr_bytes = 0;
   if (lds > 65536 && lds > attr_bytes) {
-    hipError_t e = hipFuncSetAttribute((const void*)corr_kernel<F, InT, BATCHED>,
+    hipError_t e = hipFuncSetAttribute((const void*)corr_kernel<F, InT, BATCHED, EXTRA>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {
       set_launch_detail("hipFuncSetAttribute(MaxDynamicSharedMemorySize)", e);
@@ -577,10 +579,15 @@ static hipError_t launch_f(const CorrArgs& a, hipStream_t stream) {
   const size_t lds = region * L::WPB;
   const int blocks = (n_windows + L::WPB - 1) / L::WPB;
   (void)hipGetLastError();  // do not report an earlier, unrelated error
-  hipError_t e = lds_attr<F, InT, false>(lds);
+  const bool extra = a.s2n != nullptr || a.win_pad != 0;
+  hipError_t e = extra ? lds_attr<F, InT, false, true>(lds) : lds_attr<F, InT, false, false>(lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((corr_kernel<F, InT, false>), dim3(blocks), dim3(256), lds, stream, a, CorrBatch{}, n_windows,
-                     (int)(region / sizeof(float)));
+  if (extra)
+    hipLaunchKernelGGL((corr_kernel<F, InT, false, true>), dim3(blocks), dim3(256), lds, stream, a, CorrBatch{},
+                       n_windows, (int)(region / sizeof(float)));
+  else
+    hipLaunchKernelGGL((corr_kernel<F, InT, false, false>), dim3(blocks), dim3(256), lds, stream, a, CorrBatch{},
+                       n_windows, (int)(region / sizeof(float)));
   e = hipGetLastError();
   if (e != hipSuccess) set_launch_detail("corr_kernel launch", e);
   return e;
@@ -592,9 +599,9 @@ static hipError_t launch_fb(const CorrBatch& b, int total_blocks, int max_ss, hi
   const size_t region = correlate_lds_bytes(F, max_ss);
   const size_t lds = region * L::WPB;
   (void)hipGetLastError();
-  hipError_t e = lds_attr<F, InT, true>(lds);
+  hipError_t e = lds_attr<F, InT, true, true>(lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((corr_kernel<F, InT, true>), dim3(total_blocks), dim3(256), lds, stream, CorrArgs{}, b, 0,
+  hipLaunchKernelGGL((corr_kernel<F, InT, true, true>), dim3(total_blocks), dim3(256), lds, stream, CorrArgs{}, b, 0,
                      (int)(region / sizeof(float)));
   e = hipGetLastError();
   if (e != hipSuccess) set_launch_detail("corr_kernel (batched) launch", e);
