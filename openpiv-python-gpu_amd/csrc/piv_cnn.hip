@@ -107,11 +107,11 @@ __device__ __forceinline__ float in_value(const InView& v, bool ok, long o0, lon
 // and are summed in LDS, so even the 32 x 32 layers of a single image spread
 // over enough blocks.  ci0 and ci1 are multiples of 16 (4-channel vector loads).
 // grid: x = ceil(pixels / 64), y = ceil(co / (16 TM)).
-template <int TM>
-__global__ __launch_bounds__(256) void conv_fwd_kernel(ConvGeom g, InView in, const uint16_t* __restrict__ w,
-                                                       const float* __restrict__ bias, void* out, int out_mode,
-                                                       int relu) {
-  __shared__ float red[3][TM][4][64][4];
+template <int TM, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void conv_fwd_kernel(ConvGeom g, InView in, const uint16_t* __restrict__ w,
+                                                           const float* __restrict__ bias, void* out, int out_mode,
+                                                           int relu) {
+  __shared__ float red[NW - 1][TM][4][64][4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = lane & 15, gq = lane >> 4;
   const long total = (long)g.batch * g.ho * g.wo;
@@ -149,15 +149,15 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvGeom g, InView in, co
       b[t] = !ok ? zero4 : (c < in.c0 ? load_bf4(in.s0 + o0 + cc) : load_bf4(in.s1 + o1 + cc - in.c0));
     }
   };
-  // chunks kc = wv, wv + 4, ... of this wave, in groups of G whose loads are
+  // chunks kc = wv, wv + NW, ... of this wave, in groups of G whose loads are
   // all issued before the group's MFMAs (G chunks of latency in flight at once)
   constexpr int G = 4;
   const int K = 9 * nc;
-  for (int kc0 = wv; kc0 < K; kc0 += 4 * G) {
+  for (int kc0 = wv; kc0 < K; kc0 += NW * G) {
     bf16x4_t a[G][TM], b[G][4];
 #pragma unroll
     for (int q = 0; q < G; ++q) {
-      const int kc = kc0 + 4 * q;
+      const int kc = kc0 + NW * q;
       if (kc < K) {
         load_chunk(kc, a[q], b[q]);
       } else {
@@ -196,7 +196,10 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvGeom g, InView in, co
       for (int e = 0; e < 4; ++e) {
         const int co = co0 + e;
         if (co >= g.co) continue;
-        float val = acc[m][t][e] + red[0][m][t][lane][e] + red[1][m][t][lane][e] + red[2][m][t][lane][e] + bias[co];
+        float val = acc[m][t][e];
+#pragma unroll
+        for (int q = 0; q < NW - 1; ++q) val += red[q][m][t][lane][e];
+        val += bias[co];
         if (relu) val = fmaxf(val, 0.f);
         if (out_mode == kOutBf16) {
           static_cast<uint16_t*>(out)[p * g.co + co] = f2bf(val);
@@ -720,10 +723,17 @@ hipError_t launch_conv_fwd(const ConvGeom& g, const uint16_t* s0, int c0, int up
     const long cols = (pixels + 63) / 64;
     if (g.co > 16 && cols * ((g.co + 31) / 32) >= 512) {
       const dim3 grid((unsigned)cols, (unsigned)((g.co + 31) / 32));
-      hipLaunchKernelGGL(conv_fwd_kernel<2>, grid, dim3(256), 0, s, g, in, w, bias, out, out_mode, relu);
+      if (cols * ((g.co + 31) / 32) < 1024)
+        hipLaunchKernelGGL((conv_fwd_kernel<2, 8>), grid, dim3(512), 0, s, g, in, w, bias, out, out_mode, relu);
+      else
+        hipLaunchKernelGGL(conv_fwd_kernel<2>, grid, dim3(256), 0, s, g, in, w, bias, out, out_mode, relu);
     } else {
+      // fewer tiles than CUs x 2: eight waves split each tile's K
       const dim3 grid((unsigned)cols, (unsigned)((g.co + 15) / 16));
-      hipLaunchKernelGGL(conv_fwd_kernel<1>, grid, dim3(256), 0, s, g, in, w, bias, out, out_mode, relu);
+      if (cols * ((g.co + 15) / 16) < 512)
+        hipLaunchKernelGGL((conv_fwd_kernel<1, 8>), grid, dim3(512), 0, s, g, in, w, bias, out, out_mode, relu);
+      else
+        hipLaunchKernelGGL(conv_fwd_kernel<1>, grid, dim3(256), 0, s, g, in, w, bias, out, out_mode, relu);
     }
   } else {
     const dim3 grid(blocks_for(pixels), (unsigned)((g.co + 31) / 32));
