@@ -54,7 +54,8 @@ struct Layout {
   static constexpr int SR = H + 1;         // R row stride (complex)
   static constexpr int SP = F + 1;         // plane row stride (float)
   // occupancy target (waves per SIMD) the register allocation is held to
-  static constexpr int WAVES = F <= 16 ? 6 : (F == 32 ? 4 : (F == 64 ? 2 : 1));
+  // (measured best per size; F=128 fits two 66.5 KB windows per CU)
+  static constexpr int WAVES = F <= 32 ? 8 : (F == 64 ? 4 : 2);
 };
 
 }  // namespace
@@ -202,7 +203,11 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
   c32 x[H];
   {
     const int q = t >> 1;
-    const int h = t & 1;
+    // Opaque copy of the lane bit: the per-register bounds tests below and in
+    // the column stage would otherwise be shared between the stages and held
+    // in (spilled) lane masks across the barriers.
+    int h = t & 1;
+    asm volatile("" : "+v"(h));
     const bool active = q < W;
     const int rr = active ? q : 0;
     static_for<0, H>([&](auto M) {
@@ -222,11 +227,23 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
     const int q = t >> 1;
     const bool up1 = t & 1;
     fft_reg<H, -1>(x);
-    dit2_combine<F, -1>(x, up1);
+    // the last DIT stage across the lane pair (dit2_combine) is fused with
+    // the split, so each step holds only the two registers it splits
+    auto combine = [&](auto KK) {
+      constexpr int K = decltype(KK)::value;
+      constexpr int r = bitrev(K, H);
+      const c32 own = x[r];
+      const c32 tw = twiddle<K, F, -1>(own);
+      const c32 v = up1 ? tw : own;
+      const c32 recv = swap1(v);
+      const c32 sv = up1 ? c32{-v.x, -v.y} : v;
+      x[r] = c32{recv.x + sv.x, recv.y + sv.y};
+    };
     // split: lane 0 -> A slots, lane 1 -> B slots; slot 0 packs (DC, Nyquist)
     const bool active = q < W;
     c32* row = AB + (size_t)(active ? q : 0) * SF + (up1 ? H : 0);
     {
+      combine(std::integral_constant<int, 0>{});
       c32 o = x[0];
       c32 w = swap1(o);
       c32 z0 = up1 ? w : o;  // Zr[0]
@@ -243,6 +260,8 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
       constexpr int kx2 = H - kx;
       constexpr int r1 = bitrev(kx, H);
       constexpr int r2 = bitrev(kx2, H);
+      combine(std::integral_constant<int, kx>{});
+      if constexpr (kx2 != kx) combine(std::integral_constant<int, kx2>{});
       const c32 o1 = x[r1], w1 = swap1(o1);
       const c32 o2 = x[r2], w2 = swap1(o2);
       // slot kx : Zr[kx] = (o1 | w1), Zr[F-kx] = (w2 | o2)
@@ -264,7 +283,9 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
   // ---- stage C: column FFTs (quad = {A h0, A h1, B h0, B h1} per kx) ------
   const int kx = t >> 2;
   const int sel = (t >> 1) & 1;  // 0 -> A column, 1 -> B column
-  const bool upper = t & 1;
+  int upper_bit = t & 1;
+  asm volatile("" : "+v"(upper_bit));  // see the row stage
+  const bool upper = upper_bit;
   {
     const int col = sel * H + kx;
     static_for<0, H>([&](auto M) {

@@ -98,7 +98,9 @@ hipError_t gemm2(const R* a0, const R* a1, int lda, int ta, const R* b0, const R
 //   phase 3, 4: safeguarded Newton steps on dg/dp = 0
 // then a separate launch scales the spectrum by Gamma(p).
 // ---------------------------------------------------------------------------
-constexpr int kGcvBlocks = 128;  // the decide kernel reduces 8 lanes x 16 partials
+constexpr int kGcvBlocks = 128;
+constexpr int kGroups = 8;  // partials are reduced in 8 groups of 16 blocks, then over the groups
+constexpr int kGroupBlocks = kGcvBlocks / kGroups;
 constexpr int kSpan = 10;  // N_P0, gpu_smoothn.py:33
 constexpr int kGrid = 64;
 
@@ -110,6 +112,10 @@ struct GcvState {
   int done;             // minimum fixed at a bound / by the grid
   double p;             // current estimate
   double p_min, p_max;
+  // blocks / groups of the current evaluation that finished (0 between launches)
+  uint32_t grp_arrivals[kGroups];
+  uint32_t top_arrivals;
+  alignas(16) double gsum[kGrid * 6 * kGroups];  // group totals, row (p-value, sum) then group
 };
 
 // Sums per p-value: RSS, T, RSS', T', RSS'', T'' (L = ln 10, e = 1 - Gamma):
@@ -143,15 +149,24 @@ __device__ double g_of(const double* sm, int N) {
   return sm[0] / N / (d * d) / 2.0;
 }
 
+template <int PHASE>
+__device__ void gcv_decide(int N, GcvState* st, double p_min, double p_max, double (*tot)[6]);
+
 // One evaluation: every block scores the pending p-values over its slice of
-// the spectrum and writes per-block partials (p-major).  No cross-block
-// communication inside the kernel; the decide kernel runs after it.
-template <int NS, typename R>
-__global__ __launch_bounds__(256) void gcv_eval_kernel(const R* Y, const double* lam, int N, const GcvState* st,
-                                                       double* part, int span, double p_min, double p_max,
+// the spectrum and writes per-block partials (p-major).  The last block of
+// each group of 16 to finish sums the group's partials in block order; the
+// last group to finish sums the group totals in group order and decides the
+// next p-values (fixed summation order: replays are bitwise identical).
+template <int PHASE, typename R>
+__global__ __launch_bounds__(256) void gcv_eval_kernel(const R* Y, const double* lam, int N, GcvState* st,
+                                                       double* part, double p_min, double p_max,
                                                        const int32_t* skip) {
   if (skip && *skip) return;
+  constexpr int NS = (PHASE >= 3) ? 6 : 4;
+  constexpr int span = PHASE == 0;
   __shared__ double red[256][NS];
+  __shared__ double tot[kGrid][6];
+  __shared__ int is_last;
   const int tid = threadIdx.x;
   const int pi = tid % 64, sub = tid / 64;
   // the first evaluation scores the reference's 10-point span directly
@@ -195,17 +210,44 @@ __global__ __launch_bounds__(256) void gcv_eval_kernel(const R* Y, const double*
       const double v = red[tid][q] + red[tid + 64][q] + red[tid + 128][q] + red[tid + 192][q];
       part[((size_t)tid * 6 + q) * kGcvBlocks + blockIdx.x] = v;
     }
+    __threadfence();
   }
+  __syncthreads();
+  const int grp = blockIdx.x / kGroupBlocks;
+  if (tid == 0) is_last = atomicAdd(&st->grp_arrivals[grp], 1u) == (unsigned)(kGroupBlocks - 1);
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  // group total: one (p-value, sum) row per thread, its 16 partials in flight
+  for (int r = tid; r < npv * NS; r += 256) {
+    const int pj = r / NS, q = r % NS;
+    const double2* src =
+        reinterpret_cast<const double2*>(part + ((size_t)pj * 6 + q) * kGcvBlocks + grp * kGroupBlocks);
+    double2 x[kGroupBlocks / 2];
+#pragma unroll
+    for (int e = 0; e < kGroupBlocks / 2; ++e) x[e] = src[e];
+    double sum = 0.0;
+#pragma unroll
+    for (int e = 0; e < kGroupBlocks / 2; ++e) sum += x[e].x + x[e].y;
+    st->gsum[((size_t)pj * 6 + q) * kGroups + grp] = sum;
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) is_last = atomicAdd(&st->top_arrivals, 1u) == (unsigned)(kGroups - 1);
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  if (tid < kGroups) st->grp_arrivals[tid] = 0;
+  if (tid == 0) st->top_arrivals = 0;
+  gcv_decide<PHASE>(N, st, p_min, p_max, tot);
 }
 
-// Reduces the partials of the last evaluation (block order: deterministic)
-// and picks the next p-values (phases as described above).
+// Sums the group totals of the last evaluation (group order) and picks the
+// next p-values (phases as described above).  Run by the last block of an
+// evaluation launch to finish.
 template <int PHASE>
-__global__ __launch_bounds__(1024) void gcv_decide_kernel(int N, GcvState* st, const double* part, double p_min,
-                                                          double p_max, const int32_t* skip) {
-  if (skip && *skip) return;
+__device__ void gcv_decide(int N, GcvState* st, double p_min, double p_max, double (*tot)[6]) {
   constexpr int NS = (PHASE >= 3) ? 6 : 4;
-  __shared__ double tot[kGrid][6];
   const int tid = threadIdx.x;
   if (PHASE == 0 && tid == 0) {
     st->npv = kSpan;
@@ -214,26 +256,18 @@ __global__ __launch_bounds__(1024) void gcv_decide_kernel(int N, GcvState* st, c
     st->p_max = p_max;
     st->done = 0;
   }
-  for (int j = tid; j < kGrid * 6; j += 1024) tot[j / 6][j % 6] = 0.0;
   __syncthreads();
   const int npv = st->npv;
-  // teams of 8 consecutive lanes reduce one (p-value, sum) row of 128
-  // partials: 16 per lane, all loads in flight, then a 3-step shuffle
-  const int team = tid >> 3, mate = tid & 7;
-  const int rows = npv * NS;
-  for (int r = team; r < rows; r += 128) {
+  for (int r = tid; r < npv * NS; r += blockDim.x) {
     const int pj = r / NS, q = r % NS;
-    const double2* src = reinterpret_cast<const double2*>(part + ((size_t)pj * 6 + q) * kGcvBlocks) + mate * 8;
-    double2 x[8];
+    const double2* src = reinterpret_cast<const double2*>(st->gsum + ((size_t)pj * 6 + q) * kGroups);
+    double2 x[kGroups / 2];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) x[e] = src[e];
-    double t = 0.0;
+    for (int e = 0; e < kGroups / 2; ++e) x[e] = src[e];
+    double sum = 0.0;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) t += x[e].x + x[e].y;
-    t += __shfl_xor(t, 4, 64);
-    t += __shfl_xor(t, 2, 64);
-    t += __shfl_xor(t, 1, 64);
-    if (mate == 0) tot[pj][q] = t;
+    for (int e = 0; e < kGroups / 2; ++e) sum += x[e].x + x[e].y;
+    tot[pj][q] = sum;
   }
   __syncthreads();
   if (tid >= 64) return;
@@ -356,16 +390,11 @@ template <typename R>
 void enqueue_gcv_search(const R* Y, const double* lam, int N, GcvState* st, double* part, double p_min,
                         double p_max, const int32_t* skip, hipStream_t stream) {
   const dim3 eb(kGcvBlocks), et(256);
-  hipLaunchKernelGGL((gcv_eval_kernel<4, R>), eb, et, 0, stream, Y, lam, N, st, part, 1, p_min, p_max, skip);
-  hipLaunchKernelGGL(gcv_decide_kernel<0>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
-  hipLaunchKernelGGL((gcv_eval_kernel<4, R>), eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
-  hipLaunchKernelGGL(gcv_decide_kernel<1>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
-  hipLaunchKernelGGL((gcv_eval_kernel<4, R>), eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
-  hipLaunchKernelGGL(gcv_decide_kernel<2>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
-  hipLaunchKernelGGL((gcv_eval_kernel<6, R>), eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
-  hipLaunchKernelGGL(gcv_decide_kernel<3>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
-  hipLaunchKernelGGL((gcv_eval_kernel<6, R>), eb, et, 0, stream, Y, lam, N, st, part, 0, p_min, p_max, skip);
-  hipLaunchKernelGGL(gcv_decide_kernel<4>, dim3(1), dim3(1024), 0, stream, N, st, part, p_min, p_max, skip);
+  hipLaunchKernelGGL((gcv_eval_kernel<0, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
+  hipLaunchKernelGGL((gcv_eval_kernel<1, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
+  hipLaunchKernelGGL((gcv_eval_kernel<2, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
+  hipLaunchKernelGGL((gcv_eval_kernel<3, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
+  hipLaunchKernelGGL((gcv_eval_kernel<4, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
 }
 
 }  // namespace
