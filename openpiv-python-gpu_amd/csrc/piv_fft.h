@@ -133,11 +133,12 @@ __device__ __forceinline__ void fft_reg(c32 (&x)[L]) {
 
 // ---------------------------------------------------------------------------
 // Cross-lane moves (DPP quad permutes; all four lanes of a quad must be
-// active).
+// active).  bound_ctrl is set: with every source lane valid it changes
+// nothing, and it lets the compiler fold the move into the consuming add.
 // ---------------------------------------------------------------------------
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
 }
 template <int CTRL>
 __device__ __forceinline__ c32 dpp_c(c32 v) {

@@ -291,6 +291,17 @@ __device__ __forceinline__ void sum256(double (&v)[K], double (*scratch)[4]) {
   __syncthreads();
 }
 
+// Outlier tests: the slot map of the global-memory solver, or the flags
+// themselves (a vector is an outlier exactly when it is flagged).
+struct SlotOutlier {
+  const int32_t* slot;
+  __device__ bool operator()(int l) const { return slot[l] >= 0; }
+};
+struct FlagOutlier {
+  const uint8_t* flags;
+  __device__ bool operator()(int l) const { return flags[l] != 0; }
+};
+
 // Right-hand side of outlier row k scaled by n_k (the in-bounds
 // 4-neighbour count): (n_k / c_k) times the sum of the neighbours that are
 // not outliers and not NaN, c_k counting the in-bounds neighbours that are
@@ -300,8 +311,8 @@ __device__ __forceinline__ void sum256(double (&v)[K], double (*scratch)[4]) {
 // decoupled, and the vector keeps its value (the reset of unresolved
 // vectors, gpu_process.py:829-832): its right-hand side is 0 here and
 // nan_isolated() tells the final write to skip it.
-template <typename R>
-__device__ __forceinline__ void spring_rhs(const R* u, const R* v, const int32_t* slot, int m, int n,
+template <typename R, typename Out>
+__device__ __forceinline__ void spring_rhs(const R* u, const R* v, Out is_out, int m, int n,
                                            const int (&ni)[4], const int (&nj)[4], double* bu, double* bv,
                                            int* nk) {
   double su = 0.0, sv = 0.0;
@@ -311,7 +322,7 @@ __device__ __forceinline__ void spring_rhs(const R* u, const R* v, const int32_t
     if (ni[e] < 0 || ni[e] >= m || nj[e] < 0 || nj[e] >= n) continue;
     ++cnt;
     const int l = ni[e] * n + nj[e];
-    if (slot[l] >= 0) {
+    if (is_out(l)) {
       ++cu;
       ++cv;
     } else {
@@ -331,9 +342,9 @@ __device__ __forceinline__ void spring_rhs(const R* u, const R* v, const int32_t
   *nk = cnt;
 }
 
-template <typename R>
-__device__ __forceinline__ void nan_isolated(const R* u, const R* v, const int32_t* slot, int m, int n,
-                                             int k, bool* ku, bool* kv) {
+template <typename R, typename Out>
+__device__ __forceinline__ void nan_isolated(const R* u, const R* v, Out is_out, int m, int n, int k, bool* ku,
+                                             bool* kv) {
   const int i = k / n, j = k % n;
   const int ni[4] = {i, i, i + 1, i - 1};
   const int nj[4] = {j - 1, j + 1, j, j};
@@ -342,7 +353,7 @@ __device__ __forceinline__ void nan_isolated(const R* u, const R* v, const int32
   for (int e = 0; e < 4; ++e) {
     if (ni[e] < 0 || ni[e] >= m || nj[e] < 0 || nj[e] >= n) continue;
     const int l = ni[e] * n + nj[e];
-    if (slot[l] >= 0) {
+    if (is_out(l)) {
       ++cu;
       ++cv;
     } else {
@@ -381,7 +392,7 @@ __device__ void spring_cg_global(R* u, R* v, int m, int n, int nv, const Replace
     nbrs(s, ni, nj);
     double bu = 0.0, bv = 0.0;
     int nk = 0;
-    spring_rhs(u, v, w.slot, m, n, ni, nj, &bu, &bv, &nk);
+    spring_rhs(u, v, SlotOutlier{w.slot}, m, n, ni, nj, &bu, &bv, &nk);
     xu[s] = 0.0;
     xv[s] = 0.0;
     ru[s] = bu;
@@ -461,7 +472,7 @@ __device__ void spring_cg_global(R* u, R* v, int m, int n, int nv, const Replace
   for (int s = tid; s < nv; s += T) {
     const int k = w.list[s];
     bool ku, kv;
-    nan_isolated(u, v, w.slot, m, n, k, &ku, &kv);
+    nan_isolated(u, v, SlotOutlier{w.slot}, m, n, k, &ku, &kv);
     if (!ku) u[k] = (R)xu[s];
     if (!kv) v[k] = (R)xv[s];
   }
@@ -470,10 +481,11 @@ __device__ void spring_cg_global(R* u, R* v, int m, int n, int nv, const Replace
 // LDS-resident variant of the same solve for up to kSpringLds outliers (the
 // common case): all CG vectors and the outlier neighbour lists live in LDS, so
 // an iteration touches no global memory.  The flags are compacted by the whole
-// 256-thread workgroup with 16-byte loads; up to 256 outliers are then solved
-// by wave 0 alone (shuffle reductions, no workgroup barriers).  CG starts
-// from x0 = D^-1 b (the mean of the valid neighbours), exact for isolated
-// outliers.
+// 256-thread workgroup with 16-byte loads into a row-major outlier list in
+// LDS (no slot map: a neighbour's slot is found by binary search in the
+// list); up to 256 outliers are then solved by wave 0 alone (shuffle
+// reductions, no workgroup barriers).  CG starts from x0 = D^-1 b (the mean
+// of the valid neighbours), exact for isolated outliers.
 constexpr int kSpringLdsThreads = 256;
 
 struct SpringLds {
@@ -481,7 +493,8 @@ struct SpringLds {
   double pu[kSpringLds], pv[kSpringLds], qu[kSpringLds], qv[kSpringLds];
   int16_t nb[kSpringLds][4];
   uint8_t nk[kSpringLds];
-  int32_t scan[kSpringLdsThreads];
+  int32_t list[kSpringLds];
+  int32_t scan[kSpringLdsThreads / 64];
   double red[4][4];
 };
 
@@ -608,26 +621,7 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* 
   const int per = (nchunk + T - 1) / T;
   const int c0 = tid * per, c1 = min(nchunk, c0 + per);
   const uint4* f4 = reinterpret_cast<const uint4*>(flags);
-  int local = 0;
-  for (int c = c0; c < c1; ++c) {
-    if (16 * c + 16 <= N) {
-      const uint4 q = f4[c];
-      local += __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);  // flags are 0/1 bytes
-    } else {
-      for (int k = 16 * c; k < N; ++k) local += flags[k] ? 1 : 0;
-    }
-  }
-  L.scan[tid] = local;
-  __syncthreads();
-  for (int off = 1; off < T; off <<= 1) {
-    const int add = tid >= off ? L.scan[tid - off] : 0;
-    __syncthreads();
-    L.scan[tid] += add;
-    __syncthreads();
-  }
-  int pos = L.scan[tid] - local;
-  for (int c = c0; c < c1; ++c) {
-    uint32_t words[4] = {0, 0, 0, 0};
+  auto chunk_words = [&](int c, uint32_t (&words)[4]) {
     if (16 * c + 16 <= N) {
       const uint4 q = f4[c];
       words[0] = q.x;
@@ -635,13 +629,44 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* 
       words[2] = q.z;
       words[3] = q.w;
     } else {
+      words[0] = words[1] = words[2] = words[3] = 0;
       for (int k = 16 * c; k < N; ++k) words[(k - 16 * c) >> 2] |= (uint32_t)flags[k] << (8 * ((k - 16 * c) & 3));
     }
+  };
+  int local = 0;
+  for (int c = c0; c < c1; ++c) {
+    uint32_t words[4];
+    chunk_words(c, words);
+    local += __popc(words[0]) + __popc(words[1]) + __popc(words[2]) + __popc(words[3]);  // 0/1 bytes
+  }
+  // exclusive prefix of the counts: wave scans, then the wave totals
+  const int lane = tid & 63, wave = tid >> 6;
+  int incl = local;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) L.scan[wave] = incl;
+  __syncthreads();
+  int pos = incl - local, total = 0;
+#pragma unroll
+  for (int k = 0; k < T / 64; ++k) {
+    pos += k < wave ? L.scan[k] : 0;
+    total += L.scan[k];
+  }
+  const bool in_lds = total <= kSpringLds;
+  for (int c = c0; c < c1; ++c) {
+    uint32_t words[4];
+    chunk_words(c, words);
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
       const int k = 16 * c + b;
-      if (k < N) {
-        if ((words[b >> 2] >> (8 * (b & 3))) & 0xff) {
+      const bool out = k < N && ((words[b >> 2] >> (8 * (b & 3))) & 0xff);
+      if (in_lds) {
+        if (out) L.list[pos++] = k;
+      } else if (k < N) {
+        if (out) {
           w.slot[k] = pos;
           w.list[pos] = k;
           ++pos;
@@ -652,14 +677,25 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* 
     }
   }
   __syncthreads();
-  if (nv > kSpringLds) {
-    spring_cg_global(u, v, m, n, nv, w, L.red);
+  if (!in_lds) {
+    spring_cg_global(u, v, m, n, total, w, L.red);
     return;
   }
+  const int nvl = total;
 
   // neighbour codes (slot, -1 valid, -2 outside) and x0 = b / n_k
-  for (int s = tid; s < nv; s += T) {
-    const int k = w.list[s];
+  const FlagOutlier is_out{flags};
+  auto slot_of = [&](int l) {  // l is in the (ascending) list
+    int lo = 0, hi = nvl - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (L.list[mid] < l) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  for (int s = tid; s < nvl; s += T) {
+    const int k = L.list[s];
     const int i = k / n, j = k % n;
     const int ni[4] = {i, i, i + 1, i - 1};
     const int nj[4] = {j - 1, j + 1, j, j};
@@ -669,29 +705,29 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* 
     for (int e = 0; e < 4; ++e) {
       int16_t code = -2;
       if (ni[e] >= 0 && ni[e] < m && nj[e] >= 0 && nj[e] < n) {
-        const int sl = w.slot[ni[e] * n + nj[e]];
-        code = sl >= 0 ? (int16_t)sl : (int16_t)-1;
+        const int l = ni[e] * n + nj[e];
+        code = is_out(l) ? (int16_t)slot_of(l) : (int16_t)-1;
       }
       L.nb[s][e] = code;
     }
-    spring_rhs(u, v, w.slot, m, n, ni, nj, &bu, &bv, &cnt);
+    spring_rhs(u, v, is_out, m, n, ni, nj, &bu, &bv, &cnt);
     L.nk[s] = (uint8_t)cnt;
     L.xu[s] = bu / cnt;
     L.xv[s] = bv / cnt;
   }
   __syncthreads();
-  if (nv <= 256) {
+  if (nvl <= 256) {
     if (tid >= 64) return;  // wave 0 finishes alone
-    spring_cg<64>(L, nv);
+    spring_cg<64>(L, nvl);
   } else {
-    spring_cg<T>(L, nv);
+    spring_cg<T>(L, nvl);
   }
   // all neighbour reads of nan_isolated() see pre-solve values: outlier
-  // neighbours are recognised by their slot, never read
-  for (int s = tid; s < nv; s += (nv <= 256 ? 64 : T)) {
-    const int k = w.list[s];
+  // neighbours are recognised by their flag, never read
+  for (int s = tid; s < nvl; s += (nvl <= 256 ? 64 : T)) {
+    const int k = L.list[s];
     bool ku, kv;
-    nan_isolated(u, v, w.slot, m, n, k, &ku, &kv);
+    nan_isolated(u, v, is_out, m, n, k, &ku, &kv);
     if (!ku) u[k] = (R)L.xu[s];
     if (!kv) v[k] = (R)L.xv[s];
   }

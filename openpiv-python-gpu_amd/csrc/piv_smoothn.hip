@@ -10,8 +10,9 @@
 //   RSS = sum_fields ||F (Gamma - 1)||^2,  T = sum Gamma      (:353-372)
 // and the search for p run in FP64 on the device: the reference's 10-point
 // span over [p_min, p_max] (:182-188), then, instead of host L-BFGS-B
-// (:191-193), two 64-point bracket refinements of dg/dp = 0 and two Newton
-// steps using the analytic first and second derivatives of RSS and T.
+// (:191-193), a 64-point bracket refinement of dg/dp = 0, a secant root
+// and two Newton steps using the analytic first and second derivatives of
+// RSS and T.
 #include <limits.h>
 #include <math.h>
 
@@ -87,22 +88,25 @@ hipError_t gemm2(const R* a0, const R* a1, int lda, int ta, const R* b0, const R
 
 // ---------------------------------------------------------------------------
 // GCV search.  Every evaluation launch spreads the m*n spectrum over
-// kGcvBlocks workgroups; thread t of a block evaluates p-value (t % 64) over
-// a quarter of the block's elements, so one launch scores up to 64 p-values.
-// A one-block decide launch then reduces the per-block partials (stored
-// p-major so each sum reads one contiguous run, summed in block order:
-// deterministic) and picks the next p-values:
-//   phase 0: the 10-point span of the reference, first argmin -> bracket
-//   phase 1, 2: 64 evenly spaced points in the bracket, pick the sign change
-//            of dg/dp nearest the span minimum -> 63x narrower bracket
-//   phase 3, 4: safeguarded Newton steps on dg/dp = 0
+// kGcvBlocks workgroups of 1024 threads; in the grid evaluation thread t of a
+// block scores p-value (t % 64) over a sixteenth of the block's elements, in
+// a Newton step every thread scores the one p-value.  The last block to finish reduces the per-block
+// partials (stored p-major so each sum reads one contiguous run, summed in a
+// fixed order: deterministic) and picks the next p-values:
+//   phase 0: 64 evenly spaced points over [p_min, p_max]; every seventh is a
+//            point of the reference's 10-point span.  First argmin of the
+//            span -> bracket between its neighbours; in it, the sign change
+//            of dg/dp nearest the span minimum (9x finer than the span) ->
+//            secant root
+//   phase 1, 2: safeguarded Newton steps on dg/dp = 0 from the secant root
 // then a separate launch scales the spectrum by Gamma(p).
 // ---------------------------------------------------------------------------
-constexpr int kGcvBlocks = 128;
-constexpr int kGroups = 8;  // partials are reduced in 8 groups of 16 blocks, then over the groups
-constexpr int kGroupBlocks = kGcvBlocks / kGroups;
+constexpr int kGcvBlocks = 64;
+constexpr int kGcvThreads = 1024;
 constexpr int kSpan = 10;  // N_P0, gpu_smoothn.py:33
 constexpr int kGrid = 64;
+constexpr int kSpanStep = (kGrid - 1) / (kSpan - 1);  // grid points per span interval
+static_assert((kGrid - 1) % (kSpan - 1) == 0, "span points must lie on the grid");
 
 struct GcvState {
   double pv[kGrid];     // p-values of the pending evaluation
@@ -112,10 +116,7 @@ struct GcvState {
   int done;             // minimum fixed at a bound / by the grid
   double p;             // current estimate
   double p_min, p_max;
-  // blocks / groups of the current evaluation that finished (0 between launches)
-  uint32_t grp_arrivals[kGroups];
-  uint32_t top_arrivals;
-  alignas(16) double gsum[kGrid * 6 * kGroups];  // group totals, row (p-value, sum) then group
+  uint32_t arrivals;    // blocks of the current evaluation that finished (0 between launches)
 };
 
 // Sums per p-value: RSS, T, RSS', T', RSS'', T'' (L = ln 10, e = 1 - Gamma):
@@ -149,43 +150,135 @@ __device__ double g_of(const double* sm, int N) {
   return sm[0] / N / (d * d) / 2.0;
 }
 
+// p-value of grid point j of the first evaluation; span points use the
+// span's own formula (i / 9 of the range above p_min).
+__device__ __forceinline__ double grid_p(int j, double p_min, double p_max) {
+  if (j % kSpanStep == 0) return (double)(j / kSpanStep) * (1.0 / (kSpan - 1)) * (p_max - p_min) + p_min;
+  return (double)j * (1.0 / (kGrid - 1)) * (p_max - p_min) + p_min;
+}
+
+// First (index) minimum of a wave-distributed value.
+__device__ __forceinline__ void wave_argmin(double& v, int& i) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double vo = __shfl_xor(v, off, 64);
+    const int io = __shfl_xor(i, off, 64);
+    if (vo < v || (vo == v && io < i)) {
+      v = vo;
+      i = io;
+    }
+  }
+}
+
+// Picks the next p-value from the sums tot[p-value][sum] of the evaluation
+// that just finished (phases as described above); wave 0 of the last block.
 template <int PHASE>
-__device__ void gcv_decide(int N, GcvState* st, double p_min, double p_max, double (*tot)[6]);
+__device__ void gcv_decide(int N, GcvState* st, double p_min, double p_max, const double (*tot)[6]) {
+  const int lane = threadIdx.x;
+  const int npv = PHASE == 0 ? kGrid : st->npv;
+  const bool done = PHASE == 0 ? false : st->done != 0;
+  const double pvi = PHASE == 0 ? grid_p(lane, p_min, p_max) : (lane < npv ? st->pv[lane] : 0.0);
+  const double gi = lane < npv ? g_of(tot[lane], N) : 1e300;
+  const double fi = lane < npv ? phi_of(tot[lane], N) : 0.0;
+  const double fnext = __shfl_down(fi, 1, 64);
+  const double pnext = __shfl_down(pvi, 1, 64);
+  double a = PHASE == 0 ? p_min : st->a, b = PHASE == 0 ? p_max : st->b, pnew = st->p;
+  int newly_done = 0;
+  if (PHASE == 0) {
+    // first argmin of the span scores (ties -> lowest index)
+    double gs = lane % kSpanStep == 0 ? gi : 1e300;
+    int is = lane;
+    wave_argmin(gs, is);
+    const int ib = is / kSpanStep;
+    const int ja = ib > 0 ? is - kSpanStep : 0;
+    const int jb = ib < kSpan - 1 ? is + kSpanStep : kGrid - 1;
+    const double pbs = __shfl(pvi, is, 64);
+    // a minimum sits where phi goes - -> +; take the one nearest the span
+    // minimum inside the bracket (ties -> lowest index)
+    const bool sc = lane >= ja && lane < jb && fi < 0.0 && fnext >= 0.0;
+    double dist = sc ? fabs(0.5 * (pvi + pnext) - pbs) : 1e300;
+    int jbest = sc ? lane : 1 << 20;
+    wave_argmin(dist, jbest);
+    double gbr = (lane >= ja && lane <= jb) ? gi : 1e300;
+    int ibr = lane;
+    wave_argmin(gbr, ibr);
+    const double f_first = __shfl(fi, ja, 64), f_last = __shfl(fi, jb, 64);
+    if (jbest < (1 << 20)) {
+      const double fj = __shfl(fi, jbest, 64), fj1 = __shfl(fi, jbest + 1, 64);
+      const double pj = __shfl(pvi, jbest, 64), pj1 = __shfl(pvi, jbest + 1, 64);
+      a = pj;
+      b = pj1;
+      pnew = (fj * pj1 - fj1 * pj) / (fj - fj1);  // secant root
+    } else if (f_first >= 0.0 && ja == 0) {
+      pnew = p_min;  // increasing from the lower bound
+      newly_done = 1;
+    } else if (f_last <= 0.0 && jb == kGrid - 1) {
+      pnew = p_max;  // decreasing up to the upper bound
+      newly_done = 1;
+    } else {
+      pnew = __shfl(pvi, ibr, 64);
+      newly_done = 1;
+    }
+  } else if (!done) {
+    // Newton on phi, kept inside the bracket
+    const double f = __shfl(fi, 0, 64), df = dphi_of(tot[0], N);
+    const double p0 = __shfl(pvi, 0, 64);
+    if (f < 0.0) a = p0; else b = p0;
+    double pn = (df > 0.0) ? p0 - f / df : 0.5 * (a + b);
+    if (!(pn >= a && pn <= b)) pn = 0.5 * (a + b);
+    pnew = pn;
+  }
+  if (lane == 0) {
+    st->pv[0] = pnew;
+    st->a = a;
+    st->b = b;
+    st->p = pnew;
+    st->npv = 1;
+    if (PHASE == 0) {
+      st->p_min = p_min;
+      st->p_max = p_max;
+      st->done = newly_done;
+    } else if (newly_done) {
+      st->done = 1;
+    }
+  }
+}
 
 // One evaluation: every block scores the pending p-values over its slice of
-// the spectrum and writes per-block partials (p-major).  The last block of
-// each group of 16 to finish sums the group's partials in block order; the
-// last group to finish sums the group totals in group order and decides the
-// next p-values (fixed summation order: replays are bitwise identical).
+// the spectrum and writes per-block partials (p-major); the last block to
+// finish sums them in block order and decides the next p-values.
 template <int PHASE, typename R>
-__global__ __launch_bounds__(256) void gcv_eval_kernel(const R* Y, const double* lam, int N, GcvState* st,
-                                                       double* part, double p_min, double p_max,
-                                                       const int32_t* skip) {
+__global__ __launch_bounds__(kGcvThreads) void gcv_eval_kernel(const R* Y, const double* lam, int N, GcvState* st,
+                                                               double* part, double p_min, double p_max,
+                                                               const int32_t* skip) {
   if (skip && *skip) return;
-  constexpr int NS = (PHASE >= 3) ? 6 : 4;
-  constexpr int span = PHASE == 0;
-  __shared__ double red[256][NS];
+  constexpr int NS = PHASE >= 1 ? 6 : 4;
+  // the grid evaluation spreads 64 p-values over the lanes of each wave; a
+  // Newton step puts every thread on its single p-value
+  constexpr int PV = PHASE == 0 ? kGrid : 1;
+  constexpr int SUBS = kGcvThreads / PV;
+  constexpr int WAVES = kGcvThreads / 64;
+  __shared__ double red[WAVES][PV][NS];
   __shared__ double tot[kGrid][6];
   __shared__ int is_last;
   const int tid = threadIdx.x;
-  const int pi = tid % 64, sub = tid / 64;
-  // the first evaluation scores the reference's 10-point span directly
-  const int npv = span ? kSpan : st->npv;
-  const bool done = span ? false : (st->done != 0);
+  const int pi = tid % PV, sub = tid / PV;
+  const int npv = PHASE == 0 ? kGrid : st->npv;
+  const bool done = PHASE == 0 ? false : (st->done != 0);
   double acc[NS];
 #pragma unroll
   for (int q = 0; q < NS; ++q) acc[q] = 0.0;
   if (pi < npv && !done) {
-    const double pv = span ? (double)pi * (1.0 / (kSpan - 1)) * (p_max - p_min) + p_min : st->pv[pi];
+    const double pv = PHASE == 0 ? grid_p(pi, p_min, p_max) : st->pv[pi];
     const double s = pow(10.0, pv);
     const int chunk = (N + kGcvBlocks - 1) / kGcvBlocks;
     const int k0 = blockIdx.x * chunk, k1 = min(N, k0 + chunk);
     // four independent elements per step keep several loads in flight
-    for (int k = k0 + sub; k < k1; k += 16) {
+    for (int k = k0 + sub; k < k1; k += 4 * SUBS) {
       double S4[4], L4[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int kk = k + 4 * e;
+        const int kk = k + SUBS * e;
         const bool in = kk < k1;
         const double y0 = in ? (double)Y[kk] : 0.0, y1 = in ? (double)Y[N + kk] : 0.0;
         S4[e] = y0 * y0 + y1 * y1;
@@ -195,178 +288,58 @@ __global__ __launch_bounds__(256) void gcv_eval_kernel(const R* Y, const double*
       for (int e = 0; e < 4; ++e) {
         double t[6];
         gcv_terms(S4[e], L4[e], s, t);
-        const double w = (k + 4 * e < k1) ? 1.0 : 0.0;
+        const double w = (k + SUBS * e < k1) ? 1.0 : 0.0;
 #pragma unroll
         for (int q = 0; q < NS; ++q) acc[q] += w * t[q];
       }
     }
   }
+  // lanes of a wave that share a p-value, then the waves in order
 #pragma unroll
-  for (int q = 0; q < NS; ++q) red[tid][q] = acc[q];
+  for (int off = 32; off >= PV; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q) acc[q] += __shfl_xor(acc[q], off, 64);
+  }
+  const int lane = tid & 63, wave = tid >> 6;
+  if (lane < PV) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q) red[wave][lane][q] = acc[q];
+  }
   __syncthreads();
-  if (tid < 64) {
+  if (tid < PV * NS) {
+    const int p = tid % PV, q = tid / PV;
+    double v = 0.0;
 #pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      const double v = red[tid][q] + red[tid + 64][q] + red[tid + 128][q] + red[tid + 192][q];
-      part[((size_t)tid * 6 + q) * kGcvBlocks + blockIdx.x] = v;
-    }
+    for (int u = 0; u < WAVES; ++u) v += red[u][p][q];
+    part[((size_t)p * 6 + q) * kGcvBlocks + blockIdx.x] = v;
     __threadfence();
   }
   __syncthreads();
-  const int grp = blockIdx.x / kGroupBlocks;
-  if (tid == 0) is_last = atomicAdd(&st->grp_arrivals[grp], 1u) == (unsigned)(kGroupBlocks - 1);
+  if (tid == 0) is_last = atomicAdd(&st->arrivals, 1u) == (unsigned)(gridDim.x - 1);
   __syncthreads();
   if (!is_last) return;
   __threadfence();
-  // group total: one (p-value, sum) row per thread, its 16 partials in flight
-  for (int r = tid; r < npv * NS; r += 256) {
+  // teams of 8 consecutive lanes reduce one (p-value, sum) row of the
+  // kGcvBlocks partials (8 per lane, all loads in flight, then 3 shuffles)
+  const int team = tid >> 3, mate = tid & 7;
+  constexpr int PER = kGcvBlocks / 8;
+  for (int r = team; r < npv * NS; r += kGcvThreads / 8) {
     const int pj = r / NS, q = r % NS;
-    const double2* src =
-        reinterpret_cast<const double2*>(part + ((size_t)pj * 6 + q) * kGcvBlocks + grp * kGroupBlocks);
-    double2 x[kGroupBlocks / 2];
+    const double2* src = reinterpret_cast<const double2*>(part + ((size_t)pj * 6 + q) * kGcvBlocks) + mate * (PER / 2);
+    double2 x[PER / 2];
 #pragma unroll
-    for (int e = 0; e < kGroupBlocks / 2; ++e) x[e] = src[e];
-    double sum = 0.0;
+    for (int e = 0; e < PER / 2; ++e) x[e] = src[e];
+    double t = 0.0;
 #pragma unroll
-    for (int e = 0; e < kGroupBlocks / 2; ++e) sum += x[e].x + x[e].y;
-    st->gsum[((size_t)pj * 6 + q) * kGroups + grp] = sum;
+    for (int e = 0; e < PER / 2; ++e) t += x[e].x + x[e].y;
+    t += __shfl_xor(t, 4, 64);
+    t += __shfl_xor(t, 2, 64);
+    t += __shfl_xor(t, 1, 64);
+    if (mate == 0) tot[pj][q] = t;
   }
-  __threadfence();
+  if (tid == 0) st->arrivals = 0;
   __syncthreads();
-  if (tid == 0) is_last = atomicAdd(&st->top_arrivals, 1u) == (unsigned)(kGroups - 1);
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  if (tid < kGroups) st->grp_arrivals[tid] = 0;
-  if (tid == 0) st->top_arrivals = 0;
-  gcv_decide<PHASE>(N, st, p_min, p_max, tot);
-}
-
-// Sums the group totals of the last evaluation (group order) and picks the
-// next p-values (phases as described above).  Run by the last block of an
-// evaluation launch to finish.
-template <int PHASE>
-__device__ void gcv_decide(int N, GcvState* st, double p_min, double p_max, double (*tot)[6]) {
-  constexpr int NS = (PHASE >= 3) ? 6 : 4;
-  const int tid = threadIdx.x;
-  if (PHASE == 0 && tid == 0) {
-    st->npv = kSpan;
-    for (int i = 0; i < kSpan; ++i) st->pv[i] = (double)i * (1.0 / (kSpan - 1)) * (p_max - p_min) + p_min;
-    st->p_min = p_min;
-    st->p_max = p_max;
-    st->done = 0;
-  }
-  __syncthreads();
-  const int npv = st->npv;
-  for (int r = tid; r < npv * NS; r += blockDim.x) {
-    const int pj = r / NS, q = r % NS;
-    const double2* src = reinterpret_cast<const double2*>(st->gsum + ((size_t)pj * 6 + q) * kGroups);
-    double2 x[kGroups / 2];
-#pragma unroll
-    for (int e = 0; e < kGroups / 2; ++e) x[e] = src[e];
-    double sum = 0.0;
-#pragma unroll
-    for (int e = 0; e < kGroups / 2; ++e) sum += x[e].x + x[e].y;
-    tot[pj][q] = sum;
-  }
-  __syncthreads();
-  if (tid >= 64) return;
-  // wave 0 decides; lane i owns p-value i
-  const int lane = tid;
-  const bool done = st->done != 0;
-  const double pvi = lane < npv ? st->pv[lane] : 0.0;
-  const double gi = lane < npv ? g_of(tot[lane], N) : 1e300;
-  const double fi = lane < npv ? phi_of(tot[lane], N) : 0.0;
-  const double fnext = __shfl_down(fi, 1, 64);
-  const double pnext = __shfl_down(pvi, 1, 64);
-  double a = st->a, b = st->b, pnew = st->p;
-  int newly_done = 0;
-  if (!done) {
-    if (PHASE == 0) {
-      // first argmin of the span scores (ties -> lowest index)
-      double gbest = gi;
-      int ibest = lane;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double go = __shfl_xor(gbest, off, 64);
-        const int io = __shfl_xor(ibest, off, 64);
-        if (go < gbest || (go == gbest && io < ibest)) {
-          gbest = go;
-          ibest = io;
-        }
-      }
-      const double pb = __shfl(pvi, ibest, 64);
-      const double pl = __shfl(pvi, ibest > 0 ? ibest - 1 : 0, 64);
-      const double pr = __shfl(pvi, ibest < kSpan - 1 ? ibest + 1 : 0, 64);
-      if (lane == 0) st->p_best_span = pb;
-      pnew = pb;
-      a = ibest > 0 ? pl : st->p_min;
-      b = ibest < kSpan - 1 ? pr : st->p_max;
-    } else if (PHASE <= 2) {
-      // a minimum sits where phi goes - -> +; take the one nearest the span
-      // minimum (ties -> lowest index)
-      const double pbs = st->p_best_span;
-      const bool sc = lane + 1 < npv && fi < 0.0 && fnext >= 0.0;
-      double dist = sc ? fabs(0.5 * (pvi + pnext) - pbs) : 1e300;
-      int jbest = sc ? lane : 1 << 20;
-      double gbest = gi;
-      int ibest = lane;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double dO = __shfl_xor(dist, off, 64);
-        const int jO = __shfl_xor(jbest, off, 64);
-        if (dO < dist || (dO == dist && jO < jbest)) {
-          dist = dO;
-          jbest = jO;
-        }
-        const double go = __shfl_xor(gbest, off, 64);
-        const int io = __shfl_xor(ibest, off, 64);
-        if (go < gbest || (go == gbest && io < ibest)) {
-          gbest = go;
-          ibest = io;
-        }
-      }
-      const double f_first = __shfl(fi, 0, 64), f_last = __shfl(fi, npv - 1, 64);
-      const double p_first = __shfl(pvi, 0, 64), p_last = __shfl(pvi, npv - 1, 64);
-      if (jbest < (1 << 20)) {
-        const double fj = __shfl(fi, jbest, 64), fj1 = __shfl(fi, jbest + 1, 64);
-        const double pj = __shfl(pvi, jbest, 64), pj1 = __shfl(pvi, jbest + 1, 64);
-        a = pj;
-        b = pj1;
-        pnew = (fj * pj1 - fj1 * pj) / (fj - fj1);  // secant root
-      } else if (f_first >= 0.0 && p_first <= st->p_min) {
-        pnew = st->p_min;  // increasing from the lower bound
-        newly_done = 1;
-      } else if (f_last <= 0.0 && p_last >= st->p_max) {
-        pnew = st->p_max;  // decreasing up to the upper bound
-        newly_done = 1;
-      } else {
-        pnew = __shfl(pvi, ibest, 64);
-        newly_done = 1;
-      }
-    } else {
-      // Newton on phi, kept inside the bracket
-      const double f = __shfl(fi, 0, 64), df = dphi_of(tot[0], N);
-      const double p0 = __shfl(pvi, 0, 64);
-      if (f < 0.0) a = p0; else b = p0;
-      double pn = (df > 0.0) ? p0 - f / df : 0.5 * (a + b);
-      if (!(pn >= a && pn <= b)) pn = 0.5 * (a + b);
-      pnew = pn;
-    }
-  }
-  // next evaluation points, one lane each
-  if (PHASE == 0 || PHASE == 1) {
-    st->pv[lane] = a + (b - a) * (double)lane / (kGrid - 1);
-  } else if (lane == 0) {
-    st->pv[0] = pnew;
-  }
-  if (lane == 0) {
-    st->a = a;
-    st->b = b;
-    st->p = pnew;
-    st->npv = (PHASE == 0 || PHASE == 1) ? kGrid : 1;
-    if (newly_done) st->done = 1;
-  }
+  if (tid < 64) gcv_decide<PHASE>(N, st, p_min, p_max, tot);
 }
 
 // z_dct = Gamma * f_dct (gpu_smoothn.py:196-199); p from the search state
@@ -384,17 +357,15 @@ __global__ void gamma_apply_kernel(R* Y, const double* lam, int N, double p_fixe
   if (k == 0 && p_out) *p_out = p;
 }
 
-// The GCV search on the spectrum Y (two fields, 2N floats): span, two
-// bracket refinements, two Newton steps; leaves p in st->p.
+// The GCV search on the spectrum Y (two fields, 2N floats): span with the
+// bracket grid, two Newton steps; leaves p in st->p.
 template <typename R>
 void enqueue_gcv_search(const R* Y, const double* lam, int N, GcvState* st, double* part, double p_min,
                         double p_max, const int32_t* skip, hipStream_t stream) {
-  const dim3 eb(kGcvBlocks), et(256);
+  const dim3 eb(kGcvBlocks), et(kGcvThreads);
   hipLaunchKernelGGL((gcv_eval_kernel<0, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
   hipLaunchKernelGGL((gcv_eval_kernel<1, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
   hipLaunchKernelGGL((gcv_eval_kernel<2, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
-  hipLaunchKernelGGL((gcv_eval_kernel<3, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
-  hipLaunchKernelGGL((gcv_eval_kernel<4, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
 }
 
 }  // namespace
