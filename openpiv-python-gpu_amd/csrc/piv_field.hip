@@ -314,25 +314,35 @@ struct FlagOutlier {
 template <typename R, typename Out>
 __device__ __forceinline__ void spring_rhs(const R* u, const R* v, Out is_out, int m, int n,
                                            const int (&ni)[4], const int (&nj)[4], double* bu, double* bv,
-                                           int* nk) {
+                                           int* nk, bool* iso_u = nullptr, bool* iso_v = nullptr) {
+  // every load is issued up front (out-of-bounds neighbours read a clamped,
+  // unused element), then the contributions are masked
+  bool inb[4], out[4];
+  R a[4], b[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    inb[e] = ni[e] >= 0 && ni[e] < m && nj[e] >= 0 && nj[e] < n;
+    const int l = min(max(ni[e], 0), m - 1) * n + min(max(nj[e], 0), n - 1);
+    out[e] = is_out(l);
+    a[e] = u[l];
+    b[e] = v[l];
+  }
   double su = 0.0, sv = 0.0;
   int cnt = 0, cu = 0, cv = 0;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    if (ni[e] < 0 || ni[e] >= m || nj[e] < 0 || nj[e] >= n) continue;
+    if (!inb[e]) continue;
     ++cnt;
-    const int l = ni[e] * n + nj[e];
-    if (is_out(l)) {
+    if (out[e]) {
       ++cu;
       ++cv;
     } else {
-      const R a = u[l], b = v[l];
-      if (!isnan(a)) {
-        su += (double)a;
+      if (!isnan(a[e])) {
+        su += (double)a[e];
         ++cu;
       }
-      if (!isnan(b)) {
-        sv += (double)b;
+      if (!isnan(b[e])) {
+        sv += (double)b[e];
         ++cv;
       }
     }
@@ -340,6 +350,8 @@ __device__ __forceinline__ void spring_rhs(const R* u, const R* v, Out is_out, i
   *bu = (cu > 0 && cu != cnt) ? su * cnt / cu : su;
   *bv = (cv > 0 && cv != cnt) ? sv * cnt / cv : sv;
   *nk = cnt;
+  if (iso_u) *iso_u = cu == 0;  // what nan_isolated() finds
+  if (iso_v) *iso_v = cv == 0;
 }
 
 template <typename R, typename Out>
@@ -489,10 +501,13 @@ __device__ void spring_cg_global(R* u, R* v, int m, int n, int nv, const Replace
 constexpr int kSpringLdsThreads = 256;
 
 struct SpringLds {
-  double xu[kSpringLds], xv[kSpringLds], ru[kSpringLds], rv[kSpringLds];
-  double pu[kSpringLds], pv[kSpringLds], qu[kSpringLds], qv[kSpringLds];
+  // xu, xv, pu, pv carry one more element, held at 0: neighbour gathers read
+  // it for neighbours that are not outliers
+  double xu[kSpringLds + 1], xv[kSpringLds + 1], ru[kSpringLds], rv[kSpringLds];
+  double pu[kSpringLds + 1], pv[kSpringLds + 1], qu[kSpringLds], qv[kSpringLds];
   int16_t nb[kSpringLds][4];
   uint8_t nk[kSpringLds];
+  uint8_t iso[kSpringLds];  // bit 0: u keeps its value, bit 1: v does
   int32_t list[kSpringLds];
   int32_t scan[kSpringLdsThreads / 64];
   double red[4][4];
@@ -605,6 +620,129 @@ __device__ void spring_cg(SpringLds& L, int nv) {
   sync();
 }
 
+// The same iterations for up to 256 outliers on wave 0 alone, with every
+// per-outlier value in registers (lane l owns outliers l, l + 64, ...); only
+// the search direction goes through LDS for the neighbour gathers.  Same
+// arithmetic and summation order as spring_cg<64>; 1 / n_k from a table
+// (the correctly rounded reciprocals of 1..4).
+__device__ void spring_cg_wave(SpringLds& L, int nv) {
+  constexpr int E = 256 / 64;
+  const int lane = threadIdx.x;
+  double xu[E], xv[E], ru[E], rv[E], pu[E], pv[E], nk[E], inv[E];
+  int nb[E][4];  // gather index: the neighbour's slot, or the zero element
+  bool act[E];
+  double acc[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int s = lane + 64 * e;
+    act[e] = s < nv;
+    const int sc = act[e] ? s : 0;
+    const int cnt = L.nk[sc];
+    nk[e] = (double)cnt;
+    inv[e] = cnt == 4 ? 0.25 : (cnt == 3 ? 1.0 / 3.0 : (cnt == 2 ? 0.5 : 1.0));
+    xu[e] = L.xu[sc];
+    xv[e] = L.xv[sc];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const int c = L.nb[sc][d];
+      nb[e][d] = c >= 0 ? c : kSpringLds;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    double au = 0.0, av = 0.0;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {  // + 0.0 for a neighbour that is no outlier
+      au += L.xu[nb[e][d]];
+      av += L.xv[nb[e][d]];
+    }
+    ru[e] = au;  // r0 = b - A x0 = sum of x0 over outlier neighbours
+    rv[e] = av;
+    pu[e] = au * inv[e];
+    pv[e] = av * inv[e];
+    if (act[e]) {
+      acc[0] += au * au * inv[e];
+      acc[1] += av * av * inv[e];
+      const double bu = xu[e] * nk[e], bv = xv[e] * nk[e];  // |b|^2 as the scale
+      acc[2] += bu * bu;
+      acc[3] += bv * bv;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) acc[k] = wave_sum(acc[k]);
+  double rz_u = acc[0], rz_v = acc[1];
+  const double tol2_u = 1e-20 * acc[2], tol2_v = 1e-20 * acc[3];
+  bool done_u = rz_u <= tol2_u, done_v = rz_v <= tol2_v;
+  const int max_it = nv + 200;
+  for (int it = 0; it < max_it && !(done_u && done_v); ++it) {
+    wave_sync_lds();  // earlier gathers done before p is overwritten
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (act[e]) {
+        L.pu[lane + 64 * e] = pu[e];
+        L.pv[lane + 64 * e] = pv[e];
+      }
+    }
+    wave_sync_lds();
+    double pq[2] = {0, 0}, qu[E], qv[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      double au = 0.0, av = 0.0;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        au += L.pu[nb[e][d]];
+        av += L.pv[nb[e][d]];
+      }
+      qu[e] = nk[e] * pu[e] - au;
+      qv[e] = nk[e] * pv[e] - av;
+      if (act[e]) {
+        pq[0] += pu[e] * qu[e];
+        pq[1] += pv[e] * qv[e];
+      }
+    }
+    pq[0] = wave_sum(pq[0]);
+    pq[1] = wave_sum(pq[1]);
+    const double alpha_u = (done_u || pq[0] == 0.0) ? 0.0 : rz_u / pq[0];
+    const double alpha_v = (done_v || pq[1] == 0.0) ? 0.0 : rz_v / pq[1];
+    double nr[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      xu[e] += alpha_u * pu[e];
+      xv[e] += alpha_v * pv[e];
+      ru[e] = ru[e] - alpha_u * qu[e];
+      rv[e] = rv[e] - alpha_v * qv[e];
+      if (act[e]) {
+        nr[0] += ru[e] * ru[e] * inv[e];
+        nr[1] += rv[e] * rv[e] * inv[e];
+        nr[2] += ru[e] * ru[e];
+        nr[3] += rv[e] * rv[e];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) nr[k] = wave_sum(nr[k]);
+    const double beta_u = rz_u > 0.0 ? nr[0] / rz_u : 0.0;
+    const double beta_v = rz_v > 0.0 ? nr[1] / rz_v : 0.0;
+    rz_u = nr[0];
+    rz_v = nr[1];
+    done_u = done_u || nr[2] <= tol2_u;
+    done_v = done_v || nr[3] <= tol2_v;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      pu[e] = ru[e] * inv[e] + beta_u * pu[e];
+      pv[e] = rv[e] * inv[e] + beta_v * pv[e];
+    }
+  }
+  wave_sync_lds();  // every gather of x0 / p done before x is written back
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    if (act[e]) {
+      L.xu[lane + 64 * e] = xu[e];
+      L.xv[lane + 64 * e] = xv[e];
+    }
+  }
+  wave_sync_lds();
+}
+
 template <typename R>
 __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* v, const uint8_t* flags,
                                                                        const int32_t* count, int m, int n,
@@ -629,15 +767,38 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* 
       words[2] = q.z;
       words[3] = q.w;
     } else {
-      words[0] = words[1] = words[2] = words[3] = 0;
-      for (int k = 16 * c; k < N; ++k) words[(k - 16 * c) >> 2] |= (uint32_t)flags[k] << (8 * ((k - 16 * c) & 3));
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        uint32_t wv = 0;
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) {
+          const int k = 16 * c + 4 * q + bb;
+          if (k < N) wv |= (uint32_t)flags[k] << (8 * bb);
+        }
+        words[q] = wv;
+      }
     }
   };
+  // up to kHeld chunks per thread are loaded at once and kept for the
+  // second pass (fields up to 65536 vectors); larger fields load in a loop
+  constexpr int kHeld = 16;
+  const bool held = per <= kHeld;
+  uint32_t hw[kHeld][4];
   int local = 0;
-  for (int c = c0; c < c1; ++c) {
-    uint32_t words[4];
-    chunk_words(c, words);
-    local += __popc(words[0]) + __popc(words[1]) + __popc(words[2]) + __popc(words[3]);  // 0/1 bytes
+  if (held) {
+#pragma unroll
+    for (int i = 0; i < kHeld; ++i) {
+      if (c0 + i < c1) chunk_words(c0 + i, hw[i]);
+      else hw[i][0] = hw[i][1] = hw[i][2] = hw[i][3] = 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kHeld; ++i) local += __popc(hw[i][0]) + __popc(hw[i][1]) + __popc(hw[i][2]) + __popc(hw[i][3]);
+  } else {
+    for (int c = c0; c < c1; ++c) {
+      uint32_t words[4];
+      chunk_words(c, words);
+      local += __popc(words[0]) + __popc(words[1]) + __popc(words[2]) + __popc(words[3]);  // 0/1 bytes
+    }
   }
   // exclusive prefix of the counts: wave scans, then the wave totals
   const int lane = tid & 63, wave = tid >> 6;
@@ -656,7 +817,22 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* 
     total += L.scan[k];
   }
   const bool in_lds = total <= kSpringLds;
-  for (int c = c0; c < c1; ++c) {
+  if (in_lds && held) {
+    // visit the set flag bytes only (zero words past the end)
+#pragma unroll
+    for (int i = 0; i < kHeld; ++i) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        uint32_t wv = hw[i][q];
+        while (wv) {
+          const int byte = (__builtin_ctz(wv) >> 3);
+          L.list[pos++] = 16 * (c0 + i) + 4 * q + byte;
+          wv &= ~(0xffu << (8 * byte));
+        }
+      }
+    }
+  }
+  for (int c = (in_lds && held) ? c1 : c0; c < c1; ++c) {
     uint32_t words[4];
     chunk_words(c, words);
 #pragma unroll
@@ -685,6 +861,12 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* 
 
   // neighbour codes (slot, -1 valid, -2 outside) and x0 = b / n_k
   const FlagOutlier is_out{flags};
+  if (tid == 0) {
+    L.xu[kSpringLds] = 0.0;
+    L.xv[kSpringLds] = 0.0;
+    L.pu[kSpringLds] = 0.0;
+    L.pv[kSpringLds] = 0.0;
+  }
   auto slot_of = [&](int l) {  // l is in the (ascending) list
     int lo = 0, hi = nvl - 1;
     while (lo < hi) {
@@ -701,16 +883,22 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* 
     const int nj[4] = {j - 1, j + 1, j, j};
     double bu = 0.0, bv = 0.0;
     int cnt = 0;
+    bool nout[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {  // flags of the (clamped) neighbours, loaded together
+      const int l = min(max(ni[e], 0), m - 1) * n + min(max(nj[e], 0), n - 1);
+      nout[e] = is_out(l);
+    }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       int16_t code = -2;
-      if (ni[e] >= 0 && ni[e] < m && nj[e] >= 0 && nj[e] < n) {
-        const int l = ni[e] * n + nj[e];
-        code = is_out(l) ? (int16_t)slot_of(l) : (int16_t)-1;
-      }
+      if (ni[e] >= 0 && ni[e] < m && nj[e] >= 0 && nj[e] < n)
+        code = nout[e] ? (int16_t)slot_of(ni[e] * n + nj[e]) : (int16_t)-1;
       L.nb[s][e] = code;
     }
-    spring_rhs(u, v, is_out, m, n, ni, nj, &bu, &bv, &cnt);
+    bool iu, iv;
+    spring_rhs(u, v, is_out, m, n, ni, nj, &bu, &bv, &cnt, &iu, &iv);
+    L.iso[s] = (uint8_t)((iu ? 1 : 0) | (iv ? 2 : 0));
     L.nk[s] = (uint8_t)cnt;
     L.xu[s] = bu / cnt;
     L.xv[s] = bv / cnt;
@@ -718,18 +906,17 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* 
   __syncthreads();
   if (nvl <= 256) {
     if (tid >= 64) return;  // wave 0 finishes alone
-    spring_cg<64>(L, nvl);
+    spring_cg_wave(L, nvl);
   } else {
     spring_cg<T>(L, nvl);
   }
-  // all neighbour reads of nan_isolated() see pre-solve values: outlier
-  // neighbours are recognised by their flag, never read
+  // vectors with no outlier and no finite neighbour keep their value
+  // (nan_isolated(), found while the right-hand sides were formed)
   for (int s = tid; s < nvl; s += (nvl <= 256 ? 64 : T)) {
     const int k = L.list[s];
-    bool ku, kv;
-    nan_isolated(u, v, is_out, m, n, k, &ku, &kv);
-    if (!ku) u[k] = (R)L.xu[s];
-    if (!kv) v[k] = (R)L.xv[s];
+    const int iso = L.iso[s];
+    if (!(iso & 1)) u[k] = (R)L.xu[s];
+    if (!(iso & 2)) v[k] = (R)L.xv[s];
   }
 }
 
