@@ -1526,6 +1526,7 @@ namespace {
 
 struct BatchGroup {  // one batched correlation launch
   int fsize = 0, in_dtype = 0, total_blocks = 0, max_ss = 0, xcd = 1;
+  bool line = false;  // one-lane-per-line kernel (the one a single-pair run of these fields uses)
   std::vector<int> plans;
   CorrArgs* d_fields = nullptr;
   int* d_prefix = nullptr;
@@ -1596,19 +1597,25 @@ int batch_build_tables(piv_batch* b) {
     L.deform_threads = threads;
     for (int p = 0; p < (int)b->plans.size(); ++p)
       if ((int)b->plans[p]->passes.size() > (int)l) L.plans.push_back(p);
-    // correlation groups by (FFT size, input dtype)
+    // correlation groups by (FFT size, input dtype, kernel)
     for (int p : L.plans) {
       piv_plan* P = b->plans[p];
       const PassBuffers& B = P->passes[l];
-      const int dtype = l == 0 ? P->desc.in_dtype : (int)kF32;
+      const piv_plan_desc& D = P->desc;
+      const int dtype = l == 0 ? D.in_dtype : (int)kF32;
+      CorrArgs probe = l == 0 ? corr_args(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d)
+                              : corr_args(P->def_a, P->def_b, kF32, D.height, D.width, B.d);
+      probe.s2n = B.s2n;
+      const bool line = correlate_line_enabled() && correlate_line_applies(probe);
       BatchGroup* g = nullptr;
       for (auto& q : L.groups)
-        if (q.fsize == B.d.fft_size && q.in_dtype == dtype) g = &q;
+        if (q.fsize == B.d.fft_size && q.in_dtype == dtype && q.line == line) g = &q;
       if (!g) {
         L.groups.push_back(BatchGroup{});
         g = &L.groups.back();
         g->fsize = B.d.fft_size;
         g->in_dtype = dtype;
+        g->line = line;
         g->xcd = L.groups.size() <= keep_xcd.size() ? keep_xcd[L.groups.size() - 1] : 1;
       }
       g->plans.push_back(p);
@@ -1616,7 +1623,7 @@ int batch_build_tables(piv_batch* b) {
     for (auto& g : L.groups) {
       std::vector<CorrArgs> fields;
       std::vector<int> prefix;
-      const int wpb = correlate_windows_per_block(g.fsize);
+      const int wpb = g.line ? correlate_line_windows_per_block(g.fsize) : correlate_windows_per_block(g.fsize);
       g.total_blocks = 0;
       for (int p : g.plans) {
         piv_plan* P = b->plans[p];
@@ -1670,7 +1677,10 @@ int batch_build_tables(piv_batch* b) {
 
 int batch_enqueue_corr(piv_batch* b, BatchGroup& g, hipStream_t s) {
   CorrBatch cb{g.d_fields, g.d_prefix, (int)g.plans.size(), g.xcd};
-  HIP_TRY(launch_correlate_batch(cb, g.fsize, g.in_dtype, g.total_blocks, g.max_ss, s));
+  if (g.line)
+    HIP_TRY(launch_correlate_line_batch(cb, g.fsize, g.in_dtype, g.total_blocks, g.max_ss, s));
+  else
+    HIP_TRY(launch_correlate_batch(cb, g.fsize, g.in_dtype, g.total_blocks, g.max_ss, s));
   return PIV_OK;
 }
 

@@ -37,6 +37,12 @@ struct CorrArgs {
 };
 
 hipError_t launch_correlate(const CorrArgs& a, hipStream_t stream);
+// One-lane-per-line variant for F = 16, 32 (search window <= F/2, no S2N, no
+// shrunk interrogation window); launch_correlate picks it when it applies and
+// is enabled (OPENPIV_HIP_CORR_LINE=0 in the environment turns it off).
+bool correlate_line_applies(const CorrArgs& a);
+bool correlate_line_enabled();
+hipError_t launch_correlate_line(const CorrArgs& a, hipStream_t stream);
 
 // FP64 correlation (dtype_f = float64): the same semantics with every value
 // in double; frames of any InDtype (deformed frames are kF64).  scratch:
@@ -81,6 +87,9 @@ struct CorrBatch {
 int correlate_windows_per_block(int fsize);
 hipError_t launch_correlate_batch(const CorrBatch& b, int fsize, int in_dtype, int total_blocks, int max_ss,
                                   hipStream_t stream);
+int correlate_line_windows_per_block(int fsize);
+hipError_t launch_correlate_line_batch(const CorrBatch& b, int fsize, int in_dtype, int total_blocks, int max_ss,
+                                       hipStream_t stream);
 size_t correlate_lds_bytes(int fsize, int ss);
 
 // Frame deformation: predictor bilinearly interpolated (with linear
