@@ -13,7 +13,8 @@
 //   stage  lane j of a window holds                      LDS
 //   R      row j of z = (a - mean) + i (b - mean)        staging -> AB
 //          -> A and B row spectra (DC, Nyquist in slot 0)
-//   C      columns kx = j of A and B -> conj(FA) FB      AB -> registers
+//   C      columns kx = j of A and B -> conj(FA) FB      AB -> registers,
+//          (FA parked in LDS while FB is transformed)    FA -> LDS -> registers
 //   IC     inverse column kx of the product              -> R
 //   IR     rows 2j, 2j+1 as one complex inverse          R -> plane
 //   peak   first-occurrence argmax, sub-pixel fit        plane
@@ -166,19 +167,24 @@ __global__ __launch_bounds__(256, LineLayout<F>::WAVES) void corr_line_kernel(Co
   }
   wave_lds_sync();
 
+  // The windows sit at index 0 of the F-point transforms instead of at the
+  // padding offset (F - W)/2: a circular shift common to both frames cancels
+  // in conj(FA) FB, so the plane is the same, and inputs F/2..F-1 are
+  // compile-time zeros (W <= F/2).
+
   // ---- stage R: row j of z, all F points in this lane ---------------------
   const bool row_on = j < W;
   {
     c32 x[F];
     const int rr = row_on ? j : 0;
-    static_for<0, F>([&](auto CC) {
+    static_for<0, H>([&](auto CC) {
       constexpr int c = decltype(CC)::value;
-      const int cc = c - pad;
-      const bool in = cc >= 0 && cc < W;
-      const int ci = in ? cc : 0;
+      const bool in = c < W;
+      const int ci = in ? c : 0;
       const float va = stA[rr * SW + ci], vb = stB[rr * SW + ci];
       x[c] = in ? c32{va - mean_a, vb - mean_b} : c32{0.f, 0.f};
     });
+    static_for<H, F>([&](auto CC) { x[decltype(CC)::value] = c32{0.f, 0.f}; });
     wave_lds_sync();  // staging (aliased by AB) fully read
     fft_reg<F, -1>(x);
     // Z[k] sits in x[bitrev(k)].  2A[k] = Z[k] + conj(Z[F-k]),
@@ -202,49 +208,81 @@ __global__ __launch_bounds__(256, LineLayout<F>::WAVES) void corr_line_kernel(Co
   wave_lds_sync();
 
   // ---- stage C: columns kx = j of A and B, product -------------------------
+  // FA goes to LDS (column kx of the R layout, which this lane alone
+  // touches) while FB is transformed; column H of the R layout (row padding)
+  // carries the slot-0 values of the window for the DC/Nyquist product.
   const int kx = j;
   c32 p[F];  // product P[ky][kx] in natural order of ky
   {
-    c32 fa[F], fb[F];
-    static_for<0, F>([&](auto MM) {
-      constexpr int mm = decltype(MM)::value;
-      const int r = mm - pad;
-      const bool in = r >= 0 && r < W;
-      const int ri = in ? r : 0;
-      const c32 va = AB[(size_t)ri * SF + kx], vb = AB[(size_t)ri * SF + H + kx];
-      fa[mm] = in ? va : c32{0.f, 0.f};
-      fb[mm] = in ? vb : c32{0.f, 0.f};
-    });
-    wave_lds_sync();  // AB (aliased by R) fully read
-    fft_reg<F, -1>(fa);
-    fft_reg<F, -1>(fb);
-    if (kx != 0) {
-      static_for<0, F>([&](auto KY) {
-        constexpr int ky = decltype(KY)::value;
-        p[ky] = cconjmul(fa[bitrev(ky, F)], fb[bitrev(ky, F)]);
+    c32 bin[H];
+    {
+      c32 fa[F];
+      static_for<0, H>([&](auto MM) {
+        constexpr int r = decltype(MM)::value;
+        const bool in = r < W;
+        const int ri = in ? r : 0;
+        const c32 va = AB[(size_t)ri * SF + kx], vb = AB[(size_t)ri * SF + H + kx];
+        fa[r] = in ? va : c32{0.f, 0.f};
+        bin[r] = in ? vb : c32{0.f, 0.f};
       });
-    } else {
-      // Slot 0 holds FFT(col_DC + i col_Nyquist) of two real columns: unpack
-      // with the ky <-> -ky mirror, form both products, repack as
-      // P_DC + i P_Nyquist (both invert to real columns).
+      static_for<H, F>([&](auto MM) { fa[decltype(MM)::value] = c32{0.f, 0.f}; });
+      wave_lds_sync();  // AB (aliased by R) fully read
+      fft_reg<F, -1>(fa);
       static_for<0, F>([&](auto KY) {
         constexpr int ky = decltype(KY)::value;
-        constexpr int km = (F - ky) % F;
-        const c32 qa = fa[bitrev(ky, F)], qam = fa[bitrev(km, F)];
-        const c32 qb = fb[bitrev(ky, F)], qbm = fb[bitrev(km, F)];
-        // f0 = (q + conj(qm)) / 2 ; fn = (q - conj(qm)) / (2i)
-        const c32 a0{0.5f * (qa.x + qam.x), 0.5f * (qa.y - qam.y)};
-        const c32 an{0.5f * (qa.y + qam.y), 0.5f * (qam.x - qa.x)};
-        const c32 b0{0.5f * (qb.x + qbm.x), 0.5f * (qb.y - qbm.y)};
-        const c32 bn{0.5f * (qb.y + qbm.y), 0.5f * (qbm.x - qb.x)};
-        const c32 p0 = cconjmul(a0, b0), pn = cconjmul(an, bn);
-        p[ky] = c32{p0.x - pn.y, p0.y + pn.x};
+        R[(size_t)ky * SR + kx] = fa[bitrev(ky, F)];
+      });
+    }
+    c32 fb[F];
+    static_for<0, H>([&](auto MM) { fb[decltype(MM)::value] = bin[decltype(MM)::value]; });
+    static_for<H, F>([&](auto MM) { fb[decltype(MM)::value] = c32{0.f, 0.f}; });
+    fft_reg<F, -1>(fb);
+    static_for<0, F>([&](auto KY) {
+      constexpr int ky = decltype(KY)::value;
+      p[ky] = cconjmul(R[(size_t)ky * SR + kx], fb[bitrev(ky, F)]);
+    });
+    if (kx == 0) {
+      static_for<0, F>([&](auto KY) {
+        constexpr int ky = decltype(KY)::value;
+        R[(size_t)ky * SR + H] = fb[bitrev(ky, F)];
+      });
+    }
+  }
+  wave_lds_sync();
+  {
+    // Slot 0 holds FFT(col_DC + i col_Nyquist) of two real columns: unpack
+    // with the ky <-> -ky mirror, form both products and repack as
+    // P_DC + i P_Nyquist (both invert to real columns).  Lane j of the window
+    // does ky = j and ky = j + F/2.
+    c32 q2[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int ky = j + t * H, km = (F - ky) & (F - 1);
+      const c32 qa = R[(size_t)ky * SR], qam = R[(size_t)km * SR];
+      const c32 qb = R[(size_t)ky * SR + H], qbm = R[(size_t)km * SR + H];
+      // f0 = (q + conj(qm)) / 2 ; fn = (q - conj(qm)) / (2i)
+      const c32 a0{0.5f * (qa.x + qam.x), 0.5f * (qa.y - qam.y)};
+      const c32 an{0.5f * (qa.y + qam.y), 0.5f * (qam.x - qa.x)};
+      const c32 b0{0.5f * (qb.x + qbm.x), 0.5f * (qb.y - qbm.y)};
+      const c32 bn{0.5f * (qb.y + qbm.y), 0.5f * (qbm.x - qb.x)};
+      const c32 p0 = cconjmul(a0, b0), pn = cconjmul(an, bn);
+      q2[t] = c32{p0.x - pn.y, p0.y + pn.x};
+    }
+    wave_lds_sync();  // every mirror read done before column H is overwritten
+    R[(size_t)j * SR + H] = q2[0];
+    R[(size_t)(j + H) * SR + H] = q2[1];
+    wave_lds_sync();
+    if (kx == 0) {
+      static_for<0, F>([&](auto KY) {
+        constexpr int ky = decltype(KY)::value;
+        p[ky] = R[(size_t)ky * SR + H];
       });
     }
   }
 
   // ---- stage IC: inverse column kx ------------------------------------------
   fft_reg<F, +1>(p);
+  wave_lds_sync();  // lane 0's reads of column H done before R is written
   static_for<0, F>([&](auto YY) {
     constexpr int y = decltype(YY)::value;
     R[(size_t)y * SR + kx] = p[bitrev(y, F)];
@@ -433,6 +471,8 @@ hipError_t launch_correlate_line_batch(const CorrBatch& b, int fsize, int in_dty
   return hipErrorInvalidValue;
 }
 
+// F = 64 is left to corr_kernel: its 17 KB of LDS per window would hold this
+// mapping to one wave per SIMD, which measured slower than corr_kernel.
 bool correlate_line_applies(const CorrArgs& a) {
   return (a.fsize == 16 || a.fsize == 32) && 2 * a.ss <= a.fsize && a.s2n == nullptr && a.win_pad == 0 &&
          (a.in_dtype == kU8 || a.in_dtype == kU16 || a.in_dtype == kF32);
