@@ -1527,6 +1527,7 @@ namespace {
 struct BatchGroup {  // one batched correlation launch
   int fsize = 0, in_dtype = 0, total_blocks = 0, max_ss = 0, xcd = 1;
   bool line = false;  // one-lane-per-line kernel (the one a single-pair run of these fields uses)
+  bool half = false;  // search windows <= F/2 (the two-lane kernel's variant for them)
   std::vector<int> plans;
   CorrArgs* d_fields = nullptr;
   int* d_prefix = nullptr;
@@ -1607,15 +1608,17 @@ int batch_build_tables(piv_batch* b) {
                               : corr_args(P->def_a, P->def_b, kF32, D.height, D.width, B.d);
       probe.s2n = B.s2n;
       const bool line = correlate_line_enabled() && correlate_line_applies(probe);
+      const bool half = 2 * B.d.search_size <= B.d.fft_size;
       BatchGroup* g = nullptr;
       for (auto& q : L.groups)
-        if (q.fsize == B.d.fft_size && q.in_dtype == dtype && q.line == line) g = &q;
+        if (q.fsize == B.d.fft_size && q.in_dtype == dtype && q.line == line && q.half == half) g = &q;
       if (!g) {
         L.groups.push_back(BatchGroup{});
         g = &L.groups.back();
         g->fsize = B.d.fft_size;
         g->in_dtype = dtype;
         g->line = line;
+        g->half = half;
         g->xcd = L.groups.size() <= keep_xcd.size() ? keep_xcd[L.groups.size() - 1] : 1;
       }
       g->plans.push_back(p);
@@ -1680,7 +1683,7 @@ int batch_enqueue_corr(piv_batch* b, BatchGroup& g, hipStream_t s) {
   if (g.line)
     HIP_TRY(launch_correlate_line_batch(cb, g.fsize, g.in_dtype, g.total_blocks, g.max_ss, s));
   else
-    HIP_TRY(launch_correlate_batch(cb, g.fsize, g.in_dtype, g.total_blocks, g.max_ss, s));
+    HIP_TRY(launch_correlate_batch(cb, g.fsize, g.in_dtype, g.total_blocks, g.max_ss, g.half, s));
   return PIV_OK;
 }
 

@@ -77,7 +77,11 @@ size_t correlate_lds_bytes(int fsize, int ss) {
 // works on one field and the field's arguments stay wave-uniform.
 // EXTRA: the S2N epilogue and the interrogation window inside the search
 // window are compiled in (launches that need neither use the lean variant).
-template <int F, typename InT, bool BATCHED, bool EXTRA>
+// HALF: search window <= F/2.  The window then sits at index 0 of the
+// transforms instead of at the padding offset (F - W)/2 -- a circular shift
+// common to both frames that cancels in conj(FA) FB -- so the upper half of
+// every forward input is a compile-time zero.
+template <int F, typename InT, bool BATCHED, bool EXTRA, bool HALF>
 __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_in, CorrBatch batch,
                                                                      int n_windows_in, int region_floats) {
   using L = Layout<F>;
@@ -212,9 +216,9 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
     const int rr = active ? q : 0;
     static_for<0, H>([&](auto M) {
       constexpr int m = decltype(M)::value;
-      const int c = 2 * m + h - pad;
+      const int c = 2 * m + h - (HALF ? 0 : pad);
       c32 zz{0.f, 0.f};
-      if (active && c >= 0 && c < W) {
+      if ((!HALF || m < H / 2) && active && c >= 0 && c < W) {
         const bool in_a = rr >= wp && rr < W - wp && c >= wp && c < W - wp;
         zz.x = in_a ? stA[rr * SW + c] - mean_a : 0.f;
         zz.y = stB[rr * SW + c] - mean_b;
@@ -290,9 +294,9 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
     const int col = sel * H + kx;
     static_for<0, H>([&](auto M) {
       constexpr int m = decltype(M)::value;
-      const int r = 2 * m + (upper ? 1 : 0) - pad;
+      const int r = 2 * m + (upper ? 1 : 0) - (HALF ? 0 : pad);
       c32 zz{0.f, 0.f};
-      if (r >= 0 && r < W) zz = AB[(size_t)r * SF + col];
+      if ((!HALF || m < H / 2) && r >= 0 && r < W) zz = AB[(size_t)r * SF + col];
       x[m] = zz;
     });
   }
@@ -577,11 +581,11 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
 
 // Dynamic LDS above 64 KiB must be opted into; request exactly what a
 // launch needs (gfx950 allows up to 160 KiB per workgroup).
-template <int F, typename InT, bool BATCHED, bool EXTRA>
+template <int F, typename InT, bool BATCHED, bool EXTRA, bool HALF>
 static hipError_t lds_attr(size_t lds) {
   static thread_local size_t attr_bytes = 0;
   if (lds > 65536 && lds > attr_bytes) {
-    hipError_t e = hipFuncSetAttribute((const void*)corr_kernel<F, InT, BATCHED, EXTRA>,
+    hipError_t e = hipFuncSetAttribute((const void*)corr_kernel<F, InT, BATCHED, EXTRA, HALF>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {
       set_launch_detail("hipFuncSetAttribute(MaxDynamicSharedMemorySize)", e);
@@ -592,49 +596,59 @@ static hipError_t lds_attr(size_t lds) {
   return hipSuccess;
 }
 
-template <int F, typename InT>
-static hipError_t launch_f(const CorrArgs& a, hipStream_t stream) {
+template <int F, typename InT, bool EXTRA, bool HALF>
+static hipError_t launch_fv(const CorrArgs& a, hipStream_t stream) {
   using L = Layout<F>;
   const int n_windows = a.m * a.n;
   const size_t region = correlate_lds_bytes(F, a.ss);
   const size_t lds = region * L::WPB;
   const int blocks = (n_windows + L::WPB - 1) / L::WPB;
   (void)hipGetLastError();  // do not report an earlier, unrelated error
-  const bool extra = a.s2n != nullptr || a.win_pad != 0;
-  hipError_t e = extra ? lds_attr<F, InT, false, true>(lds) : lds_attr<F, InT, false, false>(lds);
+  hipError_t e = lds_attr<F, InT, false, EXTRA, HALF>(lds);
   if (e != hipSuccess) return e;
-  if (extra)
-    hipLaunchKernelGGL((corr_kernel<F, InT, false, true>), dim3(blocks), dim3(256), lds, stream, a, CorrBatch{},
-                       n_windows, (int)(region / sizeof(float)));
-  else
-    hipLaunchKernelGGL((corr_kernel<F, InT, false, false>), dim3(blocks), dim3(256), lds, stream, a, CorrBatch{},
-                       n_windows, (int)(region / sizeof(float)));
+  hipLaunchKernelGGL((corr_kernel<F, InT, false, EXTRA, HALF>), dim3(blocks), dim3(256), lds, stream, a,
+                     CorrBatch{}, n_windows, (int)(region / sizeof(float)));
   e = hipGetLastError();
   if (e != hipSuccess) set_launch_detail("corr_kernel launch", e);
   return e;
 }
 
 template <int F, typename InT>
-static hipError_t launch_fb(const CorrBatch& b, int total_blocks, int max_ss, hipStream_t stream) {
+static hipError_t launch_f(const CorrArgs& a, hipStream_t stream) {
+  const bool extra = a.s2n != nullptr || a.win_pad != 0;
+  const bool half = 2 * a.ss <= F;
+  if (extra) return half ? launch_fv<F, InT, true, true>(a, stream) : launch_fv<F, InT, true, false>(a, stream);
+  return half ? launch_fv<F, InT, false, true>(a, stream) : launch_fv<F, InT, false, false>(a, stream);
+}
+
+template <int F, typename InT, bool HALF>
+static hipError_t launch_fbv(const CorrBatch& b, int total_blocks, int max_ss, hipStream_t stream) {
   using L = Layout<F>;
   const size_t region = correlate_lds_bytes(F, max_ss);
   const size_t lds = region * L::WPB;
   (void)hipGetLastError();
-  hipError_t e = lds_attr<F, InT, true, true>(lds);
+  hipError_t e = lds_attr<F, InT, true, true, HALF>(lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((corr_kernel<F, InT, true, true>), dim3(total_blocks), dim3(256), lds, stream, CorrArgs{}, b, 0,
-                     (int)(region / sizeof(float)));
+  hipLaunchKernelGGL((corr_kernel<F, InT, true, true, HALF>), dim3(total_blocks), dim3(256), lds, stream, CorrArgs{},
+                     b, 0, (int)(region / sizeof(float)));
   e = hipGetLastError();
   if (e != hipSuccess) set_launch_detail("corr_kernel (batched) launch", e);
   return e;
 }
 
+template <int F, typename InT>
+static hipError_t launch_fb(const CorrBatch& b, int total_blocks, int max_ss, bool half, hipStream_t stream) {
+  return half ? launch_fbv<F, InT, true>(b, total_blocks, max_ss, stream)
+              : launch_fbv<F, InT, false>(b, total_blocks, max_ss, stream);
+}
+
 template <int F>
-static hipError_t launch_tb(const CorrBatch& b, int in_dtype, int total_blocks, int max_ss, hipStream_t stream) {
+static hipError_t launch_tb(const CorrBatch& b, int in_dtype, int total_blocks, int max_ss, bool half,
+                            hipStream_t stream) {
   switch (in_dtype) {
-    case kU8: return launch_fb<F, uint8_t>(b, total_blocks, max_ss, stream);
-    case kU16: return launch_fb<F, uint16_t>(b, total_blocks, max_ss, stream);
-    case kF32: return launch_fb<F, float>(b, total_blocks, max_ss, stream);
+    case kU8: return launch_fb<F, uint8_t>(b, total_blocks, max_ss, half, stream);
+    case kU16: return launch_fb<F, uint16_t>(b, total_blocks, max_ss, half, stream);
+    case kF32: return launch_fb<F, float>(b, total_blocks, max_ss, half, stream);
   }
   return hipErrorInvalidValue;
 }
@@ -652,13 +666,13 @@ static hipError_t launch_t(const CorrArgs& a, hipStream_t stream) {
 int correlate_windows_per_block(int fsize) { return 256 / (2 * fsize); }
 
 hipError_t launch_correlate_batch(const CorrBatch& b, int fsize, int in_dtype, int total_blocks, int max_ss,
-                                  hipStream_t stream) {
+                                  bool half, hipStream_t stream) {
   if (total_blocks <= 0) return hipSuccess;
   switch (fsize) {
-    case 16: return launch_tb<16>(b, in_dtype, total_blocks, max_ss, stream);
-    case 32: return launch_tb<32>(b, in_dtype, total_blocks, max_ss, stream);
-    case 64: return launch_tb<64>(b, in_dtype, total_blocks, max_ss, stream);
-    case 128: return launch_tb<128>(b, in_dtype, total_blocks, max_ss, stream);
+    case 16: return launch_tb<16>(b, in_dtype, total_blocks, max_ss, half, stream);
+    case 32: return launch_tb<32>(b, in_dtype, total_blocks, max_ss, half, stream);
+    case 64: return launch_tb<64>(b, in_dtype, total_blocks, max_ss, half, stream);
+    case 128: return launch_tb<128>(b, in_dtype, total_blocks, max_ss, half, stream);
   }
   return hipErrorInvalidValue;
 }

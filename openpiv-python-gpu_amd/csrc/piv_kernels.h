@@ -85,8 +85,10 @@ struct CorrBatch {
   int xcd;
 };
 int correlate_windows_per_block(int fsize);
+// half: every field of the launch has search window <= F/2 (the single-pair
+// launch of such a field uses the same arithmetic)
 hipError_t launch_correlate_batch(const CorrBatch& b, int fsize, int in_dtype, int total_blocks, int max_ss,
-                                  hipStream_t stream);
+                                  bool half, hipStream_t stream);
 int correlate_line_windows_per_block(int fsize);
 hipError_t launch_correlate_line_batch(const CorrBatch& b, int fsize, int in_dtype, int total_blocks, int max_ss,
                                        hipStream_t stream);
