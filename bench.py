@@ -76,7 +76,9 @@ def stage_kernel_times(fa_u16, piv, reps=20):
                                                   u.ptr, v.ptr, None, None, None, None, sh))
         ms = timed(launch)
         flops = fft_flops_per_window(p.fft_size) * m * n
-        out.append({"pass": k, "kernel": f"corr_kernel<F={p.fft_size},{'u16' if k == 0 else 'f32'}>",
+        # F <= 32 with search window <= F/2 runs the one-lane-per-line kernel
+        kname = "corr_line_kernel" if p.fft_size <= 32 and 2 * p.search_size <= p.fft_size else "corr_kernel"
+        out.append({"pass": k, "kernel": f"{kname}<F={p.fft_size},{'u16' if k == 0 else 'f32'}>",
                     "windows": m * n, "fft": p.fft_size,
                     "avg_ms": ms, "flop": flops, "tflops": flops / (ms * 1e-3) / 1e12})
     # deformation (second pass grid): reads both uint16 frames, writes two FP32 frames

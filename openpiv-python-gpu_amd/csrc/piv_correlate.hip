@@ -451,7 +451,11 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
   }
 #pragma unroll
   for (int off = WIDTH / 2; off > 0; off >>= 1) {
-    Best o{__shfl_xor(best.v, off, 64), __shfl_xor(best.i, off, 64)};
+    // opaque offset: the shuffle addresses are recomputed here instead of
+    // being kept (spilled) from the window-sum reduction at the start
+    int o_off = off;
+    asm volatile("" : "+v"(o_off));
+    Best o{__shfl_xor(best.v, o_off, 64), __shfl_xor(best.i, o_off, 64)};
     best = better(best, o);
   }
   if constexpr (LANES > 64) {
