@@ -889,8 +889,15 @@ int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on
     HIP_TRY(hipStreamWaitEvent(s, P->wait_next, 0));
     P->wait_next = nullptr;
   }
-  HIP_TRY(hipMemcpyAsync(P->frame_a, frame_a, P->frame_bytes, in_kind, s));
-  HIP_TRY(hipMemcpyAsync(P->frame_b, frame_b, P->frame_bytes, in_kind, s));
+  if (frames_on_device) {
+    CopyList cl{};
+    cl.c[0] = CopySpec{P->frame_a, frame_a, P->frame_bytes};
+    cl.c[1] = CopySpec{P->frame_b, frame_b, P->frame_bytes};
+    HIP_TRY(launch_copy_multi(cl, 2, s));
+  } else {
+    HIP_TRY(hipMemcpyAsync(P->frame_a, frame_a, P->frame_bytes, in_kind, s));
+    HIP_TRY(hipMemcpyAsync(P->frame_b, frame_b, P->frame_bytes, in_kind, s));
+  }
   // The first run of each path is eager (it also sets kernel attributes);
   // later runs replay a captured graph when requested.
   P->last_fallback = 0;
@@ -914,11 +921,20 @@ int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on
     if (r) return r;
   }
   const PassBuffers& L = P->passes.back();
-  const hipMemcpyKind out_kind = outputs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-  if (u) HIP_TRY(hipMemcpyAsync(u, L.u, L.N * P->esize, out_kind, s));
-  if (v) HIP_TRY(hipMemcpyAsync(v, L.v, L.N * P->esize, out_kind, s));
-  if (val_init) HIP_TRY(hipMemcpyAsync(val_init, L.flags, L.N, out_kind, s));
-  if (val_final) HIP_TRY(hipMemcpyAsync(val_final, P->val_final, L.N, out_kind, s));
+  if (outputs_on_device) {  // one launch for all requested outputs
+    CopyList cl{};
+    int nc = 0;
+    if (u) cl.c[nc++] = CopySpec{u, L.u, L.N * P->esize};
+    if (v) cl.c[nc++] = CopySpec{v, L.v, L.N * P->esize};
+    if (val_init) cl.c[nc++] = CopySpec{val_init, L.flags, (size_t)L.N};
+    if (val_final) cl.c[nc++] = CopySpec{val_final, P->val_final, (size_t)L.N};
+    HIP_TRY(launch_copy_multi(cl, nc, s));
+  } else {
+    if (u) HIP_TRY(hipMemcpyAsync(u, L.u, L.N * P->esize, hipMemcpyDeviceToHost, s));
+    if (v) HIP_TRY(hipMemcpyAsync(v, L.v, L.N * P->esize, hipMemcpyDeviceToHost, s));
+    if (val_init) HIP_TRY(hipMemcpyAsync(val_init, L.flags, L.N, hipMemcpyDeviceToHost, s));
+    if (val_final) HIP_TRY(hipMemcpyAsync(val_final, P->val_final, L.N, hipMemcpyDeviceToHost, s));
+  }
   HIP_TRY(hipStreamSynchronize(s));
   return PIV_OK;
 }

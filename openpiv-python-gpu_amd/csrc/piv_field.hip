@@ -2,6 +2,7 @@
 // predictor interpolation, 3x3 median validation and spring replacement.
 #include <math.h>
 #include <stdio.h>
+#include <algorithm>
 
 #include "piv_fft.h"
 #include "piv_kernels.h"
@@ -1256,6 +1257,38 @@ hipError_t launch_copy(R* dst, const R* src, int count, hipStream_t stream) {
   hipLaunchKernelGGL(copy_kernel<R>, dim3((count + 255) / 256), dim3(256), 0, stream, dst, src, count);
   return hipGetLastError();
 }
+// Up to kCopyMax device-to-device copies in one launch (blockIdx.y picks the
+// copy): 16-byte moves where destination, source and size allow, bytes
+// otherwise.
+__global__ void copy_multi_kernel(CopyList l) {
+  const CopySpec c = l.c[blockIdx.y];
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const bool wide = ((reinterpret_cast<uintptr_t>(c.dst) | reinterpret_cast<uintptr_t>(c.src) | c.bytes) & 15) == 0;
+  if (wide) {
+    uint4* d = static_cast<uint4*>(c.dst);
+    const uint4* s = static_cast<const uint4*>(c.src);
+    const size_t n = c.bytes / 16;
+    for (size_t k = tid; k < n; k += stride) d[k] = s[k];
+  } else {
+    uint8_t* d = static_cast<uint8_t*>(c.dst);
+    const uint8_t* s = static_cast<const uint8_t*>(c.src);
+    for (size_t k = tid; k < c.bytes; k += stride) d[k] = s[k];
+  }
+}
+
+hipError_t launch_copy_multi(const CopyList& l, int n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  if (n > kCopyMax) return hipErrorInvalidValue;
+  size_t most = 0;
+  for (int k = 0; k < n; ++k) most = std::max(most, l.c[k].bytes);
+  // enough blocks to stream the largest copy at 16 bytes per thread and step
+  const size_t want = (most / 16 + 255) / 256;
+  const unsigned bx = (unsigned)std::min<size_t>(std::max<size_t>(want, 1), 2048);
+  hipLaunchKernelGGL(copy_multi_kernel, dim3(bx, n), dim3(256), 0, stream, l);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy_u8(uint8_t* dst, const uint8_t* src, int count, hipStream_t stream) {
   hipLaunchKernelGGL(copy_u8_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, dst, src, count);
   return hipGetLastError();

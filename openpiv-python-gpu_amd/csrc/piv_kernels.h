@@ -281,6 +281,17 @@ inline hipError_t launch_copy_f32(float* dst, const float* src, int count, hipSt
   return launch_copy<float>(dst, src, count, stream);
 }
 hipError_t launch_copy_u8(uint8_t* dst, const uint8_t* src, int count, hipStream_t stream);
+// Several device-to-device copies in one launch.
+constexpr int kCopyMax = 4;
+struct CopySpec {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+struct CopyList {
+  CopySpec c[kCopyMax];
+};
+hipError_t launch_copy_multi(const CopyList& l, int n, hipStream_t stream);
 // Masked pixels of both frames set to 0 in place (mask: 16-byte aligned).
 hipError_t launch_mask_frames(void* a, void* b, int in_dtype, const uint8_t* mask, size_t npx, hipStream_t stream);
 // u = v = NaN and s2n = 0 (when non-null) at masked windows.
