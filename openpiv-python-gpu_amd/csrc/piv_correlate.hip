@@ -305,7 +305,18 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
   dit2_combine<F, -1>(x, upper);
 
   // ---- stage P: spectrum product -------------------------------------------
-  if (kx != 0) {
+  // Column slot 0 holds FFT(col0 + i colN) of the real DC and Nyquist columns
+  // and needs the ky -> -ky mirror; its quad hands the two columns to the
+  // window's first F threads through LDS (one ky each) instead of working
+  // through all of them while the rest of the window waits.
+  c32* xq = reinterpret_cast<c32*>(smem + (size_t)WPB * region_floats) + (size_t)wl * 3 * F;
+  if (kx == 0) {
+    static_for<0, H>([&](auto KK) {
+      constexpr int k = decltype(KK)::value;
+      xq[sel * F + k + (upper ? H : 0)] = x[bitrev(k, H)];
+    });
+  }
+  {
     // A lanes: conj(own) * oth = conj(FA) FB; B lanes get its conjugate
     const float sgn = sel ? -1.f : 1.f;
     static_for<0, H>([&](auto RR) {
@@ -315,42 +326,26 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
       const c32 c = cconjmul(own, oth);
       x[r] = c32{c.x, sgn * c.y};
     });
-  } else {
-    // Column slot 0 holds FFT(col0 + i colN) of the real DC and Nyquist
-    // columns; unpack with the ky -> -ky mirror (partner lane), form both
-    // products and repack q = P0 + i PN (both columns invert to real data).
-    auto unpack = [](c32 qv, c32 qm, c32& f0, c32& fn) {
-      // f0 = (q + conj(qm)) / 2 ; fn = (q - conj(qm)) / (2i)
-      f0 = c32{0.5f * (qv.x + qm.x), 0.5f * (qv.y - qm.y)};
-      fn = c32{0.5f * (qv.y + qm.y), 0.5f * (qm.x - qv.x)};
-    };
-    auto product = [&](c32 f0, c32 fn) -> c32 {
-      c32 f0o = swap2(f0), fno = swap2(fn);
-      c32 a0 = sel ? f0o : f0, b0 = sel ? f0 : f0o;
-      c32 an = sel ? fno : fn, bn = sel ? fn : fno;
-      c32 p0 = cconjmul(a0, b0);
-      c32 pn = cconjmul(an, bn);
-      return c32{p0.x - pn.y, p0.y + pn.x};
-    };
-    c32 f0, fn;
-    unpack(x[0], x[0], f0, fn);  // local ky = 0 mirrors itself
-    c32 res0 = product(f0, fn);
-    static_for<1, H / 2 + 1>([&](auto KY) {
-      constexpr int ky = decltype(KY)::value;
-      constexpr int ky2 = H - ky;
-      constexpr int ra = bitrev(ky, H), rb = bitrev(ky2, H);
-      c32 qa = x[ra], qb = x[rb];
-      c32 ma = swap1(qb);  // partner local H-ky == mirror of local ky
-      c32 mb = swap1(qa);
-      c32 fa0, fan, fb0, fbn;
-      unpack(qa, ma, fa0, fan);
-      unpack(qb, mb, fb0, fbn);
-      c32 pa = product(fa0, fan);
-      c32 pb = product(fb0, fbn);
-      x[ra] = pa;
-      if constexpr (ky2 != ky) x[rb] = pb;
+  }
+  __syncthreads();
+  if (t < F) {
+    // unpack with the mirror: f0 = (q + conj(qm)) / 2, fn = (q - conj(qm)) / (2i);
+    // both products, repacked as q = P0 + i PN (both columns invert to real data)
+    const int ky = t, km = (F - t) & (F - 1);
+    const c32 qa = xq[ky], qam = xq[km], qb = xq[F + ky], qbm = xq[F + km];
+    const c32 fa0{0.5f * (qa.x + qam.x), 0.5f * (qa.y - qam.y)};
+    const c32 fan{0.5f * (qa.y + qam.y), 0.5f * (qam.x - qa.x)};
+    const c32 fb0{0.5f * (qb.x + qbm.x), 0.5f * (qb.y - qbm.y)};
+    const c32 fbn{0.5f * (qb.y + qbm.y), 0.5f * (qbm.x - qb.x)};
+    const c32 p0 = cconjmul(fa0, fb0), pn = cconjmul(fan, fbn);
+    xq[2 * F + ky] = c32{p0.x - pn.y, p0.y + pn.x};
+  }
+  __syncthreads();
+  if (kx == 0) {  // the A and B lanes of slot 0 both continue with q
+    static_for<0, H>([&](auto KK) {
+      constexpr int k = decltype(KK)::value;
+      x[bitrev(k, H)] = xq[2 * F + k + (upper ? H : 0)];
     });
-    x[0] = res0;
   }
 
   // ---- stage IC: inverse column FFT of P over the quad (4-lane DIF) -------
@@ -605,7 +600,7 @@ static hipError_t launch_fv(const CorrArgs& a, hipStream_t stream) {
   using L = Layout<F>;
   const int n_windows = a.m * a.n;
   const size_t region = correlate_lds_bytes(F, a.ss);
-  const size_t lds = region * L::WPB;
+  const size_t lds = (region + 3 * F * sizeof(c32)) * L::WPB;  // + the slot-0 exchange
   const int blocks = (n_windows + L::WPB - 1) / L::WPB;
   (void)hipGetLastError();  // do not report an earlier, unrelated error
   hipError_t e = lds_attr<F, InT, false, EXTRA, HALF>(lds);
@@ -629,7 +624,7 @@ template <int F, typename InT, bool HALF>
 static hipError_t launch_fbv(const CorrBatch& b, int total_blocks, int max_ss, hipStream_t stream) {
   using L = Layout<F>;
   const size_t region = correlate_lds_bytes(F, max_ss);
-  const size_t lds = region * L::WPB;
+  const size_t lds = (region + 3 * F * sizeof(c32)) * L::WPB;  // + the slot-0 exchange
   (void)hipGetLastError();
   hipError_t e = lds_attr<F, InT, true, true, HALF>(lds);
   if (e != hipSuccess) return e;
