@@ -55,7 +55,7 @@ struct Layout {
   static constexpr int SP = F + 1;         // plane row stride (float)
   // occupancy target (waves per SIMD) the register allocation is held to
   // (measured best per size; F=128 fits two 66.5 KB windows per CU)
-  static constexpr int WAVES = F <= 32 ? 8 : (F == 64 ? 4 : 2);
+  static constexpr int WAVES = F <= 32 ? 8 : (F == 64 ? 3 : 2);
 };
 
 }  // namespace
