@@ -423,26 +423,38 @@ __global__ __launch_bounds__(256, Layout<F>::WAVES) void corr_kernel(CorrArgs a_
   // ---- peak search over the fftshift-ed plane -----------------------------
   const float scale = 1.0f / (float)(4 * F * F);  // 4: the row split's factors of 2
   const int beta = (g == 0) ? 0 : (g == 1 ? 2 : (g == 2 ? 1 : 3));
-  Best best{-INFINITY, 0x7fffffff};
+  Best best;
   {
     const int i1 = (2 * jpair) ^ H;
-    const int i2 = (2 * jpair + 1) ^ H;
+    const int i2 = i1 + 1;
+    // this lane's columns jj = (k + beta Q) ^ H increase with k; rows i1 < i2
+    const int jj0 = (beta * Q) ^ H;
+    float* gp = (a.planes && valid) ? a.planes + (size_t)win * F * F : nullptr;
+    float zmax = -INFINITY;  // on the raw values: the scale is a power of two
     static_for<0, Q>([&](auto KK) {
       constexpr int k = decltype(KK)::value;
-      const int xc = k + beta * Q;
-      const int jj = xc ^ H;
-      c32 val = z[bitrev(k, Q)];
-      float c1 = val.x * scale, c2 = val.y * scale;
+      const int jj = jj0 + k;
+      const c32 val = z[bitrev(k, Q)];
+      const float c1 = val.x * scale, c2 = val.y * scale;
       plane[i1 * SP + jj] = c1;
       plane[i2 * SP + jj] = c2;
-      best = better(best, Best{c1, i1 * F + jj});
-      best = better(best, Best{c2, i2 * F + jj});
-      if (a.planes && valid) {
-        float* gp = a.planes + (size_t)win * F * F;
+      zmax = fmaxf(zmax, fmaxf(val.x, val.y));
+      if (gp) {
         gp[i1 * F + jj] = c1;
         gp[i2 * F + jj] = c2;
       }
     });
+    // first occurrence of the maximum: walk the lane's values backwards
+    int first = 0x7fffffff;
+    static_for<0, Q>([&](auto KK) {
+      constexpr int k = Q - 1 - decltype(KK)::value;
+      first = z[bitrev(k, Q)].y == zmax ? i2 * F + jj0 + k : first;
+    });
+    static_for<0, Q>([&](auto KK) {
+      constexpr int k = Q - 1 - decltype(KK)::value;
+      first = z[bitrev(k, Q)].x == zmax ? i1 * F + jj0 + k : first;
+    });
+    best = Best{zmax * scale, first};
   }
 #pragma unroll
   for (int off = WIDTH / 2; off > 0; off >>= 1) {

@@ -313,10 +313,11 @@ __global__ __launch_bounds__(256, LineLayout<F>::WAVES) void corr_line_kernel(Co
 
   // ---- plane (fftshift-ed) and peak ------------------------------------------
   const float scale = 1.0f / (float)(4 * F * F);  // 4: the split's factors of 2
-  Peak best{-INFINITY, 0x7fffffff};
+  Peak best;
   {
-    const int i1 = y1 ^ H, i2 = y2 ^ H;
+    const int i1 = y1 ^ H, i2 = i1 + 1;  // y1 even: rows i1 < i2
     float* gp = (a.planes && valid) ? a.planes + (size_t)win * F * F : nullptr;
+    float qmax = -INFINITY;  // on the raw values: the scale is a power of two
     static_for<0, F>([&](auto XX) {
       constexpr int xx = decltype(XX)::value;
       constexpr int jj = xx ^ H;
@@ -324,13 +325,23 @@ __global__ __launch_bounds__(256, LineLayout<F>::WAVES) void corr_line_kernel(Co
       const float c1 = val.x * scale, c2 = val.y * scale;
       plane[i1 * SP + jj] = c1;
       plane[i2 * SP + jj] = c2;
-      best = better(best, Peak{c1, i1 * F + jj});
-      best = better(best, Peak{c2, i2 * F + jj});
+      qmax = fmaxf(qmax, fmaxf(val.x, val.y));
       if (gp) {
         gp[i1 * F + jj] = c1;
         gp[i2 * F + jj] = c2;
       }
     });
+    // first occurrence of the maximum: walk the lane's columns backwards
+    int first = 0x7fffffff;
+    static_for<0, F>([&](auto JJ) {
+      constexpr int jj = F - 1 - decltype(JJ)::value;
+      first = q[bitrev(jj ^ H, F)].y == qmax ? i2 * F + jj : first;
+    });
+    static_for<0, F>([&](auto JJ) {
+      constexpr int jj = F - 1 - decltype(JJ)::value;
+      first = q[bitrev(jj ^ H, F)].x == qmax ? i1 * F + jj : first;
+    });
+    best = Peak{qmax * scale, first};
   }
 #pragma unroll
   for (int off = H / 2; off > 0; off >>= 1) {

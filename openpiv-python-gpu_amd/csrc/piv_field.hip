@@ -43,12 +43,15 @@ template <> __device__ __forceinline__ double vnan<double>() { return __builtin_
 template <typename R>
 __device__ __forceinline__ void grid_bilinear(const R* fu, const R* fv, int m, int n, R y0, R x0, R sp, R yq, R xq,
                                               R& ou, R& ov) {
-  int iy = (int)vfloor((yq - y0) / sp);
-  int ix = (int)vfloor((xq - x0) / sp);
+  // one division, then products (exact for the power-of-two spacings of the
+  // usual schedules; otherwise within a rounding of the quotient)
+  const R inv = R(1) / sp;
+  int iy = (int)vfloor((yq - y0) * inv);
+  int ix = (int)vfloor((xq - x0) * inv);
   iy = iy < 0 ? 0 : (iy > m - 2 ? m - 2 : iy);
   ix = ix < 0 ? 0 : (ix > n - 2 ? n - 2 : ix);
-  const R ty = (yq - (y0 + (R)iy * sp)) / sp;
-  const R tx = (xq - (x0 + (R)ix * sp)) / sp;
+  const R ty = (yq - (y0 + (R)iy * sp)) * inv;
+  const R tx = (xq - (x0 + (R)ix * sp)) * inv;
   const R wy0 = R(1) - ty, wx0 = R(1) - tx;
   const size_t i00 = (size_t)iy * n + ix;
   const size_t i10 = i00 + n;
