@@ -30,7 +30,7 @@ __device__ __forceinline__ double sfma(double a, double b, double c) { return fm
 //   C_f(r, c) = sum_k A_f(r, k) B_f(k, c),  A(r,k) = ta ? A[k*lda + r] : A[r*lda + k]
 //                                           B(k,c) = tb ? B[c*ldb + k] : B[k*ldb + c]
 // 16 x 16 output tile per 256-thread workgroup (one output per thread), K
-// staged through LDS in chunks of 16; FP32 with FMA accumulation.
+// staged through LDS in chunks of 64; FMA accumulation in k order.
 // ---------------------------------------------------------------------------
 template <typename R>
 struct GemmArgs {
@@ -44,33 +44,37 @@ struct GemmArgs {
 
 template <typename R>
 __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs<R> g) {
-  __shared__ R As[16][17];
-  __shared__ R Bs[16][17];
+  constexpr int KC = 64;  // K per LDS stage: four loads of A and four of B in flight per thread
+  __shared__ R As[16][KC + 1];
+  __shared__ R Bs[KC][17];
   const int f = blockIdx.z;
   const R* A = g.a[f];
   const R* B = g.b[f];
   const int r0 = blockIdx.y * 16, c0 = blockIdx.x * 16;
   const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
   R acc = R(0);
-  for (int k0 = 0; k0 < g.K; k0 += 16) {
-    {
-      // As[row][k]: the thread index runs along the contiguous direction
-      const int rr = g.ta ? tx : ty, kk = g.ta ? ty : tx;
-      const int r = r0 + rr, k = k0 + kk;
-      R val = R(0);
-      if (r < g.M && k < g.K) val = g.ta ? A[(size_t)k * g.lda + r] : A[(size_t)r * g.lda + k];
-      As[rr][kk] = val;
+  for (int k0 = 0; k0 < g.K; k0 += KC) {
+    R va[4], vb[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      // consecutive threads run along the contiguous direction of each operand
+      const int e = threadIdx.x + 256 * i;
+      const int rr = g.ta ? e % 16 : e / KC, ka = g.ta ? e / 16 : e % KC;
+      const int r = r0 + rr, k = k0 + ka;
+      va[i] = (r < g.M && k < g.K) ? (g.ta ? A[(size_t)k * g.lda + r] : A[(size_t)r * g.lda + k]) : R(0);
+      const int kb = g.tb ? e % KC : e / 16, cc = g.tb ? e / KC : e % 16;
+      const int k2 = k0 + kb, c = c0 + cc;
+      vb[i] = (k2 < g.K && c < g.N) ? (g.tb ? B[(size_t)c * g.ldb + k2] : B[(size_t)k2 * g.ldb + c]) : R(0);
     }
-    {
-      const int kk = g.tb ? tx : ty, cc = g.tb ? ty : tx;
-      const int k = k0 + kk, c = c0 + cc;
-      R val = R(0);
-      if (k < g.K && c < g.N) val = g.tb ? B[(size_t)c * g.ldb + k] : B[(size_t)k * g.ldb + c];
-      Bs[kk][cc] = val;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = threadIdx.x + 256 * i;
+      As[g.ta ? e % 16 : e / KC][g.ta ? e / 16 : e % KC] = va[i];
+      Bs[g.tb ? e % KC : e / 16][g.tb ? e / KC : e % 16] = vb[i];
     }
     __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < 16; ++kk) acc = sfma(As[ty][kk], Bs[kk][tx], acc);
+    const int kn = min(KC, g.K - k0);
+    for (int kk = 0; kk < kn; ++kk) acc = sfma(As[ty][kk], Bs[kk][tx], acc);
     __syncthreads();
   }
   const int r = r0 + ty, c = c0 + tx;
@@ -103,6 +107,7 @@ hipError_t gemm2(const R* a0, const R* a1, int lda, int ta, const R* b0, const R
 // ---------------------------------------------------------------------------
 constexpr int kGcvBlocks = 64;
 constexpr int kGcvThreads = 1024;
+constexpr int kNewtonInBlockMax = 32768;  // spectra up to this size take the Newton steps in one workgroup
 constexpr int kSpan = 10;  // N_P0, gpu_smoothn.py:33
 constexpr int kGrid = 64;
 constexpr int kSpanStep = (kGrid - 1) / (kSpan - 1);  // grid points per span interval
@@ -247,10 +252,14 @@ __device__ void gcv_decide(int N, GcvState* st, double p_min, double p_max, cons
 // One evaluation: every block scores the pending p-values over its slice of
 // the spectrum and writes per-block partials (p-major); the last block to
 // finish sums them in block order and decides the next p-values.
+// newton_here (grid evaluation only): the last block also runs both Newton
+// steps itself, on a copy of the search state in LDS, and writes the state
+// back at the end -- for spectra small enough that one workgroup evaluates a
+// p-value faster than a launch of kGcvBlocks does.
 template <int PHASE, typename R>
 __global__ __launch_bounds__(kGcvThreads) void gcv_eval_kernel(const R* Y, const double* lam, int N, GcvState* st,
                                                                double* part, double p_min, double p_max,
-                                                               const int32_t* skip) {
+                                                               const int32_t* skip, int newton_here) {
   if (skip && *skip) return;
   constexpr int NS = PHASE >= 1 ? 6 : 4;
   // the grid evaluation spreads 64 p-values over the lanes of each wave; a
@@ -339,7 +348,58 @@ __global__ __launch_bounds__(kGcvThreads) void gcv_eval_kernel(const R* Y, const
   }
   if (tid == 0) st->arrivals = 0;
   __syncthreads();
-  if (tid < 64) gcv_decide<PHASE>(N, st, p_min, p_max, tot);
+  if (PHASE != 0 || !newton_here) {
+    if (tid < 64) gcv_decide<PHASE>(N, st, p_min, p_max, tot);
+    return;
+  }
+  __shared__ GcvState ls;
+  __shared__ double red6[kGcvThreads / 64][6];
+  if (tid < 64) gcv_decide<0>(N, &ls, p_min, p_max, tot);
+  auto newton = [&](auto STEP) {
+    __syncthreads();  // ls written by wave 0
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (!ls.done) {
+      const double s = pow(10.0, ls.pv[0]);
+      for (int k = tid; k < N; k += kGcvThreads) {
+        const double y0 = (double)Y[k], y1 = (double)Y[N + k];
+        double t[6];
+        gcv_terms(y0 * y0 + y1 * y1, lam[k], s, t);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[q] += t[q];
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) acc[q] += __shfl_xor(acc[q], off, 64);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) red6[wave][q] = acc[q];
+    }
+    __syncthreads();
+    if (tid < 6) {
+      double v = 0.0;
+      for (int w = 0; w < kGcvThreads / 64; ++w) v += red6[w][tid];
+      tot[0][tid] = v;
+    }
+    __syncthreads();
+    if (tid < 64) gcv_decide<decltype(STEP)::value>(N, &ls, p_min, p_max, tot);
+  };
+  newton(std::integral_constant<int, 1>{});
+  newton(std::integral_constant<int, 2>{});
+  __syncthreads();
+  if (tid < kGrid) st->pv[tid] = ls.pv[tid];
+  if (tid == 0) {
+    st->npv = ls.npv;
+    st->a = ls.a;
+    st->b = ls.b;
+    st->p_best_span = ls.p_best_span;
+    st->done = ls.done;
+    st->p = ls.p;
+    st->p_min = ls.p_min;
+    st->p_max = ls.p_max;
+  }
 }
 
 // z_dct = Gamma * f_dct (gpu_smoothn.py:196-199); p from the search state
@@ -363,9 +423,13 @@ template <typename R>
 void enqueue_gcv_search(const R* Y, const double* lam, int N, GcvState* st, double* part, double p_min,
                         double p_max, const int32_t* skip, hipStream_t stream) {
   const dim3 eb(kGcvBlocks), et(kGcvThreads);
-  hipLaunchKernelGGL((gcv_eval_kernel<0, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
-  hipLaunchKernelGGL((gcv_eval_kernel<1, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
-  hipLaunchKernelGGL((gcv_eval_kernel<2, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
+  const int newton_here = N <= kNewtonInBlockMax;
+  hipLaunchKernelGGL((gcv_eval_kernel<0, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip,
+                     newton_here);
+  if (!newton_here) {
+    hipLaunchKernelGGL((gcv_eval_kernel<1, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip, 0);
+    hipLaunchKernelGGL((gcv_eval_kernel<2, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip, 0);
+  }
 }
 
 }  // namespace
