@@ -124,6 +124,17 @@ void piv_plan_destroy(piv_plan *plan);
 int piv_run(piv_plan *plan, const void *frame_a, const void *frame_b, int frames_on_device, void *u, void *v,
             uint8_t *val_init, uint8_t *val_final, int outputs_on_device);
 
+/* piv_run with device frames and device outputs that returns once the run is
+ * queued on the plan's stream (no counterpart in the reference, whose
+ * piv_gpu call is synchronous; for callers that stream pairs through one
+ * plan).  Runs queued back to back execute in order; inputs must stay valid
+ * and outputs and diagnostics are ready only after piv_plan_synchronize.
+ * Plans with a learned predictor return PIV_E_INVALID (use piv_run). */
+int piv_run_async(piv_plan *plan, const void *frame_a, const void *frame_b, void *u, void *v, uint8_t *val_init,
+                  uint8_t *val_final);
+/* Waits for every run queued on the plan's stream. */
+int piv_plan_synchronize(piv_plan *plan);
+
 /* Diagnostics of the last run: smoothing parameter log10(s) chosen after
  * pass k (NaN when no smoothing ran), and outlier counts before/after
  * replacement. */

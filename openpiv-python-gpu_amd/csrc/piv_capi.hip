@@ -873,8 +873,8 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
 
 void piv_plan_destroy(piv_plan* plan) { release(plan); }
 
-int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on_device, void* u, void* v,
-            uint8_t* val_init, uint8_t* val_final, int outputs_on_device) {
+static int run_impl(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on_device, void* u, void* v,
+                    uint8_t* val_init, uint8_t* val_final, int outputs_on_device, bool sync) {
   if (!P || !frame_a || !frame_b) return fail(PIV_E_INVALID, "null argument");
   HIP_TRY(hipSetDevice(P->device));
   hipStream_t s = P->stream;
@@ -935,7 +935,28 @@ int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on
     if (val_init) HIP_TRY(hipMemcpyAsync(val_init, L.flags, L.N, hipMemcpyDeviceToHost, s));
     if (val_final) HIP_TRY(hipMemcpyAsync(val_final, P->val_final, L.N, hipMemcpyDeviceToHost, s));
   }
-  HIP_TRY(hipStreamSynchronize(s));
+  if (sync) HIP_TRY(hipStreamSynchronize(s));
+  return PIV_OK;
+}
+
+int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on_device, void* u, void* v,
+            uint8_t* val_init, uint8_t* val_final, int outputs_on_device) {
+  return run_impl(P, frame_a, frame_b, frames_on_device, u, v, val_init, val_final, outputs_on_device, true);
+}
+
+int piv_run_async(piv_plan* P, const void* frame_a, const void* frame_b, void* u, void* v, uint8_t* val_init,
+                  uint8_t* val_final) {
+  if (!P) return fail(PIV_E_INVALID, "null plan");
+  if (P->net)
+    return fail(PIV_E_INVALID, "piv_run_async: a plan with a learned predictor decides its fallback on the host; "
+                               "use piv_run");
+  return run_impl(P, frame_a, frame_b, 1, u, v, val_init, val_final, 1, false);
+}
+
+int piv_plan_synchronize(piv_plan* P) {
+  if (!P) return fail(PIV_E_INVALID, "null plan");
+  HIP_TRY(hipSetDevice(P->device));
+  HIP_TRY(hipStreamSynchronize(P->stream));
   return PIV_OK;
 }
 

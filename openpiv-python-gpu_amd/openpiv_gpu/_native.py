@@ -25,7 +25,8 @@ LIB_PATH = os.environ.get("PIV_HIP_LIB", os.path.join(_HERE, "_lib", "libopenpiv
 
 # Every symbol include/openpiv_hip.h declares.
 EXPORTED_SYMBOLS = (
-    "piv_last_error", "piv_device_count", "piv_plan_create", "piv_plan_destroy", "piv_run",
+    "piv_last_error", "piv_device_count", "piv_plan_create", "piv_plan_destroy", "piv_run", "piv_run_async",
+    "piv_plan_synchronize",
     "piv_plan_last_smoothing", "piv_plan_last_counts", "piv_stage_correlate", "piv_stage_deform",
     "piv_stage_validate", "piv_stage_replace",
     "piv_stage_grid_interp", "piv_stage_median_validate", "piv_stage_spring_replace", "piv_stage_smoothn",
@@ -137,6 +138,8 @@ def _bind(lib):
         "piv_plan_create": (i32, [i32, ctypes.POINTER(PlanDesc), ctypes.POINTER(vp)]),
         "piv_plan_destroy": (None, [vp]),
         "piv_run": (i32, [vp, vp, vp, i32, vp, vp, vp, vp, i32]),
+        "piv_run_async": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+        "piv_plan_synchronize": (i32, [vp]),
         "piv_plan_last_smoothing": (i32, [vp, i32, ctypes.POINTER(f64)]),
         "piv_plan_last_counts": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
         "piv_stage_correlate": (i32, [vp, vp, i32, i32, i32, ctypes.POINTER(PassDesc), vp, vp, vp, vp, vp, vp, vp,
@@ -391,6 +394,13 @@ class Plan:
     def run(self, frame_a, frame_b, frames_on_device, u, v, val_init, val_final, outputs_on_device):
         check(self._lib.piv_run(self.handle, frame_a, frame_b, int(frames_on_device), u, v, val_init, val_final,
                                 int(outputs_on_device)))
+
+    def run_async(self, frame_a, frame_b, u, v, val_init=None, val_final=None):
+        """Queues one run on device frames and device outputs; see synchronize()."""
+        check(self._lib.piv_run_async(self.handle, frame_a, frame_b, u, v, val_init, val_final))
+
+    def synchronize(self):
+        check(self._lib.piv_plan_synchronize(self.handle))
 
     def last_smoothing(self, k: int) -> float:
         out = ctypes.c_double()

@@ -108,3 +108,34 @@ def test_2048_accuracy_vs_truth(cfg2_runs):
     rms_gpu = np.sqrt(np.mean((u - ut) ** 2 + (v - vt) ** 2) / 2)
     rms_ref = np.sqrt(np.mean((ref["u"] - ut) ** 2 + (ref["v"] - vt) ** 2) / 2)
     assert abs(rms_gpu - rms_ref) < 1e-3 and rms_gpu < 0.1
+
+
+def test_queued_runs_match_synchronous_runs():
+    # piv_run_async: runs queued back to back on the plan's stream (two pairs
+    # alternating, distinct outputs) give the synchronous call's results bit
+    # for bit once the plan is synchronised
+    from openpiv_gpu import _native, synthetic
+    from openpiv_gpu.gpu_process import PIVGPU
+    pairs = [synthetic.cfg_512_uint8(seed)[:2] for seed in (3, 4)]
+    piv = PIVGPU((512, 512), 16, search_size_iters=(1, 1), overlap_ratio=0.5, median_tol=2, dtype_f="float32")
+    plan = piv._get_plan(_native.PIV_U8)
+    m, n = piv.piv_fields[-1].field_shape
+    dev = [(_native.DeviceBuffer.from_array(a), _native.DeviceBuffer.from_array(b)) for a, b in pairs]
+    ref = []
+    for da, db in dev:
+        u = _native.DeviceBuffer((m, n), np.float32)
+        v = _native.DeviceBuffer((m, n), np.float32)
+        f = _native.DeviceBuffer((m, n), np.uint8)
+        plan.run(da.value, db.value, True, u.value, v.value, f.value, None, True)
+        ref.append((u.get(), v.get(), f.get()))
+    outs = []
+    for k in range(6):
+        da, db = dev[k % 2]
+        o = (_native.DeviceBuffer((m, n), np.float32), _native.DeviceBuffer((m, n), np.float32),
+             _native.DeviceBuffer((m, n), np.uint8))
+        plan.run_async(da.value, db.value, o[0].value, o[1].value, o[2].value, None)
+        outs.append(o)
+    plan.synchronize()
+    for k, (u, v, f) in enumerate(outs):
+        r = ref[k % 2]
+        assert np.array_equal(u.get(), r[0]) and np.array_equal(v.get(), r[1]) and np.array_equal(f.get(), r[2]), k
