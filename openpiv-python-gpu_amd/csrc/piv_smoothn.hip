@@ -32,6 +32,8 @@ __device__ __forceinline__ double sfma(double a, double b, double c) { return fm
 // 16 x 16 output tile per 256-thread workgroup (one output per thread), K
 // staged through LDS in chunks of 64; FMA accumulation in k order.
 // ---------------------------------------------------------------------------
+struct GcvState;
+
 template <typename R>
 struct GemmArgs {
   const R* a[2];
@@ -40,7 +42,15 @@ struct GemmArgs {
   int lda, ldb, ldc;
   int ta, tb;
   int M, N, K;
+  // optional: B(k, c) scaled by Gamma = 1 / (1 + 10^p lam[k * ldb + c])
+  // (untransposed B), p from the search state or fixed; p written to p_out
+  const double* lam;
+  const GcvState* gst;
+  double p_fixed;
+  double* p_out;
 };
+
+__device__ double gcv_state_p(const GcvState* st);
 
 template <typename R>
 __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs<R> g) {
@@ -52,6 +62,12 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs<R> g) {
   const R* B = g.b[f];
   const int r0 = blockIdx.y * 16, c0 = blockIdx.x * 16;
   const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+  double sgam = 0.0;
+  if (g.lam) {
+    const double p = g.gst ? gcv_state_p(g.gst) : g.p_fixed;
+    sgam = pow(10.0, p);
+    if (g.p_out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) *g.p_out = p;
+  }
   R acc = R(0);
   for (int k0 = 0; k0 < g.K; k0 += KC) {
     R va[4], vb[4];
@@ -65,6 +81,10 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs<R> g) {
       const int kb = g.tb ? e % KC : e / 16, cc = g.tb ? e / KC : e % 16;
       const int k2 = k0 + kb, c = c0 + cc;
       vb[i] = (k2 < g.K && c < g.N) ? (g.tb ? B[(size_t)c * g.ldb + k2] : B[(size_t)k2 * g.ldb + c]) : R(0);
+      if (g.lam && k2 < g.K && c < g.N) {  // z_dct = Gamma f_dct (gpu_smoothn.py:196-199)
+        const double G = 1.0 / (1.0 + sgam * g.lam[(size_t)k2 * g.ldb + c]);
+        vb[i] = (R)(G * (double)vb[i]);
+      }
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -83,8 +103,9 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs<R> g) {
 
 template <typename R>
 hipError_t gemm2(const R* a0, const R* a1, int lda, int ta, const R* b0, const R* b1, int ldb, int tb, R* c0, R* c1,
-                 int ldc, int M, int N, int K, hipStream_t s) {
-  GemmArgs<R> g{{a0, a1}, {b0, b1}, {c0, c1}, lda, ldb, ldc, ta, tb, M, N, K};
+                 int ldc, int M, int N, int K, hipStream_t s, const double* lam = nullptr,
+                 const GcvState* gst = nullptr, double p_fixed = 0.0, double* p_out = nullptr) {
+  GemmArgs<R> g{{a0, a1}, {b0, b1}, {c0, c1}, lda, ldb, ldc, ta, tb, M, N, K, lam, gst, p_fixed, p_out};
   dim3 grid((N + 15) / 16, (M + 15) / 16, 2);
   hipLaunchKernelGGL(gemm_small_kernel<R>, grid, dim3(256), 0, s, g);
   return hipGetLastError();
@@ -123,6 +144,8 @@ struct GcvState {
   double p_min, p_max;
   uint32_t arrivals;    // blocks of the current evaluation that finished (0 between launches)
 };
+
+__device__ double gcv_state_p(const GcvState* st) { return st->p; }
 
 // Sums per p-value: RSS, T, RSS', T', RSS'', T'' (L = ln 10, e = 1 - Gamma):
 //   RSS' = 2L sum S e^2 G ; RSS'' = 2L^2 sum S e^2 G (3G - 1)
@@ -402,20 +425,6 @@ __global__ __launch_bounds__(kGcvThreads) void gcv_eval_kernel(const R* Y, const
   }
 }
 
-// z_dct = Gamma * f_dct (gpu_smoothn.py:196-199); p from the search state
-// (p_state) or fixed (p_state == null).
-template <typename R>
-__global__ void gamma_apply_kernel(R* Y, const double* lam, int N, double p_fixed, const GcvState* p_state,
-                                   double* p_out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const double p = p_state ? p_state->p : p_fixed;
-  if (k < N) {
-    const double G = 1.0 / (1.0 + pow(10.0, p) * lam[k]);
-    Y[k] = (R)(G * (double)Y[k]);
-    Y[N + k] = (R)(G * (double)Y[N + k]);
-  }
-  if (k == 0 && p_out) *p_out = p;
-}
 
 // The GCV search on the spectrum Y (two fields, 2N floats): span with the
 // bracket grid, two Newton steps; leaves p in st->p.
@@ -452,16 +461,13 @@ hipError_t launch_smoothn(R* u, R* v, int m, int n, double p_min, double p_max, 
     GcvState* st = reinterpret_cast<GcvState*>(w.state);
     double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(w.state) + kGcvPartOffset);
     enqueue_gcv_search(w.t1, w.lambda, N, st, part, p_min, p_max, nullptr, stream);
-    hipLaunchKernelGGL(gamma_apply_kernel<R>, dim3((N + 255) / 256), dim3(256), 0, stream, w.t1, w.lambda, N, 0.0,
-                       (const GcvState*)st, w.p_out);
-  } else {
-    hipLaunchKernelGGL(gamma_apply_kernel<R>, dim3((N + 255) / 256), dim3(256), 0, stream, w.t1, w.lambda, N, fixed_p,
-                       (const GcvState*)nullptr, w.p_out);
   }
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  // inverse: T0 = Cm^T T1 ; Z = T0 Cn
-  e = gemm2(w.cos_m, w.cos_m, m, 1, w.t1, w.t1 + N, n, 0, w.t0, w.t0 + N, n, m, n, m, stream);
+  // inverse: T0 = Cm^T (Gamma T1) ; Z = T0 Cn  (the Gamma scaling is applied
+  // while T1 is read)
+  e = gemm2(w.cos_m, w.cos_m, m, 1, w.t1, w.t1 + N, n, 0, w.t0, w.t0 + N, n, m, n, m, stream, w.lambda,
+            auto_s ? reinterpret_cast<const GcvState*>(w.state) : nullptr, fixed_p, w.p_out);
   if (e != hipSuccess) return e;
   return gemm2(w.t0, w.t0 + N, n, 0, w.cos_n, w.cos_n, n, 0, u, v, n, m, n, n, stream);
 }
