@@ -124,7 +124,7 @@ hipError_t gemm2(const R* a0, const R* a1, int lda, int ta, const R* b0, const R
 //            of dg/dp nearest the span minimum (9x finer than the span) ->
 //            secant root
 //   phase 1, 2: safeguarded Newton steps on dg/dp = 0 from the secant root
-// then a separate launch scales the spectrum by Gamma(p).
+// then the inverse DCT scales the spectrum by Gamma(p) as it reads it.
 // ---------------------------------------------------------------------------
 constexpr int kGcvBlocks = 64;
 constexpr int kGcvThreads = 1024;
