@@ -383,12 +383,26 @@ __global__ __launch_bounds__(kGcvThreads) void gcv_eval_kernel(const R* Y, const
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (!ls.done) {
       const double s = pow(10.0, ls.pv[0]);
-      for (int k = tid; k < N; k += kGcvThreads) {
-        const double y0 = (double)Y[k], y1 = (double)Y[N + k];
-        double t[6];
-        gcv_terms(y0 * y0 + y1 * y1, lam[k], s, t);
+      // four elements' loads in flight per step; accumulated in element order
+      for (int k = tid; k < N; k += 4 * kGcvThreads) {
+        double S4[4], L4[4];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) acc[q] += t[q];
+        for (int e = 0; e < 4; ++e) {
+          const int kk = k + e * kGcvThreads;
+          const bool in = kk < N;
+          const double y0 = in ? (double)Y[kk] : 0.0, y1 = in ? (double)Y[N + kk] : 0.0;
+          S4[e] = y0 * y0 + y1 * y1;
+          L4[e] = in ? lam[kk] : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (k + e * kGcvThreads < N) {
+            double t[6];
+            gcv_terms(S4[e], L4[e], s, t);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) acc[q] += t[q];
+          }
+        }
       }
     }
 #pragma unroll
