@@ -25,7 +25,7 @@ Reported per size and path:
   * wall time per path (host call, device-resident windows).
 
 Run on a GPU host from the repository root:
-    python tests/precision_sweep.py --out gpurun_out/precision.json
+    python tests/precision_sweep.py --out precision.json
 """
 
 import argparse

@@ -1,5 +1,5 @@
 """Stage-level parity of the HIP kernels against the CPU oracle, through the C
-ABI stage entry points.  Tolerances (north-star bar):
+ABI stage entry points.  Tolerances:
   * integer peak indices and the window grid bit-exact, except windows whose
     two largest correlation values tie within 1e-5 (counted, asserted rare);
   * u, v within 1e-3 px max-abs of the oracle's FP32 path;
