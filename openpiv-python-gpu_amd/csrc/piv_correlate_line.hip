@@ -51,6 +51,8 @@ struct LineLayout {
   static constexpr int SP = F + 1;  // plane row stride (float)
   // occupancy the register allocation is held to (LDS allows about this many)
   static constexpr int WAVES = F == 16 ? 4 : 2;
+  // waves per workgroup: one, so the LDS granularity is a wave's windows
+  static constexpr int BW = 1;
 };
 
 struct Peak {
@@ -81,7 +83,7 @@ size_t correlate_line_lds_floats(int fsize, int ss) {
 // belongs to the last field whose block_prefix entry is <= b, so the field's
 // arguments stay uniform over the block.
 template <int F, typename InT, bool BATCHED>
-__global__ __launch_bounds__(256, LineLayout<F>::WAVES) void corr_line_kernel(CorrArgs a_in, CorrBatch batch,
+__global__ __launch_bounds__(64 * LineLayout<F>::BW, LineLayout<F>::WAVES) void corr_line_kernel(CorrArgs a_in, CorrBatch batch,
                                                                               int n_windows_in, int region_floats) {
   using L = LineLayout<F>;
   constexpr int H = L::H, G = L::G, SF = L::SF, SR = L::SR, SP = L::SP;
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(256, LineLayout<F>::WAVES) void corr_line_kernel(Co
     a = a_in;
     n_windows = n_windows_in;
   }
-  const int win = (bid * 4 + wave) * G + g;
+  const int win = (bid * L::BW + wave) * G + g;
   const bool valid = win < n_windows;
   const int W = a.ss;
   const int pad = (F - W) / 2;
@@ -412,13 +414,13 @@ hipError_t launch_line_t(const CorrArgs& a, hipStream_t stream) {
   using L = LineLayout<F>;
   const int n_windows = a.m * a.n;
   const int region = (int)correlate_line_lds_floats(F, a.ss);
-  const size_t lds = (size_t)region * sizeof(float) * L::G * 4;
-  const int per_block = L::G * 4;
+  const size_t lds = (size_t)region * sizeof(float) * L::G * L::BW;
+  const int per_block = L::G * L::BW;
   const int blocks = (n_windows + per_block - 1) / per_block;
   (void)hipGetLastError();
   hipError_t e = line_lds_attr<F, InT, false>(lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((corr_line_kernel<F, InT, false>), dim3(blocks), dim3(256), lds, stream, a, CorrBatch{},
+  hipLaunchKernelGGL((corr_line_kernel<F, InT, false>), dim3(blocks), dim3(64 * L::BW), lds, stream, a, CorrBatch{},
                      n_windows, region);
   e = hipGetLastError();
   if (e != hipSuccess) set_launch_detail("corr_line_kernel launch", e);
@@ -429,11 +431,11 @@ template <int F, typename InT>
 hipError_t launch_line_batch_t(const CorrBatch& b, int total_blocks, int max_ss, hipStream_t stream) {
   using L = LineLayout<F>;
   const int region = (int)correlate_line_lds_floats(F, max_ss);
-  const size_t lds = (size_t)region * sizeof(float) * L::G * 4;
+  const size_t lds = (size_t)region * sizeof(float) * L::G * L::BW;
   (void)hipGetLastError();
   hipError_t e = line_lds_attr<F, InT, true>(lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((corr_line_kernel<F, InT, true>), dim3(total_blocks), dim3(256), lds, stream, CorrArgs{}, b,
+  hipLaunchKernelGGL((corr_line_kernel<F, InT, true>), dim3(total_blocks), dim3(64 * L::BW), lds, stream, CorrArgs{}, b,
                      0, region);
   e = hipGetLastError();
   if (e != hipSuccess) set_launch_detail("corr_line_kernel (batched) launch", e);
@@ -470,7 +472,9 @@ bool correlate_line_enabled() {
   return on;
 }
 
-int correlate_line_windows_per_block(int fsize) { return 4 * (128 / fsize); }
+int correlate_line_windows_per_block(int fsize) {
+  return fsize == 16 ? LineLayout<16>::G * LineLayout<16>::BW : LineLayout<32>::G * LineLayout<32>::BW;
+}
 
 hipError_t launch_correlate_line_batch(const CorrBatch& b, int fsize, int in_dtype, int total_blocks, int max_ss,
                                        hipStream_t stream) {
