@@ -673,12 +673,12 @@ int enqueue_passes(piv_plan* P, int first) {
   return P->fp64 ? enqueue_passes_t<double>(P, first) : enqueue_passes_t<float>(P, first);
 }
 
+// The run's counters (outlier counts, replacement iteration counts) are
+// cleared by piv_run's input launch before the graph starts.
 int enqueue(piv_plan* P) {
   if (P->frame_mask)
     HIP_TRY(launch_mask_frames(P->frame_a, P->frame_b, P->desc.in_dtype, P->frame_mask,
                                (size_t)P->desc.height * P->desc.width, P->stream));
-  HIP_TRY(launch_fill_i32(P->counters, 0, 2 * (int)P->passes.size(), P->stream));
-  HIP_TRY(launch_fill_i32(P->lb.it_counts, 0, PIV_MAX_REPLACING_ITERS, P->stream));
   return enqueue_passes(P, 0);
 }
 
@@ -688,8 +688,6 @@ int enqueue_learned(piv_plan* P) {
   hipStream_t s = P->stream;
   const piv_plan_desc& D = P->desc;
   PassBuffers& B = P->passes[P->net_first];
-  HIP_TRY(launch_fill_i32(P->counters, 0, 2 * (int)P->passes.size(), s));
-  HIP_TRY(launch_fill_i32(P->lb.it_counts, 0, PIV_MAX_REPLACING_ITERS, s));
   HIP_TRY(launch_prepare_input(src_a(P), src_b(P), D.in_dtype, D.height, D.width, kCnnFactor, P->net_scratch,
                                P->net_in, s));
   int r = cnn_enqueue_forward(P->net, P->net_in, 1, s);
@@ -889,14 +887,20 @@ static int run_impl(piv_plan* P, const void* frame_a, const void* frame_b, int f
     HIP_TRY(hipStreamWaitEvent(s, P->wait_next, 0));
     P->wait_next = nullptr;
   }
-  if (frames_on_device) {
+  {
+    // frames in, and the run's counters cleared, in one launch
     CopyList cl{};
-    cl.c[0] = CopySpec{P->frame_a, frame_a, P->frame_bytes};
-    cl.c[1] = CopySpec{P->frame_b, frame_b, P->frame_bytes};
-    HIP_TRY(launch_copy_multi(cl, 2, s));
-  } else {
-    HIP_TRY(hipMemcpyAsync(P->frame_a, frame_a, P->frame_bytes, in_kind, s));
-    HIP_TRY(hipMemcpyAsync(P->frame_b, frame_b, P->frame_bytes, in_kind, s));
+    int nc = 0;
+    cl.c[nc++] = CopySpec{P->counters, nullptr, 2 * P->passes.size() * sizeof(int32_t)};
+    cl.c[nc++] = CopySpec{P->lb.it_counts, nullptr, PIV_MAX_REPLACING_ITERS * sizeof(int32_t)};
+    if (frames_on_device) {
+      cl.c[nc++] = CopySpec{P->frame_a, frame_a, P->frame_bytes};
+      cl.c[nc++] = CopySpec{P->frame_b, frame_b, P->frame_bytes};
+    } else {
+      HIP_TRY(hipMemcpyAsync(P->frame_a, frame_a, P->frame_bytes, in_kind, s));
+      HIP_TRY(hipMemcpyAsync(P->frame_b, frame_b, P->frame_bytes, in_kind, s));
+    }
+    HIP_TRY(launch_copy_multi(cl, nc, s));
   }
   // The first run of each path is eager (it also sets kernel attributes);
   // later runs replay a captured graph when requested.
@@ -913,6 +917,10 @@ static int run_impl(piv_plan* P, const void* frame_a, const void* frame_b, int f
     P->last_rejected = rejected;
     if ((double)rejected > (double)P->max_reject * L.N) {
       P->last_fallback = 1;
+      CopyList cl{};  // counters of the full-schedule run start at zero
+      cl.c[0] = CopySpec{P->counters, nullptr, 2 * P->passes.size() * sizeof(int32_t)};
+      cl.c[1] = CopySpec{P->lb.it_counts, nullptr, PIV_MAX_REPLACING_ITERS * sizeof(int32_t)};
+      HIP_TRY(launch_copy_multi(cl, 2, s));
       r = run_captured(P, enqueue, &P->graph, &P->graph_exec, &P->runs);
       if (r) return r;
     }

@@ -1262,11 +1262,16 @@ hipError_t launch_copy(R* dst, const R* src, int count, hipStream_t stream) {
 }
 // Up to kCopyMax device-to-device copies in one launch (blockIdx.y picks the
 // copy): 16-byte moves where destination, source and size allow, bytes
-// otherwise.
+// otherwise; an entry without a source zero-fills its destination.
 __global__ void copy_multi_kernel(CopyList l) {
   const CopySpec c = l.c[blockIdx.y];
   const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
+  if (!c.src) {  // zero fill
+    uint8_t* d = static_cast<uint8_t*>(c.dst);
+    for (size_t k = tid; k < c.bytes; k += stride) d[k] = 0;
+    return;
+  }
   const bool wide = ((reinterpret_cast<uintptr_t>(c.dst) | reinterpret_cast<uintptr_t>(c.src) | c.bytes) & 15) == 0;
   if (wide) {
     uint4* d = static_cast<uint4*>(c.dst);

@@ -281,7 +281,7 @@ inline hipError_t launch_copy_f32(float* dst, const float* src, int count, hipSt
   return launch_copy<float>(dst, src, count, stream);
 }
 hipError_t launch_copy_u8(uint8_t* dst, const uint8_t* src, int count, hipStream_t stream);
-// Several device-to-device copies in one launch.
+// Several device-to-device copies (or zero fills: src == nullptr) in one launch.
 constexpr int kCopyMax = 4;
 struct CopySpec {
   void* dst;
