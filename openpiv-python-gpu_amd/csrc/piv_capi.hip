@@ -356,7 +356,7 @@ struct LoopBufs {
   void* tv = nullptr;
   uint8_t* wf[2] = {nullptr, nullptr};  // flags after each revalidation (ping-pong)
   uint8_t* nanf = nullptr;    // unresolved vectors of the last replacement
-  int32_t* it_counts = nullptr;  // PIV_MAX_REPLACING_ITERS counters
+  int32_t* it_counts = nullptr;  // PIV_MAX_REPLACING_ITERS counters per pass
   double* stats = nullptr;    // mean_u, std_u, mean_v, std_v
   ReplaceWork rw{};
 };
@@ -540,11 +540,12 @@ int run_validate(const piv_pass_desc& d, const R* u, const R* v, int m, int n, c
 // pass's method; iteration 1 leaves the flagged neighbours out, later ones
 // use them unless revalidating (:824-826); revalidation resets unresolved
 // vectors and re-tests only the replaced ones (:809-822); without it the
-// unresolved ones are reset at the end (:829-832).  it_counts must be zero.
+// unresolved ones are reset at the end (:829-832).  it_counts: this pass's
+// PIV_MAX_REPLACING_ITERS revalidation counters, zero on entry.
 // No host synchronisation: an empty flag set turns every step into a no-op.
 template <typename R>
 int replace_loop(R* u, R* v, const uint8_t* flags0, int32_t* count0, int m, int n, const piv_pass_desc& d,
-                 const R* s2n, LoopBufs& L, hipStream_t s, const uint8_t* fmask = nullptr) {
+                 const R* s2n, LoopBufs& L, int32_t* it_counts, hipStream_t s, const uint8_t* fmask = nullptr) {
   const int N = m * n;
   const int iters = d.num_replacing_iters;
   if (iters <= 0) return PIV_OK;
@@ -572,7 +573,7 @@ int replace_loop(R* u, R* v, const uint8_t* flags0, int32_t* count0, int m, int 
     if (reval) {
       HIP_TRY(launch_reset_unresolved(u, v, (const R*)u0, (const R*)v0, cur, L.nanf, N, s));
       uint8_t* nxt = L.wf[it & 1];
-      int32_t* nxt_count = L.it_counts + it;
+      int32_t* nxt_count = it_counts + it;
       int r = run_validate<R>(d, u, v, m, n, s2n, L.stats, cur, L.nanf, nxt, nxt_count, s, fmask);
       if (r) return r;
       cur = nxt;
@@ -595,7 +596,9 @@ int validate_and_smooth(piv_plan* P, int k, bool last, hipStream_t s) {
   if (B.d.validate) {
     int r = run_validate<R>(B.d, u, v, m, n, s2n, P->lb.stats, nullptr, nullptr, B.flags, P->counters + 2 * k, s,
                             B.fmask);
-    if (!r) r = replace_loop<R>(u, v, B.flags, P->counters + 2 * k, m, n, B.d, s2n, P->lb, s, B.fmask);
+    if (!r)
+      r = replace_loop<R>(u, v, B.flags, P->counters + 2 * k, m, n, B.d, s2n, P->lb,
+                          P->lb.it_counts + (size_t)k * PIV_MAX_REPLACING_ITERS, s, B.fmask);
     if (!r && last)
       r = run_validate<R>(B.d, u, v, m, n, s2n, P->lb.stats, nullptr, nullptr, P->val_final, P->counters + 2 * k + 1,
                           s, B.fmask);
@@ -859,7 +862,7 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
     ok = ok && dmalloc((void**)&lb.u0, maxN * es) && dmalloc((void**)&lb.v0, maxN * es) &&
          dmalloc((void**)&lb.tu, maxN * es) && dmalloc((void**)&lb.tv, maxN * es) && dmalloc((void**)&lb.wf[0], maxN) &&
          dmalloc((void**)&lb.wf[1], maxN) && dmalloc((void**)&lb.nanf, maxN) &&
-         dmalloc((void**)&lb.it_counts, PIV_MAX_REPLACING_ITERS * 4) && dmalloc((void**)&lb.stats, 4 * 8);
+         dmalloc((void**)&lb.it_counts, (size_t)D.n_passes * PIV_MAX_REPLACING_ITERS * 4) && dmalloc((void**)&lb.stats, 4 * 8);
     lb.rw = P->rw;
   }
   if (!ok) return bail(fail(PIV_E_NOMEM, "device allocation failed"));
@@ -892,7 +895,7 @@ static int run_impl(piv_plan* P, const void* frame_a, const void* frame_b, int f
     CopyList cl{};
     int nc = 0;
     cl.c[nc++] = CopySpec{P->counters, nullptr, 2 * P->passes.size() * sizeof(int32_t)};
-    cl.c[nc++] = CopySpec{P->lb.it_counts, nullptr, PIV_MAX_REPLACING_ITERS * sizeof(int32_t)};
+    cl.c[nc++] = CopySpec{P->lb.it_counts, nullptr, P->passes.size() * PIV_MAX_REPLACING_ITERS * sizeof(int32_t)};
     if (frames_on_device) {
       cl.c[nc++] = CopySpec{P->frame_a, frame_a, P->frame_bytes};
       cl.c[nc++] = CopySpec{P->frame_b, frame_b, P->frame_bytes};
@@ -919,7 +922,7 @@ static int run_impl(piv_plan* P, const void* frame_a, const void* frame_b, int f
       P->last_fallback = 1;
       CopyList cl{};  // counters of the full-schedule run start at zero
       cl.c[0] = CopySpec{P->counters, nullptr, 2 * P->passes.size() * sizeof(int32_t)};
-      cl.c[1] = CopySpec{P->lb.it_counts, nullptr, PIV_MAX_REPLACING_ITERS * sizeof(int32_t)};
+      cl.c[1] = CopySpec{P->lb.it_counts, nullptr, P->passes.size() * PIV_MAX_REPLACING_ITERS * sizeof(int32_t)};
       HIP_TRY(launch_copy_multi(cl, 2, s));
       r = run_captured(P, enqueue, &P->graph, &P->graph_exec, &P->runs);
       if (r) return r;
@@ -1015,54 +1018,86 @@ int piv_plan_set_mask(piv_plan* P, const uint8_t* mask, int height, int width) {
   if (mask && P->net) return fail(PIV_E_INVALID, "a frame mask cannot be combined with a learned predictor");
   if (mask && (height != D.height || width != D.width))
     return fail(PIV_E_INVALID, "mask is %d x %d, the frames are %d x %d", height, width, D.height, D.width);
+  const int np = (int)P->passes.size();
+  // Every check runs before the plan changes: a rejected mask leaves the
+  // previous state (masked or not) untouched.
+  std::vector<std::vector<uint8_t>> fms(mask ? np : 0);
+  std::vector<int> n_masked(np, 0), slow_gcv(np, 0);
+  int maxN = 0, maxM = 0;
+  bool slow = false;
+  for (int k = 0; mask && k < np; ++k) {
+    const PassBuffers& B = P->passes[k];
+    fms[k] = field_mask_of(B.d, mask, D.width);
+    for (uint8_t x : fms[k]) n_masked[k] += x;
+    if (k < np - 1 && B.d.smooth && n_masked[k] > 0) {
+      if (B.d.field_m > smoothn_masked_max_dim() || B.d.field_n > smoothn_masked_max_dim())
+        return fail(PIV_E_INVALID, "masked smoothing supports fields of up to %d x %d",
+                    smoothn_masked_max_dim(), smoothn_masked_max_dim());
+      // mean weight (gpu_smoothn.py:171, 367): the masked vectors are NaN
+      slow_gcv[k] = !((double)(B.N - n_masked[k]) / B.N > 0.9);
+      slow = slow || (slow_gcv[k] && !(B.d.smoothing_par > 0));
+      maxN = std::max(maxN, B.N);
+      maxM = std::max(maxM, B.d.field_m);
+    }
+  }
   // the captured sequence changes
   if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
   if (P->graph) (void)hipGraphDestroy(P->graph);
   P->graph_exec = nullptr;
   P->graph = nullptr;
   P->runs = 0;
-  for (auto& B : P->passes) {
-    if (B.fmask) (void)hipFree(B.fmask);
-    B.fmask = nullptr;
-    B.n_masked = 0;
-    B.slow_gcv = 0;
-  }
-  free_masked_smooth(&P->msb);
-  if (P->frame_mask) (void)hipFree(P->frame_mask);
-  P->frame_mask = nullptr;
+  auto unmask = [P]() {
+    for (auto& B : P->passes) {
+      if (B.fmask) (void)hipFree(B.fmask);
+      B.fmask = nullptr;
+      B.n_masked = 0;
+      B.slow_gcv = 0;
+    }
+    free_masked_smooth(&P->msb);
+    if (P->frame_mask) (void)hipFree(P->frame_mask);
+    P->frame_mask = nullptr;
+  };
+  unmask();
   if (!mask) return PIV_OK;
+  // allocation or copy failures from here on leave the plan unmasked
+  auto undo = [&](int code) {
+    unmask();
+    return code;
+  };
   const size_t npx = (size_t)D.height * D.width;
-  const int np = (int)P->passes.size();
-  int maxN = 0, maxM = 0;
-  bool slow = false;
   for (int k = 0; k < np; ++k) {
     PassBuffers& B = P->passes[k];
-    std::vector<uint8_t> fm = field_mask_of(B.d, mask, D.width);
-    for (uint8_t x : fm) B.n_masked += x;
-    HIP_TRY(hipMalloc(&B.fmask, fm.size()));
-    HIP_TRY(hipMemcpy(B.fmask, fm.data(), fm.size(), hipMemcpyHostToDevice));
-    if (k < np - 1 && B.d.smooth && B.n_masked > 0) {
-      if (B.d.field_m > smoothn_masked_max_dim() || B.d.field_n > smoothn_masked_max_dim())
-        return fail(PIV_E_INVALID, "masked smoothing supports fields of up to %d x %d",
-                    smoothn_masked_max_dim(), smoothn_masked_max_dim());
-      // mean weight (gpu_smoothn.py:171, 367): the masked vectors are NaN
-      B.slow_gcv = !((double)(B.N - B.n_masked) / B.N > 0.9);
-      slow = slow || (B.slow_gcv && !(B.d.smoothing_par > 0));
-      maxN = std::max(maxN, B.N);
-      maxM = std::max(maxM, B.d.field_m);
+    if (hipMalloc(&B.fmask, fms[k].size()) != hipSuccess) {
+      B.fmask = nullptr;
+      return undo(fail(PIV_E_NOMEM, "device allocation failed"));
     }
+    if (hipMemcpy(B.fmask, fms[k].data(), fms[k].size(), hipMemcpyHostToDevice) != hipSuccess)
+      return undo(fail(PIV_E_HIP, "hipMemcpy failed"));
   }
-  HIP_TRY(hipMalloc(&P->frame_mask, npx));
+  if (hipMalloc(&P->frame_mask, npx) != hipSuccess) {
+    P->frame_mask = nullptr;
+    return undo(fail(PIV_E_NOMEM, "device allocation failed"));
+  }
   std::vector<uint8_t> fmk(npx);
   for (size_t q = 0; q < npx; ++q) fmk[q] = mask[q] ? 1 : 0;
-  HIP_TRY(hipMemcpy(P->frame_mask, fmk.data(), npx, hipMemcpyHostToDevice));
+  if (hipMemcpy(P->frame_mask, fmk.data(), npx, hipMemcpyHostToDevice) != hipSuccess)
+    return undo(fail(PIV_E_HIP, "hipMemcpy failed"));
   if (maxN > 0) {
     int r = alloc_masked_smooth(maxN, maxM, slow, &P->msb, P->esize);
-    if (r) return r;
+    if (r) return undo(r);
   }
   if (!P->smooth_iters) {
-    HIP_TRY(hipMalloc(&P->smooth_iters, np * sizeof(int32_t)));
-    HIP_TRY(hipMemset(P->smooth_iters, 0, np * sizeof(int32_t)));
+    if (hipMalloc(&P->smooth_iters, np * sizeof(int32_t)) != hipSuccess) {
+      P->smooth_iters = nullptr;
+      return undo(fail(PIV_E_NOMEM, "device allocation failed"));
+    }
+    if (hipMemset(P->smooth_iters, 0, np * sizeof(int32_t)) != hipSuccess)
+      return undo(fail(PIV_E_HIP, "hipMemset failed"));
+  }
+  // the masked state becomes visible only once every buffer is in place
+  for (int k = 0; k < np; ++k) {
+    P->passes[k].n_masked = n_masked[k];
+    P->passes[k].slow_gcv = slow_gcv[k];
   }
   return PIV_OK;
 }
@@ -1266,7 +1301,7 @@ int piv_stage_replace(float* u, float* v, uint8_t* flags, int m, int n, const pi
     cleanup();
     return fail(PIV_E_HIP, "replace: %s", hipGetErrorString(e));
   }
-  r = replace_loop(u, v, L.wf[1], count0, m, n, *pass, s2n, L, s);
+  r = replace_loop(u, v, L.wf[1], count0, m, n, *pass, s2n, L, L.it_counts, s);
   if (!r && pass->revalidate && pass->num_replacing_iters > 0) {
     // final flags of the loop: the last revalidation's output
     const uint8_t* last = L.wf[(pass->num_replacing_iters - 1) & 1];
@@ -1744,7 +1779,7 @@ int batch_enqueue(piv_batch* b) {
   hipStream_t s = b->stream;
   for (auto* P : b->plans) {
     HIP_TRY(launch_fill_i32(P->counters, 0, 2 * (int)P->passes.size(), s));
-    HIP_TRY(launch_fill_i32(P->lb.it_counts, 0, PIV_MAX_REPLACING_ITERS, s));
+    HIP_TRY(launch_fill_i32(P->lb.it_counts, 0, PIV_MAX_REPLACING_ITERS * (int)P->passes.size(), s));
   }
   for (size_t l = 0; l < b->levels.size(); ++l) {
     BatchLevel& L = b->levels[l];

@@ -349,11 +349,14 @@ class PIVGPU:
 
     def _get_plan(self, in_dtype: int):
         if self._plan is None or self._plan.desc.in_dtype != in_dtype:
-            self._plan = _native.Plan(self._plan_desc(in_dtype), self.device)
+            # the plan is kept only once the mask and the predictor are set:
+            # a caller who catches an error and calls again gets a fresh plan
+            plan = _native.Plan(self._plan_desc(in_dtype), self.device)
             if self.mask is not None:
-                self._plan.set_mask(self.mask)
+                plan.set_mask(self.mask)
             if self.predictor is not None:
-                self._plan.set_predictor(self.predictor, self.max_reject_fraction, self.predictor_pass)
+                plan.set_predictor(self.predictor, self.max_reject_fraction, self.predictor_pass)
+            self._plan = plan
         return self._plan
 
     @property

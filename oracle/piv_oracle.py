@@ -33,8 +33,12 @@ deform_frame           exact on affine intensity fields; brute force loop
 median_validation      np.nanmedian over NaN-padded 3x3 sliding windows
 spring_replacement     hand-solved 1-outlier and 2-outlier systems;
                        scipy.sparse.linalg.spsolve on the same matrix
-smoothn                scipy.fft.dctn/idctn; fixed-s closed form; constant
-                       field invariance; GCV minimum vs dense grid search
+smoothn_lambda         C^T diag(Lambda) C equals the square of the 2-D
+                       Neumann Laplacian built from its stencil
+smoothn                scipy.fft.dctn/idctn; fixed s equals the dense
+                       solve of (I + s L^2) z = f; constant field invariance
+gcv_score              the dense hat-matrix GCV with H = (I + s L^2)^-1, in
+(+ weighted branch)    both branches; the chosen p vs the dense GCV minimum
 piv (end to end)       RMS vs the known synthetic displacement
 =====================  =====================================================
 

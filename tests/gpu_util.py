@@ -225,3 +225,157 @@ def stack_correlate_dft(wa, wb, fsize, precision, subpixel=0, planes=False):
     if planes:
         out["corr"] = host(pl)
     return out
+
+
+def threshold_band(u, v, s2n, settings, rel=1e-5, s2n_rel=1e-4, only=None):
+    """Vectors whose validation decision is not reproducible across
+    implementations: within ``rel`` (relative) of a median / MAD / mean / RMS
+    threshold, or within ``s2n_rel`` of the S2N threshold (the S2N value
+    itself differs by FFT rounding: ~1e-5 relative for peak2peak, a few % for
+    the two mean methods).  ``settings``: an OracleSettings."""
+    u = np.asarray(u, np.float64)
+    v = np.asarray(v, np.float64)
+    near = np.zeros(u.shape, bool)
+
+    def close(x, t):
+        with np.errstate(invalid="ignore", divide="ignore"):
+            return np.abs(x - t) <= rel * np.maximum(np.abs(t), 1e-12)
+
+    s = settings
+    if s.median_tol is not None or s.mad_tol is not None or s.mean_tol is not None:
+        for f in (u, v):
+            med, mad, mean = O.neighbour_stats(f)
+            if s.median_tol is not None:
+                near |= close(np.abs(f - med), s.median_tol)
+            if s.mad_tol is not None:
+                near |= close(np.abs(f - med), s.mad_tol * mad)
+            if s.mean_tol is not None:
+                near |= close(np.abs(f - mean), s.mean_tol)
+    if s.rms_tol is not None:
+        for f in (u, v):
+            with np.errstate(invalid="ignore"):
+                mu = np.nanmean(f)
+                sd = np.nanstd(f)
+            near |= close(np.abs(f - mu), s.rms_tol * sd)
+    if s.s2n_tol is not None and s2n is not None:
+        with np.errstate(invalid="ignore", divide="ignore"):
+            near |= np.abs(np.asarray(s2n, np.float64).reshape(u.shape) / s.s2n_tol - 1.0) <= s2n_rel
+    if only is not None:
+        near &= only
+    return near
+
+
+def pass_exclusions(frame_a, frame_b, settings, passes, mode="faithful32", s2n_rel=1e-4):
+    """The oracle's own record of which vectors of each pass an FFT
+    implementation other than its own may legitimately decide differently:
+
+    * ``tie``: windows whose two largest correlation values lie within 1e-5
+      (relative) of each other, so the integer peak can flip;
+    * ``fit``: windows where one of the five Gaussian-fit samples lies within
+      1e-5 of zero, so whether the fit runs can flip;
+    * ``band``: vectors within the validation thresholds' bands
+      (threshold_band), evaluated on every field the pass validates (before
+      replacement, and after it when the last pass re-validates).
+
+    ``passes``: the oracle's PassState list of the same run (keep_passes)."""
+    dtype = np.float32 if mode == "faithful32" else np.float64
+    out = []
+    for k, st in enumerate(passes):
+        fa = np.asarray(frame_a, dtype)
+        fb = np.asarray(frame_b, dtype)
+        if settings.mask is not None:
+            fa, fb = O.mask_frames(fa, fb, np.asarray(settings.mask, bool))
+        if st.up is not None:
+            fa, fb = O.deform_frames(fa, fb, st.geom, st.up, st.vp, settings.deforming_par, dtype,
+                                     None if settings.mask is None else np.asarray(settings.mask, bool))
+        ws = None
+        if k == 0 and settings.shrink_ratio != 1:
+            ws = O.shrunk_window(st.geom.search_size, settings.shrink_ratio)
+        r = O.correlate_pass(fa, fb, st.geom, settings.n_fft, settings.normalize, settings.subpixel_method, dtype,
+                             return_planes=True, window_size=ws)
+        shape = st.geom.field_shape
+        tie = near_tie_mask(r["corr"]).reshape(shape)
+        fit = fit_sign_ambiguous(r["corr"], r["i_peak"], r["j_peak"]).reshape(shape)
+        fm = O.field_mask(st.geom, settings.mask)
+        keep = None if fm is None else ~fm
+        s2n = st.corr.get("s2n")
+        band = threshold_band(st.u_pre, st.v_pre, s2n, settings, s2n_rel=s2n_rel, only=keep)
+        if st.outliers is not None:
+            band |= threshold_band(st.u_rep, st.v_rep, s2n, settings, s2n_rel=s2n_rel, only=keep)
+        if keep is not None:
+            tie &= keep
+            fit &= keep
+        out.append({"tie": tie, "fit": fit, "band": band, "any": tie | fit | band})
+    return out
+
+
+def dilate(mask, steps):
+    """Grow a boolean field by ``steps`` 8-neighbour rings (the reach of a
+    3 x 3 validation / replacement step)."""
+    out = np.asarray(mask, bool).copy()
+    for _ in range(max(0, int(steps))):
+        p = np.pad(out, 1)
+        grown = np.zeros_like(out)
+        for di in (0, 1, 2):
+            for dj in (0, 1, 2):
+                grown |= p[di:di + out.shape[0], dj:dj + out.shape[1]]
+        out = grown
+    return out
+
+
+def _reach(seed, flags, settings):
+    """Vectors whose replaced value can depend on the vectors in ``seed``:
+    ``num_replacing_iters`` + 1 rings for the median / mean replacement
+    (each iteration reads a 3 x 3 neighbourhood), and for the spring
+    replacement every outlier cluster (4-connected) next to the seed."""
+    from scipy import ndimage
+    iters = max(1, int(settings.num_replacing_iters))
+    out = dilate(seed, iters + 1)
+    if settings.replacing_method == "spring" and flags.any():
+        lab, _ = ndimage.label(flags)
+        hit = np.unique(lab[dilate(seed, 1) & flags])
+        out |= np.isin(lab, hit[hit > 0])
+        out = dilate(out, 1)
+    return out
+
+
+def e2e_compare(u, v, init_flags, ref, excl, settings, tol=1e-3, final_flags=None):
+    """End-to-end comparison with every excused vector accounted for.
+
+    * passes before the last must have no non-reproducible vector at all
+      (``excl[k]["any"]`` empty, from pass_exclusions): the inputs are chosen
+      so, and the assert says so when they are not;
+    * last pass: a pre-replacement flag may differ only at a band vector or
+      next to a near-tie / fit-ambiguous window (its value moves the
+      neighbourhood median); u and v must agree to ``tol`` everywhere except
+      within the replacement's reach of a differing flag, a near-tie or a
+      fit-ambiguous window (_reach);
+    * post-replacement flags (``final_flags``), when given, may differ only
+      where u, v are excused or at band vectors.
+
+    Returns a short report of what was excused."""
+    for k, e in enumerate(excl[:-1]):
+        assert not e["any"].any(), (f"pass {k}: {int(e['tie'].sum())} near-tie, {int(e['fit'].sum())} "
+                                    f"fit-ambiguous, {int(e['band'].sum())} band vectors: pick inputs without them")
+    last = excl[-1]
+    ref_init = np.asarray(ref["val_locations"], bool)
+    got_init = np.asarray(init_flags, bool)
+    tie_fit = last["tie"] | last["fit"]
+    flag_diff = got_init != ref_init
+    allowed = last["band"] | dilate(tie_fit, 1)
+    assert not (flag_diff & ~allowed).any(), ("pre-replacement flags differ outside the bands",
+                                              np.argwhere(flag_diff & ~allowed)[:10])
+    excused = _reach(flag_diff | tie_fit, ref_init | got_init, settings)
+    du = np.abs(np.asarray(u, np.float64) - ref["u"])
+    dv = np.abs(np.asarray(v, np.float64) - ref["v"])
+    fin = np.isfinite(ref["u"]) & np.isfinite(ref["v"])
+    assert np.array_equal(np.isfinite(u) & np.isfinite(v), fin), "NaN pattern differs"
+    ok = fin & ~excused
+    err = np.maximum(du, dv)
+    assert err[ok].max(initial=0.0) <= tol, ("u/v differ outside the excused vectors",
+                                             float(err[ok].max()), np.argwhere(ok & (err > tol))[:10])
+    if final_flags is not None and ref.get("outliers") is not None:
+        fd = np.asarray(final_flags, bool) != np.asarray(ref["outliers"], bool)
+        assert not (fd & ~(excused | last["band"])).any(), "post-replacement flags differ outside the excused vectors"
+    return {"flag_diff": int(flag_diff.sum()), "tie": int(last["tie"].sum()), "fit": int(last["fit"].sum()),
+            "band": int(last["band"].sum()), "excused": int(excused.sum()), "compared": int(ok.sum())}

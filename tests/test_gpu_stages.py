@@ -197,6 +197,7 @@ def test_shrink_ratio_end_to_end():
                  dtype_f="float32")
     u, v = piv.run_host(fa, fb)
     s = O.OracleSettings(min_search_size=16, search_size_iters=(1, 1), shrink_ratio=0.6, smoothing_par=0.8)
-    r = O.piv(fa, fb, s)
-    err = np.maximum(np.abs(u - r["u"]), np.abs(v - r["v"]))
-    assert np.mean(err <= 1e-3) > 0.999, np.sort(err.reshape(-1))[-8:]
+    r = O.piv(fa, fb, s, keep_passes=True)
+    excl = gu.pass_exclusions(fa, fb, s, r["passes"])
+    rep = gu.e2e_compare(u, v, piv.init_val_locations, r, excl, s, final_flags=piv.val_locations)
+    assert rep["compared"] > 0.95 * u.size, rep

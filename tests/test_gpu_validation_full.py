@@ -153,21 +153,54 @@ def test_unresolved_block_is_reset_on_gpu():
         assert np.array_equal(gu_, ru.astype(np.float32)) and not np.isnan(gu_).any()
 
 
-def test_multipass_with_every_test_matches_oracle():
+@pytest.mark.parametrize("s2n_method", ["peak2energy", "peak2peak"])
+def test_multipass_with_every_test_matches_oracle(s2n_method):
     from openpiv_gpu.gpu_process import PIVGPU
     shape = (320, 384)
     fa, fb = _frames(shape, 41, shift=(3.2, 1.7))
-    kw = dict(search_size_iters=(1, 1), overlap_ratio=0.5, s2n_method="peak2energy", s2n_tol=1.05, median_tol=2.0,
+    kw = dict(search_size_iters=(1, 1), overlap_ratio=0.5, s2n_method=s2n_method, s2n_tol=1.05, median_tol=2.0,
               mad_tol=6.0, mean_tol=3.0, replacing_method="median", num_replacing_iters=2, revalidate=True)
     piv = PIVGPU(shape, 16, dtype_f="float32", **kw)
     u, v = piv.run_host(fa, fb)
-    s = O.OracleSettings(min_search_size=16, search_size_iters=(1, 1), overlap_ratio=0.5, s2n_method="peak2energy",
-                         s2n_tol=1.05, median_tol=2.0, mad_tol=6.0, mean_tol=3.0, replacing_method="median",
-                         num_replacing_iters=2, revalidate=True)
-    r = O.piv(fa, fb, s)
-    err = np.maximum(np.abs(u - r["u"]), np.abs(v - r["v"]))
-    assert np.mean(err <= 1e-3) > 0.995, np.sort(err.reshape(-1))[-10:]
-    assert np.mean(piv.init_val_locations == r["val_locations"]) > 0.995
+    s = O.OracleSettings(min_search_size=16, **kw)
+    r = O.piv(fa, fb, s, keep_passes=True)
+    # the mean-based S2N moves with the sign of near-zero plane values: a
+    # few % (see test_s2n_matches_oracle); peak2peak to 1e-4
+    excl = gu.pass_exclusions(fa, fb, s, r["passes"], s2n_rel=5e-2 if s2n_method == "peak2energy" else 1e-4)
+    rep = gu.e2e_compare(u, v, piv.init_val_locations, r, excl, s, final_flags=piv.val_locations)
+    assert rep["compared"] > 0.9 * u.size, rep
+    for k, st in enumerate(r["passes"]):
+        assert piv.outlier_counts(k)[0] == int(st.val.sum()), k
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_multipass_spring_revalidate_counts_per_pass(graph):
+    """Spring replacement with revalidation and several iterations in every
+    pass: each pass's revalidation counters start from zero (a count summed
+    over passes would reach the spring kernel's all-outlier gate and skip
+    the replacement), on the eager run and on graph replays."""
+    from openpiv_gpu.gpu_process import PIVGPU
+    shape = (320, 384)
+    fa, fb = _frames(shape, 43, shift=(3.2, 1.7))
+    fb = fb.copy()
+    rng = np.random.default_rng(43)
+    for _ in range(5):
+        r0, c0 = rng.integers(30, 260, 2)
+        fb[r0:r0 + 30, c0:c0 + 30] = rng.integers(0, 255, (30, 30))
+    kw = dict(search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=1.0, replacing_method="spring",
+              num_replacing_iters=3, revalidate=True, smoothing_par=1.5)
+    piv = PIVGPU(shape, 8, dtype_f="float32", use_graph=graph, **kw)
+    s = O.OracleSettings(min_search_size=8, **kw)
+    r = O.piv(fa, fb, s, keep_passes=True)
+    excl = gu.pass_exclusions(fa, fb, s, r["passes"])
+    for call in range(3 if graph else 1):
+        u, v = piv.run_host(fa, fb)
+        rep = gu.e2e_compare(u, v, piv.init_val_locations, r, excl, s, final_flags=piv.val_locations)
+        assert rep["compared"] > 0.9 * u.size, (call, rep)
+        for k, st in enumerate(r["passes"]):
+            assert piv.outlier_counts(k)[0] == int(st.val.sum()), (call, k)
+        assert piv.outlier_counts(len(r["passes"]) - 1)[1] == int(r["outliers"].sum()), call
+    assert all(int(st.val.sum()) > 0 for st in r["passes"])
 
 
 @pytest.mark.parametrize("dtype_f,method", [("float32", "peak2peak"), ("float32", "peak2energy"),

@@ -50,13 +50,21 @@ def test_nearest_finite_index_matches_brute_force(kind):
 
 
 def _pls_matrix(shape):
-    """A = C^T diag(Lambda^2) C for the orthonormal 2-D DCT C (dense)."""
+    """A = L^2, L the 2-D Laplacian with reflecting ends built from its
+    finite-difference stencil (no DCT, no eigenvalue formula)."""
     m, n = shape
-    N = m * n
-    eye = np.eye(N).reshape(N, m, n)
-    lam = O.smoothn_lambda(shape)
-    cols = [O.idct2(lam * O.dct2(e)).reshape(-1) for e in eye]
-    return np.array(cols).T
+
+    def d1(k):
+        d = np.zeros((k, k))
+        for i in range(k - 1):
+            d[i, i] += 1.0
+            d[i + 1, i + 1] += 1.0
+            d[i, i + 1] -= 1.0
+            d[i + 1, i] -= 1.0
+        return d
+
+    lap = np.kron(d1(m), np.eye(n)) + np.kron(np.eye(m), d1(n))
+    return lap @ lap
 
 
 def test_weighted_iteration_converges_to_dense_pls_solution():

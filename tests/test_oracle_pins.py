@@ -244,6 +244,43 @@ def test_dct_is_textbook_orthonormal_dct2():
     np.testing.assert_allclose(O.idct2(O.dct2(z)), z, atol=1e-12)
 
 
+def _neumann_second_difference(n):
+    """1-D second-difference operator built from its stencil: row i sums
+    (z_i - z_{i-1}) + (z_i - z_{i+1}) over the neighbours that exist, i.e.
+    reflecting (Neumann) ends -- the penalty Garcia's smoother uses."""
+    d = np.zeros((n, n))
+    for i in range(n - 1):
+        d[i, i] += 1.0
+        d[i + 1, i + 1] += 1.0
+        d[i, i + 1] -= 1.0
+        d[i + 1, i] -= 1.0
+    return d
+
+
+def _penalty_operator(shape):
+    """A = L^2 with L = D_m (x) I + I (x) D_n, the 2-D Neumann Laplacian
+    acting on the row-major flattening; smoothing order 2 squares it."""
+    m, n = shape
+    lap = np.kron(_neumann_second_difference(m), np.eye(n)) + np.kron(np.eye(m), _neumann_second_difference(n))
+    return lap @ lap
+
+
+@pytest.mark.parametrize("shape", [(7, 9), (1, 12), (12, 5)])
+def test_lambda_diagonalises_the_stencil_penalty(shape):
+    """C^T diag(Lambda) C == (D_m (x) I + I (x) D_n)^2 for the orthonormal
+    2-D DCT C (gpu_smoothn.py:165, 340-351): the eigenvalues are pinned by
+    an operator built from the finite-difference stencil alone."""
+    m, n = shape
+    N = m * n
+    cm, cn = _dct_matrix(m), _dct_matrix(n)
+    c = np.kron(cm, cn)
+    lam = O.smoothn_lambda(shape).reshape(-1)
+    np.testing.assert_allclose(c.T @ np.diag(lam) @ c, _penalty_operator(shape), atol=1e-11)
+    # and the oracle's DCT pair applies exactly that operator
+    cols = [O.idct2(O.smoothn_lambda(shape) * O.dct2(e)).reshape(-1) for e in np.eye(N).reshape(N, m, n)]
+    np.testing.assert_allclose(np.array(cols).T, _penalty_operator(shape), atol=1e-11)
+
+
 def test_smoothn_bounds_and_fixed_s():
     p_min, p_max = O.smoothn_p_bounds((63, 63))
     assert p_min == pytest.approx(-4.078650143894, abs=1e-9)
@@ -252,13 +289,76 @@ def test_smoothn_bounds_and_fixed_s():
     u = rng.standard_normal((9, 8))
     s = 3.7
     z = O.smoothn([u], s=s)[0]
-    m, n = u.shape
-    lam = (2 * (2 - np.cos(np.pi * np.arange(m) / m)[:, None] - np.cos(np.pi * np.arange(n) / n)[None, :])) ** 2
-    cm, cn = _dct_matrix(m), _dct_matrix(n)
-    expect = cm.T @ ((cm @ u @ cn.T) / (1 + s * lam)) @ cn
+    # penalised least squares: z minimises |z - u|^2 + s |L z|^2, so
+    # (I + s L^2) z = u, solved densely with the stencil operator
+    expect = np.linalg.solve(np.eye(u.size) + s * _penalty_operator(u.shape), u.reshape(-1)).reshape(u.shape)
     np.testing.assert_allclose(z, expect, atol=1e-12)
     c = np.full((9, 8), 2.5)
     np.testing.assert_allclose(O.smoothn([c], s=100.0)[0], c, atol=1e-12)
+
+
+def _dense_gcv(p, fields):
+    """Generalised cross-validation from the dense hat matrix
+    H = (I + 10^p A)^-1: (|(I - H) y|^2 / N) / (1 - tr H / N)^2, summed over
+    the fields and divided by their number (gpu_smoothn.py:353-372)."""
+    shape = fields[0].shape
+    N = fields[0].size
+    h = np.linalg.inv(np.eye(N) + 10.0 ** p * _penalty_operator(shape))
+    rss = sum(np.sum(((np.eye(N) - h) @ y.reshape(-1)) ** 2) for y in fields)
+    return rss / N / (1.0 - np.trace(h) / N) ** 2 / len(fields)
+
+
+@pytest.mark.parametrize("shape", [(7, 9), (10, 10)])
+def test_gcv_score_equals_dense_hat_matrix_gcv(shape):
+    rng = np.random.default_rng(12)
+    fields = [rng.standard_normal(shape), rng.standard_normal(shape) + 0.5]
+    fd = [O.dct2(a) for a in fields]
+    lam = O.smoothn_lambda(shape)
+    for p in (-3.5, -1.0, 0.0, 0.7, 2.5, 4.0):
+        assert O.gcv_score(p, fd, lam, 2) == pytest.approx(_dense_gcv(p, fields), rel=1e-9)
+
+
+def test_weighted_gcv_equals_dense_expression():
+    """Data-misfit branch (w_mean <= 0.9): the residual is taken at finite
+    points against H (W (f - z) + z), the trace term is still tr H / N,
+    and the normalisation is the finite count (gpu_smoothn.py:360-371)."""
+    rng = np.random.default_rng(13)
+    shape = (8, 7)
+    N = 56
+    f = [rng.standard_normal(shape), rng.standard_normal(shape)]
+    z = [rng.standard_normal(shape), rng.standard_normal(shape)]
+    w = (rng.random(shape) < 0.6).astype(float)
+    finite = rng.random(shape) < 0.9
+    w[~finite] = 0.0
+    nf = int(finite.sum())
+    fd = [O.dct2(w * (a - b) + b) for a, b in zip(f, z)]
+    lam = O.smoothn_lambda(shape)
+    for p in (-2.0, 0.4, 1.9):
+        h = np.linalg.inv(np.eye(N) + 10.0 ** p * _penalty_operator(shape))
+        rss = 0.0
+        for a, b in zip(f, z):
+            fh = (h @ (w * (a - b) + b).reshape(-1)).reshape(shape)
+            rss += np.sum(w[finite] * (a[finite] - fh[finite]) ** 2)
+        want = rss / nf / (1.0 - np.trace(h) / N) ** 2 / 2
+        got = O.gcv_score_weighted(p, fd, lam, 2, f, w, finite, float(w.mean()), nf)
+        assert got == pytest.approx(want, rel=1e-9)
+
+
+def test_gcv_exact_minimiser_minimises_the_dense_gcv():
+    from scipy.optimize import minimize_scalar
+    rng = np.random.default_rng(14)
+    yy, xx = np.mgrid[0:11, 0:12]
+    fields = [np.sin(xx / 3.0) + 0.2 * rng.standard_normal((11, 12)),
+              np.cos(yy / 2.5) + 0.2 * rng.standard_normal((11, 12))]
+    p_min, p_max = O.smoothn_p_bounds((11, 12))
+    grid = np.linspace(p_min, p_max, 301)
+    g = [_dense_gcv(p, fields) for p in grid]
+    k = int(np.argmin(g))
+    r = minimize_scalar(lambda p: _dense_gcv(p, fields), bounds=(grid[max(k - 1, 0)], grid[min(k + 1, 300)]),
+                        method="bounded", options={"xatol": 1e-10})
+    assert abs(O.gcv_exact_minimiser(fields) - r.x) < 1e-6
+    _, p = O.smoothn(fields, return_p=True)
+    assert abs(p - r.x) < 1e-2
 
 
 def test_gcv_choice_is_near_the_dense_minimum():
