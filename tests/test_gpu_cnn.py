@@ -163,9 +163,13 @@ def test_learned_path_matches_oracle_last_pass():
     st = O.run_pass(fa, fb, s, g_last, g_net, np.full((32, 32), 2.3), np.full((32, 32), -1.5), True)
     assert np.abs(u - st.u_out).max() <= 1e-3 and np.abs(v - st.v_out).max() <= 1e-3
     assert rejected == int(st.val.sum())
-    # second call replays the captured graph: identical output
+    assert piv.outlier_counts(1) == (int(st.val.sum()), int(st.outliers.sum()))
+    # second call replays the captured graph: identical output, and the
+    # counters start from zero again
     u2, v2 = piv.run_host(fa, fb)
     assert np.array_equal(u, u2) and np.array_equal(v, v2)
+    assert piv.last_fallback == (False, rejected)
+    assert piv.outlier_counts(1) == (int(st.val.sum()), int(st.outliers.sum()))
 
 
 def test_learned_path_falls_back_to_full_schedule():
@@ -178,8 +182,12 @@ def test_learned_path_falls_back_to_full_schedule():
     up, vp = plain.run_host(fa, fb)
     # a predictor 25 px off: the last pass alone cannot recover, most vectors are rejected
     learned = PIVGPU(shape, 16, predictor=_constant_predictor_net(25.0, 25.0), max_reject_fraction=0.05, **kw)
+    counts = [plain.outlier_counts(k) for k in range(2)]
     for _ in range(2):  # eager, then graph
         u, v = learned.run_host(fa, fb)
         fell_back, rejected = learned.last_fallback
         assert fell_back and rejected > 0.05 * u.size
         assert np.array_equal(u, up) and np.array_equal(v, vp)
+        # the full-schedule run's counters are its own (cleared after the
+        # learned attempt), equal to the plain plan's
+        assert [learned.outlier_counts(k) for k in range(2)] == counts

@@ -1,0 +1,100 @@
+"""The uneven-batch workload at its full size: 256 image pairs drawn by
+``uneven_pair_spec(4000 + i)`` (frames of 512^2, 768 x 1024, 2048^2,
+2048 x 3072 and 4096^2, uint16 12-bit particles, 1-4 Lamb-Oseen vortices plus
+shear, schedules 64->32->16 or 48->24 at 50-75 % overlap), run as one batch
+(one deformation and one correlation launch per FFT size per pass level,
+hipGraph-captured, launch shapes from the autotuner).
+
+* every pair's batched result equals its own single-pair plan bit for bit
+  (u, v, flags before and after replacement), eager, after tuning and on
+  graph replays -- the reference runs one pair per PIVGPU.__call__
+  (gpu_process.py:564-647);
+* three of the pairs (512^2 64->32->16 at 0.748 overlap, 768 x 1024 48->24
+  at 0.716, 2048^2 48->24 at 0.731) are rendered on the host and checked
+  against the CPU oracle with gpu_util.e2e_compare (u, v within 1e-3 px
+  except the excused vectors it names).
+
+Pairs other than those three are rendered on the device (same particle
+model, counter-based random numbers)."""
+
+import numpy as np
+import pytest
+
+from oracle import piv_oracle as O
+
+pytestmark = pytest.mark.gpu
+gu = pytest.importorskip("gpu_util")
+
+N_PAIRS = 256
+ORACLE_PAIRS = (91, 68, 57)
+
+
+@pytest.fixture(scope="module")
+def uneven_runs():
+    from openpiv_gpu import _native
+    from openpiv_gpu import synthetic as S
+    from openpiv_gpu.batch import PIVBatch, uneven_pair_spec
+    from openpiv_gpu.gpu_process import PIVGPU
+    specs = [uneven_pair_spec(4000 + i) for i in range(N_PAIRS)]
+    gens = {}
+    frames, host = [], {}
+    for i, (shape, _, _) in enumerate(specs):
+        if i in ORACLE_PAIRS:
+            fa, fb, _ = S.cfg_vortex_uint16(4000 + i, shape)
+            host[i] = (fa, fb)
+            frames.append((_native.DeviceBuffer.from_array(fa), _native.DeviceBuffer.from_array(fb)))
+        else:
+            gen = gens.setdefault(shape, S.DevicePairGenerator(shape, S.MODEL_12BIT))
+            frames.append(gen.render(S.vortex_field(4000 + i, shape), 4000 + i))
+    print(f"rendered {N_PAIRS} pairs", flush=True)
+    # each pair on its own plan (device frames, host outputs)
+    single = []
+    for i, ((shape, mss, kw), (da, db)) in enumerate(zip(specs, frames)):
+        piv = PIVGPU(shape, mss, dtype_f="float32", **kw)
+        plan = piv._get_plan(_native.PIV_U16)
+        m, n = piv.piv_fields[-1].field_shape
+        out = [np.empty((m, n), np.float32), np.empty((m, n), np.float32), np.empty((m, n), np.uint8),
+               np.empty((m, n), np.uint8)]
+        plan.run(da.value, db.value, True, *[a.ctypes.data for a in out], False)
+        counts = [piv.outlier_counts(k) for k in range(len(piv.piv_fields))]
+        single.append((out, counts))
+        del plan, piv
+    print("single-pair plans done", flush=True)
+    batch = PIVBatch(specs)
+    batch.set_frames(frames)
+    runs = []
+    for rep in range(4):   # eager, tuned (eager again), capture + replay, replay
+        if rep == 1:
+            batch.tune(repeats=2)
+        batch.run()
+        runs.append([batch.result(i) for i in range(N_PAIRS)])
+        print(f"batch run {rep} done", flush=True)
+    return specs, host, single, runs
+
+
+def test_uneven_batch_equals_single_pair_plans(uneven_runs):
+    specs, _, single, runs = uneven_runs
+    shapes = {s[0] for s in specs}
+    assert len(shapes) == 5 and {s[1] for s in specs} == {16, 24}
+    for rep, res in enumerate(runs):
+        for i in range(N_PAIRS):
+            u, v, vi, vf = res[i]
+            su, sv, svi, svf = single[i][0]
+            assert np.array_equal(u, su) and np.array_equal(v, sv), (rep, i, specs[i])
+            assert np.array_equal(vi, svi.astype(bool)) and np.array_equal(vf, svf.astype(bool)), (rep, i)
+
+
+@pytest.mark.parametrize("i", ORACLE_PAIRS)
+def test_uneven_pairs_match_oracle(uneven_runs, i):
+    specs, host, single, runs = uneven_runs
+    shape, mss, kw = specs[i]
+    fa, fb = host[i]
+    s = O.OracleSettings(min_search_size=mss, **kw)
+    r = O.piv(fa, fb, s, keep_passes=True)
+    excl = gu.pass_exclusions(fa, fb, s, r["passes"])
+    u, v, vi, vf = runs[-1][i]
+    rep = gu.e2e_compare(u, v, vi, r, excl, s, final_flags=vf)
+    assert rep["compared"] > 0.95 * u.size, rep
+    counts = single[i][1]
+    for k, st in enumerate(r["passes"]):
+        assert counts[k][0] == int(st.val.sum()), (k, counts[k], int(st.val.sum()))
