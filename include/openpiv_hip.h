@@ -224,6 +224,19 @@ int piv_stage_validate(const float *u, const float *v, int m, int n, const float
 int piv_stage_replace(float *u, float *v, uint8_t *flags, int m, int n, const piv_pass_desc *pass, const float *s2n,
                       void *stream);
 
+/* piv_stage_replace that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Its
+ * work buffers live in a fixed device array of the library (about 6 MB,
+ * held by every GPU the library runs on until the process ends), so
+ * m * n must not exceed PIV_NOALLOC_MAX_VECTORS (PIV_E_INVALID otherwise).
+ * Queued on the stream, no synchronisation: the results are ready when the
+ * stream's work is.  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one device (other streams, other threads) race. */
+#define PIV_NOALLOC_MAX_VECTORS 65536
+int piv_stage_replace_noalloc(float *u, float *v, uint8_t *flags, int m, int n, const piv_pass_desc *pass,
+                              const float *s2n, void *stream);
+
 /* Spring replacement of the flagged vectors in place.  Replaces
  * ReplacementGPU spring path + PIVGPU.replace_outliers
  * (gpu_validation.py:336-371, 381-415; gpu_process.py:788-834).  count is

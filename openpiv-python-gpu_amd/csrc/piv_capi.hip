@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -766,6 +767,97 @@ void release(piv_plan* P) {
   delete P;
 }
 
+// The replacement loop of piv_stage_replace on caller-provided work
+// buffers: the incoming flag count, replace_loop, and the final flags of a
+// revalidating loop.  Counters are cleared by fill kernels (no memset), so
+// the sequence can also be captured into a graph.
+int enqueue_stage_replace(float* u, float* v, uint8_t* flags, int m, int n, const piv_pass_desc& pass,
+                          const float* s2n, LoopBufs& L, int32_t* count0, hipStream_t s) {
+  const size_t N = (size_t)m * n;
+  hipError_t e = launch_fill_i32(L.it_counts, 0, PIV_MAX_REPLACING_ITERS, s);
+  if (e == hipSuccess) e = launch_fill_i32(count0, 0, 1, s);
+  // the count of the incoming flags (the validation kernel's counter)
+  if (e == hipSuccess) {
+    piv_pass_desc none = pass;
+    none.median_tol = none.mad_tol = none.mean_tol = none.rms_tol = none.s2n_tol = NAN;
+    e = launch_validate(validate_args<float>(none, u, v, m, n, nullptr, nullptr, nullptr, flags, L.wf[1], count0), s);
+  }
+  if (e != hipSuccess) return fail(PIV_E_HIP, "replace: %s", hipGetErrorString(e));
+  int r = replace_loop(u, v, L.wf[1], count0, m, n, pass, s2n, L, L.it_counts, s);
+  if (!r && pass.revalidate && pass.num_replacing_iters > 0) {
+    // final flags of the loop: the last revalidation's output
+    const uint8_t* last = L.wf[(pass.num_replacing_iters - 1) & 1];
+    e = hipMemcpyAsync(flags, last, N, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) r = fail(PIV_E_HIP, "replace: %s", hipGetErrorString(e));
+  }
+  return r;
+}
+
+}  // namespace
+
+// Fixed workspace of piv_stage_replace_noalloc, one copy per GPU for the life
+// of the process: u0, v0, tu, tv (4 B per vector each), two revalidation flag
+// buffers and the unresolved flags (1 B each), the spring slot and list
+// (4 B each) and its 8 doubles per vector, then the small counters -- each
+// part on a 256-byte boundary.
+namespace {
+constexpr size_t kNoAllocVec = PIV_NOALLOC_MAX_VECTORS;
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNoAllocBytes =
+    4 * align256(4 * kNoAllocVec) + 3 * align256(kNoAllocVec) + 2 * align256(4 * kNoAllocVec) +
+    align256(64 * kNoAllocVec) + align256(4 * PIV_MAX_REPLACING_ITERS) + align256(32) + align256(4);
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this GPU's copy of `scratch`, looked up once per
+// device with that device current (the array's name is only a host-side
+// handle: passing it to a kernel would fault).
+int noalloc_scratch(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(PIV_E_INVALID, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PIV_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return PIV_OK;
+}
+
+// Work buffers for n vectors (n <= PIV_NOALLOC_MAX_VECTORS) carved out of the
+// fixed workspace in the order of the size formula above.
+void noalloc_layout(unsigned char* base, size_t n, LoopBufs* L, int32_t** count0) {
+  (void)n;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base + off;
+    off += align256(bytes);
+    return (void*)p;
+  };
+  L->u0 = take(4 * kNoAllocVec);
+  L->v0 = take(4 * kNoAllocVec);
+  L->tu = take(4 * kNoAllocVec);
+  L->tv = take(4 * kNoAllocVec);
+  L->wf[0] = (uint8_t*)take(kNoAllocVec);
+  L->wf[1] = (uint8_t*)take(kNoAllocVec);
+  L->nanf = (uint8_t*)take(kNoAllocVec);
+  L->rw.slot = (int32_t*)take(4 * kNoAllocVec);
+  L->rw.list = (int32_t*)take(4 * kNoAllocVec);
+  L->rw.work = (double*)take(64 * kNoAllocVec);
+  L->it_counts = (int32_t*)take(4 * PIV_MAX_REPLACING_ITERS);
+  L->stats = (double*)take(32);
+  *count0 = (int32_t*)take(4);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1289,27 +1381,28 @@ int piv_stage_replace(float* u, float* v, uint8_t* flags, int m, int n, const pi
     cleanup();
     return fail(PIV_E_NOMEM, "device allocation failed");
   }
-  hipError_t e = launch_fill_i32(L.it_counts, 0, PIV_MAX_REPLACING_ITERS, s);
-  if (e == hipSuccess) e = launch_fill_i32(count0, 0, 1, s);
-  // the count of the incoming flags (the validation kernel's counter)
-  if (e == hipSuccess) {
-    piv_pass_desc none = *pass;
-    none.median_tol = none.mad_tol = none.mean_tol = none.rms_tol = none.s2n_tol = NAN;
-    e = launch_validate(validate_args<float>(none, u, v, m, n, nullptr, nullptr, nullptr, flags, L.wf[1], count0), s);
-  }
-  if (e != hipSuccess) {
-    cleanup();
-    return fail(PIV_E_HIP, "replace: %s", hipGetErrorString(e));
-  }
-  r = replace_loop(u, v, L.wf[1], count0, m, n, *pass, s2n, L, L.it_counts, s);
-  if (!r && pass->revalidate && pass->num_replacing_iters > 0) {
-    // final flags of the loop: the last revalidation's output
-    const uint8_t* last = L.wf[(pass->num_replacing_iters - 1) & 1];
-    e = hipMemcpyAsync(flags, last, N, hipMemcpyDeviceToDevice, s);
-    if (e != hipSuccess) r = fail(PIV_E_HIP, "replace: %s", hipGetErrorString(e));
-  }
+  r = enqueue_stage_replace(u, v, flags, m, n, *pass, s2n, L, count0, s);
   cleanup();
   return r;
+}
+
+int piv_stage_replace_noalloc(float* u, float* v, uint8_t* flags, int m, int n, const piv_pass_desc* pass,
+                              const float* s2n, void* stream) {
+  if (!u || !v || !flags || !pass || m < 1 || n < 1) return fail(PIV_E_INVALID, "bad argument");
+  if ((size_t)m * n > (size_t)PIV_NOALLOC_MAX_VECTORS)
+    return fail(PIV_E_INVALID, "piv_stage_replace_noalloc: %d x %d vectors exceed its fixed workspace of %d vectors",
+                m, n, PIV_NOALLOC_MAX_VECTORS);
+  int r = check_tests(*pass);
+  if (r) return r;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  unsigned char* base = nullptr;
+  r = noalloc_scratch(dev, &base);
+  if (r) return r;
+  LoopBufs L{};
+  int32_t* count0 = nullptr;
+  noalloc_layout(base, (size_t)m * n, &L, &count0);
+  return enqueue_stage_replace(u, v, flags, m, n, *pass, s2n, L, count0, (hipStream_t)stream);
 }
 
 int piv_stage_spring_replace(float* u, float* v, const uint8_t* flags, const int32_t* count, int m, int n,

@@ -40,7 +40,7 @@ EXPORTED_SYMBOLS = (
     "piv_batch_pair_shape", "piv_batch_get", "piv_tiff_info", "piv_tiff_read", "piv_host_alloc", "piv_host_free",
     "piv_memcpy_async", "piv_stream_wait_event", "piv_plan_wait_event", "piv_plan_set_mask",
     "piv_plan_last_smoothing_iterations", "piv_stage_smoothn_masked", "piv_stage_correlate64",
-    "piv_plan_last_s2n",
+    "piv_plan_last_s2n", "piv_stage_replace_noalloc",
 )
 
 SYNTH_MAX_VORTICES = 8
@@ -146,6 +146,7 @@ def _bind(lib):
                                       vp, vp]),
         "piv_stage_validate": (i32, [vp, vp, i32, i32, vp, ctypes.POINTER(PassDesc), vp, vp, vp, vp]),
         "piv_stage_replace": (i32, [vp, vp, vp, i32, i32, ctypes.POINTER(PassDesc), vp, vp]),
+        "piv_stage_replace_noalloc": (i32, [vp, vp, vp, i32, i32, ctypes.POINTER(PassDesc), vp, vp]),
         "piv_stage_deform": (i32, [vp, vp, i32, i32, i32, ctypes.POINTER(PassDesc), vp, vp, vp, vp, vp]),
         "piv_stage_grid_interp": (i32, [ctypes.POINTER(PassDesc), ctypes.POINTER(PassDesc), vp, vp, vp, vp, vp]),
         "piv_stage_median_validate": (i32, [vp, vp, i32, i32, f32, vp, vp, vp]),
