@@ -1700,6 +1700,7 @@ namespace {
 struct BatchGroup {  // one batched correlation launch
   int fsize = 0, in_dtype = 0, total_blocks = 0, max_ss = 0, xcd = 1;
   bool line = false;  // one-lane-per-line kernel (the one a single-pair run of these fields uses)
+  bool split = false;  // residue-split kernel (likewise)
   bool half = false;  // search windows <= F/2 (the two-lane kernel's variant for them)
   std::vector<int> plans;
   CorrArgs* d_fields = nullptr;
@@ -1781,16 +1782,19 @@ int batch_build_tables(piv_batch* b) {
                               : corr_args(P->def_a, P->def_b, kF32, D.height, D.width, B.d);
       probe.s2n = B.s2n;
       const bool line = correlate_line_enabled() && correlate_line_applies(probe);
+      const bool split = !line && correlate_split_enabled() && correlate_split_applies(probe);
       const bool half = 2 * B.d.search_size <= B.d.fft_size;
       BatchGroup* g = nullptr;
       for (auto& q : L.groups)
-        if (q.fsize == B.d.fft_size && q.in_dtype == dtype && q.line == line && q.half == half) g = &q;
+        if (q.fsize == B.d.fft_size && q.in_dtype == dtype && q.line == line && q.split == split && q.half == half)
+          g = &q;
       if (!g) {
         L.groups.push_back(BatchGroup{});
         g = &L.groups.back();
         g->fsize = B.d.fft_size;
         g->in_dtype = dtype;
         g->line = line;
+        g->split = split;
         g->half = half;
         g->xcd = L.groups.size() <= keep_xcd.size() ? keep_xcd[L.groups.size() - 1] : 1;
       }
@@ -1799,7 +1803,9 @@ int batch_build_tables(piv_batch* b) {
     for (auto& g : L.groups) {
       std::vector<CorrArgs> fields;
       std::vector<int> prefix;
-      const int wpb = g.line ? correlate_line_windows_per_block(g.fsize) : correlate_windows_per_block(g.fsize);
+      const int wpb = g.line    ? correlate_line_windows_per_block(g.fsize)
+                      : g.split ? correlate_split_windows_per_block(g.fsize)
+                                : correlate_windows_per_block(g.fsize);
       g.total_blocks = 0;
       for (int p : g.plans) {
         piv_plan* P = b->plans[p];
@@ -1855,6 +1861,8 @@ int batch_enqueue_corr(piv_batch* b, BatchGroup& g, hipStream_t s) {
   CorrBatch cb{g.d_fields, g.d_prefix, (int)g.plans.size(), g.xcd};
   if (g.line)
     HIP_TRY(launch_correlate_line_batch(cb, g.fsize, g.in_dtype, g.total_blocks, g.max_ss, s));
+  else if (g.split)
+    HIP_TRY(launch_correlate_split_batch(cb, g.fsize, g.in_dtype, g.total_blocks, s));
   else
     HIP_TRY(launch_correlate_batch(cb, g.fsize, g.in_dtype, g.total_blocks, g.max_ss, g.half, s));
   return PIV_OK;

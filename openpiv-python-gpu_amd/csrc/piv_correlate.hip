@@ -700,6 +700,8 @@ hipError_t launch_correlate(const CorrArgs& a, hipStream_t stream) {
   // F <= 32 with a search window <= F/2 and neither S2N nor a shrunk
   // interrogation window: the one-lane-per-line kernel
   if (correlate_line_enabled() && correlate_line_applies(a)) return launch_correlate_line(a, stream);
+  // F = 64, 128 under the same conditions: the residue-split kernel
+  if (correlate_split_enabled() && correlate_split_applies(a)) return launch_correlate_split(a, stream);
   switch (a.fsize) {
     case 16: return launch_t<16>(a, stream);
     case 32: return launch_t<32>(a, stream);

@@ -93,6 +93,17 @@ int correlate_line_windows_per_block(int fsize);
 hipError_t launch_correlate_line_batch(const CorrBatch& b, int fsize, int in_dtype, int total_blocks, int max_ss,
                                        hipStream_t stream);
 size_t correlate_lds_bytes(int fsize, int ss);
+// Residue-split variant for F = 64, 128 (search window <= F/2, no S2N, no
+// shrunk interrogation window): every 1-D transform split by output index
+// mod 4 over the four waves of a block.  launch_correlate picks it when it
+// applies and is enabled (OPENPIV_HIP_CORR_SPLIT=0 turns it off).
+bool correlate_split_applies(const CorrArgs& a);
+bool correlate_split_enabled();
+hipError_t launch_correlate_split(const CorrArgs& a, hipStream_t stream);
+int correlate_split_windows_per_block(int fsize);
+hipError_t launch_correlate_split_batch(const CorrBatch& b, int fsize, int in_dtype, int total_blocks,
+                                        hipStream_t stream);
+size_t correlate_split_lds_bytes(int fsize);
 
 // Frame deformation: predictor bilinearly interpolated (with linear
 // extrapolation) from the pass grid onto each pixel, then a backward bilinear
