@@ -16,7 +16,10 @@
 #include <limits.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "piv_kernels.h"
+#include "piv_wave.h"
 
 namespace piv {
 
@@ -26,109 +29,27 @@ __device__ __forceinline__ float sfma(float a, float b, float c) { return fmaf(a
 __device__ __forceinline__ double sfma(double a, double b, double c) { return fma(a, b, c); }
 
 // ---------------------------------------------------------------------------
-// Small batched GEMM for the 2-D DCTs: for f in {0, 1}
-//   C_f(r, c) = sum_k A_f(r, k) B_f(k, c),  A(r,k) = ta ? A[k*lda + r] : A[r*lda + k]
-//                                           B(k,c) = tb ? B[c*ldb + k] : B[k*ldb + c]
-// 16 x 16 output tile per 256-thread workgroup (one output per thread), K
-// staged through LDS in chunks of 64; FMA accumulation in k order.
+// GCV search, three launches and a final decision inside the consumer:
+//   grid:    kGrid p-values evenly spaced over [p_min, p_max] (every seventh
+//            is a point of the reference's 10-point span), one per lane; the
+//            spectrum split over up to kGcvBlocks workgroups -> partials
+//   newton1: every workgroup sums the grid partials in a fixed order and
+//            decides (first argmin of the span -> bracket between its
+//            neighbours; in it, the sign change of dg/dp nearest the span
+//            minimum, 9x finer than the span -> secant root p1), then scores
+//            p1 over its part of the spectrum -> partials
+//   newton2: every workgroup decides p2 (safeguarded Newton step on
+//            dg/dp = 0 from p1), scores it -> partials
+//   final:   the consumer (the inverse DCT that scales the spectrum by
+//            Gamma(p), or a one-workgroup kernel) decides p3 the same way.
+// Every workgroup of a launch reaches the same decision from the same
+// partials (deterministic sums); workgroup 0 also writes it to the search
+// state for the next launch.  Partials pass between launches only, so no
+// workgroup waits for another and no cache maintenance is needed.
 // ---------------------------------------------------------------------------
-struct GcvState;
-
-template <typename R>
-struct GemmArgs {
-  const R* a[2];
-  const R* b[2];
-  R* c[2];
-  int lda, ldb, ldc;
-  int ta, tb;
-  int M, N, K;
-  // optional: B(k, c) scaled by Gamma = 1 / (1 + 10^p lam[k * ldb + c])
-  // (untransposed B), p from the search state or fixed; p written to p_out
-  const double* lam;
-  const GcvState* gst;
-  double p_fixed;
-  double* p_out;
-};
-
-__device__ double gcv_state_p(const GcvState* st);
-
-template <typename R>
-__global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs<R> g) {
-  constexpr int KC = 64;  // K per LDS stage: four loads of A and four of B in flight per thread
-  __shared__ R As[16][KC + 1];
-  __shared__ R Bs[KC][17];
-  const int f = blockIdx.z;
-  const R* A = g.a[f];
-  const R* B = g.b[f];
-  const int r0 = blockIdx.y * 16, c0 = blockIdx.x * 16;
-  const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
-  double sgam = 0.0;
-  if (g.lam) {
-    const double p = g.gst ? gcv_state_p(g.gst) : g.p_fixed;
-    sgam = pow(10.0, p);
-    if (g.p_out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) *g.p_out = p;
-  }
-  R acc = R(0);
-  for (int k0 = 0; k0 < g.K; k0 += KC) {
-    R va[4], vb[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      // consecutive threads run along the contiguous direction of each operand
-      const int e = threadIdx.x + 256 * i;
-      const int rr = g.ta ? e % 16 : e / KC, ka = g.ta ? e / 16 : e % KC;
-      const int r = r0 + rr, k = k0 + ka;
-      va[i] = (r < g.M && k < g.K) ? (g.ta ? A[(size_t)k * g.lda + r] : A[(size_t)r * g.lda + k]) : R(0);
-      const int kb = g.tb ? e % KC : e / 16, cc = g.tb ? e / KC : e % 16;
-      const int k2 = k0 + kb, c = c0 + cc;
-      vb[i] = (k2 < g.K && c < g.N) ? (g.tb ? B[(size_t)c * g.ldb + k2] : B[(size_t)k2 * g.ldb + c]) : R(0);
-      if (g.lam && k2 < g.K && c < g.N) {  // z_dct = Gamma f_dct (gpu_smoothn.py:196-199)
-        const double G = 1.0 / (1.0 + sgam * g.lam[(size_t)k2 * g.ldb + c]);
-        vb[i] = (R)(G * (double)vb[i]);
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int e = threadIdx.x + 256 * i;
-      As[g.ta ? e % 16 : e / KC][g.ta ? e / 16 : e % KC] = va[i];
-      Bs[g.tb ? e % KC : e / 16][g.tb ? e / KC : e % 16] = vb[i];
-    }
-    __syncthreads();
-    const int kn = min(KC, g.K - k0);
-    for (int kk = 0; kk < kn; ++kk) acc = sfma(As[ty][kk], Bs[kk][tx], acc);
-    __syncthreads();
-  }
-  const int r = r0 + ty, c = c0 + tx;
-  if (r < g.M && c < g.N) g.c[f][(size_t)r * g.ldc + c] = acc;
-}
-
-template <typename R>
-hipError_t gemm2(const R* a0, const R* a1, int lda, int ta, const R* b0, const R* b1, int ldb, int tb, R* c0, R* c1,
-                 int ldc, int M, int N, int K, hipStream_t s, const double* lam = nullptr,
-                 const GcvState* gst = nullptr, double p_fixed = 0.0, double* p_out = nullptr) {
-  GemmArgs<R> g{{a0, a1}, {b0, b1}, {c0, c1}, lda, ldb, ldc, ta, tb, M, N, K, lam, gst, p_fixed, p_out};
-  dim3 grid((N + 15) / 16, (M + 15) / 16, 2);
-  hipLaunchKernelGGL(gemm_small_kernel<R>, grid, dim3(256), 0, s, g);
-  return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// GCV search.  Every evaluation launch spreads the m*n spectrum over
-// kGcvBlocks workgroups of 1024 threads; in the grid evaluation thread t of a
-// block scores p-value (t % 64) over a sixteenth of the block's elements, in
-// a Newton step every thread scores the one p-value.  The last block to finish reduces the per-block
-// partials (stored p-major so each sum reads one contiguous run, summed in a
-// fixed order: deterministic) and picks the next p-values:
-//   phase 0: 64 evenly spaced points over [p_min, p_max]; every seventh is a
-//            point of the reference's 10-point span.  First argmin of the
-//            span -> bracket between its neighbours; in it, the sign change
-//            of dg/dp nearest the span minimum (9x finer than the span) ->
-//            secant root
-//   phase 1, 2: safeguarded Newton steps on dg/dp = 0 from the secant root
-// then the inverse DCT scales the spectrum by Gamma(p) as it reads it.
-// ---------------------------------------------------------------------------
-constexpr int kGcvBlocks = 64;
-constexpr int kGcvThreads = 1024;
-constexpr int kNewtonInBlockMax = 32768;  // spectra up to this size take the Newton steps in one workgroup
+constexpr int kGcvThreads = 256;
+constexpr int kGcvBlocks = 128;              // at most, grid launch (128 elements each at least)
+constexpr int kGcvNewtonBlocks = 64;         // at most, per Newton step
 constexpr int kSpan = 10;  // N_P0, gpu_smoothn.py:33
 constexpr int kGrid = 64;
 constexpr int kSpanStep = (kGrid - 1) / (kSpan - 1);  // grid points per span interval
@@ -142,8 +63,15 @@ struct GcvState {
   int done;             // minimum fixed at a bound / by the grid
   double p;             // current estimate
   double p_min, p_max;
-  uint32_t arrivals;    // blocks of the current evaluation that finished (0 between launches)
+  uint32_t arrivals;    // (unused by the unweighted search)
+  int32_t nb[3];        // slot 3: workgroups of the grid / newton1 / newton2 launches
 };
+
+// Partials of the three launches, after the state.
+constexpr size_t kGcvPart0 = 0;                                   // [b][p][4]
+constexpr size_t kGcvPart1 = kGcvPart0 + (size_t)kGcvBlocks * kGrid * 4;  // [b][6]
+constexpr size_t kGcvPart2 = kGcvPart1 + (size_t)kGcvNewtonBlocks * 6;    // [b][6]
+constexpr size_t kGcvPartTotal = kGcvPart2 + (size_t)kGcvNewtonBlocks * 6;
 
 __device__ double gcv_state_p(const GcvState* st) { return st->p; }
 
@@ -199,7 +127,8 @@ __device__ __forceinline__ void wave_argmin(double& v, int& i) {
 }
 
 // Picks the next p-value from the sums tot[p-value][sum] of the evaluation
-// that just finished (phases as described above); wave 0 of the last block.
+// that just finished (phases as described above); wave 0.  st may be a
+// workgroup-local copy.
 template <int PHASE>
 __device__ void gcv_decide(int N, GcvState* st, double p_min, double p_max, const double (*tot)[6]) {
   const int lane = threadIdx.x;
@@ -210,7 +139,7 @@ __device__ void gcv_decide(int N, GcvState* st, double p_min, double p_max, cons
   const double fi = lane < npv ? phi_of(tot[lane], N) : 0.0;
   const double fnext = __shfl_down(fi, 1, 64);
   const double pnext = __shfl_down(pvi, 1, 64);
-  double a = PHASE == 0 ? p_min : st->a, b = PHASE == 0 ? p_max : st->b, pnew = st->p;
+  double a = PHASE == 0 ? p_min : st->a, b = PHASE == 0 ? p_max : st->b, pnew = PHASE == 0 ? p_min : st->p;
   int newly_done = 0;
   if (PHASE == 0) {
     // first argmin of the span scores (ties -> lowest index)
@@ -272,193 +201,370 @@ __device__ void gcv_decide(int N, GcvState* st, double p_min, double p_max, cons
   }
 }
 
-// One evaluation: every block scores the pending p-values over its slice of
-// the spectrum and writes per-block partials (p-major); the last block to
-// finish sums them in block order and decides the next p-values.
-// newton_here (grid evaluation only): the last block also runs both Newton
-// steps itself, on a copy of the search state in LDS, and writes the state
-// back at the end -- for spectra small enough that one workgroup evaluates a
-// p-value faster than a launch of kGcvBlocks does.
-template <int PHASE, typename R>
-__global__ __launch_bounds__(kGcvThreads) void gcv_eval_kernel(const R* Y, const double* lam, int N, GcvState* st,
-                                                               double* part, double p_min, double p_max,
-                                                               const int32_t* skip, int newton_here) {
-  if (skip && *skip) return;
-  constexpr int NS = PHASE >= 1 ? 6 : 4;
-  // the grid evaluation spreads 64 p-values over the lanes of each wave; a
-  // Newton step puts every thread on its single p-value
-  constexpr int PV = PHASE == 0 ? kGrid : 1;
-  constexpr int SUBS = kGcvThreads / PV;
-  constexpr int WAVES = kGcvThreads / 64;
-  __shared__ double red[WAVES][PV][NS];
-  __shared__ double tot[kGrid][6];
-  __shared__ int is_last;
-  const int tid = threadIdx.x;
-  const int pi = tid % PV, sub = tid / PV;
-  const int npv = PHASE == 0 ? kGrid : st->npv;
-  const bool done = PHASE == 0 ? false : (st->done != 0);
-  double acc[NS];
+// The search state is four GcvState slots: [1] the decision after the grid
+// launch (written by newton1's workgroup 0), [2] the decision after newton1
+// (by newton2's workgroup 0), [0] the final decision (by the consumer's
+// workgroup 0; what every later reader of the search takes), and [3].nb the
+// launch sizes.  A launch only reads slots written by earlier launches, so
+// its workgroup 0 can publish while the others still read.
+constexpr int kGcvSlots = 4;
+
+// sum_{b < nb} p[b * stride] in b order, sixteen loads in flight at a time
+// (the partials come from other workgroups' earlier launch: far away).
+__device__ __forceinline__ double sum_strided(const double* p, int stride, int nb) {
+  double t = 0.0;
+  for (int b0 = 0; b0 < nb; b0 += 16) {
+    double x[16];
 #pragma unroll
-  for (int q = 0; q < NS; ++q) acc[q] = 0.0;
-  if (pi < npv && !done) {
-    const double pv = PHASE == 0 ? grid_p(pi, p_min, p_max) : st->pv[pi];
-    const double s = pow(10.0, pv);
-    const int chunk = (N + kGcvBlocks - 1) / kGcvBlocks;
-    const int k0 = blockIdx.x * chunk, k1 = min(N, k0 + chunk);
-    // four independent elements per step keep several loads in flight
-    for (int k = k0 + sub; k < k1; k += 4 * SUBS) {
-      double S4[4], L4[4];
+    for (int e = 0; e < 16; ++e) x[e] = b0 + e < nb ? p[(size_t)(b0 + e) * stride] : 0.0;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int kk = k + SUBS * e;
-        const bool in = kk < k1;
-        const double y0 = in ? (double)Y[kk] : 0.0, y1 = in ? (double)Y[N + kk] : 0.0;
-        S4[e] = y0 * y0 + y1 * y1;
-        L4[e] = in ? lam[kk] : 0.0;
-      }
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        double t[6];
-        gcv_terms(S4[e], L4[e], s, t);
-        const double w = (k + SUBS * e < k1) ? 1.0 : 0.0;
-#pragma unroll
-        for (int q = 0; q < NS; ++q) acc[q] += w * t[q];
-      }
-    }
+    for (int e = 0; e < 16; ++e)
+      if (b0 + e < nb) t += x[e];
   }
-  // lanes of a wave that share a p-value, then the waves in order
-#pragma unroll
-  for (int off = 32; off >= PV; off >>= 1) {
-#pragma unroll
-    for (int q = 0; q < NS; ++q) acc[q] += __shfl_xor(acc[q], off, 64);
-  }
-  const int lane = tid & 63, wave = tid >> 6;
-  if (lane < PV) {
-#pragma unroll
-    for (int q = 0; q < NS; ++q) red[wave][lane][q] = acc[q];
-  }
-  __syncthreads();
-  if (tid < PV * NS) {
-    const int p = tid % PV, q = tid / PV;
-    double v = 0.0;
-#pragma unroll
-    for (int u = 0; u < WAVES; ++u) v += red[u][p][q];
-    part[((size_t)p * 6 + q) * kGcvBlocks + blockIdx.x] = v;
-    __threadfence();
-  }
-  __syncthreads();
-  if (tid == 0) is_last = atomicAdd(&st->arrivals, 1u) == (unsigned)(gridDim.x - 1);
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  // teams of 8 consecutive lanes reduce one (p-value, sum) row of the
-  // kGcvBlocks partials (8 per lane, all loads in flight, then 3 shuffles)
-  const int team = tid >> 3, mate = tid & 7;
-  constexpr int PER = kGcvBlocks / 8;
-  for (int r = team; r < npv * NS; r += kGcvThreads / 8) {
-    const int pj = r / NS, q = r % NS;
-    const double2* src = reinterpret_cast<const double2*>(part + ((size_t)pj * 6 + q) * kGcvBlocks) + mate * (PER / 2);
-    double2 x[PER / 2];
-#pragma unroll
-    for (int e = 0; e < PER / 2; ++e) x[e] = src[e];
-    double t = 0.0;
-#pragma unroll
-    for (int e = 0; e < PER / 2; ++e) t += x[e].x + x[e].y;
-    t += __shfl_xor(t, 4, 64);
-    t += __shfl_xor(t, 2, 64);
-    t += __shfl_xor(t, 1, 64);
-    if (mate == 0) tot[pj][q] = t;
-  }
-  if (tid == 0) st->arrivals = 0;
-  __syncthreads();
-  if (PHASE != 0 || !newton_here) {
-    if (tid < 64) gcv_decide<PHASE>(N, st, p_min, p_max, tot);
-    return;
-  }
-  __shared__ GcvState ls;
-  __shared__ double red6[kGcvThreads / 64][6];
-  if (tid < 64) gcv_decide<0>(N, &ls, p_min, p_max, tot);
-  auto newton = [&](auto STEP) {
-    __syncthreads();  // ls written by wave 0
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    if (!ls.done) {
-      const double s = pow(10.0, ls.pv[0]);
-      // four elements' loads in flight per step; accumulated in element order
-      for (int k = tid; k < N; k += 4 * kGcvThreads) {
-        double S4[4], L4[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int kk = k + e * kGcvThreads;
-          const bool in = kk < N;
-          const double y0 = in ? (double)Y[kk] : 0.0, y1 = in ? (double)Y[N + kk] : 0.0;
-          S4[e] = y0 * y0 + y1 * y1;
-          L4[e] = in ? lam[kk] : 0.0;
-        }
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          if (k + e * kGcvThreads < N) {
-            double t[6];
-            gcv_terms(S4[e], L4[e], s, t);
-#pragma unroll
-            for (int q = 0; q < 6; ++q) acc[q] += t[q];
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-      for (int q = 0; q < 6; ++q) acc[q] += __shfl_xor(acc[q], off, 64);
-    }
-    if (lane == 0) {
-#pragma unroll
-      for (int q = 0; q < 6; ++q) red6[wave][q] = acc[q];
-    }
-    __syncthreads();
-    if (tid < 6) {
-      double v = 0.0;
-      for (int w = 0; w < kGcvThreads / 64; ++w) v += red6[w][tid];
-      tot[0][tid] = v;
-    }
-    __syncthreads();
-    if (tid < 64) gcv_decide<decltype(STEP)::value>(N, &ls, p_min, p_max, tot);
-  };
-  newton(std::integral_constant<int, 1>{});
-  newton(std::integral_constant<int, 2>{});
-  __syncthreads();
-  if (tid < kGrid) st->pv[tid] = ls.pv[tid];
-  if (tid == 0) {
-    st->npv = ls.npv;
-    st->a = ls.a;
-    st->b = ls.b;
-    st->p_best_span = ls.p_best_span;
-    st->done = ls.done;
-    st->p = ls.p;
-    st->p_min = ls.p_min;
-    st->p_max = ls.p_max;
-  }
+  return t;
 }
 
+// The decision after launch PHASE (0 grid, 1 newton1, 2 newton2) from that
+// launch's partials, on a workgroup-local state: the previous decision is
+// slot PHASE, the partials are summed in workgroup order.  Every thread
+// returns after the decision.  Needs blockDim.x >= 256 for PHASE 0 (one
+// thread per p-value and sum) and >= 64 otherwise.
+template <int PHASE>
+__device__ void gcv_decide_from(const GcvState* gst, const double* part_base, int N, double p_min, double p_max,
+                                GcvState* ls, double (*tot)[6]) {
+  const int tid = threadIdx.x;
+  const int nb = gst[3].nb[PHASE];
+  if (PHASE == 0) {
+    const double* part = part_base + kGcvPart0;
+    const int p = tid & 63, q = tid >> 6;
+    if (tid < kGrid * 4) tot[p][q] = sum_strided(part + p * 4 + q, kGrid * 4, nb);
+  } else {
+    if (tid == 0) *ls = gst[PHASE];
+    const double* part = part_base + (PHASE == 1 ? kGcvPart1 : kGcvPart2);
+    if (tid < 6) tot[0][tid] = gst[PHASE].done ? 0.0 : sum_strided(part + tid, 6, nb);
+  }
+  __syncthreads();
+  if (tid < 64) gcv_decide<PHASE>(N, ls, p_min, p_max, tot);
+  __syncthreads();
+}
 
-// The GCV search on the spectrum Y (two fields, 2N floats): span with the
-// bracket grid, two Newton steps; leaves p in st->p.
+// Publishes the decision after launch PHASE (one thread of workgroup 0).
+template <int PHASE>
+__device__ __forceinline__ void gcv_publish(GcvState* gst, const GcvState& ls) {
+  gst[PHASE == 2 ? 0 : PHASE + 1] = ls;
+}
+
+// Grid evaluation: lane j of every wave scores grid p-value j; workgroup b
+// covers elements [b chunk, (b + 1) chunk) of the spectrum, its four waves
+// taking every fourth element (each element is read once per workgroup, by
+// all lanes at the same address).  Partials part0[b][p][sum].
+template <typename R>
+__global__ __launch_bounds__(kGcvThreads) void gcv_grid_kernel(const R* Y, const double* lam, int N, GcvState* gst,
+                                                               double* part_base, double p_min, double p_max,
+                                                               const int32_t* skip) {
+  if (skip && *skip) return;
+  __shared__ double red[kGcvThreads / 64][kGrid][4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int WAVES = kGcvThreads / 64;
+  if (blockIdx.x == 0 && tid == 0) gst[3].nb[0] = (int)gridDim.x;
+  const int chunk = (N + gridDim.x - 1) / gridDim.x;
+  const int k0 = blockIdx.x * chunk, k1 = min(N, k0 + chunk);
+  const double s = pow(10.0, grid_p(lane, p_min, p_max));
+  double acc[4] = {0, 0, 0, 0};
+  for (int k = k0 + wave; k < k1; k += 4 * WAVES) {
+    double S4[4], L4[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int kk = k + e * WAVES;
+      const bool in = kk < k1;
+      const double y0 = in ? (double)Y[kk] : 0.0, y1 = in ? (double)Y[N + kk] : 0.0;
+      S4[e] = y0 * y0 + y1 * y1;
+      L4[e] = in ? lam[kk] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (k + e * WAVES < k1) {
+        double t[6];
+        gcv_terms(S4[e], L4[e], s, t);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] += t[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) red[wave][lane][q] = acc[q];
+  __syncthreads();
+  const int p = tid & 63, q = tid >> 6;
+  double v = 0.0;
+#pragma unroll
+  for (int w = 0; w < WAVES; ++w) v += red[w][p][q];
+  part_base[kGcvPart0 + (size_t)blockIdx.x * (kGrid * 4) + p * 4 + q] = v;
+}
+
+// Second stage of the forward DCT fused with the grid evaluation: workgroup
+// (bx, by) computes the 16 x 16 tile T1_f = Cm T0_f of both fields, then
+// scores the 64 grid p-values over the tile's elements (lane j of each wave
+// takes p-value j, the four waves every fourth element) -> partials
+// part0[tile][p][sum], as gcv_grid_kernel would over the finished spectrum.
+template <typename R>
+__global__ __launch_bounds__(256) void dct2_gcv_kernel(const R* cm, const R* t0, R* t1, int m, int n,
+                                                       const double* lam, GcvState* gst, double* part_base,
+                                                       double p_min, double p_max) {
+  constexpr int KC = 64;
+  __shared__ R As[16][KC + 1];
+  __shared__ R Bs[2][KC][17];
+  __shared__ double sS[256], sL[256];
+  __shared__ double red[4][kGrid][4];
+  const int N = m * n;
+  const int tid = threadIdx.x, ty = tid / 16, tx = tid % 16;
+  const int r0 = blockIdx.y * 16, c0 = blockIdx.x * 16;
+  const int tile = blockIdx.y * gridDim.x + blockIdx.x;
+  if (tile == 0 && tid == 0) gst[3].nb[0] = (int)(gridDim.x * gridDim.y);
+  R acc0 = R(0), acc1 = R(0);
+  for (int k0 = 0; k0 < m; k0 += KC) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = tid + 256 * i;
+      {
+        const int rr = e / KC, ka = e % KC;  // A = Cm rows r0.., columns k0..
+        const int r = r0 + rr, k = k0 + ka;
+        As[rr][ka] = (r < m && k < m) ? cm[(size_t)r * m + k] : R(0);
+      }
+      {
+        const int kb = e / 16, cc = e % 16;  // B = T0 rows k0.., columns c0..
+        const int k = k0 + kb, c = c0 + cc;
+        const bool in = k < m && c < n;
+        Bs[0][kb][cc] = in ? t0[(size_t)k * n + c] : R(0);
+        Bs[1][kb][cc] = in ? t0[(size_t)N + (size_t)k * n + c] : R(0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk) {
+      const R a = As[ty][kk];
+      acc0 = sfma(a, Bs[0][kk][tx], acc0);
+      acc1 = sfma(a, Bs[1][kk][tx], acc1);
+    }
+    __syncthreads();
+  }
+  const int r = r0 + ty, c = c0 + tx;
+  const bool in = r < m && c < n;
+  if (in) {
+    t1[(size_t)r * n + c] = acc0;
+    t1[(size_t)N + (size_t)r * n + c] = acc1;
+  }
+  // out-of-range elements are marked by lam < 0 and skipped
+  sS[tid] = (double)acc0 * (double)acc0 + (double)acc1 * (double)acc1;
+  sL[tid] = in ? lam[(size_t)r * n + c] : -1.0;
+  __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6;
+  const double s = pow(10.0, grid_p(lane, p_min, p_max));
+  double a4[4] = {0, 0, 0, 0};
+  for (int e = wave; e < 256; e += 4) {
+    const double L = sL[e];
+    if (L >= 0.0) {
+      double t[6];
+      gcv_terms(sS[e], L, s, t);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a4[q] += t[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) red[wave][lane][q] = a4[q];
+  __syncthreads();
+  const int p = tid & 63, q = tid >> 6;
+  part_base[kGcvPart0 + (size_t)tile * (kGrid * 4) + p * 4 + q] = red[0][p][q] + red[1][p][q] + red[2][p][q] +
+                                                                     red[3][p][q];
+}
+
+// Newton step STEP: decide p from the previous launch's partials, then score
+// it over this workgroup's part of the spectrum.
+template <int STEP, typename R>
+__global__ __launch_bounds__(kGcvThreads) void gcv_newton_kernel(const R* Y, const double* lam, int N, GcvState* gst,
+                                                                 double* part_base, double p_min, double p_max,
+                                                                 const int32_t* skip) {
+  if (skip && *skip) return;
+  __shared__ GcvState ls;
+  __shared__ double tot[kGrid][6];
+  __shared__ double red[kGcvThreads / 64][6];
+  gcv_decide_from<STEP - 1>(gst, part_base, N, p_min, p_max, &ls, tot);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    gcv_publish<STEP - 1>(gst, ls);
+    gst[3].nb[STEP] = (int)gridDim.x;
+  }
+  if (ls.done) return;  // uniform: the minimum is fixed
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int chunk = (N + gridDim.x - 1) / gridDim.x;
+  const int k0 = blockIdx.x * chunk, k1 = min(N, k0 + chunk);
+  const double s = pow(10.0, ls.pv[0]);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = k0 + tid; k < k1; k += 4 * kGcvThreads) {
+    double S4[4], L4[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int kk = k + e * kGcvThreads;
+      const bool in = kk < k1;
+      const double y0 = in ? (double)Y[kk] : 0.0, y1 = in ? (double)Y[N + kk] : 0.0;
+      S4[e] = y0 * y0 + y1 * y1;
+      L4[e] = in ? lam[kk] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (k + e * kGcvThreads < k1) {
+        double t[6];
+        gcv_terms(S4[e], L4[e], s, t);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[q] += t[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) acc[q] = wave_sum(acc[q]);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) red[wave][q] = acc[q];
+  }
+  __syncthreads();
+  if (tid < 6)
+    part_base[(STEP == 1 ? kGcvPart1 : kGcvPart2) + (size_t)blockIdx.x * 6 + tid] =
+        red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+}
+
+// The final decision alone (for consumers that read st->p).
+__global__ __launch_bounds__(64) void gcv_final_kernel(GcvState* gst, double* part_base, int N, double p_min,
+                                                       double p_max, const int32_t* skip) {
+  if (skip && *skip) return;
+  __shared__ GcvState ls;
+  __shared__ double tot[kGrid][6];
+  gcv_decide_from<2>(gst, part_base, N, p_min, p_max, &ls, tot);
+  if (threadIdx.x == 0) gcv_publish<2>(gst, ls);
+}
+
+// The GCV search on the spectrum Y (two fields, 2N floats).  With
+// final_launch the decided p is in st->p when the launches end; without it
+// the consumer decides it (gcv_decide_from<2>).
+template <typename R>
+void enqueue_gcv_newton(const R* Y, const double* lam, int N, GcvState* st, double* part, double p_min,
+                        double p_max, const int32_t* skip, hipStream_t stream) {
+  const dim3 et(kGcvThreads);
+  const int nb = std::max(1, std::min(kGcvNewtonBlocks, (N + 4 * kGcvThreads - 1) / (4 * kGcvThreads)));
+  hipLaunchKernelGGL((gcv_newton_kernel<1, R>), dim3(nb), et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
+  hipLaunchKernelGGL((gcv_newton_kernel<2, R>), dim3(nb), et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
+}
+
 template <typename R>
 void enqueue_gcv_search(const R* Y, const double* lam, int N, GcvState* st, double* part, double p_min,
-                        double p_max, const int32_t* skip, hipStream_t stream) {
-  const dim3 eb(kGcvBlocks), et(kGcvThreads);
-  const int newton_here = N <= kNewtonInBlockMax;
-  hipLaunchKernelGGL((gcv_eval_kernel<0, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip,
-                     newton_here);
-  if (!newton_here) {
-    hipLaunchKernelGGL((gcv_eval_kernel<1, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip, 0);
-    hipLaunchKernelGGL((gcv_eval_kernel<2, R>), eb, et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip, 0);
+                        double p_max, const int32_t* skip, hipStream_t stream, bool final_launch = true) {
+  const dim3 et(kGcvThreads);
+  const int gb = std::max(1, std::min(kGcvBlocks, (N + 127) / 128));
+  hipLaunchKernelGGL((gcv_grid_kernel<R>), dim3(gb), et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
+  enqueue_gcv_newton(Y, lam, N, st, part, p_min, p_max, skip, stream);
+  if (final_launch)
+    hipLaunchKernelGGL(gcv_final_kernel, dim3(1), dim3(64), 0, stream, st, part, N, p_min, p_max, skip);
+}
+
+// ---------------------------------------------------------------------------
+// Small batched GEMM for the 2-D DCTs: for f in {0, 1}
+//   C_f(r, c) = sum_k A_f(r, k) B_f(k, c),  A(r,k) = ta ? A[k*lda + r] : A[r*lda + k]
+//                                           B(k,c) = tb ? B[c*ldb + k] : B[k*ldb + c]
+// 16 x 16 output tile per 256-thread workgroup (one output per thread), K
+// staged through LDS in chunks of 64; FMA accumulation in k order.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct GemmArgs {
+  const R* a[2];
+  const R* b[2];
+  R* c[2];
+  int lda, ldb, ldc;
+  int ta, tb;
+  int M, N, K;
+  // optional: B(k, c) scaled by Gamma = 1 / (1 + 10^p lam[k * ldb + c])
+  // (untransposed B), p fixed or decided here from the GCV search's last
+  // partials (gcv_decide_from<2>); p written to p_out
+  const double* lam;
+  GcvState* gst;
+  double p_fixed;
+  double* p_out;
+  const double* gpart;
+  int gN;
+  double gpmin, gpmax;
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs<R> g) {
+  constexpr int KC = 64;  // K per LDS stage: four loads of A and four of B in flight per thread
+  __shared__ R As[16][KC + 1];
+  __shared__ R Bs[KC][17];
+  const int f = blockIdx.z;
+  const R* A = g.a[f];
+  const R* B = g.b[f];
+  const int r0 = blockIdx.y * 16, c0 = blockIdx.x * 16;
+  const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+  double sgam = 0.0;
+  if (g.lam) {
+    double p = g.p_fixed;
+    const bool first = blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0;
+    if (g.gst) {
+      __shared__ GcvState ls;
+      __shared__ double tot[kGrid][6];
+      gcv_decide_from<2>(g.gst, g.gpart, g.gN, g.gpmin, g.gpmax, &ls, tot);
+      p = ls.p;
+      if (first) gcv_publish<2>(g.gst, ls);
+    }
+    sgam = pow(10.0, p);
+    if (g.p_out && first) *g.p_out = p;
   }
+  R acc = R(0);
+  for (int k0 = 0; k0 < g.K; k0 += KC) {
+    R va[4], vb[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      // consecutive threads run along the contiguous direction of each operand
+      const int e = threadIdx.x + 256 * i;
+      const int rr = g.ta ? e % 16 : e / KC, ka = g.ta ? e / 16 : e % KC;
+      const int r = r0 + rr, k = k0 + ka;
+      va[i] = (r < g.M && k < g.K) ? (g.ta ? A[(size_t)k * g.lda + r] : A[(size_t)r * g.lda + k]) : R(0);
+      const int kb = g.tb ? e % KC : e / 16, cc = g.tb ? e / KC : e % 16;
+      const int k2 = k0 + kb, c = c0 + cc;
+      vb[i] = (k2 < g.K && c < g.N) ? (g.tb ? B[(size_t)c * g.ldb + k2] : B[(size_t)k2 * g.ldb + c]) : R(0);
+      if (g.lam && k2 < g.K && c < g.N) {  // z_dct = Gamma f_dct (gpu_smoothn.py:196-199)
+        const double G = 1.0 / (1.0 + sgam * g.lam[(size_t)k2 * g.ldb + c]);
+        vb[i] = (R)(G * (double)vb[i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = threadIdx.x + 256 * i;
+      As[g.ta ? e % 16 : e / KC][g.ta ? e / 16 : e % KC] = va[i];
+      Bs[g.tb ? e % KC : e / 16][g.tb ? e / KC : e % 16] = vb[i];
+    }
+    __syncthreads();
+    // a whole chunk, unrolled: the staged operands are zero beyond K, so the
+    // extra terms add exact zeros and the LDS loads batch ahead of the FMAs
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk) acc = sfma(As[ty][kk], Bs[kk][tx], acc);
+    __syncthreads();
+  }
+  const int r = r0 + ty, c = c0 + tx;
+  if (r < g.M && c < g.N) g.c[f][(size_t)r * g.ldc + c] = acc;
+}
+
+template <typename R>
+hipError_t gemm2(const R* a0, const R* a1, int lda, int ta, const R* b0, const R* b1, int ldb, int tb, R* c0, R* c1,
+                 int ldc, int M, int N, int K, hipStream_t s, const double* lam = nullptr,
+                 GcvState* gst = nullptr, double p_fixed = 0.0, double* p_out = nullptr,
+                 const double* gpart = nullptr, int gN = 0, double gpmin = 0.0, double gpmax = 0.0) {
+  GemmArgs<R> g{{a0, a1}, {b0, b1}, {c0, c1}, lda, ldb, ldc, ta, tb, M, N, K, lam, gst, p_fixed, p_out,
+                gpart, gN, gpmin, gpmax};
+  dim3 grid((N + 15) / 16, (M + 15) / 16, 2);
+  hipLaunchKernelGGL(gemm_small_kernel<R>, grid, dim3(256), 0, s, g);
+  return hipGetLastError();
 }
 
 }  // namespace
 
-constexpr size_t kGcvPartOffset = (sizeof(GcvState) + 255) & ~size_t(255);
-size_t smoothn_state_bytes() { return kGcvPartOffset + (size_t)kGcvBlocks * kGrid * 6 * sizeof(double); }
+constexpr size_t kGcvPartOffset = (kGcvSlots * sizeof(GcvState) + 255) & ~size_t(255);
+size_t smoothn_state_bytes() { return kGcvPartOffset + kGcvPartTotal * sizeof(double); }
 
 template <typename R>
 hipError_t launch_smoothn(R* u, R* v, int m, int n, double p_min, double p_max, double fixed_p, int auto_s,
@@ -469,19 +575,37 @@ hipError_t launch_smoothn(R* u, R* v, int m, int n, double p_min, double p_max, 
   // forward orthonormal DCT-II: T0 = X Cn^T ; T1 = Cm T0
   e = gemm2(u, v, n, 0, w.cos_n, w.cos_n, n, 1, w.t0, w.t0 + N, n, m, n, n, stream);
   if (e != hipSuccess) return e;
-  e = gemm2(w.cos_m, w.cos_m, m, 0, w.t0, w.t0 + N, n, 0, w.t1, w.t1 + N, n, m, n, m, stream);
-  if (e != hipSuccess) return e;
-  if (auto_s) {
+  // tiles of the fused second stage: one partial block each, and the
+  // search state holds kGcvBlocks of them
+  const int tiles = ((n + 15) / 16) * ((m + 15) / 16);
+  if (auto_s && tiles <= kGcvBlocks) {
+    // T1 = Cm T0 with the GCV grid evaluation over each tile, then the
+    // Newton steps; the inverse DCT below makes the final decision
     GcvState* st = reinterpret_cast<GcvState*>(w.state);
     double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(w.state) + kGcvPartOffset);
-    enqueue_gcv_search(w.t1, w.lambda, N, st, part, p_min, p_max, nullptr, stream);
+    hipLaunchKernelGGL((dct2_gcv_kernel<R>), dim3((n + 15) / 16, (m + 15) / 16), dim3(256), 0, stream, w.cos_m,
+                       (const R*)w.t0, w.t1, m, n, w.lambda, st, part, p_min, p_max);
+    enqueue_gcv_newton(w.t1, w.lambda, N, st, part, p_min, p_max, nullptr, stream);
+  } else if (auto_s) {
+    // larger fields: the plain second stage, then the grid evaluation over
+    // at most kGcvBlocks workgroups
+    e = gemm2(w.cos_m, w.cos_m, m, 0, w.t0, w.t0 + N, n, 0, w.t1, w.t1 + N, n, m, n, m, stream);
+    if (e != hipSuccess) return e;
+    GcvState* st = reinterpret_cast<GcvState*>(w.state);
+    double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(w.state) + kGcvPartOffset);
+    enqueue_gcv_search(w.t1, w.lambda, N, st, part, p_min, p_max, nullptr, stream, false);
+  } else {
+    e = gemm2(w.cos_m, w.cos_m, m, 0, w.t0, w.t0 + N, n, 0, w.t1, w.t1 + N, n, m, n, m, stream);
+    if (e != hipSuccess) return e;
   }
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   // inverse: T0 = Cm^T (Gamma T1) ; Z = T0 Cn  (the Gamma scaling is applied
   // while T1 is read)
   e = gemm2(w.cos_m, w.cos_m, m, 1, w.t1, w.t1 + N, n, 0, w.t0, w.t0 + N, n, m, n, m, stream, w.lambda,
-            auto_s ? reinterpret_cast<const GcvState*>(w.state) : nullptr, fixed_p, w.p_out);
+            auto_s ? reinterpret_cast<GcvState*>(w.state) : nullptr, fixed_p, w.p_out,
+            reinterpret_cast<const double*>(reinterpret_cast<const char*>(w.state) + kGcvPartOffset), N, p_min,
+            p_max);
   if (e != hipSuccess) return e;
   return gemm2(w.t0, w.t0 + N, n, 0, w.cos_n, w.cos_n, n, 0, u, v, n, m, n, n, stream);
 }
@@ -920,14 +1044,13 @@ __global__ __launch_bounds__(256) void masked_out_kernel(R* u, R* v, const uint8
   }
 }
 
-// GCV partials: the spectral search's (kGcvBlocks x kGrid x 6) or the
+// GCV partials: the spectral search's (three launches) or the
 // batched-residual search's (kGrid p-values x 2 fields x up to 256 row
 // blocks x 2), whichever is larger
 constexpr size_t kGcvPartDoubles =
-    (size_t)kGcvBlocks * kGrid * 6 > (size_t)kGrid * 2 * 256 * 2 ? (size_t)kGcvBlocks * kGrid * 6
-                                                                  : (size_t)kGrid * 2 * 256 * 2;
+    kGcvPartTotal > (size_t)kGrid * 2 * 256 * 2 ? kGcvPartTotal : (size_t)kGrid * 2 * 256 * 2;
 constexpr size_t kWStateOffset =
-    ((((sizeof(GcvState) + 255) & ~size_t(255)) + kGcvPartDoubles * sizeof(double)) + 255) & ~size_t(255);
+    (kGcvPartOffset + kGcvPartDoubles * sizeof(double) + 255) & ~size_t(255);
 
 }  // namespace
 
