@@ -76,8 +76,11 @@ def stage_kernel_times(fa_u16, piv, reps=20):
                                                   u.ptr, v.ptr, None, None, None, None, sh))
         ms = timed(launch)
         flops = fft_flops_per_window(p.fft_size) * m * n
-        # F <= 32 with search window <= F/2 runs the one-lane-per-line kernel
-        kname = "corr_line_kernel" if p.fft_size <= 32 and 2 * p.search_size <= p.fft_size else "corr_kernel"
+        # search window <= F/2 without S2N: one lane per line for F <= 32, the
+        # residue-split kernel for F = 64, 128 (launch_correlate's choice)
+        half = 2 * p.search_size <= p.fft_size
+        kname = ("corr_line_kernel" if half and p.fft_size <= 32 else
+                 "corr_split_kernel" if half and p.fft_size in (64, 128) else "corr_kernel")
         out.append({"pass": k, "kernel": f"{kname}<F={p.fft_size},{'u16' if k == 0 else 'f32'}>",
                     "windows": m * n, "fft": p.fft_size,
                     "avg_ms": ms, "flop": flops, "tflops": flops / (ms * 1e-3) / 1e12})
@@ -342,16 +345,76 @@ def variant_rates(fa, fb, disp, steps):
     return out
 
 
-def cpu_baseline(fa, fb, pairs=1):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(fa, fb, timed=5):
+    """The reference pipeline on the host's cores: the NumPy/SciPy oracle
+    (FP32 correlation, FP64 smoothn) with scipy.fft and BLAS using every
+    core this process is given; one warm-up pair, then the median of `timed`
+    pairs, for the 2048x2048 64->32->16 workload and the 512x512 uint8
+    single-pass one."""
+    import scipy.fft
+    from threadpoolctl import threadpool_limits
+    from openpiv_gpu import synthetic
     from oracle import piv_oracle as O
-    s = O.OracleSettings(min_search_size=16, search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2.0)
-    t0 = time.perf_counter()
-    for _ in range(pairs):
-        O.piv(fa, fb, s)
-    dt = time.perf_counter() - t0
-    return {"value": pairs / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
-            "sample": f"{pairs} pair(s) of the 2048x2048 uint16 workload, full 64->32->16 loop, NumPy/SciPy "
-                      f"oracle (FP32 correlation, FP64 smoothn), single thread, {dt:.1f} s"}
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    a1, b1, _ = synthetic.cfg_512_uint8(1)
+    s2 = O.OracleSettings(min_search_size=16, search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2.0)
+    s1 = O.OracleSettings(min_search_size=32, overlap_ratio=0.5, median_tol=2.0)
+
+    def median_rate(a, b, s):
+        times = []
+        with scipy.fft.set_workers(cores), threadpool_limits(cores):
+            O.piv(a, b, s)  # warm-up
+            for _ in range(timed):
+                t0 = time.perf_counter()
+                O.piv(a, b, s)
+                times.append(time.perf_counter() - t0)
+        return 1.0 / float(np.median(times)), times
+
+    r2, t2 = median_rate(fa, fb, s2)
+    r1, t1 = median_rate(a1, b1, s1)
+    return {"value": round(r2, 4), "unit": "pairs/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": f"2048x2048 uint16 64->32->16 workload: 1 warm-up pair, median of {timed} pairs "
+                      f"({sum(t2):.1f} s timed); NumPy/SciPy oracle (FP32 correlation, FP64 smoothn), scipy.fft "
+                      f"workers and BLAS threads = {cores}",
+            "single_pass_512_uint8_pairs_per_s": round(r1, 3),
+            "pair_seconds": [round(t, 3) for t in t2]}
+
+
+def traffic_workload(steps=3, copies=3, copy_bytes=256 << 20):
+    """The program the HBM-traffic counter runs profile: the 2048x2048 uint16
+    64->32->16 pass loop (eager launches) on a device-rendered pair, then
+    device-to-device copies of a known size that calibrate FETCH_SIZE and
+    WRITE_SIZE."""
+    from openpiv_gpu import _native, synthetic
+    from openpiv_gpu.gpu_process import PIVGPU
+    shape = (2048, 2048)
+    gen = synthetic.DevicePairGenerator(shape, synthetic.MODEL_12BIT)
+    fa, fb = gen.render(synthetic.sine_displacement(4.0, 3.0, 256.0), seed=2)
+    piv = PIVGPU(shape, 16, search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=2, dtype_f="float32",
+                 use_graph=False)
+    plan = piv._get_plan(_native.PIV_U16)
+    m, n = piv.piv_fields[-1].field_shape
+    u, v = _native.DeviceBuffer((m, n), np.float32), _native.DeviceBuffer((m, n), np.float32)
+    for _ in range(steps):
+        plan.run(fa.value, fb.value, True, u.value, v.value, None, None, True)
+    src, dst = _native.DeviceBuffer((copy_bytes,), np.uint8), _native.DeviceBuffer((copy_bytes,), np.uint8)
+    lib = _native.load_library()
+    for _ in range(copies):
+        _native.check(lib.piv_memcpy(dst.ptr, src.ptr, copy_bytes, 2))
+    _native.synchronize()
+    return {"workload": "traffic", "pass_loops": steps, "copies": copies, "copy_bytes": copy_bytes}
 
 
 def main():
@@ -364,7 +427,7 @@ def main():
     ap.add_argument("--skip-kernel-timing", action="store_true")
     ap.add_argument("--learned-pairs", type=int, default=1000)
     ap.add_argument("--skip-learned", action="store_true")
-    ap.add_argument("--workload", choices=("pair2048", "uneven"), default="pair2048")
+    ap.add_argument("--workload", choices=("pair2048", "uneven", "traffic"), default="pair2048")
     ap.add_argument("--stream-pairs", type=int, default=48)
     ap.add_argument("--pairs", type=int, default=256)
     ap.add_argument("--variant-steps", type=int, default=20)
@@ -377,6 +440,9 @@ def main():
 
     if args.workload == "uneven":
         print(json.dumps(uneven_batch(args.pairs, max(1, args.steps))), flush=True)
+        return
+    if args.workload == "traffic":
+        print(json.dumps(traffic_workload()), flush=True)
         return
 
     from openpiv_gpu import _native, synthetic
@@ -447,7 +513,7 @@ def main():
             with open(tp) as f:
                 tk = json.load(f)["kernels"].get(dom["kernel"])
             traffic = None if tk is None else tk["traffic_bytes"]
-        roof = {"bound": "mfma", "compute": "fp32-valu (no matrix cores; FP32 vector peak)",
+        roof = {"bound": "valu-fp32", "compute": "FP32 vector ALU (no matrix cores); FP32 vector peak",
                 "kernel": dom["kernel"], "achieved": round(dom["tflops"], 3), "peak": 157.3, "unit": "TFLOP/s",
                 "frac": round(dom["tflops"] / 157.3, 4), "traffic": traffic,
                 "traffic_source": "profiles/hbm_traffic_2048_pipeline.md (FETCH_SIZE x2 + WRITE_SIZE, copy-calibrated)",
@@ -473,8 +539,8 @@ def main():
 
     cpu = None
     if not args.skip_cpu_baseline:
-        log("CPU baseline (oracle, 1 pair)")
-        cpu = cpu_baseline(fa, fb, 1)
+        log("CPU baseline (oracle on the host cores)")
+        cpu = cpu_baseline(fa, fb)
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
