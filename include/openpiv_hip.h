@@ -75,6 +75,13 @@ typedef struct piv_pass_desc {
   int32_t window_size;        /* interrogation window inside the search
                                * window (shrink_ratio, gpu_process.py:654-662,
                                * 1066-1081); 0 = search_size            */
+  int32_t validation_size;    /* half-width of the median / MAD / mean
+                               * neighbourhood, (2 size + 1)^2 kernel
+                               * (gpu_validation.py:73-76, 194-205);
+                               * 0 = 1                                  */
+  int32_t replacing_size;     /* half-width of the replacement kernel; the
+                               * spring links reach size vectors away
+                               * (gpu_validation.py:272-290); 0 = 1     */
   int32_t reserved;
 } piv_pass_desc;
 

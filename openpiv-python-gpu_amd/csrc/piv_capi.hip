@@ -46,13 +46,24 @@ namespace {
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
-// Validation / replacement settings of a pass.
-int check_tests(const piv_pass_desc& p) {
+// Kernel half-widths of a pass (0 reads as the reference's default 1).
+int validation_size(const piv_pass_desc& p) { return p.validation_size > 0 ? p.validation_size : 1; }
+int replacing_size(const piv_pass_desc& p) { return p.replacing_size > 0 ? p.replacing_size : 1; }
+
+// Validation / replacement settings of a pass on an m x n field.
+int check_tests(const piv_pass_desc& p, int m, int n) {
   if (p.s2n_method < PIV_PEAK2PEAK || p.s2n_method > PIV_PEAK2ENERGY) return fail(PIV_E_INVALID, "s2n_method");
   if (p.s2n_size < 1) return fail(PIV_E_INVALID, "s2n_size must be positive");
   if (p.replacing_method < PIV_SPRING || p.replacing_method > PIV_MEAN) return fail(PIV_E_INVALID, "replacing_method");
   if (p.num_replacing_iters < 0 || p.num_replacing_iters > PIV_MAX_REPLACING_ITERS)
     return fail(PIV_E_INVALID, "num_replacing_iters outside 0..%d", PIV_MAX_REPLACING_ITERS);
+  if (p.validation_size < 0 || p.validation_size > kMaxNeighbourhoodSize)
+    return fail(PIV_E_INVALID, "validation_size outside 1..%d", kMaxNeighbourhoodSize);
+  if (p.replacing_size < 0 || p.replacing_size > kMaxNeighbourhoodSize)
+    return fail(PIV_E_INVALID, "replacing_size outside 1..%d", kMaxNeighbourhoodSize);
+  // gpu_validation.py:74, 273
+  if (validation_size(p) > m || validation_size(p) > n || replacing_size(p) > m || replacing_size(p) > n)
+    return fail(PIV_E_INVALID, "validation_size / replacing_size exceed the %d x %d field", m, n);
   return PIV_OK;
 }
 
@@ -72,7 +83,7 @@ int check_pass(const piv_pass_desc& p, int ht, int wd) {
   if (p.window_size != 0 && (p.window_size < 2 || p.window_size > p.search_size ||
                              (p.search_size - p.window_size) % 2))
     return fail(PIV_E_INVALID, "window_size %d incompatible with search_size %d", p.window_size, p.search_size);
-  return check_tests(p);
+  return check_tests(p, p.field_m, p.field_n);
 }
 
 size_t dtype_size(int d) { return d == PIV_U8 ? 1 : (d == PIV_U16 ? 2 : (d == PIV_F32 ? 4 : 8)); }
@@ -511,6 +522,7 @@ ValidateArgsT<R> validate_args(const piv_pass_desc& d, const R* u, const R* v, i
   a.v = v;
   a.m = m;
   a.n = n;
+  a.size = validation_size(d);
   a.s2n = s2n;
   a.s2n_tol = s2n ? (R)d.s2n_tol : (R)NAN;
   a.median_tol = (R)d.median_tol;
@@ -564,11 +576,11 @@ int replace_loop(R* u, R* v, const uint8_t* flags0, int32_t* count0, int m, int 
   int32_t* cur_count = count0;
   for (int it = 0; it < iters; ++it) {
     if (spring) {
-      HIP_TRY(launch_spring_replace(u, v, cur, cur_count, m, n, L.rw, s));
+      HIP_TRY(launch_spring_replace(u, v, cur, cur_count, m, n, replacing_size(d), L.rw, s));
     } else {
       HIP_TRY(launch_copy(tu, (const R*)u, N, s));
       HIP_TRY(launch_copy(tv, (const R*)v, N, s));
-      HIP_TRY(launch_neighbour_replace((const R*)tu, (const R*)tv, u, v, cur, m, n, d.replacing_method,
+      HIP_TRY(launch_neighbour_replace((const R*)tu, (const R*)tv, u, v, cur, m, n, replacing_size(d), d.replacing_method,
                                        (it == 0 || reval) ? 1 : 0, s));
     }
     if (reval) {
@@ -1340,7 +1352,7 @@ int piv_stage_median_validate(const float* u, const float* v, int m, int n, floa
 int piv_stage_validate(const float* u, const float* v, int m, int n, const float* s2n, const piv_pass_desc* pass,
                        const uint8_t* only, uint8_t* flags, int32_t* count, void* stream) {
   if (!u || !v || !pass || !flags || !count || m < 1 || n < 1) return fail(PIV_E_INVALID, "bad argument");
-  int r = check_tests(*pass);
+  int r = check_tests(*pass, m, n);
   if (r) return r;
   double* stats = nullptr;
   hipStream_t s = (hipStream_t)stream;
@@ -1356,7 +1368,7 @@ int piv_stage_validate(const float* u, const float* v, int m, int n, const float
 int piv_stage_replace(float* u, float* v, uint8_t* flags, int m, int n, const piv_pass_desc* pass, const float* s2n,
                       void* stream) {
   if (!u || !v || !flags || !pass || m < 1 || n < 1) return fail(PIV_E_INVALID, "bad argument");
-  int r = check_tests(*pass);
+  int r = check_tests(*pass, m, n);
   if (r) return r;
   hipStream_t s = (hipStream_t)stream;
   const size_t N = (size_t)m * n;
@@ -1392,7 +1404,7 @@ int piv_stage_replace_noalloc(float* u, float* v, uint8_t* flags, int m, int n, 
   if ((size_t)m * n > (size_t)PIV_NOALLOC_MAX_VECTORS)
     return fail(PIV_E_INVALID, "piv_stage_replace_noalloc: %d x %d vectors exceed its fixed workspace of %d vectors",
                 m, n, PIV_NOALLOC_MAX_VECTORS);
-  int r = check_tests(*pass);
+  int r = check_tests(*pass, m, n);
   if (r) return r;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
@@ -1414,7 +1426,7 @@ int piv_stage_spring_replace(float* u, float* v, const uint8_t* flags, const int
   HIP_TRY(hipMalloc((void**)&w.slot, N * 4));
   HIP_TRY(hipMalloc((void**)&w.list, N * 4));
   HIP_TRY(hipMalloc((void**)&w.work, 8 * N * 8));
-  hipError_t e = launch_spring_replace(u, v, flags, count, m, n, w, s);
+  hipError_t e = launch_spring_replace(u, v, flags, count, m, n, 1, w, s);
   hipError_t e2 = hipStreamSynchronize(s);
   (void)hipFree(w.slot);
   (void)hipFree(w.list);

@@ -219,15 +219,20 @@ __global__ __launch_bounds__(256) void median_kernel(const float* u, const float
 // ---------------------------------------------------------------------------
 // Spring replacement (ReplacementGPU.spring_replacement,
 // gpu_validation.py:336-371, 381-415; PIVGPU.replace_outliers,
-// gpu_process.py:788-834).  The reference assembles
-//     x_k - (1/n_k) sum_{l in N4(k), l outlier} x_l
-//         = (1/n_k) sum_{l in N4(k), l valid} f_l
-// (n_k = in-bounds 4-neighbours) and solves it directly.  Scaling row k by
-// n_k gives a symmetric positive-definite graph Laplacian with Dirichlet
-// boundary (whenever some vector is valid), solved here by Jacobi-
-// preconditioned CG in FP64 for u and v together, in one workgroup.  When
-// every vector is an outlier the system is singular; the reference's NaN ->
-// reset path then leaves the field unchanged, and so does this kernel.
+// gpu_process.py:788-834).  The reference links each outlier k to the four
+// vectors N4(k) that lie s = replacing_size rows or columns away
+// (:279-283) and assembles
+//     x_k - (1/p_k) sum_{l in N4(k), l outlier} x_l = rhs_k,
+// rhs_k the NaN-mean over the in-bounds N4(k) with the outliers set to 0,
+// and 1/p_k by position (:365-371): p_k = 2 in the four s x s corner
+// blocks, 3 in the rest of the s-wide border, 4 inside -- the in-bounds
+// link count whenever the field is at least 2 s wide, at least that count
+// otherwise.  Scaling row k by p_k gives a symmetric matrix, diagonally
+// dominant, positive definite whenever each linked outlier cluster touches a
+// valid vector or the border, solved here by Jacobi-preconditioned CG in
+// FP64 for u and v together, in one workgroup.  When every vector is an
+// outlier the system is singular; the reference's NaN -> reset path then
+// leaves the field unchanged, and so does this kernel.
 // ---------------------------------------------------------------------------
 constexpr int kReplaceThreads = 1024;
 constexpr int kSpringLds = 1024;  // outlier count handled by the LDS-resident solver
@@ -280,8 +285,15 @@ struct FlagOutlier {
   __device__ bool operator()(int l) const { return flags[l] != 0; }
 };
 
-// Right-hand side of outlier row k scaled by n_k (the in-bounds
-// 4-neighbour count): (n_k / c_k) times the sum of the neighbours that are
+// Diagonal p_k of the scaled spring row at (i, j) for link distance s
+// (gpu_validation.py:365-371).
+__device__ __forceinline__ int spring_diag(int i, int j, int m, int n, int s) {
+  const bool bi = i < s || i >= m - s, bj = j < s || j >= n - s;
+  return (bi && bj) ? 2 : ((bi || bj) ? 3 : 4);
+}
+
+// Right-hand side of outlier row k scaled by p_k (spring_diag):
+// (p_k / c_k) times the sum of the neighbours that are
 // not outliers and not NaN, c_k counting the in-bounds neighbours that are
 // outliers or not NaN -- the reference's NaN-mean over the neighbours with
 // the outliers set to 0 (gpu_validation.py:342, 381-394).  c_k = 0 gives a
@@ -290,9 +302,9 @@ struct FlagOutlier {
 // vectors, gpu_process.py:829-832): its right-hand side is 0 here and
 // nan_isolated() tells the final write to skip it.
 template <typename R, typename Out>
-__device__ __forceinline__ void spring_rhs(const R* u, const R* v, Out is_out, int m, int n,
+__device__ __forceinline__ void spring_rhs(const R* u, const R* v, Out is_out, int m, int n, int pk,
                                            const int (&ni)[4], const int (&nj)[4], double* bu, double* bv,
-                                           int* nk, bool* iso_u = nullptr, bool* iso_v = nullptr) {
+                                           bool* iso_u = nullptr, bool* iso_v = nullptr) {
   // every load is issued up front (out-of-bounds neighbours read a clamped,
   // unused element), then the contributions are masked
   bool inb[4], out[4];
@@ -325,19 +337,19 @@ __device__ __forceinline__ void spring_rhs(const R* u, const R* v, Out is_out, i
       }
     }
   }
-  *bu = (cu > 0 && cu != cnt) ? su * cnt / cu : su;
-  *bv = (cv > 0 && cv != cnt) ? sv * cnt / cv : sv;
-  *nk = cnt;
+  (void)cnt;
+  *bu = (cu > 0 && cu != pk) ? su * pk / cu : su;
+  *bv = (cv > 0 && cv != pk) ? sv * pk / cv : sv;
   if (iso_u) *iso_u = cu == 0;  // what nan_isolated() finds
   if (iso_v) *iso_v = cv == 0;
 }
 
 template <typename R, typename Out>
-__device__ __forceinline__ void nan_isolated(const R* u, const R* v, Out is_out, int m, int n, int k, bool* ku,
-                                             bool* kv) {
+__device__ __forceinline__ void nan_isolated(const R* u, const R* v, Out is_out, int m, int n, int sz, int k,
+                                             bool* ku, bool* kv) {
   const int i = k / n, j = k % n;
-  const int ni[4] = {i, i, i + 1, i - 1};
-  const int nj[4] = {j - 1, j + 1, j, j};
+  const int ni[4] = {i, i, i + sz, i - sz};
+  const int nj[4] = {j - sz, j + sz, j, j};
   int cu = 0, cv = 0;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
@@ -356,8 +368,8 @@ __device__ __forceinline__ void nan_isolated(const R* u, const R* v, Out is_out,
 }
 
 template <typename R>
-__device__ void spring_cg_global(R* u, R* v, int m, int n, int nv, const ReplaceWork& w,
-                                 double (*red)[4]) {
+__device__ void spring_cg_global(R* u, R* v, int m, int n, int sz, int nv, const ReplaceWork& w,
+                                 double (*red)[4], const int32_t* cls) {
   constexpr int T = 256;
   const int tid = threadIdx.x;
   double* xu = w.work;
@@ -371,18 +383,19 @@ __device__ void spring_cg_global(R* u, R* v, int m, int n, int nv, const Replace
   auto nbrs = [&](int s, int (&ni)[4], int (&nj)[4]) {
     const int k = w.list[s];
     const int i = k / n, j = k % n;
-    ni[0] = i; nj[0] = j - 1;
-    ni[1] = i; nj[1] = j + 1;
-    ni[2] = i + 1; nj[2] = j;
-    ni[3] = i - 1; nj[3] = j;
+    ni[0] = i; nj[0] = j - sz;
+    ni[1] = i; nj[1] = j + sz;
+    ni[2] = i + sz; nj[2] = j;
+    ni[3] = i - sz; nj[3] = j;
   };
   double acc[4] = {0, 0, 0, 0};
   for (int s = tid; s < nv; s += T) {
     int ni[4], nj[4];
     nbrs(s, ni, nj);
     double bu = 0.0, bv = 0.0;
-    int nk = 0;
-    spring_rhs(u, v, SlotOutlier{w.slot}, m, n, ni, nj, &bu, &bv, &nk);
+    const int kk = w.list[s];
+    const int nk = spring_diag(kk / n, kk % n, m, n, sz);
+    spring_rhs(u, v, SlotOutlier{w.slot}, m, n, nk, ni, nj, &bu, &bv);
     xu[s] = 0.0;
     xv[s] = 0.0;
     ru[s] = bu;
@@ -405,12 +418,12 @@ __device__ void spring_cg_global(R* u, R* v, int m, int n, int nv, const Replace
     for (int s = tid; s < nv; s += T) {
       int ni[4], nj[4];
       nbrs(s, ni, nj);
-      int nk = 0;
+      const int kk = w.list[s];
+      const int nk = spring_diag(kk / n, kk % n, m, n, sz);
       double au = 0.0, av = 0.0;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         if (ni[e] < 0 || ni[e] >= m || nj[e] < 0 || nj[e] >= n) continue;
-        ++nk;
         const int sl = w.slot[ni[e] * n + nj[e]];
         if (sl >= 0) {
           au += pu[sl];
@@ -430,8 +443,7 @@ __device__ void spring_cg_global(R* u, R* v, int m, int n, int nv, const Replace
     double nr[4] = {0, 0, 0, 0};
     for (int s = tid; s < nv; s += T) {
       const int k = w.list[s];
-      const int i = k / n, j = k % n;
-      const int nk = (i > 0) + (i < m - 1) + (j > 0) + (j < n - 1);
+      const int nk = spring_diag(k / n, k % n, m, n, sz);
       xu[s] += alpha_u * pu[s];
       xv[s] += alpha_v * pv[s];
       const double r1 = ru[s] - alpha_u * qu[s], r2 = rv[s] - alpha_v * qv[s];
@@ -451,8 +463,7 @@ __device__ void spring_cg_global(R* u, R* v, int m, int n, int nv, const Replace
     done_v = done_v || nr[3] <= tol2_v;
     for (int s = tid; s < nv; s += T) {
       const int k = w.list[s];
-      const int i = k / n, j = k % n;
-      const int nk = (i > 0) + (i < m - 1) + (j > 0) + (j < n - 1);
+      const int nk = spring_diag(k / n, k % n, m, n, sz);
       pu[s] = ru[s] / nk + beta_u * pu[s];
       pv[s] = rv[s] / nk + beta_v * pv[s];
     }
@@ -462,7 +473,8 @@ __device__ void spring_cg_global(R* u, R* v, int m, int n, int nv, const Replace
   for (int s = tid; s < nv; s += T) {
     const int k = w.list[s];
     bool ku, kv;
-    nan_isolated(u, v, SlotOutlier{w.slot}, m, n, k, &ku, &kv);
+    nan_isolated(u, v, SlotOutlier{w.slot}, m, n, sz, k, &ku, &kv);
+    if (cls && cls[(k / n) % sz * sz + (k % n) % sz]) continue;  // singular block: keeps its values
     if (!ku) u[k] = (R)xu[s];
     if (!kv) v[k] = (R)xv[s];
   }
@@ -488,6 +500,7 @@ struct SpringLds {
   uint8_t iso[kSpringLds];  // bit 0: u keeps its value, bit 1: v does
   int32_t list[kSpringLds];
   int32_t scan[kSpringLdsThreads / 64];
+  int32_t cls[kMaxNeighbourhoodSize * kMaxNeighbourhoodSize];  // singular residue classes
   double red[4][4];
 };
 
@@ -723,7 +736,7 @@ __device__ void spring_cg_wave(SpringLds& L, int nv) {
 
 template <typename R>
 __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* v, const uint8_t* flags,
-                                                                       const int32_t* count, int m, int n,
+                                                                       const int32_t* count, int m, int n, int sz,
                                                                        ReplaceWork w) {
   constexpr int T = kSpringLdsThreads;
   const int N = m * n;
@@ -831,8 +844,30 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* 
     }
   }
   __syncthreads();
+  // With links sz > 1 apart the system splits by residue class (row mod sz,
+  // column mod sz).  A class whose every vector is an outlier, with no
+  // member in rows [m - sz, sz) or columns [n - sz, sz) (where the
+  // position-based diagonal exceeds the link count), is a singular block:
+  // its vectors are unresolved and keep their values.
+  const int32_t* olist = in_lds ? L.list : w.list;
+  if (sz > 1) {
+    for (int c = tid; c < sz * sz; c += T) L.cls[c] = 0;
+    __syncthreads();
+    for (int q = tid; q < total; q += T) {
+      const int k = olist[q];
+      atomicAdd(&L.cls[(k / n) % sz * sz + (k % n) % sz], 1);
+    }
+    __syncthreads();
+    for (int c = tid; c < sz * sz; c += T) {
+      const int a = c / sz, b = c % sz;
+      const int members = ((m - a + sz - 1) / sz) * ((n - b + sz - 1) / sz);  // sz <= m, n: a < m, b < n
+      L.cls[c] = (a < m - sz && b < n - sz && L.cls[c] == members) ? 1 : 0;
+    }
+    __syncthreads();
+  }
+  const int32_t* cls = sz > 1 ? L.cls : nullptr;
   if (!in_lds) {
-    spring_cg_global(u, v, m, n, total, w, L.red);
+    spring_cg_global(u, v, m, n, sz, total, w, L.red, cls);
     return;
   }
   const int nvl = total;
@@ -857,10 +892,10 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* 
   for (int s = tid; s < nvl; s += T) {
     const int k = L.list[s];
     const int i = k / n, j = k % n;
-    const int ni[4] = {i, i, i + 1, i - 1};
-    const int nj[4] = {j - 1, j + 1, j, j};
+    const int ni[4] = {i, i, i + sz, i - sz};
+    const int nj[4] = {j - sz, j + sz, j, j};
     double bu = 0.0, bv = 0.0;
-    int cnt = 0;
+    const int pk = spring_diag(i, j, m, n, sz);
     bool nout[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {  // flags of the (clamped) neighbours, loaded together
@@ -875,11 +910,12 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* 
       L.nb[s][e] = code;
     }
     bool iu, iv;
-    spring_rhs(u, v, is_out, m, n, ni, nj, &bu, &bv, &cnt, &iu, &iv);
-    L.iso[s] = (uint8_t)((iu ? 1 : 0) | (iv ? 2 : 0));
-    L.nk[s] = (uint8_t)cnt;
-    L.xu[s] = bu / cnt;
-    L.xv[s] = bv / cnt;
+    spring_rhs(u, v, is_out, m, n, pk, ni, nj, &bu, &bv, &iu, &iv);
+    const bool sing = cls && cls[i % sz * sz + j % sz];
+    L.iso[s] = (uint8_t)((iu || sing ? 1 : 0) | (iv || sing ? 2 : 0));
+    L.nk[s] = (uint8_t)pk;
+    L.xu[s] = bu / pk;
+    L.xv[s] = bv / pk;
   }
   __syncthreads();
   if (nvl <= 256) {
@@ -1016,7 +1052,83 @@ __device__ __forceinline__ void stats8(R (&s)[8], int cnt, R* med, R* mad, R* me
   }
 }
 
+// Neighbourhoods wider than 3x3 (half-width 2..kMaxNeighbourhoodSize).  A
+// neighbour is addressed by its index e in the (2 size + 1)^2 block, row
+// major; the centre, positions outside the field, skipped (flagged)
+// positions and NaN values read as NaN and take no part, like the
+// reference's NaN padding and NaN centre (gpu_validation.py:194-205,
+// 393-398).  Order statistics come from rank counting -- the value whose
+// rank range [#less, #less + #equal) holds the wanted rank -- so no
+// per-thread array is needed.
 template <typename R>
+struct Nbhd {
+  const R* f;
+  const uint8_t* skip;
+  int m, n, i, j, size, K;
+  __device__ R at(int e) const {
+    const int w = 2 * size + 1;
+    const int di = e / w - size, dj = e % w - size;
+    const int ii = i + di, jj = j + dj;
+    if ((di == 0 && dj == 0) || ii < 0 || ii >= m || jj < 0 || jj >= n) return vnan<R>();
+    const int idx = ii * n + jj;
+    if (skip && skip[idx]) return vnan<R>();
+    return f[idx];
+  }
+};
+
+// Values of ranks r0 <= r1 among the non-NaN values of get(0..K-1); DEV:
+// rank |x - centre| instead of x.
+template <bool DEV, typename R, typename G>
+__device__ void two_ranks(const G& g, R centre, int r0, int r1, R* y0, R* y1) {
+  R a0 = R(0), a1 = R(0);
+  for (int e = 0; e < g.K; ++e) {
+    R x = g.at(e);
+    if (isnan(x)) continue;
+    if (DEV) x = vabs(x - centre);
+    int less = 0, eq = 0;
+    for (int q = 0; q < g.K; ++q) {
+      R y = g.at(q);
+      if (DEV) y = vabs(y - centre);  // NaN stays NaN and compares false
+      less += (y < x) ? 1 : 0;
+      eq += (y == x) ? 1 : 0;
+    }
+    if (less <= r0 && r0 < less + eq) a0 = x;
+    if (less <= r1 && r1 < less + eq) a1 = x;
+  }
+  *y0 = a0;
+  *y1 = a1;
+}
+
+// nanmedian (the mean of the two middle values for an even count, NaN for
+// none), MAD (median of |x - median|) and nanmean of the neighbourhood.
+template <typename R, typename G>
+__device__ void wide_stats(const G& g, R* med, R* mad, R* mean) {
+  R sum = R(0);
+  int cnt = 0;
+  for (int e = 0; e < g.K; ++e) {
+    const R x = g.at(e);
+    if (!isnan(x)) {
+      sum += x;
+      ++cnt;
+    }
+  }
+  if (cnt == 0) {
+    *med = *mean = vnan<R>();
+    if (mad) *mad = vnan<R>();
+    return;
+  }
+  *mean = sum / (R)cnt;
+  R y0, y1;
+  two_ranks<false>(g, R(0), (cnt - 1) / 2, cnt / 2, &y0, &y1);
+  const R md = (cnt & 1) ? y0 : R(0.5) * (y0 + y1);
+  *med = md;
+  if (mad) {
+    two_ranks<true>(g, md, (cnt - 1) / 2, cnt / 2, &y0, &y1);
+    *mad = (cnt & 1) ? y0 : R(0.5) * (y0 + y1);
+  }
+}
+
+template <typename R, bool WIDE>
 __global__ __launch_bounds__(256) void validate_kernel(ValidateArgsT<R> a) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   bool flag = false;
@@ -1027,12 +1139,20 @@ __global__ __launch_bounds__(256) void validate_kernel(ValidateArgsT<R> a) {
       if (a.s2n && !isnan(a.s2n_tol)) flag = flag || (a.s2n[k] < a.s2n_tol);
       const bool nb = !isnan(a.median_tol) || !isnan(a.mad_tol) || !isnan(a.mean_tol);
       if (nb) {
-        R s[8];
         R medu, madu, meanu, medv, madv, meanv;
-        int c = gather8(a.u, a.m, a.n, i, j, nullptr, s);
-        stats8(s, c, &medu, isnan(a.mad_tol) ? nullptr : &madu, &meanu);
-        c = gather8(a.v, a.m, a.n, i, j, nullptr, s);
-        stats8(s, c, &medv, isnan(a.mad_tol) ? nullptr : &madv, &meanv);
+        if constexpr (WIDE) {
+          const int K = (2 * a.size + 1) * (2 * a.size + 1);
+          wide_stats(Nbhd<R>{a.u, nullptr, a.m, a.n, i, j, a.size, K}, &medu, isnan(a.mad_tol) ? nullptr : &madu,
+                     &meanu);
+          wide_stats(Nbhd<R>{a.v, nullptr, a.m, a.n, i, j, a.size, K}, &medv, isnan(a.mad_tol) ? nullptr : &madv,
+                     &meanv);
+        } else {
+          R s[8];
+          int c = gather8(a.u, a.m, a.n, i, j, nullptr, s);
+          stats8(s, c, &medu, isnan(a.mad_tol) ? nullptr : &madu, &meanu);
+          c = gather8(a.v, a.m, a.n, i, j, nullptr, s);
+          stats8(s, c, &medv, isnan(a.mad_tol) ? nullptr : &madv, &meanv);
+        }
         if (!isnan(a.median_tol))
           flag = flag || (vabs(uk - medu) > a.median_tol) || (vabs(vk - medv) > a.median_tol);
         if (!isnan(a.mad_tol))
@@ -1096,21 +1216,30 @@ __global__ __launch_bounds__(1024) void field_stats_kernel(const R* u, const R* 
   }
 }
 
-template <typename R>
+template <typename R, bool WIDE>
 __global__ __launch_bounds__(256) void neighbour_replace_kernel(const R* u_in, const R* v_in, R* u, R* v,
-                                                                const uint8_t* flags, int m, int n, int method,
-                                                                int exclude) {
+                                                                const uint8_t* flags, int m, int n, int size,
+                                                                int method, int exclude) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m * n || !flags[k]) return;
   const int i = k / n, j = k % n;
-  R s[8];
+  const uint8_t* skip = exclude ? flags : nullptr;
   R med, mean;
-  int c = gather8(u_in, m, n, i, j, exclude ? flags : nullptr, s);
-  stats8<R>(s, c, &med, nullptr, &mean);
-  u[k] = method == 1 ? med : mean;
-  c = gather8(v_in, m, n, i, j, exclude ? flags : nullptr, s);
-  stats8<R>(s, c, &med, nullptr, &mean);
-  v[k] = method == 1 ? med : mean;
+  if constexpr (WIDE) {
+    const int K = (2 * size + 1) * (2 * size + 1);
+    wide_stats(Nbhd<R>{u_in, skip, m, n, i, j, size, K}, &med, (R*)nullptr, &mean);
+    u[k] = method == 1 ? med : mean;
+    wide_stats(Nbhd<R>{v_in, skip, m, n, i, j, size, K}, &med, (R*)nullptr, &mean);
+    v[k] = method == 1 ? med : mean;
+  } else {
+    R s[8];
+    int c = gather8(u_in, m, n, i, j, skip, s);
+    stats8<R>(s, c, &med, nullptr, &mean);
+    u[k] = method == 1 ? med : mean;
+    c = gather8(v_in, m, n, i, j, skip, s);
+    stats8<R>(s, c, &med, nullptr, &mean);
+    v[k] = method == 1 ? med : mean;
+  }
 }
 
 template <typename R>
@@ -1180,7 +1309,11 @@ hipError_t launch_median_validate(const float* u, const float* v, int m, int n, 
 template <typename R>
 hipError_t launch_validate(const ValidateArgsT<R>& a, hipStream_t stream) {
   const int total = a.m * a.n;
-  hipLaunchKernelGGL(validate_kernel<R>, dim3((total + 255) / 256), dim3(256), 0, stream, a);
+  if (a.size < 1 || a.size > kMaxNeighbourhoodSize) return hipErrorInvalidValue;
+  if (a.size == 1)
+    hipLaunchKernelGGL((validate_kernel<R, false>), dim3((total + 255) / 256), dim3(256), 0, stream, a);
+  else
+    hipLaunchKernelGGL((validate_kernel<R, true>), dim3((total + 255) / 256), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
@@ -1192,10 +1325,15 @@ hipError_t launch_field_stats(const R* u, const R* v, int count, double* stats, 
 
 template <typename R>
 hipError_t launch_neighbour_replace(const R* u_in, const R* v_in, R* u, R* v, const uint8_t* flags, int m, int n,
-                                    int method, int exclude, hipStream_t stream) {
+                                    int size, int method, int exclude, hipStream_t stream) {
   const int total = m * n;
-  hipLaunchKernelGGL(neighbour_replace_kernel<R>, dim3((total + 255) / 256), dim3(256), 0, stream, u_in, v_in, u, v,
-                     flags, m, n, method, exclude);
+  if (size < 1 || size > kMaxNeighbourhoodSize) return hipErrorInvalidValue;
+  if (size == 1)
+    hipLaunchKernelGGL((neighbour_replace_kernel<R, false>), dim3((total + 255) / 256), dim3(256), 0, stream, u_in,
+                       v_in, u, v, flags, m, n, size, method, exclude);
+  else
+    hipLaunchKernelGGL((neighbour_replace_kernel<R, true>), dim3((total + 255) / 256), dim3(256), 0, stream, u_in,
+                       v_in, u, v, flags, m, n, size, method, exclude);
   return hipGetLastError();
 }
 
@@ -1208,11 +1346,13 @@ hipError_t launch_reset_unresolved(R* u, R* v, const R* u0, const R* v0, const u
 }
 
 template <typename R>
-hipError_t launch_spring_replace(R* u, R* v, const uint8_t* flags, const int32_t* count, int m, int n,
+hipError_t launch_spring_replace(R* u, R* v, const uint8_t* flags, const int32_t* count, int m, int n, int size,
                                  const ReplaceWork& w, hipStream_t stream) {
+  if (size < 1 || size > kMaxNeighbourhoodSize || size > m || size > n) return hipErrorInvalidValue;
   // the kernel reads the outlier count on the device (graph-capturable, no
   // host sync) and returns at once when there is nothing to solve
-  hipLaunchKernelGGL(spring_lds_kernel<R>, dim3(1), dim3(kSpringLdsThreads), 0, stream, u, v, flags, count, m, n, w);
+  hipLaunchKernelGGL(spring_lds_kernel<R>, dim3(1), dim3(kSpringLdsThreads), 0, stream, u, v, flags, count, m, n,
+                     size, w);
   return hipGetLastError();
 }
 
@@ -1311,11 +1451,11 @@ hipError_t launch_apply_field_mask(R* u, R* v, R* s2n, const uint8_t* fmask, int
   template hipError_t launch_validate<R>(const ValidateArgsT<R>&, hipStream_t);                                  \
   template hipError_t launch_field_stats<R>(const R*, const R*, int, double*, hipStream_t);                      \
   template hipError_t launch_neighbour_replace<R>(const R*, const R*, R*, R*, const uint8_t*, int, int, int, int, \
-                                                  hipStream_t);                                                  \
+                                                  int, hipStream_t);                                             \
   template hipError_t launch_reset_unresolved<R>(R*, R*, const R*, const R*, const uint8_t*, uint8_t*, int,       \
                                                  hipStream_t);                                                   \
-  template hipError_t launch_spring_replace<R>(R*, R*, const uint8_t*, const int32_t*, int, int, const ReplaceWork&, \
-                                               hipStream_t);                                                     \
+  template hipError_t launch_spring_replace<R>(R*, R*, const uint8_t*, const int32_t*, int, int, int,              \
+                                               const ReplaceWork&, hipStream_t);                                 \
   template hipError_t launch_scale<R>(R*, R*, int, R, hipStream_t);                                              \
   template hipError_t launch_copy<R>(R*, const R*, int, hipStream_t);                                            \
   template hipError_t launch_apply_field_mask<R>(R*, R*, R*, const uint8_t*, int, hipStream_t);

@@ -154,9 +154,16 @@ hipError_t launch_grid_interp(const GridInterpArgsT<R>& a, hipStream_t stream);
 hipError_t launch_median_validate(const float* u, const float* v, int m, int n, float tol, uint8_t* flags,
                                   int32_t* count, hipStream_t stream);
 
+// Largest neighbourhood half-width of the validation tests and of
+// replacement: (2 * 8 + 1)^2 - 1 = 288 neighbours, whose statistics are found
+// by rank counting (O(K^2) loads per vector for K neighbours); the spring
+// system of half-width s splits into s^2 residue classes.
+constexpr int kMaxNeighbourhoodSize = 8;
+
 // Every validation test of ValidationGPU (gpu_validation.py:84-164), OR-ed:
 // s2n < s2n_tol; |f - med| > median_tol; |f - med| > mad_tol * mad;
-// |f - mean| > mean_tol over the 3x3 neighbourhood (centre excluded); and
+// |f - mean| > mean_tol over the (2 size + 1)^2 neighbourhood (centre
+// excluded, gpu_validation.py:73-76, 194-205); and
 // |f - mean(f)| > rms_tol * std(f) with the field statistics in rms
 // (mean_u, std_u, mean_v, std_v, FP64).  A NaN tolerance turns a test off.
 // only: validate where only[k] != 0 (others 0); extra: OR-ed in.
@@ -165,6 +172,7 @@ struct ValidateArgsT {
   const R* u;
   const R* v;
   int m, n;
+  int size;  // neighbourhood half-width, 1..kMaxNeighbourhoodSize
   const R* s2n;
   R s2n_tol, median_tol, mad_tol, mean_tol, rms_tol;
   const double* rms;
@@ -182,26 +190,29 @@ template <typename R>
 hipError_t launch_field_stats(const R* u, const R* v, int count, double* stats, hipStream_t stream);
 
 // Median (method 1) or mean (2) replacement of the flagged vectors from the
-// 3x3 neighbourhood of (u_in, v_in) (ReplacementGPU, gpu_validation.py:
-// 373-398); exclude: flagged neighbours left out (first iteration).  Writes
-// u, v at flagged positions only; NaN where no neighbour is usable.
+// (2 size + 1)^2 neighbourhood of (u_in, v_in) (ReplacementGPU,
+// gpu_validation.py:272-290, 373-398); exclude: flagged neighbours left out
+// (first iteration).  Writes u, v at flagged positions only; NaN where no
+// neighbour is usable.
 template <typename R>
 hipError_t launch_neighbour_replace(const R* u_in, const R* v_in, R* u, R* v, const uint8_t* flags, int m, int n,
-                                    int method, int exclude, hipStream_t stream);
+                                    int size, int method, int exclude, hipStream_t stream);
 // Flagged vectors that came out NaN go back to (u0, v0) (ReplacementGPU.reset,
 // gpu_validation.py:401-415); nan_flags[k] = 1 for them, else 0.
 template <typename R>
 hipError_t launch_reset_unresolved(R* u, R* v, const R* u0, const R* v0, const uint8_t* flags, uint8_t* nan_flags,
                                    int count, hipStream_t stream);
 
-// Spring (Laplace) replacement of flagged vectors, solved on the device.
+// Spring (Laplace) replacement of flagged vectors, solved on the device;
+// each outlier is linked to the four vectors size rows / columns away
+// (gpu_validation.py:279-283).
 struct ReplaceWork {
   int32_t* slot;      // m*n
   int32_t* list;      // m*n
   double* work;       // 8 * m*n doubles
 };
 template <typename R>
-hipError_t launch_spring_replace(R* u, R* v, const uint8_t* flags, const int32_t* count, int m, int n,
+hipError_t launch_spring_replace(R* u, R* v, const uint8_t* flags, const int32_t* count, int m, int n, int size,
                                  const ReplaceWork& w, hipStream_t stream);
 
 // Unweighted DCT-PLS smoothing (GCV-chosen s) of u and v, in place.

@@ -43,6 +43,9 @@ EXPORTED_SYMBOLS = (
     "piv_plan_last_s2n", "piv_stage_replace_noalloc",
 )
 
+# largest validation_size / replacing_size the kernels take: (2 * 8 + 1)^2 - 1
+# = 288 neighbours, 64 spring residue classes
+MAX_NEIGHBOURHOOD_SIZE = 8
 SYNTH_MAX_VORTICES = 8
 FIELD_UNIFORM, FIELD_SINE, FIELD_VORTICES = 0, 1, 2
 
@@ -73,6 +76,8 @@ class PassDesc(ctypes.Structure):
         ("replacing_method", ctypes.c_int32),
         ("revalidate", ctypes.c_int32),
         ("window_size", ctypes.c_int32),
+        ("validation_size", ctypes.c_int32),
+        ("replacing_size", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
     ]
 

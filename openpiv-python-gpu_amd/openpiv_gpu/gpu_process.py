@@ -253,10 +253,10 @@ class PIVGPU:
         self.predictor_pass = int(predictor_pass)  # first pass fed by the network (-1: the last)
 
         for i in range(n):
-            if self.validation_size[i] != 1:
-                raise NotImplementedError("validation_size: only 1 (3x3 kernels) is implemented")
-            if self.replacing_size[i] != 1:
-                raise NotImplementedError("replacing_size: only 1 is implemented")
+            if self.validation_size[i] > _native.MAX_NEIGHBOURHOOD_SIZE:
+                raise NotImplementedError(f"validation_size above {_native.MAX_NEIGHBOURHOOD_SIZE}")
+            if (self.replacing_size[i] or 1) > _native.MAX_NEIGHBOURHOOD_SIZE:
+                raise NotImplementedError(f"replacing_size above {_native.MAX_NEIGHBOURHOOD_SIZE}")
 
         self.piv_fields = None
         self.init_val_locations = self.val_locations = None
@@ -330,6 +330,12 @@ class PIVGPU:
             p.replacing_method = _native.REPLACING_CODES[self.replacing_method[i]]
             p.revalidate = int(bool(self.is_revalidated[i]))
             p.num_replacing_iters = int(self.num_replacing_iters[i])
+            # gpu_validation.py:74, 273: the kernel half-width may not exceed
+            # the field
+            rsize = self.replacing_size[i] if self.replacing_size[i] is not None else 1
+            for name, size in (("validation_size", self.validation_size[i]), ("replacing_size", rsize)):
+                assert all(size <= item for item in f.field_shape), "size cannot exceed field_shape."
+                setattr(p, name, int(size))
             if p.num_replacing_iters > _native.MAX_REPLACING_ITERS:
                 raise NotImplementedError(f"num_replacing_iters above {_native.MAX_REPLACING_ITERS}")
             # gpu_process.py:841 tests the whole tuple, so every non-final pass

@@ -554,7 +554,7 @@ def replace_outliers(u, v, val, method="spring", n_iters=1, revalidate=False, va
         if not val.any():
             break
         if method == "spring":
-            u, v = spring_replacement(u, v, val)
+            u, v = spring_replacement(u, v, val, size)
         else:
             u, v = neighbour_replacement(u, v, val, method, exclude, size)
         last_val = val
@@ -578,18 +578,21 @@ def replace_outliers(u, v, val, method="spring", n_iters=1, revalidate=False, va
 # (gpu_process.py:788-834).
 # ---------------------------------------------------------------------------
 
-def spring_system(val, f_list):
+def spring_system(val, f_list, size=1):
     """Builds the reference's link matrix and right-hand sides.
 
     Row k (outlier at (i, j)): ``x_k + c_k * sum_{l in N4(k) & O} x_l = rhs``
-    with ``c_k`` = -1/4 interior, -1/3 edge, -1/2 corner by *position*
-    (gpu_validation.py:365-371) and ``rhs`` the NaN-mean over the in-bounds
-    4-neighbours with outliers set to 0 (:342, 381-394).
+    where N4(k) are the four vectors ``size`` rows or columns away
+    (gpu_validation.py:279-283), ``c_k`` = -1/4 interior, -1/3 in the
+    ``size``-wide border, -1/2 in its four size x size corners, by *position*
+    (:365-371), and ``rhs`` the NaN-mean over the in-bounds N4(k) with the
+    outliers set to 0 (:342, 381-394).
     """
     m, n = val.shape
+    s = size
     coef = np.full((m, n), -1.0 / 3.0)
-    coef[1:-1, 1:-1] = -0.25
-    coef[0, 0] = coef[0, -1] = coef[-1, 0] = coef[-1, -1] = -0.5
+    coef[s:-s, s:-s] = -0.25
+    coef[:s, :s] = coef[:s, -s:] = coef[-s:, :s] = coef[-s:, -s:] = -0.5
     iv, jv = np.nonzero(val)
     nv = iv.size
     slot = -np.ones((m, n), np.int64)
@@ -597,7 +600,7 @@ def spring_system(val, f_list):
     rows, cols, data = [], [], []
     for r in range(nv):
         rows.append(r); cols.append(r); data.append(1.0)
-        for di, dj in ((0, -1), (0, 1), (1, 0), (-1, 0)):
+        for di, dj in ((0, -s), (0, s), (s, 0), (-s, 0)):
             a, b = iv[r] + di, jv[r] + dj
             if 0 <= a < m and 0 <= b < n and slot[a, b] >= 0:
                 rows.append(r); cols.append(slot[a, b]); data.append(coef[iv[r], jv[r]])
@@ -605,8 +608,8 @@ def spring_system(val, f_list):
     rhs = []
     for f in f_list:
         g = np.where(val, 0.0, np.asarray(f, np.float64))
-        gp = np.pad(g, 1, constant_values=np.nan)
-        nb = np.stack([gp[1 + iv, jv], gp[1 + iv, 2 + jv], gp[2 + iv, 1 + jv], gp[iv, 1 + jv]], 0)
+        gp = np.pad(g, s, constant_values=np.nan)
+        nb = np.stack([gp[s + iv, jv], gp[s + iv, 2 * s + jv], gp[2 * s + iv, s + jv], gp[iv, s + jv]], 0)
         import warnings
         with warnings.catch_warnings():
             warnings.simplefilter("ignore", RuntimeWarning)
@@ -614,11 +617,37 @@ def spring_system(val, f_list):
     return link, rhs, (iv, jv)
 
 
-def spring_replacement(u, v, val):
+def spring_singular(val, size=1):
+    """Outliers whose spring rows form a singular block.  Links ``size``
+    apart join only vectors of one residue class (row mod size, column mod
+    size), so the system splits into size^2 independent blocks; a block is
+    singular when its whole class is flagged and no member sits where the
+    position-based coefficient exceeds its in-bounds link count (rows in
+    [m - size, size) or columns in [n - size, size), possible only in fields
+    narrower than 2 size).  Such vectors have no solution; the reference's
+    unresolved-vector reset (gpu_process.py:829-832) keeps their values."""
+    val = np.asarray(val, bool)
+    m, n = val.shape
+    s = size
+    out = np.zeros_like(val)
+    for a in range(min(s, m)):
+        for b in range(min(s, n)):
+            if a >= m - s or b >= n - s:
+                continue
+            cls = val[a::s, b::s]
+            if cls.all():
+                out[a::s, b::s] = True
+    return out
+
+
+def spring_replacement(u, v, val, size=1):
     """Solves the spring system for both fields (direct sparse solve) and
-    writes the results at the outlier positions.  A singular system -- only
-    possible when every vector is an outlier -- leaves NaN, which the caller
-    resets to the original values (gpu_process.py:829-832)."""
+    writes the results at the outlier positions.  A singular system leaves
+    NaN, which the caller resets to the original values
+    (gpu_process.py:829-832); with size 1 only a field of outliers is
+    singular, with wider links also an outlier cluster that covers a whole
+    residue class of the field (its rows and columns mod size), which
+    :func:`spring_singular` detects."""
     u = np.array(u, copy=True)
     v = np.array(v, copy=True)
     n_vals = int(np.count_nonzero(val))
@@ -626,7 +655,12 @@ def spring_replacement(u, v, val):
         return u, v
     if n_vals == val.size:
         return u, v  # singular: unresolved everywhere -> reset to originals
-    link, rhs, (iv, jv) = spring_system(val, (u, v))
+    sing = spring_singular(val, size)
+    if sing.any():
+        val = val & ~sing
+        if not val.any():
+            return u, v
+    link, rhs, (iv, jv) = spring_system(val, (u, v), size)
     for f, b in zip((u, v), rhs):
         x = spla.spsolve(link.tocsc(), b)
         x = np.where(np.isfinite(x), x, f[iv, jv])
@@ -844,6 +878,8 @@ class OracleSettings:
     rms_tol: Optional[float] = None
     replacing_method: str = "spring"
     revalidate: bool = False
+    validation_size: int = 1
+    replacing_size: int = 1
     shrink_ratio: float = 1.0
     dt: float = 1.0
     scaling_par: float = 1.0
@@ -907,11 +943,11 @@ def run_pass(frame_a, frame_b, settings: OracleSettings, geom: Geometry, prev_ge
     def validate_fn(a, b, only=None):                                        # :758-769
         o = keep if only is None else (only if keep is None else only & keep)
         return validate(a, b, s2n, settings.s2n_tol, settings.median_tol, settings.mad_tol, settings.mean_tol,
-                        settings.rms_tol, only=o)
+                        settings.rms_tol, size=settings.validation_size, only=o)
 
     val = validate_fn(u, v)
     u_rep, v_rep, _ = replace_outliers(u, v, val, settings.replacing_method, settings.num_replacing_iters,
-                                       settings.revalidate, validate_fn)     # :788-834
+                                       settings.revalidate, validate_fn, settings.replacing_size)  # :788-834
     outliers = None
     if is_last:
         outliers = validate_fn(u_rep, v_rep)                                 # :783-784

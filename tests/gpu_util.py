@@ -49,7 +49,7 @@ def pass_desc(geom: O.Geometry, fsize: int, normalize=True, subpixel=0, deformin
 
 
 def tests_desc(median_tol=2.0, s2n_tol=None, mad_tol=None, mean_tol=None, rms_tol=None, method="spring", n_iters=1,
-               revalidate=False, s2n_method="peak2peak", s2n_size=2):
+               revalidate=False, s2n_method="peak2peak", s2n_size=2, validation_size=1, replacing_size=1):
     """A pass descriptor carrying only validation / replacement settings."""
     p = _native.PassDesc()
     nan = float("nan")
@@ -61,6 +61,8 @@ def tests_desc(median_tol=2.0, s2n_tol=None, mad_tol=None, mean_tol=None, rms_to
     p.replacing_method = _native.REPLACING_CODES[method]
     p.num_replacing_iters = n_iters
     p.revalidate = int(revalidate)
+    p.validation_size = validation_size
+    p.replacing_size = replacing_size
     return p
 
 
@@ -244,7 +246,7 @@ def threshold_band(u, v, s2n, settings, rel=1e-5, s2n_rel=1e-4, only=None):
     s = settings
     if s.median_tol is not None or s.mad_tol is not None or s.mean_tol is not None:
         for f in (u, v):
-            med, mad, mean = O.neighbour_stats(f)
+            med, mad, mean = O.neighbour_stats(f, getattr(s, "validation_size", 1))
             if s.median_tol is not None:
                 near |= close(np.abs(f - med), s.median_tol)
             if s.mad_tol is not None:
@@ -309,33 +311,51 @@ def pass_exclusions(frame_a, frame_b, settings, passes, mode="faithful32", s2n_r
     return out
 
 
-def dilate(mask, steps):
-    """Grow a boolean field by ``steps`` 8-neighbour rings (the reach of a
-    3 x 3 validation / replacement step)."""
+def dilate(mask, steps, size=1):
+    """Grow a boolean field by ``steps`` rings of width ``size`` (the reach
+    of a (2 size + 1)^2 validation / replacement step)."""
     out = np.asarray(mask, bool).copy()
+    w = 2 * size + 1
     for _ in range(max(0, int(steps))):
-        p = np.pad(out, 1)
+        p = np.pad(out, size)
         grown = np.zeros_like(out)
-        for di in (0, 1, 2):
-            for dj in (0, 1, 2):
+        for di in range(w):
+            for dj in range(w):
                 grown |= p[di:di + out.shape[0], dj:dj + out.shape[1]]
         out = grown
     return out
 
 
+def _spring_clusters(flags, size):
+    """Labels of the outlier clusters the spring system couples: links join
+    vectors ``size`` apart in a row or column, so clusters are 4-connected
+    components of each residue sublattice flags[a::size, b::size]."""
+    from scipy import ndimage
+    lab = np.zeros(flags.shape, np.int64)
+    base = 0
+    for a in range(size):
+        for b in range(size):
+            sub, n = ndimage.label(flags[a::size, b::size])
+            lab[a::size, b::size] = np.where(sub > 0, sub + base, 0)
+            base += n
+    return lab
+
+
 def _reach(seed, flags, settings):
     """Vectors whose replaced value can depend on the vectors in ``seed``:
     ``num_replacing_iters`` + 1 rings for the median / mean replacement
-    (each iteration reads a 3 x 3 neighbourhood), and for the spring
-    replacement every outlier cluster (4-connected) next to the seed."""
-    from scipy import ndimage
+    (each iteration reads a (2 size + 1)^2 neighbourhood, revalidation one of
+    validation_size), and for the spring replacement every outlier cluster
+    (_spring_clusters) next to the seed."""
     iters = max(1, int(settings.num_replacing_iters))
-    out = dilate(seed, iters + 1)
+    rs = int(getattr(settings, "replacing_size", 1))
+    size = max(rs, int(getattr(settings, "validation_size", 1)))
+    out = dilate(seed, iters + 1, size)
     if settings.replacing_method == "spring" and flags.any():
-        lab, _ = ndimage.label(flags)
-        hit = np.unique(lab[dilate(seed, 1) & flags])
+        lab = _spring_clusters(flags, rs)
+        hit = np.unique(lab[dilate(seed, 1, rs) & flags])
         out |= np.isin(lab, hit[hit > 0])
-        out = dilate(out, 1)
+        out = dilate(out, 1, size)
     return out
 
 
@@ -362,7 +382,7 @@ def e2e_compare(u, v, init_flags, ref, excl, settings, tol=1e-3, final_flags=Non
     got_init = np.asarray(init_flags, bool)
     tie_fit = last["tie"] | last["fit"]
     flag_diff = got_init != ref_init
-    allowed = last["band"] | dilate(tie_fit, 1)
+    allowed = last["band"] | dilate(tie_fit, 1, int(getattr(settings, "validation_size", 1)))
     assert not (flag_diff & ~allowed).any(), ("pre-replacement flags differ outside the bands",
                                               np.argwhere(flag_diff & ~allowed)[:10])
     excused = _reach(flag_diff | tie_fit, ref_init | got_init, settings)

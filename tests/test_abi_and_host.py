@@ -115,12 +115,23 @@ def test_min_search_size_and_frame_shape_assertions():
 
 
 @pytest.mark.parametrize("kwargs,name", [
-    (dict(validation_size=2), "validation_size"),
-    (dict(replacing_size=2), "replacing_size"),
+    (dict(validation_size=9), "validation_size"),
+    (dict(replacing_size=9), "replacing_size"),
 ])
 def test_unimplemented_options_raise(kwargs, name):
     with pytest.raises(NotImplementedError, match=name):
         gp.piv_gpu((256, 256), 16, **kwargs)
+
+
+def test_kernel_sizes_reach_the_plan_description():
+    """validation_size / replacing_size per pass (gpu_process.py:182-187,
+    239-244) land in the pass descriptors; a half-width above a field
+    dimension fails the reference's assertion (gpu_validation.py:74, 273)."""
+    piv = gp.PIVGPU((256, 256), 16, search_size_iters=(1, 1), validation_size=(3, 2), replacing_size=(1, 4))
+    d = piv._plan_desc(_native.PIV_U8)
+    assert [(d.passes[k].validation_size, d.passes[k].replacing_size) for k in range(2)] == [(3, 1), (2, 4)]
+    with pytest.raises(AssertionError, match="size cannot exceed field_shape"):
+        gp.PIVGPU((64, 256), 32, search_size_iters=(1,), validation_size=4)._plan_desc(_native.PIV_U8)
 
 
 def test_fp64_settings_on_the_host():

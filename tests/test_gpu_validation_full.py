@@ -81,10 +81,10 @@ def _field(seed, shape=(40, 48), n_out=60):
     return u, v
 
 
-def _band(u, v, s2n, tests, rel=1e-4):
+def _band(u, v, s2n, tests, rel=1e-4, size=1):
     """Vectors within rel of any enabled test's threshold."""
-    med_u, mad_u, mean_u = O.neighbour_stats(u)
-    med_v, mad_v, mean_v = O.neighbour_stats(v)
+    med_u, mad_u, mean_u = O.neighbour_stats(u, size)
+    med_v, mad_v, mean_v = O.neighbour_stats(v, size)
     near = np.zeros(u.shape, bool)
 
     def close(x, t):
@@ -141,6 +141,62 @@ def test_replacement_loop_matches_oracle(method, n_iters, revalidate):
     assert np.array_equal(gu_[~flags], u[~flags])
 
 
+@pytest.mark.parametrize("size", [2, 3, 8])
+@pytest.mark.parametrize("tests", [dict(median_tol=2.0), dict(median_tol=None, mad_tol=3.0),
+                                   dict(median_tol=1.0, mad_tol=4.0, mean_tol=2.0)])
+def test_wide_validation_matches_oracle(size, tests):
+    """validation_size 2, 3 and the largest, 8: (2 size + 1)^2 neighbourhoods
+    (gpu_validation.py:73-76, 194-205), NaN vectors left out."""
+    u, v = _field(13)
+    u[7, 9] = np.nan
+    v[20, 3] = np.nan
+    desc = gu.tests_desc(validation_size=size, **tests)
+    got, count = gu.stage_validate_all(u, v, desc)
+    ref = O.validate(u, v, None, None, tests.get("median_tol"), tests.get("mad_tol"), tests.get("mean_tol"),
+                     None, size=size)
+    ok = ~_band(u.astype(np.float64), v.astype(np.float64), None, tests, size=size)
+    assert ok.mean() > 0.98
+    assert np.array_equal(got[ok], ref[ok]) and count == int(got.sum()) and got.sum() > 0
+
+
+@pytest.mark.parametrize("method", ["median", "mean", "spring"])
+@pytest.mark.parametrize("size", [2, 3])
+@pytest.mark.parametrize("n_iters,revalidate", [(1, False), (3, False), (2, True)])
+def test_wide_replacement_matches_oracle(method, size, n_iters, revalidate):
+    """replacing_size 2 and 3 (gpu_validation.py:272-290, 336-398): median
+    and mean over (2 size + 1)^2 neighbourhoods, spring links size apart;
+    revalidation with validation_size = size."""
+    u, v = _field(17)
+    flags = O.validate(u, v, median_tol=1.0, size=size)
+    assert not O.spring_singular(flags, size).any()
+    desc = gu.tests_desc(method=method, n_iters=n_iters, revalidate=revalidate, median_tol=1.0,
+                         validation_size=size, replacing_size=size)
+    gu_, gv_, gflags = gu.stage_replace_all(u, v, flags, desc)
+    fn = lambda a, b, only=None: O.validate(a, b, median_tol=1.0, size=size, only=only)
+    ru, rv, rflags = O.replace_outliers(u, v, flags, method, n_iters, revalidate, fn, size)
+    assert np.abs(gu_ - ru).max() <= 1e-4 and np.abs(gv_ - rv).max() <= 1e-4
+    assert np.array_equal(gflags, rflags)
+    assert np.array_equal(gu_[~flags], u[~flags])
+
+
+def test_wide_spring_singular_class_keeps_its_values():
+    """A residue class (rows and columns mod size) flagged entirely is a
+    singular block of the size-2 spring system: its vectors stay as they
+    were, the other outliers are solved."""
+    rng = np.random.default_rng(19)
+    u = rng.standard_normal((12, 14)).astype(np.float32)
+    v = rng.standard_normal((12, 14)).astype(np.float32)
+    flags = np.zeros((12, 14), bool)
+    flags[1::2, 0::2] = True
+    flags[6, 7] = flags[2, 3] = True
+    assert O.spring_singular(flags, 2).sum() == 6 * 7
+    desc = gu.tests_desc(method="spring", replacing_size=2)
+    gu_, gv_, _ = gu.stage_replace_all(u, v, flags, desc)
+    ru, rv, _ = O.replace_outliers(u, v, flags, "spring", 1, size=2)
+    assert np.array_equal(gu_[1::2, 0::2], u[1::2, 0::2])
+    assert np.abs(gu_ - ru).max() <= 1e-5 and np.abs(gv_ - rv).max() <= 1e-5
+
+
 def test_unresolved_block_is_reset_on_gpu():
     u = np.ones((10, 10), np.float32)
     v = np.ones((10, 10), np.float32)
@@ -171,6 +227,34 @@ def test_multipass_with_every_test_matches_oracle(s2n_method):
     assert rep["compared"] > 0.9 * u.size, rep
     for k, st in enumerate(r["passes"]):
         assert piv.outlier_counts(k)[0] == int(st.val.sum()), k
+
+
+@pytest.mark.parametrize("method,vsize,rsize", [("spring", 2, 2), ("median", 3, 2), ("mean", 1, 3)])
+def test_multipass_wide_kernels_match_oracle(method, vsize, rsize):
+    """A three-pass run with validation_size / replacing_size above 1 in
+    every pass (gpu_process.py:747-780)."""
+    from openpiv_gpu.gpu_process import PIVGPU
+    shape = (320, 384)
+    fa, fb = _frames(shape, 47, shift=(3.2, 1.7))
+    fb = fb.copy()
+    rng = np.random.default_rng(47)
+    for _ in range(5):
+        r0, c0 = rng.integers(20, 260, 2)
+        fb[r0:r0 + 48, c0:c0 + 48] = rng.integers(0, 255, (48, 48))
+    kw = dict(search_size_iters=(1, 1, 1), overlap_ratio=0.5, median_tol=1.0, replacing_method=method,
+              num_replacing_iters=2, validation_size=vsize, replacing_size=rsize, smoothing_par=1.5)
+    piv = PIVGPU(shape, 8, dtype_f="float32", **kw)
+    u, v = piv.run_host(fa, fb)
+    s = O.OracleSettings(min_search_size=8, **kw)
+    r = O.piv(fa, fb, s, keep_passes=True)
+    excl = gu.pass_exclusions(fa, fb, s, r["passes"])
+    rep = gu.e2e_compare(u, v, piv.init_val_locations, r, excl, s, final_flags=piv.val_locations)
+    # ~8 % of the last field are outliers; a spring cluster touched by one of
+    # the few fit-ambiguous windows is excused whole, so fewer vectors remain
+    assert rep["compared"] > (0.85 if method == "spring" else 0.9) * u.size, rep
+    for k, st in enumerate(r["passes"]):
+        assert piv.outlier_counts(k)[0] == int(st.val.sum()), k
+    assert all(int(st.val.sum()) > 0 for st in r["passes"])
 
 
 @pytest.mark.parametrize("graph", [False, True])

@@ -3,6 +3,7 @@
 planes, brute-force neighbourhood loops and hand-stepped iterations."""
 
 import numpy as np
+import pytest
 
 from oracle import piv_oracle as O
 
@@ -53,10 +54,10 @@ def test_s2n_special_cases():
     assert s[0] == 0 and np.isnan(s[2])
 
 
-def _brute_stats(f, i, j):
+def _brute_stats(f, i, j, size=1, skip=None):
     m, n = f.shape
-    nb = [f[a, b] for a in range(i - 1, i + 2) for b in range(j - 1, j + 2)
-          if (a, b) != (i, j) and 0 <= a < m and 0 <= b < n]
+    nb = [f[a, b] for a in range(i - size, i + size + 1) for b in range(j - size, j + size + 1)
+          if (a, b) != (i, j) and 0 <= a < m and 0 <= b < n and not (skip is not None and skip[a, b])]
     srt = sorted(nb)
     k = len(srt)
     med = srt[k // 2] if k % 2 else 0.5 * (srt[k // 2 - 1] + srt[k // 2])
@@ -169,3 +170,63 @@ def test_shrunk_window_correlation_by_direct_sum():
                 c[dy, dx] = np.sum(a * np.roll(np.roll(b, -dy, 0), -dx, 1))
         assert np.allclose(got[k], np.fft.fftshift(c), atol=1e-10)
     assert O.shrunk_window(64, 0.1) == 8 and O.shrunk_window(32, 0.55) == 18 and O.shrunk_window(32, 0.6) == 20
+
+
+def test_wide_neighbourhood_stats_and_replacement_by_brute_force():
+    """validation_size / replacing_size 2 and 3: (2 size + 1)^2 kernels
+    (gpu_validation.py:73-76, 194-205, 272-290, 393-398), with NaN entries
+    and flagged neighbours left out."""
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal((9, 11))
+    f[4, 6] = np.nan
+    for size in (2, 3):
+        med, mad, mean = O.neighbour_stats(f, size)
+        for i in range(9):
+            for j in range(11):
+                nb = [x for a in range(i - size, i + size + 1) for b in range(j - size, j + size + 1)
+                      if (a, b) != (i, j) and 0 <= a < 9 and 0 <= b < 11 and np.isfinite(x := f[a, b])]
+                srt = sorted(nb)
+                k = len(srt)
+                want_med = srt[k // 2] if k % 2 else 0.5 * (srt[k // 2 - 1] + srt[k // 2])
+                dev = sorted(abs(x - want_med) for x in nb)
+                want_mad = dev[k // 2] if k % 2 else 0.5 * (dev[k // 2 - 1] + dev[k // 2])
+                assert np.isclose(med[i, j], want_med) and np.isclose(mad[i, j], want_mad)
+                assert np.isclose(mean[i, j], sum(nb) / k)
+        val = rng.random((9, 11)) < 0.2
+        for method in ("median", "mean"):
+            ru, _ = O.neighbour_replacement(f, -f, val, method, exclude_flagged=True, size=size)
+            for i, j in zip(*np.nonzero(val)):
+                nb = [f[a, b] for a in range(i - size, i + size + 1) for b in range(j - size, j + size + 1)
+                      if (a, b) != (i, j) and 0 <= a < 9 and 0 <= b < 11 and not val[a, b] and np.isfinite(f[a, b])]
+                want = np.median(nb) if method == "median" else np.mean(nb)
+                assert np.isclose(ru[i, j], want), (size, method, i, j)
+
+
+def test_wide_spring_splits_into_residue_sublattices():
+    """Spring links size apart (gpu_validation.py:279-283, 365-371) join
+    only vectors of one residue class, and the position bands of width size
+    are the border rows and columns of each class: on fields at least
+    2 size wide, spring replacement with size s equals size-1 replacement
+    on every sublattice f[a::s, b::s]."""
+    rng = np.random.default_rng(12)
+    for size, shape in ((2, (13, 10)), (3, (11, 14)), (2, (4, 5))):
+        u = rng.standard_normal(shape)
+        v = rng.standard_normal(shape)
+        val = rng.random(shape) < 0.3
+        val[0, 0] = val[-1, -1] = True
+        ru, rv = O.spring_replacement(u, v, val, size)
+        for a in range(size):
+            for b in range(size):
+                su, sv = O.spring_replacement(u[a::size, b::size], v[a::size, b::size], val[a::size, b::size])
+                assert np.allclose(ru[a::size, b::size], su, atol=1e-12)
+                assert np.allclose(rv[a::size, b::size], sv, atol=1e-12)
+    # a fully flagged class is singular and keeps its values; the others solve
+    u = rng.standard_normal((8, 9))
+    val = np.zeros((8, 9), bool)
+    val[1::2, 0::2] = True
+    val[4, 4] = True
+    assert np.array_equal(O.spring_singular(val, 2), np.where(np.arange(8)[:, None] % 2 == 1, True, False)
+                          & (np.arange(9)[None, :] % 2 == 0))
+    ru, _ = O.spring_replacement(u, u, val, 2)
+    assert np.array_equal(ru[1::2, 0::2], u[1::2, 0::2])
+    assert ru[4, 4] == pytest.approx((u[2, 4] + u[6, 4] + u[4, 2] + u[4, 6]) / 4, abs=1e-12)
