@@ -52,7 +52,7 @@ typedef struct piv_pass_desc {
   int32_t spacing;            /* ss - int(overlap_ratio * ss)           */
   int32_t x_offset, y_offset; /* centring offsets                        */
   int32_t field_m, field_n;   /* field shape                             */
-  int32_t fft_size;           /* 2**ceil(log2(ss * n_fft))               */
+  int32_t fft_size;           /* 2**ceil(log2(ss * n_fft)), 16..1024     */
   int32_t normalize;          /* subtract window means                   */
   int32_t subpixel;           /* enum piv_subpixel                       */
   float deforming_par;        /* gpu_process.py:1165-1171                */

@@ -71,8 +71,8 @@ bool needs_s2n(const piv_pass_desc& p) { return p.validate && !std::isnan(p.s2n_
 
 int check_pass(const piv_pass_desc& p, int ht, int wd) {
   if (p.search_size < 2 || p.spacing < 1) return fail(PIV_E_INVALID, "search_size/spacing out of range");
-  if (!is_pow2(p.fft_size) || p.fft_size < 16 || p.fft_size > 128)
-    return fail(PIV_E_INVALID, "fft_size %d unsupported (power of two in [16, 128])", p.fft_size);
+  if (!is_pow2(p.fft_size) || p.fft_size < 16 || p.fft_size > kMaxFft)
+    return fail(PIV_E_INVALID, "fft_size %d unsupported (power of two in [16, %d])", p.fft_size, kMaxFft);
   if (p.search_size > p.fft_size || (p.fft_size - p.search_size) % 2)
     return fail(PIV_E_INVALID, "search_size %d incompatible with fft_size %d", p.search_size, p.fft_size);
   if (p.field_m < 1 || p.field_n < 1) return fail(PIV_E_INVALID, "empty field");
@@ -385,8 +385,8 @@ struct piv_plan {
   size_t esize = 4;             // bytes per value
   void* def_a = nullptr;        // deformed frames (value type)
   void* def_b = nullptr;
-  double* twiddle[8] = {};      // FP64 FFT twiddles per log2(F) (F = 16..128)
-  double2* corr_scratch = nullptr;  // FP64 planes of F = 128
+  double* twiddle[11] = {};     // FP64 FFT twiddles per log2(F) (F = 16..1024)
+  void* corr_scratch = nullptr;  // plane slabs: FP64 F >= 128, FP32 F >= 256
   int32_t* counters = nullptr;  // 2 per pass
   double* p_out = nullptr;      // 1 per pass
   uint8_t* val_final = nullptr;
@@ -458,7 +458,8 @@ CorrArgs64 corr_args64_for(const void* fa, const void* fb, int dtype, int ht, in
 CorrArgs64 corr_args64(const piv_plan* P, const void* fa, const void* fb, int dtype, const piv_pass_desc& p) {
   int lg = 0;
   while ((1 << lg) < p.fft_size) ++lg;
-  return corr_args64_for(fa, fb, dtype, P->desc.height, P->desc.width, p, P->twiddle[lg], P->corr_scratch);
+  return corr_args64_for(fa, fb, dtype, P->desc.height, P->desc.width, p, P->twiddle[lg],
+                         (double2*)P->corr_scratch);
 }
 
 // Pass k > 0 (or the learned path's single pass): deform the original
@@ -481,6 +482,7 @@ int deform_and_correlate(piv_plan* P, PassBuffers& B, hipStream_t s) {
     HIP_TRY(launch_correlate64(a, s));
   } else {
     CorrArgs a = corr_args(P->def_a, P->def_b, kF32, D.height, D.width, B.d);
+    a.scratch = P->corr_scratch;
     a.up = B.up;
     a.vp = B.vp;
     a.u = B.u;
@@ -504,6 +506,7 @@ int correlate_first(piv_plan* P, PassBuffers& B, hipStream_t s) {
     HIP_TRY(launch_correlate64(a, s));
   } else {
     CorrArgs a = corr_args(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d);
+    a.scratch = P->corr_scratch;
     a.u = B.u;
     a.v = B.v;
     a.s2n = B.s2n;
@@ -917,6 +920,7 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
   const size_t npx = (size_t)D.height * D.width;
   P->frame_bytes = npx * dtype_size(D.in_dtype);
   int maxN = 0;
+  size_t scratch_bytes = 0;
   auto dmalloc = [&](void** p, size_t bytes) -> bool { return hipMalloc(p, bytes ? bytes : 16) == hipSuccess; };
   bool ok = dmalloc(&P->frame_a, P->frame_bytes) && dmalloc(&P->frame_b, P->frame_bytes);
   if (D.n_passes > 1) ok = ok && dmalloc((void**)&P->def_a, npx * es) && dmalloc((void**)&P->def_b, npx * es);
@@ -946,16 +950,18 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
         int r = upload((void**)&P->twiddle[lg], fft_twiddles(B.d.fft_size));
         if (r) return bail(r);
       }
-      const size_t sb = correlate64_scratch_bytes(B.d.fft_size);
-      if (sb && !P->corr_scratch && hipMalloc((void**)&P->corr_scratch, sb) != hipSuccess)
-        return bail(fail(PIV_E_NOMEM, "device allocation failed"));
     }
+    // one scratch for every pass: the largest any of them needs
+    scratch_bytes = std::max(scratch_bytes, P->fp64 ? correlate64_scratch_bytes(B.d.fft_size)
+                                                    : correlate_big_scratch_bytes(B.d.fft_size, 4, B.N));
     if (k < D.n_passes - 1 && B.d.smooth) {
       int r = make_tables(B.d.field_m, B.d.field_n, &B.tab, P->fp64 != 0);
       if (r) return bail(r);
       p_bounds(B.d.field_m, B.d.field_n, &B.p_min, &B.p_max);
     }
   }
+  if (scratch_bytes && !dmalloc(&P->corr_scratch, scratch_bytes))
+    return bail(fail(PIV_E_NOMEM, "device allocation failed"));
   const PassBuffers& L = P->passes.back();
   ok = dmalloc((void**)&P->val_final, L.N) && dmalloc((void**)&P->t0, 2ull * maxN * es) &&
        dmalloc((void**)&P->t1, 2ull * maxN * es) && dmalloc((void**)&P->rw.slot, maxN * 4) &&
@@ -1251,6 +1257,7 @@ int piv_plan_last_s2n(piv_plan* P, void* s2n_out) {
         e = launch_apply_field_mask(as<double>(tu), as<double>(tv), as<double>(ts), B.fmask, B.N, s);
     } else {
       CorrArgs a = corr_args(fa, fb, dtype, D.height, D.width, B.d);
+      a.scratch = P->corr_scratch;
       if (!raw) {
         a.up = B.up;
         a.vp = B.vp;
@@ -1318,7 +1325,19 @@ int piv_stage_correlate(const void* frame_a, const void* frame_b, int in_dtype, 
   a.j_peak = j_peak;
   a.planes = planes;
   a.s2n = s2n;
-  HIP_TRY(launch_correlate(a, (hipStream_t)stream));
+  const size_t sb = correlate_big_scratch_bytes(pass->fft_size, 4, pass->field_m * pass->field_n);
+  if (!sb) {
+    HIP_TRY(launch_correlate(a, (hipStream_t)stream));
+    return PIV_OK;
+  }
+  // FFT sizes above 128: plane slabs for this call, released after it
+  hipStream_t s = (hipStream_t)stream;
+  if (hipMalloc(&a.scratch, sb) != hipSuccess) return fail(PIV_E_NOMEM, "device allocation failed");
+  const hipError_t e = launch_correlate(a, s);
+  const hipError_t e2 = hipStreamSynchronize(s);
+  (void)hipFree(a.scratch);
+  if (e != hipSuccess) return fail(PIV_E_HIP, "correlate: %s [%s]", hipGetErrorString(e), launch_detail());
+  if (e2 != hipSuccess) return fail(PIV_E_HIP, "correlate: %s", hipGetErrorString(e2));
   return PIV_OK;
 }
 
@@ -1930,8 +1949,13 @@ extern "C" {
 int piv_batch_create(int device, int n_pairs, const piv_plan_desc* descs, piv_batch** out) {
   if (!out || !descs || n_pairs < 1) return fail(PIV_E_INVALID, "batch: bad argument");
   *out = nullptr;
-  for (int i = 0; i < n_pairs; ++i)
+  for (int i = 0; i < n_pairs; ++i) {
     if (descs[i].precision != PIV_FP32) return fail(PIV_E_INVALID, "batch: FP32 plans only");
+    for (int k = 0; k < descs[i].n_passes && k < PIV_MAX_PASSES; ++k)
+      if (descs[i].passes[k].fft_size > kMaxSmallFft)
+        return fail(PIV_E_INVALID, "batch: FFT sizes up to %d (pair %d pass %d has %d)", kMaxSmallFft, i, k,
+                    descs[i].passes[k].fft_size);
+  }
   if (piv_device_count() <= 0) return fail(PIV_E_NO_DEVICE, "no HIP device visible");
   HIP_TRY(hipSetDevice(device));
   piv_batch* b = new piv_batch();

@@ -702,6 +702,7 @@ hipError_t launch_correlate(const CorrArgs& a, hipStream_t stream) {
   if (correlate_line_enabled() && correlate_line_applies(a)) return launch_correlate_line(a, stream);
   // F = 64, 128 under the same conditions: the residue-split kernel
   if (correlate_split_enabled() && correlate_split_applies(a)) return launch_correlate_split(a, stream);
+  if (a.fsize > kMaxSmallFft) return launch_correlate_big(a, stream);
   switch (a.fsize) {
     case 16: return launch_t<16>(a, stream);
     case 32: return launch_t<32>(a, stream);

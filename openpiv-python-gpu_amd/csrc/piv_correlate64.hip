@@ -331,13 +331,15 @@ int correlate64_blocks(int fsize, int n_windows) {
 }
 
 size_t correlate64_scratch_bytes(int fsize) {
+  if (fsize > kMaxSmallFft) return correlate_big_scratch_bytes(fsize, 8, 1 << 30);
   return fsize > 64 ? (size_t)kMaxBlocks * fsize * fsize * sizeof(double2) : 0;
 }
 
 hipError_t launch_correlate64(const CorrArgs64& a, hipStream_t stream) {
   if (a.m * a.n <= 0) return hipSuccess;
-  if (a.ss > a.fsize || (a.fsize - a.ss) % 2 || !a.twiddle) return hipErrorInvalidValue;
+  if (a.ss > a.fsize || (a.fsize - a.ss) % 2 || (!a.twiddle && a.fsize <= kMaxSmallFft)) return hipErrorInvalidValue;
   if (a.fsize > 64 && !a.scratch) return hipErrorInvalidValue;
+  if (a.fsize > kMaxSmallFft) return launch_correlate_big64(a, stream);
   switch (a.fsize) {
     case 16: return launch64_f<16>(a, stream);
     case 32: return launch64_f<32>(a, stream);

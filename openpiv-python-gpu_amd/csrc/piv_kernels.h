@@ -8,6 +8,10 @@ namespace piv {
 
 enum InDtype { kU8 = 0, kU16 = 1, kF32 = 2, kF64 = 3 };
 enum SubpixelMethod { kGaussian = 0, kParabolic = 1, kCentroid = 2 };
+// FFT sizes up to kMaxSmallFft keep the correlation plane on chip; larger
+// ones (to kMaxFft) keep it in a device-memory slab per workgroup.
+constexpr int kMaxSmallFft = 128;
+constexpr int kMaxFft = 1024;
 
 // Correlation of one pass: every window of the field is extracted from the
 // two frames, mean-normalised, zero-padded to F x F, cross-correlated by FFT,
@@ -34,6 +38,7 @@ struct CorrArgs {
   int s2n_method;          // 0 peak2peak, 1 peak2mean, 2 peak2energy (gpu_process.py:1312-1365)
   int s2n_size;            // peak2peak exclusion half-width
   float* s2n;              // per-window signal-to-noise ratio, or null (not computed)
+  void* scratch;           // F > kMaxSmallFft: correlate_big_scratch_bytes(F, 4, m * n) bytes
 };
 
 hipError_t launch_correlate(const CorrArgs& a, hipStream_t stream);
@@ -72,6 +77,15 @@ struct CorrArgs64 {
   double2* scratch;
 };
 size_t correlate64_scratch_bytes(int fsize);
+
+// FFT sizes 256..1024 (both precisions): each workgroup keeps its F x F
+// complex plane in a device-memory slab; correlate_big_blocks workgroups
+// (capped by a 512 MiB slab budget) walk the windows.  FP32 takes its slabs
+// from CorrArgs::scratch, FP64 from CorrArgs64::scratch; 0 bytes for F <= 128.
+int correlate_big_blocks(int fsize, int real_bytes, int n_windows);
+size_t correlate_big_scratch_bytes(int fsize, int real_bytes, int n_windows);
+hipError_t launch_correlate_big(const CorrArgs& a, hipStream_t stream);
+hipError_t launch_correlate_big64(const CorrArgs64& a, hipStream_t stream);
 int correlate64_blocks(int fsize, int n_windows);
 hipError_t launch_correlate64(const CorrArgs64& a, hipStream_t stream);
 

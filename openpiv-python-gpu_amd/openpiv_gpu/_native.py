@@ -43,6 +43,9 @@ EXPORTED_SYMBOLS = (
     "piv_plan_last_s2n", "piv_stage_replace_noalloc",
 )
 
+# largest FFT size of the correlation kernels (planes above 128 points live in
+# device-memory slabs, 512 MiB per plan)
+MAX_FFT_SIZE = 1024
 # largest validation_size / replacing_size the kernels take: (2 * 8 + 1)^2 - 1
 # = 288 neighbours, 64 spring residue classes
 MAX_NEIGHBOURHOOD_SIZE = 8

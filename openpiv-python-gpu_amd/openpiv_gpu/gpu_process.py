@@ -300,9 +300,9 @@ class PIVGPU:
         d.n_passes = len(fields)
         for k, (f, i) in enumerate(zip(fields, self._size_index())):
             fft = 2 ** ceil(log2(f.search_size * self.n_fft[i]))
-            if fft > 128:
+            if fft > _native.MAX_FFT_SIZE:
                 raise NotImplementedError(f"FFT size {fft} (search_size {f.search_size}, n_fft {self.n_fft[i]}): "
-                                          "the MI355X correlation kernels cover FFT sizes 16-128")
+                                          f"the correlation kernels cover FFT sizes 16-{_native.MAX_FFT_SIZE}")
             if (fft - f.search_size) % 2:
                 raise NotImplementedError("odd padding: search_size must be even")
             p = d.passes[k]

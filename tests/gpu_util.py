@@ -118,6 +118,32 @@ def stage_correlate(fa, fb, geom, fsize, up=None, vpred=None, planes=False, subp
     return out
 
 
+def stage_correlate64(fa, fb, geom, fsize, up=None, vpred=None, subpixel=0, s2n_method=None, s2n_size=2, window_size=0):
+    lib = _native.load_library()
+    m, n = geom.field_shape
+    da, db = dev(fa), dev(fb)
+    u = empty((m, n), np.float64)
+    v = empty((m, n), np.float64)
+    ip = empty((m, n), np.int32)
+    jp = empty((m, n), np.int32)
+    pl = empty((m * n, fsize, fsize), np.float64)
+    sn = empty((m, n), np.float64) if s2n_method is not None else None
+    dup = dev(np.asarray(up, np.float64)) if up is not None else None
+    dvp = dev(np.asarray(vpred, np.float64)) if vpred is not None else None
+    desc = pass_desc(geom, fsize, subpixel=subpixel)
+    desc.window_size = window_size
+    if s2n_method is not None:
+        desc.s2n_method = _native.S2N_CODES[s2n_method]
+        desc.s2n_size = s2n_size
+    _native.check(lib.piv_stage_correlate64(vp(da), vp(db), _native.in_dtype_code(fa.dtype), fa.shape[0],
+                                            fa.shape[1], ctypes.byref(desc), vp(dup), vp(dvp), vp(u),
+                                            vp(v), vp(ip), vp(jp), vp(pl), vp(sn), None))
+    out = {"u": host(u), "v": host(v), "i_peak": host(ip), "j_peak": host(jp), "corr": host(pl)}
+    if sn is not None:
+        out["s2n"] = host(sn)
+    return out
+
+
 def stage_deform(fa, fb, geom, up, vpred, deforming_par=0.5):
     lib = _native.load_library()
     da, db = dev(fa), dev(fb)

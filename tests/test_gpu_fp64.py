@@ -4,7 +4,6 @@ stage (planes, peaks, fits, S2N, interrogation window inside the search
 window), single- and multi-pass runs end to end, masks, and every
 validation test."""
 
-import ctypes
 
 import numpy as np
 import pytest
@@ -15,31 +14,7 @@ pytestmark = pytest.mark.gpu
 gu = pytest.importorskip("gpu_util")
 
 
-def _stage64(fa, fb, geom, fsize, up=None, vpred=None, subpixel=0, s2n_method=None, s2n_size=2, window_size=0):
-    from openpiv_gpu import _native
-    lib = _native.load_library()
-    m, n = geom.field_shape
-    da, db = gu.dev(fa), gu.dev(fb)
-    u = gu.empty((m, n), np.float64)
-    v = gu.empty((m, n), np.float64)
-    ip = gu.empty((m, n), np.int32)
-    jp = gu.empty((m, n), np.int32)
-    pl = gu.empty((m * n, fsize, fsize), np.float64)
-    sn = gu.empty((m, n), np.float64) if s2n_method is not None else None
-    dup = gu.dev(np.asarray(up, np.float64)) if up is not None else None
-    dvp = gu.dev(np.asarray(vpred, np.float64)) if vpred is not None else None
-    desc = gu.pass_desc(geom, fsize, subpixel=subpixel)
-    desc.window_size = window_size
-    if s2n_method is not None:
-        desc.s2n_method = _native.S2N_CODES[s2n_method]
-        desc.s2n_size = s2n_size
-    _native.check(lib.piv_stage_correlate64(gu.vp(da), gu.vp(db), _native.in_dtype_code(fa.dtype), fa.shape[0],
-                                            fa.shape[1], ctypes.byref(desc), gu.vp(dup), gu.vp(dvp), gu.vp(u),
-                                            gu.vp(v), gu.vp(ip), gu.vp(jp), gu.vp(pl), gu.vp(sn), None))
-    out = {"u": gu.host(u), "v": gu.host(v), "i_peak": gu.host(ip), "j_peak": gu.host(jp), "corr": gu.host(pl)}
-    if sn is not None:
-        out["s2n"] = gu.host(sn)
-    return out
+_stage64 = gu.stage_correlate64
 
 
 def _pair(shape, dtype, seed):
