@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <climits>
 #include <mutex>
 #include <cmath>
 #include <string>
@@ -13,6 +14,7 @@
 #include "../../include/openpiv_hip.h"
 #include "piv_cnn.h"
 #include "piv_kernels.h"
+#include "piv_record.h"
 
 using namespace piv;
 
@@ -1738,12 +1740,23 @@ struct BatchGroup {  // one batched correlation launch
   int* d_prefix = nullptr;
 };
 
+// One launch of a recorded field-stage kernel over a group of pairs whose
+// recorded sequences match (piv_record.h).
+struct BatchedStep {
+  LaunchKind kind = LaunchKind::kFill;
+  int n = 0, total_blocks = 0;
+  void* d_args = nullptr;    // n argument structs
+  int4* d_grids = nullptr;   // n (gx, gy, gz, first block)
+};
+
 struct BatchLevel {
   std::vector<int> plans;  // pairs with a pass at this level
   std::vector<BatchGroup> groups;
   DeformArgs* d_deform = nullptr;  // pairs with L > 0
   int* d_rows = nullptr;
   int n_deform = 0, total_rows = 0, max_wd = 0, deform_threads = 256;
+  std::vector<BatchedStep> pre;   // predictor interpolation (L > 0)
+  std::vector<BatchedStep> post;  // validation, replacement, smoothing
 };
 
 }  // namespace
@@ -1757,9 +1770,19 @@ struct piv_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int runs = 0;
+  std::vector<BatchedStep> start;   // counter resets
+  std::vector<BatchedStep> finish;  // output scaling
 };
 
 namespace {
+
+void free_steps(std::vector<BatchedStep>& steps) {
+  for (auto& st : steps) {
+    if (st.d_args) (void)hipFree(st.d_args);
+    if (st.d_grids) (void)hipFree(st.d_grids);
+  }
+  steps.clear();
+}
 
 void batch_free_tables(piv_batch* b) {
   for (auto& L : b->levels) {
@@ -1769,7 +1792,122 @@ void batch_free_tables(piv_batch* b) {
     }
     if (L.d_deform) (void)hipFree(L.d_deform);
     if (L.d_rows) (void)hipFree(L.d_rows);
+    free_steps(L.pre);
+    free_steps(L.post);
   }
+  free_steps(b->start);
+  free_steps(b->finish);
+}
+
+// The launches fn() makes, recorded instead of launched.
+template <typename Fn>
+int record_sequence(std::vector<RecordedLaunch>* seq, Fn&& fn) {
+  g_launch_record = seq;
+  const int r = fn();
+  g_launch_record = nullptr;
+  return r;
+}
+
+// Batched steps from the recorded sequences of several pairs: pairs whose
+// sequences have the same kinds in the same order form a group, and step j
+// of a group is one launch over the group's launch j.
+int build_steps(const std::vector<std::vector<RecordedLaunch>>& seqs, std::vector<BatchedStep>& out) {
+  std::vector<bool> taken(seqs.size(), false);
+  for (size_t i = 0; i < seqs.size(); ++i) {
+    if (taken[i] || seqs[i].empty()) continue;
+    std::vector<size_t> group;
+    for (size_t q = i; q < seqs.size(); ++q) {
+      if (taken[q] || seqs[q].size() != seqs[i].size()) continue;
+      bool same = true;
+      for (size_t j = 0; j < seqs[i].size() && same; ++j)
+        same = seqs[q][j].kind == seqs[i][j].kind && seqs[q][j].block == seqs[i][j].block &&
+               seqs[q][j].args.size() == seqs[i][j].args.size();
+      if (same) {
+        group.push_back(q);
+        taken[q] = true;
+      }
+    }
+    for (size_t j = 0; j < seqs[i].size(); ++j) {
+      BatchedStep st;
+      st.kind = seqs[i][j].kind;
+      const size_t asz = seqs[i][j].args.size();
+      std::vector<unsigned char> args;
+      std::vector<int4> grids;
+      long long blocks = 0;
+      for (size_t q : group) {
+        const RecordedLaunch& r = seqs[q][j];
+        args.insert(args.end(), r.args.begin(), r.args.end());
+        grids.push_back(make_int4(r.gx, r.gy, r.gz, (int)blocks));
+        blocks += (long long)r.gx * r.gy * r.gz;
+      }
+      if (blocks > INT_MAX) return fail(PIV_E_INVALID, "batch: too many blocks in one field-stage launch");
+      st.n = (int)group.size();
+      st.total_blocks = (int)blocks;
+      if (hipMalloc(&st.d_args, args.size()) != hipSuccess ||
+          hipMalloc((void**)&st.d_grids, grids.size() * sizeof(int4)) != hipSuccess) {
+        out.push_back(st);  // freed with the tables
+        return fail(PIV_E_NOMEM, "batch: table allocation failed");
+      }
+      out.push_back(st);
+      HIP_TRY(hipMemcpy(st.d_args, args.data(), st.n * asz, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(st.d_grids, grids.data(), grids.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+  }
+  return PIV_OK;
+}
+
+int run_steps(const std::vector<BatchedStep>& steps, hipStream_t s) {
+  for (const BatchedStep& st : steps) {
+    const hipError_t e = st.kind >= LaunchKind::kGemm
+                             ? launch_smoothn_batch(st.kind, st.d_args, st.d_grids, st.n, st.total_blocks, s)
+                             : launch_field_batch(st.kind, st.d_args, st.d_grids, st.n, st.total_blocks, s);
+    if (e != hipSuccess) return fail(PIV_E_HIP, "batch: field-stage launch: %s", hipGetErrorString(e));
+  }
+  return PIV_OK;
+}
+
+// Records every pair's field-stage launches (start, per level before and
+// after the correlation, finish) and builds the batched steps.
+int batch_build_steps(piv_batch* b) {
+  const size_t np = b->plans.size();
+  std::vector<std::vector<RecordedLaunch>> seqs(np);
+  for (size_t i = 0; i < np; ++i) {
+    piv_plan* P = b->plans[i];
+    int r = record_sequence(&seqs[i], [&] {
+      HIP_TRY(launch_fill_i32(P->counters, 0, 2 * (int)P->passes.size(), nullptr));
+      HIP_TRY(launch_fill_i32(P->lb.it_counts, 0, PIV_MAX_REPLACING_ITERS * (int)P->passes.size(), nullptr));
+      return (int)PIV_OK;
+    });
+    if (r) return r;
+  }
+  int r = build_steps(seqs, b->start);
+  if (r) return r;
+  for (size_t l = 0; l < b->levels.size(); ++l) {
+    BatchLevel& L = b->levels[l];
+    std::vector<std::vector<RecordedLaunch>> pre(L.plans.size()), post(L.plans.size());
+    for (size_t q = 0; q < L.plans.size(); ++q) {
+      piv_plan* P = b->plans[L.plans[q]];
+      const bool last = l + 1 == P->passes.size();
+      if (l > 0) r = record_sequence(&pre[q], [&] { return predictor_from_previous<float>(P, (int)l, nullptr); });
+      if (!r) r = record_sequence(&post[q], [&] { return validate_and_smooth<float>(P, (int)l, last, nullptr); });
+      if (r) return r;
+    }
+    r = build_steps(pre, L.pre);
+    if (!r) r = build_steps(post, L.post);
+    if (r) return r;
+  }
+  std::vector<std::vector<RecordedLaunch>> fin(np);
+  for (size_t i = 0; i < np; ++i) {
+    piv_plan* P = b->plans[i];
+    if (P->desc.out_scale == 1.0f) continue;
+    PassBuffers& Lp = P->passes.back();
+    r = record_sequence(&fin[i], [&] {
+      HIP_TRY(launch_scale(Lp.u, Lp.v, Lp.N, P->desc.out_scale, nullptr));
+      return (int)PIV_OK;
+    });
+    if (r) return r;
+  }
+  return build_steps(fin, b->finish);
 }
 
 void batch_drop_graph(piv_batch* b) {
@@ -1885,7 +2023,7 @@ int batch_build_tables(piv_batch* b) {
       }
     }
   }
-  return PIV_OK;
+  return batch_build_steps(b);
 }
 
 int batch_enqueue_corr(piv_batch* b, BatchGroup& g, hipStream_t s) {
@@ -1907,39 +2045,24 @@ int batch_enqueue_deform(piv_batch* b, BatchLevel& L, hipStream_t s) {
   return PIV_OK;
 }
 
+// The batch's pass loop: per level one batched launch per field-stage
+// kernel (the recorded steps), one deformation launch and one correlation
+// launch per group.
 int batch_enqueue(piv_batch* b) {
   hipStream_t s = b->stream;
-  for (auto* P : b->plans) {
-    HIP_TRY(launch_fill_i32(P->counters, 0, 2 * (int)P->passes.size(), s));
-    HIP_TRY(launch_fill_i32(P->lb.it_counts, 0, PIV_MAX_REPLACING_ITERS * (int)P->passes.size(), s));
-  }
-  for (size_t l = 0; l < b->levels.size(); ++l) {
+  int r = run_steps(b->start, s);
+  for (size_t l = 0; l < b->levels.size() && !r; ++l) {
     BatchLevel& L = b->levels[l];
     if (l > 0) {
-      for (int p : L.plans) {
-        int r = predictor_from_previous<float>(b->plans[p], (int)l, s);
-        if (r) return r;
-      }
-      int r = batch_enqueue_deform(b, L, s);
-      if (r) return r;
+      r = run_steps(L.pre, s);
+      if (!r) r = batch_enqueue_deform(b, L, s);
     }
-    for (auto& g : L.groups) {
-      int r = batch_enqueue_corr(b, g, s);
-      if (r) return r;
-    }
-    for (int p : L.plans) {
-      piv_plan* P = b->plans[p];
-      int r = validate_and_smooth<float>(P, (int)l, l + 1 == P->passes.size(), s);
-      if (r) return r;
-    }
+    for (auto& g : L.groups)
+      if (!r) r = batch_enqueue_corr(b, g, s);
+    if (!r) r = run_steps(L.post, s);
   }
-  for (auto* P : b->plans) {
-    if (P->desc.out_scale != 1.0f) {
-      PassBuffers& Lp = P->passes.back();
-      HIP_TRY(launch_scale(Lp.u, Lp.v, Lp.N, P->desc.out_scale, s));
-    }
-  }
-  return PIV_OK;
+  if (!r) r = run_steps(b->finish, s);
+  return r;
 }
 
 }  // namespace

@@ -6,6 +6,7 @@
 
 #include "piv_fft.h"
 #include "piv_kernels.h"
+#include "piv_record.h"
 #include "piv_wave.h"
 
 namespace piv {
@@ -124,16 +125,23 @@ __global__ __launch_bounds__(512) void deform_batch_kernel(const DeformArgs* fie
 }
 
 template <typename R>
+struct GridInterpBody {
+  using Args = GridInterpArgsT<R>;
+  static __device__ void run(const Args& a, VB b) {
+    const int k = b.bx * blockDim.x + threadIdx.x;
+    if (k >= a.m_out * a.n_out) return;
+    const int i = k / a.n_out, j = k % a.n_out;
+    const R yq = a.y0_out + (R)i * a.sp_out;
+    const R xq = a.x0_out + (R)j * a.sp_out;
+    R ou, ov;
+    grid_bilinear<R>(a.u_in, a.v_in, a.m_in, a.n_in, a.y0_in, a.x0_in, a.sp_in, yq, xq, ou, ov);
+    a.u_out[k] = ou;
+    a.v_out[k] = ov;
+  }
+};
+template <typename R>
 __global__ __launch_bounds__(256) void grid_interp_kernel(GridInterpArgsT<R> a) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.m_out * a.n_out) return;
-  const int i = k / a.n_out, j = k % a.n_out;
-  const R yq = a.y0_out + (R)i * a.sp_out;
-  const R xq = a.x0_out + (R)j * a.sp_out;
-  R ou, ov;
-  grid_bilinear<R>(a.u_in, a.v_in, a.m_in, a.n_in, a.y0_in, a.x0_in, a.sp_in, yq, xq, ou, ov);
-  a.u_out[k] = ou;
-  a.v_out[k] = ov;
+  GridInterpBody<R>::run(a, vb_native());
 }
 
 // Median of the valid entries of a[0..cnt) after sorting (cnt <= 8).
@@ -735,9 +743,31 @@ __device__ void spring_cg_wave(SpringLds& L, int nv) {
 }
 
 template <typename R>
-__global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* v, const uint8_t* flags,
-                                                                       const int32_t* count, int m, int n, int sz,
-                                                                       ReplaceWork w) {
+struct SpringArgs {
+  R* u;
+  R* v;
+  const uint8_t* flags;
+  const int32_t* count;
+  int m, n, sz;
+  ReplaceWork w;
+};
+template <typename R>
+struct SpringBody {
+  using Args = SpringArgs<R>;
+  static __device__ void run(const Args& a, VB vb);
+};
+template <typename R>
+__global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(SpringArgs<R> a) {
+  SpringBody<R>::run(a, vb_native());
+}
+template <typename R>
+__device__ void SpringBody<R>::run(const Args& sa, VB) {
+  R* u = sa.u;
+  R* v = sa.v;
+  const uint8_t* flags = sa.flags;
+  const int32_t* count = sa.count;
+  const int m = sa.m, n = sa.n, sz = sa.sz;
+  const ReplaceWork& w = sa.w;
   constexpr int T = kSpringLdsThreads;
   const int N = m * n;
   const int nv = *count;
@@ -934,18 +964,40 @@ __global__ __launch_bounds__(kSpringLdsThreads) void spring_lds_kernel(R* u, R* 
   }
 }
 
-__global__ void fill_i32_kernel(int32_t* p, int32_t value, int count) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < count) p[k] = value;
-}
-template <typename R>
-__global__ void scale_kernel(R* u, R* v, int count, R factor) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < count) {
-    u[k] *= factor;
-    v[k] *= factor;
+struct FillArgs {
+  int32_t* p;
+  int32_t value;
+  int count;
+};
+struct FillBody {
+  using Args = FillArgs;
+  static __device__ void run(const Args& a, VB b) {
+    const int k = b.bx * blockDim.x + threadIdx.x;
+    if (k < a.count) a.p[k] = a.value;
   }
-}
+};
+__global__ __launch_bounds__(256) void fill_i32_kernel(FillArgs a) { FillBody::run(a, vb_native()); }
+
+template <typename R>
+struct ScaleArgs {
+  R* u;
+  R* v;
+  int count;
+  R factor;
+};
+template <typename R>
+struct ScaleBody {
+  using Args = ScaleArgs<R>;
+  static __device__ void run(const Args& a, VB b) {
+    const int k = b.bx * blockDim.x + threadIdx.x;
+    if (k < a.count) {
+      a.u[k] *= a.factor;
+      a.v[k] *= a.factor;
+    }
+  }
+};
+template <typename R>
+__global__ __launch_bounds__(256) void scale_kernel(ScaleArgs<R> a) { ScaleBody<R>::run(a, vb_native()); }
 __global__ void scale_div_kernel(double* u, double* v, int count, double mul, double div) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < count) {
@@ -954,10 +1006,21 @@ __global__ void scale_div_kernel(double* u, double* v, int count, double mul, do
   }
 }
 template <typename R>
-__global__ void copy_kernel(R* dst, const R* src, int count) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < count) dst[k] = src[k];
-}
+struct CopyArgs {
+  R* dst;
+  const R* src;
+  int count;
+};
+template <typename R>
+struct CopyBody {
+  using Args = CopyArgs<R>;
+  static __device__ void run(const Args& a, VB b) {
+    const int k = b.bx * blockDim.x + threadIdx.x;
+    if (k < a.count) a.dst[k] = a.src[k];
+  }
+};
+template <typename R>
+__global__ __launch_bounds__(256) void copy_kernel(CopyArgs<R> a) { CopyBody<R>::run(a, vb_native()); }
 __global__ void copy_u8_kernel(uint8_t* dst, const uint8_t* src, int count) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < count) dst[k] = src[k];
@@ -1129,8 +1192,17 @@ __device__ void wide_stats(const G& g, R* med, R* mad, R* mean) {
 }
 
 template <typename R, bool WIDE>
+struct ValidateBody {
+  using Args = ValidateArgsT<R>;
+  static __device__ void run(const Args& a, VB vb);
+};
+template <typename R, bool WIDE>
 __global__ __launch_bounds__(256) void validate_kernel(ValidateArgsT<R> a) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  ValidateBody<R, WIDE>::run(a, vb_native());
+}
+template <typename R, bool WIDE>
+__device__ void ValidateBody<R, WIDE>::run(const Args& a, VB vb) {
+  const int k = vb.bx * blockDim.x + threadIdx.x;
   bool flag = false;
   if (k < a.m * a.n) {
     if ((!a.only || a.only[k]) && !(a.fmask && a.fmask[k])) {
@@ -1175,7 +1247,27 @@ __global__ __launch_bounds__(256) void validate_kernel(ValidateArgsT<R> a) {
 // One 1024-thread block: NaN-mean and population std of u and v in FP64,
 // two passes (mean, then sum of squared deviations).
 template <typename R>
-__global__ __launch_bounds__(1024) void field_stats_kernel(const R* u, const R* v, int count, double* stats) {
+struct FieldStatsArgs {
+  const R* u;
+  const R* v;
+  int count;
+  double* stats;
+};
+template <typename R>
+struct FieldStatsBody {
+  using Args = FieldStatsArgs<R>;
+  static __device__ void run(const Args& a, VB vb);
+};
+template <typename R>
+__global__ __launch_bounds__(1024) void field_stats_kernel(FieldStatsArgs<R> a) {
+  FieldStatsBody<R>::run(a, vb_native());
+}
+template <typename R>
+__device__ void FieldStatsBody<R>::run(const Args& fa, VB) {
+  const R* u = fa.u;
+  const R* v = fa.v;
+  const int count = fa.count;
+  double* stats = fa.stats;
   __shared__ double red[2][32];
   __shared__ double mean[2];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1216,11 +1308,33 @@ __global__ __launch_bounds__(1024) void field_stats_kernel(const R* u, const R* 
   }
 }
 
+template <typename R>
+struct NbrArgs {
+  const R* u_in;
+  const R* v_in;
+  R* u;
+  R* v;
+  const uint8_t* flags;
+  int m, n, size, method, exclude;
+};
 template <typename R, bool WIDE>
-__global__ __launch_bounds__(256) void neighbour_replace_kernel(const R* u_in, const R* v_in, R* u, R* v,
-                                                                const uint8_t* flags, int m, int n, int size,
-                                                                int method, int exclude) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+struct NbrBody {
+  using Args = NbrArgs<R>;
+  static __device__ void run(const Args& a, VB vb);
+};
+template <typename R, bool WIDE>
+__global__ __launch_bounds__(256) void neighbour_replace_kernel(NbrArgs<R> a) {
+  NbrBody<R, WIDE>::run(a, vb_native());
+}
+template <typename R, bool WIDE>
+__device__ void NbrBody<R, WIDE>::run(const Args& na, VB vb) {
+  const R* u_in = na.u_in;
+  const R* v_in = na.v_in;
+  R* u = na.u;
+  R* v = na.v;
+  const uint8_t* flags = na.flags;
+  const int m = na.m, n = na.n, size = na.size, method = na.method, exclude = na.exclude;
+  const int k = vb.bx * blockDim.x + threadIdx.x;
   if (k >= m * n || !flags[k]) return;
   const int i = k / n, j = k % n;
   const uint8_t* skip = exclude ? flags : nullptr;
@@ -1243,9 +1357,34 @@ __global__ __launch_bounds__(256) void neighbour_replace_kernel(const R* u_in, c
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void reset_unresolved_kernel(R* u, R* v, const R* u0, const R* v0,
-                                                               const uint8_t* flags, uint8_t* nan_flags, int count) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+struct ResetArgs {
+  R* u;
+  R* v;
+  const R* u0;
+  const R* v0;
+  const uint8_t* flags;
+  uint8_t* nan_flags;
+  int count;
+};
+template <typename R>
+struct ResetBody {
+  using Args = ResetArgs<R>;
+  static __device__ void run(const Args& a, VB vb);
+};
+template <typename R>
+__global__ __launch_bounds__(256) void reset_unresolved_kernel(ResetArgs<R> a) {
+  ResetBody<R>::run(a, vb_native());
+}
+template <typename R>
+__device__ void ResetBody<R>::run(const Args& ra, VB vb) {
+  R* u = ra.u;
+  R* v = ra.v;
+  const R* u0 = ra.u0;
+  const R* v0 = ra.v0;
+  const uint8_t* flags = ra.flags;
+  uint8_t* nan_flags = ra.nan_flags;
+  const int count = ra.count;
+  const int k = vb.bx * blockDim.x + threadIdx.x;
   if (k >= count) return;
   const bool bad = flags[k] && (isnan(u[k]) || isnan(v[k]));
   if (bad) {
@@ -1290,11 +1429,20 @@ hipError_t launch_deform_batch(const DeformArgs* fields, const int* row_prefix, 
   return hipGetLastError();
 }
 
+// Recording of a launch (FP32 only: batches are FP32).
+template <typename R, typename A>
+bool recorded(LaunchKind kind, const A& a, dim3 grid, int block) {
+  if constexpr (sizeof(R) != 4) return false;
+  else return record_launch(kind, a, grid, block);
+}
+
 template <typename R>
 hipError_t launch_grid_interp(const GridInterpArgsT<R>& a, hipStream_t stream) {
   if (a.m_in < 2 || a.n_in < 2) return hipErrorInvalidValue;
   const int total = a.m_out * a.n_out;
-  hipLaunchKernelGGL(grid_interp_kernel<R>, dim3((total + 255) / 256), dim3(256), 0, stream, a);
+  const dim3 grid((total + 255) / 256);
+  if (recorded<R>(LaunchKind::kGridInterp, a, grid, 256)) return hipSuccess;
+  hipLaunchKernelGGL(grid_interp_kernel<R>, grid, dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
@@ -1310,16 +1458,20 @@ template <typename R>
 hipError_t launch_validate(const ValidateArgsT<R>& a, hipStream_t stream) {
   const int total = a.m * a.n;
   if (a.size < 1 || a.size > kMaxNeighbourhoodSize) return hipErrorInvalidValue;
+  const dim3 grid((total + 255) / 256);
+  if (recorded<R>(a.size == 1 ? LaunchKind::kValidate : LaunchKind::kValidateWide, a, grid, 256)) return hipSuccess;
   if (a.size == 1)
-    hipLaunchKernelGGL((validate_kernel<R, false>), dim3((total + 255) / 256), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((validate_kernel<R, false>), grid, dim3(256), 0, stream, a);
   else
-    hipLaunchKernelGGL((validate_kernel<R, true>), dim3((total + 255) / 256), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((validate_kernel<R, true>), grid, dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_field_stats(const R* u, const R* v, int count, double* stats, hipStream_t stream) {
-  hipLaunchKernelGGL(field_stats_kernel<R>, dim3(1), dim3(1024), 0, stream, u, v, count, stats);
+  const FieldStatsArgs<R> a{u, v, count, stats};
+  if (recorded<R>(LaunchKind::kFieldStats, a, dim3(1), 1024)) return hipSuccess;
+  hipLaunchKernelGGL(field_stats_kernel<R>, dim3(1), dim3(1024), 0, stream, a);
   return hipGetLastError();
 }
 
@@ -1328,20 +1480,23 @@ hipError_t launch_neighbour_replace(const R* u_in, const R* v_in, R* u, R* v, co
                                     int size, int method, int exclude, hipStream_t stream) {
   const int total = m * n;
   if (size < 1 || size > kMaxNeighbourhoodSize) return hipErrorInvalidValue;
+  const NbrArgs<R> a{u_in, v_in, u, v, flags, m, n, size, method, exclude};
+  const dim3 grid((total + 255) / 256);
+  if (recorded<R>(size == 1 ? LaunchKind::kNbrReplace : LaunchKind::kNbrReplaceWide, a, grid, 256)) return hipSuccess;
   if (size == 1)
-    hipLaunchKernelGGL((neighbour_replace_kernel<R, false>), dim3((total + 255) / 256), dim3(256), 0, stream, u_in,
-                       v_in, u, v, flags, m, n, size, method, exclude);
+    hipLaunchKernelGGL((neighbour_replace_kernel<R, false>), grid, dim3(256), 0, stream, a);
   else
-    hipLaunchKernelGGL((neighbour_replace_kernel<R, true>), dim3((total + 255) / 256), dim3(256), 0, stream, u_in,
-                       v_in, u, v, flags, m, n, size, method, exclude);
+    hipLaunchKernelGGL((neighbour_replace_kernel<R, true>), grid, dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_reset_unresolved(R* u, R* v, const R* u0, const R* v0, const uint8_t* flags, uint8_t* nan_flags,
                                    int count, hipStream_t stream) {
-  hipLaunchKernelGGL(reset_unresolved_kernel<R>, dim3((count + 255) / 256), dim3(256), 0, stream, u, v, u0, v0,
-                     flags, nan_flags, count);
+  const ResetArgs<R> a{u, v, u0, v0, flags, nan_flags, count};
+  const dim3 grid((count + 255) / 256);
+  if (recorded<R>(LaunchKind::kReset, a, grid, 256)) return hipSuccess;
+  hipLaunchKernelGGL(reset_unresolved_kernel<R>, grid, dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
@@ -1351,18 +1506,25 @@ hipError_t launch_spring_replace(R* u, R* v, const uint8_t* flags, const int32_t
   if (size < 1 || size > kMaxNeighbourhoodSize || size > m || size > n) return hipErrorInvalidValue;
   // the kernel reads the outlier count on the device (graph-capturable, no
   // host sync) and returns at once when there is nothing to solve
-  hipLaunchKernelGGL(spring_lds_kernel<R>, dim3(1), dim3(kSpringLdsThreads), 0, stream, u, v, flags, count, m, n,
-                     size, w);
+  const SpringArgs<R> a{u, v, flags, count, m, n, size, w};
+  if (recorded<R>(LaunchKind::kSpring, a, dim3(1), kSpringLdsThreads)) return hipSuccess;
+  hipLaunchKernelGGL(spring_lds_kernel<R>, dim3(1), dim3(kSpringLdsThreads), 0, stream, a);
   return hipGetLastError();
 }
 
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int count, hipStream_t stream) {
-  hipLaunchKernelGGL(fill_i32_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, p, value, count);
+  const FillArgs a{p, value, count};
+  const dim3 grid((count + 255) / 256);
+  if (record_launch(LaunchKind::kFill, a, grid, 256)) return hipSuccess;
+  hipLaunchKernelGGL(fill_i32_kernel, grid, dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 template <typename R>
 hipError_t launch_scale(R* u, R* v, int count, R factor, hipStream_t stream) {
-  hipLaunchKernelGGL(scale_kernel<R>, dim3((count + 255) / 256), dim3(256), 0, stream, u, v, count, factor);
+  const ScaleArgs<R> a{u, v, count, factor};
+  const dim3 grid((count + 255) / 256);
+  if (recorded<R>(LaunchKind::kScale, a, grid, 256)) return hipSuccess;
+  hipLaunchKernelGGL(scale_kernel<R>, grid, dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 hipError_t launch_scale_div(double* u, double* v, int count, double mul, double div, hipStream_t stream) {
@@ -1371,8 +1533,35 @@ hipError_t launch_scale_div(double* u, double* v, int count, double mul, double 
 }
 template <typename R>
 hipError_t launch_copy(R* dst, const R* src, int count, hipStream_t stream) {
-  hipLaunchKernelGGL(copy_kernel<R>, dim3((count + 255) / 256), dim3(256), 0, stream, dst, src, count);
+  const CopyArgs<R> a{dst, src, count};
+  const dim3 grid((count + 255) / 256);
+  if (recorded<R>(LaunchKind::kCopy, a, grid, 256)) return hipSuccess;
+  hipLaunchKernelGGL(copy_kernel<R>, grid, dim3(256), 0, stream, a);
   return hipGetLastError();
+}
+
+thread_local std::vector<RecordedLaunch>* g_launch_record = nullptr;
+
+hipError_t launch_field_batch(LaunchKind kind, const void* args, const int4* grids, int n, int total_blocks,
+                              hipStream_t s) {
+  switch (kind) {
+    case LaunchKind::kFill: return launch_batched<FillBody, 256>(args, grids, n, total_blocks, s);
+    case LaunchKind::kCopy: return launch_batched<CopyBody<float>, 256>(args, grids, n, total_blocks, s);
+    case LaunchKind::kScale: return launch_batched<ScaleBody<float>, 256>(args, grids, n, total_blocks, s);
+    case LaunchKind::kGridInterp: return launch_batched<GridInterpBody<float>, 256>(args, grids, n, total_blocks, s);
+    case LaunchKind::kValidate:
+      return launch_batched<ValidateBody<float, false>, 256>(args, grids, n, total_blocks, s);
+    case LaunchKind::kValidateWide:
+      return launch_batched<ValidateBody<float, true>, 256>(args, grids, n, total_blocks, s);
+    case LaunchKind::kFieldStats: return launch_batched<FieldStatsBody<float>, 1024>(args, grids, n, total_blocks, s);
+    case LaunchKind::kSpring:
+      return launch_batched<SpringBody<float>, kSpringLdsThreads>(args, grids, n, total_blocks, s);
+    case LaunchKind::kNbrReplace: return launch_batched<NbrBody<float, false>, 256>(args, grids, n, total_blocks, s);
+    case LaunchKind::kNbrReplaceWide:
+      return launch_batched<NbrBody<float, true>, 256>(args, grids, n, total_blocks, s);
+    case LaunchKind::kReset: return launch_batched<ResetBody<float>, 256>(args, grids, n, total_blocks, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 // Up to kCopyMax device-to-device copies in one launch (blockIdx.y picks the
 // copy): 16-byte moves where destination, source and size allow, bytes

@@ -19,6 +19,7 @@
 #include <algorithm>
 
 #include "piv_kernels.h"
+#include "piv_record.h"
 #include "piv_wave.h"
 
 namespace piv {
@@ -259,16 +260,39 @@ __device__ __forceinline__ void gcv_publish(GcvState* gst, const GcvState& ls) {
 // taking every fourth element (each element is read once per workgroup, by
 // all lanes at the same address).  Partials part0[b][p][sum].
 template <typename R>
-__global__ __launch_bounds__(kGcvThreads) void gcv_grid_kernel(const R* Y, const double* lam, int N, GcvState* gst,
-                                                               double* part_base, double p_min, double p_max,
-                                                               const int32_t* skip) {
-  if (skip && *skip) return;
+struct GcvArgs {
+  const R* Y;
+  const double* lam;
+  int N;
+  GcvState* gst;
+  double* part_base;
+  double p_min, p_max;
+  const int32_t* skip;
+};
+template <typename R>
+struct GcvGridBody {
+  using Args = GcvArgs<R>;
+  static __device__ void run(const Args& a, VB vb);
+};
+template <typename R>
+__global__ __launch_bounds__(kGcvThreads) void gcv_grid_kernel(GcvArgs<R> a) {
+  GcvGridBody<R>::run(a, vb_native());
+}
+template <typename R>
+__device__ void GcvGridBody<R>::run(const Args& ga, VB vb) {
+  const R* Y = ga.Y;
+  const double* lam = ga.lam;
+  const int N = ga.N;
+  GcvState* gst = ga.gst;
+  double* part_base = ga.part_base;
+  const double p_min = ga.p_min, p_max = ga.p_max;
+  if (ga.skip && *ga.skip) return;
   __shared__ double red[kGcvThreads / 64][kGrid][4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   constexpr int WAVES = kGcvThreads / 64;
-  if (blockIdx.x == 0 && tid == 0) gst[3].nb[0] = (int)gridDim.x;
-  const int chunk = (N + gridDim.x - 1) / gridDim.x;
-  const int k0 = blockIdx.x * chunk, k1 = min(N, k0 + chunk);
+  if (vb.bx == 0 && tid == 0) gst[3].nb[0] = vb.gx;
+  const int chunk = (N + vb.gx - 1) / vb.gx;
+  const int k0 = vb.bx * chunk, k1 = min(N, k0 + chunk);
   const double s = pow(10.0, grid_p(lane, p_min, p_max));
   double acc[4] = {0, 0, 0, 0};
   for (int k = k0 + wave; k < k1; k += 4 * WAVES) {
@@ -298,7 +322,7 @@ __global__ __launch_bounds__(kGcvThreads) void gcv_grid_kernel(const R* Y, const
   double v = 0.0;
 #pragma unroll
   for (int w = 0; w < WAVES; ++w) v += red[w][p][q];
-  part_base[kGcvPart0 + (size_t)blockIdx.x * (kGrid * 4) + p * 4 + q] = v;
+  part_base[kGcvPart0 + (size_t)vb.bx * (kGrid * 4) + p * 4 + q] = v;
 }
 
 // Second stage of the forward DCT fused with the grid evaluation: workgroup
@@ -307,9 +331,35 @@ __global__ __launch_bounds__(kGcvThreads) void gcv_grid_kernel(const R* Y, const
 // takes p-value j, the four waves every fourth element) -> partials
 // part0[tile][p][sum], as gcv_grid_kernel would over the finished spectrum.
 template <typename R>
-__global__ __launch_bounds__(256) void dct2_gcv_kernel(const R* cm, const R* t0, R* t1, int m, int n,
-                                                       const double* lam, GcvState* gst, double* part_base,
-                                                       double p_min, double p_max) {
+struct Dct2GcvArgs {
+  const R* cm;
+  const R* t0;
+  R* t1;
+  int m, n;
+  const double* lam;
+  GcvState* gst;
+  double* part_base;
+  double p_min, p_max;
+};
+template <typename R>
+struct Dct2GcvBody {
+  using Args = Dct2GcvArgs<R>;
+  static __device__ void run(const Args& a, VB vb);
+};
+template <typename R>
+__global__ __launch_bounds__(256) void dct2_gcv_kernel(Dct2GcvArgs<R> a) {
+  Dct2GcvBody<R>::run(a, vb_native());
+}
+template <typename R>
+__device__ void Dct2GcvBody<R>::run(const Args& da, VB vb) {
+  const R* cm = da.cm;
+  const R* t0 = da.t0;
+  R* t1 = da.t1;
+  const int m = da.m, n = da.n;
+  const double* lam = da.lam;
+  GcvState* gst = da.gst;
+  double* part_base = da.part_base;
+  const double p_min = da.p_min, p_max = da.p_max;
   constexpr int KC = 64;
   __shared__ R As[16][KC + 1];
   __shared__ R Bs[2][KC][17];
@@ -317,9 +367,9 @@ __global__ __launch_bounds__(256) void dct2_gcv_kernel(const R* cm, const R* t0,
   __shared__ double red[4][kGrid][4];
   const int N = m * n;
   const int tid = threadIdx.x, ty = tid / 16, tx = tid % 16;
-  const int r0 = blockIdx.y * 16, c0 = blockIdx.x * 16;
-  const int tile = blockIdx.y * gridDim.x + blockIdx.x;
-  if (tile == 0 && tid == 0) gst[3].nb[0] = (int)(gridDim.x * gridDim.y);
+  const int r0 = vb.by * 16, c0 = vb.bx * 16;
+  const int tile = vb.by * vb.gx + vb.bx;
+  if (tile == 0 && tid == 0) gst[3].nb[0] = vb.gx * vb.gy;
   R acc0 = R(0), acc1 = R(0);
   for (int k0 = 0; k0 < m; k0 += KC) {
 #pragma unroll
@@ -380,22 +430,35 @@ __global__ __launch_bounds__(256) void dct2_gcv_kernel(const R* cm, const R* t0,
 // Newton step STEP: decide p from the previous launch's partials, then score
 // it over this workgroup's part of the spectrum.
 template <int STEP, typename R>
-__global__ __launch_bounds__(kGcvThreads) void gcv_newton_kernel(const R* Y, const double* lam, int N, GcvState* gst,
-                                                                 double* part_base, double p_min, double p_max,
-                                                                 const int32_t* skip) {
-  if (skip && *skip) return;
+struct NewtonBody {
+  using Args = GcvArgs<R>;
+  static __device__ void run(const Args& a, VB vb);
+};
+template <int STEP, typename R>
+__global__ __launch_bounds__(kGcvThreads) void gcv_newton_kernel(GcvArgs<R> a) {
+  NewtonBody<STEP, R>::run(a, vb_native());
+}
+template <int STEP, typename R>
+__device__ void NewtonBody<STEP, R>::run(const Args& ga, VB vb) {
+  const R* Y = ga.Y;
+  const double* lam = ga.lam;
+  const int N = ga.N;
+  GcvState* gst = ga.gst;
+  double* part_base = ga.part_base;
+  const double p_min = ga.p_min, p_max = ga.p_max;
+  if (ga.skip && *ga.skip) return;
   __shared__ GcvState ls;
   __shared__ double tot[kGrid][6];
   __shared__ double red[kGcvThreads / 64][6];
   gcv_decide_from<STEP - 1>(gst, part_base, N, p_min, p_max, &ls, tot);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (vb.bx == 0 && threadIdx.x == 0) {
     gcv_publish<STEP - 1>(gst, ls);
-    gst[3].nb[STEP] = (int)gridDim.x;
+    gst[3].nb[STEP] = vb.gx;
   }
   if (ls.done) return;  // uniform: the minimum is fixed
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int chunk = (N + gridDim.x - 1) / gridDim.x;
-  const int k0 = blockIdx.x * chunk, k1 = min(N, k0 + chunk);
+  const int chunk = (N + vb.gx - 1) / vb.gx;
+  const int k0 = vb.bx * chunk, k1 = min(N, k0 + chunk);
   const double s = pow(10.0, ls.pv[0]);
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int k = k0 + tid; k < k1; k += 4 * kGcvThreads) {
@@ -426,30 +489,43 @@ __global__ __launch_bounds__(kGcvThreads) void gcv_newton_kernel(const R* Y, con
   }
   __syncthreads();
   if (tid < 6)
-    part_base[(STEP == 1 ? kGcvPart1 : kGcvPart2) + (size_t)blockIdx.x * 6 + tid] =
+    part_base[(STEP == 1 ? kGcvPart1 : kGcvPart2) + (size_t)vb.bx * 6 + tid] =
         red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
 }
 
 // The final decision alone (for consumers that read st->p).
-__global__ __launch_bounds__(64) void gcv_final_kernel(GcvState* gst, double* part_base, int N, double p_min,
-                                                       double p_max, const int32_t* skip) {
-  if (skip && *skip) return;
-  __shared__ GcvState ls;
-  __shared__ double tot[kGrid][6];
-  gcv_decide_from<2>(gst, part_base, N, p_min, p_max, &ls, tot);
-  if (threadIdx.x == 0) gcv_publish<2>(gst, ls);
-}
+struct GcvFinalBody {
+  using Args = GcvArgs<float>;  // Y and lam unused
+  static __device__ void run(const Args& a, VB) {
+    if (a.skip && *a.skip) return;
+    __shared__ GcvState ls;
+    __shared__ double tot[kGrid][6];
+    gcv_decide_from<2>(a.gst, a.part_base, a.N, a.p_min, a.p_max, &ls, tot);
+    if (threadIdx.x == 0) gcv_publish<2>(a.gst, ls);
+  }
+};
+__global__ __launch_bounds__(64) void gcv_final_kernel(GcvArgs<float> a) { GcvFinalBody::run(a, vb_native()); }
 
 // The GCV search on the spectrum Y (two fields, 2N floats).  With
 // final_launch the decided p is in st->p when the launches end; without it
 // the consumer decides it (gcv_decide_from<2>).
+// Recording of a launch (FP32 only: batches are FP32).
+template <typename R, typename A>
+bool recorded(LaunchKind kind, const A& a, dim3 grid, int block) {
+  if constexpr (sizeof(R) != 4) return false;
+  else return record_launch(kind, a, grid, block);
+}
+
 template <typename R>
 void enqueue_gcv_newton(const R* Y, const double* lam, int N, GcvState* st, double* part, double p_min,
                         double p_max, const int32_t* skip, hipStream_t stream) {
   const dim3 et(kGcvThreads);
   const int nb = std::max(1, std::min(kGcvNewtonBlocks, (N + 4 * kGcvThreads - 1) / (4 * kGcvThreads)));
-  hipLaunchKernelGGL((gcv_newton_kernel<1, R>), dim3(nb), et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
-  hipLaunchKernelGGL((gcv_newton_kernel<2, R>), dim3(nb), et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
+  const GcvArgs<R> a{Y, lam, N, st, part, p_min, p_max, skip};
+  if (!recorded<R>(LaunchKind::kNewton1, a, dim3(nb), kGcvThreads))
+    hipLaunchKernelGGL((gcv_newton_kernel<1, R>), dim3(nb), et, 0, stream, a);
+  if (!recorded<R>(LaunchKind::kNewton2, a, dim3(nb), kGcvThreads))
+    hipLaunchKernelGGL((gcv_newton_kernel<2, R>), dim3(nb), et, 0, stream, a);
 }
 
 template <typename R>
@@ -457,10 +533,15 @@ void enqueue_gcv_search(const R* Y, const double* lam, int N, GcvState* st, doub
                         double p_max, const int32_t* skip, hipStream_t stream, bool final_launch = true) {
   const dim3 et(kGcvThreads);
   const int gb = std::max(1, std::min(kGcvBlocks, (N + 127) / 128));
-  hipLaunchKernelGGL((gcv_grid_kernel<R>), dim3(gb), et, 0, stream, Y, lam, N, st, part, p_min, p_max, skip);
+  const GcvArgs<R> a{Y, lam, N, st, part, p_min, p_max, skip};
+  if (!recorded<R>(LaunchKind::kGcvGrid, a, dim3(gb), kGcvThreads))
+    hipLaunchKernelGGL((gcv_grid_kernel<R>), dim3(gb), et, 0, stream, a);
   enqueue_gcv_newton(Y, lam, N, st, part, p_min, p_max, skip, stream);
-  if (final_launch)
-    hipLaunchKernelGGL(gcv_final_kernel, dim3(1), dim3(64), 0, stream, st, part, N, p_min, p_max, skip);
+  if (final_launch) {
+    const GcvArgs<float> f{nullptr, nullptr, N, st, part, p_min, p_max, skip};
+    if (!recorded<R>(LaunchKind::kGcvFinal, f, dim3(1), 64))
+      hipLaunchKernelGGL(gcv_final_kernel, dim3(1), dim3(64), 0, stream, f);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -491,19 +572,28 @@ struct GemmArgs {
 };
 
 template <typename R>
+struct GemmBody {
+  using Args = GemmArgs<R>;
+  static __device__ void run(const Args& g, VB vb);
+};
+template <typename R>
 __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs<R> g) {
+  GemmBody<R>::run(g, vb_native());
+}
+template <typename R>
+__device__ void GemmBody<R>::run(const Args& g, VB vb) {
   constexpr int KC = 64;  // K per LDS stage: four loads of A and four of B in flight per thread
   __shared__ R As[16][KC + 1];
   __shared__ R Bs[KC][17];
-  const int f = blockIdx.z;
+  const int f = vb.bz;
   const R* A = g.a[f];
   const R* B = g.b[f];
-  const int r0 = blockIdx.y * 16, c0 = blockIdx.x * 16;
+  const int r0 = vb.by * 16, c0 = vb.bx * 16;
   const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
   double sgam = 0.0;
   if (g.lam) {
     double p = g.p_fixed;
-    const bool first = blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0;
+    const bool first = vb.bx == 0 && vb.by == 0 && vb.bz == 0 && threadIdx.x == 0;
     if (g.gst) {
       __shared__ GcvState ls;
       __shared__ double tot[kGrid][6];
@@ -557,6 +647,7 @@ hipError_t gemm2(const R* a0, const R* a1, int lda, int ta, const R* b0, const R
   GemmArgs<R> g{{a0, a1}, {b0, b1}, {c0, c1}, lda, ldb, ldc, ta, tb, M, N, K, lam, gst, p_fixed, p_out,
                 gpart, gN, gpmin, gpmax};
   dim3 grid((N + 15) / 16, (M + 15) / 16, 2);
+  if (recorded<R>(LaunchKind::kGemm, g, grid, 256)) return hipSuccess;
   hipLaunchKernelGGL(gemm_small_kernel<R>, grid, dim3(256), 0, s, g);
   return hipGetLastError();
 }
@@ -565,6 +656,19 @@ hipError_t gemm2(const R* a0, const R* a1, int lda, int ta, const R* b0, const R
 
 constexpr size_t kGcvPartOffset = (kGcvSlots * sizeof(GcvState) + 255) & ~size_t(255);
 size_t smoothn_state_bytes() { return kGcvPartOffset + kGcvPartTotal * sizeof(double); }
+
+hipError_t launch_smoothn_batch(LaunchKind kind, const void* args, const int4* grids, int n, int total_blocks,
+                                hipStream_t s) {
+  switch (kind) {
+    case LaunchKind::kGemm: return launch_batched<GemmBody<float>, 256>(args, grids, n, total_blocks, s);
+    case LaunchKind::kDct2Gcv: return launch_batched<Dct2GcvBody<float>, 256>(args, grids, n, total_blocks, s);
+    case LaunchKind::kGcvGrid: return launch_batched<GcvGridBody<float>, kGcvThreads>(args, grids, n, total_blocks, s);
+    case LaunchKind::kNewton1: return launch_batched<NewtonBody<1, float>, kGcvThreads>(args, grids, n, total_blocks, s);
+    case LaunchKind::kNewton2: return launch_batched<NewtonBody<2, float>, kGcvThreads>(args, grids, n, total_blocks, s);
+    case LaunchKind::kGcvFinal: return launch_batched<GcvFinalBody, 64>(args, grids, n, total_blocks, s);
+    default: return hipErrorInvalidValue;
+  }
+}
 
 template <typename R>
 hipError_t launch_smoothn(R* u, R* v, int m, int n, double p_min, double p_max, double fixed_p, int auto_s,
@@ -583,8 +687,10 @@ hipError_t launch_smoothn(R* u, R* v, int m, int n, double p_min, double p_max, 
     // Newton steps; the inverse DCT below makes the final decision
     GcvState* st = reinterpret_cast<GcvState*>(w.state);
     double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(w.state) + kGcvPartOffset);
-    hipLaunchKernelGGL((dct2_gcv_kernel<R>), dim3((n + 15) / 16, (m + 15) / 16), dim3(256), 0, stream, w.cos_m,
-                       (const R*)w.t0, w.t1, m, n, w.lambda, st, part, p_min, p_max);
+    const Dct2GcvArgs<R> da{w.cos_m, (const R*)w.t0, w.t1, m, n, w.lambda, st, part, p_min, p_max};
+    const dim3 grid((n + 15) / 16, (m + 15) / 16);
+    if (!recorded<R>(LaunchKind::kDct2Gcv, da, grid, 256))
+      hipLaunchKernelGGL((dct2_gcv_kernel<R>), grid, dim3(256), 0, stream, da);
     enqueue_gcv_newton(w.t1, w.lambda, N, st, part, p_min, p_max, nullptr, stream);
   } else if (auto_s) {
     // larger fields: the plain second stage, then the grid evaluation over
@@ -1063,6 +1169,7 @@ int smoothn_masked_max_dim() { return kMaxDim; }
 template <typename R>
 hipError_t launch_smoothn_masked(R* u, R* v, int m, int n, double p_min, double p_max, double fixed_p, int auto_s,
                                  const SmoothnMaskedWorkT<R>& w, hipStream_t stream) {
+  if (g_launch_record) return hipErrorNotSupported;  // masked smoothing is not recordable
   constexpr int SH = kSH<R>, SW = kSW<R>;
   const int N = m * n;
   if (N < 2) return hipSuccess;  // gpu_smoothn.py:137-140
