@@ -225,7 +225,7 @@ def uneven_batch(n_pairs, reps):
     batch.set_frames(frames)
     t_build = time.perf_counter() - t0
     batch.run()                    # eager
-    choices = batch.tune(repeats=2)
+    choices = batch.tune(repeats=2, cache=False)
     batch.run()                    # capture + first replay
     t0 = time.perf_counter()
     for _ in range(reps):

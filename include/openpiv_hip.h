@@ -297,6 +297,15 @@ void piv_batch_destroy(piv_batch *batch);
 int piv_batch_set_frames(piv_batch *batch, const void *const *frames_a, const void *const *frames_b);
 int piv_batch_run(piv_batch *batch);
 int piv_batch_tune(piv_batch *batch, int repeats, int32_t *choices);
+/* Number of tuning choices of the batch (one per level plus one per
+ * correlation group), and applying a set of them -- e.g. from a tuning
+ * cache keyed by the batch's description -- without timing anything
+ * (PIV_E_INVALID on a wrong count or value; nothing changes then). */
+int piv_batch_choice_count(const piv_batch *batch, int32_t *count);
+int piv_batch_set_choices(piv_batch *batch, const int32_t *choices, int count);
+/* GPU architecture name of a device (e.g. "gfx950:sramecc+:xnack-"), for
+ * keying tuning caches.  No counterpart in the reference. */
+int piv_device_arch(int device, char *name, int len);
 int piv_batch_pair_shape(const piv_batch *batch, int pair, int32_t *m, int32_t *n);
 int piv_batch_get(const piv_batch *batch, int pair, float *u, float *v, uint8_t *val_init, uint8_t *val_final,
                   int outputs_on_device);

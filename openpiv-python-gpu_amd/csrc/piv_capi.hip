@@ -2219,6 +2219,43 @@ int piv_batch_tune(piv_batch* b, int repeats, int32_t* choices) {
   return PIV_OK;
 }
 
+int piv_batch_choice_count(const piv_batch* b, int32_t* count) {
+  if (!b || !count) return fail(PIV_E_INVALID, "batch: bad argument");
+  int c = 0;
+  for (const auto& L : b->levels) c += 1 + (int)L.groups.size();
+  *count = c;
+  return PIV_OK;
+}
+
+int piv_batch_set_choices(piv_batch* b, const int32_t* choices, int count) {
+  if (!b || !choices) return fail(PIV_E_INVALID, "batch: bad argument");
+  int32_t want = 0;
+  (void)piv_batch_choice_count(b, &want);
+  if (count != want) return fail(PIV_E_INVALID, "batch: %d choices given, the batch takes %d", count, want);
+  int ci = 0;
+  for (const auto& L : b->levels) {  // check everything before changing anything
+    const int t = choices[ci++];
+    if (t != 128 && t != 256 && t != 512) return fail(PIV_E_INVALID, "batch: deformation block width %d", t);
+    for (size_t g = 0; g < L.groups.size(); ++g, ++ci)
+      if (choices[ci] != 0 && choices[ci] != 1) return fail(PIV_E_INVALID, "batch: XCD flag %d", choices[ci]);
+  }
+  ci = 0;
+  for (auto& L : b->levels) {
+    L.deform_threads = choices[ci++];
+    for (auto& g : L.groups) g.xcd = choices[ci++];
+  }
+  batch_drop_graph(b);  // the next run captures the new shapes
+  return PIV_OK;
+}
+
+int piv_device_arch(int device, char* name, int len) {
+  if (!name || len < 1) return fail(PIV_E_INVALID, "bad argument");
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  snprintf(name, (size_t)len, "%s", prop.gcnArchName);
+  return PIV_OK;
+}
+
 int piv_batch_pair_shape(const piv_batch* b, int pair, int32_t* m, int32_t* n) {
   if (!b || pair < 0 || pair >= (int)b->plans.size()) return fail(PIV_E_INVALID, "batch: pair out of range");
   const PassBuffers& L = b->plans[pair]->passes.back();

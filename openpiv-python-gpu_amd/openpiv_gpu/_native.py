@@ -40,7 +40,8 @@ EXPORTED_SYMBOLS = (
     "piv_batch_pair_shape", "piv_batch_get", "piv_tiff_info", "piv_tiff_read", "piv_host_alloc", "piv_host_free",
     "piv_memcpy_async", "piv_stream_wait_event", "piv_plan_wait_event", "piv_plan_set_mask",
     "piv_plan_last_smoothing_iterations", "piv_stage_smoothn_masked", "piv_stage_correlate64",
-    "piv_plan_last_s2n", "piv_stage_replace_noalloc",
+    "piv_plan_last_s2n", "piv_stage_replace_noalloc", "piv_batch_choice_count", "piv_batch_set_choices",
+    "piv_device_arch",
 )
 
 # largest FFT size of the correlation kernels (planes above 128 points live in
@@ -187,6 +188,9 @@ def _bind(lib):
         "piv_batch_set_frames": (i32, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
         "piv_batch_run": (i32, [vp]),
         "piv_batch_tune": (i32, [vp, i32, vp]),
+        "piv_batch_choice_count": (i32, [vp, vp]),
+        "piv_batch_set_choices": (i32, [vp, vp, i32]),
+        "piv_device_arch": (i32, [i32, vp, i32]),
         "piv_batch_pair_shape": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
         "piv_batch_get": (i32, [vp, i32, vp, vp, vp, vp, i32]),
         "piv_tiff_info": (i32, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),

@@ -5,12 +5,18 @@ gpu_process.py:564-647).  Each pair is described by the keyword arguments a
 ``PIVGPU`` would take; the batch runs every pass level of all pairs as one
 deformation launch and one correlation launch per FFT size, then each pair's
 validation / replacement / smoothing, with the whole loop captured in one
-hipGraph.  ``tune()`` picks launch shapes per level by timing them.
+hipGraph.  ``tune()`` picks launch shapes per level by timing them, and
+keeps the choices in a tuning cache keyed by the GPU architecture and the
+batch's description, so a later batch of the same description applies them
+without timing.
 """
 
 from __future__ import annotations
 
 import ctypes
+import hashlib
+import json
+import os
 
 import numpy as np
 
@@ -31,6 +37,10 @@ class PIVBatch:
         self.handle = h
         self.n = len(self.pivs)
         self._frames = None
+        self._key_src = {"pairs": [[list(map(int, shape)), int(mss), _jsonable(kw)] for shape, mss, kw in pairs],
+                         "in_dtype": np.dtype(in_dtype).name}
+        self._device = device
+        self.tuning_source = None
 
     def set_frames(self, frames):
         """``frames``: sequence of ``(frame_a, frame_b)`` device buffers (kept
@@ -45,10 +55,46 @@ class PIVBatch:
     def run(self):
         _native.check(self._lib.piv_batch_run(self.handle))
 
-    def tune(self, repeats: int = 3):
-        """Times the launch-shape candidates; returns the choices per level."""
-        out = np.zeros(64, np.int32)
+    def choice_count(self) -> int:
+        c = ctypes.c_int32()
+        _native.check(self._lib.piv_batch_choice_count(self.handle, ctypes.byref(c)))
+        return c.value
+
+    def set_choices(self, choices):
+        arr = np.ascontiguousarray(choices, np.int32)
+        _native.check(self._lib.piv_batch_set_choices(self.handle, arr.ctypes.data, int(arr.size)))
+
+    def tuning_key(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        _native.check(self._lib.piv_device_arch(self._device, buf, 256))
+        src = dict(self._key_src, arch=buf.value.decode(), choices=self.choice_count())
+        return hashlib.sha1(json.dumps(src, sort_keys=True).encode()).hexdigest()
+
+    def tune(self, repeats: int = 3, cache=True):
+        """Launch shapes per level: from the tuning cache when it holds this
+        batch's key, else timed (piv_batch_tune) and stored.  ``cache``:
+        True for the default file (``$OPENPIV_HIP_TUNE_CACHE``, else
+        ``~/.cache/openpiv_hip/batch_tuning.json``), a path, or False.
+        Returns the choices (per level: deformation block width, then each
+        correlation group's XCD-banded flag); ``tuning_source`` says "cache"
+        or "timed"."""
+        path = _cache_path(cache)
+        key = self.tuning_key() if path else None
+        n = self.choice_count()
+        if path:
+            entry = _read_cache(path).get(key)
+            if entry is not None and len(entry) == n:
+                self.set_choices(entry)
+                self.tuning_source = "cache"
+                return np.asarray(entry, np.int32)
+        out = np.zeros(max(n, 1), np.int32)
         _native.check(self._lib.piv_batch_tune(self.handle, repeats, out.ctypes.data))
+        out = out[:n]
+        self.tuning_source = "timed"
+        if path:
+            data = _read_cache(path)
+            data[key] = [int(c) for c in out]
+            _write_cache(path, data)
         return out
 
     def field_shape(self, i):
@@ -74,6 +120,49 @@ class PIVBatch:
                 self.handle = ctypes.c_void_p()
         except Exception:
             pass
+
+
+def _jsonable(kw):
+    out = {}
+    for k, v in sorted(dict(kw).items()):
+        if isinstance(v, np.ndarray):
+            v = hashlib.sha1(np.ascontiguousarray(v).tobytes()).hexdigest()
+        elif isinstance(v, (tuple, list)):
+            v = [x if isinstance(x, (int, float, str, bool)) or x is None else repr(x) for x in v]
+        elif not (isinstance(v, (int, float, str, bool)) or v is None):
+            v = repr(v)
+        out[k] = v
+    return out
+
+
+def _cache_path(cache):
+    if cache is False or cache is None:
+        return None
+    if cache is True:
+        env = os.environ.get("OPENPIV_HIP_TUNE_CACHE")
+        if env:
+            return env
+        return os.path.join(os.path.expanduser("~"), ".cache", "openpiv_hip", "batch_tuning.json")
+    return os.fspath(cache)
+
+
+def _read_cache(path):
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data if isinstance(data, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def _write_cache(path, data):
+    d = os.path.dirname(path)
+    if d:
+        os.makedirs(d, exist_ok=True)
+    tmp = f"{path}.{os.getpid()}.tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
 
 
 def uneven_pair_spec(seed: int, sizes=((512, 512), (768, 1024), (2048, 2048), (2048, 3072), (4096, 4096))):

@@ -206,3 +206,21 @@ def test_gpu_piv_alias_and_frame_assertion():
     assert piv.field_shape == (7, 7)
     assert not piv.val_locations.any() and not piv.outliers.any()
     assert piv.field_mask.shape == (7, 7) and not piv.field_mask.any()
+
+
+def test_batch_tuning_cache_helpers(tmp_path, monkeypatch):
+    """The tuning cache's path rules, JSON round trip (atomic replace) and the
+    stable description of pair keyword arguments."""
+    from openpiv_gpu import batch as B
+    monkeypatch.setenv("OPENPIV_HIP_TUNE_CACHE", str(tmp_path / "x.json"))
+    assert B._cache_path(True) == str(tmp_path / "x.json")
+    assert B._cache_path(False) is None and B._cache_path(tmp_path / "y.json") == str(tmp_path / "y.json")
+    p = str(tmp_path / "sub" / "c.json")
+    assert B._read_cache(p) == {}
+    B._write_cache(p, {"k": [256, 1]})
+    assert B._read_cache(p) == {"k": [256, 1]}
+    (tmp_path / "bad.json").write_text("not json")
+    assert B._read_cache(str(tmp_path / "bad.json")) == {}
+    kw = dict(search_size_iters=(1, 1), overlap_ratio=0.5, mask=np.ones((4, 4), bool))
+    j1, j2 = B._jsonable(kw), B._jsonable(dict(reversed(list(kw.items()))))
+    assert j1 == j2 and isinstance(j1["mask"], str) and j1["search_size_iters"] == [1, 1]

@@ -65,7 +65,7 @@ def uneven_runs():
     runs = []
     for rep in range(4):   # eager, tuned (eager again), capture + replay, replay
         if rep == 1:
-            batch.tune(repeats=2)
+            batch.tune(repeats=2, cache=False)
         batch.run()
         runs.append([batch.result(i) for i in range(N_PAIRS)])
         print(f"batch run {rep} done", flush=True)
