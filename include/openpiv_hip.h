@@ -136,11 +136,19 @@ int piv_run(piv_plan *plan, const void *frame_a, const void *frame_b, int frames
  * piv_gpu call is synchronous; for callers that stream pairs through one
  * plan).  Runs queued back to back execute in order; inputs must stay valid
  * and outputs and diagnostics are ready only after piv_plan_synchronize.
- * Plans with a learned predictor return PIV_E_INVALID (use piv_run). */
+ * With a learned predictor the queued run is the learned path; the
+ * fallback decision (gpu_process.py:619-628 has no counterpart; the plan's
+ * max_reject_fraction) waits for piv_plan_synchronize, which reruns the full
+ * schedule on the frames of every queued pair above the threshold, so its
+ * frames must stay valid until then too. */
 int piv_run_async(piv_plan *plan, const void *frame_a, const void *frame_b, void *u, void *v, uint8_t *val_init,
                   uint8_t *val_final);
-/* Waits for every run queued on the plan's stream. */
+/* Waits for every run queued on the plan's stream (and runs the fallbacks of
+ * queued learned runs). */
 int piv_plan_synchronize(piv_plan *plan);
+/* Learned runs of the plan so far and how many of them fell back to the full
+ * schedule (synchronous and queued runs; queued ones once synchronised). */
+int piv_plan_learned_counts(const piv_plan *plan, int64_t *runs, int64_t *fallbacks);
 
 /* Diagnostics of the last run: smoothing parameter log10(s) chosen after
  * pass k (NaN when no smoothing ran), and outlier counts before/after

@@ -416,6 +416,20 @@ struct piv_plan {
   int lruns = 0;
   int32_t last_fallback = 0;
   int32_t last_rejected = 0;
+  // learned runs queued by piv_run_async: their outputs and frames, and the
+  // rejection counts copied to pinned host slots on the stream; the
+  // fallbacks run in piv_plan_synchronize
+  struct Pending {
+    const void* fa;
+    const void* fb;
+    void* u;
+    void* v;
+    uint8_t* vi;
+    uint8_t* vf;
+  };
+  std::vector<Pending> pending;
+  int32_t* rej_host = nullptr;  // kMaxPendingLearned slots
+  int64_t learned_runs = 0, learned_fallbacks = 0;
   // frame mask (piv_plan_set_mask)
   uint8_t* frame_mask = nullptr;
   MaskedSmoothBufs msb{};
@@ -760,6 +774,7 @@ void release(piv_plan* P) {
   if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
   if (P->graph) (void)hipGraphDestroy(P->graph);
   drop_learned_graph(P);
+  if (P->rej_host) (void)hipHostFree(P->rej_host);
   if (P->net_in) (void)hipFree(P->net_in);
   if (P->net_scratch) (void)hipFree(P->net_scratch);
   for (auto& B : P->passes) {
@@ -986,8 +1001,15 @@ int piv_plan_create(int device, const piv_plan_desc* desc, piv_plan** out) {
 
 void piv_plan_destroy(piv_plan* plan) { release(plan); }
 
+// Learned runs queued per plan between synchronisations (the pinned slots of
+// their rejection counts).
+constexpr int kMaxPendingLearned = 1024;
+
+static int copy_outputs(piv_plan* P, void* u, void* v, uint8_t* val_init, uint8_t* val_final, int outputs_on_device);
+static int resolve_pending(piv_plan* P);
+
 static int run_impl(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on_device, void* u, void* v,
-                    uint8_t* val_init, uint8_t* val_final, int outputs_on_device, bool sync) {
+                    uint8_t* val_init, uint8_t* val_final, int outputs_on_device, bool sync, bool defer = false) {
   if (!P || !frame_a || !frame_b) return fail(PIV_E_INVALID, "null argument");
   HIP_TRY(hipSetDevice(P->device));
   hipStream_t s = P->stream;
@@ -1021,6 +1043,17 @@ static int run_impl(piv_plan* P, const void* frame_a, const void* frame_b, int f
   // later runs replay a captured graph when requested.
   P->last_fallback = 0;
   P->last_rejected = 0;
+  if (P->net && defer) {
+    // queued learned run: the rejection count goes to a pinned slot on the
+    // stream; piv_plan_synchronize runs the fallbacks
+    int r = run_captured(P, enqueue_learned, &P->lgraph, &P->lgraph_exec, &P->lruns);
+    if (r) return r;
+    const size_t slot = P->pending.size();
+    HIP_TRY(hipMemcpyAsync(P->rej_host + slot, P->counters + 2 * (P->passes.size() - 1), sizeof(int32_t),
+                           hipMemcpyDeviceToHost, s));
+    P->pending.push_back(piv_plan::Pending{frame_a, frame_b, u, v, val_init, val_final});
+    return copy_outputs(P, u, v, val_init, val_final, outputs_on_device);
+  }
   if (P->net) {
     int r = run_captured(P, enqueue_learned, &P->lgraph, &P->lgraph_exec, &P->lruns);
     if (r) return r;
@@ -1030,8 +1063,10 @@ static int run_impl(piv_plan* P, const void* frame_a, const void* frame_b, int f
                            hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     P->last_rejected = rejected;
+    P->learned_runs++;
     if ((double)rejected > (double)P->max_reject * L.N) {
       P->last_fallback = 1;
+      P->learned_fallbacks++;
       CopyList cl{};  // counters of the full-schedule run start at zero
       cl.c[0] = CopySpec{P->counters, nullptr, 2 * P->passes.size() * sizeof(int32_t)};
       cl.c[1] = CopySpec{P->lb.it_counts, nullptr, P->passes.size() * PIV_MAX_REPLACING_ITERS * sizeof(int32_t)};
@@ -1043,6 +1078,14 @@ static int run_impl(piv_plan* P, const void* frame_a, const void* frame_b, int f
     int r = run_captured(P, enqueue, &P->graph, &P->graph_exec, &P->runs);
     if (r) return r;
   }
+  int r = copy_outputs(P, u, v, val_init, val_final, outputs_on_device);
+  if (r) return r;
+  if (sync) HIP_TRY(hipStreamSynchronize(s));
+  return PIV_OK;
+}
+
+static int copy_outputs(piv_plan* P, void* u, void* v, uint8_t* val_init, uint8_t* val_final, int outputs_on_device) {
+  hipStream_t s = P->stream;
   const PassBuffers& L = P->passes.back();
   if (outputs_on_device) {  // one launch for all requested outputs
     CopyList cl{};
@@ -1058,21 +1101,68 @@ static int run_impl(piv_plan* P, const void* frame_a, const void* frame_b, int f
     if (val_init) HIP_TRY(hipMemcpyAsync(val_init, L.flags, L.N, hipMemcpyDeviceToHost, s));
     if (val_final) HIP_TRY(hipMemcpyAsync(val_final, P->val_final, L.N, hipMemcpyDeviceToHost, s));
   }
-  if (sync) HIP_TRY(hipStreamSynchronize(s));
+  return PIV_OK;
+}
+
+// After the stream has drained: every queued learned run whose last-pass
+// test rejected more than max_reject of the vectors reruns the full schedule
+// on its frames (still valid by the async contract) into its outputs.
+static int resolve_pending(piv_plan* P) {
+  if (P->pending.empty()) return PIV_OK;
+  std::vector<piv_plan::Pending> q;
+  q.swap(P->pending);
+  const PassBuffers& L = P->passes.back();
+  hipStream_t s = P->stream;
+  for (size_t i = 0; i < q.size(); ++i) {
+    const int32_t rejected = P->rej_host[i];
+    P->learned_runs++;
+    P->last_rejected = rejected;
+    P->last_fallback = 0;
+    if ((double)rejected > (double)P->max_reject * L.N) {
+      P->last_fallback = 1;
+      P->learned_fallbacks++;
+      const piv_plan::Pending& e = q[i];
+      CopyList cl{};
+      int nc = 0;
+      cl.c[nc++] = CopySpec{P->counters, nullptr, 2 * P->passes.size() * sizeof(int32_t)};
+      cl.c[nc++] = CopySpec{P->lb.it_counts, nullptr, P->passes.size() * PIV_MAX_REPLACING_ITERS * sizeof(int32_t)};
+      cl.c[nc++] = CopySpec{P->frame_a, e.fa, P->frame_bytes};
+      cl.c[nc++] = CopySpec{P->frame_b, e.fb, P->frame_bytes};
+      HIP_TRY(launch_copy_multi(cl, nc, s));
+      int r = run_captured(P, enqueue, &P->graph, &P->graph_exec, &P->runs);
+      if (!r) r = copy_outputs(P, e.u, e.v, e.vi, e.vf, 1);
+      if (r) return r;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(s));
   return PIV_OK;
 }
 
 int piv_run(piv_plan* P, const void* frame_a, const void* frame_b, int frames_on_device, void* u, void* v,
             uint8_t* val_init, uint8_t* val_final, int outputs_on_device) {
+  if (P && !P->pending.empty()) {  // queued learned runs first, fallbacks included
+    int r = piv_plan_synchronize(P);
+    if (r) return r;
+  }
   return run_impl(P, frame_a, frame_b, frames_on_device, u, v, val_init, val_final, outputs_on_device, true);
 }
 
 int piv_run_async(piv_plan* P, const void* frame_a, const void* frame_b, void* u, void* v, uint8_t* val_init,
                   uint8_t* val_final) {
   if (!P) return fail(PIV_E_INVALID, "null plan");
-  if (P->net)
-    return fail(PIV_E_INVALID, "piv_run_async: a plan with a learned predictor decides its fallback on the host; "
-                               "use piv_run");
+  if (P->net) {
+    HIP_TRY(hipSetDevice(P->device));
+    if (!P->rej_host && hipHostMalloc((void**)&P->rej_host, kMaxPendingLearned * sizeof(int32_t)) != hipSuccess) {
+      P->rej_host = nullptr;
+      return fail(PIV_E_NOMEM, "pinned allocation failed");
+    }
+    if ((int)P->pending.size() >= kMaxPendingLearned) {  // make room: resolve what is queued
+      HIP_TRY(hipStreamSynchronize(P->stream));
+      int r = resolve_pending(P);
+      if (r) return r;
+    }
+    return run_impl(P, frame_a, frame_b, 1, u, v, val_init, val_final, 1, false, true);
+  }
   return run_impl(P, frame_a, frame_b, 1, u, v, val_init, val_final, 1, false);
 }
 
@@ -1080,6 +1170,13 @@ int piv_plan_synchronize(piv_plan* P) {
   if (!P) return fail(PIV_E_INVALID, "null plan");
   HIP_TRY(hipSetDevice(P->device));
   HIP_TRY(hipStreamSynchronize(P->stream));
+  return resolve_pending(P);
+}
+
+int piv_plan_learned_counts(const piv_plan* P, int64_t* runs, int64_t* fallbacks) {
+  if (!P) return fail(PIV_E_INVALID, "null plan");
+  if (runs) *runs = P->learned_runs;
+  if (fallbacks) *fallbacks = P->learned_fallbacks;
   return PIV_OK;
 }
 
@@ -1087,6 +1184,10 @@ int piv_plan_set_predictor(piv_plan* P, piv_cnn* net, int first_pass, float max_
   if (!P) return fail(PIV_E_INVALID, "null plan");
   HIP_TRY(hipSetDevice(P->device));
   HIP_TRY(hipStreamSynchronize(P->stream));
+  {
+    const int r = resolve_pending(P);  // queued runs finish under the predictor they were queued with
+    if (r) return r;
+  }
   drop_learned_graph(P);
   if (!net) {
     P->net = nullptr;

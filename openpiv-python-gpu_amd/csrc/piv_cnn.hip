@@ -17,6 +17,7 @@
 // nearest-neighbour x2 upsampling of one tensor concatenated with a skip
 // tensor, so neither the upsampled nor the concatenated tensor exists.
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <math.h>
 #include <stdint.h>
 
@@ -96,6 +97,30 @@ __device__ __forceinline__ bool in_offsets(const InView& v, int n, int yi, int x
   *o1 = (((long)n * v.hi + yi) * v.wi + xi) * v.c1;
   return true;
 }
+// 32-bit forms for the forward kernels (the launcher checks that every pixel
+// index and element offset fits): 64-bit index arithmetic and division
+// emulation were most of a small layer's instructions.
+__device__ __forceinline__ Pix pix_of32(int p, int total, int h, int w) {
+  Pix r;
+  r.ok = p < total;
+  const int q = r.ok ? p : 0;
+  const int t = q / w;
+  r.x = q - t * w;
+  r.n = t / h;
+  r.y = t - r.n * h;
+  return r;
+}
+__device__ __forceinline__ bool in_offsets32(const InView& v, int n, int yi, int xi, int* o0, int* o1) {
+  if (yi < 0 || yi >= v.hi || xi < 0 || xi >= v.wi) return false;
+  if (v.up0) {
+    const int h0 = v.hi >> 1, w0 = v.wi >> 1;
+    *o0 = ((n * h0 + (yi >> 1)) * w0 + (xi >> 1)) * v.c0;
+  } else {
+    *o0 = ((n * v.hi + yi) * v.wi + xi) * v.c0;
+  }
+  *o1 = ((n * v.hi + yi) * v.wi + xi) * v.c1;
+  return true;
+}
 __device__ __forceinline__ float in_value(const InView& v, bool ok, long o0, long o1, int ci) {
   if (!ok) return 0.f;
   return ci < v.c0 ? bf2f(v.s0[o0 + ci]) : bf2f(v.s1[o1 + ci - v.c0]);
@@ -114,13 +139,13 @@ __global__ __launch_bounds__(64 * NW) void conv_fwd_kernel(ConvGeom g, InView in
   __shared__ float red[NW - 1][TM][4][64][4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = lane & 15, gq = lane >> 4;
-  const long total = (long)g.batch * g.ho * g.wo;
-  const long pbase = (long)blockIdx.x * 64;
+  const int total = g.batch * g.ho * g.wo;
+  const int pbase = blockIdx.x * 64;
   const int cobase = blockIdx.y * (16 * TM);
   const int ci = in.c0 + in.c1, nc = ci / 16;
   Pix px[4];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) px[t] = pix_of(pbase + t * 16 + r, total, g.ho, g.wo);
+  for (int t = 0; t < 4; ++t) px[t] = pix_of32(pbase + t * 16 + r, total, g.ho, g.wo);
   int coA[TM];
   bool okA[TM];
 #pragma unroll
@@ -140,13 +165,13 @@ __global__ __launch_bounds__(64 * NW) void conv_fwd_kernel(ConvGeom g, InView in
     const int ky = tap / 3, kx = tap - 3 * ky;
     const int cc = c + 4 * gq;
 #pragma unroll
-    for (int m = 0; m < TM; ++m) a[m] = okA[m] ? load_bf4(w + ((long)coA[m] * 9 + tap) * ci + cc) : zero4;
+    for (int m = 0; m < TM; ++m) a[m] = okA[m] ? load_bf4(w + ((coA[m] * 9 + tap) * ci + cc)) : zero4;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      long o0, o1;
-      const bool ok = px[t].ok && in_offsets(in, px[t].n, px[t].y * g.stride + ky - 1,
-                                             px[t].x * g.stride + kx - 1, &o0, &o1);
-      b[t] = !ok ? zero4 : (c < in.c0 ? load_bf4(in.s0 + o0 + cc) : load_bf4(in.s1 + o1 + cc - in.c0));
+      int o0 = 0, o1 = 0;
+      const bool ok = px[t].ok && in_offsets32(in, px[t].n, px[t].y * g.stride + ky - 1,
+                                               px[t].x * g.stride + kx - 1, &o0, &o1);
+      b[t] = !ok ? zero4 : (c < in.c0 ? load_bf4(in.s0 + (o0 + cc)) : load_bf4(in.s1 + (o1 + cc - in.c0)));
     }
   };
   // chunks kc = wv, wv + NW, ... of this wave, in groups of G whose loads are
@@ -188,7 +213,7 @@ __global__ __launch_bounds__(64 * NW) void conv_fwd_kernel(ConvGeom g, InView in
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     if (!px[t].ok) continue;
-    const long p = pbase + t * 16 + r;
+    const int p = pbase + t * 16 + r;
 #pragma unroll
     for (int m = 0; m < TM; ++m) {
       const int co0 = cobase + m * 16 + 4 * gq;
@@ -206,8 +231,8 @@ __global__ __launch_bounds__(64 * NW) void conv_fwd_kernel(ConvGeom g, InView in
         } else if (out_mode == kOutF32) {
           static_cast<float*>(out)[p * g.co + co] = val;
         } else {  // per-image planar [n][co][y][x]
-          const long hw = (long)g.ho * g.wo;
-          static_cast<float*>(out)[((long)px[t].n * g.co + co) * hw + p % hw] = val;
+          const int hw = g.ho * g.wo;
+          static_cast<float*>(out)[(px[t].n * g.co + co) * hw + (px[t].y * g.wo + px[t].x)] = val;
         }
       }
     }
@@ -223,14 +248,14 @@ __global__ __launch_bounds__(256) void conv_fwd_small_kernel(ConvGeom g, InView 
                                                              int relu) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = lane & 15, gq = lane >> 4;
-  const long total = (long)g.batch * g.ho * g.wo;
-  const long pbase = ((long)blockIdx.x * 4 + wv) * 64;
+  const int total = g.batch * g.ho * g.wo;
+  const int pbase = (blockIdx.x * 4 + wv) * 64;
   if (pbase >= total) return;
   const int cobase = blockIdx.y * 32;
   const int ci = CI > 0 ? CI : in.c0 + in.c1, K = 9 * ci;
   Pix px[4];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) px[t] = pix_of(pbase + t * 16 + r, total, g.ho, g.wo);
+  for (int t = 0; t < 4; ++t) px[t] = pix_of32(pbase + t * 16 + r, total, g.ho, g.wo);
   int coA[2];
   bool okA[2];
 #pragma unroll
@@ -250,7 +275,7 @@ __global__ __launch_bounds__(256) void conv_fwd_small_kernel(ConvGeom g, InView 
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int k = k0 + 4 * gq + e;
-        a[m][e] = (okA[m] && k < K) ? (short)w[(long)coA[m] * K + k] : (short)0;
+        a[m][e] = (okA[m] && k < K) ? (short)w[coA[m] * K + k] : (short)0;
       }
 #pragma unroll
     for (int t = 0; t < 4; ++t)
@@ -260,9 +285,9 @@ __global__ __launch_bounds__(256) void conv_fwd_small_kernel(ConvGeom g, InView 
         float val = 0.f;
         if (k < K && px[t].ok) {
           const int tap = k / ci, cch = k - tap * ci;
-          long o0, o1;
-          const bool ok = in_offsets(in, px[t].n, px[t].y * g.stride + tap / 3 - 1,
-                                     px[t].x * g.stride + tap % 3 - 1, &o0, &o1);
+          int o0 = 0, o1 = 0;
+          const bool ok = in_offsets32(in, px[t].n, px[t].y * g.stride + tap / 3 - 1,
+                                       px[t].x * g.stride + tap % 3 - 1, &o0, &o1);
           val = in_value(in, ok, o0, o1, cch);
         }
         b[t][e] = (short)f2bf(val);
@@ -275,7 +300,7 @@ __global__ __launch_bounds__(256) void conv_fwd_small_kernel(ConvGeom g, InView 
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     if (!px[t].ok) continue;
-    const long p = pbase + t * 16 + r;
+    const int p = pbase + t * 16 + r;
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
       const int co0 = cobase + m * 16 + 4 * gq;
@@ -290,8 +315,8 @@ __global__ __launch_bounds__(256) void conv_fwd_small_kernel(ConvGeom g, InView 
         } else if (out_mode == kOutF32) {
           static_cast<float*>(out)[p * g.co + co] = val;
         } else {
-          const long hw = (long)g.ho * g.wo;
-          static_cast<float*>(out)[((long)px[t].n * g.co + co) * hw + p % hw] = val;
+          const int hw = g.ho * g.wo;
+          static_cast<float*>(out)[(px[t].n * g.co + co) * hw + (px[t].y * g.wo + px[t].x)] = val;
         }
       }
     }
@@ -718,6 +743,10 @@ hipError_t launch_conv_fwd(const ConvGeom& g, const uint16_t* s0, int c0, int up
                            const uint16_t* w, const float* bias, void* out, int out_mode, int relu, hipStream_t s) {
   const InView in{s0, s1, c0, c1, up0, g.hi, g.wi};
   const long pixels = (long)g.batch * g.ho * g.wo;
+  // the forward kernels index in 32 bits
+  if ((long)g.batch * g.hi * g.wi * (c0 + c1 + 1) >= INT_MAX || (pixels + 256) * (g.co + 32) >= INT_MAX ||
+      (long)g.co * 9 * (c0 + c1) >= INT_MAX)
+    return hipErrorInvalidValue;
   if (c0 % 16 == 0 && c1 % 16 == 0) {
     // 32-row tiles once there are enough 64-pixel columns to fill the chip
     const long cols = (pixels + 63) / 64;

@@ -41,7 +41,7 @@ EXPORTED_SYMBOLS = (
     "piv_memcpy_async", "piv_stream_wait_event", "piv_plan_wait_event", "piv_plan_set_mask",
     "piv_plan_last_smoothing_iterations", "piv_stage_smoothn_masked", "piv_stage_correlate64",
     "piv_plan_last_s2n", "piv_stage_replace_noalloc", "piv_batch_choice_count", "piv_batch_set_choices",
-    "piv_device_arch",
+    "piv_device_arch", "piv_plan_learned_counts",
 )
 
 # largest FFT size of the correlation kernels (planes above 128 points live in
@@ -149,6 +149,7 @@ def _bind(lib):
         "piv_run": (i32, [vp, vp, vp, i32, vp, vp, vp, vp, i32]),
         "piv_run_async": (i32, [vp, vp, vp, vp, vp, vp, vp]),
         "piv_plan_synchronize": (i32, [vp]),
+        "piv_plan_learned_counts": (i32, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
         "piv_plan_last_smoothing": (i32, [vp, i32, ctypes.POINTER(f64)]),
         "piv_plan_last_counts": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
         "piv_stage_correlate": (i32, [vp, vp, i32, i32, i32, ctypes.POINTER(PassDesc), vp, vp, vp, vp, vp, vp, vp,
@@ -450,6 +451,12 @@ class Plan:
         self._net = net  # keep the network alive while the plan uses it
         check(self._lib.piv_plan_set_predictor(self.handle, None if net is None else net.handle, int(first_pass),
                                                float(max_reject_fraction)))
+
+    def learned_counts(self):
+        """(learned runs, fallbacks to the full schedule) of the plan so far."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        check(self._lib.piv_plan_learned_counts(self.handle, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def last_fallback(self):
         """``(fell_back, rejected)`` of the last run with a predictor."""

@@ -191,3 +191,40 @@ def test_learned_path_falls_back_to_full_schedule():
         # the full-schedule run's counters are its own (cleared after the
         # learned attempt), equal to the plain plan's
         assert [learned.outlier_counts(k) for k in range(2)] == counts
+
+
+def test_learned_runs_queued_async_equal_synchronous_runs():
+    """piv_run_async on learned plans: pairs queued back to back, the
+    fallback decided at piv_plan_synchronize.  Every queued pair equals its
+    synchronous run bit for bit -- the learned result when the network's
+    field passes the 5 % test, the full schedule's when it does not -- and
+    the learned-run counters count both."""
+    from openpiv_gpu import _native
+    from openpiv_gpu import synthetic as S
+    from openpiv_gpu.gpu_process import PIVGPU
+    shape = (512, 512)
+    kw = dict(search_size_iters=(1, 1), overlap_ratio=0.5, median_tol=2, dtype_f="float32")
+    pairs = [S.make_pair(shape, S.uniform_displacement(2.35, -1.6), S.ParticleModel(), 30 + i) for i in range(4)]
+    frames = _native.DeviceBuffer.from_array(np.stack([np.stack(p) for p in pairs]).astype(np.uint8))
+    for net, fell in ((_constant_predictor_net(2.3, -1.5), False), (_constant_predictor_net(25.0, 25.0), True)):
+        sync = PIVGPU(shape, 16, predictor=net, max_reject_fraction=0.05, **kw)
+        asy = PIVGPU(shape, 16, predictor=net, max_reject_fraction=0.05, **kw)
+        ps, pa = sync._get_plan(_native.PIV_U8), asy._get_plan(_native.PIV_U8)
+        m, n = sync.piv_fields[-1].field_shape
+        outs = _native.DeviceBuffer((2, len(pairs), 2, m, n), np.float32)
+        flags = _native.DeviceBuffer((2, len(pairs), 2, m, n), np.uint8)
+        for rep in range(2):  # eager, then graph replays
+            for i in range(len(pairs)):
+                f, o, fl = frames.view(i), outs.view(0).view(i), flags.view(0).view(i)
+                ps.run(f.view(0).value, f.view(1).value, True, o.view(0).value, o.view(1).value, fl.view(0).value,
+                       fl.view(1).value, True)
+                assert ps.last_fallback()[0] == fell
+            for i in range(len(pairs)):
+                f, o, fl = frames.view(i), outs.view(1).view(i), flags.view(1).view(i)
+                pa.run_async(f.view(0).value, f.view(1).value, o.view(0).value, o.view(1).value, fl.view(0).value,
+                             fl.view(1).value)
+            pa.synchronize()
+            got, want = outs.get(), flags.get()
+            assert np.array_equal(got[0], got[1]) and np.array_equal(want[0], want[1]), (fell, rep)
+        assert pa.learned_counts() == (2 * len(pairs), 2 * len(pairs) if fell else 0)
+        assert ps.learned_counts() == pa.learned_counts()
