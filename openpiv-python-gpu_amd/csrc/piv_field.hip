@@ -42,16 +42,21 @@ template <> __device__ __forceinline__ double vnan<double>() { return __builtin_
 // [0, n-2] while the normalised distance is not (cupyx/scipy
 // RegularGridInterpolator(method="linear", fill_value=None) semantics used at
 // gpu_process.py:737-740 and :1159-1162).
+// Grid cell (clamped) of a coordinate along one axis.
+template <typename R>
+__device__ __forceinline__ int grid_cell(R q, R q0, R inv, int cells) {
+  const int i = (int)vfloor((q - q0) * inv);
+  return i < 0 ? 0 : (i > cells - 2 ? cells - 2 : i);
+}
+
 template <typename R>
 __device__ __forceinline__ void grid_bilinear(const R* fu, const R* fv, int m, int n, R y0, R x0, R sp, R yq, R xq,
                                               R& ou, R& ov) {
   // one division, then products (exact for the power-of-two spacings of the
   // usual schedules; otherwise within a rounding of the quotient)
   const R inv = R(1) / sp;
-  int iy = (int)vfloor((yq - y0) * inv);
-  int ix = (int)vfloor((xq - x0) * inv);
-  iy = iy < 0 ? 0 : (iy > m - 2 ? m - 2 : iy);
-  ix = ix < 0 ? 0 : (ix > n - 2 ? n - 2 : ix);
+  const int iy = grid_cell(yq, y0, inv, m);
+  const int ix = grid_cell(xq, x0, inv, n);
   const R ty = (yq - (y0 + (R)iy * sp)) * inv;
   const R tx = (xq - (x0 + (R)ix * sp)) * inv;
   const R wy0 = R(1) - ty, wx0 = R(1) - tx;
@@ -59,6 +64,23 @@ __device__ __forceinline__ void grid_bilinear(const R* fu, const R* fv, int m, i
   const size_t i10 = i00 + n;
   ou = fu[i00] * wy0 * wx0 + fu[i00 + 1] * wy0 * tx + fu[i10] * ty * wx0 + fu[i10 + 1] * ty * tx;
   ov = fv[i00] * wy0 * wx0 + fv[i00 + 1] * wy0 * tx + fv[i10] * ty * wx0 + fv[i10 + 1] * ty * tx;
+}
+
+// grid_bilinear with the grid read from an LDS patch holding grid rows
+// iy_lo.. and columns ix_lo.. (row stride PS).
+template <typename R, int PS>
+__device__ __forceinline__ void grid_bilinear_lds(const R* su, const R* sv, int iy_lo, int ix_lo, int m, int n, R y0,
+                                                  R x0, R sp, R yq, R xq, R& ou, R& ov) {
+  const R inv = R(1) / sp;
+  const int iy = grid_cell(yq, y0, inv, m);
+  const int ix = grid_cell(xq, x0, inv, n);
+  const R ty = (yq - (y0 + (R)iy * sp)) * inv;
+  const R tx = (xq - (x0 + (R)ix * sp)) * inv;
+  const R wy0 = R(1) - ty, wx0 = R(1) - tx;
+  const int i00 = (iy - iy_lo) * PS + (ix - ix_lo);
+  const int i10 = i00 + PS;
+  ou = su[i00] * wy0 * wx0 + su[i00 + 1] * wy0 * tx + su[i10] * ty * wx0 + su[i10 + 1] * ty * tx;
+  ov = sv[i00] * wy0 * wx0 + sv[i00 + 1] * wy0 * tx + sv[i10] * ty * wx0 + sv[i10 + 1] * ty * tx;
 }
 
 // cuda_interpolate_frame (gpu_process.py:1528-1546): clamped base index,
@@ -99,12 +121,65 @@ __device__ __forceinline__ void deform_px(const DeformArgsT<R>& a, int X, int Y)
                           : ldr<R, T>(a.frame_b, idx);
 }
 
-template <typename R, typename T>
-__global__ __launch_bounds__(256) void deform_kernel(DeformArgsT<R> a) {
-  const int X = blockIdx.x * blockDim.x + threadIdx.x;
-  const int Y = blockIdx.y;
+// ROWS pixels of one column per thread, in three phases -- every predictor
+// load, then every frame tap, then the stores -- so a thread has ROWS
+// independent load chains in flight instead of one; the arithmetic per pixel
+// is deform_px's.
+template <typename R, typename T, int ROWS>
+__global__ __launch_bounds__(256) void deform_rows_kernel(DeformArgsT<R> a) {
+  // the block's patch of the predictor grid (rows and columns of the cells
+  // its pixels fall in, plus one) staged in LDS once, when it fits
+  constexpr int PR = 4, PS = 72;
+  __shared__ R su[PR * PS], sv[PR * PS];
+  const int Xb = blockIdx.x * blockDim.x;
+  const int X = Xb + threadIdx.x;
+  const int Y0 = blockIdx.y * ROWS;
+  const R inv = R(1) / a.spacing;
+  const int iy_lo = grid_cell((R)Y0, a.y0, inv, a.m);
+  const int iy_hi = grid_cell((R)min(Y0 + ROWS - 1, a.ht - 1), a.y0, inv, a.m) + 1;
+  const int ix_lo = grid_cell((R)Xb, a.x0, inv, a.n);
+  const int ix_hi = grid_cell((R)min(Xb + (int)blockDim.x - 1, a.wd - 1), a.x0, inv, a.n) + 1;
+  const bool staged = iy_hi - iy_lo < PR && ix_hi - ix_lo < PS;  // uniform over the block
+  if (staged) {
+    const int rows = iy_hi - iy_lo + 1, cols = ix_hi - ix_lo + 1;
+    for (int e = threadIdx.x; e < rows * cols; e += blockDim.x) {
+      const int r = e / cols, c = e - r * cols;
+      const size_t g = (size_t)(iy_lo + r) * a.n + ix_lo + c;
+      su[r * PS + c] = a.up[g];
+      sv[r * PS + c] = a.vp[g];
+    }
+    __syncthreads();
+  }
   if (X >= a.wd) return;
-  deform_px<R, T>(a, X, Y);
+  R up[ROWS], vp[ROWS], oa[ROWS], ob[ROWS];
+#pragma unroll
+  for (int k = 0; k < ROWS; ++k) {
+    const int Y = min(Y0 + k, a.ht - 1);
+    if (staged)
+      grid_bilinear_lds<R, PS>(su, sv, iy_lo, ix_lo, a.m, a.n, a.y0, a.x0, a.spacing, (R)Y, (R)X, up[k], vp[k]);
+    else
+      grid_bilinear<R>(a.up, a.vp, a.m, a.n, a.y0, a.x0, a.spacing, (R)Y, (R)X, up[k], vp[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < ROWS; ++k) {
+    const int Y = min(Y0 + k, a.ht - 1);
+    const size_t idx = (size_t)Y * a.wd + X;
+    oa[k] = a.warp_a ? warp_px<R, T>(a.frame_a, a.ht, a.wd, (R)X, (R)Y, up[k], vp[k], a.dt_a)
+                     : ldr<R, T>(a.frame_a, idx);
+    ob[k] = a.warp_b ? warp_px<R, T>(a.frame_b, a.ht, a.wd, (R)X, (R)Y, up[k], vp[k], a.dt_b)
+                     : ldr<R, T>(a.frame_b, idx);
+  }
+#pragma unroll
+  for (int k = 0; k < ROWS; ++k) {
+    const int Y = Y0 + k;
+    if (Y >= a.ht) break;
+    const size_t idx = (size_t)Y * a.wd + X;
+    // cuda_interpolate_masked_frame (gpu_process.py:1461-1506) leaves masked
+    // pixels at 0
+    const bool masked = a.mask && a.mask[idx];
+    a.out_a[idx] = masked ? R(0) : oa[k];
+    a.out_b[idx] = masked ? R(0) : ob[k];
+  }
 }
 
 // Many frames in one launch: grid.y runs over the concatenated rows.
@@ -1404,12 +1479,13 @@ void set_launch_detail(const char* what, hipError_t e) {
 template <typename R>
 hipError_t launch_deform(const DeformArgsT<R>& a, hipStream_t stream) {
   if (a.m < 2 || a.n < 2) return hipErrorInvalidValue;
-  dim3 grid((a.wd + 255) / 256, a.ht);
+  constexpr int ROWS = 4;
+  dim3 grid((a.wd + 255) / 256, (a.ht + ROWS - 1) / ROWS);
   switch (a.in_dtype) {
-    case kU8: hipLaunchKernelGGL((deform_kernel<R, uint8_t>), grid, dim3(256), 0, stream, a); break;
-    case kU16: hipLaunchKernelGGL((deform_kernel<R, uint16_t>), grid, dim3(256), 0, stream, a); break;
-    case kF32: hipLaunchKernelGGL((deform_kernel<R, float>), grid, dim3(256), 0, stream, a); break;
-    case kF64: hipLaunchKernelGGL((deform_kernel<R, double>), grid, dim3(256), 0, stream, a); break;
+    case kU8: hipLaunchKernelGGL((deform_rows_kernel<R, uint8_t, ROWS>), grid, dim3(256), 0, stream, a); break;
+    case kU16: hipLaunchKernelGGL((deform_rows_kernel<R, uint16_t, ROWS>), grid, dim3(256), 0, stream, a); break;
+    case kF32: hipLaunchKernelGGL((deform_rows_kernel<R, float, ROWS>), grid, dim3(256), 0, stream, a); break;
+    case kF64: hipLaunchKernelGGL((deform_rows_kernel<R, double, ROWS>), grid, dim3(256), 0, stream, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
