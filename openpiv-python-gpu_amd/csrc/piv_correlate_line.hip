@@ -188,7 +188,7 @@ __global__ __launch_bounds__(64 * LineLayout<F>::BW, LineLayout<F>::WAVES) void 
     });
     static_for<H, F>([&](auto CC) { x[decltype(CC)::value] = c32{0.f, 0.f}; });
     wave_lds_sync();  // staging (aliased by AB) fully read
-    fft_reg<F, -1>(x);
+    fft_reg<F, -1, true>(x);
     // Z[k] sits in x[bitrev(k)].  2A[k] = Z[k] + conj(Z[F-k]),
     // 2B[k] = -i (Z[k] - conj(Z[F-k])); A and B are real at k = 0 and k = H,
     // packed as slot 0 = (DC, Nyquist).  The factors 2 are folded into the
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(64 * LineLayout<F>::BW, LineLayout<F>::WAVES) void 
       });
       static_for<H, F>([&](auto MM) { fa[decltype(MM)::value] = c32{0.f, 0.f}; });
       wave_lds_sync();  // AB (aliased by R) fully read
-      fft_reg<F, -1>(fa);
+      fft_reg<F, -1, true>(fa);
       static_for<0, F>([&](auto KY) {
         constexpr int ky = decltype(KY)::value;
         R[(size_t)ky * SR + kx] = fa[bitrev(ky, F)];
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(64 * LineLayout<F>::BW, LineLayout<F>::WAVES) void 
     c32 fb[F];
     static_for<0, H>([&](auto MM) { fb[decltype(MM)::value] = bin[decltype(MM)::value]; });
     static_for<H, F>([&](auto MM) { fb[decltype(MM)::value] = c32{0.f, 0.f}; });
-    fft_reg<F, -1>(fb);
+    fft_reg<F, -1, true>(fb);
     static_for<0, F>([&](auto KY) {
       constexpr int ky = decltype(KY)::value;
       p[ky] = cconjmul(R[(size_t)ky * SR + kx], fb[bitrev(ky, F)]);

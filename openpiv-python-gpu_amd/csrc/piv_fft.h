@@ -101,34 +101,56 @@ constexpr int bitrev(int v, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// In-register DIF FFT of length L: natural-order input x[n]; on return the
+// In-register FFT of length L: natural-order input x[n]; on return the
 // frequency-k value sits in x[bitrev(k, L)].
+//
+// Radix-2 decimation in time on the bit-reversed view y[i] = x[bitrev(i)]:
+// DIT takes its input bit-reversed and returns its output in natural order,
+// so with this view both conventions above hold and the permutation is only
+// register naming.  A butterfly with a non-trivial twiddle w is
+//   u' = u + w v  (four FMAs),   v' = 2u - u'  (two FMAs),
+// six instructions instead of the eight of a multiply and two add pairs;
+// twiddles 1, -1, +-i cost only the adds.  HALF_ZERO: inputs x[L/2..L-1]
+// are zero, so the first stage (pairs x[n], x[n + L/2]) only copies.
 // ---------------------------------------------------------------------------
-template <int L, int DIR, int S>
-struct DifStage {
-  // S = half block size at this stage.
-  template <int B, int J>
-  __device__ __forceinline__ static void bfly(c32 (&x)[L]) {
-    if constexpr (B < L) {
-      if constexpr (J < S) {
-        c32 u = x[B + J], v = x[B + J + S];
-        x[B + J] = cadd(u, v);
-        x[B + J + S] = twiddle<J * (L / (2 * S)), L, DIR>(csub(u, v));
-        bfly<B, J + 1>(x);
-      } else {
-        bfly<B + 2 * S, 0>(x);
-      }
-    }
+template <int K, int N, int DIR>
+__device__ __forceinline__ void dit_bfly(c32& a, c32& b) {
+  constexpr int k = ((K % N) + N) % N;
+  if constexpr (k == 0 || 4 * k == N || 2 * k == N || 4 * k == 3 * N) {
+    const c32 t = twiddle<K, N, DIR>(b);
+    const c32 u = a;
+    a = cadd(u, t);
+    b = csub(u, t);
+  } else {
+    constexpr float c = (float)cos_frac(k, N);
+    constexpr float s = (float)(DIR * sin_frac(k, N));
+    const c32 u = a, v = b;
+    const float ux = fmaf(v.x, c, fmaf(-v.y, s, u.x));
+    const float uy = fmaf(v.x, s, fmaf(v.y, c, u.y));
+    a = c32{ux, uy};
+    b = c32{fmaf(2.0f, u.x, -ux), fmaf(2.0f, u.y, -uy)};
   }
-  __device__ __forceinline__ static void run(c32 (&x)[L]) {
-    bfly<0, 0>(x);
-    if constexpr (S > 1) DifStage<L, DIR, S / 2>::run(x);
-  }
-};
+}
 
-template <int L, int DIR>
+template <int L, int DIR, bool HALF_ZERO = false>
 __device__ __forceinline__ void fft_reg(c32 (&x)[L]) {
-  if constexpr (L > 1) DifStage<L, DIR, L / 2>::run(x);
+  static_for<0, ilog2(L)>([&](auto SS) {
+    constexpr int half = 1 << decltype(SS)::value, m = 2 * half;
+    if constexpr (HALF_ZERO && half == 1) {
+      // y[2q + 1] = x[bitrev(2q) + L/2] = 0: the butterfly copies y[2q]
+      static_for<0, L / 2>([&](auto QQ) {
+        constexpr int q = decltype(QQ)::value;
+        x[bitrev(2 * q + 1, L)] = x[bitrev(2 * q, L)];
+      });
+    } else {
+      static_for<0, L / 2>([&](auto BB) {
+        constexpr int bi = decltype(BB)::value;  // butterfly index
+        constexpr int blk = bi / half, j = bi % half;
+        constexpr int i0 = blk * m + j, i1 = i0 + half;
+        dit_bfly<j * (L / m), L, DIR>(x[bitrev(i0, L)], x[bitrev(i1, L)]);
+      });
+    }
+  });
 }
 
 // ---------------------------------------------------------------------------
