@@ -297,18 +297,20 @@ int piv_stage_correlate_dft(const float *win_a, const float *win_b, int n_window
  * buffers registered once (read in place; re-register after moving them).
  * piv_batch_tune times the launch-shape candidates of every level on the
  * registered frames and keeps the fastest; choices (optional) receives, per
- * level, the deformation block width then each correlation group's
- * XCD-banded flag. */
+ * level, the deformation block width then, per correlation group, its
+ * XCD-banded flag and its workgroup width in waves (1, 2 or 4 for the
+ * one-lane-per-line kernel, 1 otherwise). */
 typedef struct piv_batch piv_batch;
 int piv_batch_create(int device, int n_pairs, const piv_plan_desc *descs, piv_batch **out);
 void piv_batch_destroy(piv_batch *batch);
 int piv_batch_set_frames(piv_batch *batch, const void *const *frames_a, const void *const *frames_b);
 int piv_batch_run(piv_batch *batch);
 int piv_batch_tune(piv_batch *batch, int repeats, int32_t *choices);
-/* Number of tuning choices of the batch (one per level plus one per
+/* Number of tuning choices of the batch (one per level plus two per
  * correlation group), and applying a set of them -- e.g. from a tuning
  * cache keyed by the batch's description -- without timing anything
- * (PIV_E_INVALID on a wrong count or value; nothing changes then). */
+ * (PIV_E_INVALID on a wrong count or value, or before the frames are set;
+ * nothing changes then). */
 int piv_batch_choice_count(const piv_batch *batch, int32_t *count);
 int piv_batch_set_choices(piv_batch *batch, const int32_t *choices, int count);
 /* GPU architecture name of a device (e.g. "gfx950:sramecc+:xnack-"), for

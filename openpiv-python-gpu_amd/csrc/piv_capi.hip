@@ -1833,13 +1833,19 @@ namespace {
 
 struct BatchGroup {  // one batched correlation launch
   int fsize = 0, in_dtype = 0, total_blocks = 0, max_ss = 0, xcd = 1;
+  int bw = 1;  // line kernel: waves per workgroup (tuned)
   bool line = false;  // one-lane-per-line kernel (the one a single-pair run of these fields uses)
   bool split = false;  // residue-split kernel (likewise)
   bool half = false;  // search windows <= F/2 (the two-lane kernel's variant for them)
   std::vector<int> plans;
   CorrArgs* d_fields = nullptr;
   int* d_prefix = nullptr;
+  // line groups: block prefixes for workgroups of 1, 2 and 4 waves
+  int* d_prefix_bw[3] = {nullptr, nullptr, nullptr};
+  int total_blocks_bw[3] = {0, 0, 0};
 };
+constexpr int kLineWidths[3] = {1, 2, 4};
+int line_width_index(int bw) { return bw == 4 ? 2 : (bw == 2 ? 1 : 0); }
 
 // One launch of a recorded field-stage kernel over a group of pairs whose
 // recorded sequences match (piv_record.h).
@@ -1889,6 +1895,8 @@ void batch_free_tables(piv_batch* b) {
   for (auto& L : b->levels) {
     for (auto& g : L.groups) {
       if (g.d_fields) (void)hipFree(g.d_fields);
+      for (int* p : g.d_prefix_bw)
+        if (p && p != g.d_prefix) (void)hipFree(p);
       if (g.d_prefix) (void)hipFree(g.d_prefix);
     }
     if (L.d_deform) (void)hipFree(L.d_deform);
@@ -2035,8 +2043,11 @@ int batch_build_tables(piv_batch* b) {
   batch_free_tables(b);
   for (size_t l = 0; l < b->levels.size(); ++l) {
     BatchLevel& L = b->levels[l];
-    std::vector<int> keep_xcd;
-    for (auto& g : L.groups) keep_xcd.push_back(g.xcd);
+    std::vector<int> keep_xcd, keep_bw;
+    for (auto& g : L.groups) {
+      keep_xcd.push_back(g.xcd);
+      keep_bw.push_back(g.bw);
+    }
     const int threads = L.deform_threads;
     L = BatchLevel{};
     L.deform_threads = threads;
@@ -2067,13 +2078,14 @@ int batch_build_tables(piv_batch* b) {
         g->split = split;
         g->half = half;
         g->xcd = L.groups.size() <= keep_xcd.size() ? keep_xcd[L.groups.size() - 1] : 1;
+        g->bw = line && L.groups.size() <= keep_bw.size() ? keep_bw[L.groups.size() - 1] : 1;
       }
       g->plans.push_back(p);
     }
     for (auto& g : L.groups) {
       std::vector<CorrArgs> fields;
       std::vector<int> prefix;
-      const int wpb = g.line    ? correlate_line_windows_per_block(g.fsize)
+      const int wpb = g.line    ? correlate_line_windows_per_block(g.fsize, g.bw)
                       : g.split ? correlate_split_windows_per_block(g.fsize)
                                 : correlate_windows_per_block(g.fsize);
       g.total_blocks = 0;
@@ -2100,6 +2112,26 @@ int batch_build_tables(piv_batch* b) {
         return fail(PIV_E_NOMEM, "batch: table allocation failed");
       HIP_TRY(hipMemcpy(g.d_fields, fields.data(), fields.size() * sizeof(CorrArgs), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(g.d_prefix, prefix.data(), prefix.size() * sizeof(int), hipMemcpyHostToDevice));
+      if (g.line) {  // the prefixes of the other workgroup widths, for the tuner
+        for (int wi = 0; wi < 3; ++wi) {
+          if (kLineWidths[wi] == g.bw) {
+            g.d_prefix_bw[wi] = g.d_prefix;
+            g.total_blocks_bw[wi] = g.total_blocks;
+            continue;
+          }
+          const int w = correlate_line_windows_per_block(g.fsize, kLineWidths[wi]);
+          std::vector<int> pre;
+          int tot = 0;
+          for (int p : g.plans) {
+            pre.push_back(tot);
+            tot += (b->plans[p]->passes[l].N + w - 1) / w;
+          }
+          if (hipMalloc((void**)&g.d_prefix_bw[wi], pre.size() * sizeof(int)) != hipSuccess)
+            return fail(PIV_E_NOMEM, "batch: table allocation failed");
+          HIP_TRY(hipMemcpy(g.d_prefix_bw[wi], pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
+          g.total_blocks_bw[wi] = tot;
+        }
+      }
     }
     if (l > 0) {
       std::vector<DeformArgs> dfs;
@@ -2130,7 +2162,7 @@ int batch_build_tables(piv_batch* b) {
 int batch_enqueue_corr(piv_batch* b, BatchGroup& g, hipStream_t s) {
   CorrBatch cb{g.d_fields, g.d_prefix, (int)g.plans.size(), g.xcd};
   if (g.line)
-    HIP_TRY(launch_correlate_line_batch(cb, g.fsize, g.in_dtype, g.total_blocks, g.max_ss, s));
+    HIP_TRY(launch_correlate_line_batch(cb, g.fsize, g.in_dtype, g.total_blocks, g.max_ss, g.bw, s));
   else if (g.split)
     HIP_TRY(launch_correlate_split_batch(cb, g.fsize, g.in_dtype, g.total_blocks, s));
   else
@@ -2311,7 +2343,27 @@ int piv_batch_tune(piv_batch* b, int repeats, int32_t* choices) {
         }
       }
       g.xcd = best_x;
-      if (choices) choices[ci++] = g.xcd;
+      if (g.line) {  // then the workgroup width of the line kernel
+        int best_w = 0;
+        best = 1e30f;
+        for (int wi = 0; wi < 3; ++wi) {
+          g.bw = kLineWidths[wi];
+          g.d_prefix = g.d_prefix_bw[wi];
+          g.total_blocks = g.total_blocks_bw[wi];
+          const float ms = time_it([&] { return batch_enqueue_corr(b, g, s); });
+          if (ms < best) {
+            best = ms;
+            best_w = wi;
+          }
+        }
+        g.bw = kLineWidths[best_w];
+        g.d_prefix = g.d_prefix_bw[best_w];
+        g.total_blocks = g.total_blocks_bw[best_w];
+      }
+      if (choices) {
+        choices[ci++] = g.xcd;
+        choices[ci++] = g.bw;
+      }
     }
   }
   (void)hipEventDestroy(e0);
@@ -2322,14 +2374,16 @@ int piv_batch_tune(piv_batch* b, int repeats, int32_t* choices) {
 
 int piv_batch_choice_count(const piv_batch* b, int32_t* count) {
   if (!b || !count) return fail(PIV_E_INVALID, "batch: bad argument");
+  if (!b->frames_set) return fail(PIV_E_INVALID, "batch: frames not set");
   int c = 0;
-  for (const auto& L : b->levels) c += 1 + (int)L.groups.size();
+  for (const auto& L : b->levels) c += 1 + 2 * (int)L.groups.size();
   *count = c;
   return PIV_OK;
 }
 
 int piv_batch_set_choices(piv_batch* b, const int32_t* choices, int count) {
   if (!b || !choices) return fail(PIV_E_INVALID, "batch: bad argument");
+  if (!b->frames_set) return fail(PIV_E_INVALID, "batch: frames not set");
   int32_t want = 0;
   (void)piv_batch_choice_count(b, &want);
   if (count != want) return fail(PIV_E_INVALID, "batch: %d choices given, the batch takes %d", count, want);
@@ -2337,13 +2391,27 @@ int piv_batch_set_choices(piv_batch* b, const int32_t* choices, int count) {
   for (const auto& L : b->levels) {  // check everything before changing anything
     const int t = choices[ci++];
     if (t != 128 && t != 256 && t != 512) return fail(PIV_E_INVALID, "batch: deformation block width %d", t);
-    for (size_t g = 0; g < L.groups.size(); ++g, ++ci)
+    for (const auto& g : L.groups) {
       if (choices[ci] != 0 && choices[ci] != 1) return fail(PIV_E_INVALID, "batch: XCD flag %d", choices[ci]);
+      const int w = choices[ci + 1];
+      if (g.line ? (w != 1 && w != 2 && w != 4) : w != 1)
+        return fail(PIV_E_INVALID, "batch: workgroup width %d", w);
+      ci += 2;
+    }
   }
   ci = 0;
   for (auto& L : b->levels) {
     L.deform_threads = choices[ci++];
-    for (auto& g : L.groups) g.xcd = choices[ci++];
+    for (auto& g : L.groups) {
+      g.xcd = choices[ci++];
+      const int w = choices[ci++];
+      if (g.line) {
+        const int wi = line_width_index(w);
+        g.bw = w;
+        g.d_prefix = g.d_prefix_bw[wi];
+        g.total_blocks = g.total_blocks_bw[wi];
+      }
+    }
   }
   batch_drop_graph(b);  // the next run captures the new shapes
   return PIV_OK;

@@ -51,8 +51,6 @@ struct LineLayout {
   static constexpr int SP = F + 1;  // plane row stride (float)
   // occupancy the register allocation is held to (LDS allows about this many)
   static constexpr int WAVES = F == 16 ? 4 : 2;
-  // waves per workgroup: one, so the LDS granularity is a wave's windows
-  static constexpr int BW = 1;
 };
 
 struct Peak {
@@ -81,10 +79,13 @@ size_t correlate_line_lds_floats(int fsize, int ss) {
 
 // BATCHED: one launch over the windows of many fields (pairs); block b
 // belongs to the last field whose block_prefix entry is <= b, so the field's
-// arguments stay uniform over the block.
-template <int F, typename InT, bool BATCHED>
-__global__ __launch_bounds__(64 * LineLayout<F>::BW, LineLayout<F>::WAVES) void corr_line_kernel(CorrArgs a_in, CorrBatch batch,
-                                                                              int n_windows_in, int region_floats) {
+// arguments stay uniform over the block.  BW: waves per workgroup (a
+// launch-shape choice: every window is computed by one wave with the same
+// code whatever BW is; single-pair launches use 1, so the LDS granularity is
+// one wave's windows).
+template <int F, typename InT, bool BATCHED, int BW>
+__global__ __launch_bounds__(64 * BW, LineLayout<F>::WAVES) void corr_line_kernel(CorrArgs a_in, CorrBatch batch,
+                                                                            int n_windows_in, int region_floats) {
   using L = LineLayout<F>;
   constexpr int H = L::H, G = L::G, SF = L::SF, SR = L::SR, SP = L::SP;
   extern __shared__ float4 smem4[];
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(64 * LineLayout<F>::BW, LineLayout<F>::WAVES) void 
     a = a_in;
     n_windows = n_windows_in;
   }
-  const int win = (bid * L::BW + wave) * G + g;
+  const int win = (bid * BW + wave) * G + g;
   const bool valid = win < n_windows;
   const int W = a.ss;
   const int pad = (F - W) / 2;
@@ -394,11 +395,11 @@ __global__ __launch_bounds__(64 * LineLayout<F>::BW, LineLayout<F>::WAVES) void 
 
 namespace {
 
-template <int F, typename InT, bool BATCHED>
+template <int F, typename InT, bool BATCHED, int BW>
 hipError_t line_lds_attr(size_t lds) {
   static thread_local size_t attr_bytes = 0;
   if (lds > 65536 && lds > attr_bytes) {
-    hipError_t e = hipFuncSetAttribute((const void*)corr_line_kernel<F, InT, BATCHED>,
+    hipError_t e = hipFuncSetAttribute((const void*)corr_line_kernel<F, InT, BATCHED, BW>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {
       set_launch_detail("hipFuncSetAttribute(MaxDynamicSharedMemorySize)", e);
@@ -414,29 +415,29 @@ hipError_t launch_line_t(const CorrArgs& a, hipStream_t stream) {
   using L = LineLayout<F>;
   const int n_windows = a.m * a.n;
   const int region = (int)correlate_line_lds_floats(F, a.ss);
-  const size_t lds = (size_t)region * sizeof(float) * L::G * L::BW;
-  const int per_block = L::G * L::BW;
+  const size_t lds = (size_t)region * sizeof(float) * L::G;
+  const int per_block = L::G;
   const int blocks = (n_windows + per_block - 1) / per_block;
   (void)hipGetLastError();
-  hipError_t e = line_lds_attr<F, InT, false>(lds);
+  hipError_t e = line_lds_attr<F, InT, false, 1>(lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((corr_line_kernel<F, InT, false>), dim3(blocks), dim3(64 * L::BW), lds, stream, a, CorrBatch{},
+  hipLaunchKernelGGL((corr_line_kernel<F, InT, false, 1>), dim3(blocks), dim3(64), lds, stream, a, CorrBatch{},
                      n_windows, region);
   e = hipGetLastError();
   if (e != hipSuccess) set_launch_detail("corr_line_kernel launch", e);
   return e;
 }
 
-template <int F, typename InT>
+template <int F, typename InT, int BW>
 hipError_t launch_line_batch_t(const CorrBatch& b, int total_blocks, int max_ss, hipStream_t stream) {
   using L = LineLayout<F>;
   const int region = (int)correlate_line_lds_floats(F, max_ss);
-  const size_t lds = (size_t)region * sizeof(float) * L::G * L::BW;
+  const size_t lds = (size_t)region * sizeof(float) * L::G * BW;
   (void)hipGetLastError();
-  hipError_t e = line_lds_attr<F, InT, true>(lds);
+  hipError_t e = line_lds_attr<F, InT, true, BW>(lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((corr_line_kernel<F, InT, true>), dim3(total_blocks), dim3(64 * L::BW), lds, stream, CorrArgs{}, b,
-                     0, region);
+  hipLaunchKernelGGL((corr_line_kernel<F, InT, true, BW>), dim3(total_blocks), dim3(64 * BW), lds, stream, CorrArgs{},
+                     b, 0, region);
   e = hipGetLastError();
   if (e != hipSuccess) set_launch_detail("corr_line_kernel (batched) launch", e);
   return e;
@@ -452,12 +453,22 @@ hipError_t launch_line_f(const CorrArgs& a, hipStream_t stream) {
   return hipErrorInvalidValue;
 }
 
-template <int F>
-hipError_t launch_line_batch_f(const CorrBatch& b, int in_dtype, int total_blocks, int max_ss, hipStream_t stream) {
+template <int F, int BW>
+hipError_t launch_line_batch_fw(const CorrBatch& b, int in_dtype, int total_blocks, int max_ss, hipStream_t stream) {
   switch (in_dtype) {
-    case kU8: return launch_line_batch_t<F, uint8_t>(b, total_blocks, max_ss, stream);
-    case kU16: return launch_line_batch_t<F, uint16_t>(b, total_blocks, max_ss, stream);
-    case kF32: return launch_line_batch_t<F, float>(b, total_blocks, max_ss, stream);
+    case kU8: return launch_line_batch_t<F, uint8_t, BW>(b, total_blocks, max_ss, stream);
+    case kU16: return launch_line_batch_t<F, uint16_t, BW>(b, total_blocks, max_ss, stream);
+    case kF32: return launch_line_batch_t<F, float, BW>(b, total_blocks, max_ss, stream);
+  }
+  return hipErrorInvalidValue;
+}
+template <int F>
+hipError_t launch_line_batch_f(const CorrBatch& b, int in_dtype, int total_blocks, int max_ss, int bw,
+                               hipStream_t stream) {
+  switch (bw) {
+    case 1: return launch_line_batch_fw<F, 1>(b, in_dtype, total_blocks, max_ss, stream);
+    case 2: return launch_line_batch_fw<F, 2>(b, in_dtype, total_blocks, max_ss, stream);
+    case 4: return launch_line_batch_fw<F, 4>(b, in_dtype, total_blocks, max_ss, stream);
   }
   return hipErrorInvalidValue;
 }
@@ -472,16 +483,16 @@ bool correlate_line_enabled() {
   return on;
 }
 
-int correlate_line_windows_per_block(int fsize) {
-  return fsize == 16 ? LineLayout<16>::G * LineLayout<16>::BW : LineLayout<32>::G * LineLayout<32>::BW;
+int correlate_line_windows_per_block(int fsize, int bw) {
+  return (fsize == 16 ? LineLayout<16>::G : LineLayout<32>::G) * bw;
 }
 
 hipError_t launch_correlate_line_batch(const CorrBatch& b, int fsize, int in_dtype, int total_blocks, int max_ss,
-                                       hipStream_t stream) {
+                                       int bw, hipStream_t stream) {
   if (total_blocks <= 0) return hipSuccess;
   switch (fsize) {
-    case 16: return launch_line_batch_f<16>(b, in_dtype, total_blocks, max_ss, stream);
-    case 32: return launch_line_batch_f<32>(b, in_dtype, total_blocks, max_ss, stream);
+    case 16: return launch_line_batch_f<16>(b, in_dtype, total_blocks, max_ss, bw, stream);
+    case 32: return launch_line_batch_f<32>(b, in_dtype, total_blocks, max_ss, bw, stream);
   }
   return hipErrorInvalidValue;
 }

@@ -103,9 +103,11 @@ int correlate_windows_per_block(int fsize);
 // launch of such a field uses the same arithmetic)
 hipError_t launch_correlate_batch(const CorrBatch& b, int fsize, int in_dtype, int total_blocks, int max_ss,
                                   bool half, hipStream_t stream);
-int correlate_line_windows_per_block(int fsize);
+// bw: waves per workgroup of the batched line kernel (1, 2 or 4; a launch
+// shape, the same arithmetic per window)
+int correlate_line_windows_per_block(int fsize, int bw = 1);
 hipError_t launch_correlate_line_batch(const CorrBatch& b, int fsize, int in_dtype, int total_blocks, int max_ss,
-                                       hipStream_t stream);
+                                       int bw, hipStream_t stream);
 size_t correlate_lds_bytes(int fsize, int ss);
 // Residue-split variant for F = 64, 128 (search window <= F/2, no S2N, no
 // shrunk interrogation window): every 1-D transform split by output index

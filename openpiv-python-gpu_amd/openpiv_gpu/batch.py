@@ -67,7 +67,8 @@ class PIVBatch:
     def tuning_key(self) -> str:
         buf = ctypes.create_string_buffer(256)
         _native.check(self._lib.piv_device_arch(self._device, buf, 256))
-        src = dict(self._key_src, arch=buf.value.decode(), choices=self.choice_count())
+        # layout 2: per level the deformation width, per group XCD flag and workgroup width
+        src = dict(self._key_src, arch=buf.value.decode(), layout=2)
         return hashlib.sha1(json.dumps(src, sort_keys=True).encode()).hexdigest()
 
     def tune(self, repeats: int = 3, cache=True):
@@ -75,9 +76,9 @@ class PIVBatch:
         batch's key, else timed (piv_batch_tune) and stored.  ``cache``:
         True for the default file (``$OPENPIV_HIP_TUNE_CACHE``, else
         ``~/.cache/openpiv_hip/batch_tuning.json``), a path, or False.
-        Returns the choices (per level: deformation block width, then each
-        correlation group's XCD-banded flag); ``tuning_source`` says "cache"
-        or "timed"."""
+        Returns the choices (per level: deformation block width, then per
+        correlation group its XCD-banded flag and workgroup width in waves);
+        ``tuning_source`` says "cache" or "timed"."""
         path = _cache_path(cache)
         key = self.tuning_key() if path else None
         n = self.choice_count()
