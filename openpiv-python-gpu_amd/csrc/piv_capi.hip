@@ -1860,8 +1860,8 @@ struct BatchLevel {
   std::vector<int> plans;  // pairs with a pass at this level
   std::vector<BatchGroup> groups;
   DeformArgs* d_deform = nullptr;  // pairs with L > 0
-  int* d_rows = nullptr;
-  int n_deform = 0, total_rows = 0, max_wd = 0, deform_threads = 256;
+  int* d_rows = nullptr;           // first row group of each pair's frames
+  int n_deform = 0, total_rows = 0, max_wd = 0, deform_threads = 256;  // total_rows: row groups
   std::vector<BatchedStep> pre;   // predictor interpolation (L > 0)
   std::vector<BatchedStep> post;  // validation, replacement, smoothing
 };
@@ -2143,7 +2143,7 @@ int batch_build_tables(piv_batch* b) {
         dfs.push_back(deform_args<float>(src_a(P), src_b(P), D.in_dtype, D.height, D.width, B.d, B.up, B.vp,
                                          as<float>(P->def_a), as<float>(P->def_b)));
         rows.push_back(L.total_rows);
-        L.total_rows += D.height;
+        L.total_rows += (D.height + kDeformRows - 1) / kDeformRows;
         L.max_wd = std::max(L.max_wd, D.width);
       }
       L.n_deform = (int)dfs.size();

@@ -103,28 +103,12 @@ __device__ __forceinline__ R warp_px(const void* f, int ht, int wd, R X, R Y, R 
   return a0 * b0 * fdl + a0 * b1 * fdr + a1 * b0 * ful + a1 * b1 * fur;
 }
 
-template <typename R, typename T>
-__device__ __forceinline__ void deform_px(const DeformArgsT<R>& a, int X, int Y) {
-  const size_t idx = (size_t)Y * a.wd + X;
-  if (a.mask && a.mask[idx]) {
-    // cuda_interpolate_masked_frame (gpu_process.py:1461-1506) leaves
-    // masked pixels at 0; an unwarped frame is already 0 there
-    a.out_a[idx] = R(0);
-    a.out_b[idx] = R(0);
-    return;
-  }
-  R up, vp;
-  grid_bilinear<R>(a.up, a.vp, a.m, a.n, a.y0, a.x0, a.spacing, (R)Y, (R)X, up, vp);
-  a.out_a[idx] = a.warp_a ? warp_px<R, T>(a.frame_a, a.ht, a.wd, (R)X, (R)Y, up, vp, a.dt_a)
-                          : ldr<R, T>(a.frame_a, idx);
-  a.out_b[idx] = a.warp_b ? warp_px<R, T>(a.frame_b, a.ht, a.wd, (R)X, (R)Y, up, vp, a.dt_b)
-                          : ldr<R, T>(a.frame_b, idx);
-}
-
-// ROWS pixels of one column per thread, in three phases -- every predictor
-// load, then every frame tap, then the stores -- so a thread has ROWS
-// independent load chains in flight instead of one; the arithmetic per pixel
-// is deform_px's.
+// Deformation in the value type R (FP64 plans, FP64 frames): ROWS pixels of
+// one column per thread, in three phases -- every predictor load, then every
+// frame tap, then the stores -- so a thread has ROWS independent load chains
+// in flight instead of one.  Predictor by grid_bilinear, warp by warp_px
+// (cuda_interpolate_frame's formula); masked pixels 0
+// (cuda_interpolate_masked_frame, gpu_process.py:1461-1506).
 template <typename R, typename T, int ROWS>
 __global__ __launch_bounds__(256) void deform_rows_kernel(DeformArgsT<R> a) {
   // the block's patch of the predictor grid (rows and columns of the cells
@@ -182,21 +166,259 @@ __global__ __launch_bounds__(256) void deform_rows_kernel(DeformArgsT<R> a) {
   }
 }
 
-// Many frames in one launch: grid.y runs over the concatenated rows.
+// ---------------------------------------------------------------------------
+// FP32 deformation of u8 / u16 / f32 frames, DR rows of one column per
+// thread (the single-pair and the batched launch run this body).
+//
+// Predictor (RegularGridInterpolator semantics as grid_bilinear: clamped
+// cell, unclamped fraction).  The thread's column fixes its grid cell ix and
+// fraction tx once; the grid rows its DR rows fall between are interpolated
+// along x once (h_r = f[g0 + r][ix] (1 - tx) + f[g0 + r][ix + 1] tx), and
+// each row then needs (1 - ty) h_d + ty h_{d+1} with its block-uniform cell
+// g0 + d and fraction ty.  Warp (cuda_interpolate_frame,
+// gpu_process.py:1528-1546): clamped base pixel, unclamped fraction; the two
+// horizontally adjacent taps of a row come from one buffer load (32-bit
+// offsets, the lower row through the scalar offset), and the bilinear value
+// is formed as a1 (b0 ful + b1 fur) + a0 (b0 fdl + b1 fdr).  Stores go
+// through buffer stores with the row in the scalar offset.
+// ---------------------------------------------------------------------------
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buf_f32(rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+
+// Two horizontally adjacent pixels starting at element idx (byte offset
+// idx * sizeof(T)) of the row selected by the scalar byte offset soff: one
+// load into a raw word, split into two floats afterwards.
 template <typename T>
-__global__ __launch_bounds__(512) void deform_batch_kernel(const DeformArgs* fields, const int* row_prefix,
-                                                           int n_fields) {
+struct PixelPair;
+template <>
+struct PixelPair<uint8_t> {
+  struct Raw {
+    uint32_t a, b;
+  };
+  static __device__ __forceinline__ Raw load(rsrc_t r, uint32_t idx, uint32_t soff) {
+    return Raw{__builtin_amdgcn_raw_buffer_load_b8(r, (int)idx, (int)soff, 0),
+               __builtin_amdgcn_raw_buffer_load_b8(r, (int)idx + 1, (int)soff, 0)};
+  }
+  static __device__ __forceinline__ void split(Raw w, float& a, float& b) {
+    a = (float)w.a;
+    b = (float)w.b;
+  }
+};
+template <>
+struct PixelPair<uint16_t> {
+  // two naturally aligned 2-byte loads: a 4-byte load at an odd pixel is
+  // misaligned and costs more than the two
+  struct Raw {
+    uint32_t a, b;
+  };
+  static __device__ __forceinline__ Raw load(rsrc_t r, uint32_t idx, uint32_t soff) {
+    return Raw{__builtin_amdgcn_raw_buffer_load_b16(r, (int)(idx * 2u), (int)soff, 0),
+               __builtin_amdgcn_raw_buffer_load_b16(r, (int)(idx * 2u) + 2, (int)soff, 0)};
+  }
+  static __device__ __forceinline__ void split(Raw w, float& a, float& b) {
+    a = (float)w.a;
+    b = (float)w.b;
+  }
+};
+template <>
+struct PixelPair<float> {
+  struct Raw {
+    float a, b;
+  };
+  static __device__ __forceinline__ Raw load(rsrc_t r, uint32_t idx, uint32_t soff) {
+    return Raw{buf_f32(r, idx * 4u, soff), buf_f32(r, idx * 4u + 4u, soff)};
+  }
+  static __device__ __forceinline__ void split(Raw w, float& a, float& b) {
+    a = w.a;
+    b = w.b;
+  }
+};
+
+template <typename T>
+struct DeformTile {
+  static constexpr int DR = kDeformRows;  // rows per thread
+  using PP = PixelPair<T>;
+
+  // Predictor of the DR rows from y0 (RegularGridInterpolator semantics as
+  // grid_bilinear): h_r = f[g0 + r][ix] (1 - tx) + f[g0 + r][ix + 1] tx for
+  // the grid rows g0, g0 + 1, g0 + 2, then (1 - ty) h_d + ty h_{d+1} per row
+  // (row cell g0 + d, block-uniform).  Lane k of the wave finds row k's cell
+  // and fraction, read back as wave-uniform values.
+  static __device__ __forceinline__ void predictor(const DeformArgs& a, int ix, float tx, float wx0, float inv, int y0,
+                                                   float (&up)[DR], float (&vp)[DR]) {
+    const rsrc_t ru = make_rsrc(a.up, (uint32_t)(a.m * a.n) * 4u);
+    const rsrc_t rv = make_rsrc(a.vp, (uint32_t)(a.m * a.n) * 4u);
+    const int g0 = __builtin_amdgcn_readfirstlane(grid_cell((float)y0, a.y0, inv, a.m));
+    float hu[3], hv[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int gr = g0 + r < a.m ? g0 + r : a.m - 1;
+      const uint32_t so = (uint32_t)(gr * a.n) * 4u;
+      const float u0 = buf_f32(ru, (uint32_t)ix * 4u, so), u1 = buf_f32(ru, (uint32_t)ix * 4u + 4u, so);
+      const float v0 = buf_f32(rv, (uint32_t)ix * 4u, so), v1 = buf_f32(rv, (uint32_t)ix * 4u + 4u, so);
+      hu[r] = fmaf(u1, tx, u0 * wx0);
+      hv[r] = fmaf(v1, tx, v0 * wx0);
+    }
+    const int lane = (int)(threadIdx.x & 63);
+    const float Yl = (float)min(y0 + (lane < DR ? lane : 0), a.ht - 1);
+    const int iyl = grid_cell(Yl, a.y0, inv, a.m);
+    const float tyl = (Yl - (a.y0 + (float)iyl * a.spacing)) * inv;
+#pragma unroll
+    for (int k = 0; k < DR; ++k) {
+      const int iy = __builtin_amdgcn_readlane(iyl, k);
+      const float ty = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tyl), k));
+      const float wy0 = 1.0f - ty;
+      const int d = iy - g0;  // block-uniform
+      if (d == 0) {
+        up[k] = fmaf(ty, hu[1], wy0 * hu[0]);
+        vp[k] = fmaf(ty, hv[1], wy0 * hv[0]);
+      } else if (d == 1) {
+        up[k] = fmaf(ty, hu[2], wy0 * hu[1]);
+        vp[k] = fmaf(ty, hv[2], wy0 * hv[1]);
+      } else {  // grid spacing below DR pixels: the two grid rows directly
+        const uint32_t s0 = (uint32_t)(iy * a.n) * 4u, s1 = s0 + (uint32_t)a.n * 4u;
+        const float u00 = buf_f32(ru, (uint32_t)ix * 4u, s0), u01 = buf_f32(ru, (uint32_t)ix * 4u + 4u, s0);
+        const float u10 = buf_f32(ru, (uint32_t)ix * 4u, s1), u11 = buf_f32(ru, (uint32_t)ix * 4u + 4u, s1);
+        const float v00 = buf_f32(rv, (uint32_t)ix * 4u, s0), v01 = buf_f32(rv, (uint32_t)ix * 4u + 4u, s0);
+        const float v10 = buf_f32(rv, (uint32_t)ix * 4u, s1), v11 = buf_f32(rv, (uint32_t)ix * 4u + 4u, s1);
+        up[k] = fmaf(ty, fmaf(u11, tx, u10 * wx0), wy0 * fmaf(u01, tx, u00 * wx0));
+        vp[k] = fmaf(ty, fmaf(v11, tx, v10 * wx0), wy0 * fmaf(v01, tx, v00 * wx0));
+      }
+    }
+  }
+
+  // One frame's taps at pixel (Xf, Yf) moved by (up, vp) * dt: the clamped
+  // base pixel (floats hold these integers exactly; a NaN coordinate clamps
+  // to 0 and yields a NaN value), the tap pairs of its two rows, the
+  // unclamped fractions.  tap() only issues the loads; value() consumes them,
+  // so that every row's loads are in flight before the first is used.
+  struct Tap {
+    typename PP::Raw top, bot;
+    float b1, a0;
+  };
+  static __device__ __forceinline__ Tap tap(rsrc_t r, float ht2, float wd2, uint32_t wd, float Xf, float Yf, float up,
+                                            float vp, float dt) {
+    const float xi = fmaf(up, dt, Xf);
+    const float yi = fmaf(vp, dt, Yf);
+    const float xl = __builtin_amdgcn_fmed3f(floorf(xi), 0.0f, wd2);
+    const float yu = __builtin_amdgcn_fmed3f(floorf(yi), 0.0f, ht2);
+    const uint32_t idx = __umul24((uint32_t)yu, wd) + (uint32_t)xl;  // frames below 2^24 pixels a side
+    Tap t;
+    t.top = PP::load(r, idx, 0u);
+    t.bot = PP::load(r, idx, wd * (uint32_t)sizeof(T));
+    t.b1 = xi - xl;
+    t.a0 = yi - yu;
+    return t;
+  }
+  // (1 - a0) ((1 - b1) ful + b1 fur) + a0 ((1 - b1) fdl + b1 fdr), as two
+  // horizontal lerps and a vertical one
+  static __device__ __forceinline__ float value(const Tap& t) {
+    float ful, fur, fdl, fdr;
+    PP::split(t.top, ful, fur);
+    PP::split(t.bot, fdl, fdr);
+    const float top = fmaf(t.b1, fur - ful, ful);
+    const float bot = fmaf(t.b1, fdr - fdl, fdl);
+    return fmaf(t.a0, bot - top, top);
+  }
+
+  // the unwarped pixel X of the row starting at element row_off
+  static __device__ __forceinline__ float raw(rsrc_t r, uint32_t X, uint32_t row_off) {
+    if constexpr (sizeof(T) == 4)
+      return buf_f32(r, X * 4u, row_off * 4u);
+    else if constexpr (sizeof(T) == 2)
+      return (float)(uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)(X * 2u), (int)(row_off * 2u), 0);
+    else
+      return (float)(uint32_t)__builtin_amdgcn_raw_buffer_load_b8(r, (int)X, (int)row_off, 0);
+  }
+
+  // Rows by * DR .. by * DR + DR - 1 of column bx * blockDim.x + tid; stores
+  // leave masked pixels at 0 (cuda_interpolate_masked_frame,
+  // gpu_process.py:1461-1506).
+  static __device__ void run(const DeformArgs& a, int bx, int by) {
+    const int X = bx * (int)blockDim.x + (int)threadIdx.x;
+    if (X >= a.wd) return;  // no barrier below
+    const int Y0 = by * DR;
+    const float inv = 1.0f / a.spacing;
+    const float Xf = (float)X;
+    const int ix = grid_cell(Xf, a.x0, inv, a.n);
+    const float tx = (Xf - (a.x0 + (float)ix * a.spacing)) * inv;
+    const float wx0 = 1.0f - tx;
+    float up[DR], vp[DR];
+    predictor(a, ix, tx, wx0, inv, Y0, up, vp);
+
+    const uint32_t npx = (uint32_t)a.ht * (uint32_t)a.wd;
+    const rsrc_t fa = make_rsrc(a.frame_a, npx * (uint32_t)sizeof(T));
+    const rsrc_t fb = make_rsrc(a.frame_b, npx * (uint32_t)sizeof(T));
+    const float ht2 = (float)(a.ht - 2), wd2 = (float)(a.wd - 2);
+    float oa[DR], ob[DR];
+    if (a.warp_a && a.warp_b) {  // both frames move (0 < deforming_par < 1): every load first
+      Tap ta[DR], tb[DR];
+#pragma unroll
+      for (int k = 0; k < DR; ++k) {
+        const float Yf = (float)min(Y0 + k, a.ht - 1);
+        ta[k] = tap(fa, ht2, wd2, (uint32_t)a.wd, Xf, Yf, up[k], vp[k], a.dt_a);
+        tb[k] = tap(fb, ht2, wd2, (uint32_t)a.wd, Xf, Yf, up[k], vp[k], a.dt_b);
+      }
+#pragma unroll
+      for (int k = 0; k < DR; ++k) {
+        oa[k] = value(ta[k]);
+        ob[k] = value(tb[k]);
+      }
+    } else {  // deforming_par 0 or 1: one frame copied unchanged
+#pragma unroll
+      for (int k = 0; k < DR; ++k) {
+        const int Y = min(Y0 + k, a.ht - 1);
+        const float Yf = (float)Y;
+        const uint32_t row = (uint32_t)Y * (uint32_t)a.wd;
+        oa[k] = a.warp_a ? value(tap(fa, ht2, wd2, (uint32_t)a.wd, Xf, Yf, up[k], vp[k], a.dt_a))
+                         : raw(fa, (uint32_t)X, row);
+        ob[k] = a.warp_b ? value(tap(fb, ht2, wd2, (uint32_t)a.wd, Xf, Yf, up[k], vp[k], a.dt_b))
+                         : raw(fb, (uint32_t)X, row);
+      }
+    }
+    const rsrc_t oa_r = make_rsrc(a.out_a, npx * 4u);
+    const rsrc_t ob_r = make_rsrc(a.out_b, npx * 4u);
+    const rsrc_t mk = make_rsrc(a.mask, a.mask ? npx : 0u);
+#pragma unroll
+    for (int k = 0; k < DR; ++k) {
+      const int Y = Y0 + k;
+      if (Y >= a.ht) break;
+      const uint32_t row = (uint32_t)Y * (uint32_t)a.wd;
+      float va = oa[k], vb = ob[k];
+      if (a.mask) {
+        const uint32_t mv = __builtin_amdgcn_raw_buffer_load_b8(mk, X, (int)row, 0);
+        if (mv) va = vb = 0.0f;
+      }
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, va), oa_r, X * 4, (int)(row * 4u), 0);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, vb), ob_r, X * 4, (int)(row * 4u), 0);
+    }
+  }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void deform_tile_kernel(DeformArgs a) {
+  DeformTile<T>::run(a, blockIdx.x, blockIdx.y);
+}
+
+// Many frames in one launch: grid.y runs over the concatenated row groups
+// (kDeformRows rows each) of the frames; TW: the workgroup width.
+template <typename T, int TW>
+__global__ __launch_bounds__(TW) void deform_batch_kernel(const DeformArgs* fields, const int* group_prefix,
+                                                          int n_fields) {
   const int r = blockIdx.y;
   int lo = 0, hi = n_fields - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (row_prefix[mid] <= r) lo = mid;
+    if (group_prefix[mid] <= r) lo = mid;
     else hi = mid - 1;
   }
   const DeformArgs a = fields[lo];
-  const int X = blockIdx.x * blockDim.x + threadIdx.x;
-  if (X >= a.wd) return;
-  deform_px<float, T>(a, X, r - row_prefix[lo]);
+  DeformTile<T>::run(a, blockIdx.x, r - group_prefix[lo]);
 }
 
 template <typename R>
@@ -1481,6 +1703,17 @@ hipError_t launch_deform(const DeformArgsT<R>& a, hipStream_t stream) {
   if (a.m < 2 || a.n < 2) return hipErrorInvalidValue;
   constexpr int ROWS = 4;
   dim3 grid((a.wd + 255) / 256, (a.ht + ROWS - 1) / ROWS);
+  if constexpr (sizeof(R) == 4) {
+    const dim3 tgrid((a.wd + 255) / 256, (a.ht + kDeformRows - 1) / kDeformRows);
+    switch (a.in_dtype) {
+      case kU8: hipLaunchKernelGGL(deform_tile_kernel<uint8_t>, tgrid, dim3(256), 0, stream, a); break;
+      case kU16: hipLaunchKernelGGL(deform_tile_kernel<uint16_t>, tgrid, dim3(256), 0, stream, a); break;
+      case kF32: hipLaunchKernelGGL(deform_tile_kernel<float>, tgrid, dim3(256), 0, stream, a); break;
+      case kF64: hipLaunchKernelGGL((deform_rows_kernel<R, double, ROWS>), grid, dim3(256), 0, stream, a); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   switch (a.in_dtype) {
     case kU8: hipLaunchKernelGGL((deform_rows_kernel<R, uint8_t, ROWS>), grid, dim3(256), 0, stream, a); break;
     case kU16: hipLaunchKernelGGL((deform_rows_kernel<R, uint16_t, ROWS>), grid, dim3(256), 0, stream, a); break;
@@ -1491,15 +1724,26 @@ hipError_t launch_deform(const DeformArgsT<R>& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_deform_batch(const DeformArgs* fields, const int* row_prefix, int n_fields, int total_rows,
+template <typename T>
+void launch_deform_batch_t(const DeformArgs* fields, const int* group_prefix, int n_fields, dim3 grid, int threads,
+                           hipStream_t stream) {
+  if (threads == 128)
+    hipLaunchKernelGGL((deform_batch_kernel<T, 128>), grid, dim3(128), 0, stream, fields, group_prefix, n_fields);
+  else if (threads == 256)
+    hipLaunchKernelGGL((deform_batch_kernel<T, 256>), grid, dim3(256), 0, stream, fields, group_prefix, n_fields);
+  else
+    hipLaunchKernelGGL((deform_batch_kernel<T, 512>), grid, dim3(512), 0, stream, fields, group_prefix, n_fields);
+}
+
+hipError_t launch_deform_batch(const DeformArgs* fields, const int* group_prefix, int n_fields, int total_groups,
                                int max_wd, int in_dtype, int threads, hipStream_t stream) {
-  if (total_rows <= 0) return hipSuccess;
+  if (total_groups <= 0) return hipSuccess;
   if (threads != 128 && threads != 256 && threads != 512) return hipErrorInvalidValue;
-  dim3 grid((max_wd + threads - 1) / threads, total_rows);
+  dim3 grid((max_wd + threads - 1) / threads, total_groups);
   switch (in_dtype) {
-    case kU8: hipLaunchKernelGGL(deform_batch_kernel<uint8_t>, grid, dim3(threads), 0, stream, fields, row_prefix, n_fields); break;
-    case kU16: hipLaunchKernelGGL(deform_batch_kernel<uint16_t>, grid, dim3(threads), 0, stream, fields, row_prefix, n_fields); break;
-    case kF32: hipLaunchKernelGGL(deform_batch_kernel<float>, grid, dim3(threads), 0, stream, fields, row_prefix, n_fields); break;
+    case kU8: launch_deform_batch_t<uint8_t>(fields, group_prefix, n_fields, grid, threads, stream); break;
+    case kU16: launch_deform_batch_t<uint16_t>(fields, group_prefix, n_fields, grid, threads, stream); break;
+    case kF32: launch_deform_batch_t<float>(fields, group_prefix, n_fields, grid, threads, stream); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

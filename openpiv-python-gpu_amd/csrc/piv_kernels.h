@@ -145,9 +145,12 @@ struct DeformArgsT {
 using DeformArgs = DeformArgsT<float>;
 template <typename R>
 hipError_t launch_deform(const DeformArgsT<R>& a, hipStream_t stream);
-// One launch over the rows of many frames (device tables); row r belongs to
-// the last frame whose row_prefix entry is <= r.  threads: block width.
-hipError_t launch_deform_batch(const DeformArgs* fields, const int* row_prefix, int n_fields, int total_rows,
+// Rows per thread of the FP32 deformation (a row group).
+constexpr int kDeformRows = 4;
+// One launch over the row groups of many frames (device tables); group r
+// belongs to the last frame whose group_prefix entry is <= r.  threads: block
+// width.
+hipError_t launch_deform_batch(const DeformArgs* fields, const int* group_prefix, int n_fields, int total_groups,
                                int max_wd, int in_dtype, int threads, hipStream_t stream);
 
 // Linear interpolation with extrapolation between two regular grids.
