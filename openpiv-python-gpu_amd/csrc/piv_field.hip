@@ -541,6 +541,7 @@ __global__ __launch_bounds__(256) void median_kernel(const float* u, const float
 // ---------------------------------------------------------------------------
 constexpr int kReplaceThreads = 1024;
 constexpr int kSpringLds = 1024;  // outlier count handled by the LDS-resident solver
+constexpr int kClusterMax = 6;    // largest outlier cluster solved directly (spring_clusters)
 
 template <int K>
 __device__ __forceinline__ void block_sum(double (&v)[K], double (*scratch)[K]) {
@@ -807,6 +808,11 @@ struct SpringLds {
   int32_t scan[kSpringLdsThreads / 64];
   int32_t cls[kMaxNeighbourhoodSize * kMaxNeighbourhoodSize];  // singular residue classes
   double red[4][4];
+  // cluster solve of up to 256 outliers (spring_clusters)
+  int16_t lab[256];        // cluster label: smallest member slot
+  int32_t ccount[256];     // members per label
+  int16_t cmem[256][kClusterMax];  // members per label (the first kClusterMax)
+  int16_t croot[256];      // labels of the clusters solved directly
 };
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -916,6 +922,155 @@ __device__ void spring_cg(SpringLds& L, int nv) {
   sync();
 }
 
+// Outlier clusters (vectors joined by outlier links) of 2..kClusterMax
+// vectors, solved exactly on wave 0 before the CG iterations: the system is
+// block diagonal by cluster, so CG started from these solutions converges at
+// once unless a larger cluster remains (its rows start from x = b / n_k as
+// before).  Clusters are labelled by their smallest slot (minimum-label
+// propagation with pointer jumping), their members listed per label, and
+// each lane takes one cluster: members sorted by slot (so the system and its
+// rounding do not depend on the listing order), Gaussian elimination without
+// pivoting (symmetric, diagonally dominant), every member's value written.
+// Stores b = n_k x in ru / rv first (the right-hand sides the CG uses).
+// Returns whether a larger cluster is left (then CG runs; otherwise every
+// cluster is solved: single vectors by x = b / n_k already).
+
+__device__ __noinline__ bool spring_clusters(SpringLds& L, int nv) {
+  constexpr int E = 256 / 64, C = kClusterMax;
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int s = lane + 64 * e;
+    if (s < nv) {
+      const double nk = (double)L.nk[s];
+      L.ru[s] = L.xu[s] * nk;
+      L.rv[s] = L.xv[s] * nk;
+      L.lab[s] = (int16_t)s;
+      L.ccount[s] = 0;
+    }
+  }
+  wave_sync_lds();
+  // labels: the smallest slot of each cluster (a label only decreases; stops
+  // when no lane changes one, after at most nv rounds)
+  for (int it = 0; it < nv; ++it) {
+    bool changed = false;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int s = lane + 64 * e;
+      if (s < nv) {
+        int m = L.lab[s];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          const int c = L.nb[s][d];
+          if (c >= 0) m = min(m, (int)L.lab[c]);
+        }
+        m = min(m, (int)L.lab[m]);
+        if (m < L.lab[s]) {
+          L.lab[s] = (int16_t)m;
+          changed = true;
+        }
+      }
+    }
+    wave_sync_lds();
+    if (!__any(changed)) break;
+  }
+  // members per label (listing order is free: the solver sorts them)
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int s = lane + 64 * e;
+    if (s < nv) {
+      const int root = L.lab[s];
+      const int k = atomicAdd(&L.ccount[root], 1);
+      if (k < C) L.cmem[root][k] = (int16_t)s;
+    }
+  }
+  wave_sync_lds();
+  // clusters of 2..C vectors, one per lane
+  int nroots = 0;
+  bool larger = false;  // a cluster of more than C vectors is left to CG
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int s = lane + 64 * e;
+    larger = larger || (s < nv && L.lab[s] == s && L.ccount[s] > C);
+    const bool take = s < nv && L.lab[s] == s && L.ccount[s] >= 2 && L.ccount[s] <= C;
+    const uint64_t bal = __ballot(take);
+    if (take) L.croot[nroots + __popcll(bal & ((1ull << lane) - 1))] = (int16_t)s;
+    nroots += __popcll(bal);
+  }
+  wave_sync_lds();
+  for (int base = 0; base < nroots; base += 64) {
+    const int ci = base + lane;
+    if (ci < nroots) {
+      const int root = L.croot[ci];
+      const int cnt = L.ccount[root];
+      int q[C];
+#pragma unroll
+      for (int j = 0; j < C; ++j) q[j] = j < cnt ? (int)L.cmem[root][j] : INT_MAX;
+#pragma unroll
+      for (int r = 0; r < C; ++r) {  // odd-even transposition sort
+#pragma unroll
+        for (int j = r & 1; j + 1 < C; j += 2) {
+          const int lo = min(q[j], q[j + 1]), hi = max(q[j], q[j + 1]);
+          q[j] = lo;
+          q[j + 1] = hi;
+        }
+      }
+      double A[C][C], bu[C], bv[C];
+#pragma unroll
+      for (int p = 0; p < C; ++p) {
+        const bool real = p < cnt;
+        const int sp = real ? q[p] : 0;
+#pragma unroll
+        for (int j = 0; j < C; ++j) A[p][j] = 0.0;
+        A[p][p] = real ? (double)L.nk[sp] : 1.0;
+        bu[p] = real ? L.ru[sp] : 0.0;
+        bv[p] = real ? L.rv[sp] : 0.0;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          const int c = real ? (int)L.nb[sp][d] : -1;
+#pragma unroll
+          for (int j = 0; j < C; ++j)
+            if (c >= 0 && q[j] == c) A[p][j] -= 1.0;
+        }
+      }
+      double piv[C];
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        piv[k] = 1.0 / A[k][k];
+#pragma unroll
+        for (int i = k + 1; i < C; ++i) {
+          const double f = A[i][k] * piv[k];
+#pragma unroll
+          for (int j = k + 1; j < C; ++j) A[i][j] = fma(-f, A[k][j], A[i][j]);
+          bu[i] = fma(-f, bu[k], bu[i]);
+          bv[i] = fma(-f, bv[k], bv[i]);
+        }
+      }
+      double xu[C], xv[C];
+#pragma unroll
+      for (int k = C - 1; k >= 0; --k) {
+        double su = bu[k], sv = bv[k];
+#pragma unroll
+        for (int j = k + 1; j < C; ++j) {
+          su = fma(-A[k][j], xu[j], su);
+          sv = fma(-A[k][j], xv[j], sv);
+        }
+        xu[k] = su * piv[k];
+        xv[k] = sv * piv[k];
+      }
+#pragma unroll
+      for (int p = 0; p < C; ++p) {
+        if (p < cnt) {
+          L.xu[q[p]] = xu[p];
+          L.xv[q[p]] = xv[p];
+        }
+      }
+    }
+  }
+  wave_sync_lds();
+  return __any(larger);
+}
+
 // The same iterations for up to 256 outliers on wave 0 alone, with every
 // per-outlier value in registers (lane l owns outliers l, l + 64, ...); only
 // the search direction goes through LDS for the neighbour gathers.  Same
@@ -946,21 +1101,25 @@ __device__ void spring_cg_wave(SpringLds& L, int nv) {
   }
 #pragma unroll
   for (int e = 0; e < E; ++e) {
+    const int sc = act[e] ? lane + 64 * e : 0;
     double au = 0.0, av = 0.0;
 #pragma unroll
     for (int d = 0; d < 4; ++d) {  // + 0.0 for a neighbour that is no outlier
       au += L.xu[nb[e][d]];
       av += L.xv[nb[e][d]];
     }
-    ru[e] = au;  // r0 = b - A x0 = sum of x0 over outlier neighbours
-    rv[e] = av;
-    pu[e] = au * inv[e];
-    pv[e] = av * inv[e];
+    // r0 = b - A x = b - n_k x + (sum of x over outlier neighbours); exactly
+    // the neighbour sum where x is still b / n_k
+    const double bu = L.ru[sc], bv = L.rv[sc];
+    const double r_u = (bu - nk[e] * xu[e]) + au, r_v = (bv - nk[e] * xv[e]) + av;
+    ru[e] = r_u;
+    rv[e] = r_v;
+    pu[e] = r_u * inv[e];
+    pv[e] = r_v * inv[e];
     if (act[e]) {
-      acc[0] += au * au * inv[e];
-      acc[1] += av * av * inv[e];
-      const double bu = xu[e] * nk[e], bv = xv[e] * nk[e];  // |b|^2 as the scale
-      acc[2] += bu * bu;
+      acc[0] += r_u * r_u * inv[e];
+      acc[1] += r_v * r_v * inv[e];
+      acc[2] += bu * bu;  // |b|^2 as the scale
       acc[3] += bv * bv;
     }
   }
@@ -1247,7 +1406,17 @@ __device__ void SpringBody<R>::run(const Args& sa, VB) {
   __syncthreads();
   if (nvl <= 256) {
     if (tid >= 64) return;  // wave 0 finishes alone
-    spring_cg_wave(L, nvl);
+    bool cg = true;
+    if (!cls) {
+      cg = spring_clusters(L, nvl);
+    } else {  // singular residue classes possible: CG alone (b = n_k x)
+      for (int s = tid; s < nvl; s += 64) {
+        L.ru[s] = L.xu[s] * (double)L.nk[s];
+        L.rv[s] = L.xv[s] * (double)L.nk[s];
+      }
+      wave_sync_lds();
+    }
+    if (cg) spring_cg_wave(L, nvl);
   } else {
     spring_cg<T>(L, nvl);
   }

@@ -142,6 +142,56 @@ def test_spring_replace_matches_oracle(shape, n_out):
     assert np.array_equal(gu_[~flags], u[~flags])
 
 
+def _planted_clusters(shape, sizes, seed):
+    """Flags with outlier clusters of the given sizes (random 4-connected
+    blobs, corners and borders included), kept apart from each other."""
+    rng = np.random.default_rng(seed)
+    m, n = shape
+    flags = np.zeros(shape, bool)
+    taken = np.zeros(shape, bool)  # clusters and their 4-neighbours
+    starts = [(0, 0), (m - 1, n - 1), (0, n // 2), (m // 2, 0)]
+    for c, size in enumerate(sizes):
+        for _ in range(200):
+            i, j = starts[c] if c < len(starts) else (int(rng.integers(m)), int(rng.integers(n)))
+            blob = [(i, j)]
+            while len(blob) < size:
+                bi, bj = blob[int(rng.integers(len(blob)))]
+                di, dj = [(0, 1), (0, -1), (1, 0), (-1, 0)][int(rng.integers(4))]
+                a, b = bi + di, bj + dj
+                if 0 <= a < m and 0 <= b < n and (a, b) not in blob:
+                    blob.append((a, b))
+            if not any(taken[a, b] for a, b in blob):
+                break
+            starts = []
+        for a, b in blob:
+            flags[a, b] = True
+        for a, b in blob:
+            for da, db in ((0, 0), (0, 1), (0, -1), (1, 0), (-1, 0)):
+                if 0 <= a + da < m and 0 <= b + db < n:
+                    taken[a + da, b + db] = True
+    return flags
+
+
+@pytest.mark.parametrize("sizes", [
+    [1, 2, 3, 4, 5, 6, 2, 3, 6, 5, 1, 4] * 4,   # every cluster solved directly, no CG
+    [9, 2, 7, 3, 1, 6, 12, 4, 5, 2],            # larger clusters left to CG
+    [1] * 30,                                   # isolated outliers only
+])
+def test_spring_replace_clusters_match_oracle(sizes):
+    """Small outlier clusters are solved exactly per cluster and larger ones by
+    CG started from those solutions; both must equal the sparse direct solve
+    of the whole system (gpu_validation.py:336-371)."""
+    u, v = _noisy_field((63, 63), 8, 0)
+    flags = _planted_clusters((63, 63), sizes, 9)
+    assert 0 < int(flags.sum()) <= 256
+    u = np.where(flags, u + 12.0, u).astype(np.float32)
+    ru, rv = O.spring_replacement(u.astype(np.float64), v.astype(np.float64), flags)
+    gu_, gv_ = gu.stage_replace(u, v, flags)
+    # tolerance: FP32 storage of the solution plus the CG stopping rule
+    assert np.abs(gu_ - ru).max() < 2e-5 and np.abs(gv_ - rv).max() < 2e-5
+    assert np.array_equal(gu_[~flags], u[~flags]) and np.array_equal(gv_[~flags], v[~flags])
+
+
 def test_spring_replace_all_outliers_keeps_field():
     u, v = _noisy_field((9, 9), 6, 0)
     flags = np.ones((9, 9), bool)
