@@ -471,6 +471,8 @@ def main():
     ap.add_argument("--stream-pairs", type=int, default=48)
     ap.add_argument("--pairs", type=int, default=256)
     ap.add_argument("--variant-steps", type=int, default=20)
+    ap.add_argument("--pairs-per-step", type=int, default=12,
+                    help="pairs in one step of the headline loop (a batch of pairs spread over the plans)")
     ap.add_argument("--plans", type=int, default=3,
                     help="pairs in flight: one plan (stream) each, pairs assigned round robin")
     args = ap.parse_args()
@@ -515,14 +517,17 @@ def main():
             pl.synchronize()
         _native.synchronize()
 
-    log(f"warmup {args.warmup} per plan")
+    pps = max(1, args.pairs_per_step)
+    log(f"warmup {args.warmup} steps of {pps} pairs")
     for pl, (_, p) in zip(plans, slots):
-        for _ in range(args.warmup):
-            pl.run(p[0], p[1], True, p[2], p[3], p[4], p[5], True)
+        pl.run(p[0], p[1], True, p[2], p[3], p[4], p[5], True)  # first call: eager run, then the graph
+    for k in range(args.warmup * pps):
+        p = slots[k % n_plans][1]
+        plans[k % n_plans].run_async(p[0], p[1], p[2], p[3], p[4], p[5])
     sync_all()
-    log(f"timing {args.steps} steps over {n_plans} concurrent plans")
+    log(f"timing {args.steps} steps of {pps} pairs over {n_plans} concurrent plans")
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(args.steps * pps):
         # each pair queued on the next plan's stream (round robin); the
         # timed region ends when every plan's stream has drained
         p = slots[k % n_plans][1]
@@ -530,18 +535,18 @@ def main():
     sync_all()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = 1e3 / ms_per_step
+    value = 1e3 * pps / ms_per_step
     # one plan alone: pairs queued back to back on its stream
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(args.steps * pps):
         plan.run_async(ptrs[0], ptrs[1], ptrs[2], ptrs[3], ptrs[4], ptrs[5])
     sync_all()
-    single_plan_pairs_per_s = args.steps / (time.perf_counter() - t0)
+    single_plan_pairs_per_s = args.steps * pps / (time.perf_counter() - t0)
     # the synchronous call (returns after each pair's outputs are written)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(args.steps * pps):
         plan.run(ptrs[0], ptrs[1], True, ptrs[2], ptrs[3], ptrs[4], ptrs[5], True)
-    sync_pairs_per_s = args.steps / (time.perf_counter() - t0)
+    sync_pairs_per_s = args.steps * pps / (time.perf_counter() - t0)
 
     # accuracy of this run against the known displacement
     x, y = piv.coords
@@ -604,7 +609,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "2048x2048 uint16 pair (12-bit particles), passes 64->32->16 px, 50% overlap, "
                                "deformation + median(2) + spring + GCV smoothn, plain method",
-                   "pairs_per_step": 1, "graph": not args.no_graph,
+                   "pairs_per_step": pps, "graph": not args.no_graph,
                    "submission": f"queued (piv_run_async) round robin over {n_plans} plans, one stream each",
                    "concurrent_plans": n_plans},
         "rms_px_vs_truth": round(rms, 5),
