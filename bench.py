@@ -98,7 +98,7 @@ def stage_kernel_times(fa_u16, piv, reps=20):
                                            up.ptr, up.ptr, oa.ptr, ob.ptr, sh))
     ms = timed(dlaunch)
     nbytes = ht * wd * (2 * 2 + 2 * 4)
-    deform = {"kernel": "deform_rows_kernel<u16>", "avg_ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9}
+    deform = {"kernel": "deform_tile_kernel<u16>", "avg_ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9}
     return out, deform
 
 
