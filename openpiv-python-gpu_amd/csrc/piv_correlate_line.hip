@@ -23,6 +23,7 @@
 // epilogue nor an interrogation window inside the search window.
 #include <stdlib.h>
 
+#include "piv_buffer.h"
 #include "piv_fft.h"
 #include "piv_kernels.h"
 
@@ -138,7 +139,28 @@ __global__ __launch_bounds__(64 * BW, LineLayout<F>::WAVES) void corr_line_kerne
 
   // ---- staging: the window's lanes load W x W pixels row by row ----------
   float sa = 0.f, sb = 0.f;
-  {
+  if (W == H && buffer_fits((uint64_t)a.ht * a.wd, sizeof(InT))) {
+    // full window (the usual case): lane j loads column j of every row, all
+    // loads issued before the first is used
+    const uint32_t npx = (uint32_t)a.ht * (uint32_t)a.wd;
+    const rsrc_t ra = make_rsrc(a.frame_a, npx * (uint32_t)sizeof(InT));
+    const rsrc_t rb = make_rsrc(a.frame_b, npx * (uint32_t)sizeof(InT));
+    const uint32_t base = (uint32_t)row0 * (uint32_t)a.wd + (uint32_t)(col0 + j);
+    float va[H], vb[H];
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      va[r] = buf_px<InT>(ra, base + (uint32_t)r * (uint32_t)a.wd);
+      vb[r] = buf_px<InT>(rb, base + (uint32_t)r * (uint32_t)a.wd);
+    }
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      const float x = valid ? va[r] : 0.f, y = valid ? vb[r] : 0.f;
+      stA[r * SW + j] = x;
+      stB[r * SW + j] = y;
+      sa += x;
+      sb += y;
+    }
+  } else {
     int r = j / W, c = j % W;
     for (int e = j; e < W * W; e += H) {
       float va = 0.f, vb = 0.f;
@@ -320,7 +342,9 @@ __global__ __launch_bounds__(64 * BW, LineLayout<F>::WAVES) void corr_line_kerne
   {
     const int i1 = y1 ^ H, i2 = i1 + 1;  // y1 even: rows i1 < i2
     float* gp = (a.planes && valid) ? a.planes + (size_t)win * F * F : nullptr;
-    float qmax = -INFINITY;  // on the raw values: the scale is a power of two
+    // maximum on the raw values (the scale is a power of two), four partial
+    // maxima so the comparisons do not form one chain
+    float pm[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
     static_for<0, F>([&](auto XX) {
       constexpr int xx = decltype(XX)::value;
       constexpr int jj = xx ^ H;
@@ -328,22 +352,28 @@ __global__ __launch_bounds__(64 * BW, LineLayout<F>::WAVES) void corr_line_kerne
       const float c1 = val.x * scale, c2 = val.y * scale;
       plane[i1 * SP + jj] = c1;
       plane[i2 * SP + jj] = c2;
-      qmax = fmaxf(qmax, fmaxf(val.x, val.y));
+      pm[xx & 3] = fmaxf(pm[xx & 3], fmaxf(val.x, val.y));
       if (gp) {
         gp[i1 * F + jj] = c1;
         gp[i2 * F + jj] = c2;
       }
     });
-    // first occurrence of the maximum: walk the lane's columns backwards
-    int first = 0x7fffffff;
-    static_for<0, F>([&](auto JJ) {
-      constexpr int jj = F - 1 - decltype(JJ)::value;
-      first = q[bitrev(jj ^ H, F)].y == qmax ? i2 * F + jj : first;
+    const float qmax = fmaxf(fmaxf(pm[0], pm[1]), fmaxf(pm[2], pm[3]));
+    // first occurrence of the maximum: flat index i * F + jj, here as the code
+    // row * F + jj (row 0 = i1, 1 = i2), found by four independent chains
+    // (rows x column halves) that walk their columns downwards with
+    // compile-time codes; the smallest code wins
+    int fc[4] = {2 * F, 2 * F, 2 * F, 2 * F};
+    static_for<0, H>([&](auto JJ) {
+      constexpr int jl = H - 1 - decltype(JJ)::value;  // lower column half
+      constexpr int jh = jl + H;                         // upper column half
+      fc[0] = q[bitrev(jl ^ H, F)].x == qmax ? jl : fc[0];
+      fc[1] = q[bitrev(jh ^ H, F)].x == qmax ? jh : fc[1];
+      fc[2] = q[bitrev(jl ^ H, F)].y == qmax ? F + jl : fc[2];
+      fc[3] = q[bitrev(jh ^ H, F)].y == qmax ? F + jh : fc[3];
     });
-    static_for<0, F>([&](auto JJ) {
-      constexpr int jj = F - 1 - decltype(JJ)::value;
-      first = q[bitrev(jj ^ H, F)].x == qmax ? i1 * F + jj : first;
-    });
+    const int code = min(min(fc[0], fc[1]), min(fc[2], fc[3]));
+    const int first = code < 2 * F ? i1 * F + code : 0x7fffffff;  // i2 = i1 + 1
     best = Peak{qmax * scale, first};
   }
 #pragma unroll

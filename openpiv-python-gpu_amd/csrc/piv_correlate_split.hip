@@ -38,6 +38,7 @@
 // the window's threads compute through a small LDS exchange.
 #include <stdlib.h>
 
+#include "piv_buffer.h"
 #include "piv_fft.h"
 #include "piv_kernels.h"
 
@@ -230,21 +231,39 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
     const int col0 = wcol * a.spacing + a.x_off;
     constexpr int PER = (HW * HW) / (2 * F);  // pixels per thread (HW x HW staging area)
     float va[PER], vb[PER];
+    // every load issued before any is used: outside the window (or for an
+    // invalid window) the address is clamped into it and the value set to 0
+    if (buffer_fits((uint64_t)a.ht * a.wd, sizeof(InT))) {
+      const uint32_t npx = (uint32_t)a.ht * (uint32_t)a.wd;
+      const rsrc_t ra = make_rsrc(a.frame_a, npx * (uint32_t)sizeof(InT));
+      const rsrc_t rb = make_rsrc(a.frame_b, npx * (uint32_t)sizeof(InT));
+      // pixel e of this thread: row t / HW + e * (2F / HW), column t % HW
+      const uint32_t base = (uint32_t)row0 * (uint32_t)a.wd + (uint32_t)(col0 + min(t % HW, W - 1));
+#pragma unroll
+      for (int e = 0; e < PER; ++e) {
+        const int r = t / HW + e * (2 * F / HW);
+        const uint32_t gi = base + (uint32_t)min(r, W - 1) * (uint32_t)a.wd;
+        va[e] = buf_px<InT>(ra, gi);
+        vb[e] = buf_px<InT>(rb, gi);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < PER; ++e) {
+        const int r = t / HW + e * (2 * F / HW);
+        const size_t gi = (size_t)(row0 + min(r, W - 1)) * a.wd + (col0 + min(t % HW, W - 1));
+        va[e] = load_px<InT>(a.frame_a, gi);
+        vb[e] = load_px<InT>(a.frame_b, gi);
+      }
+    }
+    static_assert((2 * F) % HW == 0, "whole staging rows per step");
     float sa = 0.f, sb = 0.f;
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      const int idx = t + e * 2 * F;
-      const int r = idx / HW, c = idx % HW;
-      float x = 0.f, y = 0.f;
-      if (valid && r < W && c < W) {
-        const size_t gi = (size_t)(row0 + r) * a.wd + (col0 + c);
-        x = load_px<InT>(a.frame_a, gi);
-        y = load_px<InT>(a.frame_b, gi);
-      }
-      va[e] = x;
-      vb[e] = y;
-      sa += x;
-      sb += y;
+      const bool in = valid && t / HW + e * (2 * F / HW) < W && t % HW < W;
+      va[e] = in ? va[e] : 0.f;
+      vb[e] = in ? vb[e] : 0.f;
+      sa += va[e];
+      sb += vb[e];
     }
 #pragma unroll
     for (int off = HW / 2; off > 0; off >>= 1) {
@@ -453,29 +472,35 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
   {
     const int i1 = (2 * n) ^ (F / 2);
     const int i2 = i1 + 1;
-    float zmax = -INFINITY;
+    float pm[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};  // partial maxima: no single chain
     static_for<0, L>([&](auto QQ) {
       constexpr int q = decltype(QQ)::value;
       const int jj = (NR * q + role) ^ (F / 2);
       plane[i1 * SP + jj] = o[q].x;
       plane[i2 * SP + jj] = o[q].y;
-      zmax = fmaxf(zmax, fmaxf(o[q].x, o[q].y));
+      pm[q & 3] = fmaxf(pm[q & 3], fmaxf(o[q].x, o[q].y));
     });
-    // first occurrence: flat index i * F + jj; walk this lane's values from
-    // the largest index down (row i2, then i1; columns jj(q) descending:
-    // q = L/2-1 .. 0, then L-1 .. L/2)
+    const float zmax = fmaxf(fmaxf(pm[0], pm[1]), fmaxf(pm[2], pm[3]));
+    // first occurrence: the lane's values carry codes in flat-index order,
+    // code = row * L + (q ^ L/2) (row 0 = i1, 1 = i2; column jj(q) ascends
+    // with q ^ L/2 for the wave's residue).  Four independent chains (rows x
+    // code halves) walk their codes downwards with compile-time codes; the
+    // smallest code wins.
+    int fc[4] = {2 * L, 2 * L, 2 * L, 2 * L};
+    static_for<0, L / 2>([&](auto KK) {
+      constexpr int cl = L / 2 - 1 - decltype(KK)::value;  // lower code half
+      constexpr int ch = cl + L / 2;                        // upper code half
+      fc[0] = o[cl ^ (L / 2)].x == zmax ? cl : fc[0];
+      fc[1] = o[ch ^ (L / 2)].x == zmax ? ch : fc[1];
+      fc[2] = o[cl ^ (L / 2)].y == zmax ? L + cl : fc[2];
+      fc[3] = o[ch ^ (L / 2)].y == zmax ? L + ch : fc[3];
+    });
+    const int code = min(min(fc[0], fc[1]), min(fc[2], fc[3]));
     int first = 0x7fffffff;
-    auto visit = [&](float val, int idx) { first = val == zmax ? idx : first; };
-    static_for<0, L>([&](auto KK) {
-      constexpr int k = decltype(KK)::value;
-      constexpr int q = k < L / 2 ? L / 2 - 1 - k : L - 1 - (k - L / 2);
-      visit(o[q].y, i2 * F + ((NR * q + role) ^ (F / 2)));
-    });
-    static_for<0, L>([&](auto KK) {
-      constexpr int k = decltype(KK)::value;
-      constexpr int q = k < L / 2 ? L / 2 - 1 - k : L - 1 - (k - L / 2);
-      visit(o[q].x, i1 * F + ((NR * q + role) ^ (F / 2)));
-    });
+    if (code < 2 * L) {
+      const int q = (code & (L - 1)) ^ (L / 2);
+      first = (code < L ? i1 : i2) * F + ((NR * q + role) ^ (F / 2));
+    }
     best = SPeak{zmax, first};
   }
 #pragma unroll

@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <algorithm>
 
+#include "piv_buffer.h"
 #include "piv_fft.h"
 #include "piv_kernels.h"
 #include "piv_record.h"
@@ -182,13 +183,6 @@ __global__ __launch_bounds__(256) void deform_rows_kernel(DeformArgsT<R> a) {
 // is formed as a1 (b0 ful + b1 fur) + a0 (b0 fdl + b1 fdr).  Stores go
 // through buffer stores with the row in the scalar offset.
 // ---------------------------------------------------------------------------
-using rsrc_t = __amdgpu_buffer_rsrc_t;
-__device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ float buf_f32(rsrc_t r, uint32_t voff, uint32_t soff) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
-}
 
 // Two horizontally adjacent pixels starting at element idx (byte offset
 // idx * sizeof(T)) of the row selected by the scalar byte offset soff: one
