@@ -214,6 +214,12 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
   const int win = bid * G + g;
   const bool valid = win < n_windows;
   const int W = a.ss;
+  // the predictor of the window, loaded now so the epilogue does not wait
+  float pred_u = 0.f, pred_v = 0.f;
+  if (a.up && t == 0 && valid) {
+    pred_u = a.up[win];
+    pred_v = a.vp[win];
+  }
 
   float* region = smem + (size_t)g * Ly::REGION;
   float* stA = region;
@@ -543,8 +549,8 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
     float uu = jsp - (float)(F / 2);
     float vv = isp - (float)(F / 2);
     if (a.up) {
-      uu += a.up[win];
-      vv += a.vp[win];
+      uu += pred_u;
+      vv += pred_v;
     }
     a.u[win] = uu;
     a.v[win] = vv;
