@@ -410,14 +410,22 @@ __device__ void Dct2GcvBody<R>::run(const Args& da, VB vb) {
   const int lane = tid & 63, wave = tid >> 6;
   const double s = pow(10.0, grid_p(lane, p_min, p_max));
   double a4[4] = {0, 0, 0, 0};
-  for (int e = wave; e < 256; e += 4) {
-    const double L = sL[e];
-    if (L >= 0.0) {
-      double t[6];
-      gcv_terms(sS[e], L, s, t);
+  // 64 elements per wave in groups of eight independent evaluations (their
+  // divisions overlap), each group summed as a tree before it is added
+#pragma unroll 1
+  for (int e0 = wave; e0 < 256; e0 += 32) {
+    double g[8][4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) a4[q] += t[q];
+    for (int i = 0; i < 8; ++i) {
+      const double L = sL[e0 + 4 * i];
+      double t[6];
+      gcv_terms(sS[e0 + 4 * i], L >= 0.0 ? L : 0.0, s, t);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) g[i][q] = L >= 0.0 ? t[q] : 0.0;
     }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      a4[q] += ((g[0][q] + g[1][q]) + (g[2][q] + g[3][q])) + ((g[4][q] + g[5][q]) + (g[6][q] + g[7][q]));
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) red[wave][lane][q] = a4[q];
