@@ -403,36 +403,37 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
   }
   __syncthreads();
 
-  // ---- combine the four residues: R[y' + L t][s] = sum_r i^{r t} H_r[y'] ---
-  c32 rc[L];
+  // ---- combine the four residues: R[y' + L t] = sum_r i^{r t} H_r[y'] ---
+  // Wave w takes y' in [w L/4, (w + 1) L/4) and forms all four t: it reads
+  // and rewrites only its own slots, so the combine is in place without a
+  // barrier between its loads and stores.
   {
-    const c32* hs = Hb + (size_t)n * HS;
-    with_residue(role, [&](auto TT) {
-      constexpr int T = decltype(TT)::value;
-      static_for<0, L / 2>([&](auto YY) {
-        constexpr int y = 2 * decltype(YY)::value;
-        const float4 h0 = lds_f4(hs + y), h1 = lds_f4(hs + L + y), h2 = lds_f4(hs + 2 * L + y),
-                     h3 = lds_f4(hs + 3 * L + y);
-        {
-          const c32 s02 = cadd(c32{h0.x, h0.y}, rot_p<2 * T>(c32{h2.x, h2.y}));
-          const c32 s13 = cadd(c32{h1.x, h1.y}, rot_p<2 * T>(c32{h3.x, h3.y}));
-          rc[y] = cadd(s02, rot_p<T>(s13));
-        }
-        {
-          const c32 s02 = cadd(c32{h0.z, h0.w}, rot_p<2 * T>(c32{h2.z, h2.w}));
-          const c32 s13 = cadd(c32{h1.z, h1.w}, rot_p<2 * T>(c32{h3.z, h3.w}));
-          rc[y + 1] = cadd(s02, rot_p<T>(s13));
-        }
-        load_fence<decltype(YY)::value, 4>();
-      });
-    });
-  }
-  __syncthreads();  // H consumed (R overwrites it in place)
-  {
-    c32* rs = Hb + (size_t)n * HS + role * L;
-    static_for<0, L / 2>([&](auto YY) {
+    c32* hs = Hb + (size_t)n * HS + role * (L / 4);
+    static_for<0, L / 8>([&](auto YY) {
       constexpr int y = 2 * decltype(YY)::value;
-      lds_st4(rs + y, rc[y], rc[y + 1]);
+      const float4 h0 = lds_f4(hs + y), h1 = lds_f4(hs + L + y), h2 = lds_f4(hs + 2 * L + y),
+                   h3 = lds_f4(hs + 3 * L + y);
+      c32 r0[2], r1[2], r2[2], r3[2];
+      {
+        const c32 s02 = cadd(c32{h0.x, h0.y}, c32{h2.x, h2.y}), d02 = csub(c32{h0.x, h0.y}, c32{h2.x, h2.y});
+        const c32 s13 = cadd(c32{h1.x, h1.y}, c32{h3.x, h3.y}), d13 = csub(c32{h1.x, h1.y}, c32{h3.x, h3.y});
+        r0[0] = cadd(s02, s13);
+        r2[0] = csub(s02, s13);
+        r1[0] = cadd(d02, rot_p<1>(d13));
+        r3[0] = csub(d02, rot_p<1>(d13));
+      }
+      {
+        const c32 s02 = cadd(c32{h0.z, h0.w}, c32{h2.z, h2.w}), d02 = csub(c32{h0.z, h0.w}, c32{h2.z, h2.w});
+        const c32 s13 = cadd(c32{h1.z, h1.w}, c32{h3.z, h3.w}), d13 = csub(c32{h1.z, h1.w}, c32{h3.z, h3.w});
+        r0[1] = cadd(s02, s13);
+        r2[1] = csub(s02, s13);
+        r1[1] = cadd(d02, rot_p<1>(d13));
+        r3[1] = csub(d02, rot_p<1>(d13));
+      }
+      lds_st4(hs + y, r0[0], r0[1]);
+      lds_st4(hs + L + y, r1[0], r1[1]);
+      lds_st4(hs + 2 * L + y, r2[0], r2[1]);
+      lds_st4(hs + 3 * L + y, r3[0], r3[1]);
     });
   }
   __syncthreads();
@@ -446,27 +447,31 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
     // slot 0 carries (DC, Nyquist) of each row
     auto z_direct = [](float4 v) { return c32{v.x - v.w, v.y + v.z}; };  // 0 < k < F/2
     auto z_mirror = [](float4 v) { return c32{v.x + v.w, v.z - v.y}; };  // F/2 < k: slot F - k
-    with_residue(role, [&](auto RR) {
+    // output k reads slots k, k + L, 2L - k, L - k; output L - k reads the
+    // same four slots, so the two are formed from one set of loads
+    auto out = [&](auto RR, auto KK, c32 z0, c32 z1, c32 z2, c32 z3) {
       constexpr int R = decltype(RR)::value;
-      static_for<0, L>([&](auto KK) {
+      constexpr int k = decltype(KK)::value;
+      const c32 s02 = cadd(z0, rot_p<2 * R>(z2));
+      const c32 s13 = cadd(z1, rot_p<2 * R>(z3));
+      o[bitrev(k, L)] = twiddle<R * k, F, +1>(cadd(s02, rot_p<R>(s13)));
+    };
+    with_residue(role, [&](auto RR) {
+      {
+        const float4 s0 = ld(0), sl = ld(L);
+        // DC and Nyquist of both rows in slot 0; k = L and k = 3L -> slot L
+        out(RR, std::integral_constant<int, 0>{}, c32{s0.x, s0.z}, z_direct(sl), c32{s0.y, s0.w}, z_mirror(sl));
+      }
+      {
+        const float4 sa = ld(L / 2), sb = ld(L + L / 2);
+        out(RR, std::integral_constant<int, L / 2>{}, z_direct(sa), z_direct(sb), z_mirror(sb), z_mirror(sa));
+      }
+      static_for<1, L / 2>([&](auto KK) {
         constexpr int k = decltype(KK)::value;
-        c32 z0, z1, z2, z3;
-        if constexpr (k == 0) {
-          const float4 s0 = ld(0), sl = ld(L);
-          z0 = c32{s0.x, s0.z};         // DC of both rows
-          z2 = c32{s0.y, s0.w};         // Nyquist of both rows
-          z1 = z_direct(sl);            // k = L
-          z3 = z_mirror(sl);            // k = 3L -> slot L
-        } else {
-          z0 = z_direct(ld(k));          // k
-          z1 = z_direct(ld(k + L));      // k + L < F/2
-          z2 = z_mirror(ld(2 * L - k));  // k + 2L -> slot F/2 - k
-          z3 = z_mirror(ld(L - k));      // k + 3L -> slot F/4 - k
-        }
-        const c32 s02 = cadd(z0, rot_p<2 * R>(z2));
-        const c32 s13 = cadd(z1, rot_p<2 * R>(z3));
-        o[bitrev(k, L)] = twiddle<R * k, F, +1>(cadd(s02, rot_p<R>(s13)));
-        load_fence<k, 8>();
+        const float4 sa = ld(k), sb = ld(k + L), sc = ld(2 * L - k), sd = ld(L - k);
+        out(RR, KK, z_direct(sa), z_direct(sb), z_mirror(sc), z_mirror(sd));
+        out(RR, std::integral_constant<int, L - k>{}, z_direct(sd), z_direct(sc), z_mirror(sb), z_mirror(sa));
+        load_fence<k, 4>();
       });
     });
   }
