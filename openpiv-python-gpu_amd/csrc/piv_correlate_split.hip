@@ -57,7 +57,7 @@ struct SplitLayout {
   static constexpr int L = F / NR;      // transform length per lane
   static constexpr int HW = F / 2;      // lanes per window and wave
   static constexpr int G = 64 / HW;     // windows per block
-  static constexpr int SW = HW + 1;     // staging row stride (float)
+  static constexpr int SW = HW + 1;     // staging row stride (complex; odd in 8-byte units)
   static constexpr int ZS = HW + 2;     // Z column stride (complex; odd in 16-byte units)
   static constexpr int HS = F + 2;      // H / R slot stride (complex; odd in 16-byte units)
   static constexpr int SP = F + 1;      // plane row stride (float)
@@ -222,8 +222,7 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
   }
 
   float* region = smem + (size_t)g * Ly::REGION;
-  float* stA = region;
-  float* stB = region + HW * SW;
+  float2* st = reinterpret_cast<float2*>(region);  // staging: (a, b) pixel pairs, row stride SW
   c32* Z = reinterpret_cast<c32*>(region);
   c32* Hb = reinterpret_cast<c32*>(region);
   float* plane = region;
@@ -291,8 +290,7 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
       const int idx = t + e * 2 * F;
       const int r = idx / HW, c = idx % HW;
       const bool in = r < W && c < W;
-      stA[r * SW + c] = in ? va[e] - ma : 0.f;
-      stB[r * SW + c] = in ? vb[e] - mb : 0.f;
+      st[r * SW + c] = in ? float2{va[e] - ma, vb[e] - mb} : float2{0.f, 0.f};
     }
   }
   __syncthreads();
@@ -304,8 +302,8 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
       constexpr int R = decltype(RR)::value;
       static_for<0, L>([&](auto CC) {
         constexpr int c = decltype(CC)::value;
-        const c32 z0{stA[n * SW + c], stB[n * SW + c]};
-        const c32 z1{stA[n * SW + c + L], stB[n * SW + c + L]};
+        const float2 s0 = st[n * SW + c], s1 = st[n * SW + c + L];
+        const c32 z0{s0.x, s0.y}, z1{s1.x, s1.y};
         y[bitrev(c, L)] = twiddle<R * c, F, -1>(cadd(z0, rot_m<R>(z1)));
       });
     });
