@@ -336,15 +336,19 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
         constexpr int c = 2 * decltype(CC)::value;
         const float4 k0 = lds_f4(zk + c), k1 = lds_f4(zk + c + L);
         const float4 m0 = lds_f4(zm + c), m1 = lds_f4(zm + c + L);
-        // A = Zk + conj(Zm), B = -i (Zk - conj(Zm)) (each twice the true value)
-        const c32 a0{k0.x + m0.x, k0.y - m0.y}, a0b{k0.z + m0.z, k0.w - m0.w};
-        const c32 a1{k1.x + m1.x, k1.y - m1.y}, a1b{k1.z + m1.z, k1.w - m1.w};
-        const c32 b0{k0.y + m0.y, m0.x - k0.x}, b0b{k0.w + m0.w, m0.z - k0.z};
-        const c32 b1{k1.y + m1.y, m1.x - k1.x}, b1b{k1.w + m1.w, m1.z - k1.z};
-        fa[bitrev(c, L)] = twiddle<R * c, F, -1>(cadd(a0, rot_m<R>(a1)));
-        fa[bitrev(c + 1, L)] = twiddle<R * (c + 1), F, -1>(cadd(a0b, rot_m<R>(a1b)));
-        fb[bitrev(c, L)] = twiddle<R * c, F, -1>(cadd(b0, rot_m<R>(b1)));
-        fb[bitrev(c + 1, L)] = twiddle<R * (c + 1), F, -1>(cadd(b0b, rot_m<R>(b1b)));
+        // A = Zk + conj(Zm), B = -i (Zk - conj(Zm)) (each twice the true
+        // value); the residue input of A is P + conj(Q) and of B
+        // -i (P - conj(Q)), with P = w^{Rc} (Zk[c] + (-i)^R Zk[c + L]) and
+        // Q = w^{-Rc} (Zm[c] + i^R Zm[c + L])
+        auto put = [&](auto EE, c32 k0e, c32 k1e, c32 m0e, c32 m1e) {
+          constexpr int e = c + decltype(EE)::value;
+          const c32 p = twiddle<R * e, F, -1>(cadd(k0e, rot_m<R>(k1e)));
+          const c32 q = twiddle<R * e, F, +1>(cadd(m0e, rot_p<R>(m1e)));
+          fa[bitrev(e, L)] = c32{p.x + q.x, p.y - q.y};
+          fb[bitrev(e, L)] = c32{p.y + q.y, q.x - p.x};
+        };
+        put(std::integral_constant<int, 0>{}, c32{k0.x, k0.y}, c32{k1.x, k1.y}, c32{m0.x, m0.y}, c32{m1.x, m1.y});
+        put(std::integral_constant<int, 1>{}, c32{k0.z, k0.w}, c32{k1.z, k1.w}, c32{m0.z, m0.w}, c32{m1.z, m1.w});
         load_fence<decltype(CC)::value, 4>();
       });
     });
