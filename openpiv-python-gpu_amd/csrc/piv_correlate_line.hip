@@ -75,11 +75,12 @@ size_t correlate_line_lds_floats(int fsize, int ss) {
   if (ab > m) m = ab;
   if (r > m) m = r;
   if (plane > m) m = plane;
-  // a wave's windows sit RF floats apart; RF = 16 (mod 64) puts their
-  // same-index accesses on different LDS banks (measured: fewer bank-conflict
-  // cycles, 2-3 % less time at F = 32 than RF = 0 mod 64)
+  // a wave's windows sit RF floats apart; RF = 48 (mod 64) puts their
+  // same-index accesses on different LDS banks (measured at F = 32: fewer
+  // bank-conflict cycles and 2-3 % less time than RF = 0 or 32 mod 64, about
+  // 1 % less than RF = 16)
   m = (m + 3) & ~size_t(3);
-  return m + (80 - m % 64) % 64;
+  return m + (112 - m % 64) % 64;
 }
 
 // BATCHED: one launch over the windows of many fields (pairs); block b
@@ -140,7 +141,9 @@ __global__ __launch_bounds__(64 * BW, LineLayout<F>::WAVES) void corr_line_kerne
   float* stB = region + W * SW;
   c32* AB = reinterpret_cast<c32*>(region);
   c32* R = reinterpret_cast<c32*>(region);
-  float* plane = region;
+  // the plane of every second window one float further: the two windows of a
+  // half-wave then write their planes on banks of opposite parity
+  float* plane = region + (g & 1);
 
   const int wrow = valid ? (win / a.n) : 0;
   const int wcol = valid ? (win % a.n) : 0;
@@ -316,23 +319,25 @@ __global__ __launch_bounds__(64 * BW, LineLayout<F>::WAVES) void corr_line_kerne
   }
 
   // ---- stage IC: inverse column kx ------------------------------------------
+  // output row y goes to slot (y >> 1) + H (y & 1): the rows 2j, 2j+1 that
+  // lane j reads next sit in slots j and j + H, an odd stride between lanes
   fft_reg<F, +1>(p);
   wave_lds_sync();  // lane 0's reads of column H done before R is written
   static_for<0, F>([&](auto YY) {
     constexpr int y = decltype(YY)::value;
-    R[(size_t)y * SR + kx] = p[bitrev(y, F)];
+    R[(size_t)((y >> 1) + H * (y & 1)) * SR + kx] = p[bitrev(y, F)];
   });
   wave_lds_sync();
 
   // ---- stage IR: rows 2j and 2j+1 as real and imaginary part ---------------
-  const int y1 = 2 * j, y2 = y1 + 1;
+  const int y1 = 2 * j;  // rows y1 and y1 + 1 (slots j and j + H)
   c32 q[F];
   {
     c32 l1[H], l2[H];
     static_for<0, H>([&](auto SS) {
       constexpr int s = decltype(SS)::value;
-      l1[s] = R[(size_t)y1 * SR + s];
-      l2[s] = R[(size_t)y2 * SR + s];
+      l1[s] = R[(size_t)j * SR + s];
+      l2[s] = R[(size_t)(j + H) * SR + s];
     });
     // row spectra are Hermitian: S[F - k] = conj(S[k]); slot 0 = (DC, Nyquist)
     q[0] = c32{l1[0].x, l2[0].x};
