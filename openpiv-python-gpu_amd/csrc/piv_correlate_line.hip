@@ -75,12 +75,12 @@ size_t correlate_line_lds_floats(int fsize, int ss) {
   if (ab > m) m = ab;
   if (r > m) m = r;
   if (plane > m) m = plane;
-  // a wave's windows sit RF floats apart; RF = 48 (mod 64) puts their
-  // same-index accesses on different LDS banks (measured at F = 32: fewer
-  // bank-conflict cycles and 2-3 % less time than RF = 0 or 32 mod 64, about
-  // 1 % less than RF = 16)
-  m = (m + 3) & ~size_t(3);
-  return m + (112 - m % 64) % 64;
+  // a wave's windows sit RF floats apart, RF odd: complex values move as
+  // dword pairs (ds_read2/ds_write2_b32, 4-byte aligned), so neighbouring
+  // windows fall on banks of opposite parity and the even lane strides of the
+  // transposes do not collide (measured at F = 32: RF = 57 mod 64 about 5 %
+  // less time than any multiple of 4; odd residues are within 1 % of it)
+  return m + (57 + 64 - m % 64) % 64;
 }
 
 // BATCHED: one launch over the windows of many fields (pairs); block b
@@ -141,9 +141,7 @@ __global__ __launch_bounds__(64 * BW, LineLayout<F>::WAVES) void corr_line_kerne
   float* stB = region + W * SW;
   c32* AB = reinterpret_cast<c32*>(region);
   c32* R = reinterpret_cast<c32*>(region);
-  // the plane of every second window one float further: the two windows of a
-  // half-wave then write their planes on banks of opposite parity
-  float* plane = region + (g & 1);
+  float* plane = region;
 
   const int wrow = valid ? (win / a.n) : 0;
   const int wcol = valid ? (win % a.n) : 0;
