@@ -335,7 +335,9 @@ struct DeformTile {
   // gpu_process.py:1461-1506).
   static __device__ void run(const DeformArgs& a, int bx, int by) {
     const int X = bx * (int)blockDim.x + (int)threadIdx.x;
-    if (X >= a.wd) return;  // no barrier below
+    // every lane of the wave forms the predictor (lane k supplies row k's
+    // cell and fraction to the others); columns beyond the frame leave after
+    if (X - (int)(threadIdx.x & 63) >= a.wd) return;  // the whole wave is beyond the frame
     const int Y0 = by * DR;
     const float inv = 1.0f / a.spacing;
     const float Xf = (float)X;
@@ -344,6 +346,7 @@ struct DeformTile {
     const float wx0 = 1.0f - tx;
     float up[DR], vp[DR];
     predictor(a, ix, tx, wx0, inv, Y0, up, vp);
+    if (X >= a.wd) return;  // no barrier below
 
     const uint32_t npx = (uint32_t)a.ht * (uint32_t)a.wd;
     const rsrc_t fa = make_rsrc(a.frame_a, npx * (uint32_t)sizeof(T));

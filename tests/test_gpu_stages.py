@@ -97,6 +97,25 @@ def test_deform_matches_oracle(deforming_par, dtype):
     assert np.abs(gb - rb).max() <= tol
 
 
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("wd", [130, 195, 67])
+def test_deform_narrow_last_wave_matches_oracle(dtype, wd):
+    """Frame widths whose last wave holds one to three columns: lane k of a
+    wave supplies row k's predictor cell to the others, so every lane forms
+    the predictor before the columns beyond the frame leave."""
+    shape = (96, wd)
+    fa, fb = _frames(shape, dtype, seed=11)
+    geom = O.field_geometry(shape, 32, 0.5)
+    rng = np.random.default_rng(12)
+    up = rng.uniform(-4, 4, geom.field_shape)
+    vpred = rng.uniform(-4, 4, geom.field_shape)
+    ra, rb = O.deform_frames(fa, fb, geom, up, vpred, 0.5)
+    ga, gb = gu.stage_deform(fa, fb, geom, up.astype(np.float32), vpred.astype(np.float32), 0.5)
+    tol = 1e-3 * float(max(fa.max(), fb.max()))  # FP32 warp vs FP64-interpolated predictor
+    assert np.abs(ga - ra).max() <= tol
+    assert np.abs(gb - rb).max() <= tol
+
+
 def test_grid_interp_matches_oracle():
     shape = (512, 640)
     g1 = O.field_geometry(shape, 64, 0.5)
