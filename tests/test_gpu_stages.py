@@ -116,6 +116,25 @@ def test_deform_narrow_last_wave_matches_oracle(dtype, wd):
     assert np.abs(gb - rb).max() <= tol
 
 
+@pytest.mark.parametrize("shape,win,amp,par", [((160, 200), 32, 4.0, 0.5), ((257, 130), 32, 30.0, 0.3),
+                                              ((64, 66), 16, 60.0, 0.5), ((300, 512), 8, 12.0, 0.5),
+                                              ((96, 34), 16, 3.0, 0.5)])
+def test_deform_uint16_matches_float_frames(shape, win, amp, par):
+    """uint16 frames against the same frames as float32: the integer tap
+    loads and their conversion give bit-identical deformed frames, including
+    large displacements, frame edges and widths whose last wave is nearly
+    empty."""
+    fa, fb = _frames(shape, np.uint16, seed=5)
+    geom = O.field_geometry(shape, win, 0.5)
+    rng = np.random.default_rng(7)
+    up = rng.uniform(-amp, amp, geom.field_shape).astype(np.float32)
+    vpred = rng.uniform(-amp, amp, geom.field_shape).astype(np.float32)
+    wa, wb = gu.stage_deform(fa, fb, geom, up, vpred, par)
+    ra, rb = gu.stage_deform(fa.astype(np.float32), fb.astype(np.float32), geom, up, vpred, par)
+    assert np.array_equal(wa.view(np.uint32), ra.view(np.uint32))
+    assert np.array_equal(wb.view(np.uint32), rb.view(np.uint32))
+
+
 def test_grid_interp_matches_oracle():
     shape = (512, 640)
     g1 = O.field_geometry(shape, 64, 0.5)
