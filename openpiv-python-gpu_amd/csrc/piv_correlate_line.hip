@@ -244,45 +244,49 @@ __global__ __launch_bounds__(64 * BW, LineLayout<F>::WAVES) void corr_line_kerne
   wave_lds_sync();
 
   // ---- stage C: columns kx = j of A and B, product -------------------------
-  // FA goes to LDS (column kx of the R layout, which this lane alone
-  // touches) while FB is transformed; column H of the R layout (row padding)
-  // carries the slot-0 values of the window for the DC/Nyquist product.
+  // FA stays in registers while FB is transformed (occupancy is set by LDS,
+  // not registers); lane 0 of the window copies its slot-0 spectra (FA to
+  // column 0, FB to column H of the R layout) for the DC/Nyquist product.
   const int kx = j;
   c32 p[F];  // product P[ky][kx] in natural order of ky
   {
     c32 bin[H];
-    {
-      c32 fa[F];
-      static_for<0, H>([&](auto MM) {
-        constexpr int r = decltype(MM)::value;
-        const bool in = r < W;
-        const int ri = in ? r : 0;
-        const c32 va = AB[(size_t)ri * SF + kx], vb = AB[(size_t)ri * SF + H + kx];
-        fa[r] = in ? va : c32{0.f, 0.f};
-        bin[r] = in ? vb : c32{0.f, 0.f};
-      });
-      static_for<H, F>([&](auto MM) { fa[decltype(MM)::value] = c32{0.f, 0.f}; });
-      wave_lds_sync();  // AB (aliased by R) fully read
-      fft_reg<F, -1, true>(fa);
-      static_for<0, F>([&](auto KY) {
-        constexpr int ky = decltype(KY)::value;
-        R[(size_t)ky * SR + kx] = fa[bitrev(ky, F)];
-      });
-    }
+    static_for<0, H>([&](auto MM) {
+      constexpr int r = decltype(MM)::value;
+      const bool in = r < W;
+      const int ri = in ? r : 0;
+      const c32 va = AB[(size_t)ri * SF + kx], vb = AB[(size_t)ri * SF + H + kx];
+      p[r] = in ? va : c32{0.f, 0.f};
+      bin[r] = in ? vb : c32{0.f, 0.f};
+    });
+    static_for<H, F>([&](auto MM) { p[decltype(MM)::value] = c32{0.f, 0.f}; });
+    wave_lds_sync();  // AB (aliased by R) fully read
+    fft_reg<F, -1, true>(p);  // FA of column kx, frequency ky in p[bitrev(ky)]
     c32 fb[F];
     static_for<0, H>([&](auto MM) { fb[decltype(MM)::value] = bin[decltype(MM)::value]; });
     static_for<H, F>([&](auto MM) { fb[decltype(MM)::value] = c32{0.f, 0.f}; });
     fft_reg<F, -1, true>(fb);
-    static_for<0, F>([&](auto KY) {
-      constexpr int ky = decltype(KY)::value;
-      p[ky] = cconjmul(R[(size_t)ky * SR + kx], fb[bitrev(ky, F)]);
-    });
     if (kx == 0) {
       static_for<0, F>([&](auto KY) {
         constexpr int ky = decltype(KY)::value;
+        R[(size_t)ky * SR] = p[bitrev(ky, F)];
         R[(size_t)ky * SR + H] = fb[bitrev(ky, F)];
       });
     }
+    // conj(FA) FB in place, then back to natural order of ky
+    static_for<0, F>([&](auto KY) {
+      constexpr int ky = decltype(KY)::value;
+      p[ky] = cconjmul(p[ky], fb[ky]);
+    });
+    static_for<0, F>([&](auto KY) {
+      constexpr int ky = decltype(KY)::value;
+      constexpr int b = bitrev(ky, F);
+      if constexpr (ky < b) {
+        const c32 tmp = p[ky];
+        p[ky] = p[b];
+        p[b] = tmp;
+      }
+    });
   }
   wave_lds_sync();
   {
