@@ -136,7 +136,7 @@ template <int TM, int NW = 4>
 __global__ __launch_bounds__(64 * NW) void conv_fwd_kernel(ConvGeom g, InView in, const uint16_t* __restrict__ w,
                                                            const float* __restrict__ bias, void* out, int out_mode,
                                                            int relu) {
-  __shared__ float red[NW - 1][TM][4][64][4];
+  __shared__ float4_t red[NW][TM][4][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = lane & 15, gq = lane >> 4;
   const int total = g.batch * g.ho * g.wo;
@@ -199,49 +199,49 @@ __global__ __launch_bounds__(64 * NW) void conv_fwd_kernel(ConvGeom g, InView in
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[m][t] = mfma(a[q][m], b[q][t], acc[m][t]);
   }
-  if (wv > 0) {
+  // every wave's partial sums to LDS; wave w then sums and stores the
+  // (m, t) tiles mt = 4m + t with mt mod NW = w, in wave order (the order of
+  // a single accumulator walking the waves)
 #pragma unroll
-    for (int m = 0; m < TM; ++m)
+  for (int m = 0; m < TM; ++m)
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) red[wv - 1][m][t][lane][e] = acc[m][t][e];
-  }
+    for (int t = 0; t < 4; ++t) red[wv][m][t][lane] = acc[m][t];
   __syncthreads();
-  if (wv != 0) return;
-  // epilogue: rows co = cobase + 16m + 4gq + e of pixel column r
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    if (!px[t].ok) continue;
+  for (int mt = 0; mt < TM * 4; ++mt) {
+    const int m = mt / 4, t = mt % 4;  // compile-time after unrolling: no indexed register arrays
+    if (mt % NW != wv || !px[t].ok) continue;
+    float4_t sum = red[0][m][t][lane];
+#pragma unroll
+    for (int q = 1; q < NW; ++q) sum += red[q][m][t][lane];
+    // rows co = cobase + 16m + 4gq + e of pixel column r
     const int p = pbase + t * 16 + r;
+    const int co0 = cobase + m * 16 + 4 * gq;
 #pragma unroll
-    for (int m = 0; m < TM; ++m) {
-      const int co0 = cobase + m * 16 + 4 * gq;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int co = co0 + e;
-        if (co >= g.co) continue;
-        float val = acc[m][t][e];
-#pragma unroll
-        for (int q = 0; q < NW - 1; ++q) val += red[q][m][t][lane][e];
-        val += bias[co];
-        if (relu) val = fmaxf(val, 0.f);
-        if (out_mode == kOutBf16) {
-          static_cast<uint16_t*>(out)[p * g.co + co] = f2bf(val);
-        } else if (out_mode == kOutF32) {
-          static_cast<float*>(out)[p * g.co + co] = val;
-        } else {  // per-image planar [n][co][y][x]
-          const int hw = g.ho * g.wo;
-          static_cast<float*>(out)[(px[t].n * g.co + co) * hw + (px[t].y * g.wo + px[t].x)] = val;
-        }
+    for (int e = 0; e < 4; ++e) {
+      const int co = co0 + e;
+      if (co >= g.co) continue;
+      float val = sum[e] + bias[co];
+      if (relu) val = fmaxf(val, 0.f);
+      if (out_mode == kOutBf16) {
+        static_cast<uint16_t*>(out)[p * g.co + co] = f2bf(val);
+      } else if (out_mode == kOutF32) {
+        static_cast<float*>(out)[p * g.co + co] = val;
+      } else {  // per-image planar [n][co][y][x]
+        const int hw = g.ho * g.wo;
+        static_cast<float*>(out)[(px[t].n * g.co + co) * hw + (px[t].y * g.wo + px[t].x)] = val;
       }
     }
   }
 }
 
 // Layers whose input channels are not multiples of 16 (the 2-channel input
-// layer): each wave owns a 32 x 64 tile and walks the flat (tap, ci) index
-// element by element.  grid: x = ceil(pixels / 256), y = ceil(co / 32).
+// layer): each wave owns a 32 x (16 TP) tile and walks the flat (tap, ci)
+// index element by element.  The layer is a few MFLOP over a full-resolution
+// grid: one 16-pixel tile per wave keeps four times more waves in flight
+// than four tiles would, which is what its load latency needs.
+// grid: x = ceil(pixels / (64 TP)), y = ceil(co / 32).
+constexpr int kSmallTP = 1;
 template <int CI>  // CI > 0: channel count known at compile time
 __global__ __launch_bounds__(256) void conv_fwd_small_kernel(ConvGeom g, InView in, const uint16_t* __restrict__ w,
                                                              const float* __restrict__ bias, void* out, int out_mode,
@@ -249,13 +249,13 @@ __global__ __launch_bounds__(256) void conv_fwd_small_kernel(ConvGeom g, InView 
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = lane & 15, gq = lane >> 4;
   const int total = g.batch * g.ho * g.wo;
-  const int pbase = (blockIdx.x * 4 + wv) * 64;
+  const int pbase = (blockIdx.x * 4 + wv) * (16 * kSmallTP);
   if (pbase >= total) return;
   const int cobase = blockIdx.y * 32;
   const int ci = CI > 0 ? CI : in.c0 + in.c1, K = 9 * ci;
-  Pix px[4];
+  Pix px[kSmallTP];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) px[t] = pix_of32(pbase + t * 16 + r, total, g.ho, g.wo);
+  for (int t = 0; t < kSmallTP; ++t) px[t] = pix_of32(pbase + t * 16 + r, total, g.ho, g.wo);
   int coA[2];
   bool okA[2];
 #pragma unroll
@@ -263,13 +263,13 @@ __global__ __launch_bounds__(256) void conv_fwd_small_kernel(ConvGeom g, InView 
     coA[m] = cobase + m * 16 + r;
     okA[m] = coA[m] < g.co;
   }
-  float4_t acc[2][4];
+  float4_t acc[2][kSmallTP];
 #pragma unroll
   for (int m = 0; m < 2; ++m)
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[m][t] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < kSmallTP; ++t) acc[m][t] = float4_t{0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < K; k0 += 16) {
-    bf16x4_t a[2], b[4];
+    bf16x4_t a[2], b[kSmallTP];
 #pragma unroll
     for (int m = 0; m < 2; ++m)
 #pragma unroll
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(256) void conv_fwd_small_kernel(ConvGeom g, InView 
         a[m][e] = (okA[m] && k < K) ? (short)w[coA[m] * K + k] : (short)0;
       }
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < kSmallTP; ++t)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int k = k0 + 4 * gq + e;
@@ -295,10 +295,10 @@ __global__ __launch_bounds__(256) void conv_fwd_small_kernel(ConvGeom g, InView 
 #pragma unroll
     for (int m = 0; m < 2; ++m)
 #pragma unroll
-      for (int t = 0; t < 4; ++t) acc[m][t] = mfma(a[m], b[t], acc[m][t]);
+      for (int t = 0; t < kSmallTP; ++t) acc[m][t] = mfma(a[m], b[t], acc[m][t]);
   }
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
+  for (int t = 0; t < kSmallTP; ++t) {
     if (!px[t].ok) continue;
     const int p = pbase + t * 16 + r;
 #pragma unroll
@@ -765,7 +765,7 @@ hipError_t launch_conv_fwd(const ConvGeom& g, const uint16_t* s0, int c0, int up
         hipLaunchKernelGGL(conv_fwd_kernel<1>, grid, dim3(256), 0, s, g, in, w, bias, out, out_mode, relu);
     }
   } else {
-    const dim3 grid(blocks_for(pixels), (unsigned)((g.co + 31) / 32));
+    const dim3 grid(blocks_for(pixels, 64 * kSmallTP), (unsigned)((g.co + 31) / 32));
     if (c0 + c1 == 2)
       hipLaunchKernelGGL(conv_fwd_small_kernel<2>, grid, dim3(256), 0, s, g, in, w, bias, out, out_mode, relu);
     else
