@@ -148,28 +148,29 @@ __global__ __launch_bounds__(64 * BW, LineLayout<F>::WAVES) void corr_line_kerne
   const int row0 = wrow * a.spacing + a.y_off;
   const int col0 = wcol * a.spacing + a.x_off;
 
-  // ---- staging: the window's lanes load W x W pixels row by row ----------
+  // ---- staging: the window's lanes load W x W pixels ----------------------
+  // Full windows (the usual case): lane j loads row j itself, which is what
+  // the row stage needs, so the pixels never pass through LDS.  Otherwise the
+  // window goes through an LDS staging area, lane j reading row j back.
   float sa = 0.f, sb = 0.f;
-  if (W == H && buffer_fits((uint64_t)a.ht * a.wd, sizeof(InT))) {
-    // full window (the usual case): lane j loads column j of every row, all
-    // loads issued before the first is used
+  const bool direct = W == H && buffer_fits((uint64_t)a.ht * a.wd, sizeof(InT));
+  float rowa[H], rowb[H];
+  if (direct) {
     const uint32_t npx = (uint32_t)a.ht * (uint32_t)a.wd;
     const rsrc_t ra = make_rsrc(a.frame_a, npx * (uint32_t)sizeof(InT));
     const rsrc_t rb = make_rsrc(a.frame_b, npx * (uint32_t)sizeof(InT));
-    const uint32_t base = (uint32_t)row0 * (uint32_t)a.wd + (uint32_t)(col0 + j);
-    float va[H], vb[H];
+    const uint32_t base = (uint32_t)(row0 + j) * (uint32_t)a.wd + (uint32_t)col0;
 #pragma unroll
-    for (int r = 0; r < H; ++r) {
-      va[r] = buf_px<InT>(ra, base + (uint32_t)r * (uint32_t)a.wd);
-      vb[r] = buf_px<InT>(rb, base + (uint32_t)r * (uint32_t)a.wd);
+    for (int c = 0; c < H; ++c) {
+      rowa[c] = buf_px<InT>(ra, base + (uint32_t)c);
+      rowb[c] = buf_px<InT>(rb, base + (uint32_t)c);
     }
 #pragma unroll
-    for (int r = 0; r < H; ++r) {
-      const float x = valid ? va[r] : 0.f, y = valid ? vb[r] : 0.f;
-      stA[r * SW + j] = x;
-      stB[r * SW + j] = y;
-      sa += x;
-      sb += y;
+    for (int c = 0; c < H; ++c) {
+      rowa[c] = valid ? rowa[c] : 0.f;
+      rowb[c] = valid ? rowb[c] : 0.f;
+      sa += rowa[c];
+      sb += rowb[c];
     }
   } else {
     int r = j / W, c = j % W;
@@ -189,6 +190,15 @@ __global__ __launch_bounds__(64 * BW, LineLayout<F>::WAVES) void corr_line_kerne
         c -= W;
         r += 1;
       }
+    }
+    wave_lds_sync();
+    const int rr = j < W ? j : 0;
+#pragma unroll
+    for (int c2 = 0; c2 < H; ++c2) {
+      const bool in = c2 < W;
+      const int ci = in ? c2 : 0;
+      rowa[c2] = in ? stA[rr * SW + ci] : 0.f;
+      rowb[c2] = in ? stB[rr * SW + ci] : 0.f;
     }
   }
 #pragma unroll
@@ -216,9 +226,7 @@ __global__ __launch_bounds__(64 * BW, LineLayout<F>::WAVES) void corr_line_kerne
     static_for<0, H>([&](auto CC) {
       constexpr int c = decltype(CC)::value;
       const bool in = c < W;
-      const int ci = in ? c : 0;
-      const float va = stA[rr * SW + ci], vb = stB[rr * SW + ci];
-      x[c] = in ? c32{va - mean_a, vb - mean_b} : c32{0.f, 0.f};
+      x[c] = in ? c32{rowa[c] - mean_a, rowb[c] - mean_b} : c32{0.f, 0.f};
     });
     static_for<H, F>([&](auto CC) { x[decltype(CC)::value] = c32{0.f, 0.f}; });
     wave_lds_sync();  // staging (aliased by AB) fully read
