@@ -11,10 +11,11 @@
 // waits for other waves: all synchronisation is within the wave.
 //
 //   stage  lane j of a window holds                      LDS
-//   R      row j of z = (a - mean) + i (b - mean)        staging -> AB
-//          -> A and B row spectra (DC, Nyquist in slot 0)
-//   C      columns kx = j of A and B -> conj(FA) FB      AB -> registers,
-//          (FA parked in LDS while FB is transformed)    FA -> LDS -> registers
+//   R      row j of z = (a - mean) + i (b - mean)        (full windows: rows
+//          -> A and B row spectra (DC, Nyquist in slot 0) loaded by their lane;
+//                                                        else staging) -> AB
+//   C      columns kx = j of A and B -> conj(FA) FB      AB -> registers
+//          (slot-0 spectra through LDS for the DC/Nyquist product)
 //   IC     inverse column kx of the product              -> R
 //   IR     rows 2j, 2j+1 as one complex inverse          R -> plane
 //   peak   first-occurrence argmax, sub-pixel fit        plane
