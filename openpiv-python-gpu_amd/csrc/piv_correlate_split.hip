@@ -235,40 +235,86 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
     const int row0 = wrow * a.spacing + a.y_off;
     const int col0 = wcol * a.spacing + a.x_off;
     constexpr int PER = (HW * HW) / (2 * F);  // pixels per thread (HW x HW staging area)
+    // 2-byte and 4-byte frames inside the 32-bit buffer range: horizontally
+    // adjacent pixel pairs per thread (one 4-byte or two merged loads per
+    // pair and frame, one paired LDS store), else one pixel per thread
+    constexpr bool kPairs = sizeof(InT) == 2 || sizeof(InT) == 4;
+    constexpr int HP = HW / 2;        // pixel pairs per staging row
+    constexpr int RS = 2 * F / HP;    // staging rows per step of the window's threads
+    constexpr int P2 = HW / RS;       // pairs per thread
+    static_assert(HW % RS == 0 && (2 * F) % HP == 0, "whole staging rows per step");
+    const bool pairs = kPairs && buffer_fits((uint64_t)a.ht * a.wd, sizeof(InT));
     float va[PER], vb[PER];
-    // every load issued before any is used: outside the window (or for an
-    // invalid window) the address is clamped into it and the value set to 0
-    if (buffer_fits((uint64_t)a.ht * a.wd, sizeof(InT))) {
+    float sa = 0.f, sb = 0.f;
+    if (pairs) {
       const uint32_t npx = (uint32_t)a.ht * (uint32_t)a.wd;
       const rsrc_t ra = make_rsrc(a.frame_a, npx * (uint32_t)sizeof(InT));
       const rsrc_t rb = make_rsrc(a.frame_b, npx * (uint32_t)sizeof(InT));
-      // pixel e of this thread: row t / HW + e * (2F / HW), column t % HW
-      const uint32_t base = (uint32_t)row0 * (uint32_t)a.wd + (uint32_t)(col0 + min(t % HW, W - 1));
+      // pair e of this thread: row t / HP + e * RS, columns 2 (t % HP) and + 1
+      const int c2 = 2 * (t % HP);
+      const uint32_t base = (uint32_t)row0 * (uint32_t)a.wd + (uint32_t)(col0 + min(c2, max(W - 2, 0)));
 #pragma unroll
-      for (int e = 0; e < PER; ++e) {
-        const int r = t / HW + e * (2 * F / HW);
+      for (int e = 0; e < P2; ++e) {
+        const int r = t / HP + e * RS;
         const uint32_t gi = base + (uint32_t)min(r, W - 1) * (uint32_t)a.wd;
-        va[e] = buf_px<InT>(ra, gi);
-        vb[e] = buf_px<InT>(rb, gi);
+        if constexpr (sizeof(InT) == 2) {
+          const uint32_t wa = __builtin_amdgcn_raw_buffer_load_b32(ra, (int)(gi * 2u), 0, 0);
+          const uint32_t wb = __builtin_amdgcn_raw_buffer_load_b32(rb, (int)(gi * 2u), 0, 0);
+          va[2 * e] = (float)(wa & 0xffffu);
+          va[2 * e + 1] = (float)(wa >> 16);
+          vb[2 * e] = (float)(wb & 0xffffu);
+          vb[2 * e + 1] = (float)(wb >> 16);
+        } else {
+          va[2 * e] = buf_f32(ra, gi * 4u, 0u);
+          va[2 * e + 1] = buf_f32(ra, gi * 4u + 4u, 0u);
+          vb[2 * e] = buf_f32(rb, gi * 4u, 0u);
+          vb[2 * e + 1] = buf_f32(rb, gi * 4u + 4u, 0u);
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < P2; ++e) {
+        const int r = t / HP + e * RS;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const bool in = valid && r < W && c2 + h < W;
+          va[2 * e + h] = in ? va[2 * e + h] : 0.f;
+          vb[2 * e + h] = in ? vb[2 * e + h] : 0.f;
+          sa += va[2 * e + h];
+          sb += vb[2 * e + h];
+        }
       }
     } else {
+      if (buffer_fits((uint64_t)a.ht * a.wd, sizeof(InT))) {
+        const uint32_t npx = (uint32_t)a.ht * (uint32_t)a.wd;
+        const rsrc_t ra = make_rsrc(a.frame_a, npx * (uint32_t)sizeof(InT));
+        const rsrc_t rb = make_rsrc(a.frame_b, npx * (uint32_t)sizeof(InT));
+        // pixel e of this thread: row t / HW + e * (2F / HW), column t % HW
+        const uint32_t base = (uint32_t)row0 * (uint32_t)a.wd + (uint32_t)(col0 + min(t % HW, W - 1));
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+          const int r = t / HW + e * (2 * F / HW);
+          const uint32_t gi = base + (uint32_t)min(r, W - 1) * (uint32_t)a.wd;
+          va[e] = buf_px<InT>(ra, gi);
+          vb[e] = buf_px<InT>(rb, gi);
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+          const int r = t / HW + e * (2 * F / HW);
+          const size_t gi = (size_t)(row0 + min(r, W - 1)) * a.wd + (col0 + min(t % HW, W - 1));
+          va[e] = load_px<InT>(a.frame_a, gi);
+          vb[e] = load_px<InT>(a.frame_b, gi);
+        }
+      }
+      static_assert((2 * F) % HW == 0, "whole staging rows per step");
 #pragma unroll
       for (int e = 0; e < PER; ++e) {
-        const int r = t / HW + e * (2 * F / HW);
-        const size_t gi = (size_t)(row0 + min(r, W - 1)) * a.wd + (col0 + min(t % HW, W - 1));
-        va[e] = load_px<InT>(a.frame_a, gi);
-        vb[e] = load_px<InT>(a.frame_b, gi);
+        const bool in = valid && t / HW + e * (2 * F / HW) < W && t % HW < W;
+        va[e] = in ? va[e] : 0.f;
+        vb[e] = in ? vb[e] : 0.f;
+        sa += va[e];
+        sb += vb[e];
       }
-    }
-    static_assert((2 * F) % HW == 0, "whole staging rows per step");
-    float sa = 0.f, sb = 0.f;
-#pragma unroll
-    for (int e = 0; e < PER; ++e) {
-      const bool in = valid && t / HW + e * (2 * F / HW) < W && t % HW < W;
-      va[e] = in ? va[e] : 0.f;
-      vb[e] = in ? vb[e] : 0.f;
-      sa += va[e];
-      sb += vb[e];
     }
 #pragma unroll
     for (int off = HW / 2; off > 0; off >>= 1) {
@@ -285,12 +331,23 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
     // means over the window (forced on: the padding is nonzero,
     // gpu_process.py:1065, 1201-1205)
     const float ma = sa / (float)(W * W), mb = sb / (float)(W * W);
+    if (pairs) {
+      const int c2 = 2 * (t % HP);
 #pragma unroll
-    for (int e = 0; e < PER; ++e) {
-      const int idx = t + e * 2 * F;
-      const int r = idx / HW, c = idx % HW;
-      const bool in = r < W && c < W;
-      st[r * SW + c] = in ? float2{va[e] - ma, vb[e] - mb} : float2{0.f, 0.f};
+      for (int e = 0; e < P2; ++e) {
+        const int r = t / HP + e * RS;
+        const bool in0 = r < W && c2 < W, in1 = r < W && c2 + 1 < W;
+        st[r * SW + c2] = in0 ? float2{va[2 * e] - ma, vb[2 * e] - mb} : float2{0.f, 0.f};
+        st[r * SW + c2 + 1] = in1 ? float2{va[2 * e + 1] - ma, vb[2 * e + 1] - mb} : float2{0.f, 0.f};
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < PER; ++e) {
+        const int idx = t + e * 2 * F;
+        const int r = idx / HW, c = idx % HW;
+        const bool in = r < W && c < W;
+        st[r * SW + c] = in ? float2{va[e] - ma, vb[e] - mb} : float2{0.f, 0.f};
+      }
     }
   }
   __syncthreads();
