@@ -413,12 +413,15 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
   fft_dit<L, -1>(fa);
   fft_dit<L, -1>(fb);
   // slot 0 holds FFT(A0 + i AN) and FFT(B0 + i BN): hand them to the
-  // window's threads for the mirror unpack
+  // window's threads for the mirror unpack.  The exchange stores frequency
+  // ky = NR q + r at (ky mod NR) L + ky / NR, so a residue's values are
+  // contiguous and lane 0 moves two per 16-byte access.
+  auto xslot = [](int ky) { return (ky & (NR - 1)) * L + (ky >> 2); };
   if (n == 0) {
-    static_for<0, L>([&](auto QQ) {
-      constexpr int q = decltype(QQ)::value;
-      xq[NR * q + role] = fa[q];
-      xq[F + NR * q + role] = fb[q];
+    static_for<0, L / 2>([&](auto QQ) {
+      constexpr int q = 2 * decltype(QQ)::value;
+      lds_st4(xq + role * L + q, fa[q], fa[q + 1]);
+      lds_st4(xq + F + role * L + q, fb[q], fb[q + 1]);
     });
   }
   // product conj(FA) FB (slot 0's is replaced below)
@@ -430,19 +433,21 @@ __global__ __launch_bounds__(256, SplitLayout<F>::WAVES) void corr_split_kernel(
   if (t < F) {
     // f0 = (q + conj(qm)) / 2, fn = (q - conj(qm)) / (2i); P0 + i PN
     const int ky = t, kmir = (F - t) & (F - 1);
-    const c32 qa = xq[ky], qam = xq[kmir], qb = xq[F + ky], qbm = xq[F + kmir];
+    const c32 qa = xq[xslot(ky)], qam = xq[xslot(kmir)], qb = xq[F + xslot(ky)], qbm = xq[F + xslot(kmir)];
     const c32 fa0{0.5f * (qa.x + qam.x), 0.5f * (qa.y - qam.y)};
     const c32 fan{0.5f * (qa.y + qam.y), 0.5f * (qam.x - qa.x)};
     const c32 fb0{0.5f * (qb.x + qbm.x), 0.5f * (qb.y - qbm.y)};
     const c32 fbn{0.5f * (qb.y + qbm.y), 0.5f * (qbm.x - qb.x)};
     const c32 p0 = cconjmul(fa0, fb0), pn = cconjmul(fan, fbn);
-    xq[2 * F + ky] = c32{p0.x - pn.y, p0.y + pn.x};
+    xq[2 * F + xslot(ky)] = c32{p0.x - pn.y, p0.y + pn.x};
   }
   __syncthreads();
   if (n == 0) {
-    static_for<0, L>([&](auto QQ) {
-      constexpr int q = decltype(QQ)::value;
-      fa[q] = xq[2 * F + NR * q + role];
+    static_for<0, L / 2>([&](auto QQ) {
+      constexpr int q = 2 * decltype(QQ)::value;
+      const float4 v = lds_f4(xq + 2 * F + role * L + q);
+      fa[q] = c32{v.x, v.y};
+      fa[q + 1] = c32{v.z, v.w};
     });
   }
 
