@@ -536,7 +536,6 @@ __global__ __launch_bounds__(256) void median_kernel(const float* u, const float
 // outlier the system is singular; the reference's NaN -> reset path then
 // leaves the field unchanged, and so does this kernel.
 // ---------------------------------------------------------------------------
-constexpr int kReplaceThreads = 1024;
 constexpr int kSpringLds = 1024;  // outlier count handled by the LDS-resident solver
 constexpr int kClusterMax = 6;    // largest outlier cluster solved directly (spring_clusters)
 

@@ -74,8 +74,6 @@ constexpr size_t kGcvPart1 = kGcvPart0 + (size_t)kGcvBlocks * kGrid * 4;  // [b]
 constexpr size_t kGcvPart2 = kGcvPart1 + (size_t)kGcvNewtonBlocks * 6;    // [b][6]
 constexpr size_t kGcvPartTotal = kGcvPart2 + (size_t)kGcvNewtonBlocks * 6;
 
-__device__ double gcv_state_p(const GcvState* st) { return st->p; }
-
 // Sums per p-value: RSS, T, RSS', T', RSS'', T'' (L = ln 10, e = 1 - Gamma):
 //   RSS' = 2L sum S e^2 G ; RSS'' = 2L^2 sum S e^2 G (3G - 1)
 //   T'   = -L sum G e    ; T''   = L^2 sum G e (1 - 2G)
